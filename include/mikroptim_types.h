@@ -1,0 +1,145 @@
+/* Plain-C data layout shared by the HIP library (libmikroptim_hip.so), its C++ host layer and
+ * the CPU oracle: a system batch in structure-of-arrays form, the residual settings and the
+ * variable map. Mirrors the reference's value types field by field:
+ *   mko_batch              <- SystemBatchView<double>  include/mikroptim/batch/system_batch_view.hpp:108
+ *                             (BatchShape :29, BatchTopology include/mikroptim/batch/system_batch.hpp:26)
+ *   mko_residual_settings  <- ResidualSettings         include/mikroptim/batch/batch_residuals.hpp:24
+ *   mko_variables          <- VariableSet / Variable   include/mikroptim/optim/variable_set.hpp:20,62
+ *   mko_lm_options         <- LmOptions                include/mikroptim/optim/levenberg_marquardt.hpp:85
+ * All indices are element (not byte) offsets, variant outermost. */
+#ifndef MIKROPTIM_TYPES_H
+#define MIKROPTIM_TYPES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MKO_MAX_SURFACES 64 /* kMaxSurfaces, include/mikroptim/trace/surface_table.hpp:15 */
+#define MKO_MAX_RINGS 16    /* kMaxPupilRings, include/mikroptim/pupil/quadrature.hpp:21 */
+
+/* SurfaceKind (include/mikroptim/trace/surface_table.hpp:26); ghost plates are not supported. */
+enum { MKO_SURFACE_STOP = 0, MKO_SURFACE_PLANE = 1, MKO_SURFACE_SPHERE = 2 };
+/* PupilMode (include/mikroptim/pupil/quadrature.hpp:23) */
+enum { MKO_PUPIL_HALF = 0, MKO_PUPIL_FULL = 1, MKO_PUPIL_AXIS = 2 };
+/* SourceKind (include/mikroptim/batch/system_batch_view.hpp:21) */
+enum { MKO_SOURCE_COLLIMATED = 0, MKO_SOURCE_POINT = 1 };
+/* ReferenceSphereKind (include/mikroptim/trace/reference_kind.hpp:11) */
+enum { MKO_REF_LAST_SURFACE = 0, MKO_REF_IMAGE_EXIT_PUPIL = 1, MKO_REF_IMAGE_INFINITE = 2 };
+/* FailureMode (include/mikroptim/trace/surface_ops.hpp:79) */
+enum { MKO_FAILURE_SOFT = 0, MKO_FAILURE_HARD = 1 };
+/* TraceStatus (include/mikroptim/trace/surface_ops.hpp:24), stored as one byte per ray */
+enum {
+    MKO_STATUS_OK = 0,
+    MKO_STATUS_MISSED = 1,
+    MKO_STATUS_TIR = 2,
+    MKO_STATUS_BACKWARD = 3,
+    MKO_STATUS_UNSUPPORTED = 4,
+    MKO_STATUS_SOFT_MISSED = 5,
+    MKO_STATUS_SOFT_TIR = 6
+};
+/* VariableType (include/mikroptim/model/variables.hpp:17), the kinds this build supports */
+enum { MKO_VAR_CURVATURE = 0, MKO_VAR_THICKNESS = 1, MKO_VAR_WORKING_DISTANCE = 2 };
+/* FdStencil (include/mikroptim/optim/fd_formula.hpp:15) */
+enum { MKO_STENCIL_CENTRAL = 0, MKO_STENCIL_ONE_SIDED = 1, MKO_STENCIL_TRUNCATED = 2 };
+/* JacobianMethod (include/mikroptim/optim/levenberg_marquardt.hpp:73) */
+enum { MKO_JACOBIAN_FD = 0, MKO_JACOBIAN_ADJOINT = 1 };
+/* LmStatus (include/mikroptim/optim/levenberg_marquardt.hpp:28) */
+enum {
+    MKO_LM_RUNNING = 0,
+    MKO_LM_CONVERGED_MERIT = 1,
+    MKO_LM_CONVERGED_STEP = 2,
+    MKO_LM_CONVERGED_GRADIENT = 3,
+    MKO_LM_CONVERGED_STALL = 4,
+    MKO_LM_MAX_ITERATIONS = 5,
+    MKO_LM_DAMPING_LIMIT = 6,
+    MKO_LM_NUMERICAL_ERROR = 7
+};
+/* Arithmetic of the device path. FP64: every kernel in double. MIXED: forward trace (OPL,
+ * residuals, merit) in double, reverse sweep of the adjoint in float, JtJ on bf16x3 MFMA with
+ * float accumulation, Jtr and the LM step in double. */
+enum { MKO_PRECISION_FP64 = 0, MKO_PRECISION_MIXED = 1 };
+
+/* One batch of V variants of one topology. `type`, `pupil_mode`, `wavelength_weight` and
+ * `field_weight` are batch-wide and always live in host memory; the per-variant arrays are host
+ * memory for host entry points and device memory for the *_device entry points. */
+typedef struct mko_batch {
+    int32_t variants;    /* V */
+    int32_t surfaces;    /* S, stop included, 2 ... MKO_MAX_SURFACES */
+    int32_t wavelengths; /* L >= 1 */
+    int32_t fields;      /* F >= 1 */
+    int32_t source;      /* MKO_SOURCE_* */
+    int32_t reference_sphere;       /* MKO_REF_* */
+    double reference_wavelength_nm; /* lambda_ref */
+    double source_point[3];         /* point source only */
+    double source_na;               /* point source only */
+    const uint8_t* type;               /* [S]  MKO_SURFACE_* (host) */
+    const uint8_t* pupil_mode;         /* [F]  MKO_PUPIL_* (host) */
+    const double* wavelength_weight;   /* [L]  (host) */
+    const double* field_weight;        /* [F]  (host) */
+    const double* curvature;           /* [V][S]  1/mm, 0 for stop and plane */
+    const double* thickness;           /* [V][S]  mm; [v][S-1] = working distance */
+    const double* index;               /* [V][S][L] medium after surface s */
+    const double* pupil_radius;        /* [V] */
+    const double* field_angle;         /* [V][F]  rad */
+    const double* field_target;        /* [V][F]  ideal image height h_f */
+    const double* reference_curvature; /* [V][F]  kappa (image references only) or NULL */
+} mko_batch;
+
+typedef struct mko_residual_settings {
+    int32_t rings;  /* N, 1 ... 16 */
+    int32_t spokes; /* M (axis fields: 1) */
+    int32_t failure; /* MKO_FAILURE_* */
+    double fail_weight; /* w_fail [waves^2] */
+} mko_residual_settings;
+
+/* P variables. Curvature k -> curvature[v][batch_surface[k]]; thickness and working distance
+ * -> thickness[v][batch_surface[k]] (working distance: batch_surface = S - 1). Bounds may be
+ * +-inf; typical > 0. (host memory) */
+typedef struct mko_variables {
+    int32_t count;
+    const int32_t* type;          /* [P] MKO_VAR_* */
+    const int32_t* batch_surface; /* [P] */
+    const double* lower;          /* [P] */
+    const double* upper;          /* [P] */
+    const double* typical;        /* [P] */
+} mko_variables;
+
+typedef struct mko_lm_options {
+    int32_t max_iterations; /* 100 */
+    double ftol, xtol, gtol; /* 1e-10 each */
+    int32_t stall_window;   /* 0 = off */
+    double stall_tolerance; /* 1e-3 */
+    double initial_damping; /* 1e-3 */
+    double max_damping;     /* 1e16 */
+    double diag_floor;      /* 1e-12 */
+    int32_t reject_new_failures; /* 1 */
+    int32_t threads;        /* host threads, <= 0: hardware */
+    int32_t jacobian_method; /* MKO_JACOBIAN_* */
+    double fail_weight_factor; /* 10 */
+    double max_fail_weight;    /* 1e14 */
+} mko_lm_options;
+
+/* Per-run LM result (LmRunResult, include/mikroptim/optim/levenberg_marquardt.hpp:130). */
+typedef struct mko_lm_run {
+    double merit;
+    int64_t failed_rays;
+    int64_t hard_failed_rays;
+    int32_t status; /* MKO_LM_* */
+    int32_t iterations;
+    int32_t accepted;
+    int32_t jacobians;
+    double damping;
+    double gradient_norm;
+    double model_decrease;
+    double fail_weight;
+    int32_t penalty_rounds;
+} mko_lm_run;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,88 @@
+"""ctypes mirror of include/mikroptim_types.h (the plain-C layout of a system batch, residual
+settings, variable map and LM options shared by libmikroptim_hip.so and its host layer)."""
+
+import ctypes as C
+
+MAX_SURFACES = 64
+MAX_RINGS = 16
+
+SURFACE_STOP, SURFACE_PLANE, SURFACE_SPHERE = 0, 1, 2
+PUPIL_HALF, PUPIL_FULL, PUPIL_AXIS = 0, 1, 2
+SOURCE_COLLIMATED, SOURCE_POINT = 0, 1
+REF_LAST_SURFACE, REF_IMAGE_EXIT_PUPIL, REF_IMAGE_INFINITE = 0, 1, 2
+FAILURE_SOFT, FAILURE_HARD = 0, 1
+(STATUS_OK, STATUS_MISSED, STATUS_TIR, STATUS_BACKWARD, STATUS_UNSUPPORTED, STATUS_SOFT_MISSED,
+ STATUS_SOFT_TIR) = range(7)
+VAR_CURVATURE, VAR_THICKNESS, VAR_WORKING_DISTANCE = 0, 1, 2
+STENCIL_CENTRAL, STENCIL_ONE_SIDED, STENCIL_TRUNCATED = 0, 1, 2
+JACOBIAN_FD, JACOBIAN_ADJOINT = 0, 1
+(LM_RUNNING, LM_CONVERGED_MERIT, LM_CONVERGED_STEP, LM_CONVERGED_GRADIENT, LM_CONVERGED_STALL,
+ LM_MAX_ITERATIONS, LM_DAMPING_LIMIT, LM_NUMERICAL_ERROR) = range(8)
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+LM_STATUS_NAMES = ["running", "converged_merit", "converged_step", "converged_gradient",
+                   "converged_stall", "max_iterations", "damping_limit", "numerical_error"]
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_i32p = C.POINTER(C.c_int32)
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("variants", C.c_int32), ("surfaces", C.c_int32), ("wavelengths", C.c_int32),
+        ("fields", C.c_int32), ("source", C.c_int32), ("reference_sphere", C.c_int32),
+        ("reference_wavelength_nm", C.c_double), ("source_point", C.c_double * 3),
+        ("source_na", C.c_double),
+        ("type", _u8p), ("pupil_mode", _u8p), ("wavelength_weight", _dp), ("field_weight", _dp),
+        ("curvature", _dp), ("thickness", _dp), ("index", _dp), ("pupil_radius", _dp),
+        ("field_angle", _dp), ("field_target", _dp), ("reference_curvature", _dp),
+    ]
+
+
+class ResidualSettings(C.Structure):
+    _fields_ = [("rings", C.c_int32), ("spokes", C.c_int32), ("failure", C.c_int32),
+                ("fail_weight", C.c_double)]
+
+
+class Variables(C.Structure):
+    _fields_ = [("count", C.c_int32), ("type", _i32p), ("batch_surface", _i32p),
+                ("lower", _dp), ("upper", _dp), ("typical", _dp)]
+
+
+class LmOptions(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int32), ("ftol", C.c_double), ("xtol", C.c_double),
+        ("gtol", C.c_double), ("stall_window", C.c_int32), ("stall_tolerance", C.c_double),
+        ("initial_damping", C.c_double), ("max_damping", C.c_double), ("diag_floor", C.c_double),
+        ("reject_new_failures", C.c_int32), ("threads", C.c_int32),
+        ("jacobian_method", C.c_int32), ("fail_weight_factor", C.c_double),
+        ("max_fail_weight", C.c_double),
+    ]
+
+
+class LmRun(C.Structure):
+    _fields_ = [
+        ("merit", C.c_double), ("failed_rays", C.c_int64), ("hard_failed_rays", C.c_int64),
+        ("status", C.c_int32), ("iterations", C.c_int32), ("accepted", C.c_int32),
+        ("jacobians", C.c_int32), ("damping", C.c_double), ("gradient_norm", C.c_double),
+        ("model_decrease", C.c_double), ("fail_weight", C.c_double),
+        ("penalty_rounds", C.c_int32),
+    ]
+
+
+def lm_options(max_iterations=100, ftol=1e-10, xtol=1e-10, gtol=1e-10, stall_window=0,
+               stall_tolerance=1e-3, initial_damping=1e-3, max_damping=1e16, diag_floor=1e-12,
+               reject_new_failures=True, threads=0, jacobian_method=JACOBIAN_FD,
+               fail_weight_factor=10.0, max_fail_weight=1e14):
+    """LmOptions with the reference defaults (include/mikroptim/optim/levenberg_marquardt.hpp:85)."""
+    return LmOptions(max_iterations, ftol, xtol, gtol, stall_window, stall_tolerance,
+                     initial_damping, max_damping, diag_floor, int(reject_new_failures), threads,
+                     jacobian_method, fail_weight_factor, max_fail_weight)
+
+
+def ptr(array, ctype=C.c_double):
+    """Pointer to a contiguous numpy array (None -> NULL)."""
+    if array is None:
+        return None
+    return array.ctypes.data_as(C.POINTER(ctype))

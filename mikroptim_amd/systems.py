@@ -1,0 +1,242 @@
+"""Synthetic systems and variable sets of the measured workloads (built in code from a seed; the
+reference ships none of these systems, and there is no system-file reader in this build).
+
+singlet(...)        stop + biconvex N-BK7 singlet (c1 = +0.02, c2 = -0.005 1/mm, t = 5 mm),
+                    EPD 10 mm, fields 0 / 3.5 / 5 deg, 486.13 / 587.56 / 656.27 nm, image plane at
+                    the paraxial focus, field targets f tan(theta); curvatures perturbed
+                    uniformly by +-5 % from a seed (variant 0 unperturbed).
+objective(...)      24 refracting spherical surfaces (12 elements) behind the stop, collimated
+                    input, image-space NA 0.95 in air, 9 synthetic Sellmeier glasses
+                    nd 1.45 ... 1.90, 5 fields, 7 wavelengths 405 ... 700 nm; 48 variables
+                    (24 curvatures, 23 inner thicknesses, working distance), parameters jittered
+                    by N(0, 1 %) (Irwin-Hall) from a seed.
+The layout of the objective: a converging relay of ten thin positive elements (beam focused at
+sin u = 0.263), then two aplanatic-concentric menisci of index 1.90 that multiply sin u by n each
+(0.263 -> 0.5 -> 0.95): in trace direction, each meniscus' first surface is aplanatic for the point
+its exit beam converges to, and its second surface is concentric about that point. The pupil radius
+follows from the sine condition (a = NA f = 5.47 mm), the working distance is 0.6 mm, the field
+reaches 0.3 deg. With that, none of the 4096 x 17920 rays of the seeded batch fails (hard mode),
+checked with the oracle.
+"""
+
+import functools
+import math
+
+import numpy as np
+
+from . import abi, optics, rng
+from .batch import ResidualSettings, SystemBatch, VariableSet
+
+LAMBDA_REF = 587.56
+F_C_D = (486.13, 587.56, 656.27)
+
+
+def _index_table(glasses, wavelengths_nm):
+    """index[s][l] for a list of media per surface (None = air, else (B, C))."""
+    out = np.ones((len(glasses), len(wavelengths_nm)))
+    for s, g in enumerate(glasses):
+        if g is None:
+            continue
+        for l, nm in enumerate(wavelengths_nm):
+            out[s, l] = optics.sellmeier_index(g[0], g[1], nm / 1000.0)
+    return out
+
+
+def _paraxial(c, t, glasses):
+    n = [1.0 if g is None else optics.sellmeier_index(g[0], g[1], LAMBDA_REF / 1000.0)
+         for g in glasses]
+    return optics.focal_and_bfd(c, t, n)
+
+
+def singlet(variants=1024, seed=1, perturb=0.05):
+    """The singlet workload (one variant unperturbed, the rest curvatures * (1 + perturb xi))."""
+    nbk7 = (optics.NBK7_B, optics.NBK7_C)
+    glasses = [None, nbk7, None]
+    c = [0.0, 0.02, -0.005]
+    t = [5.0, 5.0, 0.0]
+    f, bfd = _paraxial(c, t, glasses)
+    t[2] = bfd
+    fields_deg = (0.0, 3.5, 5.0)
+    b = SystemBatch([abi.SURFACE_STOP, abi.SURFACE_SPHERE, abi.SURFACE_SPHERE], F_C_D,
+                    [1.0, 1.0, 1.0], [1.0, 1.0, 1.0],
+                    [abi.PUPIL_AXIS if a == 0.0 else abi.PUPIL_HALF for a in fields_deg],
+                    variants, LAMBDA_REF)
+    b.curvature[:] = c
+    b.thickness[:] = t
+    b.index[:] = _index_table(glasses, F_C_D)
+    b.pupil_radius[:] = 5.0
+    theta = np.radians(fields_deg)
+    b.field_angle[:] = theta
+    b.field_target[:] = f * np.tan(theta)
+    variables = singlet_variables()
+    x0 = variables.extract(b, 0)
+    xs = rng.perturbed_vectors(variables, x0, variants, seed=seed,
+                               relative=(perturb, 0.0, 0.0), first_is_base=True)
+    variables.apply(b, xs)
+    return b, variables
+
+
+def singlet_variables():
+    """Both curvatures, the lens thickness and the working distance."""
+    inf = math.inf
+    return VariableSet(
+        type=np.array([abi.VAR_CURVATURE, abi.VAR_CURVATURE, abi.VAR_THICKNESS,
+                       abi.VAR_WORKING_DISTANCE], dtype=np.int32),
+        batch_surface=np.array([1, 2, 1, 2], dtype=np.int32),
+        lower=np.array([-0.1, -0.1, 1.0, 10.0]), upper=np.array([0.1, 0.1, 20.0, 500.0]),
+        typical=np.array([0.01, 0.01, 1.0, 1.0]), names=["c[1]", "c[2]", "t[1]", "w"])
+
+
+def singlet_settings(failure=abi.FAILURE_HARD):
+    return ResidualSettings(rings=6, spokes=12, failure=failure)
+
+
+# ---- the 24-surface objective ------------------------------------------------------------------
+
+OBJECTIVE_GLASSES = [  # (nd, Vd) of the nine synthetic glasses
+    (1.45, 68.0), (1.50, 64.0), (1.55, 58.0), (1.60, 52.0), (1.65, 46.0), (1.70, 42.0),
+    (1.75, 36.0), (1.80, 30.0), (1.90, 40.0)]
+
+OBJECTIVE_NA = 0.95
+OBJECTIVE_PUPIL_RADIUS = 3.125
+OBJECTIVE_WD = 0.6
+
+
+@functools.lru_cache(maxsize=None)
+def _glass(i):
+    return optics.synthetic_glass(*OBJECTIVE_GLASSES[i])
+
+
+def _meniscus_chain(n, stages=2, working_distance=OBJECTIVE_WD, r_front=1.5, gaps=(0.8, 1.0)):
+    """Aplanatic-concentric menisci of index n, from the image point backwards (trace-direction
+    coordinates). Meniscus k: first surface radius r1 (center on +z); a beam converging to the
+    far aplanatic conjugate P (distance r1 (1 + n) from the vertex) converges after it to
+    O (distance r1 (1 + 1/n)); the second surface is centered on O. Returns the per-meniscus
+    (r1, thickness, r2, gap_after) upstream-first and the distance from the first vertex to the
+    point the upstream optics must focus on."""
+    out = []
+    r2 = working_distance
+    gap_after = working_distance
+    r1 = r_front
+    for k in range(stages):
+        if k > 0:
+            r1 = max(r1 * 2.2, (r2 + 1.0) / (1.0 + 1.0 / n))
+        thick = r1 * (1.0 + 1.0 / n) - r2
+        out.append((r1, thick, r2, gap_after))
+        p_dist = r1 * (1.0 + n)
+        gap_after = gaps[min(k, len(gaps) - 1)]
+        r2 = p_dist + gap_after
+    out.reverse()
+    return out, p_dist
+
+
+def _relay(scale, glasses, lens_t=1.6, air_t=0.4):
+    """Ten thin positive elements of equal power `scale` / 10, alternating bending."""
+    curv, thick, media = [], [], []
+    for i, g in enumerate(glasses):
+        gl = _glass(g)
+        n = optics.sellmeier_index(*gl, LAMBDA_REF / 1000.0)
+        d = scale / len(glasses) / (n - 1.0)
+        shape = 0.8 if i % 2 == 0 else 0.5
+        curv += [0.5 * d * (1.0 + shape), -0.5 * d * (1.0 - shape)]
+        thick += [lens_t, air_t]
+        media += [gl, None]
+    return curv, thick, media
+
+
+RELAY_GLASSES = (0, 1, 2, 3, 4, 5, 6, 7, 0, 1)
+
+
+def objective_design():
+    """Nominal 24-surface system: (types, curvature, thickness, media, pupil radius). The relay
+    power is solved paraxially at lambda_ref so that the relay focuses on the far aplanatic
+    conjugate of the first meniscus (RELAY_GAP behind its last vertex); the pupil radius is then
+    a = NA * f (sine condition), which makes the image-space NA 0.95 at lambda_ref."""
+    n_front = optics.sellmeier_index(*_glass(8), LAMBDA_REF / 1000.0)
+    menisci, p_dist = _meniscus_chain(n_front)
+
+    def build(scale):
+        c_r, t_r, m_r = _relay(scale, RELAY_GLASSES)
+        types = [abi.SURFACE_STOP] + [abi.SURFACE_SPHERE] * (len(c_r) + 2 * len(menisci))
+        curv = [0.0] + c_r
+        thick = [2.0] + t_r
+        media = [None] + m_r
+        thick[-1] = RELAY_GAP
+        for r1, t_m, r2, gap_after in menisci:
+            curv += [1.0 / r1, 1.0 / r2]
+            thick += [t_m, gap_after]
+            media += [_glass(8), None]
+        return types, curv, thick, media
+
+    def relay_bfd(scale):
+        c_r, t_r, m_r = _relay(scale, RELAY_GLASSES)
+        t_r[-1] = 0.0
+        return _paraxial([0.0] + c_r, [2.0] + t_r, [None] + m_r)[1]
+
+    lo, hi = 1e-3, 1.0  # more power -> shorter back focal distance
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if relay_bfd(mid) > p_dist + RELAY_GAP:
+            lo = mid
+        else:
+            hi = mid
+    types, curv, thick, media = build(0.5 * (lo + hi))
+    f, _ = _paraxial(curv, thick, media)
+    return types, np.array(curv), np.array(thick), media, OBJECTIVE_NA * f
+
+
+RELAY_GAP = 1.5
+
+
+def objective(variants=4096, seed=2, jitter=0.01, fields=5, wavelengths=7, max_field_deg=0.3):
+    types, c, t, media, a = objective_design()
+    wl = np.linspace(405.0, 700.0, wavelengths)
+    fields_deg = np.linspace(0.0, max_field_deg, fields)
+    b = SystemBatch(types, wl, np.ones(wavelengths), np.ones(fields),
+                    [abi.PUPIL_HALF] * fields, variants, LAMBDA_REF)
+    f, _ = _paraxial(list(c), list(t), media)
+    b.curvature[:] = c
+    b.thickness[:] = t
+    b.index[:] = _index_table(media, wl)
+    b.pupil_radius[:] = a
+    theta = np.radians(fields_deg)
+    b.field_angle[:] = theta
+    b.field_target[:] = f * np.tan(theta)
+    variables = objective_variables(b)
+    x0 = variables.extract(b, 0)
+    xs = rng.perturbed_vectors(variables, x0, variants, seed=seed,
+                               relative=(jitter, jitter, jitter), distribution="irwin_hall",
+                               first_is_base=True)
+    variables.apply(b, xs)
+    return b, variables
+
+
+def objective_variables(batch):
+    S = batch.surfaces
+    types, surf, lo, hi, typ, names = [], [], [], [], [], []
+    for s in range(1, S):
+        types.append(abi.VAR_CURVATURE)
+        surf.append(s)
+        lo.append(-math.inf)
+        hi.append(math.inf)
+        typ.append(0.01)
+        names.append(f"c[{s}]")
+    for s in range(1, S - 1):
+        types.append(abi.VAR_THICKNESS)
+        surf.append(s)
+        lo.append(0.05)
+        hi.append(50.0)
+        typ.append(1.0)
+        names.append(f"t[{s}]")
+    types.append(abi.VAR_WORKING_DISTANCE)
+    surf.append(S - 1)
+    lo.append(0.01)
+    hi.append(10.0)
+    typ.append(1.0)
+    names.append("w")
+    return VariableSet(np.array(types, np.int32), np.array(surf, np.int32), np.array(lo),
+                       np.array(hi), np.array(typ), names)
+
+
+def objective_settings(failure=abi.FAILURE_HARD):
+    return ResidualSettings(rings=16, spokes=32, failure=failure)

@@ -1,0 +1,1863 @@
+// CPU FP64 oracle of the batched forward-and-Jacobian path. TEST INFRASTRUCTURE ONLY: this
+// library is the checker used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+// leg; the product (mikroptim_amd, libmikroptim_hip.so) never links, loads or calls it.
+//
+// A from-scratch restatement of the reference's CPU algorithms in plain C++20, compiled with
+// -ffp-contract=off like the reference (CMakeLists.txt:27-28) so that operation order fixes every
+// rounding. Each function names the reference file:line whose formulas and summation order it
+// follows. Scope: coaxial systems of stop/plane/sphere surfaces (no ghost plates, no
+// decenter/tilt), collimated and point sources, the three reference-sphere kinds, soft and hard
+// ray failure, curvature/thickness/working-distance variables.
+//
+// How each part is pinned (tests/test_oracle_*.py): bitwise against the reference's own CPU
+// code built from its sources into oracle/_ref (residuals_cpu, merit_cpu, fd_jacobian_cpu,
+// adjoint_jacobian_cpu, normal_equations_host, levenberg_marquardt, perturbed_vector, the
+// Gauss-Legendre table); against the reference's golden vectors (mt19937_64/splitmix64 values and
+// perturbed vectors of tests/test_variant_generator.cpp:72-101, kFdStepFactor); and against closed
+// forms (plane-parallel plate, Snell's law, aplanatic points, exact spherical waves, GL moments,
+// merit = weighted sum of wavefront RMS^2, adjoint vs Richardson-extrapolated differences, piston
+// invariant, LM on a linear problem).
+
+#include "mikroptim_types.h"
+#include "gauss_legendre_table.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <mutex>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace orc {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+
+struct Vec3 {
+    double x, y, z;
+};
+inline Vec3 operator+(Vec3 a, Vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline Vec3 operator-(Vec3 a, Vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline Vec3 operator*(double s, Vec3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline Vec3& operator+=(Vec3& a, Vec3 b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+    return a;
+}
+inline double dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 cross(Vec3 a, Vec3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(Vec3 a) { return std::sqrt(dot(a, a)); }
+
+struct Ray {
+    Vec3 p, d;
+    double opl;
+};
+
+enum Status : uint8_t {
+    Ok = MKO_STATUS_OK,
+    Missed = MKO_STATUS_MISSED,
+    TIR = MKO_STATUS_TIR,
+    Backward = MKO_STATUS_BACKWARD,
+    Unsupported = MKO_STATUS_UNSUPPORTED,
+    SoftMissed = MKO_STATUS_SOFT_MISSED,
+    SoftTIR = MKO_STATUS_SOFT_TIR,
+};
+inline bool is_soft(Status s) { return s == SoftMissed || s == SoftTIR; }
+inline bool in_piston(Status s) { return s == Ok || is_soft(s); }
+inline bool is_hard(Status s) { return !in_piston(s); }
+
+constexpr uint8_t kBranchRegular = 0, kBranchSoftMissed = 1, kBranchSoftTIR = 2;
+
+// SoftFailure, include/mikroptim/trace/surface_ops.hpp:94
+struct Soft {
+    double penalty = 0.0;
+    Status first = Ok;
+    void add(double amount, Status kind) {
+        penalty += amount;
+        if (first == Ok) first = kind;
+    }
+};
+
+// ---- surface operations: include/mikroptim/trace/surface_ops.hpp:122-203 --------------------
+
+inline double transfer_to_tangent_plane(Ray& r, double t) {
+    r.p.z -= t;
+    const double s = -r.p.z / r.d.z;
+    r.p += s * r.d;
+    r.p.z = 0.0;
+    return s;
+}
+
+inline Status intersect_sphere(Ray& r, double c, double& s, Soft* soft) {
+    const double H = c * (r.p.x * r.p.x + r.p.y * r.p.y);
+    const double B = r.d.z - c * (r.p.x * r.d.x + r.p.y * r.d.y);
+    const double D = B * B - c * H;
+    if (D < 0.0) {
+        if (soft == nullptr || (!(c > 0.0) && !(c < 0.0))) return Missed;
+        soft->add(-D, SoftMissed);
+        s = B / c;
+        r.p += s * r.d;
+        return SoftMissed;
+    }
+    const double denom = B + std::sqrt(D);
+    if (!(denom > 0.0)) return Missed;
+    s = H / denom;
+    r.p += s * r.d;
+    return Ok;
+}
+
+inline Status refract_sphere(Ray& r, double c, double mu, Soft* soft, bool off_surface) {
+    Vec3 N{-c * r.p.x, -c * r.p.y, 1.0 - c * r.p.z};
+    if (off_surface) N = (1.0 / norm(N)) * N;
+    const double cosI = dot(r.d, N);
+    const double k = 1.0 - mu * mu * (1.0 - cosI * cosI);
+    if (k < 0.0) {
+        if (soft == nullptr) return TIR;
+        soft->add(-k, SoftTIR);
+        const Vec3 t = r.d - cosI * N;
+        r.d = (1.0 / norm(t)) * t;
+        return SoftTIR;
+    }
+    r.d = mu * r.d + (std::sqrt(k) - mu * cosI) * N;
+    return Ok;
+}
+
+// propagate_surface_local, include/mikroptim/trace/trace.hpp:175-229 (no ghost plates)
+inline Status propagate_surface(Ray& ray, double t, uint8_t kind, double c, double n_before,
+                                double n_after, Soft* soft, uint8_t* branch) {
+    double curv;
+    switch (kind) {
+    case MKO_SURFACE_STOP:
+    case MKO_SURFACE_PLANE: curv = 0.0; break;
+    case MKO_SURFACE_SPHERE: curv = c; break;
+    default: return Unsupported;
+    }
+    if (!(ray.d.z > 0.0)) return Backward;
+    const double s0 = transfer_to_tangent_plane(ray, t);
+    double s = 0.0;
+    const Status hit = intersect_sphere(ray, curv, s, soft);
+    if (hit != Ok && hit != SoftMissed) return hit;
+    ray.opl += n_before * (s0 + s);
+    const Status refr = refract_sphere(ray, curv, n_before / n_after, soft, hit == SoftMissed);
+    if (branch) {
+        *branch = static_cast<uint8_t>((hit == SoftMissed ? kBranchSoftMissed : kBranchRegular) |
+                                       (refr == SoftTIR ? kBranchSoftTIR : kBranchRegular));
+    }
+    return refr == SoftTIR ? Ok : refr;
+}
+
+// ---- one variant's surface table -------------------------------------------------------------
+
+struct Table {
+    int S, L;
+    const uint8_t* type;
+    const double* c;
+    const double* t;
+    const double* n; // [S][L]
+};
+
+// AdjointRecord, include/mikroptim/trace/adjoint_record.hpp:66
+struct Record {
+    int count = 0;
+    Vec3 p[MKO_MAX_SURFACES];
+    Vec3 d[MKO_MAX_SURFACES];
+    uint8_t branch[MKO_MAX_SURFACES];
+    double image_s = 0.0;
+    uint8_t image_branch = 0;
+};
+
+// trace_segment / trace_ray_recorded, include/mikroptim/trace/trace.hpp:277-358
+inline Status trace_ray(const Table& tb, int l, Ray& ray, Soft* soft, Record* rec, int* failed) {
+    if (rec) {
+        rec->count = 1;
+        rec->p[0] = ray.p;
+        rec->d[0] = ray.d;
+        rec->branch[0] = kBranchRegular;
+        rec->image_s = 0.0;
+        rec->image_branch = kBranchRegular;
+    }
+    int soft_surface = -1;
+    for (int s = 1; s < tb.S; ++s) {
+        const double nb = tb.n[(s - 1) * tb.L + l];
+        const double na = tb.n[s * tb.L + l];
+        uint8_t branch = kBranchRegular;
+        const Status st =
+            propagate_surface(ray, tb.t[s - 1], tb.type[s], tb.c[s], nb, na, soft, &branch);
+        if (st != Ok) {
+            if (failed) *failed = s;
+            return st;
+        }
+        if (soft && soft_surface < 0 && soft->first != Ok) soft_surface = s;
+        if (rec) {
+            rec->count = s + 1;
+            rec->p[s] = ray.p;
+            rec->d[s] = ray.d;
+            rec->branch[s] = branch;
+        }
+    }
+    if (failed) *failed = soft_surface;
+    return soft ? soft->first : Ok;
+}
+
+struct RefSphere {
+    Vec3 center;
+    double kappa;
+    int kind;
+};
+
+// reference_sphere_step, include/mikroptim/trace/reference_sphere.hpp:56-90
+inline Status reference_sphere_step(Ray& ray, Vec3 center, double n_image, double* s_out,
+                                    Soft* soft) {
+    const Vec3 q = ray.p - center;
+    const double b = -dot(q, ray.d);
+    const double g = dot(ray.p, ray.p - 2.0 * center);
+    const double disc = b * b - g;
+    if (disc < 0.0) {
+        const double radius_sq = dot(center, center);
+        if (soft == nullptr || !(radius_sq > 0.0)) return Missed;
+        soft->add(-disc / radius_sq, SoftMissed);
+        ray.p += b * ray.d;
+        ray.opl += n_image * b;
+        if (s_out) *s_out = b;
+        return SoftMissed;
+    }
+    const double root = std::sqrt(disc);
+    const bool real_image = center.z >= 0.0;
+    const double denom = real_image ? b + root : b - root;
+    if (!(real_image ? denom > 0.0 : denom < 0.0)) return Missed;
+    const double s = g / denom;
+    ray.p += s * ray.d;
+    ray.opl += n_image * s;
+    if (s_out) *s_out = s;
+    return Ok;
+}
+
+// image_sphere_step, include/mikroptim/trace/reference_sphere.hpp:110-143
+inline Status image_sphere_step(Ray& ray, Vec3 center, double kappa, double n_image,
+                                double* s_out, Soft* soft) {
+    const Vec3 q = ray.p - center;
+    const double b = -dot(q, ray.d);
+    const Vec3 c{q.y * ray.d.z - q.z * ray.d.y, q.z * ray.d.x - q.x * ray.d.z,
+                 q.x * ray.d.y - q.y * ray.d.x};
+    const double t = -dot(c, c);
+    const double disc = 1.0 + kappa * kappa * t;
+    double step;
+    Status st = Ok;
+    if (disc >= 0.0) {
+        step = b - kappa * t / (std::sqrt(disc) + 1.0);
+    } else if (soft != nullptr && disc < 0.0) {
+        step = b - kappa * t;
+        st = SoftMissed;
+    } else {
+        return Missed;
+    }
+    if (!(step >= 0.0) && !(step < 0.0)) return Missed;
+    if (st == SoftMissed) soft->add(-disc, SoftMissed);
+    ray.p += b * ray.d;
+    ray.opl += n_image * step;
+    if (s_out) *s_out = step;
+    return st;
+}
+
+// trace_to_reference_sphere, include/mikroptim/trace/reference_sphere.hpp:183-222
+inline Status trace_to_reference(const Table& tb, int l, Ray& ray, const RefSphere& ref,
+                                 Soft* soft, Record* rec) {
+    const Status st = trace_ray(tb, l, ray, soft, rec, nullptr);
+    if (is_hard(st)) return st;
+    if (!(ray.d.z > 0.0)) return Backward;
+    const double n_image = tb.n[(tb.S - 1) * tb.L + l];
+    double s_ref = 0.0;
+    const Status rst = ref.kind == MKO_REF_LAST_SURFACE
+                           ? reference_sphere_step(ray, ref.center, n_image, &s_ref, soft)
+                           : image_sphere_step(ray, ref.center, ref.kappa, n_image, &s_ref, soft);
+    if (is_hard(rst)) return rst;
+    if (rec) {
+        rec->image_s = s_ref;
+        rec->image_branch = rst == SoftMissed ? kBranchSoftMissed : kBranchRegular;
+    }
+    if (st != Ok) return st;
+    return rst;
+}
+
+// ---- pupil quadrature: include/mikroptim/pupil/quadrature.hpp:40-132 ------------------------
+
+struct Quadrature {
+    int rings, spokes, mode;
+    double u[MKO_MAX_RINGS];
+    double w[MKO_MAX_RINGS];
+};
+
+inline Quadrature make_quadrature(int rings, int spokes, int mode) {
+    if (rings < 1 || rings > MKO_MAX_RINGS) throw std::invalid_argument("rings outside [1, 16]");
+    if (spokes < 1) throw std::invalid_argument("spokes < 1");
+    if (mode != MKO_PUPIL_HALF && mode != MKO_PUPIL_FULL && mode != MKO_PUPIL_AXIS)
+        throw std::invalid_argument("unknown pupil mode");
+    Quadrature q{};
+    q.rings = rings;
+    q.spokes = mode == MKO_PUPIL_AXIS ? 1 : spokes;
+    q.mode = mode;
+    const int off = rings * (rings - 1) / 2;
+    for (int i = 0; i < rings; ++i) {
+        q.u[i] = mko_gl_u[off + i];
+        q.w[i] = mko_gl_w[off + i];
+    }
+    return q;
+}
+
+struct Sample {
+    double rho, phi, weight;
+};
+
+inline Sample sample(const Quadrature& q, int index) {
+    const int i = index / q.spokes;
+    const int j = index - i * q.spokes;
+    Sample s;
+    s.rho = std::sqrt(q.u[i]);
+    s.weight = q.w[i] / double(q.spokes);
+    switch (q.mode) {
+    case MKO_PUPIL_HALF: s.phi = double(2 * j + 1) * kPi / double(2 * q.spokes); break;
+    case MKO_PUPIL_FULL: s.phi = double(2 * j + 1) * kPi / double(q.spokes); break;
+    default: s.phi = 0.0; break;
+    }
+    return s;
+}
+
+inline double residual_weight(const Sample& s) { return std::sqrt(s.weight); }
+
+// ---- batch access ------------------------------------------------------------------------------
+
+struct Batch {
+    mko_batch b;
+    size_t V() const { return size_t(b.variants); }
+    int S() const { return b.surfaces; }
+    int L() const { return b.wavelengths; }
+    int F() const { return b.fields; }
+    Table table(int v) const {
+        const size_t S_ = size_t(S());
+        return Table{S(), L(), b.type, b.curvature + size_t(v) * S_, b.thickness + size_t(v) * S_,
+                     b.index + size_t(v) * S_ * size_t(L())};
+    }
+};
+
+// collimated_ray / point_source_ray, include/mikroptim/trace/trace.hpp:109-131, selected as in
+// batch_source_ray, include/mikroptim/batch/system_batch_view.hpp:190-201
+inline Ray source_ray(const Batch& B, int v, int f, int l, const Sample& s) {
+    Ray ray;
+    if (B.b.source == MKO_SOURCE_POINT) {
+        const double sin_max =
+            B.b.source_na / B.b.index[(size_t(v) * size_t(B.S()) + 0) * size_t(B.L()) + size_t(l)];
+        const double sr = sin_max * s.rho;
+        ray.p = Vec3{B.b.source_point[0], B.b.source_point[1], B.b.source_point[2]};
+        ray.d = Vec3{sr * std::sin(s.phi), sr * std::cos(s.phi), std::sqrt((1.0 - sr) * (1.0 + sr))};
+        ray.opl = 0.0;
+        return ray;
+    }
+    const double a = B.b.pupil_radius[v];
+    const double theta = B.b.field_angle[size_t(v) * size_t(B.F()) + size_t(f)];
+    const double r = a * s.rho;
+    ray.p = Vec3{r * std::sin(s.phi), r * std::cos(s.phi), 0.0};
+    ray.d = Vec3{0.0, std::sin(theta), std::cos(theta)};
+    ray.opl = dot(ray.p, ray.d);
+    return ray;
+}
+
+// batch_reference, include/mikroptim/merit/ray_residual.hpp:230-252 (ideal image point)
+inline RefSphere reference_of(const Batch& B, const Table& tb, int v, int f) {
+    const size_t i = size_t(v) * size_t(B.F()) + size_t(f);
+    RefSphere r{Vec3{0.0, B.b.field_target[i], tb.t[tb.S - 1]}, 0.0, B.b.reference_sphere};
+    if (B.b.reference_sphere != MKO_REF_LAST_SURFACE) r.kappa = B.b.reference_curvature[i];
+    return r;
+}
+
+// tuple_weight / residual_scale / penalty_scale, include/mikroptim/merit/ray_residual.hpp:301-385
+inline double tuple_weight(const Batch& B, int f, int l) {
+    const double wf = B.b.field_weight[f];
+    const double wl = B.b.wavelength_weight[l];
+    if (!(wf > 0.0) || !(wl > 0.0)) return 0.0;
+    double sum_f = 0.0;
+    for (int i = 0; i < B.F(); ++i) sum_f += B.b.field_weight[i];
+    double sum_l = 0.0;
+    for (int i = 0; i < B.L(); ++i) sum_l += B.b.wavelength_weight[i];
+    return (wf / sum_f) * (wl / sum_l);
+}
+inline double residual_scale(const Batch& B, int f, int l) {
+    const double w = tuple_weight(B, f, l);
+    if (!(w > 0.0)) return 0.0;
+    const double lambda_ref_mm = B.b.reference_wavelength_nm * 1e-6;
+    return std::sqrt(w) / lambda_ref_mm;
+}
+inline double penalty_scale(const Batch& B, int f, int l, double fail_weight) {
+    const double w = tuple_weight(B, f, l);
+    if (!(w > 0.0)) return 0.0;
+    return std::sqrt(fail_weight * w);
+}
+inline double failure_penalty(Status st, double soft_penalty) {
+    if (st == Ok) return 0.0;
+    if (is_soft(st)) return soft_penalty;
+    return soft_penalty + 1.0;
+}
+
+// ResidualLayout, src/batch/batch_residuals.cpp:25-57
+struct Layout {
+    int F = 0, L = 0;
+    mko_residual_settings settings{};
+    std::vector<Quadrature> quad;
+    std::vector<int64_t> field_offset;
+    std::vector<int> samples;
+    int64_t rays = 0;
+    bool soft() const { return settings.failure == MKO_FAILURE_SOFT; }
+    int64_t penalty_rows() const { return soft() ? rays : 0; }
+    int64_t trace_rows() const { return rays + penalty_rows(); }
+    int64_t offset(int f, int l) const { return field_offset[f] + int64_t(l) * samples[f]; }
+};
+
+inline Layout make_layout(const Batch& B, const mko_residual_settings& s) {
+    if (s.failure != MKO_FAILURE_SOFT && s.failure != MKO_FAILURE_HARD)
+        throw std::invalid_argument("unknown failure mode");
+    if (!(s.fail_weight >= 0.0) || !std::isfinite(s.fail_weight))
+        throw std::invalid_argument("fail_weight must be finite and >= 0");
+    Layout lay;
+    lay.F = B.F();
+    lay.L = B.L();
+    lay.settings = s;
+    int64_t offset = 0;
+    for (int f = 0; f < lay.F; ++f) {
+        const Quadrature q = make_quadrature(s.rings, s.spokes, B.b.pupil_mode[f]);
+        const int n = q.rings * q.spokes;
+        lay.quad.push_back(q);
+        lay.field_offset.push_back(offset);
+        lay.samples.push_back(n);
+        offset += int64_t(lay.L) * n;
+    }
+    lay.rays = offset;
+    return lay;
+}
+
+// opd_statistics, include/mikroptim/merit/opd.hpp:449-487
+inline void opd_statistics(const double* opl, const double* weight, const Status* st, int n,
+                           double* opd) {
+    double valid = 0.0;
+    double sum = 0.0;
+    for (int k = 0; k < n; ++k) {
+        if (in_piston(st[k])) {
+            valid += weight[k];
+            sum += weight[k] * opl[k];
+        }
+    }
+    if (!(valid > 0.0)) {
+        for (int k = 0; k < n; ++k) opd[k] = 0.0;
+        return;
+    }
+    const double piston = sum / valid;
+    for (int k = 0; k < n; ++k) {
+        double e = 0.0;
+        if (in_piston(st[k])) e = opl[k] - piston;
+        opd[k] = e;
+    }
+}
+
+// variant_residuals, src/batch/batch_residuals.cpp:181-231
+void variant_residuals(const Batch& B, int v, const Layout& lay, double* residuals, double* opd,
+                       uint8_t* status) {
+    const Table tb = B.table(v);
+    const size_t rays = size_t(lay.rays);
+    std::vector<double> opl, weight, penalty;
+    std::vector<Status> st;
+    std::vector<Sample> pupil;
+    for (int f = 0; f < lay.F; ++f) {
+        const Quadrature& q = lay.quad[f];
+        const int n = lay.samples[f];
+        pupil.resize(n);
+        weight.resize(n);
+        opl.resize(n);
+        penalty.resize(n);
+        st.resize(n);
+        for (int k = 0; k < n; ++k) {
+            pupil[k] = sample(q, k);
+            weight[k] = pupil[k].weight;
+        }
+        const RefSphere ref = reference_of(B, tb, v, f);
+        for (int l = 0; l < lay.L; ++l) {
+            const size_t o = size_t(lay.offset(f, l));
+            for (int k = 0; k < n; ++k) {
+                Soft soft;
+                Ray ray = source_ray(B, v, f, l, pupil[k]);
+                st[k] = trace_to_reference(tb, l, ray, ref, lay.soft() ? &soft : nullptr, nullptr);
+                opl[k] = ray.opl;
+                penalty[k] = soft.penalty;
+            }
+            opd_statistics(opl.data(), weight.data(), st.data(), n, opd + o);
+            const double scale = residual_scale(B, f, l);
+            for (int k = 0; k < n; ++k) {
+                residuals[o + k] = scale * residual_weight(pupil[k]) * opd[o + k];
+                status[o + k] = st[k];
+            }
+            if (lay.soft()) {
+                const double pscale = penalty_scale(B, f, l, lay.settings.fail_weight);
+                for (int k = 0; k < n; ++k) {
+                    residuals[rays + o + k] =
+                        pscale * residual_weight(pupil[k]) * failure_penalty(st[k], penalty[k]);
+                }
+            }
+        }
+    }
+}
+
+// variant_merit, src/batch/batch_merit.cpp:62-89
+void variant_merit(const Batch& B, int v, const Layout& lay, double& merit, int64_t& failed,
+                   int64_t& hard_failed) {
+    const size_t R = size_t(lay.trace_rows());
+    std::vector<double> r(R, 0.0), opd(size_t(lay.rays));
+    std::vector<uint8_t> st(size_t(lay.rays));
+    variant_residuals(B, v, lay, r.data(), opd.data(), st.data());
+    double sum = 0.0;
+    for (int f = 0; f < lay.F; ++f) {
+        for (int l = 0; l < lay.L; ++l) {
+            const int64_t o = lay.offset(f, l);
+            double tuple = 0.0;
+            for (int k = 0; k < lay.samples[f]; ++k) {
+                const double x = r[size_t(o + k)];
+                tuple += x * x;
+            }
+            if (lay.soft()) {
+                double pen = 0.0;
+                for (int k = 0; k < lay.samples[f]; ++k) {
+                    const double p = r[size_t(lay.rays + o + k)];
+                    pen += p * p;
+                }
+                tuple += pen;
+            }
+            sum += tuple;
+        }
+    }
+    merit = sum;
+    failed = 0;
+    hard_failed = 0;
+    for (uint8_t s : st) {
+        failed += s != Ok ? 1 : 0;
+        hard_failed += is_hard(Status(s)) ? 1 : 0;
+    }
+}
+
+// contiguous split over host threads, result independent of the thread count
+// (src/optim/parallel_variants.hpp)
+void parallel_for(int first, int last, int threads, const std::function<void(int)>& fn) {
+    const int n = last - first;
+    if (n <= 0) return;
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    threads = std::min(threads, n);
+    if (threads <= 1) {
+        for (int i = first; i < last; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::exception_ptr error;
+    std::mutex m;
+    for (int t = 0; t < threads; ++t) {
+        const int a = first + int(int64_t(n) * t / threads);
+        const int b = first + int(int64_t(n) * (t + 1) / threads);
+        pool.emplace_back([&, a, b] {
+            try {
+                for (int i = a; i < b; ++i) fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> g(m);
+                if (!error) error = std::current_exception();
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    if (error) std::rethrow_exception(error);
+}
+
+// ---- variables, FD pattern and perturbations ---------------------------------------------------
+
+struct Vars {
+    int P = 0;
+    std::vector<int> type, surface;
+    std::vector<double> lower, upper, typical;
+};
+
+inline Vars make_vars(const mko_variables* v, int S) {
+    Vars o;
+    o.P = v->count;
+    for (int k = 0; k < o.P; ++k) {
+        if (v->type[k] < MKO_VAR_CURVATURE || v->type[k] > MKO_VAR_WORKING_DISTANCE)
+            throw std::invalid_argument("unsupported variable type");
+        if (v->batch_surface[k] < 0 || v->batch_surface[k] >= S)
+            throw std::invalid_argument("variable surface out of range");
+        if (!(v->typical[k] > 0.0)) throw std::invalid_argument("typical must be > 0");
+        o.type.push_back(v->type[k]);
+        o.surface.push_back(v->batch_surface[k]);
+        o.lower.push_back(v->lower[k]);
+        o.upper.push_back(v->upper[k]);
+        o.typical.push_back(v->typical[k]);
+    }
+    return o;
+}
+
+// VariableSet::apply for curvature / thickness / working distance, src/optim/variable_set.cpp:237
+inline void apply(const Vars& vars, int S, double* curvature, double* thickness, int v,
+                  const double* x) {
+    for (int k = 0; k < vars.P; ++k) {
+        if (!std::isfinite(x[k])) throw std::invalid_argument("variable value not finite");
+        if (vars.type[k] == MKO_VAR_CURVATURE)
+            curvature[size_t(v) * S + vars.surface[k]] = x[k];
+        else
+            thickness[size_t(v) * S + vars.surface[k]] = x[k];
+    }
+}
+
+inline void clamp_to_bounds(const Vars& vars, double* x) {
+    for (int k = 0; k < vars.P; ++k) x[k] = std::min(std::max(x[k], vars.lower[k]), vars.upper[k]);
+}
+
+constexpr double kFdStepFactor = 6.0554544523933395e-06; // 2^(-52/3), variant_generator.hpp:81
+
+struct FdPattern {
+    std::vector<double> nominal, plus, minus, step_plus, step_minus;
+    std::vector<uint8_t> stencil;
+};
+
+// fd_pattern (central scheme), src/optim/variant_generator.cpp:177-237
+inline FdPattern fd_pattern(const Vars& vars, const double* x0) {
+    FdPattern pat;
+    const int P = vars.P;
+    pat.nominal.resize(P);
+    pat.plus.resize(P);
+    pat.minus.resize(P);
+    pat.step_plus.resize(P);
+    pat.step_minus.resize(P);
+    pat.stencil.resize(P);
+    for (int k = 0; k < P; ++k) {
+        const double x = x0[k];
+        if (!std::isfinite(x) || !(x >= vars.lower[k] && x <= vars.upper[k]))
+            throw std::invalid_argument("x0 not finite or outside bounds");
+        const double h = kFdStepFactor * std::max(std::abs(x), vars.typical[k]);
+        pat.nominal[k] = h;
+        const double up = x + h;
+        const double down = x - h;
+        double xa, xb;
+        uint8_t stencil = MKO_STENCIL_CENTRAL;
+        if (up <= vars.upper[k] && down >= vars.lower[k]) {
+            xa = up;
+            xb = down;
+        } else {
+            const bool towards_upper = vars.upper[k] - x >= x - vars.lower[k];
+            const double far = towards_upper ? x + 2.0 * h : x - 2.0 * h;
+            if (towards_upper ? far <= vars.upper[k] : far >= vars.lower[k]) {
+                stencil = MKO_STENCIL_ONE_SIDED;
+                xa = towards_upper ? up : down;
+                xb = far;
+            } else {
+                stencil = MKO_STENCIL_TRUNCATED;
+                xa = up <= vars.upper[k] ? up : vars.upper[k];
+                xb = down >= vars.lower[k] ? down : vars.lower[k];
+            }
+        }
+        pat.stencil[k] = stencil;
+        pat.plus[k] = xa;
+        pat.minus[k] = xb;
+        pat.step_plus[k] = xa - x;
+        pat.step_minus[k] = xb - x;
+    }
+    return pat;
+}
+
+// fd_derivative, include/mikroptim/optim/fd_formula.hpp:39-53
+inline double fd_derivative(uint8_t stencil, double r0, double ra, double rb, double da,
+                            double db) {
+    if (stencil == MKO_STENCIL_ONE_SIDED) {
+        const double ga = ra - r0;
+        const double gb = rb - r0;
+        return (db * db * ga - da * da * gb) / (da * db * (db - da));
+    }
+    return (ra - rb) / (da - db);
+}
+
+// splitmix64 / variant_seed / unit_uniform, src/optim/variant_generator.cpp:67-82
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+inline uint64_t variant_seed(uint64_t seed, uint64_t j) {
+    return splitmix64(seed + j * 0x9E3779B97F4A7C15ULL);
+}
+inline double unit_uniform(uint64_t bits) { return double(bits >> 11) * 0x1.0p-53; }
+
+// perturbed_vector, src/optim/variant_generator.cpp:95-127; rel/abs per variable type
+inline void perturbed_vector(const Vars& vars, const double* x0, const double* relative,
+                             const double* absolute, int distribution, uint64_t seed,
+                             int first_is_base, uint64_t j, double* y) {
+    for (int k = 0; k < vars.P; ++k) y[k] = x0[k];
+    if (first_is_base && j == 0) return;
+    std::mt19937_64 engine(variant_seed(seed, j));
+    for (int k = 0; k < vars.P; ++k) {
+        double xi;
+        if (distribution == 0) {
+            xi = 2.0 * unit_uniform(engine()) - 1.0;
+        } else {
+            double sum = 0.0;
+            for (int i = 0; i < 12; ++i) sum += unit_uniform(engine());
+            xi = sum - 6.0;
+        }
+        const int t = vars.type[k];
+        const double s = std::fma(relative[t], std::abs(x0[k]), absolute[t]);
+        y[k] = std::fma(s, xi, x0[k]);
+    }
+    clamp_to_bounds(vars, y);
+}
+
+// ---- residual batches ----------------------------------------------------------------------------
+
+// A batch whose per-variant arrays this oracle owns (FD batches, point batches).
+struct OwnedBatch {
+    mko_batch b{};
+    std::vector<double> curvature, thickness, index, pupil, fangle, ftarget, refcurv;
+    // copies variant `from` of `src` into variant `to`
+    void init(const mko_batch& src, int V) {
+        b = src;
+        b.variants = V;
+        const size_t S = size_t(src.surfaces), L = size_t(src.wavelengths), F = size_t(src.fields);
+        curvature.assign(size_t(V) * S, 0.0);
+        thickness.assign(size_t(V) * S, 0.0);
+        index.assign(size_t(V) * S * L, 0.0);
+        pupil.assign(size_t(V), 0.0);
+        fangle.assign(size_t(V) * F, 0.0);
+        ftarget.assign(size_t(V) * F, 0.0);
+        if (src.reference_curvature) refcurv.assign(size_t(V) * F, 0.0);
+        b.curvature = curvature.data();
+        b.thickness = thickness.data();
+        b.index = index.data();
+        b.pupil_radius = pupil.data();
+        b.field_angle = fangle.data();
+        b.field_target = ftarget.data();
+        b.reference_curvature = src.reference_curvature ? refcurv.data() : nullptr;
+    }
+    void copy_variant(const mko_batch& src, int from, int to) {
+        const size_t S = size_t(src.surfaces), L = size_t(src.wavelengths), F = size_t(src.fields);
+        std::copy_n(src.curvature + from * S, S, curvature.data() + to * S);
+        std::copy_n(src.thickness + from * S, S, thickness.data() + to * S);
+        std::copy_n(src.index + from * S * L, S * L, index.data() + to * S * L);
+        pupil[to] = src.pupil_radius[from];
+        std::copy_n(src.field_angle + from * F, F, fangle.data() + to * F);
+        std::copy_n(src.field_target + from * F, F, ftarget.data() + to * F);
+        if (src.reference_curvature)
+            std::copy_n(src.reference_curvature + from * F, F, refcurv.data() + to * F);
+    }
+};
+
+void residuals_all(const Batch& B, const Layout& lay, double* residuals, double* opd,
+                   uint8_t* status, int threads) {
+    const size_t R = size_t(lay.trace_rows());
+    const size_t rays = size_t(lay.rays);
+    parallel_for(0, B.b.variants, threads, [&](int v) {
+        variant_residuals(B, v, lay, residuals + size_t(v) * R, opd + size_t(v) * rays,
+                          status + size_t(v) * rays);
+    });
+}
+
+// FD Jacobian (central scheme), src/optim/fd_jacobian.cpp:46-182: batch of 2P+1 variants per
+// run (variant 0 = x0, 2k+1 = x0 + d+ e_k, 2k+2 = x0 + d- e_k), RESIDUALS of all, then
+// fd_derivative per row; J[run][p][R], r0[run][R], status_changes[run][p].
+void fd_jacobian(const Batch& base, const Layout& lay, const Vars& vars, int runs, const double* x0,
+                 double* J, double* r0, int64_t* changes, uint8_t* status0, int threads) {
+    const int P = vars.P;
+    const int n = 2 * P + 1;
+    const int S = base.S();
+    OwnedBatch fd;
+    fd.init(base.b, runs * n);
+    std::vector<FdPattern> pats;
+    std::vector<double> y(P);
+    for (int i = 0; i < runs; ++i) {
+        const double* xi = x0 + size_t(i) * P;
+        const int tmpl = base.b.variants == 1 ? 0 : i;
+        pats.push_back(fd_pattern(vars, xi));
+        for (int j = 0; j < n; ++j) fd.copy_variant(base.b, tmpl, i * n + j);
+        std::copy_n(xi, P, y.data());
+        apply(vars, S, fd.curvature.data(), fd.thickness.data(), i * n, y.data());
+        for (int k = 0; k < P; ++k) {
+            y[k] = pats[i].plus[k];
+            apply(vars, S, fd.curvature.data(), fd.thickness.data(), i * n + 2 * k + 1, y.data());
+            y[k] = pats[i].minus[k];
+            apply(vars, S, fd.curvature.data(), fd.thickness.data(), i * n + 2 * k + 2, y.data());
+            y[k] = xi[k];
+        }
+    }
+    const Batch FB{fd.b};
+    const size_t R = size_t(lay.trace_rows());
+    const size_t rays = size_t(lay.rays);
+    std::vector<double> res(size_t(runs) * n * R), opd(size_t(runs) * n * rays);
+    std::vector<uint8_t> st(size_t(runs) * n * rays);
+    residuals_all(FB, lay, res.data(), opd.data(), st.data(), threads);
+    for (int i = 0; i < runs; ++i) {
+        const FdPattern& pat = pats[i];
+        const size_t v0 = size_t(i) * n;
+        double* r0i = r0 + size_t(i) * R;
+        for (size_t r = 0; r < R; ++r) r0i[r] = res[v0 * R + r];
+        if (status0) std::copy_n(st.data() + v0 * rays, rays, status0 + size_t(i) * rays);
+        for (int k = 0; k < P; ++k) {
+            const size_t va = v0 + size_t(2 * k + 1), vb = v0 + size_t(2 * k + 2);
+            double* col = J + (size_t(i) * P + size_t(k)) * R;
+            for (size_t r = 0; r < R; ++r)
+                col[r] = fd_derivative(pat.stencil[k], r0i[r], res[va * R + r], res[vb * R + r],
+                                       pat.step_plus[k], pat.step_minus[k]);
+            int64_t c = 0;
+            for (size_t r = 0; r < rays; ++r) {
+                const bool ok0 = st[v0 * rays + r] == Ok;
+                if ((st[va * rays + r] == Ok) != ok0 || (st[vb * rays + r] == Ok) != ok0) ++c;
+            }
+            if (changes) changes[size_t(i) * P + k] = c;
+        }
+    }
+}
+
+// ---- adjoint: include/mikroptim/trace/adjoint_backward.hpp ------------------------------------
+
+struct Seed {
+    double opl, penalty;
+};
+struct RayAdj {
+    Vec3 p, d;
+};
+struct SurfAdj {
+    double thickness = 0, curvature = 0, n_before = 0, n_after = 0;
+};
+
+// refracting_step_backward, include/mikroptim/trace/adjoint_backward.hpp:115-241
+inline void refracting_step_backward(const Vec3& p, const Vec3& d, double t, double c,
+                                     double n_before, double n_after, uint8_t branch, Seed seed,
+                                     RayAdj& adj, SurfAdj& out) {
+    // local forward step (surface_ops.hpp formulas)
+    const double z = p.z - t;
+    const double s0 = -z / d.z;
+    const double x = p.x + s0 * d.x;
+    const double y = p.y + s0 * d.y;
+    const double rr = x * x + y * y;
+    const double H = c * rr;
+    const double xd = x * d.x + y * d.y;
+    const double B = d.z - c * xd;
+    const double D = B * B - c * H;
+    const bool missed = (branch & kBranchSoftMissed) != 0;
+    const bool tir = (branch & kBranchSoftTIR) != 0;
+    double root_d = 0.0;
+    double s;
+    if (missed) {
+        s = B / c;
+    } else {
+        root_d = std::sqrt(D);
+        s = H / (B + root_d);
+    }
+    const Vec3 q{x + s * d.x, y + s * d.y, s * d.z};
+    const Vec3 n_raw{-c * q.x, -c * q.y, 1.0 - c * q.z};
+    double inv_norm = 1.0;
+    Vec3 N = n_raw;
+    if (missed) {
+        inv_norm = 1.0 / norm(n_raw);
+        N = inv_norm * n_raw;
+    }
+    const double mu = n_before / n_after;
+    const double cos_i = dot(d, N);
+    const double sin2 = 1.0 - cos_i * cos_i;
+    const double k = 1.0 - mu * mu * sin2;
+
+    // reverse
+    const Vec3 qb_out = adj.p;
+    const Vec3 d1b = adj.d;
+    RayAdj in{{0, 0, 0}, {0, 0, 0}};
+    Vec3 nb{0, 0, 0};
+    double cosb = 0.0, kb = 0.0, mub = 0.0;
+    if (tir) {
+        const Vec3 tau = d - cos_i * N;
+        const double inv_tau = 1.0 / norm(tau);
+        const Vec3 d1 = inv_tau * tau;
+        const Vec3 taub = inv_tau * (d1b - dot(d1, d1b) * d1);
+        in.d += taub;
+        cosb += -dot(taub, N);
+        nb += (-cos_i) * taub;
+        kb += -seed.penalty;
+    } else {
+        const double root_k = std::sqrt(k);
+        const double a = root_k - mu * cos_i;
+        in.d += mu * d1b;
+        nb += a * d1b;
+        const double ab = dot(d1b, N);
+        mub += dot(d1b, d);
+        kb += ab / (2.0 * root_k);
+        mub += -ab * cos_i;
+        cosb += -ab * mu;
+    }
+    mub += -kb * 2.0 * mu * sin2;
+    cosb += kb * 2.0 * mu * mu * cos_i;
+    in.d += cosb * N;
+    nb += cosb * d;
+    out.n_before += mub / n_after;
+    out.n_after += -mub * mu / n_after;
+    Vec3 nrb = nb;
+    if (missed) nrb = inv_norm * (nb - dot(N, nb) * N);
+    const Vec3 qb = qb_out + (-c) * nrb;
+    double cb = -dot(q, nrb);
+    double xb = qb.x;
+    double yb = qb.y;
+    double sb = dot(qb, d);
+    in.d += s * qb;
+    sb += seed.opl * n_before;
+    double s0b = seed.opl * n_before;
+    out.n_before += seed.opl * (s0 + s);
+    double hb = 0.0, bb = 0.0;
+    if (missed) {
+        bb += sb / c;
+        cb += -sb * s / c;
+        const double db = -seed.penalty;
+        bb += db * 2.0 * B;
+        cb += -db * H;
+        hb += -db * c;
+    } else {
+        const double inv = 1.0 / (2.0 * root_d);
+        hb += sb * inv;
+        bb += -sb * s * 2.0 * inv;
+        cb += sb * s * s * inv;
+    }
+    cb += hb * rr;
+    xb += hb * 2.0 * c * x;
+    yb += hb * 2.0 * c * y;
+    in.d.z += bb;
+    cb += -bb * xd;
+    xb += -bb * c * d.x;
+    yb += -bb * c * d.y;
+    in.d.x += -bb * c * x;
+    in.d.y += -bb * c * y;
+    // transfer_backward, adjoint_backward.hpp:93-108
+    in.p.x += xb;
+    in.p.y += yb;
+    s0b += xb * d.x + yb * d.y;
+    in.d.x += xb * s0;
+    in.d.y += yb * s0;
+    const double qq = s0b / d.z;
+    out.thickness += qq;
+    in.p.z += -qq;
+    in.d.z += -qq * s0;
+    out.curvature += cb;
+    adj = in;
+}
+
+// reference_step_backward, adjoint_backward.hpp:373-419
+inline void reference_step_backward(const Vec3& p, const Vec3& d, const Vec3& center,
+                                    double n_image, uint8_t branch, Seed seed, RayAdj& adj,
+                                    Vec3& center_adj, double& n_image_adj) {
+    const Vec3 q = p - center;
+    const double b = -dot(q, d);
+    const double g = dot(p, p - 2.0 * center);
+    const double disc = b * b - g;
+    const bool missed = (branch & kBranchSoftMissed) != 0;
+    const double sigma = center.z >= 0.0 ? 1.0 : -1.0;
+    double s, root = 0.0;
+    if (missed) {
+        s = b;
+    } else {
+        root = std::sqrt(disc);
+        s = g / (b + sigma * root);
+    }
+    const double sb = seed.opl * n_image;
+    n_image_adj += seed.opl * s;
+    double bb, gb;
+    if (missed) {
+        bb = sb;
+        gb = 0.0;
+        const double radius_sq = dot(center, center);
+        const double discb = -seed.penalty / radius_sq;
+        bb += discb * 2.0 * b;
+        gb += -discb;
+        center_adj += (seed.penalty * 2.0 * disc / (radius_sq * radius_sq)) * center;
+    } else {
+        gb = sb * sigma / (2.0 * root);
+        bb = -sb * sigma * s / root;
+    }
+    RayAdj in{{0, 0, 0}, {0, 0, 0}};
+    in.p += (-bb) * d;
+    in.d += (-bb) * q;
+    center_adj += bb * d;
+    in.p += (2.0 * gb) * (p - center);
+    center_adj += (-2.0 * gb) * p;
+    adj = in;
+}
+
+// image_step_backward, adjoint_backward.hpp:429-464
+inline void image_step_backward(const Vec3& p, const Vec3& d, const Vec3& center, double kappa,
+                                double n_image, uint8_t branch, Seed seed, RayAdj& adj,
+                                Vec3& center_adj, double& n_image_adj) {
+    const Vec3 q = p - center;
+    const double b = -dot(q, d);
+    const Vec3 c = cross(q, d);
+    const double t = -dot(c, c);
+    const bool missed = (branch & kBranchSoftMissed) != 0;
+    double step, dstep_dt, tb = 0.0;
+    if (missed) {
+        step = b - kappa * t;
+        dstep_dt = -kappa;
+        tb += seed.penalty * (-(kappa * kappa));
+    } else {
+        const double u = std::sqrt(1.0 + kappa * kappa * t);
+        step = b - kappa * t / (u + 1.0);
+        dstep_dt = -kappa / (2.0 * u);
+    }
+    const double sb = seed.opl * n_image;
+    n_image_adj += seed.opl * step;
+    const double bb = sb;
+    tb += sb * dstep_dt;
+    const Vec3 qb = (-bb) * d + (-2.0 * tb) * cross(d, c);
+    const Vec3 db = (-bb) * q + (-2.0 * tb) * cross(c, q);
+    adj = RayAdj{qb, db};
+    center_adj += (-1.0) * qb;
+}
+
+// Writes the per-ray derivative of one seed into J rows (JacobianRowWriter,
+// include/mikroptim/merit/ray_jacobian.hpp:50-78; only curvature/thickness parameters exist).
+struct RowWriter {
+    const int* curv_col; // [S] variable index or -1
+    const int* thick_col;
+    double* column0;
+    int64_t rows;
+    int64_t row;
+    double factor;
+    void write(int p, double value) const {
+        if (p >= 0) column0[int64_t(p) * rows + row] = factor * value;
+    }
+};
+
+// backward_ray, include/mikroptim/trace/adjoint_backward.hpp:496-580 (no ghosts, coaxial)
+inline void backward_ray(const Table& tb, int l, const Record& rec, bool reached_image,
+                         const RefSphere& ref, bool point_source, Seed seed, const RowWriter& w) {
+    const int S = tb.S, L = tb.L, count = rec.count;
+    RayAdj adj{{0, 0, 0}, {0, 0, 0}};
+    uint8_t next_branch = kBranchRegular;
+    Vec3 d0{0, 0, 0};
+    for (int s = count - 1; s >= 0; --s) {
+        const Vec3& p = rec.p[s];
+        const Vec3& d = rec.d[s];
+        if (s == count - 1) {
+            if (reached_image && count == S) {
+                Vec3 center_adj{0, 0, 0};
+                double n_image_adj = 0.0;
+                const double n_image = tb.n[(S - 1) * L + l];
+                if (ref.kind == MKO_REF_LAST_SURFACE)
+                    reference_step_backward(p, d, ref.center, n_image, rec.image_branch, seed, adj,
+                                            center_adj, n_image_adj);
+                else
+                    image_step_backward(p, d, ref.center, ref.kappa, n_image, rec.image_branch,
+                                        seed, adj, center_adj, n_image_adj);
+                w.write(w.thick_col[S - 1], center_adj.z);
+            }
+        } else {
+            const int j = s + 1;
+            const uint8_t kind = tb.type[j];
+            SurfAdj par;
+            const double c = kind == MKO_SURFACE_SPHERE ? tb.c[j] : 0.0;
+            refracting_step_backward(p, d, tb.t[s], c, tb.n[s * L + l], tb.n[j * L + l],
+                                     next_branch, seed, adj, par);
+            if (kind == MKO_SURFACE_SPHERE) w.write(w.curv_col[j], par.curvature);
+            w.write(w.thick_col[s], par.thickness);
+        }
+        next_branch = rec.branch[s];
+        if (s == 0) d0 = d;
+    }
+    (void)point_source;
+    (void)d0; // index derivatives (glass variables) are not part of this build
+}
+
+// adjoint_jacobian_cpu, src/optim/adjoint_jacobian.cpp:370-481 (no extra rows)
+void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, int runs,
+                      const double* x0, double* J, double* r0, uint8_t* status_out, int threads) {
+    const int P = vars.P;
+    const int S = base.S();
+    OwnedBatch ab;
+    ab.init(base.b, runs);
+    for (int i = 0; i < runs; ++i) {
+        ab.copy_variant(base.b, base.b.variants == 1 ? 0 : i, i);
+        apply(vars, S, ab.curvature.data(), ab.thickness.data(), i, x0 + size_t(i) * P);
+    }
+    const Batch B{ab.b};
+    std::vector<int> curv_col(S, -1), thick_col(S, -1);
+    for (int k = 0; k < P; ++k) {
+        if (vars.type[k] == MKO_VAR_CURVATURE) {
+            if (curv_col[vars.surface[k]] >= 0) throw std::invalid_argument("duplicate parameter");
+            curv_col[vars.surface[k]] = k;
+        } else {
+            if (thick_col[vars.surface[k]] >= 0) throw std::invalid_argument("duplicate parameter");
+            thick_col[vars.surface[k]] = k;
+        }
+    }
+    const int64_t rays = lay.rays;
+    const int64_t R = lay.trace_rows();
+    std::fill(J, J + size_t(runs) * P * R, 0.0);
+    std::vector<uint8_t> status(size_t(runs) * rays);
+    parallel_for(0, runs, threads, [&](int v) {
+        std::vector<double> opd(static_cast<size_t>(rays));
+        variant_residuals(B, v, lay, r0 + size_t(v) * R, opd.data(), status.data() + size_t(v) * rays);
+    });
+    const bool soft_mode = lay.soft();
+    const bool point_source = B.b.source == MKO_SOURCE_POINT;
+    const int F = lay.F;
+    parallel_for(0, runs * F, threads, [&](int pair) {
+        const int v = pair / F;
+        const int f = pair % F;
+        Record rec;
+        const Table tb = B.table(v);
+        double* col0 = J + size_t(v) * P * R;
+        const Quadrature& q = lay.quad[f];
+        const RefSphere ref = reference_of(B, tb, v, f);
+        const int n = lay.samples[f];
+        std::vector<double> weight(n), root_weight(n);
+        std::vector<uint8_t> piston(n);
+        for (int l = 0; l < lay.L; ++l) {
+            const int64_t offset = lay.offset(f, l);
+            const double scale = residual_scale(B, f, l);
+            const double pscale =
+                soft_mode ? penalty_scale(B, f, l, lay.settings.fail_weight) : 0.0;
+            for (int k = 0; k < n; ++k) {
+                const Sample s = sample(q, k);
+                Soft soft;
+                Ray ray = source_ray(B, v, f, l, s);
+                const Status st =
+                    trace_to_reference(tb, l, ray, ref, soft_mode ? &soft : nullptr, &rec);
+                if (st != status[size_t(v) * rays + size_t(offset + k)])
+                    throw std::logic_error("recorded trace gives another status");
+                weight[k] = s.weight;
+                root_weight[k] = residual_weight(s);
+                piston[k] = in_piston(st) ? 1 : 0;
+                const RowWriter opl_row{curv_col.data(), thick_col.data(), col0, R, offset + k, 1.0};
+                const RowWriter pen_row{curv_col.data(), thick_col.data(), col0, R,
+                                        rays + offset + k, pscale * root_weight[k]};
+                // ray_gradient_rows, include/mikroptim/merit/ray_jacobian.hpp:93-107
+                const bool reached = in_piston(st);
+                if (reached) backward_ray(tb, l, rec, true, ref, point_source, Seed{1.0, 0.0}, opl_row);
+                if (soft_mode && soft.first != Ok)
+                    backward_ray(tb, l, rec, reached, ref, point_source, Seed{0.0, 1.0}, pen_row);
+            }
+            double w_ok = 0.0;
+            for (int k = 0; k < n; ++k)
+                if (piston[k]) w_ok += weight[k];
+            for (int p = 0; p < P; ++p) {
+                double* col = col0 + int64_t(p) * R + offset;
+                if (!(w_ok > 0.0)) {
+                    for (int k = 0; k < n; ++k) col[k] = 0.0;
+                    continue;
+                }
+                double sum = 0.0;
+                for (int k = 0; k < n; ++k)
+                    if (piston[k]) sum += weight[k] * col[k];
+                const double mean = sum / w_ok;
+                for (int k = 0; k < n; ++k)
+                    col[k] = piston[k] ? scale * root_weight[k] * (col[k] - mean) : 0.0;
+            }
+        }
+    });
+    if (status_out) std::copy(status.begin(), status.end(), status_out);
+}
+
+// normal_equations_host, include/mikroptim/optim/normal_equations.hpp:19-56
+inline double gram_dot(const double* u, const double* v, int64_t n) {
+    double acc = 0.0;
+    for (int64_t r = 0; r < n; ++r) acc = std::fma(u[r], v[r], acc);
+    return acc;
+}
+void normal_equations(const double* J, const double* r, int P, int64_t R, double* a, double* g) {
+    for (int q = 0; q < P; ++q) {
+        const double* cq = J + size_t(q) * R;
+        for (int p = 0; p <= q; ++p) {
+            const double v = gram_dot(J + size_t(p) * R, cq, R);
+            a[size_t(q) * P + p] = v;
+            a[size_t(p) * P + q] = v;
+        }
+        g[q] = gram_dot(cq, r, R);
+    }
+}
+
+// ---- Levenberg-Marquardt control logic: src/optim/levenberg_marquardt.cpp:37-572 ------------
+
+struct Backend {
+    // a[j*P*P + q*P + p], g[j*P + p]
+    std::function<void(const std::vector<int>&, const std::vector<double>&, std::vector<double>&,
+                       std::vector<double>&)>
+        normal_equations;
+    std::function<void(const std::vector<int>&, const std::vector<double>&, std::vector<double>&,
+                       std::vector<int64_t>&, std::vector<int64_t>&)>
+        merit;
+};
+
+struct Bounds {
+    const double *lower, *upper, *typical;
+};
+
+struct RunState {
+    std::vector<double> x, trial, a, g, d;
+    double merit = 0.0;
+    int64_t failed = 0, hard_failed = 0;
+    double mu = 0.0, nu = 2.0;
+    int status = MKO_LM_RUNNING;
+    bool need_jacobian = true, rejected = false, solved = false;
+    double step_norm = 0.0, predicted = 0.0, step_damping = 0.0, gradient_norm = 0.0;
+    double model_decrease = kNaN;
+    int iterations = 0, accepted = 0, jacobians = 0;
+    std::vector<double> history;
+};
+
+constexpr double kModelDamping = 1e-12;
+
+inline double scale_of(double x, double typical) { return std::max(std::abs(x), typical); }
+
+bool cholesky(std::vector<double>& m, int n) {
+    const size_t N = size_t(n);
+    for (size_t j = 0; j < N; ++j) {
+        double diag = m[j * N + j];
+        for (size_t k = 0; k < j; ++k) diag -= m[k * N + j] * m[k * N + j];
+        if (!(diag > 0.0) || !std::isfinite(diag)) return false;
+        const double ljj = std::sqrt(diag);
+        m[j * N + j] = ljj;
+        for (size_t i = j + 1; i < N; ++i) {
+            double s = m[j * N + i];
+            for (size_t k = 0; k < j; ++k) s -= m[k * N + i] * m[k * N + j];
+            m[j * N + i] = s / ljj;
+        }
+    }
+    return true;
+}
+
+void cholesky_solve(const std::vector<double>& l, int n, std::vector<double>& b) {
+    const size_t N = size_t(n);
+    for (size_t i = 0; i < N; ++i) {
+        double s = b[i];
+        for (size_t k = 0; k < i; ++k) s -= l[k * N + i] * b[k];
+        b[i] = s / l[i * N + i];
+    }
+    for (size_t i = N; i-- > 0;) {
+        double s = b[i];
+        for (size_t k = i + 1; k < N; ++k) s -= l[i * N + k] * b[k];
+        b[i] = s / l[i * N + i];
+    }
+}
+
+inline bool bound_blocks(const Bounds& b, size_t k, double x, double dir) {
+    if (b.lower[k] == b.upper[k]) return true;
+    return (x <= b.lower[k] && dir < 0.0) || (x >= b.upper[k] && dir > 0.0);
+}
+
+void prepare_step(RunState& s, const Bounds& b, int P, const mko_lm_options& o) {
+    const size_t n = size_t(P);
+    double smax = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double sk = scale_of(s.x[k], b.typical[k]);
+        smax = std::max(smax, s.a[k * n + k] * sk * sk);
+    }
+    double gmax = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double sk = scale_of(s.x[k], b.typical[k]);
+        const double akk = s.a[k * n + k];
+        const double floor = smax > 0.0 ? o.diag_floor * smax / (sk * sk) : 1.0 / (sk * sk);
+        s.d[k] = std::max(akk, floor);
+        if (!bound_blocks(b, k, s.x[k], -s.g[k])) {
+            const double denom = std::sqrt(akk * s.merit);
+            if (denom > 0.0) gmax = std::max(gmax, std::abs(s.g[k]) / denom);
+        }
+    }
+    s.gradient_norm = gmax;
+}
+
+void solve_step(RunState& s, const Bounds& b, int P, double mu) {
+    const size_t n = size_t(P);
+    std::vector<char> fixed(n);
+    for (size_t k = 0; k < n; ++k) fixed[k] = bound_blocks(b, k, s.x[k], -s.g[k]) ? 1 : 0;
+    std::vector<double> delta(n, 0.0), m, rhs;
+    std::vector<size_t> idx;
+    s.solved = true;
+    s.step_damping = mu;
+    for (size_t round = 0; round <= n; ++round) {
+        idx.clear();
+        for (size_t k = 0; k < n; ++k)
+            if (!fixed[k]) idx.push_back(k);
+        const size_t f = idx.size();
+        std::fill(delta.begin(), delta.end(), 0.0);
+        if (f == 0) break;
+        m.assign(f * f, 0.0);
+        rhs.assign(f, 0.0);
+        for (size_t j = 0; j < f; ++j) {
+            for (size_t i = j; i < f; ++i) m[j * f + i] = s.a[idx[j] * n + idx[i]];
+            m[j * f + j] += mu * s.d[idx[j]];
+            rhs[j] = -s.g[idx[j]];
+        }
+        if (!cholesky(m, int(f))) {
+            s.solved = false;
+            return;
+        }
+        cholesky_solve(m, int(f), rhs);
+        for (size_t j = 0; j < f; ++j) delta[idx[j]] = rhs[j];
+        bool changed = false;
+        for (const size_t k : idx) {
+            if (bound_blocks(b, k, s.x[k], delta[k])) {
+                fixed[k] = 1;
+                changed = true;
+            }
+        }
+        if (!changed) break;
+    }
+    double nrm = 0.0;
+    std::vector<double> step(n);
+    for (size_t k = 0; k < n; ++k) {
+        const double t = std::min(std::max(s.x[k] + delta[k], b.lower[k]), b.upper[k]);
+        s.trial[k] = t;
+        step[k] = t - s.x[k];
+        nrm = std::max(nrm, std::abs(step[k]) / scale_of(s.x[k], b.typical[k]));
+    }
+    s.step_norm = nrm;
+    double lin = 0.0, quad = 0.0;
+    for (size_t q = 0; q < n; ++q) {
+        lin += step[q] * s.g[q];
+        double aq = 0.0;
+        for (size_t p = 0; p < n; ++p) aq += s.a[q * n + p] * step[p];
+        quad += step[q] * aq;
+    }
+    s.predicted = -(2.0 * lin + quad);
+}
+
+inline bool all_finite(const double* v, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+// levenberg_marquardt, src/optim/levenberg_marquardt.cpp:332-572
+std::vector<RunState> levenberg_marquardt(Backend& backend, int P, const Bounds& bounds,
+                                          const double* x0, int runs, const mko_lm_options& o,
+                                          int* cycles_out) {
+    const size_t Pn = size_t(P);
+    std::vector<RunState> state(static_cast<size_t>(runs));
+    std::vector<int> list;
+    std::vector<double> xs, buf_a, buf_g, buf_f;
+    std::vector<int64_t> buf_failed, buf_hard;
+    auto gather = [&](bool trial) {
+        xs.resize(list.size() * Pn);
+        for (size_t j = 0; j < list.size(); ++j) {
+            const RunState& s = state[size_t(list[j])];
+            const std::vector<double>& src = trial ? s.trial : s.x;
+            std::copy(src.begin(), src.end(), xs.begin() + std::ptrdiff_t(j * Pn));
+        }
+    };
+    auto evaluate = [&](bool trial) {
+        gather(trial);
+        buf_f.assign(list.size(), 0.0);
+        buf_failed.assign(list.size(), 0);
+        buf_hard.assign(list.size(), 0);
+        backend.merit(list, xs, buf_f, buf_failed, buf_hard);
+    };
+    for (int i = 0; i < runs; ++i) {
+        RunState& s = state[size_t(i)];
+        s.x.assign(x0 + size_t(i) * Pn, x0 + size_t(i + 1) * Pn);
+        s.trial = s.x;
+        s.a.assign(Pn * Pn, 0.0);
+        s.g.assign(Pn, 0.0);
+        s.d.assign(Pn, 0.0);
+        s.mu = o.initial_damping;
+        list.push_back(i);
+    }
+    evaluate(false);
+    for (size_t j = 0; j < list.size(); ++j) {
+        RunState& s = state[size_t(list[j])];
+        s.merit = buf_f[j];
+        s.failed = buf_failed[j];
+        s.hard_failed = buf_hard[j];
+        s.history.push_back(s.merit);
+        if (!std::isfinite(s.merit))
+            s.status = MKO_LM_NUMERICAL_ERROR;
+        else if (s.merit == 0.0)
+            s.status = MKO_LM_CONVERGED_MERIT;
+    }
+    int cycles = 0;
+    for (;;) {
+        list.clear();
+        for (int i = 0; i < runs; ++i) {
+            const RunState& s = state[size_t(i)];
+            if (s.status == MKO_LM_RUNNING && s.need_jacobian && s.iterations < o.max_iterations)
+                list.push_back(i);
+        }
+        if (!list.empty()) {
+            gather(false);
+            buf_a.assign(list.size() * Pn * Pn, 0.0);
+            buf_g.assign(list.size() * Pn, 0.0);
+            backend.normal_equations(list, xs, buf_a, buf_g);
+            for (size_t j = 0; j < list.size(); ++j) {
+                RunState& s = state[size_t(list[j])];
+                std::copy_n(buf_a.data() + j * Pn * Pn, Pn * Pn, s.a.begin());
+                std::copy_n(buf_g.data() + j * Pn, Pn, s.g.begin());
+                s.need_jacobian = false;
+                ++s.jacobians;
+                if (!all_finite(buf_a.data() + j * Pn * Pn, Pn * Pn) ||
+                    !all_finite(buf_g.data() + j * Pn, Pn)) {
+                    s.status = MKO_LM_NUMERICAL_ERROR;
+                    continue;
+                }
+                prepare_step(s, bounds, P, o);
+            }
+            parallel_for(0, int(list.size()), o.threads, [&](int j) {
+                RunState& s = state[size_t(list[size_t(j)])];
+                if (s.status == MKO_LM_RUNNING) solve_step(s, bounds, P, kModelDamping);
+            });
+            for (const int i : list) {
+                RunState& s = state[size_t(i)];
+                if (s.status != MKO_LM_RUNNING) continue;
+                s.model_decrease = s.solved ? s.predicted / s.merit : kNaN;
+                if (s.gradient_norm <= o.gtol)
+                    s.status = MKO_LM_CONVERGED_GRADIENT;
+                else if (s.solved && s.predicted >= 0.0 && s.predicted <= o.ftol * s.merit)
+                    s.status = MKO_LM_CONVERGED_MERIT;
+                else if (s.solved && s.step_norm <= o.xtol)
+                    s.status = MKO_LM_CONVERGED_STEP;
+            }
+        }
+        list.clear();
+        for (int i = 0; i < runs; ++i) {
+            RunState& s = state[size_t(i)];
+            if (s.status != MKO_LM_RUNNING) continue;
+            if (s.iterations >= o.max_iterations) {
+                s.status = MKO_LM_MAX_ITERATIONS;
+                continue;
+            }
+            list.push_back(i);
+        }
+        if (list.empty()) break;
+        ++cycles;
+        parallel_for(0, int(list.size()), o.threads, [&](int j) {
+            RunState& s = state[size_t(list[size_t(j)])];
+            solve_step(s, bounds, P, s.mu);
+        });
+        std::vector<int> cycle_runs = list, eval;
+        for (const int i : cycle_runs) {
+            RunState& s = state[size_t(i)];
+            if (s.solved && s.rejected && s.step_norm <= o.xtol)
+                s.status = MKO_LM_CONVERGED_STEP;
+            else if (s.solved)
+                eval.push_back(i);
+        }
+        list = eval;
+        if (!list.empty()) evaluate(true);
+        size_t e = 0;
+        for (const int i : cycle_runs) {
+            RunState& s = state[size_t(i)];
+            if (s.status == MKO_LM_CONVERGED_STEP) continue;
+            ++s.iterations;
+            bool accept = false;
+            if (s.solved) {
+                const double f_new = buf_f[e];
+                const int64_t failed_new = buf_failed[e];
+                const int64_t hard_new = buf_hard[e];
+                ++e;
+                const double rho = (s.merit - f_new) / s.predicted;
+                accept = std::isfinite(f_new) && s.predicted > 0.0 && f_new < s.merit &&
+                         !(o.reject_new_failures && hard_new > s.hard_failed);
+                if (accept) {
+                    s.x = s.trial;
+                    s.merit = f_new;
+                    s.failed = failed_new;
+                    s.hard_failed = hard_new;
+                    const double t = 2.0 * rho - 1.0;
+                    s.mu *= std::max(1.0 / 3.0, 1.0 - t * t * t);
+                    s.nu = 2.0;
+                    s.need_jacobian = true;
+                    s.rejected = false;
+                    ++s.accepted;
+                    if (f_new == 0.0) s.status = MKO_LM_CONVERGED_MERIT;
+                }
+            }
+            if (!accept) {
+                s.mu *= s.nu;
+                s.nu *= 2.0;
+                s.rejected = true;
+                if (s.mu > o.max_damping) s.status = MKO_LM_DAMPING_LIMIT;
+            }
+            s.history.push_back(s.merit);
+            if (o.stall_window > 0 && s.status == MKO_LM_RUNNING) {
+                const size_t w = size_t(o.stall_window);
+                if (s.history.size() > w) {
+                    const double old = s.history[s.history.size() - 1 - w];
+                    if (old - s.merit <= o.stall_tolerance * old) s.status = MKO_LM_CONVERGED_STALL;
+                }
+            }
+        }
+    }
+    if (cycles_out) *cycles_out = cycles;
+    return state;
+}
+
+// CpuLmBackend, src/optim/levenberg_marquardt.cpp:678-866 (no extra residuals)
+Backend cpu_backend(const Batch& base, const Layout& lay, const Vars& vars, int method,
+                    int threads) {
+    Backend be;
+    const int P = vars.P;
+    be.normal_equations = [=, &base, &lay, &vars](const std::vector<int>& runs,
+                                                 const std::vector<double>& x,
+                                                 std::vector<double>& a, std::vector<double>& g) {
+        const int count = int(runs.size());
+        OwnedBatch tmpl;
+        const mko_batch* tb = &base.b;
+        if (base.b.variants != 1) {
+            tmpl.init(base.b, count);
+            for (int j = 0; j < count; ++j) tmpl.copy_variant(base.b, runs[size_t(j)], j);
+            tb = &tmpl.b;
+        }
+        const Batch TB{*tb};
+        const int64_t R = lay.trace_rows();
+        std::vector<double> J(size_t(count) * P * R), r0(size_t(count) * R);
+        if (method == MKO_JACOBIAN_ADJOINT)
+            adjoint_jacobian(TB, lay, vars, count, x.data(), J.data(), r0.data(), nullptr, threads);
+        else
+            fd_jacobian(TB, lay, vars, count, x.data(), J.data(), r0.data(), nullptr, nullptr,
+                        threads);
+        parallel_for(0, count, threads, [&](int j) {
+            normal_equations(J.data() + size_t(j) * P * R, r0.data() + size_t(j) * R, P, R,
+                             a.data() + size_t(j) * P * P, g.data() + size_t(j) * P);
+        });
+    };
+    be.merit = [=, &base, &lay, &vars](const std::vector<int>& runs, const std::vector<double>& x,
+                                      std::vector<double>& merit, std::vector<int64_t>& failed,
+                                      std::vector<int64_t>& hard) {
+        const int count = int(runs.size());
+        OwnedBatch pb;
+        pb.init(base.b, count);
+        for (int j = 0; j < count; ++j) {
+            pb.copy_variant(base.b, base.b.variants == 1 ? 0 : runs[size_t(j)], j);
+            apply(vars, base.S(), pb.curvature.data(), pb.thickness.data(), j,
+                  x.data() + size_t(j) * P);
+        }
+        const Batch PB{pb.b};
+        parallel_for(0, count, threads, [&](int v) {
+            variant_merit(PB, v, lay, merit[size_t(v)], failed[size_t(v)], hard[size_t(v)]);
+        });
+    };
+    return be;
+}
+
+thread_local std::string g_error;
+
+} // namespace orc
+
+using namespace orc;
+
+#define ORC_TRY(...)                                                                              \
+    try {                                                                                          \
+        __VA_ARGS__;                                                                               \
+        return 0;                                                                                  \
+    } catch (const std::exception& e) {                                                            \
+        orc::g_error = e.what();                                                                   \
+        return -1;                                                                                 \
+    }
+
+extern "C" {
+
+const char* orc_last_error(void) { return orc::g_error.c_str(); }
+
+int orc_gauss_legendre(int n, double* u, double* w) {
+    ORC_TRY({
+        const Quadrature q = make_quadrature(n, 1, MKO_PUPIL_HALF);
+        for (int i = 0; i < n; ++i) {
+            u[i] = q.u[i];
+            w[i] = q.w[i];
+        }
+    })
+}
+
+// pupil samples of one field mode: rho, phi, weight per index k = i*M + j
+int orc_pupil_samples(int rings, int spokes, int mode, double* rho, double* phi, double* weight) {
+    ORC_TRY({
+        const Quadrature q = make_quadrature(rings, spokes, mode);
+        for (int k = 0; k < q.rings * q.spokes; ++k) {
+            const Sample s = sample(q, k);
+            rho[k] = s.rho;
+            phi[k] = s.phi;
+            weight[k] = s.weight;
+        }
+    })
+}
+
+int64_t orc_rays_per_variant(const mko_batch* b, const mko_residual_settings* s) {
+    try {
+        return make_layout(Batch{*b}, *s).rays;
+    } catch (const std::exception& e) {
+        orc::g_error = e.what();
+        return -1;
+    }
+}
+
+// One ray through one variant's table, start ray given; returns the status, fills final state.
+int orc_trace_ray(const mko_batch* b, int v, int l, const double* p0, const double* d0,
+                  double opl0, int to_reference, int f, int soft_mode, double* p_out,
+                  double* d_out, double* opl_out, double* penalty_out) {
+    const Batch B{*b};
+    const Table tb = B.table(v);
+    Ray ray{{p0[0], p0[1], p0[2]}, {d0[0], d0[1], d0[2]}, opl0};
+    Soft soft;
+    Status st;
+    if (to_reference)
+        st = trace_to_reference(tb, l, ray, reference_of(B, tb, v, f), soft_mode ? &soft : nullptr,
+                                nullptr);
+    else
+        st = trace_ray(tb, l, ray, soft_mode ? &soft : nullptr, nullptr, nullptr);
+    p_out[0] = ray.p.x;
+    p_out[1] = ray.p.y;
+    p_out[2] = ray.p.z;
+    d_out[0] = ray.d.x;
+    d_out[1] = ray.d.y;
+    d_out[2] = ray.d.z;
+    *opl_out = ray.opl;
+    if (penalty_out) *penalty_out = soft.penalty;
+    return int(st);
+}
+
+int orc_residuals(const mko_batch* b, const mko_residual_settings* s, double* residuals,
+                  double* opd, uint8_t* status, int threads) {
+    ORC_TRY({
+        const Batch B{*b};
+        const Layout lay = make_layout(B, *s);
+        residuals_all(B, lay, residuals, opd, status, threads);
+    })
+}
+
+int orc_merit(const mko_batch* b, const mko_residual_settings* s, double* merit, int64_t* failed,
+              int64_t* hard_failed, int threads) {
+    ORC_TRY({
+        const Batch B{*b};
+        const Layout lay = make_layout(B, *s);
+        parallel_for(0, b->variants, threads, [&](int v) {
+            variant_merit(B, v, lay, merit[v], failed[v], hard_failed[v]);
+        });
+    })
+}
+
+int orc_fd_pattern(const mko_variables* vars, int surfaces, const double* x0, double* nominal,
+                   double* plus, double* minus, double* step_plus, double* step_minus,
+                   uint8_t* stencil) {
+    ORC_TRY({
+        const Vars V = make_vars(vars, surfaces);
+        const FdPattern p = fd_pattern(V, x0);
+        for (int k = 0; k < V.P; ++k) {
+            nominal[k] = p.nominal[k];
+            plus[k] = p.plus[k];
+            minus[k] = p.minus[k];
+            step_plus[k] = p.step_plus[k];
+            step_minus[k] = p.step_minus[k];
+            stencil[k] = p.stencil[k];
+        }
+    })
+}
+
+int orc_fd_jacobian(const mko_batch* base, const mko_residual_settings* s,
+                    const mko_variables* vars, int runs, const double* x0, double* J, double* r0,
+                    int64_t* status_changes, uint8_t* status0, int threads) {
+    ORC_TRY({
+        const Batch B{*base};
+        const Layout lay = make_layout(B, *s);
+        fd_jacobian(B, lay, make_vars(vars, base->surfaces), runs, x0, J, r0, status_changes,
+                    status0, threads);
+    })
+}
+
+int orc_adjoint_jacobian(const mko_batch* base, const mko_residual_settings* s,
+                         const mko_variables* vars, int runs, const double* x0, double* J,
+                         double* r0, uint8_t* status, int threads) {
+    ORC_TRY({
+        const Batch B{*base};
+        const Layout lay = make_layout(B, *s);
+        adjoint_jacobian(B, lay, make_vars(vars, base->surfaces), runs, x0, J, r0, status, threads);
+    })
+}
+
+int orc_normal_equations(const double* J, const double* r, int runs, int P, int64_t R, double* a,
+                         double* g) {
+    ORC_TRY({
+        for (int j = 0; j < runs; ++j)
+            normal_equations(J + size_t(j) * P * R, r + size_t(j) * R, P, R,
+                             a + size_t(j) * P * P, g + size_t(j) * P);
+    })
+}
+
+// Writes x0 of each run into a copy of the base variant (VariableSet::apply).
+int orc_apply(mko_batch* b, const mko_variables* vars, int v, const double* x) {
+    ORC_TRY({
+        apply(make_vars(vars, b->surfaces), b->surfaces, const_cast<double*>(b->curvature),
+              const_cast<double*>(b->thickness), v, x);
+    })
+}
+
+uint64_t orc_splitmix64(uint64_t x) { return splitmix64(x); }
+uint64_t orc_variant_seed(uint64_t seed, uint64_t j) { return variant_seed(seed, j); }
+double orc_unit_uniform(uint64_t bits) { return unit_uniform(bits); }
+uint64_t orc_mt19937_64(uint64_t seed, uint64_t discard) {
+    std::mt19937_64 e(seed);
+    e.discard(discard);
+    return e();
+}
+
+// relative/absolute: [3] per MKO_VAR_* type; distribution 0 uniform, 1 Irwin-Hall
+int orc_perturbed_vector(const mko_variables* vars, int surfaces, const double* x0,
+                         const double* relative, const double* absolute, int distribution,
+                         uint64_t seed, int first_is_base, uint64_t j, double* y) {
+    ORC_TRY({
+        perturbed_vector(make_vars(vars, surfaces), x0, relative, absolute, distribution, seed,
+                         first_is_base, j, y);
+    })
+}
+
+// optimize_lm_cpu (src/optim/levenberg_marquardt.cpp:891-983, 985-991): LM with the CPU backend,
+// outer penalty rounds in soft mode. x_out [runs][P], run results [runs], history optional
+// ([runs][max_iterations+1] merits, NaN-padded; first round only).
+int orc_optimize_lm(const mko_batch* base, const mko_residual_settings* s,
+                    const mko_variables* vars, int runs, const double* x0,
+                    const mko_lm_options* options, double* x_out, mko_lm_run* run_out,
+                    int* cycles_out) {
+    ORC_TRY({
+        const Batch B{*base};
+        const Vars V = make_vars(vars, base->surfaces);
+        const int P = V.P;
+        const Bounds bounds{V.lower.data(), V.upper.data(), V.typical.data()};
+        mko_residual_settings rs = *s;
+        int total_cycles = 0;
+        std::vector<double> weight_of(size_t(runs), rs.fail_weight);
+        std::vector<int> rounds(size_t(runs), 1);
+        std::vector<RunState> result;
+        {
+            const Layout lay = make_layout(B, rs);
+            Backend be = cpu_backend(B, lay, V, options->jacobian_method, options->threads);
+            int cyc = 0;
+            result = levenberg_marquardt(be, P, bounds, x0, runs, *options, &cyc);
+            total_cycles += cyc;
+        }
+        double weight = rs.fail_weight;
+        if (rs.failure == MKO_FAILURE_SOFT && options->fail_weight_factor > 1.0 && weight > 0.0) {
+            for (;;) {
+                weight *= options->fail_weight_factor;
+                if (!(weight <= options->max_fail_weight)) break;
+                std::vector<int> again;
+                std::vector<double> xs;
+                for (int i = 0; i < runs; ++i) {
+                    if (result[size_t(i)].failed > 0 &&
+                        result[size_t(i)].status != MKO_LM_NUMERICAL_ERROR) {
+                        again.push_back(i);
+                        xs.insert(xs.end(), result[size_t(i)].x.begin(), result[size_t(i)].x.end());
+                    }
+                }
+                if (again.empty()) break;
+                rs.fail_weight = weight;
+                OwnedBatch tmpl;
+                const mko_batch* tb = &B.b;
+                if (B.b.variants != 1) {
+                    tmpl.init(B.b, int(again.size()));
+                    for (size_t j = 0; j < again.size(); ++j) tmpl.copy_variant(B.b, again[j], int(j));
+                    tb = &tmpl.b;
+                }
+                const Batch TB{*tb};
+                const Layout lay = make_layout(TB, rs);
+                Backend be = cpu_backend(TB, lay, V, options->jacobian_method, options->threads);
+                int cyc = 0;
+                std::vector<RunState> part = levenberg_marquardt(be, P, bounds, xs.data(),
+                                                                 int(again.size()), *options, &cyc);
+                total_cycles += cyc;
+                for (size_t j = 0; j < again.size(); ++j) {
+                    RunState& r = result[size_t(again[j])];
+                    RunState& q = part[j];
+                    r.x = q.x;
+                    r.merit = q.merit;
+                    r.failed = q.failed;
+                    r.hard_failed = q.hard_failed;
+                    r.status = q.status;
+                    r.iterations += q.iterations;
+                    r.accepted += q.accepted;
+                    r.jacobians += q.jacobians;
+                    r.mu = q.mu;
+                    r.gradient_norm = q.gradient_norm;
+                    r.model_decrease = q.model_decrease;
+                    weight_of[size_t(again[j])] = weight;
+                    ++rounds[size_t(again[j])];
+                }
+            }
+        }
+        for (int i = 0; i < runs; ++i) {
+            const RunState& r = result[size_t(i)];
+            std::copy(r.x.begin(), r.x.end(), x_out + size_t(i) * P);
+            mko_lm_run& o = run_out[i];
+            o.merit = r.merit;
+            o.failed_rays = r.failed;
+            o.hard_failed_rays = r.hard_failed;
+            o.status = r.status;
+            o.iterations = r.iterations;
+            o.accepted = r.accepted;
+            o.jacobians = r.jacobians;
+            o.damping = r.mu;
+            o.gradient_norm = r.gradient_norm;
+            o.model_decrease = r.model_decrease;
+            o.fail_weight = weight_of[size_t(i)];
+            o.penalty_rounds = rounds[size_t(i)];
+        }
+        if (cycles_out) *cycles_out = total_cycles;
+    })
+}
+
+// LM over an analytic residual function r(x) = M x - b (linear least squares; DenseBackend of
+// tests/lm_test_systems.hpp:38): A = MtM and g = Mtr exactly, merit = |r|^2.
+int orc_lm_linear(const double* M, const double* bvec, int rows, int P, const double* lower,
+                  const double* upper, const double* typical, int runs, const double* x0,
+                  const mko_lm_options* options, double* x_out, mko_lm_run* run_out) {
+    ORC_TRY({
+        Backend be;
+        auto residual = [&](const double* x, std::vector<double>& r) {
+            r.assign(size_t(rows), 0.0);
+            for (int i = 0; i < rows; ++i) {
+                double acc = -bvec[i];
+                for (int k = 0; k < P; ++k) acc += M[size_t(i) * P + k] * x[k];
+                r[size_t(i)] = acc;
+            }
+        };
+        be.normal_equations = [&](const std::vector<int>& list, const std::vector<double>& x,
+                                  std::vector<double>& a, std::vector<double>& g) {
+            std::vector<double> J(size_t(P) * rows), r;
+            for (int p = 0; p < P; ++p)
+                for (int i = 0; i < rows; ++i) J[size_t(p) * rows + i] = M[size_t(i) * P + p];
+            for (size_t j = 0; j < list.size(); ++j) {
+                residual(x.data() + j * P, r);
+                normal_equations(J.data(), r.data(), P, rows, a.data() + j * P * P,
+                                 g.data() + j * P);
+            }
+        };
+        be.merit = [&](const std::vector<int>& list, const std::vector<double>& x,
+                       std::vector<double>& f, std::vector<int64_t>& fl, std::vector<int64_t>& hf) {
+            std::vector<double> r;
+            for (size_t j = 0; j < list.size(); ++j) {
+                residual(x.data() + j * P, r);
+                double s = 0.0;
+                for (double v : r) s += v * v;
+                f[j] = s;
+                fl[j] = 0;
+                hf[j] = 0;
+            }
+        };
+        const Bounds bounds{lower, upper, typical};
+        std::vector<RunState> res = levenberg_marquardt(be, P, bounds, x0, runs, *options, nullptr);
+        for (int i = 0; i < runs; ++i) {
+            std::copy(res[size_t(i)].x.begin(), res[size_t(i)].x.end(), x_out + size_t(i) * P);
+            run_out[i].merit = res[size_t(i)].merit;
+            run_out[i].status = res[size_t(i)].status;
+            run_out[i].iterations = res[size_t(i)].iterations;
+            run_out[i].accepted = res[size_t(i)].accepted;
+            run_out[i].jacobians = res[size_t(i)].jacobians;
+        }
+    })
+}
+
+} // extern "C"

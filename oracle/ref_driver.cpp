@@ -1,0 +1,319 @@
+// C-ABI driver over the reference's own CPU hot-path code (TEST INFRASTRUCTURE ONLY). Built by
+// oracle/Makefile together with the reference sources where they lie (never copied) into
+// oracle/_ref/libmkoref.so. The entry points take the same plain-C batch/settings/variables
+// structs as the oracle (mikroptim_types.h) and call residuals_cpu, merit_cpu, fd_jacobian_cpu,
+// adjoint_jacobian_cpu, normal_equations_host, perturbed_vector, fd_pattern and optimize_lm_cpu
+// of the reference, so tests can compare the oracle's restatement with the original bit for bit.
+
+#include "mikroptim_types.h"
+
+#include "mikroptim/batch/batch_merit.hpp"
+#include "mikroptim/batch/batch_residuals.hpp"
+#include "mikroptim/batch/system_batch.hpp"
+#include "mikroptim/model/system.hpp"
+#include "mikroptim/optim/adjoint_jacobian.hpp"
+#include "mikroptim/optim/fd_jacobian.hpp"
+#include "mikroptim/optim/levenberg_marquardt.hpp"
+#include "mikroptim/optim/normal_equations.hpp"
+#include "mikroptim/optim/variable_set.hpp"
+#include "mikroptim/optim/variant_generator.hpp"
+#include "mikroptim/pupil/quadrature.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_error;
+
+mko::BatchTopology topology_of(const mko_batch& b) {
+    mko::BatchTopology t;
+    t.surfaces = b.surfaces;
+    t.wavelengths = b.wavelengths;
+    t.fields = b.fields;
+    t.source = b.source == MKO_SOURCE_POINT ? mko::SourceKind::Point : mko::SourceKind::Collimated;
+    for (int s = 0; s < b.surfaces; ++s) t.type.push_back(static_cast<mko::SurfaceKind>(b.type[s]));
+    for (int f = 0; f < b.fields; ++f) t.pupil_mode.push_back(static_cast<mko::PupilMode>(b.pupil_mode[f]));
+    t.reference_sphere = static_cast<mko::ReferenceSphereKind>(b.reference_sphere);
+    return t;
+}
+
+mko::SystemBatchHost batch_of(const mko_batch& b) {
+    mko::SystemBatchHost h(topology_of(b), b.variants);
+    const size_t V = size_t(b.variants), S = size_t(b.surfaces), L = size_t(b.wavelengths),
+                 F = size_t(b.fields);
+    h.reference_wavelength_nm = b.reference_wavelength_nm;
+    h.source_point = {b.source_point[0], b.source_point[1], b.source_point[2]};
+    h.source_na = b.source_na;
+    for (size_t l = 0; l < L; ++l) {
+        h.wavelength_weight(int(l)) = b.wavelength_weight[l];
+        h.wavelength_nm(int(l)) = 500.0; // not used by the trace
+    }
+    for (size_t f = 0; f < F; ++f) h.field_weight(int(f)) = b.field_weight[f];
+    std::copy_n(b.curvature, V * S, h.curvature_data().begin());
+    std::copy_n(b.thickness, V * S, h.thickness_data().begin());
+    std::copy_n(b.index, V * S * L, h.index_data().begin());
+    std::copy_n(b.pupil_radius, V, h.pupil_radius_data().begin());
+    std::copy_n(b.field_angle, V * F, h.field_angle_data().begin());
+    std::copy_n(b.field_target, V * F, h.field_target_data().begin());
+    if (b.reference_sphere != MKO_REF_LAST_SURFACE)
+        std::copy_n(b.reference_curvature, V * F, h.reference_curvature_data().begin());
+    return h;
+}
+
+mko::ResidualSettings settings_of(const mko_residual_settings& s) {
+    mko::ResidualSettings r;
+    r.rings = s.rings;
+    r.spokes = s.spokes;
+    r.failure = s.failure == MKO_FAILURE_HARD ? mko::FailureMode::Hard : mko::FailureMode::Soft;
+    r.fail_weight = s.fail_weight;
+    return r;
+}
+
+// A system description whose surfaces match the batch (types, values of variant 0) and whose
+// `variables` section lists one item per variable in order.
+mko::SystemDescription description_of(const mko_batch& b, const mko_variables& v) {
+    mko::SystemDescription d;
+    d.name = "driver";
+    for (int s = 0; s < b.surfaces; ++s) {
+        mko::Surface sf;
+        sf.type = b.type[s] == MKO_SURFACE_STOP    ? mko::SurfaceType::Stop
+                  : b.type[s] == MKO_SURFACE_PLANE ? mko::SurfaceType::Plane
+                                                   : mko::SurfaceType::Sphere;
+        sf.curvature = b.curvature[s];
+        sf.thickness = b.thickness[s];
+        d.surfaces.push_back(sf);
+    }
+    mko::Variables vars;
+    for (int k = 0; k < v.count; ++k) {
+        mko::VariableSpec spec;
+        spec.type = v.type[k] == MKO_VAR_CURVATURE   ? mko::VariableType::Curvature
+                    : v.type[k] == MKO_VAR_THICKNESS ? mko::VariableType::Thickness
+                                                     : mko::VariableType::WorkingDistance;
+        if (v.type[k] == MKO_VAR_WORKING_DISTANCE) {
+            spec.selection = mko::SurfaceSelection::None;
+        } else {
+            spec.selection = mko::SurfaceSelection::One;
+            spec.surfaces = {v.batch_surface[k]};
+        }
+        if (std::isfinite(v.lower[k])) spec.min = v.lower[k];
+        if (std::isfinite(v.upper[k])) spec.max = v.upper[k];
+        spec.typical = v.typical[k];
+        vars.items.push_back(spec);
+    }
+    d.variables = vars;
+    return d;
+}
+
+#define REF_TRY(...)                                                                               \
+    try {                                                                                          \
+        __VA_ARGS__;                                                                               \
+        return 0;                                                                                  \
+    } catch (const std::exception& e) {                                                            \
+        g_error = e.what();                                                                        \
+        return -1;                                                                                 \
+    }
+
+mko::LmOptions lm_options_of(const mko_lm_options& o, const mko_residual_settings& s) {
+    mko::LmOptions r;
+    r.max_iterations = o.max_iterations;
+    r.ftol = o.ftol;
+    r.xtol = o.xtol;
+    r.gtol = o.gtol;
+    r.stall_window = o.stall_window;
+    r.stall_tolerance = o.stall_tolerance;
+    r.initial_damping = o.initial_damping;
+    r.max_damping = o.max_damping;
+    r.diag_floor = o.diag_floor;
+    r.reject_new_failures = o.reject_new_failures != 0;
+    r.threads = o.threads;
+    r.jacobian.quadrature = settings_of(s);
+    r.jacobian_method = o.jacobian_method == MKO_JACOBIAN_ADJOINT
+                            ? mko::JacobianMethod::Adjoint
+                            : mko::JacobianMethod::FiniteDifferences;
+    r.fail_weight_factor = o.fail_weight_factor;
+    r.max_fail_weight = o.max_fail_weight;
+    return r;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* ref_last_error(void) { return g_error.c_str(); }
+
+int ref_gauss_legendre(int n, double* u, double* w) {
+    REF_TRY({
+        const auto q = mko::make_pupil_quadrature<double>(n, 1, mko::PupilMode::Half);
+        for (int i = 0; i < n; ++i) {
+            u[i] = q.u[i];
+            w[i] = q.w[i];
+        }
+    })
+}
+
+int ref_pupil_samples(int rings, int spokes, int mode, double* rho, double* phi, double* weight) {
+    REF_TRY({
+        const auto q =
+            mko::make_pupil_quadrature<double>(rings, spokes, static_cast<mko::PupilMode>(mode));
+        for (int k = 0; k < mko::sample_count(q); ++k) {
+            const auto s = mko::sample(q, k);
+            rho[k] = s.rho;
+            phi[k] = s.phi;
+            weight[k] = s.weight;
+        }
+    })
+}
+
+int ref_residuals(const mko_batch* b, const mko_residual_settings* s, double* residuals,
+                  double* opd, uint8_t* status) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::ResidualLayout layout(h.topology(), settings_of(*s));
+        const mko::ResidualResult r = mko::residuals_cpu(h, layout);
+        std::copy(r.residuals.begin(), r.residuals.end(), residuals);
+        std::copy(r.opd.begin(), r.opd.end(), opd);
+        std::copy(r.status.begin(), r.status.end(), status);
+    })
+}
+
+int ref_merit(const mko_batch* b, const mko_residual_settings* s, double* merit, int64_t* failed,
+              int64_t* hard_failed) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::ResidualLayout layout(h.topology(), settings_of(*s));
+        const mko::MeritResult r = mko::merit_cpu(h, layout, false);
+        std::copy(r.merit.begin(), r.merit.end(), merit);
+        std::copy(r.failed.begin(), r.failed.end(), failed);
+        std::copy(r.hard_failed.begin(), r.hard_failed.end(), hard_failed);
+    })
+}
+
+int ref_fd_pattern(const mko_batch* b, const mko_variables* vars, const double* x0,
+                   double* nominal, double* plus, double* minus, double* step_plus,
+                   double* step_minus, uint8_t* stencil) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::VariableSet set(description_of(*b, *vars), h.topology());
+        const mko::FdPattern p = mko::fd_pattern(
+            set, std::span<const double>(x0, size_t(vars->count)), mko::FdScheme::Central);
+        for (int k = 0; k < vars->count; ++k) {
+            nominal[k] = p.nominal[size_t(k)];
+            plus[k] = p.point_plus[size_t(k)];
+            minus[k] = p.point_minus[size_t(k)];
+            step_plus[k] = p.step_plus[size_t(k)];
+            step_minus[k] = p.step_minus[size_t(k)];
+            stencil[k] = static_cast<uint8_t>(p.stencil[size_t(k)]);
+        }
+    })
+}
+
+int ref_fd_jacobian(const mko_batch* base, const mko_residual_settings* s,
+                    const mko_variables* vars, int runs, const double* x0, double* J, double* r0,
+                    int64_t* status_changes) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*base);
+        const mko::VariableSet set(description_of(*base, *vars), h.topology());
+        mko::FdJacobianOptions o;
+        o.quadrature = settings_of(*s);
+        const mko::FdJacobian jac = mko::fd_jacobian_cpu(
+            set, h, std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o);
+        std::copy(jac.jacobian.begin(), jac.jacobian.end(), J);
+        std::copy(jac.residuals.begin(), jac.residuals.end(), r0);
+        std::copy(jac.status_changes.begin(), jac.status_changes.end(), status_changes);
+    })
+}
+
+int ref_adjoint_jacobian(const mko_batch* base, const mko_residual_settings* s,
+                         const mko_variables* vars, int runs, const double* x0, double* J,
+                         double* r0, uint8_t* status, int threads) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*base);
+        const mko::VariableSet set(description_of(*base, *vars), h.topology());
+        mko::AdjointJacobianOptions o;
+        o.quadrature = settings_of(*s);
+        o.threads = threads;
+        const mko::AdjointJacobian jac = mko::adjoint_jacobian_cpu(
+            set, h, std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o);
+        std::copy(jac.jacobian.begin(), jac.jacobian.end(), J);
+        std::copy(jac.residuals.begin(), jac.residuals.end(), r0);
+        std::copy(jac.status.begin(), jac.status.end(), status);
+    })
+}
+
+int ref_normal_equations(const double* J, const double* r, int runs, int P, int64_t R, double* a,
+                         double* g) {
+    REF_TRY({
+        const size_t p = size_t(P), rr = size_t(R);
+        for (int j = 0; j < runs; ++j) {
+            mko::normal_equations_host(std::span<const double>(J + size_t(j) * p * rr, p * rr),
+                                       std::span<const double>(r + size_t(j) * rr, rr), P,
+                                       std::span<double>(a + size_t(j) * p * p, p * p),
+                                       std::span<double>(g + size_t(j) * p, p));
+        }
+    })
+}
+
+uint64_t ref_splitmix64(uint64_t x) { return mko::splitmix64(x); }
+uint64_t ref_variant_seed(uint64_t seed, uint64_t j) { return mko::variant_seed(seed, j); }
+double ref_unit_uniform(uint64_t bits) { return mko::unit_uniform(bits); }
+
+int ref_perturbed_vector(const mko_batch* b, const mko_variables* vars, const double* x0,
+                         const double* relative, const double* absolute, int distribution,
+                         uint64_t seed, int first_is_base, uint64_t j, double* y) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::VariableSet set(description_of(*b, *vars), h.topology());
+        mko::PerturbationOptions o;
+        o.curvature = {relative[MKO_VAR_CURVATURE], absolute[MKO_VAR_CURVATURE]};
+        o.thickness = {relative[MKO_VAR_THICKNESS], absolute[MKO_VAR_THICKNESS]};
+        o.working_distance = {relative[MKO_VAR_WORKING_DISTANCE],
+                              absolute[MKO_VAR_WORKING_DISTANCE]};
+        o.distribution =
+            distribution == 0 ? mko::Distribution::Uniform : mko::Distribution::IrwinHall;
+        o.seed = seed;
+        o.first_is_base = first_is_base != 0;
+        const std::vector<double> r = mko::perturbed_vector(
+            set, std::span<const double>(x0, size_t(vars->count)), o, j);
+        std::copy(r.begin(), r.end(), y);
+    })
+}
+
+int ref_optimize_lm(const mko_batch* base, const mko_residual_settings* s,
+                    const mko_variables* vars, int runs, const double* x0,
+                    const mko_lm_options* options, double* x_out, mko_lm_run* run_out,
+                    int* cycles_out) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*base);
+        const mko::VariableSet set(description_of(*base, *vars), h.topology());
+        const mko::LmOptions o = lm_options_of(*options, *s);
+        const mko::LmResult r = mko::optimize_lm_cpu(
+            set, h, std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o);
+        const size_t P = size_t(vars->count);
+        for (int i = 0; i < runs; ++i) {
+            const mko::LmRunResult& q = r.runs[size_t(i)];
+            std::copy(q.x.begin(), q.x.end(), x_out + size_t(i) * P);
+            mko_lm_run& out = run_out[i];
+            out.merit = q.merit;
+            out.failed_rays = q.failed_rays;
+            out.hard_failed_rays = q.hard_failed_rays;
+            out.status = static_cast<int32_t>(q.status);
+            out.iterations = q.iterations;
+            out.accepted = q.accepted;
+            out.jacobians = q.jacobians;
+            out.damping = q.damping;
+            out.gradient_norm = q.gradient_norm;
+            out.model_decrease = q.model_decrease;
+            out.fail_weight = q.fail_weight;
+            out.penalty_rounds = q.penalty_rounds;
+        }
+        if (cycles_out) *cycles_out = r.cycles;
+    })
+}
+
+} // extern "C"

@@ -1,0 +1,223 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the reference driver
+(oracle/_ref/libmkoref.so). Test infrastructure: the checker side of every parity test."""
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from mikroptim_amd import abi
+from mikroptim_amd.batch import rays_per_variant, trace_rows
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libmkoref.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR, "-j8"], check=True)
+
+
+def _declare(lib, prefix):
+    p = prefix
+    bp, sp, vp = C.POINTER(abi.Batch), C.POINTER(abi.ResidualSettings), C.POINTER(abi.Variables)
+    sig = {
+        "last_error": ([], C.c_char_p),
+        "gauss_legendre": ([C.c_int, _dp, _dp], C.c_int),
+        "pupil_samples": ([C.c_int, C.c_int, C.c_int, _dp, _dp, _dp], C.c_int),
+        "normal_equations": ([_dp, _dp, C.c_int, C.c_int, C.c_int64, _dp, _dp], C.c_int),
+        "splitmix64": ([C.c_uint64], C.c_uint64),
+        "variant_seed": ([C.c_uint64, C.c_uint64], C.c_uint64),
+        "unit_uniform": ([C.c_uint64], C.c_double),
+    }
+    if prefix == "orc_":
+        sig.update({
+            "residuals": ([bp, sp, _dp, _dp, C.POINTER(C.c_uint8), C.c_int], C.c_int),
+            "merit": ([bp, sp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int], C.c_int),
+            "fd_jacobian": ([bp, sp, vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int64),
+                             C.POINTER(C.c_uint8), C.c_int], C.c_int),
+            "adjoint_jacobian": ([bp, sp, vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint8),
+                                  C.c_int], C.c_int),
+            "fd_pattern": ([vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8)],
+                           C.c_int),
+            "perturbed_vector": ([vp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_uint64, C.c_int,
+                                  C.c_uint64, _dp], C.c_int),
+            "optimize_lm": ([bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions), _dp,
+                             C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
+            "lm_linear": ([_dp, _dp, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
+                           C.POINTER(abi.LmOptions), _dp, C.POINTER(abi.LmRun)], C.c_int),
+            "trace_ray": ([bp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int,
+                           _dp, _dp, _dp, _dp], C.c_int),
+            "mt19937_64": ([C.c_uint64, C.c_uint64], C.c_uint64),
+        })
+    else:
+        sig.update({
+            "residuals": ([bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
+            "merit": ([bp, sp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+            "fd_jacobian": ([bp, sp, vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int64)], C.c_int),
+            "adjoint_jacobian": ([bp, sp, vp, C.c_int, _dp, _dp, _dp, C.POINTER(C.c_uint8),
+                                  C.c_int], C.c_int),
+            "fd_pattern": ([bp, vp, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
+            "perturbed_vector": ([bp, vp, _dp, _dp, _dp, C.c_int, C.c_uint64, C.c_int,
+                                  C.c_uint64, _dp], C.c_int),
+            "optimize_lm": ([bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions), _dp,
+                             C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
+        })
+    for name, (args, res) in sig.items():
+        f = getattr(lib, p + name)
+        f.argtypes = args
+        f.restype = res
+    return lib
+
+
+class Lib:
+    """Uniform Python API over liboracle.so ("orc_") or libmkoref.so ("ref_")."""
+
+    def __init__(self, path, prefix):
+        self.lib = _declare(C.CDLL(path), prefix)
+        self.prefix = prefix
+
+    def fn(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def check(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.fn("last_error")().decode())
+
+    def residuals(self, batch, settings, threads=0):
+        V = batch.variants
+        R = trace_rows(batch, settings)
+        rays = rays_per_variant(batch, settings)
+        res = np.zeros((V, R))
+        opd = np.zeros((V, rays))
+        st = np.zeros((V, rays), dtype=np.uint8)
+        args = [C.byref(batch.c_struct()), C.byref(settings.c_struct()), abi.ptr(res),
+                abi.ptr(opd), abi.ptr(st, C.c_uint8)]
+        if self.prefix == "orc_":
+            args.append(threads)
+        self.check(self.fn("residuals")(*args))
+        return res, opd, st
+
+    def merit(self, batch, settings, threads=0):
+        V = batch.variants
+        m = np.zeros(V)
+        f = np.zeros(V, dtype=np.int64)
+        h = np.zeros(V, dtype=np.int64)
+        args = [C.byref(batch.c_struct()), C.byref(settings.c_struct()), abi.ptr(m),
+                abi.ptr(f, C.c_int64), abi.ptr(h, C.c_int64)]
+        if self.prefix == "orc_":
+            args.append(threads)
+        self.check(self.fn("merit")(*args))
+        return m, f, h
+
+    def fd_jacobian(self, base, settings, variables, x0, threads=0):
+        x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
+        runs, P = x0.shape
+        R = trace_rows(base, settings)
+        J = np.zeros((runs, P, R))
+        r0 = np.zeros((runs, R))
+        ch = np.zeros((runs, P), dtype=np.int64)
+        args = [C.byref(base.c_struct()), C.byref(settings.c_struct()),
+                C.byref(variables.c_struct()), runs, abi.ptr(x0), abi.ptr(J), abi.ptr(r0),
+                abi.ptr(ch, C.c_int64)]
+        if self.prefix == "orc_":
+            args += [None, threads]
+        self.check(self.fn("fd_jacobian")(*args))
+        return J, r0, ch
+
+    def adjoint_jacobian(self, base, settings, variables, x0, threads=0):
+        x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
+        runs, P = x0.shape
+        R = trace_rows(base, settings)
+        rays = rays_per_variant(base, settings)
+        J = np.zeros((runs, P, R))
+        r0 = np.zeros((runs, R))
+        st = np.zeros((runs, rays), dtype=np.uint8)
+        self.check(self.fn("adjoint_jacobian")(
+            C.byref(base.c_struct()), C.byref(settings.c_struct()),
+            C.byref(variables.c_struct()), runs, abi.ptr(x0), abi.ptr(J), abi.ptr(r0),
+            abi.ptr(st, C.c_uint8), threads))
+        return J, r0, st
+
+    def normal_equations(self, J, r):
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        runs, P, R = J.shape
+        a = np.zeros((runs, P, P))
+        g = np.zeros((runs, P))
+        self.check(self.fn("normal_equations")(abi.ptr(J), abi.ptr(r), runs, P, R, abi.ptr(a),
+                                               abi.ptr(g)))
+        return a, g
+
+    def fd_pattern(self, batch, variables, x0):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        P = len(x0)
+        out = [np.zeros(P) for _ in range(5)]
+        st = np.zeros(P, dtype=np.uint8)
+        first = (C.byref(variables.c_struct()), batch.surfaces) if self.prefix == "orc_" else (
+            C.byref(batch.c_struct()), C.byref(variables.c_struct()))
+        self.check(self.fn("fd_pattern")(*first, abi.ptr(x0), *[abi.ptr(o) for o in out],
+                                         abi.ptr(st, C.c_uint8)))
+        return out + [st]
+
+    def perturbed_vector(self, batch, variables, x0, relative, absolute, distribution, seed,
+                         first_is_base, j):
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        rel = np.ascontiguousarray(relative, dtype=np.float64)
+        ab = np.ascontiguousarray(absolute, dtype=np.float64)
+        y = np.zeros(len(x0))
+        first = (C.byref(variables.c_struct()), batch.surfaces) if self.prefix == "orc_" else (
+            C.byref(batch.c_struct()), C.byref(variables.c_struct()))
+        self.check(self.fn("perturbed_vector")(*first, abi.ptr(x0), abi.ptr(rel), abi.ptr(ab),
+                                               distribution, seed, int(first_is_base), j,
+                                               abi.ptr(y)))
+        return y
+
+    def optimize_lm(self, base, settings, variables, x0, options):
+        x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
+        runs, P = x0.shape
+        x = np.zeros((runs, P))
+        res = (abi.LmRun * runs)()
+        cycles = C.c_int(0)
+        self.check(self.fn("optimize_lm")(
+            C.byref(base.c_struct()), C.byref(settings.c_struct()),
+            C.byref(variables.c_struct()), runs, abi.ptr(x0), C.byref(options), abi.ptr(x),
+            res, C.byref(cycles)))
+        return x, list(res), cycles.value
+
+    def gauss_legendre(self, n):
+        u = np.zeros(n)
+        w = np.zeros(n)
+        self.check(self.fn("gauss_legendre")(n, abi.ptr(u), abi.ptr(w)))
+        return u, w
+
+    def pupil_samples(self, rings, spokes, mode):
+        n = rings * (1 if mode == abi.PUPIL_AXIS else spokes)
+        rho, phi, w = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.check(self.fn("pupil_samples")(rings, spokes, mode, abi.ptr(rho), abi.ptr(phi),
+                                            abi.ptr(w)))
+        return rho, phi, w
+
+
+_cache = {}
+
+
+def oracle():
+    if "orc" not in _cache:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        _cache["orc"] = Lib(ORACLE_SO, "orc_")
+    return _cache["orc"]
+
+
+def reference():
+    """The reference build, or None where it is absent (e.g. no sources to build it from)."""
+    if "ref" not in _cache:
+        if not os.path.exists(REF_SO) and os.path.isdir("/root/reference/include"):
+            build()
+        _cache["ref"] = Lib(REF_SO, "ref_") if os.path.exists(REF_SO) else None
+    return _cache["ref"]
