@@ -1,0 +1,174 @@
+"""Benchmark: batched Levenberg-Marquardt iterations per second at 4096 variants x 48 variables on
+the 24-surface NA 0.95 objective (5 fields x 7 wavelengths x 16 rings x 32 spokes = 17920 rays
+per variant, hard ray failure), adjoint Jacobian + mixed precision, next to plain FP64
+finite-difference (FD) Jacobian LM on the same starts.
+
+A step is one LM cycle over all 4096 runs (src/optim/levenberg_marquardt.cpp:398-572): normal
+equations for the runs that need a new Jacobian, the damped step solve, one trial-point MERIT.
+Inputs are generated in host memory, uploaded once per backend call by the engine (inputs are
+the template system and the run points; the timed region includes that small H2D traffic).
+
+Prints one JSON line (see DESIGN.md for the fields). Single GPU only.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--fd-steps", type=int, default=3)
+    ap.add_argument("--variants", type=int, default=4096)
+    ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def sync():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps, threads):
+    from mikroptim_amd import abi
+
+    opt = abi.lm_options(max_iterations=warmup + steps + 5, threads=threads,
+                         jacobian_method=method, reject_new_failures=True)
+    sess = eng.lm(base, settings, variables, x0, opt, precision)
+    for _ in range(warmup):
+        sess.cycle()
+    sync()
+    eng.kernel_timing(True)
+    t0 = time.perf_counter()
+    done = 0
+    for _ in range(steps):
+        if not sess.cycle():
+            break
+        done += 1
+    sync()
+    dt = time.perf_counter() - t0
+    kt = eng.kernel_timing(False)
+    res = sess.results()
+    sess.close()
+    return done, dt, kt, res
+
+
+def cpu_baseline(batch, variables, settings, x_gpu_final, eng, sample_runs=64, threads=16):
+    """Reference CPU LM (the reference's own CpuLmBackend with its adjoint Jacobian, built from
+    source into oracle/_ref; the oracle's restatement if that build is absent) timed on a bounded
+    sample of runs for one LM cycle, scaled to 4096 runs. Also checks the GPU forward trace
+    against the FP64 oracle on the final points of a few runs (OPD error in waves)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from mikroptim_amd import abi
+    from oracle_lib import oracle, reference
+
+    lib, kind = reference(), "reference"
+    if lib is None:
+        lib, kind = oracle(), "port"
+    base = batch.take([0])
+    x0 = variables.extract(batch)[:sample_runs]
+    opt = abi.lm_options(max_iterations=1, threads=threads, jacobian_method=abi.JACOBIAN_ADJOINT)
+    t0 = time.perf_counter()
+    lib.optimize_lm(base, settings, variables, x0, opt)
+    dt = time.perf_counter() - t0
+    it_s = 1.0 / (dt * batch.variants / sample_runs)
+    # OPD error of the GPU forward trace vs the FP64 oracle at the GPU's final points
+    pts = base.take(np.zeros(4, dtype=np.int64))
+    variables.apply(pts, x_gpu_final[:4])
+    _, opd_g, st_g = eng.residuals(pts, settings)
+    _, opd_o, st_o = oracle().residuals(pts, settings, threads=threads)
+    lam = batch.reference_wavelength_nm * 1e-6
+    ok = (st_g == 0) & (st_o == 0)
+    err = float(np.sqrt(np.mean(((opd_g - opd_o)[ok] / lam) ** 2)))
+    return {"value": it_s, "unit": "LM iter/s @4096 variants", "cores": threads, "kind": kind,
+            "sample": f"{sample_runs} runs x 1 LM cycle (adjoint Jacobian, FP64, {threads} host "
+                      f"threads), scaled to {batch.variants} runs",
+            "seconds": dt}, err, bool(np.array_equal(st_g, st_o))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
+              "by variant ranges with no inter-GPU traffic, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from mikroptim_amd import abi, systems
+    from mikroptim_amd.engine import TraceEngine
+
+    batch, variables = systems.objective(variants=args.variants, seed=2)
+    settings = systems.objective_settings(abi.FAILURE_HARD)
+    base = batch.take([0])
+    x0 = variables.extract(batch)
+    eng = TraceEngine(0)
+
+    n_adj, t_adj, kt_adj, res_adj = run_lm(eng, base, settings, variables, x0,
+                                           abi.JACOBIAN_ADJOINT, "mixed", args.warmup,
+                                           args.steps, args.threads)
+    it_adj = n_adj / t_adj
+    n_fd, t_fd, kt_fd, res_fd = run_lm(eng, base, settings, variables, x0, abi.JACOBIAN_FD,
+                                       "fp64", 1, args.fd_steps, args.threads)
+    it_fd = n_fd / t_fd
+
+    # roofline of the dominant kernel of the measured path (the adjoint kernel): algorithmic
+    # FLOP per launch = rays * (FLOP_fwd + FLOP_adj), FLOP_fwd = 59 (S-1) + 60,
+    # FLOP_adj = FLOP_fwd + 210 (S-1) + 6 P (counting div/sqrt as 1, FMA as 2)
+    S, P = batch.surfaces, variables.size
+    rays = 17920
+    flop_fwd = 59 * (S - 1) + 60
+    flop_adj = flop_fwd + 210 * (S - 1) + 6 * P
+    adj_ms, adj_n = kt_adj["adjoint"]
+    runs_per_launch = args.variants  # every Jacobian call covers the runs needing one
+    achieved = None
+    if adj_n:
+        # average runs per launch from the LM statistics is not exposed; the per-launch FLOP
+        # uses the runs that needed a Jacobian in the timed cycles
+        jac_runs = int(sum(r.jacobians for r in res_adj.runs))
+        per_launch_runs = jac_runs / max(1, (args.warmup + n_adj))
+        achieved = per_launch_runs * rays * (flop_fwd + flop_adj) / (adj_ms / adj_n * 1e-3) / 1e12
+        runs_per_launch = per_launch_runs
+    roof = {"bound": "valu", "kernel": "adjoint_kernel", "achieved": achieved,
+            "peak": 78.6, "unit": "TFLOP/s",
+            "frac": (achieved / 78.6) if achieved else None, "traffic": None,
+            "avg_launch_ms": (adj_ms / adj_n) if adj_n else None,
+            "runs_per_launch": runs_per_launch}
+
+    out = {
+        "metric": "LM iter/s @4096 variants: adjoint+mixed vs FP64 FD; err <=1e-4 waves RMS",
+        "value": it_adj, "unit": "LM iter/s", "n_gpus": 1, "steps": n_adj,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_adj / max(1, n_adj),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+        "data": "synthetic (seeded 24-surface NA 0.95 objective, N(0,1%) jitter)",
+        "config": {"workload": "4096 variants x 48 variables, 24-surface NA 0.95 objective, "
+                               "5 fields x 7 wavelengths x 16x32 pupil (17920 rays/variant), "
+                               "hard failure", "variants": args.variants, "variables": P,
+                   "rays_per_variant": rays},
+        "fd_fp64_iter_per_s": it_fd, "fd_steps": n_fd,
+        "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
+        "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
+        "kernel_ms_fd_path": {k: v[0] for k, v in kt_fd.items()},
+        "final_mean_merit": {"adjoint_mixed": float(np.mean(res_adj.merit)),
+                             "fd_fp64": float(np.mean(res_fd.merit))},
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        base_cpu, err, same_status = cpu_baseline(batch, variables, settings, res_adj.x, eng)
+        out["cpu_baseline"] = base_cpu
+        out["opd_err_waves_rms"] = err
+        out["opd_status_equal"] = same_status
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

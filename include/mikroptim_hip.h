@@ -1,0 +1,143 @@
+/* C-ABI of libmikroptim_hip.so: the MI355X (gfx950) implementation of Mikroptim's batched
+ * forward-and-Jacobian path. Plain C types only (pointers, sizes, the structs of
+ * mikroptim_types.h); every entry point returns 0 or a negative MKO_ERR_* code, and
+ * mko_last_error() gives the message of the last failure on the calling thread. Arguments are
+ * checked before any device work (std::invalid_argument in the reference -> MKO_ERR_INVALID);
+ * device and runtime failures give MKO_ERR_DEVICE (mko::cuda::CudaError in the reference).
+ *
+ * Two layers:
+ *  - *_device: one kernel family each, on a device-resident batch (per-variant arrays of the
+ *    mko_batch in device memory, batch-wide arrays in host memory), asynchronous on `stream`
+ *    (a hipStream_t, NULL = default stream).
+ *  - mko_engine_* / mko_lm_*: the host layer (TraceEngine / GpuLmBackend / levenberg_marquardt
+ *    of the reference) on host-resident batches; synchronous like the reference. */
+#ifndef MIKROPTIM_HIP_H
+#define MIKROPTIM_HIP_H
+
+#include "mikroptim_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MKO_ERR_INVALID (-1) /* bad arguments, nothing launched */
+#define MKO_ERR_DEVICE (-2)  /* HIP runtime or device failure */
+
+const char* mko_last_error(void);
+
+/* ---- device layer ------------------------------------------------------------------------ */
+
+/* RESIDUALS mode: residual rows r_k (and, in soft mode, penalty rows after the ray rows) of
+ * every variant at residuals[v * stride ...], OPD [V][rays] and status [V][rays] (both may be
+ * NULL). Replaces residuals_kernel + DeviceResiduals::run (src/cuda/residuals.cu:58, :240;
+ * include/mikroptim/cuda/residuals.hpp). */
+int mko_residuals_device(const mko_batch* batch, const mko_residual_settings* settings,
+                         double* residuals, int64_t stride, double* opd, uint8_t* status,
+                         void* stream);
+
+/* MERIT mode: merit[v] = sum over tuples of (sum r^2 + sum p^2) + extra_merit[v] (NULL = 0),
+ * failed / hard-failed ray counts. Workspace: partial [V * F * L] doubles, partial_counts
+ * [2 * V * F * L] ints. Replaces merit_kernel + merit_finalize_kernel + DeviceMerit::run
+ * (src/cuda/merit.cu:63, :207; include/mikroptim/cuda/merit.hpp). */
+int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settings,
+                     const double* extra_merit, double* merit, int64_t* failed,
+                     int64_t* hard_failed, double* partial, int* partial_counts, void* stream);
+
+/* Writes runs * per_run variants: variant (i, j) is a copy of template variant i (or 0 if
+ * !template_per_run) with the variables x0[i] applied; for j > 0, component (j - 1) / 2 is
+ * replaced by plus[i] (j odd) or minus[i] (j even). per_run = 2P + 1 gives the central FD
+ * pattern (write_fd_variants, src/optim/variant_generator.cpp:240-262), per_run = 1 the point
+ * batch of LM merit / adjoint calls (lm_point_batch, src/optim/levenberg_marquardt.cpp:634).
+ * vars_device: variable type / surface arrays in device memory; x0, plus, minus [runs][P] in
+ * device memory; outputs are the per-variant arrays of the destination batch. */
+int mko_point_batch_device(const mko_batch* templ, int template_per_run, int runs, int per_run,
+                           const mko_variables* vars_device, const double* x0,
+                           const double* plus, const double* minus, double* curvature,
+                           double* thickness, double* index, double* pupil_radius,
+                           double* field_angle, double* field_target,
+                           double* reference_curvature, void* stream);
+
+/* FD Jacobian from the residuals of a central FD batch (runs * (2P + 1) variants, rows per
+ * variant at stride): J [runs][P][rows] by fd_derivative, r0 [runs][rows], status changes
+ * [runs][P]. Replaces fd_difference_kernel (src/cuda/fd_jacobian.cu:65). */
+int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
+                             const uint8_t* fd_status, int64_t rays, int runs, int variables,
+                             int64_t rows, const double* step_plus, const double* step_minus,
+                             const uint8_t* stencil, double* jacobian, double* r0,
+                             int64_t* status_changes, void* stream);
+
+/* ADJOINT Jacobian of a batch with one variant per run: J [V][P][trace_rows] (double for
+ * MKO_PRECISION_FP64, float for MKO_PRECISION_MIXED), r0 [V][trace_rows] (double), status
+ * [V][rays] (may be NULL). Replaces adjoint_jacobian_kernel + DeviceAdjointJacobian::compute
+ * (src/cuda/adjoint_jacobian.cu:119, :401). */
+int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_settings* settings,
+                                const mko_variables* vars, int precision, void* jacobian,
+                                double* r0, uint8_t* status, void* stream);
+
+/* Normal equations of each run: a [runs][P][P] = J^T J (a[q*P + p]), g [runs][P] = J^T r0.
+ * Every element is one FMA chain over the rows in ascending order (for FP64 J bitwise equal to
+ * normal_equations_host). Replaces gram_kernel + DeviceNormalEquations::compute
+ * (src/cuda/normal_equations.cu:48, :155). */
+int mko_gram_device(const void* jacobian, const double* r0, int runs, int variables,
+                    int64_t rows, int precision, double* a, double* g, void* stream);
+
+/* ---- host layer ----------------------------------------------------------------------------- */
+
+typedef struct mko_engine mko_engine;
+
+/* TraceEngine(int device, EngineOptions) (include/mikroptim/cuda/trace_engine.hpp):
+ * memory_limit_bytes = 0 uses the device's free memory minus a reserve. */
+mko_engine* mko_engine_create(int device, size_t memory_limit_bytes);
+void mko_engine_destroy(mko_engine* engine);
+
+/* Device time per kernel family, measured with HIP events on the launch stream around each
+ * launch while enabled (AdjointJacobianTiming, include/mikroptim/cuda/adjoint_jacobian.hpp:39):
+ * ms / launches [5] = trace (RESIDUALS and MERIT), adjoint, gram, fd difference, point batch.
+ * Returns the totals since the previous call, resets them and sets the enable flag. */
+int mko_engine_kernel_timing(mko_engine* engine, int enable, double* ms, int64_t* launches);
+
+/* TraceEngine::run(batch, TraceMode::Residuals, ...): host batch in, host results out
+ * (residuals [V][trace_rows], opd / status [V][rays], either may be NULL). */
+int mko_engine_residuals(mko_engine* engine, const mko_batch* batch,
+                         const mko_residual_settings* settings, double* residuals, double* opd,
+                         uint8_t* status);
+
+/* TraceEngine::run(batch, TraceMode::Merit, ...) with MeritRunOptions::extra_merit. */
+int mko_engine_merit(mko_engine* engine, const mko_batch* batch,
+                     const mko_residual_settings* settings, const double* extra_merit,
+                     double* merit, int64_t* failed, int64_t* hard_failed);
+
+/* DeviceFdJacobian::compute / DeviceAdjointJacobian::compute + download
+ * (include/mikroptim/cuda/fd_jacobian.hpp:58, adjoint_jacobian.hpp:70): base has one variant
+ * (shared template) or one per run; x0 [runs][P]. Outputs (host, any may be NULL):
+ * J [runs][P][trace_rows] double, r0 [runs][trace_rows], a [runs][P][P], g [runs][P]. */
+int mko_engine_jacobian(mko_engine* engine, const mko_batch* base,
+                        const mko_residual_settings* settings, const mko_variables* vars,
+                        int runs, const double* x0, int method, int precision, double* jacobian,
+                        double* r0, double* a, double* g);
+
+/* Batched Levenberg-Marquardt on the GPU backend (optimize_lm_gpu,
+ * include/mikroptim/cuda/levenberg_marquardt.hpp:91; control logic of
+ * src/optim/levenberg_marquardt.cpp:332-572, outer penalty rounds :891). precision applies to
+ * the adjoint Jacobian. x_out [runs][P], run results [runs]. */
+int mko_optimize_lm(mko_engine* engine, const mko_batch* base,
+                    const mko_residual_settings* settings, const mko_variables* vars, int runs,
+                    const double* x0, const mko_lm_options* options, int precision,
+                    double* x_out, mko_lm_run* run_out, int* cycles_out);
+
+/* Stepwise form of the same LM (one handle, one cycle per mko_lm_cycle call) for callers that
+ * time or interleave cycles. mko_lm_cycle returns 1 while runs are active, 0 when all are done,
+ * < 0 on error. */
+typedef struct mko_lm mko_lm;
+mko_lm* mko_lm_create(mko_engine* engine, const mko_batch* base,
+                      const mko_residual_settings* settings, const mko_variables* vars, int runs,
+                      const double* x0, const mko_lm_options* options, int precision);
+int mko_lm_cycle(mko_lm* lm);
+int mko_lm_results(mko_lm* lm, double* x_out, mko_lm_run* run_out, int* cycles_out);
+void mko_lm_destroy(mko_lm* lm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1146 @@
+// Host layer of libmikroptim_hip.so (C++20): the TraceEngine / GPU LM backend / batched
+// Levenberg-Marquardt of the reference, over the device entry points of kernels.hip.
+//
+//   Engine            TraceEngine (include/mikroptim/cuda/trace_engine.hpp; tranches
+//                     src/cuda/trace_engine.cu:199, run :275): device buffers that only grow,
+//                     per-call memory budget, variants split into tranches that fit it.
+//   Engine::jacobian  DeviceFdJacobian::compute (src/cuda/fd_jacobian.cu:203) and
+//                     DeviceAdjointJacobian::compute (src/cuda/adjoint_jacobian.cu:401) followed
+//                     by DeviceNormalEquations (src/cuda/normal_equations.cu:155); runs chunked so
+//                     the Jacobian fits the budget (GpuLmBackend::max_jacobian_runs,
+//                     src/cuda/levenberg_marquardt.cpp:35-67).
+//   Lm                levenberg_marquardt (src/optim/levenberg_marquardt.cpp:332-572) with the
+//                     GPU backend (GpuLmBackend, src/cuda/levenberg_marquardt.cpp:24-248) and
+//                     the outer penalty rounds of optimize_lm_rounds (:891-983), cut into cycles.
+// All calls are synchronous; arguments are checked before device work.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "error.hpp"
+#include "mikroptim_hip.h"
+
+namespace mko_host {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+inline void dev_check(int rc) {
+    if (rc == MKO_ERR_INVALID) throw std::invalid_argument(mko_detail::last_error);
+    if (rc != 0) throw DeviceError(mko_detail::last_error);
+}
+
+// Device allocation that only grows (reused across calls).
+class Buffer {
+public:
+    Buffer() = default;
+    Buffer(const Buffer&) = delete;
+    Buffer& operator=(const Buffer&) = delete;
+    ~Buffer() {
+        if (ptr_) (void)hipFree(ptr_);
+    }
+    void* reserve(size_t bytes) {
+        if (bytes > size_) {
+            if (ptr_) hip_check(hipFree(ptr_), "hipFree");
+            ptr_ = nullptr;
+            size_ = 0;
+            hip_check(hipMalloc(&ptr_, std::max<size_t>(bytes, 256)), "hipMalloc");
+            size_ = bytes;
+        }
+        return ptr_;
+    }
+    template <class T>
+    T* as(size_t count) {
+        return static_cast<T*>(reserve(count * sizeof(T)));
+    }
+    size_t size() const { return size_; }
+
+private:
+    void* ptr_ = nullptr;
+    size_t size_ = 0;
+};
+
+// Restores the caller's current device (DeviceGuard of the reference).
+struct DeviceGuard {
+    int old = 0;
+    explicit DeviceGuard(int dev) {
+        hip_check(hipGetDevice(&old), "hipGetDevice");
+        hip_check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard() { (void)hipSetDevice(old); }
+};
+
+inline int64_t rays_of(const mko_batch& b, const mko_residual_settings& s) {
+    int64_t rays = 0;
+    for (int f = 0; f < b.fields; ++f)
+        rays += int64_t(b.wavelengths) * s.rings * (b.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : s.spokes);
+    return rays;
+}
+inline int64_t trace_rows_of(const mko_batch& b, const mko_residual_settings& s) {
+    const int64_t r = rays_of(b, s);
+    return s.failure == MKO_FAILURE_SOFT ? 2 * r : r;
+}
+
+// Per-variant arrays of a batch: element counts per variant.
+struct Strides {
+    size_t S, SL, F;
+    bool refc;
+    explicit Strides(const mko_batch& b)
+        : S(size_t(b.surfaces)), SL(size_t(b.surfaces) * size_t(b.wavelengths)),
+          F(size_t(b.fields)), refc(b.reference_sphere != MKO_REF_LAST_SURFACE) {}
+    size_t doubles() const { return 2 * S + SL + 1 + 2 * F + (refc ? F : 0); }
+};
+
+void check_batch(const mko_batch& b) {
+    if (b.variants < 0) throw std::invalid_argument("negative variant count");
+    if (b.surfaces < 2 || b.surfaces > MKO_MAX_SURFACES)
+        throw std::invalid_argument("surfaces outside [2, 64]");
+    if (b.wavelengths < 1 || b.fields < 1) throw std::invalid_argument("no wavelengths or fields");
+    if (!b.type || !b.pupil_mode || !b.wavelength_weight || !b.field_weight)
+        throw std::invalid_argument("batch-wide arrays missing");
+    if (b.reference_sphere != MKO_REF_LAST_SURFACE && !b.reference_curvature)
+        throw std::invalid_argument("image reference sphere needs reference_curvature");
+}
+
+// validate_values (src/batch/system_batch.cpp:558): finite values, pupil radius > 0, indices > 0.
+void validate_values(const mko_batch& b, int first, int count) {
+    const Strides st(b);
+    for (int v = first; v < first + count; ++v) {
+        auto bad = [&](const char* what) {
+            throw std::invalid_argument("variant " + std::to_string(v) + ": " + what);
+        };
+        for (size_t i = 0; i < st.S; ++i) {
+            if (!std::isfinite(b.curvature[v * st.S + i]) || !std::isfinite(b.thickness[v * st.S + i]))
+                bad("curvature or thickness not finite");
+        }
+        for (size_t i = 0; i < st.SL; ++i) {
+            const double n = b.index[v * st.SL + i];
+            if (!(n > 0.0) || !std::isfinite(n)) bad("refractive index not finite and > 0");
+        }
+        if (!(b.pupil_radius[v] > 0.0) || !std::isfinite(b.pupil_radius[v])) bad("pupil radius");
+        for (size_t f = 0; f < st.F; ++f) {
+            if (!std::isfinite(b.field_angle[v * st.F + f]) || !std::isfinite(b.field_target[v * st.F + f]))
+                bad("field angle or target not finite");
+        }
+    }
+}
+
+// Device copy of (a range of) a host batch in one allocation.
+struct DeviceBatch {
+    Buffer mem;
+    mko_batch view{};
+    void upload(const mko_batch& h, int first, int count, hipStream_t stream) {
+        const Strides st(h);
+        const size_t V = size_t(count);
+        double* base = mem.as<double>(std::max<size_t>(1, V * st.doubles()));
+        view = h;
+        view.variants = count;
+        double* p = base;
+        auto put = [&](const double* src, size_t per, const double** dst) {
+            if (src == nullptr) {
+                *dst = nullptr;
+                return;
+            }
+            hip_check(hipMemcpyAsync(p, src + size_t(first) * per, V * per * sizeof(double),
+                                     hipMemcpyHostToDevice, stream),
+                      "upload batch");
+            *dst = p;
+            p += V * per;
+        };
+        put(h.curvature, st.S, &view.curvature);
+        put(h.thickness, st.S, &view.thickness);
+        put(h.index, st.SL, &view.index);
+        put(h.pupil_radius, 1, &view.pupil_radius);
+        put(h.field_angle, st.F, &view.field_angle);
+        put(h.field_target, st.F, &view.field_target);
+        put(st.refc ? h.reference_curvature : nullptr, st.F, &view.reference_curvature);
+    }
+    // Uninitialised device batch of `count` variants shaped like h (filled by a kernel).
+    void allocate(const mko_batch& h, int count) {
+        const Strides st(h);
+        const size_t V = size_t(count);
+        double* p = mem.as<double>(std::max<size_t>(1, V * st.doubles()));
+        view = h;
+        view.variants = count;
+        auto take = [&](size_t per, const double** dst, bool present = true) {
+            if (!present) {
+                *dst = nullptr;
+                return;
+            }
+            *dst = p;
+            p += V * per;
+        };
+        take(st.S, &view.curvature);
+        take(st.S, &view.thickness);
+        take(st.SL, &view.index);
+        take(1, &view.pupil_radius);
+        take(st.F, &view.field_angle);
+        take(st.F, &view.field_target);
+        take(st.F, &view.reference_curvature, st.refc);
+    }
+    double* mut(const double* p) { return const_cast<double*>(p); }
+};
+
+// ---- FD pattern (central scheme): fd_pattern, src/optim/variant_generator.cpp:177-237 -------
+
+constexpr double kFdStepFactor = 6.0554544523933395e-06; // 2^(-52/3)
+
+struct Pattern {
+    std::vector<double> plus, minus, step_plus, step_minus;
+    std::vector<uint8_t> stencil;
+};
+
+void fd_pattern(const mko_variables& v, const double* x0, Pattern& out, size_t at) {
+    for (int k = 0; k < v.count; ++k) {
+        const double x = x0[k];
+        const double lo = v.lower[k], hi = v.upper[k];
+        if (!std::isfinite(x) || !(x >= lo && x <= hi))
+            throw std::invalid_argument("x0 not finite or outside the bounds");
+        const double h = kFdStepFactor * std::max(std::abs(x), v.typical[k]);
+        const double up = x + h, down = x - h;
+        double xa, xb;
+        uint8_t sten = MKO_STENCIL_CENTRAL;
+        if (up <= hi && down >= lo) {
+            xa = up;
+            xb = down;
+        } else {
+            const bool towards_upper = hi - x >= x - lo;
+            const double far = towards_upper ? x + 2.0 * h : x - 2.0 * h;
+            if (towards_upper ? far <= hi : far >= lo) {
+                sten = MKO_STENCIL_ONE_SIDED;
+                xa = towards_upper ? up : down;
+                xb = far;
+            } else {
+                sten = MKO_STENCIL_TRUNCATED;
+                xa = up <= hi ? up : hi;
+                xb = down >= lo ? down : lo;
+            }
+        }
+        out.plus[at + k] = xa;
+        out.minus[at + k] = xb;
+        out.step_plus[at + k] = xa - x;
+        out.step_minus[at + k] = xb - x;
+        out.stencil[at + k] = sten;
+    }
+}
+
+// ---- kernel timing (AdjointJacobianTiming-like, include/mikroptim/cuda/adjoint_jacobian.hpp:39) --
+
+enum KernelFamily { kTrace = 0, kAdjoint = 1, kGram = 2, kFdDifference = 3, kPointBatch = 4,
+                    kFamilies = 5 };
+
+// Accumulates the device time of each kernel family with HIP events recorded on the launch
+// stream (the null stream) around each launch, when enabled.
+class KernelTimer {
+public:
+    bool enabled = false;
+    double ms[kFamilies] = {};
+    int64_t launches[kFamilies] = {};
+    template <class F>
+    void run(KernelFamily fam, F&& launch) {
+        if (!enabled) {
+            launch();
+            return;
+        }
+        hipEvent_t a, b;
+        hip_check(hipEventCreate(&a), "hipEventCreate");
+        hip_check(hipEventCreate(&b), "hipEventCreate");
+        hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
+        launch();
+        hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
+        hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+        float t = 0.0f;
+        hip_check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        ms[fam] += t;
+        ++launches[fam];
+    }
+    void reset() {
+        for (int i = 0; i < kFamilies; ++i) {
+            ms[i] = 0.0;
+            launches[i] = 0;
+        }
+    }
+};
+
+// ---- Engine --------------------------------------------------------------------------------
+
+class Engine {
+public:
+    Engine(int device, size_t memory_limit) : device_(device), limit_(memory_limit) {
+        int n = 0;
+        hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+        if (n == 0) throw DeviceError("no HIP device");
+        if (device < 0 || device >= n) throw std::invalid_argument("device index out of range");
+    }
+
+    // Usable bytes for one call: free device memory minus a reserve of max(256 MiB, 5 %)
+    // (TraceEngine::plan_tranches, src/cuda/trace_engine.cu:199), capped by the option.
+    size_t budget() const {
+        size_t free_b = 0, total = 0;
+        hip_check(hipMemGetInfo(&free_b, &total), "hipMemGetInfo");
+        const size_t reserve = std::max<size_t>(size_t(256) << 20, total / 20);
+        size_t b = free_b > reserve ? free_b - reserve : 0;
+        if (limit_ > 0) b = std::min(b, limit_);
+        return b;
+    }
+
+    void residuals(const mko_batch& b, const mko_residual_settings& s, double* res, double* opd,
+                   uint8_t* status) {
+        DeviceGuard g(device_);
+        check_batch(b);
+        validate_values(b, 0, b.variants);
+        const int64_t R = trace_rows_of(b, s), rays = rays_of(b, s);
+        const size_t per = Strides(b).doubles() * 8 + size_t(R) * 8 + size_t(rays) * 9;
+        const int tranche = tranche_size(b.variants, per);
+        for (int first = 0; first < b.variants; first += tranche) {
+            const int count = std::min(tranche, b.variants - first);
+            batch_.upload(b, first, count, nullptr);
+            double* dres = out_res_.as<double>(size_t(count) * R);
+            double* dopd = opd ? out_opd_.as<double>(size_t(count) * rays) : nullptr;
+            uint8_t* dst = status ? out_st_.as<uint8_t>(size_t(count) * rays) : nullptr;
+            timer.run(kTrace, [&] {
+                dev_check(mko_residuals_device(&batch_.view, &s, dres, R, dopd, dst, nullptr));
+            });
+            if (res) copy_out(res + size_t(first) * R, dres, size_t(count) * R * 8);
+            if (opd) copy_out(opd + size_t(first) * rays, dopd, size_t(count) * rays * 8);
+            if (status) copy_out(status + size_t(first) * rays, dst, size_t(count) * rays);
+        }
+        hip_check(hipDeviceSynchronize(), "residuals");
+    }
+
+    void merit(const mko_batch& b, const mko_residual_settings& s, const double* extra,
+               double* merit, int64_t* failed, int64_t* hard) {
+        DeviceGuard g(device_);
+        check_batch(b);
+        validate_values(b, 0, b.variants);
+        const size_t T = size_t(b.fields) * b.wavelengths;
+        const size_t per = Strides(b).doubles() * 8 + T * 16 + 32;
+        const int tranche = tranche_size(b.variants, per);
+        for (int first = 0; first < b.variants; first += tranche) {
+            const int count = std::min(tranche, b.variants - first);
+            batch_.upload(b, first, count, nullptr);
+            merit_device(batch_.view, s, extra ? extra + first : nullptr, count);
+            copy_out(merit + first, d_merit_, size_t(count) * 8);
+            copy_out(failed + first, d_failed_, size_t(count) * 8);
+            copy_out(hard + first, d_hard_, size_t(count) * 8);
+        }
+        hip_check(hipDeviceSynchronize(), "merit");
+    }
+
+    // merit of a device-resident batch into d_merit_/d_failed_/d_hard_
+    void merit_device(const mko_batch& dev, const mko_residual_settings& s, const double* extra_host,
+                      int count) {
+        const size_t T = size_t(dev.fields) * dev.wavelengths;
+        double* part = part_.as<double>(size_t(count) * T);
+        int* pc = partc_.as<int>(size_t(count) * T * 2);
+        d_merit_ = out_merit_.as<double>(size_t(count));
+        d_failed_ = out_failed_.as<int64_t>(size_t(count));
+        d_hard_ = out_hard_.as<int64_t>(size_t(count));
+        const double* dextra = nullptr;
+        if (extra_host) {
+            double* e = extra_.as<double>(size_t(count));
+            hip_check(hipMemcpy(e, extra_host, size_t(count) * 8, hipMemcpyHostToDevice), "extra");
+            dextra = e;
+        }
+        timer.run(kTrace, [&] {
+            dev_check(mko_merit_device(&dev, &s, dextra, d_merit_, d_failed_, d_hard_, part, pc, nullptr));
+        });
+    }
+
+    // J / r0 / A / g for `runs` points; base has 1 variant or one per run.
+    void jacobian(const mko_batch& base, const mko_residual_settings& s, const mko_variables& vars,
+                  int runs, const double* x0, int method, int precision, double* J, double* r0,
+                  double* a, double* g) {
+        DeviceGuard guard(device_);
+        check_batch(base);
+        if (base.variants != 1 && base.variants != runs)
+            throw std::invalid_argument("base must have one variant or one per run");
+        if (vars.count < 1) throw std::invalid_argument("no variables");
+        if (method != MKO_JACOBIAN_FD && method != MKO_JACOBIAN_ADJOINT)
+            throw std::invalid_argument("unknown Jacobian method");
+        if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED)
+            throw std::invalid_argument("unknown precision");
+        validate_values(base, 0, base.variants);
+        const int P = vars.count;
+        const int64_t R = trace_rows_of(base, s), rays = rays_of(base, s);
+        const bool fd = method == MKO_JACOBIAN_FD;
+        const int per_run = fd ? 2 * P + 1 : 1;
+        const size_t jbytes = (fd || precision == MKO_PRECISION_FP64) ? 8 : 4;
+        const size_t per = size_t(per_run) * (Strides(base).doubles() * 8 +
+                                              (fd ? size_t(R) * 8 + size_t(rays) : 0)) +
+                           size_t(P) * R * jbytes + size_t(R) * 8 + size_t(rays) +
+                           size_t(P) * (P + 1) * 8 + 64 * size_t(P);
+        const int chunk = tranche_size(runs, per);
+        upload_vars(vars);
+        Pattern pat;
+        if (fd) {
+            pat.plus.resize(size_t(runs) * P);
+            pat.minus.resize(size_t(runs) * P);
+            pat.step_plus.resize(size_t(runs) * P);
+            pat.step_minus.resize(size_t(runs) * P);
+            pat.stencil.resize(size_t(runs) * P);
+            for (int i = 0; i < runs; ++i) fd_pattern(vars, x0 + size_t(i) * P, pat, size_t(i) * P);
+        } else {
+            for (size_t i = 0; i < size_t(runs) * P; ++i)
+                if (!std::isfinite(x0[i])) throw std::invalid_argument("x0 not finite");
+        }
+        if (base.variants == 1) tmpl_.upload(base, 0, 1, nullptr);
+        for (int first = 0; first < runs; first += chunk) {
+            const int count = std::min(chunk, runs - first);
+            if (base.variants != 1) tmpl_.upload(base, first, count, nullptr);
+            double* dx = x_.as<double>(size_t(count) * P);
+            hip_check(hipMemcpy(dx, x0 + size_t(first) * P, size_t(count) * P * 8,
+                                hipMemcpyHostToDevice), "x0");
+            double *dplus = nullptr, *dminus = nullptr;
+            if (fd) {
+                dplus = plus_.as<double>(size_t(count) * P);
+                dminus = minus_.as<double>(size_t(count) * P);
+                hip_check(hipMemcpy(dplus, pat.plus.data() + size_t(first) * P, size_t(count) * P * 8,
+                                    hipMemcpyHostToDevice), "pattern");
+                hip_check(hipMemcpy(dminus, pat.minus.data() + size_t(first) * P, size_t(count) * P * 8,
+                                    hipMemcpyHostToDevice), "pattern");
+            }
+            work_.allocate(base, count * per_run);
+            dev_check(mko_point_batch_device(
+                &tmpl_.view, base.variants != 1, count, per_run, &dvars_, dx, dplus, dminus,
+                work_.mut(work_.view.curvature), work_.mut(work_.view.thickness),
+                work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
+                work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
+                work_.mut(work_.view.reference_curvature), nullptr));
+            void* dJ = jac_.reserve(size_t(count) * P * R * jbytes);
+            double* dr0 = r0_.as<double>(size_t(count) * R);
+            if (fd) {
+                double* fres = fdres_.as<double>(size_t(count) * per_run * R);
+                uint8_t* fst = fdst_.as<uint8_t>(size_t(count) * per_run * rays);
+                timer.run(kTrace, [&] {
+                    dev_check(mko_residuals_device(&work_.view, &s, fres, R, nullptr, fst, nullptr));
+                });
+                double* dsp = sp_.as<double>(size_t(count) * P);
+                double* dsm = sm_.as<double>(size_t(count) * P);
+                uint8_t* dsten = sten_.as<uint8_t>(size_t(count) * P);
+                int64_t* dch = ch_.as<int64_t>(size_t(count) * P);
+                hip_check(hipMemcpy(dsp, pat.step_plus.data() + size_t(first) * P, size_t(count) * P * 8,
+                                    hipMemcpyHostToDevice), "steps");
+                hip_check(hipMemcpy(dsm, pat.step_minus.data() + size_t(first) * P, size_t(count) * P * 8,
+                                    hipMemcpyHostToDevice), "steps");
+                hip_check(hipMemcpy(dsten, pat.stencil.data() + size_t(first) * P, size_t(count) * P,
+                                    hipMemcpyHostToDevice), "stencil");
+                timer.run(kFdDifference, [&] {
+                    dev_check(mko_fd_difference_device(fres, R, fst, rays, count, P, R, dsp, dsm,
+                                                       dsten, static_cast<double*>(dJ), dr0, dch,
+                                                       nullptr));
+                });
+            } else {
+                timer.run(kAdjoint, [&] {
+                    dev_check(mko_adjoint_jacobian_device(&work_.view, &s, &vars_host_dev_,
+                                                          precision, dJ, dr0, nullptr, nullptr));
+                });
+            }
+            const int jprec = fd ? MKO_PRECISION_FP64 : precision;
+            if (a || g) {
+                double* da = a_.as<double>(size_t(count) * P * P);
+                double* dg = g_.as<double>(size_t(count) * P);
+                timer.run(kGram, [&] {
+                    dev_check(mko_gram_device(dJ, dr0, count, P, R, jprec, da, dg, nullptr));
+                });
+                if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
+                if (g) copy_out(g + size_t(first) * P, dg, size_t(count) * P * 8);
+            }
+            if (J) {
+                if (jprec == MKO_PRECISION_FP64) {
+                    copy_out(J + size_t(first) * P * R, dJ, size_t(count) * P * R * 8);
+                } else {
+                    std::vector<float> tmp(size_t(count) * P * R);
+                    copy_out(tmp.data(), dJ, tmp.size() * 4);
+                    for (size_t i = 0; i < tmp.size(); ++i) J[size_t(first) * P * R + i] = tmp[i];
+                }
+            }
+            if (r0) copy_out(r0 + size_t(first) * R, dr0, size_t(count) * R * 8);
+            hip_check(hipDeviceSynchronize(), "jacobian");
+        }
+    }
+
+    // Merit of `runs` points: copies of the template with x applied (lm_point_batch).
+    void point_merit(const mko_batch& base, const mko_residual_settings& s, const mko_variables& vars,
+                     int runs, const double* x, double* merit, int64_t* failed, int64_t* hard) {
+        DeviceGuard guard(device_);
+        const int P = vars.count;
+        for (size_t i = 0; i < size_t(runs) * P; ++i)
+            if (!std::isfinite(x[i])) throw std::invalid_argument("x not finite");
+        const size_t T = size_t(base.fields) * base.wavelengths;
+        const size_t per = Strides(base).doubles() * 8 + T * 16 + 64 + size_t(P) * 8;
+        const int chunk = tranche_size(runs, per);
+        upload_vars(vars);
+        if (base.variants == 1) tmpl_.upload(base, 0, 1, nullptr);
+        for (int first = 0; first < runs; first += chunk) {
+            const int count = std::min(chunk, runs - first);
+            if (base.variants != 1) tmpl_.upload(base, first, count, nullptr);
+            double* dx = x_.as<double>(size_t(count) * P);
+            hip_check(hipMemcpy(dx, x + size_t(first) * P, size_t(count) * P * 8,
+                                hipMemcpyHostToDevice), "x");
+            work_.allocate(base, count);
+            dev_check(mko_point_batch_device(
+                &tmpl_.view, base.variants != 1, count, 1, &dvars_, dx, nullptr, nullptr,
+                work_.mut(work_.view.curvature), work_.mut(work_.view.thickness),
+                work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
+                work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
+                work_.mut(work_.view.reference_curvature), nullptr));
+            merit_device(work_.view, s, nullptr, count);
+            copy_out(merit + first, d_merit_, size_t(count) * 8);
+            copy_out(failed + first, d_failed_, size_t(count) * 8);
+            copy_out(hard + first, d_hard_, size_t(count) * 8);
+        }
+        hip_check(hipDeviceSynchronize(), "point merit");
+    }
+
+    int device() const { return device_; }
+    KernelTimer timer;
+
+private:
+    int tranche_size(int items, size_t per_item) const {
+        if (items <= 0) return 1;
+        const size_t b = budget();
+        const size_t n = per_item ? b / per_item : size_t(items);
+        if (n == 0) throw DeviceError("too little device memory for one variant");
+        return int(std::min<size_t>(n, size_t(items)));
+    }
+    void copy_out(void* host, const void* dev, size_t bytes) {
+        hip_check(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost), "download");
+    }
+    void upload_vars(const mko_variables& v) {
+        int32_t* t = vtype_.as<int32_t>(size_t(v.count));
+        int32_t* s = vsurf_.as<int32_t>(size_t(v.count));
+        hip_check(hipMemcpy(t, v.type, size_t(v.count) * 4, hipMemcpyHostToDevice), "vars");
+        hip_check(hipMemcpy(s, v.batch_surface, size_t(v.count) * 4, hipMemcpyHostToDevice), "vars");
+        dvars_ = v;
+        dvars_.type = t;
+        dvars_.batch_surface = s;
+        vars_host_dev_ = v; // the adjoint entry point reads type/surface on the host
+    }
+
+    int device_;
+    size_t limit_;
+    DeviceBatch batch_, tmpl_, work_;
+    Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
+    Buffer x_, plus_, minus_, jac_, r0_, fdres_, fdst_, sp_, sm_, sten_, ch_, a_, g_, vtype_, vsurf_;
+    double* d_merit_ = nullptr;
+    int64_t* d_failed_ = nullptr;
+    int64_t* d_hard_ = nullptr;
+    mko_variables dvars_{};
+    mko_variables vars_host_dev_{};
+};
+
+// ---- batched Levenberg-Marquardt (src/optim/levenberg_marquardt.cpp:37-572) --------------------
+
+constexpr double kNaN = std::numeric_limits<double>::quiet_NaN();
+constexpr double kModelDamping = 1e-12;
+
+void parallel_for(int n, int threads, const std::function<void(int)>& fn) {
+    if (n <= 0) return;
+    if (threads <= 0) threads = int(std::max(1u, std::thread::hardware_concurrency()));
+    threads = std::min(threads, n);
+    if (threads <= 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::exception_ptr err;
+    std::mutex m;
+    for (int t = 0; t < threads; ++t) {
+        const int a = int(int64_t(n) * t / threads), b = int(int64_t(n) * (t + 1) / threads);
+        pool.emplace_back([&, a, b] {
+            try {
+                for (int i = a; i < b; ++i) fn(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> l(m);
+                if (!err) err = std::current_exception();
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    if (err) std::rethrow_exception(err);
+}
+
+struct Run {
+    std::vector<double> x, trial, a, g, d;
+    double merit = 0.0;
+    int64_t failed = 0, hard = 0;
+    double mu = 0.0, nu = 2.0;
+    int status = MKO_LM_RUNNING;
+    bool need_jacobian = true, rejected = false, solved = false;
+    double step_norm = 0.0, predicted = 0.0, gradient_norm = 0.0, model_decrease = kNaN;
+    int iterations = 0, accepted = 0, jacobians = 0;
+    std::vector<double> history;
+};
+
+inline double scale_of(double x, double typical) { return std::max(std::abs(x), typical); }
+
+// In-place Cholesky of the lower triangle of an n x n column-major matrix.
+bool cholesky(std::vector<double>& m, size_t n) {
+    for (size_t j = 0; j < n; ++j) {
+        double diag = m[j * n + j];
+        for (size_t k = 0; k < j; ++k) diag -= m[k * n + j] * m[k * n + j];
+        if (!(diag > 0.0) || !std::isfinite(diag)) return false;
+        const double l = std::sqrt(diag);
+        m[j * n + j] = l;
+        for (size_t i = j + 1; i < n; ++i) {
+            double s = m[j * n + i];
+            for (size_t k = 0; k < j; ++k) s -= m[k * n + i] * m[k * n + j];
+            m[j * n + i] = s / l;
+        }
+    }
+    return true;
+}
+
+void cholesky_solve(const std::vector<double>& l, size_t n, std::vector<double>& b) {
+    for (size_t i = 0; i < n; ++i) {
+        double s = b[i];
+        for (size_t k = 0; k < i; ++k) s -= l[k * n + i] * b[k];
+        b[i] = s / l[i * n + i];
+    }
+    for (size_t i = n; i-- > 0;) {
+        double s = b[i];
+        for (size_t k = i + 1; k < n; ++k) s -= l[i * n + k] * b[k];
+        b[i] = s / l[i * n + i];
+    }
+}
+
+struct Bounds {
+    const double *lower, *upper, *typical;
+    bool blocks(size_t k, double x, double dir) const {
+        if (lower[k] == upper[k]) return true;
+        return (x <= lower[k] && dir < 0.0) || (x >= upper[k] && dir > 0.0);
+    }
+};
+
+void prepare_step(Run& s, const Bounds& b, size_t n, const mko_lm_options& o) {
+    double smax = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double sk = scale_of(s.x[k], b.typical[k]);
+        smax = std::max(smax, s.a[k * n + k] * sk * sk);
+    }
+    double gmax = 0.0;
+    for (size_t k = 0; k < n; ++k) {
+        const double sk = scale_of(s.x[k], b.typical[k]);
+        const double akk = s.a[k * n + k];
+        const double floor = smax > 0.0 ? o.diag_floor * smax / (sk * sk) : 1.0 / (sk * sk);
+        s.d[k] = std::max(akk, floor);
+        if (!b.blocks(k, s.x[k], -s.g[k])) {
+            const double den = std::sqrt(akk * s.merit);
+            if (den > 0.0) gmax = std::max(gmax, std::abs(s.g[k]) / den);
+        }
+    }
+    s.gradient_norm = gmax;
+}
+
+// (A_FF + mu D_FF) delta_F = -g_F on the free variables with the active set (solve_step).
+void solve_step(Run& s, const Bounds& b, size_t n, double mu) {
+    std::vector<char> fixed(n);
+    for (size_t k = 0; k < n; ++k) fixed[k] = b.blocks(k, s.x[k], -s.g[k]);
+    std::vector<double> delta(n, 0.0), m, rhs;
+    std::vector<size_t> idx;
+    s.solved = true;
+    for (size_t round = 0; round <= n; ++round) {
+        idx.clear();
+        for (size_t k = 0; k < n; ++k)
+            if (!fixed[k]) idx.push_back(k);
+        const size_t f = idx.size();
+        std::fill(delta.begin(), delta.end(), 0.0);
+        if (f == 0) break;
+        m.assign(f * f, 0.0);
+        rhs.assign(f, 0.0);
+        for (size_t j = 0; j < f; ++j) {
+            for (size_t i = j; i < f; ++i) m[j * f + i] = s.a[idx[j] * n + idx[i]];
+            m[j * f + j] += mu * s.d[idx[j]];
+            rhs[j] = -s.g[idx[j]];
+        }
+        if (!cholesky(m, f)) {
+            s.solved = false;
+            return;
+        }
+        cholesky_solve(m, f, rhs);
+        for (size_t j = 0; j < f; ++j) delta[idx[j]] = rhs[j];
+        bool changed = false;
+        for (size_t k : idx)
+            if (b.blocks(k, s.x[k], delta[k])) {
+                fixed[k] = 1;
+                changed = true;
+            }
+        if (!changed) break;
+    }
+    double nrm = 0.0;
+    std::vector<double> step(n);
+    for (size_t k = 0; k < n; ++k) {
+        const double t = std::min(std::max(s.x[k] + delta[k], b.lower[k]), b.upper[k]);
+        s.trial[k] = t;
+        step[k] = t - s.x[k];
+        nrm = std::max(nrm, std::abs(step[k]) / scale_of(s.x[k], b.typical[k]));
+    }
+    s.step_norm = nrm;
+    double lin = 0.0, quad = 0.0;
+    for (size_t q = 0; q < n; ++q) {
+        lin += step[q] * s.g[q];
+        double aq = 0.0;
+        for (size_t p = 0; p < n; ++p) aq += s.a[q * n + p] * step[p];
+        quad += step[q] * aq;
+    }
+    s.predicted = -(2.0 * lin + quad);
+}
+
+class Lm {
+public:
+    Lm(Engine& eng, const mko_batch& base, const mko_residual_settings& s, const mko_variables& v,
+       int runs, const double* x0, const mko_lm_options& o, int precision)
+        : eng_(eng), base_(base), settings_(s), vars_(v), runs_(runs), o_(o), precision_(precision) {
+        check_batch(base);
+        if (runs < 1) throw std::invalid_argument("no runs");
+        if (base.variants != 1 && base.variants != runs)
+            throw std::invalid_argument("base must have one variant or one per run");
+        P_ = size_t(v.count);
+        if (P_ < 1) throw std::invalid_argument("no variables");
+        if (o.max_iterations < 0 || !(o.initial_damping > 0.0) || !(o.max_damping > o.initial_damping))
+            throw std::invalid_argument("invalid LM options");
+        if (o.jacobian_method != MKO_JACOBIAN_FD && o.jacobian_method != MKO_JACOBIAN_ADJOINT)
+            throw std::invalid_argument("unknown Jacobian method");
+        for (size_t k = 0; k < P_; ++k) {
+            if (std::isnan(v.lower[k]) || std::isnan(v.upper[k]) || v.lower[k] > v.upper[k])
+                throw std::invalid_argument("invalid bounds");
+            if (!(v.typical[k] > 0.0) || !std::isfinite(v.typical[k]))
+                throw std::invalid_argument("typical must be finite and > 0");
+        }
+        for (size_t i = 0; i < size_t(runs) * P_; ++i) {
+            const size_t k = i % P_;
+            if (!std::isfinite(x0[i]) || x0[i] < v.lower[k] || x0[i] > v.upper[k])
+                throw std::invalid_argument("start point not finite or outside the bounds");
+        }
+        bounds_ = Bounds{v.lower, v.upper, v.typical};
+        x0_.assign(x0, x0 + size_t(runs) * P_);
+        weight_.assign(size_t(runs), s.fail_weight);
+        rounds_.assign(size_t(runs), 1);
+        start_round(std::vector<int>(), x0_);
+    }
+
+    // One LM cycle (plus, when a round ends, the next penalty round). false when done.
+    bool cycle() {
+        if (done_) return false;
+        if (!round_cycle()) {
+            finish_round();
+            if (!next_round()) {
+                done_ = true;
+                return false;
+            }
+        }
+        return true;
+    }
+
+    void results(double* x, mko_lm_run* out, int* cycles) const {
+        const std::vector<Run>& src = (done_ || !ids_.empty()) ? final_ : state_;
+        for (int i = 0; i < runs_; ++i) {
+            const Run& r = src[size_t(i)];
+            std::copy(r.x.begin(), r.x.end(), x + size_t(i) * P_);
+            mko_lm_run& o = out[i];
+            o.merit = r.merit;
+            o.failed_rays = r.failed;
+            o.hard_failed_rays = r.hard;
+            o.status = r.status;
+            o.iterations = r.iterations;
+            o.accepted = r.accepted;
+            o.jacobians = r.jacobians;
+            o.damping = r.mu;
+            o.gradient_norm = r.gradient_norm;
+            o.model_decrease = r.model_decrease;
+            o.fail_weight = weight_[size_t(i)];
+            o.penalty_rounds = rounds_[size_t(i)];
+        }
+        if (cycles) *cycles = cycles_;
+    }
+
+private:
+    // ---- one pass of levenberg_marquardt over the runs `ids` (empty = all) ----
+    void start_round(const std::vector<int>& ids, const std::vector<double>& xs) {
+        ids_ = ids;
+        const int n = ids.empty() ? runs_ : int(ids.size());
+        state_.assign(size_t(n), Run());
+        std::vector<int> list;
+        for (int i = 0; i < n; ++i) {
+            Run& s = state_[size_t(i)];
+            s.x.assign(xs.begin() + std::ptrdiff_t(size_t(i) * P_), xs.begin() + std::ptrdiff_t(size_t(i + 1) * P_));
+            s.trial = s.x;
+            s.a.assign(P_ * P_, 0.0);
+            s.g.assign(P_, 0.0);
+            s.d.assign(P_, 0.0);
+            s.mu = o_.initial_damping;
+            list.push_back(i);
+        }
+        evaluate(list, false);
+        for (size_t j = 0; j < list.size(); ++j) {
+            Run& s = state_[size_t(list[j])];
+            s.merit = f_[j];
+            s.failed = fl_[j];
+            s.hard = hf_[j];
+            s.history.push_back(s.merit);
+            if (!std::isfinite(s.merit)) s.status = MKO_LM_NUMERICAL_ERROR;
+            else if (s.merit == 0.0) s.status = MKO_LM_CONVERGED_MERIT;
+        }
+    }
+
+    // template batch for the runs of this round: base (one variant) or the listed runs' variants
+    const mko_batch& template_for(const std::vector<int>& list, std::vector<int>& global) {
+        global.clear();
+        for (int i : list) global.push_back(ids_.empty() ? i : ids_[size_t(i)]);
+        if (base_.variants == 1) return base_;
+        gather(global);
+        return gathered_;
+    }
+
+    void gather(const std::vector<int>& global) {
+        const Strides st(base_);
+        const size_t n = global.size();
+        g_curv_.resize(n * st.S);
+        g_thick_.resize(n * st.S);
+        g_index_.resize(n * st.SL);
+        g_pupil_.resize(n);
+        g_fa_.resize(n * st.F);
+        g_ft_.resize(n * st.F);
+        g_rc_.resize(st.refc ? n * st.F : 0);
+        for (size_t j = 0; j < n; ++j) {
+            const size_t v = size_t(global[j]);
+            std::copy_n(base_.curvature + v * st.S, st.S, g_curv_.data() + j * st.S);
+            std::copy_n(base_.thickness + v * st.S, st.S, g_thick_.data() + j * st.S);
+            std::copy_n(base_.index + v * st.SL, st.SL, g_index_.data() + j * st.SL);
+            g_pupil_[j] = base_.pupil_radius[v];
+            std::copy_n(base_.field_angle + v * st.F, st.F, g_fa_.data() + j * st.F);
+            std::copy_n(base_.field_target + v * st.F, st.F, g_ft_.data() + j * st.F);
+            if (st.refc) std::copy_n(base_.reference_curvature + v * st.F, st.F, g_rc_.data() + j * st.F);
+        }
+        gathered_ = base_;
+        gathered_.variants = int(n);
+        gathered_.curvature = g_curv_.data();
+        gathered_.thickness = g_thick_.data();
+        gathered_.index = g_index_.data();
+        gathered_.pupil_radius = g_pupil_.data();
+        gathered_.field_angle = g_fa_.data();
+        gathered_.field_target = g_ft_.data();
+        gathered_.reference_curvature = st.refc ? g_rc_.data() : nullptr;
+    }
+
+    void gather_x(const std::vector<int>& list, bool trial) {
+        xs_.resize(list.size() * P_);
+        for (size_t j = 0; j < list.size(); ++j) {
+            const Run& s = state_[size_t(list[j])];
+            const std::vector<double>& src = trial ? s.trial : s.x;
+            std::copy(src.begin(), src.end(), xs_.begin() + std::ptrdiff_t(j * P_));
+        }
+    }
+
+    void evaluate(const std::vector<int>& list, bool trial) {
+        gather_x(list, trial);
+        f_.assign(list.size(), 0.0);
+        fl_.assign(list.size(), 0);
+        hf_.assign(list.size(), 0);
+        if (list.empty()) return;
+        std::vector<int> global;
+        const mko_batch& tmpl = template_for(list, global);
+        eng_.point_merit(tmpl, settings_, vars_, int(list.size()), xs_.data(), f_.data(),
+                         fl_.data(), hf_.data());
+    }
+
+    bool round_cycle() {
+        const int n = int(state_.size());
+        std::vector<int> list;
+        for (int i = 0; i < n; ++i) {
+            const Run& s = state_[size_t(i)];
+            if (s.status == MKO_LM_RUNNING && s.need_jacobian && s.iterations < o_.max_iterations)
+                list.push_back(i);
+        }
+        if (!list.empty()) {
+            gather_x(list, false);
+            a_.assign(list.size() * P_ * P_, 0.0);
+            g_.assign(list.size() * P_, 0.0);
+            std::vector<int> global;
+            const mko_batch& tmpl = template_for(list, global);
+            eng_.jacobian(tmpl, settings_, vars_, int(list.size()), xs_.data(), o_.jacobian_method,
+                          precision_, nullptr, nullptr, a_.data(), g_.data());
+            for (size_t j = 0; j < list.size(); ++j) {
+                Run& s = state_[size_t(list[j])];
+                std::copy_n(a_.data() + j * P_ * P_, P_ * P_, s.a.begin());
+                std::copy_n(g_.data() + j * P_, P_, s.g.begin());
+                s.need_jacobian = false;
+                ++s.jacobians;
+                bool finite = true;
+                for (double v : s.a) finite = finite && std::isfinite(v);
+                for (double v : s.g) finite = finite && std::isfinite(v);
+                if (!finite) {
+                    s.status = MKO_LM_NUMERICAL_ERROR;
+                    continue;
+                }
+                prepare_step(s, bounds_, P_, o_);
+            }
+            parallel_for(int(list.size()), o_.threads, [&](int j) {
+                Run& s = state_[size_t(list[size_t(j)])];
+                if (s.status == MKO_LM_RUNNING) solve_step(s, bounds_, P_, kModelDamping);
+            });
+            for (int i : list) {
+                Run& s = state_[size_t(i)];
+                if (s.status != MKO_LM_RUNNING) continue;
+                s.model_decrease = s.solved ? s.predicted / s.merit : kNaN;
+                if (s.gradient_norm <= o_.gtol) s.status = MKO_LM_CONVERGED_GRADIENT;
+                else if (s.solved && s.predicted >= 0.0 && s.predicted <= o_.ftol * s.merit)
+                    s.status = MKO_LM_CONVERGED_MERIT;
+                else if (s.solved && s.step_norm <= o_.xtol) s.status = MKO_LM_CONVERGED_STEP;
+            }
+        }
+        list.clear();
+        for (int i = 0; i < n; ++i) {
+            Run& s = state_[size_t(i)];
+            if (s.status != MKO_LM_RUNNING) continue;
+            if (s.iterations >= o_.max_iterations) {
+                s.status = MKO_LM_MAX_ITERATIONS;
+                continue;
+            }
+            list.push_back(i);
+        }
+        if (list.empty()) return false;
+        ++cycles_;
+        parallel_for(int(list.size()), o_.threads, [&](int j) {
+            Run& s = state_[size_t(list[size_t(j)])];
+            solve_step(s, bounds_, P_, s.mu);
+        });
+        std::vector<int> cycle_runs = list, eval;
+        for (int i : cycle_runs) {
+            Run& s = state_[size_t(i)];
+            if (s.solved && s.rejected && s.step_norm <= o_.xtol) s.status = MKO_LM_CONVERGED_STEP;
+            else if (s.solved) eval.push_back(i);
+        }
+        evaluate(eval, true);
+        size_t e = 0;
+        for (int i : cycle_runs) {
+            Run& s = state_[size_t(i)];
+            if (s.status == MKO_LM_CONVERGED_STEP) continue;
+            ++s.iterations;
+            bool accept = false;
+            if (s.solved) {
+                const double f_new = f_[e];
+                const int64_t failed_new = fl_[e], hard_new = hf_[e];
+                ++e;
+                const double rho = (s.merit - f_new) / s.predicted;
+                accept = std::isfinite(f_new) && s.predicted > 0.0 && f_new < s.merit &&
+                         !(o_.reject_new_failures && hard_new > s.hard);
+                if (accept) {
+                    s.x = s.trial;
+                    s.merit = f_new;
+                    s.failed = failed_new;
+                    s.hard = hard_new;
+                    const double t = 2.0 * rho - 1.0;
+                    s.mu *= std::max(1.0 / 3.0, 1.0 - t * t * t);
+                    s.nu = 2.0;
+                    s.need_jacobian = true;
+                    s.rejected = false;
+                    ++s.accepted;
+                    if (f_new == 0.0) s.status = MKO_LM_CONVERGED_MERIT;
+                }
+            }
+            if (!accept) {
+                s.mu *= s.nu;
+                s.nu *= 2.0;
+                s.rejected = true;
+                if (s.mu > o_.max_damping) s.status = MKO_LM_DAMPING_LIMIT;
+            }
+            s.history.push_back(s.merit);
+            if (o_.stall_window > 0 && s.status == MKO_LM_RUNNING) {
+                const size_t w = size_t(o_.stall_window);
+                if (s.history.size() > w) {
+                    const double old = s.history[s.history.size() - 1 - w];
+                    if (old - s.merit <= o_.stall_tolerance * old) s.status = MKO_LM_CONVERGED_STALL;
+                }
+            }
+        }
+        return true;
+    }
+
+    void finish_round() {
+        if (ids_.empty()) {
+            final_ = state_;
+            return;
+        }
+        for (size_t j = 0; j < ids_.size(); ++j) {
+            Run& r = final_[size_t(ids_[j])];
+            const Run& q = state_[j];
+            r.x = q.x;
+            r.merit = q.merit;
+            r.failed = q.failed;
+            r.hard = q.hard;
+            r.status = q.status;
+            r.iterations += q.iterations;
+            r.accepted += q.accepted;
+            r.jacobians += q.jacobians;
+            r.mu = q.mu;
+            r.gradient_norm = q.gradient_norm;
+            r.model_decrease = q.model_decrease;
+            weight_[size_t(ids_[j])] = settings_.fail_weight;
+            ++rounds_[size_t(ids_[j])];
+        }
+    }
+
+    // outer penalty rounds (optimize_lm_rounds): restart runs with failed rays with a larger
+    // fail weight while it stays below max_fail_weight
+    bool next_round() {
+        if (settings_.failure != MKO_FAILURE_SOFT || !(o_.fail_weight_factor > 1.0)) return false;
+        double w = settings_.fail_weight;
+        if (!(w > 0.0)) return false;
+        w *= o_.fail_weight_factor;
+        if (!(w <= o_.max_fail_weight)) return false;
+        std::vector<int> again;
+        std::vector<double> xs;
+        for (int i = 0; i < runs_; ++i) {
+            const Run& r = final_[size_t(i)];
+            if (r.failed > 0 && r.status != MKO_LM_NUMERICAL_ERROR) {
+                again.push_back(i);
+                xs.insert(xs.end(), r.x.begin(), r.x.end());
+            }
+        }
+        if (again.empty()) return false;
+        settings_.fail_weight = w;
+        start_round(again, xs);
+        return true;
+    }
+
+    Engine& eng_;
+    mko_batch base_;
+    mko_residual_settings settings_;
+    mko_variables vars_;
+    int runs_;
+    mko_lm_options o_;
+    int precision_;
+    size_t P_ = 0;
+    Bounds bounds_{};
+    std::vector<double> x0_;
+    std::vector<int> ids_;
+    std::vector<Run> state_, final_;
+    std::vector<double> weight_;
+    std::vector<int> rounds_;
+    int cycles_ = 0;
+    bool done_ = false;
+    std::vector<double> xs_, a_, g_, f_;
+    std::vector<int64_t> fl_, hf_;
+    mko_batch gathered_{};
+    std::vector<double> g_curv_, g_thick_, g_index_, g_pupil_, g_fa_, g_ft_, g_rc_;
+};
+
+} // namespace mko_host
+
+using namespace mko_host;
+
+struct mko_engine {
+    Engine engine;
+    mko_engine(int dev, size_t lim) : engine(dev, lim) {}
+};
+struct mko_lm {
+    Lm lm;
+    // the LM keeps pointers into the caller's batch and variable arrays; they must stay valid
+};
+
+#define MKO_HOST_TRY(...)                                                                          \
+    try {                                                                                          \
+        __VA_ARGS__;                                                                               \
+    } catch (const std::invalid_argument& e) {                                                     \
+        mko_detail::last_error = e.what();                                                         \
+        return MKO_ERR_INVALID;                                                                    \
+    } catch (const std::exception& e) {                                                            \
+        mko_detail::last_error = e.what();                                                         \
+        return MKO_ERR_DEVICE;                                                                     \
+    }
+
+extern "C" {
+
+mko_engine* mko_engine_create(int device, size_t memory_limit_bytes) {
+    try {
+        return new mko_engine(device, memory_limit_bytes);
+    } catch (const std::exception& e) {
+        mko_detail::last_error = e.what();
+        return nullptr;
+    }
+}
+
+void mko_engine_destroy(mko_engine* e) { delete e; }
+
+int mko_engine_kernel_timing(mko_engine* e, int enable, double* ms, int64_t* launches) {
+    MKO_HOST_TRY({
+        KernelTimer& t = e->engine.timer;
+        if (ms)
+            for (int i = 0; i < kFamilies; ++i) ms[i] = t.ms[i];
+        if (launches)
+            for (int i = 0; i < kFamilies; ++i) launches[i] = t.launches[i];
+        t.reset();
+        t.enabled = enable != 0;
+        return 0;
+    })
+}
+
+int mko_engine_residuals(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
+                         double* res, double* opd, uint8_t* st) {
+    MKO_HOST_TRY(e->engine.residuals(*b, *s, res, opd, st); return 0;)
+}
+
+int mko_engine_merit(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
+                     const double* extra, double* merit, int64_t* failed, int64_t* hard) {
+    MKO_HOST_TRY(e->engine.merit(*b, *s, extra, merit, failed, hard); return 0;)
+}
+
+int mko_engine_jacobian(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
+                        const mko_variables* v, int runs, const double* x0, int method,
+                        int precision, double* J, double* r0, double* a, double* g) {
+    MKO_HOST_TRY(e->engine.jacobian(*base, *s, *v, runs, x0, method, precision, J, r0, a, g);
+                 return 0;)
+}
+
+mko_lm* mko_lm_create(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
+                      const mko_variables* v, int runs, const double* x0,
+                      const mko_lm_options* o, int precision) {
+    try {
+        return new mko_lm{Lm(e->engine, *base, *s, *v, runs, x0, *o, precision)};
+    } catch (const std::exception& ex) {
+        mko_detail::last_error = ex.what();
+        return nullptr;
+    }
+}
+
+int mko_lm_cycle(mko_lm* lm) { MKO_HOST_TRY(return lm->lm.cycle() ? 1 : 0;) }
+
+int mko_lm_results(mko_lm* lm, double* x, mko_lm_run* out, int* cycles) {
+    MKO_HOST_TRY(lm->lm.results(x, out, cycles); return 0;)
+}
+
+void mko_lm_destroy(mko_lm* lm) { delete lm; }
+
+int mko_optimize_lm(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
+                    const mko_variables* v, int runs, const double* x0, const mko_lm_options* o,
+                    int precision, double* x_out, mko_lm_run* run_out, int* cycles_out) {
+    MKO_HOST_TRY({
+        Lm lm(e->engine, *base, *s, *v, runs, x0, *o, precision);
+        while (lm.cycle()) {
+        }
+        lm.results(x_out, run_out, cycles_out);
+        return 0;
+    })
+}
+
+} // extern "C"

@@ -1,0 +1,948 @@
+// gfx950 kernels of the batched forward-and-Jacobian path and their device-level C entry points.
+//
+// Work decomposition (all kernels): one workgroup of 256 lanes (4 wave64) per (variant, tuple),
+// tuple = (field f, wavelength l), t = f * L + l; the tuple's rays sit on lanes, ray k of the
+// tuple on lane k mod 256 (iteration k / 256). The surface table of the (variant, wavelength)
+// is staged once per workgroup in LDS (curvature, vertex distance, index, mu = n/n'), so the
+// per-ray surface loop reads wave-uniform LDS broadcasts. Every reduction over rays (piston,
+// merit partials, piston projection of the Jacobian) is a fixed-order tree: lane-local sum in
+// iteration order, xor-butterfly inside the wave, waves summed in index order; results depend
+// only on the tuple, never on the batch composition or the launch.
+//
+// Reference counterparts: residuals_kernel src/cuda/residuals.cu:58, merit_kernel +
+// merit_finalize_kernel src/cuda/merit.cu:63/207, fd_difference_kernel src/cuda/fd_jacobian.cu:65,
+// adjoint_jacobian_kernel src/cuda/adjoint_jacobian.cu:119, gram_kernel
+// src/cuda/normal_equations.cu:48.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "error.hpp"
+#include "gauss_legendre_table.h"
+#include "mikroptim_hip.h"
+#include "trace_device.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxFields = 32;
+constexpr int kMaxWavelengths = 32;
+constexpr int kMaxRaysPerLane = 8; // tuples of at most 2048 rays
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+using mkod::Ray;
+using mkod::Soft;
+using mkod::V3;
+
+// Everything a trace kernel needs, passed by value (< 1.5 KB of kernel arguments).
+struct TraceParams {
+    const double* curvature;
+    const double* thickness;
+    const double* index;
+    const double* pupil_radius;
+    const double* field_angle;
+    const double* field_target;
+    const double* reference_curvature;
+    int V, S, L, F, T;
+    int source, ref_kind;
+    int rings, spokes, soft;
+    double fail_weight;
+    double lambda_ref_nm;
+    double src_pt[3];
+    double src_na;
+    uint64_t sphere_mask;
+    int64_t rays, trace_rows, stride;
+    double gl_u[MKO_MAX_RINGS], gl_w[MKO_MAX_RINGS];
+    double field_weight[kMaxFields];
+    double wavelength_weight[kMaxWavelengths];
+    uint8_t pupil_mode[kMaxFields];
+};
+
+// Adjoint column map: variable index of curvature[s] / thickness[s] (-1 if not a variable).
+struct ParamMap {
+    int P;
+    int8_t curv_col[MKO_MAX_SURFACES];
+    int8_t thick_col[MKO_MAX_SURFACES];
+};
+
+// ---- tuple geometry ------------------------------------------------------------------------
+
+struct Tuple {
+    int v, f, l, n, M;
+    int64_t offset;
+    double scale, pscale;
+};
+
+__device__ __forceinline__ Tuple tuple_of(const TraceParams& P, int block) {
+    Tuple t;
+    t.v = block / P.T;
+    const int tt = block - t.v * P.T;
+    t.f = tt / P.L;
+    t.l = tt - t.f * P.L;
+    int64_t off = 0;
+    for (int f = 0; f < t.f; ++f)
+        off += int64_t(P.L) * P.rings * (P.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : P.spokes);
+    t.M = P.pupil_mode[t.f] == MKO_PUPIL_AXIS ? 1 : P.spokes;
+    t.n = P.rings * t.M;
+    t.offset = off + int64_t(t.l) * t.n;
+    // tuple_weight / residual_scale / penalty_scale (ray_residual.hpp:301-385): no a*b+c,
+    // correctly rounded division and square root, hence bitwise equal to the host values.
+    const double wf = P.field_weight[t.f];
+    const double wl = P.wavelength_weight[t.l];
+    double w = 0.0;
+    if (wf > 0.0 && wl > 0.0) {
+        double sf = 0.0, sl = 0.0;
+        for (int i = 0; i < P.F; ++i) sf += P.field_weight[i];
+        for (int i = 0; i < P.L; ++i) sl += P.wavelength_weight[i];
+        w = __dmul_rn(__ddiv_rn(wf, sf), __ddiv_rn(wl, sl));
+    }
+    t.scale = w > 0.0 ? __ddiv_rn(__dsqrt_rn(w), __dmul_rn(P.lambda_ref_nm, 1e-6)) : 0.0;
+    t.pscale = w > 0.0 ? __dsqrt_rn(__dmul_rn(P.fail_weight, w)) : 0.0;
+    return t;
+}
+
+struct Sample {
+    double rho, sphi, cphi, weight, rootw;
+};
+
+// PupilQuadrature::sample (quadrature.hpp:71-92): ring-major index k = i * M + j.
+__device__ __forceinline__ Sample sample_of(const TraceParams& P, const Tuple& t, int k) {
+    const int i = k / t.M;
+    const int j = k - i * t.M;
+    Sample s;
+    s.rho = __dsqrt_rn(P.gl_u[i]);
+    s.weight = __ddiv_rn(P.gl_w[i], double(t.M));
+    s.rootw = __dsqrt_rn(s.weight);
+    const uint8_t mode = P.pupil_mode[t.f];
+    double phi = 0.0;
+    if (mode == MKO_PUPIL_HALF)
+        phi = __ddiv_rn(__dmul_rn(double(2 * j + 1), kPi), double(2 * t.M));
+    else if (mode == MKO_PUPIL_FULL)
+        phi = __ddiv_rn(__dmul_rn(double(2 * j + 1), kPi), double(t.M));
+    if (mode == MKO_PUPIL_AXIS) {
+        s.sphi = 0.0;
+        s.cphi = 1.0;
+    } else {
+        sincos(phi, &s.sphi, &s.cphi);
+    }
+    return s;
+}
+
+// Surface table of (variant, wavelength) in LDS.
+struct SurfLds {
+    double c[MKO_MAX_SURFACES];  // effective curvature (0 for stop/plane)
+    double t[MKO_MAX_SURFACES];  // thickness[s]
+    double n[MKO_MAX_SURFACES];  // index[s][l]
+    double mu[MKO_MAX_SURFACES]; // n[s-1] / n[s]
+};
+
+__device__ __forceinline__ void stage_surfaces(const TraceParams& P, const Tuple& t, SurfLds& L) {
+    const int s = threadIdx.x;
+    if (s < P.S) {
+        const size_t vs = size_t(t.v) * P.S + s;
+        const bool sphere = (P.sphere_mask >> s) & 1ull;
+        L.c[s] = sphere ? P.curvature[vs] : 0.0;
+        L.t[s] = P.thickness[vs];
+        L.n[s] = P.index[vs * P.L + t.l];
+    }
+    __syncthreads();
+    if (s >= 1 && s < P.S) L.mu[s] = __ddiv_rn(L.n[s - 1], L.n[s]);
+    __syncthreads();
+}
+
+__device__ __forceinline__ Ray source_ray(const TraceParams& P, const Tuple& t, const Sample& s,
+                                          double a, double sin_t, double cos_t, double n0) {
+    Ray r;
+    if (P.source == MKO_SOURCE_POINT) {
+        const double sin_max = P.src_na / n0;
+        const double sr = sin_max * s.rho;
+        r.p = V3<double>{P.src_pt[0], P.src_pt[1], P.src_pt[2]};
+        r.d = V3<double>{sr * s.sphi, sr * s.cphi, __dsqrt_rn((1.0 - sr) * (1.0 + sr))};
+        r.opl = 0.0;
+        return r;
+    }
+    const double rad = a * s.rho;
+    r.p = V3<double>{rad * s.sphi, rad * s.cphi, 0.0};
+    r.d = V3<double>{0.0, sin_t, cos_t};
+    r.opl = mkod::dot(r.p, r.d);
+    return r;
+}
+
+// Record of one ray's states for the reverse sweep (AdjointRecord, adjoint_record.hpp:66).
+template <class RR>
+struct Record {
+    RR p[MKO_MAX_SURFACES][3];
+    RR d[MKO_MAX_SURFACES][3];
+    uint8_t branch[MKO_MAX_SURFACES];
+    uint8_t image_branch;
+    int count;
+    __device__ __forceinline__ void put(int s, const Ray& r, uint8_t b) {
+        p[s][0] = RR(r.p.x);
+        p[s][1] = RR(r.p.y);
+        p[s][2] = RR(r.p.z);
+        d[s][0] = RR(r.d.x);
+        d[s][1] = RR(r.d.y);
+        d[s][2] = RR(r.d.z);
+        branch[s] = b;
+        count = s + 1;
+    }
+};
+
+struct NoRecord {
+    __device__ __forceinline__ void put(int, const Ray&, uint8_t) {}
+    uint8_t image_branch;
+};
+
+// trace_to_reference_sphere (reference_sphere.hpp:183-222) over the LDS table.
+// Returns the final status; soft.penalty holds the accumulated penalty.
+template <bool kSoft, class Rec>
+__device__ __forceinline__ uint8_t trace_ray(const TraceParams& P, const SurfLds& L, Ray& r,
+                                             V3<double> M, double kappa, Soft& soft, Rec& rec) {
+    rec.put(0, r, mkod::kBranchRegular);
+    for (int s = 1; s < P.S; ++s) {
+        uint8_t branch = 0;
+        const uint8_t st = mkod::surface_step<kSoft>(r, L.c[s], L.t[s - 1], L.n[s - 1], L.mu[s],
+                                                     soft, branch);
+        if (st != MKO_STATUS_OK) return st;
+        rec.put(s, r, branch);
+    }
+    if (!(r.d.z > 0.0)) return MKO_STATUS_BACKWARD;
+    uint8_t rb = 0;
+    const uint8_t rst = mkod::reference_step<kSoft>(r, M, kappa, P.ref_kind, L.n[P.S - 1], soft, rb);
+    if (!mkod::in_piston(rst)) return rst;
+    rec.image_branch = rb;
+    return kSoft ? soft.first : uint8_t(MKO_STATUS_OK);
+}
+
+// ---- deterministic block reductions -----------------------------------------------------------
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T block_sum(T v, T* lds) {
+    v = wave_sum(v);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[wave] = v;
+    __syncthreads();
+    T total = lds[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) total += lds[w];
+    __syncthreads();
+    return total;
+}
+
+// ---- RESIDUALS / MERIT ----------------------------------------------------------------------
+
+// Mode 0: RESIDUALS (residual rows, penalty rows, OPD, status). Mode 1: MERIT partials per
+// (variant, tuple): sum r^2 + sum p^2, failed and hard-failed counts.
+template <int kMode, bool kSoft, int kRpl>
+__global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* residuals,
+                                                       double* opd, uint8_t* status,
+                                                       double* partial, int* pfailed,
+                                                       int* phard) {
+    __shared__ SurfLds L;
+    __shared__ double red[kWaves];
+    __shared__ int ired[kWaves];
+    const Tuple t = tuple_of(P, blockIdx.x);
+    stage_surfaces(P, t, L);
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const double a = P.pupil_radius[t.v];
+    double sin_t, cos_t;
+    sincos(P.field_angle[vf], &sin_t, &cos_t);
+    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+
+    double opl[kRpl], pen[kRpl];
+    uint8_t st[kRpl];
+    double sw = 0.0, swo = 0.0;
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+        const int k = threadIdx.x + i * kBlock;
+        st[i] = MKO_STATUS_UNSUPPORTED;
+        opl[i] = 0.0;
+        pen[i] = 0.0;
+        if (k < t.n) {
+            const Sample s = sample_of(P, t, k);
+            Ray r = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
+            Soft soft{0.0, MKO_STATUS_OK};
+            NoRecord nr;
+            st[i] = trace_ray<kSoft>(P, L, r, M, kappa, soft, nr);
+            opl[i] = r.opl;
+            pen[i] = soft.penalty;
+            if (mkod::in_piston(st[i])) {
+                sw += s.weight;
+                swo += s.weight * r.opl;
+            }
+        }
+    }
+    const double w_ok = block_sum(sw, red);
+    const double sum = block_sum(swo, red);
+    const bool valid = w_ok > 0.0;
+    const double piston = valid ? sum / w_ok : 0.0;
+    double r2 = 0.0, p2 = 0.0;
+    int nfail = 0, nhard = 0;
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+        const int k = threadIdx.x + i * kBlock;
+        if (k < t.n) {
+            const Sample s = sample_of(P, t, k);
+            const bool ok = valid && mkod::in_piston(st[i]);
+            const double e = ok ? opl[i] - piston : 0.0;
+            const double r = t.scale * s.rootw * e;
+            double pr = 0.0;
+            if (kSoft) {
+                const double fp = st[i] == MKO_STATUS_OK ? 0.0
+                                  : mkod::is_soft(st[i]) ? pen[i]
+                                                         : pen[i] + 1.0;
+                pr = t.pscale * s.rootw * fp;
+            }
+            const int64_t row = t.offset + k;
+            if (kMode == 0) {
+                const size_t base = size_t(t.v) * P.stride;
+                residuals[base + row] = r;
+                if (kSoft) residuals[base + P.rays + row] = pr;
+                if (opd) opd[size_t(t.v) * P.rays + row] = e;
+                if (status) status[size_t(t.v) * P.rays + row] = st[i];
+            } else {
+                r2 += r * r;
+                p2 += pr * pr;
+                nfail += st[i] != MKO_STATUS_OK;
+                nhard += !mkod::in_piston(st[i]);
+            }
+        }
+    }
+    if (kMode == 1) {
+        const double tr2 = block_sum(r2, red);
+        const double tp2 = kSoft ? block_sum(p2, red) : 0.0;
+        const int tf = block_sum(nfail, ired);
+        const int th = block_sum(nhard, ired);
+        if (threadIdx.x == 0) {
+            const size_t idx = size_t(t.v) * P.T + size_t(blockIdx.x - t.v * P.T);
+            partial[idx] = kSoft ? tr2 + tp2 : tr2;
+            pfailed[idx] = tf;
+            phard[idx] = th;
+        }
+    }
+}
+
+// merit[v] = sum over tuples in (f, l) order + extra[v] (merit_finalize_kernel, merit.cu:207)
+__global__ void merit_finalize_kernel(int V, int T, const double* partial, const int* pfailed,
+                                      const int* phard, const double* extra, double* merit,
+                                      int64_t* failed, int64_t* hard) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double s = 0.0;
+    int64_t f = 0, h = 0;
+    for (int t = 0; t < T; ++t) {
+        s += partial[size_t(v) * T + t];
+        f += pfailed[size_t(v) * T + t];
+        h += phard[size_t(v) * T + t];
+    }
+    merit[v] = extra ? s + extra[v] : s;
+    failed[v] = f;
+    hard[v] = h;
+}
+
+// ---- FD batch and FD difference -------------------------------------------------------------
+
+// Builds the 2P+1 FD variants of each run from its template variant: variant 0 = x0,
+// 2k+1 = x0 + d+_k e_k, 2k+2 = x0 + d-_k e_k (write_fd_variants, variant_generator.cpp:240).
+// The template's per-variant arrays are copied, then the variables written.
+struct BatchPtrs {
+    double* curvature;
+    double* thickness;
+    double* index;
+    double* pupil_radius;
+    double* field_angle;
+    double* field_target;
+    double* reference_curvature;
+};
+struct ConstBatchPtrs {
+    const double* curvature;
+    const double* thickness;
+    const double* index;
+    const double* pupil_radius;
+    const double* field_angle;
+    const double* field_target;
+    const double* reference_curvature;
+};
+
+__global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, int per_run,
+                                   int template_per_run, int S, int L, int F, int P,
+                                   const int* var_type, const int* var_surface, const double* x0,
+                                   const double* plus, const double* minus) {
+    const int vv = blockIdx.x; // destination variant
+    const int run = vv / per_run;
+    const int j = vv - run * per_run;
+    if (run >= runs) return;
+    const int from = template_per_run ? run : 0;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        dst.curvature[size_t(vv) * S + i] = src.curvature[size_t(from) * S + i];
+        dst.thickness[size_t(vv) * S + i] = src.thickness[size_t(from) * S + i];
+    }
+    for (int i = threadIdx.x; i < S * L; i += blockDim.x)
+        dst.index[size_t(vv) * S * L + i] = src.index[size_t(from) * S * L + i];
+    for (int i = threadIdx.x; i < F; i += blockDim.x) {
+        dst.field_angle[size_t(vv) * F + i] = src.field_angle[size_t(from) * F + i];
+        dst.field_target[size_t(vv) * F + i] = src.field_target[size_t(from) * F + i];
+        if (src.reference_curvature)
+            dst.reference_curvature[size_t(vv) * F + i] = src.reference_curvature[size_t(from) * F + i];
+    }
+    if (threadIdx.x == 0) dst.pupil_radius[vv] = src.pupil_radius[from];
+    __syncthreads();
+    // variables: x0 of the run, component k replaced for FD variants
+    const int kk = j == 0 ? -1 : (j - 1) / 2;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) {
+        double x = x0[size_t(run) * P + k];
+        if (k == kk) x = (j & 1) ? plus[size_t(run) * P + k] : minus[size_t(run) * P + k];
+        const int s = var_surface[k];
+        if (var_type[k] == MKO_VAR_CURVATURE)
+            dst.curvature[size_t(vv) * S + s] = x;
+        else
+            dst.thickness[size_t(vv) * S + s] = x;
+    }
+}
+
+// J[i][k][r] = fd_derivative (fd_formula.hpp:39) from the FD residuals; block (i, P) copies r0.
+// Status changes per (i, k): rays whose Ok-ness differs between x0 and either point.
+__global__ __launch_bounds__(kBlock) void fd_difference_kernel(
+    const double* res, int64_t stride, const uint8_t* st, int64_t rays, int runs, int P,
+    int64_t rows, const double* step_plus, const double* step_minus, const uint8_t* stencil,
+    double* J, double* r0, int64_t* changes) {
+    __shared__ int ired[kWaves];
+    const int i = blockIdx.x / (P + 1);
+    const int k = blockIdx.x - i * (P + 1);
+    const int n = 2 * P + 1;
+    const size_t v0 = size_t(i) * n;
+    const double* r_0 = res + v0 * stride;
+    if (k == P) {
+        for (int64_t r = threadIdx.x; r < rows; r += kBlock) r0[size_t(i) * rows + r] = r_0[r];
+        return;
+    }
+    const double* ra = res + (v0 + 2 * k + 1) * stride;
+    const double* rb = res + (v0 + 2 * k + 2) * stride;
+    const double da = step_plus[size_t(i) * P + k];
+    const double db = step_minus[size_t(i) * P + k];
+    const uint8_t sten = stencil[size_t(i) * P + k];
+    double* col = J + (size_t(i) * P + k) * rows;
+    for (int64_t r = threadIdx.x; r < rows; r += kBlock) {
+        const double a0 = r_0[r], aa = ra[r], ab = rb[r];
+        double v;
+        if (sten == MKO_STENCIL_ONE_SIDED) {
+            const double ga = aa - a0, gb = ab - a0;
+            v = (db * db * ga - da * da * gb) / (da * db * (db - da));
+        } else {
+            v = (aa - ab) / (da - db);
+        }
+        col[r] = v;
+    }
+    int c = 0;
+    const uint8_t* s0 = st + v0 * rays;
+    const uint8_t* sa = st + (v0 + 2 * k + 1) * rays;
+    const uint8_t* sb = st + (v0 + 2 * k + 2) * rays;
+    for (int64_t r = threadIdx.x; r < rays; r += kBlock) {
+        const bool ok0 = s0[r] == MKO_STATUS_OK;
+        c += ((sa[r] == MKO_STATUS_OK) != ok0) || ((sb[r] == MKO_STATUS_OK) != ok0);
+    }
+    const int total = block_sum(c, ired);
+    if (threadIdx.x == 0) changes[size_t(i) * P + k] = total;
+}
+
+// ---- ADJOINT Jacobian -----------------------------------------------------------------------
+
+// Writes one seed's per-ray parameter adjoints into J rows: value * factor at column p, row.
+template <class TJ>
+struct RowSink {
+    TJ* col0;     // J of the variant
+    int64_t rows; // R
+    int64_t row;
+    double factor;
+    __device__ __forceinline__ void write(int p, double value) const {
+        if (p >= 0) col0[int64_t(p) * rows + row] = TJ(factor * value);
+    }
+};
+
+// backward_ray (adjoint_backward.hpp:496-580) over the record, reverse arithmetic in RB.
+template <class RB, class RR, class TJ>
+__device__ __forceinline__ void backward_ray(const TraceParams& P, const SurfLds& L,
+                                             const ParamMap& map, const Record<RR>& rec,
+                                             bool reached_image, V3<double> M, double kappa,
+                                             RB seed_opl, RB seed_pen, const RowSink<TJ>& sink) {
+    const int S = P.S;
+    const int count = rec.count;
+    mkod::RayAdj<RB> adj{{RB(0), RB(0), RB(0)}, {RB(0), RB(0), RB(0)}};
+    uint8_t next_branch = 0;
+    for (int s = count - 1; s >= 0; --s) {
+        const V3<RB> p{RB(rec.p[s][0]), RB(rec.p[s][1]), RB(rec.p[s][2])};
+        const V3<RB> d{RB(rec.d[s][0]), RB(rec.d[s][1]), RB(rec.d[s][2])};
+        if (s == count - 1) {
+            if (reached_image && count == S) {
+                RB nib = RB(0);
+                const RB mz = mkod::reference_step_backward<RB>(
+                    p, d, V3<RB>{RB(M.x), RB(M.y), RB(M.z)}, RB(kappa), P.ref_kind,
+                    RB(L.n[S - 1]), rec.image_branch, seed_opl, seed_pen, adj, nib);
+                sink.write(map.thick_col[S - 1], double(mz));
+            }
+        } else {
+            const int j = s + 1;
+            RB tb = RB(0), cb = RB(0), nbb = RB(0), nab = RB(0);
+            mkod::refracting_step_backward<RB>(p, d, RB(L.t[s]), RB(L.c[j]), RB(L.n[s]),
+                                               RB(L.n[j]), next_branch, seed_opl, seed_pen, adj,
+                                               tb, cb, nbb, nab);
+            if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], double(cb));
+            sink.write(map.thick_col[s], double(tb));
+        }
+        next_branch = rec.branch[s];
+    }
+}
+
+// One workgroup per (variant = run, tuple). Raw per-ray gradients go straight into the
+// column-major J of the variant (J[v][p][row]); after the tuple's piston reduction the
+// workgroup projects its rows in place: J(k,p) = s sqrt(W_k) (g_k - gbar_p) for rays in the
+// piston, 0 otherwise (adjoint_jacobian.cpp:462-479). Penalty rows (soft mode) get
+// pscale sqrt(W_k) times the penalty gradient. r0 rows and the status are written as by
+// RESIDUALS.
+template <bool kSoft, int kRpl, class RB, class RR, class TJ>
+__global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap map, TJ* J,
+                                                         double* r0, uint8_t* status) {
+    __shared__ SurfLds L;
+    __shared__ double red[kWaves];
+    const Tuple t = tuple_of(P, blockIdx.x);
+    stage_surfaces(P, t, L);
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const double a = P.pupil_radius[t.v];
+    double sin_t, cos_t;
+    sincos(P.field_angle[vf], &sin_t, &cos_t);
+    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+    const int64_t R = P.stride;
+    TJ* col0 = J + size_t(t.v) * map.P * R;
+
+    double opl[kRpl];
+    uint8_t st[kRpl];
+    double pen[kRpl];
+    double sw = 0.0, swo = 0.0;
+    Record<RR> rec;
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+        const int k = threadIdx.x + i * kBlock;
+        st[i] = MKO_STATUS_UNSUPPORTED;
+        opl[i] = 0.0;
+        pen[i] = 0.0;
+        if (k >= t.n) continue;
+        const int64_t row = t.offset + k;
+        for (int p = 0; p < map.P; ++p) {
+            col0[int64_t(p) * R + row] = TJ(0);
+            if (kSoft) col0[int64_t(p) * R + P.rays + row] = TJ(0);
+        }
+        const Sample s = sample_of(P, t, k);
+        Ray r = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
+        Soft soft{0.0, MKO_STATUS_OK};
+        rec.count = 0;
+        rec.image_branch = 0;
+        st[i] = trace_ray<kSoft>(P, L, r, M, kappa, soft, rec);
+        opl[i] = r.opl;
+        pen[i] = soft.penalty;
+        const bool reached = mkod::in_piston(st[i]);
+        if (reached) {
+            sw += s.weight;
+            swo += s.weight * r.opl;
+            backward_ray<RB, RR, TJ>(P, L, map, rec, true, M, kappa, RB(1), RB(0),
+                                     RowSink<TJ>{col0, R, row, 1.0});
+        }
+        if (kSoft && soft.first != MKO_STATUS_OK) {
+            backward_ray<RB, RR, TJ>(P, L, map, rec, reached, M, kappa, RB(0), RB(1),
+                                     RowSink<TJ>{col0, R, P.rays + row, t.pscale * s.rootw});
+        }
+    }
+    const double w_ok = block_sum(sw, red);
+    const double sum = block_sum(swo, red);
+    const bool valid = w_ok > 0.0;
+    const double piston = valid ? sum / w_ok : 0.0;
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+        const int k = threadIdx.x + i * kBlock;
+        if (k >= t.n) continue;
+        const Sample s = sample_of(P, t, k);
+        const bool ok = valid && mkod::in_piston(st[i]);
+        const double e = ok ? opl[i] - piston : 0.0;
+        const int64_t row = t.offset + k;
+        r0[size_t(t.v) * R + row] = t.scale * s.rootw * e;
+        if (kSoft) {
+            const double fp = st[i] == MKO_STATUS_OK ? 0.0
+                              : mkod::is_soft(st[i]) ? pen[i]
+                                                     : pen[i] + 1.0;
+            r0[size_t(t.v) * R + P.rays + row] = t.pscale * s.rootw * fp;
+        }
+        if (status) status[size_t(t.v) * P.rays + row] = st[i];
+    }
+    __threadfence_block();
+    __syncthreads();
+    // piston projection, one column at a time (sums in FP64, fixed order)
+    for (int p = 0; p < map.P; ++p) {
+        TJ* col = col0 + int64_t(p) * R + t.offset;
+        double part = 0.0;
+#pragma unroll
+        for (int i = 0; i < kRpl; ++i) {
+            const int k = threadIdx.x + i * kBlock;
+            if (k < t.n && mkod::in_piston(st[i])) part += sample_of(P, t, k).weight * double(col[k]);
+        }
+        const double mean = valid ? block_sum(part, red) / w_ok : 0.0;
+#pragma unroll
+        for (int i = 0; i < kRpl; ++i) {
+            const int k = threadIdx.x + i * kBlock;
+            if (k >= t.n) continue;
+            const Sample s = sample_of(P, t, k);
+            const bool ok = valid && mkod::in_piston(st[i]);
+            col[k] = ok ? TJ(t.scale * s.rootw * (double(col[k]) - mean)) : TJ(0);
+        }
+    }
+}
+
+// ---- Gram: A = [J r0]^T [J r0] per run -----------------------------------------------------
+
+// 16 x 16 output tile per workgroup over the upper triangle of the (P+1) x (P+1) matrix;
+// rows staged through LDS in chunks of 64. Each element is one FMA chain over the rows in
+// ascending order starting from 0 (gram_dot, normal_equations.hpp:19), products of the stored
+// J values formed in double: for double J this is bitwise the host definition.
+template <class TJ>
+__global__ __launch_bounds__(256) void gram_kernel(const TJ* J, const double* r0, int runs, int P,
+                                                   int64_t rows, double* a, double* g) {
+    constexpr int kT = 16, kC = 64;
+    __shared__ double As[kT][kC + 1];
+    __shared__ double Bs[kT][kC + 1];
+    const int T = (P + 1 + kT - 1) / kT;
+    const int tiles = T * (T + 1) / 2;
+    const int run = blockIdx.x / tiles;
+    int tile = blockIdx.x - run * tiles;
+    int tq = 0;
+    while (tile >= T - tq) {
+        tile -= T - tq;
+        ++tq;
+    }
+    const int tp = tq + tile; // tp >= tq
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int p = tp * kT + tx; // row of A tile (column of J)
+    const int q = tq * kT + ty;
+    const TJ* Jr = J + size_t(run) * P * rows;
+    const double* rr = r0 + size_t(run) * rows;
+    double acc = 0.0;
+    for (int64_t r0c = 0; r0c < rows; r0c += kC) {
+        for (int e = threadIdx.x; e < kT * kC; e += 256) {
+            const int cc = e / kC, rw = e - cc * kC;
+            const int64_t row = r0c + rw;
+            const int pc = tp * kT + cc, qc = tq * kT + cc;
+            double va = 0.0, vb = 0.0;
+            if (row < rows) {
+                va = pc < P ? double(Jr[size_t(pc) * rows + row]) : pc == P ? rr[row] : 0.0;
+                vb = qc < P ? double(Jr[size_t(qc) * rows + row]) : qc == P ? rr[row] : 0.0;
+            }
+            As[cc][rw] = va;
+            Bs[cc][rw] = vb;
+        }
+        __syncthreads();
+        const int n = int(rows - r0c < kC ? rows - r0c : kC);
+        for (int rw = 0; rw < n; ++rw) acc = __fma_rn(As[tx][rw], Bs[ty][rw], acc);
+        __syncthreads();
+    }
+    if (p <= P && q <= P && q <= p) {
+        // element (p, q) with q <= p; mirror into A, column P is g
+        if (p < P && q < P) {
+            a[size_t(run) * P * P + size_t(q) * P + p] = acc;
+            a[size_t(run) * P * P + size_t(p) * P + q] = acc;
+        } else if (p == P && q < P) {
+            g[size_t(run) * P + q] = acc;
+        }
+    }
+}
+
+// ---- helpers ---------------------------------------------------------------------------------
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+TraceParams make_params(const mko_batch& b, const mko_residual_settings& s) {
+    if (b.surfaces < 2 || b.surfaces > MKO_MAX_SURFACES)
+        throw std::invalid_argument("surfaces outside [2, 64]");
+    if (b.fields < 1 || b.fields > kMaxFields)
+        throw std::invalid_argument("fields outside [1, 32]");
+    if (b.wavelengths < 1 || b.wavelengths > kMaxWavelengths)
+        throw std::invalid_argument("wavelengths outside [1, 32]");
+    if (s.rings < 1 || s.rings > MKO_MAX_RINGS) throw std::invalid_argument("rings outside [1, 16]");
+    if (s.spokes < 1) throw std::invalid_argument("spokes < 1");
+    if (s.failure != MKO_FAILURE_SOFT && s.failure != MKO_FAILURE_HARD)
+        throw std::invalid_argument("unknown failure mode");
+    if (!(s.fail_weight >= 0.0) || !std::isfinite(s.fail_weight))
+        throw std::invalid_argument("fail_weight must be finite and >= 0");
+    if (b.reference_sphere != MKO_REF_LAST_SURFACE && b.reference_curvature == nullptr)
+        throw std::invalid_argument("image reference sphere needs reference_curvature");
+    if (b.source == MKO_SOURCE_POINT && b.fields != 1)
+        throw std::invalid_argument("point source needs exactly one field");
+    TraceParams P{};
+    P.curvature = b.curvature;
+    P.thickness = b.thickness;
+    P.index = b.index;
+    P.pupil_radius = b.pupil_radius;
+    P.field_angle = b.field_angle;
+    P.field_target = b.field_target;
+    P.reference_curvature = b.reference_curvature;
+    P.V = b.variants;
+    P.S = b.surfaces;
+    P.L = b.wavelengths;
+    P.F = b.fields;
+    P.T = b.fields * b.wavelengths;
+    P.source = b.source;
+    P.ref_kind = b.reference_sphere;
+    P.rings = s.rings;
+    P.spokes = s.spokes;
+    P.soft = s.failure == MKO_FAILURE_SOFT;
+    P.fail_weight = s.fail_weight;
+    P.lambda_ref_nm = b.reference_wavelength_nm;
+    for (int i = 0; i < 3; ++i) P.src_pt[i] = b.source_point[i];
+    P.src_na = b.source_na;
+    P.sphere_mask = 0;
+    for (int i = 0; i < b.surfaces; ++i) {
+        if (b.type[i] > MKO_SURFACE_SPHERE)
+            throw std::invalid_argument("unsupported surface type (ghost plates are not part of this build)");
+        if (b.type[i] == MKO_SURFACE_SPHERE) P.sphere_mask |= 1ull << i;
+    }
+    int64_t rays = 0;
+    int nmax = 0;
+    for (int f = 0; f < b.fields; ++f) {
+        if (b.pupil_mode[f] > MKO_PUPIL_AXIS) throw std::invalid_argument("unknown pupil mode");
+        P.pupil_mode[f] = b.pupil_mode[f];
+        P.field_weight[f] = b.field_weight[f];
+        const int n = s.rings * (b.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : s.spokes);
+        nmax = std::max(nmax, n);
+        rays += int64_t(b.wavelengths) * n;
+    }
+    if (nmax > kBlock * kMaxRaysPerLane)
+        throw std::invalid_argument("more than 2048 pupil samples per tuple");
+    for (int l = 0; l < b.wavelengths; ++l) P.wavelength_weight[l] = b.wavelength_weight[l];
+    const int off = s.rings * (s.rings - 1) / 2;
+    for (int i = 0; i < s.rings; ++i) {
+        P.gl_u[i] = mko_gl_u[off + i];
+        P.gl_w[i] = mko_gl_w[off + i];
+    }
+    P.rays = rays;
+    P.trace_rows = P.soft ? 2 * rays : rays;
+    P.stride = P.trace_rows;
+    return P;
+}
+
+int rays_per_lane(const TraceParams& P) {
+    int nmax = 0;
+    for (int f = 0; f < P.F; ++f)
+        nmax = std::max(nmax, P.rings * (P.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : P.spokes));
+    const int r = (nmax + kBlock - 1) / kBlock;
+    return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : 8;
+}
+
+template <int kMode, bool kSoft>
+void launch_trace_rpl(int rpl, const TraceParams& P, double* res, double* opd, uint8_t* st,
+                      double* part, int* pf, int* ph, hipStream_t stream) {
+    const dim3 grid(unsigned(P.V) * unsigned(P.T));
+#define MKO_LAUNCH(R)                                                                            \
+    hipLaunchKernelGGL((trace_kernel<kMode, kSoft, R>), grid, dim3(kBlock), 0, stream, P, res,    \
+                       opd, st, part, pf, ph)
+    switch (rpl) {
+    case 1: MKO_LAUNCH(1); break;
+    case 2: MKO_LAUNCH(2); break;
+    case 4: MKO_LAUNCH(4); break;
+    default: MKO_LAUNCH(8); break;
+    }
+#undef MKO_LAUNCH
+}
+
+void launch_trace(int mode, const TraceParams& P, double* res, double* opd, uint8_t* st,
+                  double* part, int* pf, int* ph, hipStream_t stream) {
+    if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large for one launch");
+    const int rpl = rays_per_lane(P);
+    if (mode == 0) {
+        if (P.soft) launch_trace_rpl<0, true>(rpl, P, res, opd, st, part, pf, ph, stream);
+        else launch_trace_rpl<0, false>(rpl, P, res, opd, st, part, pf, ph, stream);
+    } else {
+        if (P.soft) launch_trace_rpl<1, true>(rpl, P, res, opd, st, part, pf, ph, stream);
+        else launch_trace_rpl<1, false>(rpl, P, res, opd, st, part, pf, ph, stream);
+    }
+    check_hip(hipGetLastError(), "trace kernel launch");
+}
+
+ParamMap make_map(const mko_variables& vars, int S) {
+    ParamMap m{};
+    m.P = vars.count;
+    if (vars.count < 1 || vars.count > 127) throw std::invalid_argument("1 ... 127 variables");
+    for (int s = 0; s < MKO_MAX_SURFACES; ++s) {
+        m.curv_col[s] = -1;
+        m.thick_col[s] = -1;
+    }
+    for (int k = 0; k < vars.count; ++k) {
+        const int s = vars.batch_surface[k];
+        if (s < 0 || s >= S) throw std::invalid_argument("variable surface out of range");
+        int8_t& slot = vars.type[k] == MKO_VAR_CURVATURE ? m.curv_col[s] : m.thick_col[s];
+        if (vars.type[k] < MKO_VAR_CURVATURE || vars.type[k] > MKO_VAR_WORKING_DISTANCE)
+            throw std::invalid_argument("unsupported variable type");
+        if (slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
+        slot = int8_t(k);
+    }
+    return m;
+}
+
+template <bool kSoft, class RB, class RR, class TJ>
+void launch_adjoint_rpl(int rpl, const TraceParams& P, const ParamMap& m, TJ* J, double* r0,
+                        uint8_t* st, hipStream_t stream) {
+    const dim3 grid(unsigned(P.V) * unsigned(P.T));
+#define MKO_LAUNCH(R)                                                                            \
+    hipLaunchKernelGGL((adjoint_kernel<kSoft, R, RB, RR, TJ>), grid, dim3(kBlock), 0, stream, P,  \
+                       m, J, r0, st)
+    switch (rpl) {
+    case 1: MKO_LAUNCH(1); break;
+    case 2: MKO_LAUNCH(2); break;
+    case 4: MKO_LAUNCH(4); break;
+    default: MKO_LAUNCH(8); break;
+    }
+#undef MKO_LAUNCH
+}
+
+#define MKO_TRY(...)                                                                               \
+    try {                                                                                          \
+        __VA_ARGS__;                                                                               \
+        return 0;                                                                                  \
+    } catch (const std::invalid_argument& e) {                                                     \
+        mko_detail::last_error = e.what();                                                                        \
+        return MKO_ERR_INVALID;                                                                    \
+    } catch (const std::exception& e) {                                                            \
+        mko_detail::last_error = e.what();                                                                        \
+        return MKO_ERR_DEVICE;                                                                     \
+    }
+
+} // namespace
+
+extern "C" {
+
+const char* mko_last_error(void) { return mko_detail::last_error.c_str(); }
+
+int mko_residuals_device(const mko_batch* batch, const mko_residual_settings* settings,
+                         double* residuals, int64_t stride, double* opd, uint8_t* status,
+                         void* stream) {
+    MKO_TRY({
+        TraceParams P = make_params(*batch, *settings);
+        if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
+        P.stride = stride;
+        if (P.V == 0) return 0;
+        launch_trace(0, P, residuals, opd, status, nullptr, nullptr, nullptr,
+                     static_cast<hipStream_t>(stream));
+    })
+}
+
+int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settings,
+                     const double* extra_merit, double* merit, int64_t* failed,
+                     int64_t* hard_failed, double* partial, int* partial_counts, void* stream) {
+    MKO_TRY({
+        const TraceParams P = make_params(*batch, *settings);
+        if (P.V == 0) return 0;
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        int* pf = partial_counts;
+        int* ph = partial_counts + size_t(P.V) * P.T;
+        launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, s);
+        hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
+                           P.T, partial, pf, ph, extra_merit, merit, failed, hard_failed);
+        check_hip(hipGetLastError(), "merit finalize launch");
+    })
+}
+
+int mko_point_batch_device(const mko_batch* templ, int template_per_run, int runs, int per_run,
+                           const mko_variables* vars_device, const double* x0,
+                           const double* plus, const double* minus, double* curvature,
+                           double* thickness, double* index, double* pupil_radius,
+                           double* field_angle, double* field_target,
+                           double* reference_curvature, void* stream) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        const ConstBatchPtrs src{templ->curvature, templ->thickness, templ->index,
+                                 templ->pupil_radius, templ->field_angle, templ->field_target,
+                                 templ->reference_curvature};
+        const BatchPtrs dst{curvature, thickness, index, pupil_radius, field_angle, field_target,
+                            reference_curvature};
+        hipLaunchKernelGGL(point_batch_kernel, dim3(unsigned(runs) * unsigned(per_run)), dim3(64),
+                           0, static_cast<hipStream_t>(stream), src, dst, runs, per_run,
+                           template_per_run, templ->surfaces, templ->wavelengths, templ->fields,
+                           vars_device->count, vars_device->type, vars_device->batch_surface, x0,
+                           plus, minus);
+        check_hip(hipGetLastError(), "point batch launch");
+    })
+}
+
+int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
+                             const uint8_t* fd_status, int64_t rays, int runs, int variables,
+                             int64_t rows, const double* step_plus, const double* step_minus,
+                             const uint8_t* stencil, double* jacobian, double* r0,
+                             int64_t* status_changes, void* stream) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        hipLaunchKernelGGL(fd_difference_kernel, dim3(unsigned(runs) * unsigned(variables + 1)),
+                           dim3(kBlock), 0, static_cast<hipStream_t>(stream), fd_residuals,
+                           stride, fd_status, rays, runs, variables, rows, step_plus, step_minus,
+                           stencil, jacobian, r0, status_changes);
+        check_hip(hipGetLastError(), "fd difference launch");
+    })
+}
+
+int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_settings* settings,
+                                const mko_variables* vars, int precision, void* jacobian,
+                                double* r0, uint8_t* status, void* stream) {
+    MKO_TRY({
+        const TraceParams P = make_params(*batch, *settings);
+        const ParamMap m = make_map(*vars, batch->surfaces);
+        if (P.V == 0) return 0;
+        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
+        const int rpl = rays_per_lane(P);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (precision == MKO_PRECISION_FP64) {
+            double* Jd = static_cast<double*>(jacobian);
+            if (P.soft) launch_adjoint_rpl<true, double, double, double>(rpl, P, m, Jd, r0, status, s);
+            else launch_adjoint_rpl<false, double, double, double>(rpl, P, m, Jd, r0, status, s);
+        } else if (precision == MKO_PRECISION_MIXED) {
+            float* Jf = static_cast<float*>(jacobian);
+            if (P.soft) launch_adjoint_rpl<true, float, float, float>(rpl, P, m, Jf, r0, status, s);
+            else launch_adjoint_rpl<false, float, float, float>(rpl, P, m, Jf, r0, status, s);
+        } else {
+            throw std::invalid_argument("unknown precision");
+        }
+        check_hip(hipGetLastError(), "adjoint launch");
+    })
+}
+
+int mko_gram_device(const void* jacobian, const double* r0, int runs, int variables,
+                    int64_t rows, int precision, double* a, double* g, void* stream) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        const int T = (variables + 1 + 15) / 16;
+        const dim3 grid(unsigned(runs) * unsigned(T * (T + 1) / 2));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (precision == MKO_PRECISION_FP64)
+            hipLaunchKernelGGL(gram_kernel<double>, grid, dim3(256), 0, s,
+                               static_cast<const double*>(jacobian), r0, runs, variables, rows, a, g);
+        else
+            hipLaunchKernelGGL(gram_kernel<float>, grid, dim3(256), 0, s,
+                               static_cast<const float*>(jacobian), r0, runs, variables, rows, a, g);
+        check_hip(hipGetLastError(), "gram launch");
+    })
+}
+
+} // extern "C"

@@ -1,0 +1,226 @@
+"""Python face of libmikroptim_hip.so, shaped like the reference's Python module
+(python/src/bind_trace.cpp:411-623 TraceEngine / SystemBatch, python/src/bind_optimize.cpp:476
+optimize): TraceEngine.residuals / merit (TraceMode RESIDUALS / MERIT), jacobian (FD or adjoint
+Jacobian with normal equations) and optimize (batched Levenberg-Marquardt on the GPU backend).
+
+The HIP library is required: importing works without it (so CPU-only tooling can load the
+package), but every call raises if libmikroptim_hip.so is missing or has no device."""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .batch import rays_per_variant, trace_rows
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmikroptim_hip.so")
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+# every symbol include/mikroptim_hip.h declares
+EXPORTS = [
+    "mko_last_error", "mko_residuals_device", "mko_merit_device", "mko_point_batch_device",
+    "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_gram_device",
+    "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
+    "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
+    "mko_lm_results", "mko_lm_destroy",
+]
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def library():
+    """The loaded libmikroptim_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipError(f"{LIB_PATH} is missing: build it first (python -c "
+                       "'import __graft_entry__; __graft_entry__.build()')")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 (same SONAME); loading torch
+    # first makes this library bind to that runtime, so torch tensors and this library share the
+    # device context (the torch.autograd.Function path depends on it).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = C.CDLL(LIB_PATH)
+    bp, sp, vp = C.POINTER(abi.Batch), C.POINTER(abi.ResidualSettings), C.POINTER(abi.Variables)
+    sig = {
+        "mko_last_error": ([], C.c_char_p),
+        "mko_residuals_device": ([bp, sp, _dp, C.c_int64, _dp, C.POINTER(C.c_uint8), C.c_void_p],
+                                 C.c_int),
+        "mko_merit_device": ([bp, sp, _dp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp,
+                              C.POINTER(C.c_int), C.c_void_p], C.c_int),
+        "mko_gram_device": ([C.c_void_p, _dp, C.c_int, C.c_int, C.c_int64, C.c_int, _dp, _dp,
+                             C.c_void_p], C.c_int),
+        "mko_adjoint_jacobian_device": ([bp, sp, vp, C.c_int, C.c_void_p, _dp,
+                                         C.POINTER(C.c_uint8), C.c_void_p], C.c_int),
+        "mko_engine_create": ([C.c_int, C.c_size_t], C.c_void_p),
+        "mko_engine_destroy": ([C.c_void_p], None),
+        "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
+        "mko_engine_merit": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_int64),
+                              C.POINTER(C.c_int64)], C.c_int),
+        "mko_engine_jacobian": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
+                                 _dp, _dp], C.c_int),
+        "mko_engine_kernel_timing": ([C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
+        "mko_optimize_lm": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions),
+                             C.c_int, _dp, C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
+        "mko_lm_create": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions),
+                           C.c_int], C.c_void_p),
+        "mko_lm_cycle": ([C.c_void_p], C.c_int),
+        "mko_lm_results": ([C.c_void_p, _dp, C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
+        "mko_lm_destroy": ([C.c_void_p], None),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HipError(library().mko_last_error().decode())
+
+
+_PRECISION = {"fp64": abi.PRECISION_FP64, "mixed": abi.PRECISION_MIXED}
+_METHOD = {"fd": abi.JACOBIAN_FD, "adjoint": abi.JACOBIAN_ADJOINT}
+
+
+class LmResult:
+    """Per-run results (LmRunResult) and the cycle count of the whole call."""
+
+    def __init__(self, x, runs, cycles):
+        self.x = x
+        self.runs = runs
+        self.cycles = cycles
+        self.merit = np.array([r.merit for r in runs])
+        self.status = [abi.LM_STATUS_NAMES[r.status] for r in runs]
+        self.iterations = np.array([r.iterations for r in runs])
+
+
+class LmSession:
+    """One batched LM, advanced one cycle per call of cycle()."""
+
+    def __init__(self, engine, base, settings, variables, x0, options, precision):
+        self._keep = (base.c_struct(), settings.c_struct(), variables.c_struct(),
+                      np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64), options)
+        b, s, v, x, o = self._keep
+        self.runs, self.P = x.shape
+        self._h = library().mko_lm_create(engine._h, C.byref(b), C.byref(s), C.byref(v), self.runs,
+                                          abi.ptr(x), C.byref(o), _PRECISION[precision])
+        if not self._h:
+            raise HipError(library().mko_last_error().decode())
+
+    def cycle(self):
+        rc = library().mko_lm_cycle(self._h)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def results(self):
+        x = np.zeros((self.runs, self.P))
+        res = (abi.LmRun * self.runs)()
+        cyc = C.c_int(0)
+        _check(library().mko_lm_results(self._h, abi.ptr(x), res, C.byref(cyc)))
+        return LmResult(x, list(res), cyc.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mko_lm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class TraceEngine:
+    """TraceEngine(device, memory_limit_bytes) of include/mikroptim/cuda/trace_engine.hpp."""
+
+    def __init__(self, device=0, memory_limit_bytes=0):
+        self._h = library().mko_engine_create(device, memory_limit_bytes)
+        if not self._h:
+            raise HipError(library().mko_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            library().mko_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    KERNEL_FAMILIES = ("trace", "adjoint", "gram", "fd_difference", "point_batch")
+
+    def kernel_timing(self, enable=True):
+        """Device ms and launch counts per kernel family since the last call (HIP events on the
+        launch stream); then resets and enables/disables timing."""
+        ms = np.zeros(5)
+        n = np.zeros(5, dtype=np.int64)
+        _check(library().mko_engine_kernel_timing(self._h, int(enable), abi.ptr(ms),
+                                                  abi.ptr(n, C.c_int64)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_FAMILIES)}
+
+    def residuals(self, batch, settings):
+        V = batch.variants
+        R = trace_rows(batch, settings)
+        rays = rays_per_variant(batch, settings)
+        res = np.zeros((V, R))
+        opd = np.zeros((V, rays))
+        st = np.zeros((V, rays), dtype=np.uint8)
+        b, s = batch.c_struct(), settings.c_struct()
+        _check(library().mko_engine_residuals(self._h, C.byref(b), C.byref(s), abi.ptr(res),
+                                              abi.ptr(opd), abi.ptr(st, C.c_uint8)))
+        return res, opd, st
+
+    def merit(self, batch, settings, extra=None):
+        V = batch.variants
+        m = np.zeros(V)
+        f = np.zeros(V, dtype=np.int64)
+        h = np.zeros(V, dtype=np.int64)
+        ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float64)
+        b, s = batch.c_struct(), settings.c_struct()
+        _check(library().mko_engine_merit(self._h, C.byref(b), C.byref(s), abi.ptr(ex),
+                                          abi.ptr(m), abi.ptr(f, C.c_int64),
+                                          abi.ptr(h, C.c_int64)))
+        return m, f, h
+
+    def jacobian(self, base, settings, variables, x0, method="adjoint", precision="fp64",
+                 want_jacobian=True):
+        """(J [runs][P][R] or None, r0 [runs][R], A [runs][P][P], g [runs][P])."""
+        x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
+        runs, P = x0.shape
+        R = trace_rows(base, settings)
+        J = np.zeros((runs, P, R)) if want_jacobian else None
+        r0 = np.zeros((runs, R))
+        a = np.zeros((runs, P, P))
+        g = np.zeros((runs, P))
+        b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
+        _check(library().mko_engine_jacobian(self._h, C.byref(b), C.byref(s), C.byref(v), runs,
+                                             abi.ptr(x0), _METHOD[method], _PRECISION[precision],
+                                             abi.ptr(J), abi.ptr(r0), abi.ptr(a), abi.ptr(g)))
+        return J, r0, a, g
+
+    def optimize(self, base, settings, variables, x0, options=None, precision="fp64"):
+        options = options or abi.lm_options()
+        x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
+        runs, P = x0.shape
+        x = np.zeros((runs, P))
+        res = (abi.LmRun * runs)()
+        cyc = C.c_int(0)
+        b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
+        _check(library().mko_optimize_lm(self._h, C.byref(b), C.byref(s), C.byref(v), runs,
+                                         abi.ptr(x0), C.byref(options), _PRECISION[precision],
+                                         abi.ptr(x), res, C.byref(cyc)))
+        return LmResult(x, list(res), cyc.value)
+
+    def lm(self, base, settings, variables, x0, options=None, precision="fp64"):
+        return LmSession(self, base, settings, variables, x0, options or abi.lm_options(),
+                         precision)
