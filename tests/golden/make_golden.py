@@ -1,0 +1,46 @@
+"""Writes tests/golden/reference_outputs.npz: outputs of the reference's own CPU code
+(oracle/_ref/libmkoref.so, built from the reference sources by oracle/Makefile) on small seeded
+inputs of this repository's workloads, so the oracle stays pinned where the reference build is
+absent. Run from the repository root: python tests/golden/make_golden.py"""
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from mikroptim_amd import abi, systems  # noqa: E402
+from oracle_lib import reference  # noqa: E402
+
+
+def main():
+    ref = reference()
+    if ref is None:
+        raise SystemExit("the reference build oracle/_ref/libmkoref.so is missing")
+    out = {}
+    b, v = systems.singlet(4, seed=7)
+    for name, fail in (("soft", abi.FAILURE_SOFT), ("hard", abi.FAILURE_HARD)):
+        s = systems.singlet_settings(fail)
+        res, opd, st = ref.residuals(b, s)
+        m, f, h = ref.merit(b, s)
+        x0 = v.extract(b)[:2]
+        J, r0, ch = ref.fd_jacobian(b.take([0]), s, v, x0)
+        Ja, r0a, sta = ref.adjoint_jacobian(b.take([0]), s, v, x0)
+        out.update({f"singlet_{name}_residuals": res, f"singlet_{name}_opd": opd,
+                    f"singlet_{name}_status": st, f"singlet_{name}_merit": m,
+                    f"singlet_{name}_fd_J": J, f"singlet_{name}_fd_r0": r0,
+                    f"singlet_{name}_adj_J": Ja, f"singlet_{name}_x0": x0})
+    ob, ov = systems.objective(variants=2, seed=2)
+    s = systems.objective_settings(abi.FAILURE_HARD)
+    s.rings, s.spokes = 3, 4
+    res, opd, st = ref.residuals(ob, s)
+    out.update({"objective_coarse_residuals": res, "objective_coarse_status": st})
+    np.savez_compressed(os.path.join(ROOT, "tests", "golden", "reference_outputs.npz"), **out)
+    print("wrote", sorted(out))
+
+
+if __name__ == "__main__":
+    main()
