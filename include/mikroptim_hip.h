@@ -66,20 +66,35 @@ int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
                              const uint8_t* stencil, double* jacobian, double* r0,
                              int64_t* status_changes, void* stream);
 
-/* ADJOINT Jacobian of a batch with one variant per run: J [V][P][trace_rows] (double for
- * MKO_PRECISION_FP64, float for MKO_PRECISION_MIXED), r0 [V][trace_rows] (double), status
- * [V][rays] (may be NULL). Replaces adjoint_jacobian_kernel + DeviceAdjointJacobian::compute
- * (src/cuda/adjoint_jacobian.cu:119, :401). */
+/* ADJOINT Jacobian of a batch with one variant per run: J [V][P][trace_rows], r0 [V][trace_rows],
+ * status [V][rays] (may be NULL), all FP64. precision MKO_PRECISION_FP64 or MKO_PRECISION_MIXED;
+ * both compute the stored Jacobian in FP64 (the mixed precision of this library is the MFMA
+ * normal equations of mko_adjoint_normal_device). Replaces adjoint_jacobian_kernel +
+ * DeviceAdjointJacobian::compute (src/cuda/adjoint_jacobian.cu:119, :401). */
 int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_settings* settings,
-                                const mko_variables* vars, int precision, void* jacobian,
+                                const mko_variables* vars, int precision, double* jacobian,
                                 double* r0, uint8_t* status, void* stream);
 
+/* ADJOINT Jacobian fused with the normal equations, the mixed-precision path of the LM: per run
+ * (one variant per run) a [V][P][P] = J^T J and g [V][P] = J^T r0 without storing J. Forward
+ * trace and reverse sweep in FP64; per-ray gradients staged in LDS as float differences to a
+ * per-wave pilot ray; J^T J from split-bf16 products (hi*hi + hi*lo + lo*hi) on MFMA with float
+ * accumulation per (field, wavelength) tuple and FP64 across tuples; J^T r0 in FP64. Hard failure
+ * mode, at most 512 rays per tuple and 64 variables. Workspace: mko_adjoint_normal_workspace()
+ * bytes of device memory. Replaces adjoint_jacobian_kernel + gram_kernel
+ * (src/cuda/adjoint_jacobian.cu:119, src/cuda/normal_equations.cu:48) on the LM path, where only
+ * J^T J and J^T r0 are consumed (src/optim/levenberg_marquardt.cpp:427-447). */
+size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, int variables);
+int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_settings* settings,
+                              const mko_variables* vars, double* a, double* g, void* workspace,
+                              size_t workspace_bytes, void* stream);
+
 /* Normal equations of each run: a [runs][P][P] = J^T J (a[q*P + p]), g [runs][P] = J^T r0.
- * Every element is one FMA chain over the rows in ascending order (for FP64 J bitwise equal to
+ * Every element is one FMA chain over the rows in ascending order (bitwise equal to
  * normal_equations_host). Replaces gram_kernel + DeviceNormalEquations::compute
  * (src/cuda/normal_equations.cu:48, :155). */
-int mko_gram_device(const void* jacobian, const double* r0, int runs, int variables,
-                    int64_t rows, int precision, double* a, double* g, void* stream);
+int mko_gram_device(const double* jacobian, const double* r0, int runs, int variables,
+                    int64_t rows, double* a, double* g, void* stream);
 
 /* ---- host layer ----------------------------------------------------------------------------- */
 
@@ -110,7 +125,10 @@ int mko_engine_merit(mko_engine* engine, const mko_batch* batch,
 /* DeviceFdJacobian::compute / DeviceAdjointJacobian::compute + download
  * (include/mikroptim/cuda/fd_jacobian.hpp:58, adjoint_jacobian.hpp:70): base has one variant
  * (shared template) or one per run; x0 [runs][P]. Outputs (host, any may be NULL):
- * J [runs][P][trace_rows] double, r0 [runs][trace_rows], a [runs][P][P], g [runs][P]. */
+ * J [runs][P][trace_rows] double, r0 [runs][trace_rows], a [runs][P][P], g [runs][P].
+ * method ADJOINT with precision MKO_PRECISION_MIXED, J and r0 NULL, hard failure mode, <= 64
+ * variables and <= 512 rays per tuple takes a and g from mko_adjoint_normal_device; every other
+ * combination computes J in FP64 and a, g with mko_gram_device. */
 int mko_engine_jacobian(mko_engine* engine, const mko_batch* base,
                         const mko_residual_settings* settings, const mko_variables* vars,
                         int runs, const double* x0, int method, int precision, double* jacobian,

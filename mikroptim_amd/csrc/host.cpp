@@ -382,11 +382,19 @@ public:
         const int P = vars.count;
         const int64_t R = trace_rows_of(base, s), rays = rays_of(base, s);
         const bool fd = method == MKO_JACOBIAN_FD;
+        // The LM consumes only J^T J and J^T r0: for the mixed-precision adjoint those come from
+        // the fused kernel and J is never formed (hard failure mode, <= 64 variables, <= 512
+        // rays per tuple).
+        const bool fused = !fd && precision == MKO_PRECISION_MIXED && !J && !r0 && (a || g) &&
+                           s.failure == MKO_FAILURE_HARD && P <= 64 &&
+                           int64_t(s.rings) * s.spokes <= 2 * 256;
         const int per_run = fd ? 2 * P + 1 : 1;
-        const size_t jbytes = (fd || precision == MKO_PRECISION_FP64) ? 8 : 4;
+        const size_t jbytes = 8;
+        const size_t T = size_t(base.fields) * base.wavelengths;
         const size_t per = size_t(per_run) * (Strides(base).doubles() * 8 +
                                               (fd ? size_t(R) * 8 + size_t(rays) : 0)) +
-                           size_t(P) * R * jbytes + size_t(R) * 8 + size_t(rays) +
+                           (fused ? T * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
+                                  : size_t(P) * R * jbytes + size_t(R) * 8 + size_t(rays)) +
                            size_t(P) * (P + 1) * 8 + 64 * size_t(P);
         const int chunk = tranche_size(runs, per);
         upload_vars(vars);
@@ -425,6 +433,21 @@ public:
                 work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
                 work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
                 work_.mut(work_.view.reference_curvature), nullptr));
+            if (fused) {
+                double* da = a_.as<double>(size_t(count) * P * P);
+                double* dg = g_.as<double>(size_t(count) * P);
+                const size_t wbytes =
+                    mko_adjoint_normal_workspace(count, base.fields, base.wavelengths, P);
+                void* ws = jac_.reserve(wbytes);
+                timer.run(kAdjoint, [&] {
+                    dev_check(mko_adjoint_normal_device(&work_.view, &s, &vars_host_dev_, da, dg,
+                                                        ws, wbytes, nullptr));
+                });
+                if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
+                if (g) copy_out(g + size_t(first) * P, dg, size_t(count) * P * 8);
+                hip_check(hipDeviceSynchronize(), "jacobian");
+                continue;
+            }
             void* dJ = jac_.reserve(size_t(count) * P * R * jbytes);
             double* dr0 = r0_.as<double>(size_t(count) * R);
             if (fd) {
@@ -451,28 +474,21 @@ public:
             } else {
                 timer.run(kAdjoint, [&] {
                     dev_check(mko_adjoint_jacobian_device(&work_.view, &s, &vars_host_dev_,
-                                                          precision, dJ, dr0, nullptr, nullptr));
+                                                          precision, static_cast<double*>(dJ), dr0,
+                                                          nullptr, nullptr));
                 });
             }
-            const int jprec = fd ? MKO_PRECISION_FP64 : precision;
             if (a || g) {
                 double* da = a_.as<double>(size_t(count) * P * P);
                 double* dg = g_.as<double>(size_t(count) * P);
                 timer.run(kGram, [&] {
-                    dev_check(mko_gram_device(dJ, dr0, count, P, R, jprec, da, dg, nullptr));
+                    dev_check(mko_gram_device(static_cast<const double*>(dJ), dr0, count, P, R, da,
+                                              dg, nullptr));
                 });
                 if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
                 if (g) copy_out(g + size_t(first) * P, dg, size_t(count) * P * 8);
             }
-            if (J) {
-                if (jprec == MKO_PRECISION_FP64) {
-                    copy_out(J + size_t(first) * P * R, dJ, size_t(count) * P * R * 8);
-                } else {
-                    std::vector<float> tmp(size_t(count) * P * R);
-                    copy_out(tmp.data(), dJ, tmp.size() * 4);
-                    for (size_t i = 0; i < tmp.size(); ++i) J[size_t(first) * P * R + i] = tmp[i];
-                }
-            }
+            if (J) copy_out(J + size_t(first) * P * R, dJ, size_t(count) * P * R * 8);
             if (r0) copy_out(r0 + size_t(first) * R, dr0, size_t(count) * R * 8);
             hip_check(hipDeviceSynchronize(), "jacobian");
         }

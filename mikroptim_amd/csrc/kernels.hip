@@ -612,14 +612,307 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
     }
 }
 
+// ---- ADJOINT fused with the normal equations (hard failure mode) ---------------------------
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+
+// Per-(run, tuple) partial of the normal equations. With a per-tuple shift c (the weighted mean
+// gradient of the tuple's first 256 rays), o (their weighted mean OPL) and u_k = sqrt(W_k) (g_k - c)
+// over the rays in the piston:
+//   C  = sum_k u_k u_k^T                  (P x P, MFMA, float)
+//   S  = sum_k W_k (g_k - c)              (P, double)
+//   Tr = sum_k W_k (g_k - c) (OPL_k - o)  (P, double)
+// the tuple adds s^2 (C - S S^T / W_ok) to J^T J and s^2 (Tr - (piston - o) S) to J^T r0, because
+// J(k,p) = s sqrt(W_k) (g_kp - gbar_p), r_k = s sqrt(W_k) (OPL_k - piston) and
+// sum_k W_k (OPL_k - piston) = 0 (adjoint_jacobian.cpp:462-479 for J, normal_equations.cpp for
+// the products).
+struct TupleNormal {
+    double w_ok, piston, s2;
+};
+
+constexpr int kGStride = 256 + 4; // LDS row stride of the gradient block (floats)
+
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+    // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ double first_lane(double x) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+// Gradient sink of the fused kernel. Every column is written at most once per ray and, in hard
+// mode, at the same sweep step by every lane whose ray reached the image, so the lanes of a wave
+// write column p together: the value of the wave's first such lane (pilot) goes to pilot[p] and
+// each lane stores its difference to it in float. The float then carries the variation of the
+// gradient over the pupil, not its (often much larger) common part.
+struct LdsSink {
+    float* G;      // [P][kGStride], column = lane of the block
+    double* pilot; // [64] of this wave
+    int col;
+    bool leader;   // first lane of the wave with a reached ray
+    __device__ __forceinline__ void write(int p, double value) const {
+        if (p < 0) return;
+        const double c = first_lane(value);
+        G[p * kGStride + col] = float(value - c);
+        if (leader) pilot[p] = c;
+    }
+};
+
+template <class RB, class RR>
+__device__ __forceinline__ void backward_to_lds(const TraceParams& P, const SurfLds& L,
+                                                const ParamMap& map, const Record<RR>& rec,
+                                                V3<double> M, double kappa, const LdsSink& sink) {
+    const int S = P.S;
+    mkod::RayAdj<RB> adj{{RB(0), RB(0), RB(0)}, {RB(0), RB(0), RB(0)}};
+    {
+        const V3<RB> p{RB(rec.p[S - 1][0]), RB(rec.p[S - 1][1]), RB(rec.p[S - 1][2])};
+        const V3<RB> d{RB(rec.d[S - 1][0]), RB(rec.d[S - 1][1]), RB(rec.d[S - 1][2])};
+        RB nib = RB(0);
+        const RB mz = mkod::reference_step_backward<RB>(p, d, V3<RB>{RB(M.x), RB(M.y), RB(M.z)},
+                                                         RB(kappa), P.ref_kind, RB(L.n[S - 1]),
+                                                         rec.image_branch, RB(1), RB(0), adj, nib);
+        sink.write(map.thick_col[S - 1], double(mz));
+    }
+    uint8_t next_branch = rec.branch[S - 1];
+    for (int s = S - 2; s >= 0; --s) {
+        const V3<RB> p{RB(rec.p[s][0]), RB(rec.p[s][1]), RB(rec.p[s][2])};
+        const V3<RB> d{RB(rec.d[s][0]), RB(rec.d[s][1]), RB(rec.d[s][2])};
+        const int j = s + 1;
+        RB tb = RB(0), cb = RB(0), nbb = RB(0), nab = RB(0);
+        mkod::refracting_step_backward<RB>(p, d, RB(L.t[s]), RB(L.c[j]), RB(L.n[s]), RB(L.n[j]),
+                                           next_branch, RB(1), RB(0), adj, tb, cb, nbb, nab);
+        if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], double(cb));
+        sink.write(map.thick_col[s], double(tb));
+        next_branch = rec.branch[s];
+    }
+}
+
+// One workgroup per (run, tuple), 256 lanes; tuples of at most 512 rays in halves of 256 rays.
+// Per half: forward trace in double (states recorded in RR), reverse sweep in RB, gradients
+// staged in LDS as a [P][256] block (pilot-relative floats); then the shifted moments of the
+// half: C on MFMA (v_mfma_f32_32x32x16_bf16, u split into bf16 hi + lo, hi*hi + hi*lo + lo*hi,
+// each of the 4 waves one 32 x 32 tile of the padded 64 x 64 C), S and Tr on VALU in double.
+template <class RB, class RR, int kHalves>
+__global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, ParamMap map,
+                                                                float* partC, double* partST,
+                                                                TupleNormal* partT) {
+    extern __shared__ float G[]; // [P][kGStride]
+    __shared__ SurfLds L;
+    __shared__ double red[kWaves];
+    __shared__ double opl_l[kBlock];
+    __shared__ float w_l[kBlock], sw_l[kBlock];
+    __shared__ double pilot[kWaves][64];
+    __shared__ double shift[64];          // c
+    __shared__ double delta[kWaves][64];  // pilot - c
+    __shared__ float delta_f[kWaves][64];
+    __shared__ double Sacc[64], Tacc[64];
+    __shared__ double o_shift;
+    const Tuple t = tuple_of(P, blockIdx.x);
+    stage_surfaces(P, t, L);
+    const int Pn = map.P;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    if (tid < 64) {
+        Sacc[tid] = 0.0;
+        Tacc[tid] = 0.0;
+    }
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const double a = P.pupil_radius[t.v];
+    double sin_t, cos_t;
+    sincos(P.field_angle[vf], &sin_t, &cos_t);
+    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+    const int I = wave >> 1, Jt = wave & 1; // C tile of this wave
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    double sw = 0.0, swo = 0.0;
+    Record<RR> rec;
+    for (int h = 0; h < kHalves; ++h) {
+        const int k = tid + h * kBlock;
+        for (int p = 0; p < Pn; ++p) G[p * kGStride + tid] = 0.0f;
+        if (lane < 64) pilot[wave][lane] = 0.0;
+        __syncthreads();
+        float wk = 0.0f;
+        double opl = 0.0;
+        bool ok = false;
+        Sample smp{};
+        Ray r;
+        if (k < t.n) {
+            smp = sample_of(P, t, k);
+            r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+            Soft soft{0.0, MKO_STATUS_OK};
+            rec.count = 0;
+            rec.image_branch = 0;
+            ok = trace_ray<false>(P, L, r, M, kappa, soft, rec) == MKO_STATUS_OK;
+        }
+        const uint64_t okmask = __ballot(ok);
+        if (ok) {
+            sw += smp.weight;
+            swo += smp.weight * r.opl;
+            wk = float(smp.weight);
+            opl = r.opl;
+            const bool leader = lane == int(__builtin_ctzll(okmask));
+            backward_to_lds<RB, RR>(P, L, map, rec, M, kappa, LdsSink{G, pilot[wave], tid, leader});
+        }
+        w_l[tid] = wk;
+        sw_l[tid] = __builtin_sqrtf(wk);
+        opl_l[tid] = opl;
+        __syncthreads();
+        if (h == 0) {
+            // c = weighted mean gradient and o = weighted mean OPL of this half (wave w handles
+            // the parameters p = w mod 4; every wave forms the same sums in the same order)
+            double wsum = 0.0, osum = 0.0;
+            double wwave[kWaves];
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const double wi = double(w_l[w * 64 + lane]);
+                wwave[w] = wave_sum(wi);
+                wsum += wwave[w];
+                osum += wi * opl_l[w * 64 + lane];
+            }
+            osum = wave_sum(osum);
+            for (int p = wave; p < Pn; p += kWaves) {
+                double accp = 0.0;
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w)
+                    accp += double(w_l[w * 64 + lane]) * double(G[p * kGStride + w * 64 + lane]);
+                accp = wave_sum(accp);
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) accp += wwave[w] * pilot[w][p];
+                if (lane == 0) shift[p] = wsum > 0.0 ? accp / wsum : 0.0;
+            }
+            if (tid == 0) o_shift = wsum > 0.0 ? osum / wsum : 0.0;
+            __syncthreads();
+        }
+        if (lane < 64) {
+            const double d = lane < Pn ? pilot[wave][lane] - shift[lane] : 0.0;
+            delta[wave][lane] = d;
+            delta_f[wave][lane] = float(d);
+        }
+        __syncthreads();
+        // S and Tr of this half, fixed order
+        const double o = o_shift;
+        for (int p = wave; p < Pn; p += kWaves) {
+            double sp = 0.0, tp = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const int i = w * 64 + lane;
+                const double wi = double(w_l[i]);
+                const double du = wi * (double(G[p * kGStride + i]) + delta[w][p]);
+                sp += du;
+                tp += wi > 0.0 ? du * (opl_l[i] - o) : 0.0;
+            }
+            sp = wave_sum(sp);
+            tp = wave_sum(tp);
+            if (lane == 0) {
+                Sacc[p] += sp;
+                Tacc[p] += tp;
+            }
+        }
+        // C += u u^T on MFMA: operand lane = (m = lane % 32, kb = lane / 32), 8 rays per operand
+        {
+            const int m = lane & 31, kb = lane >> 5;
+            const int pa = I * 32 + m, pb = Jt * 32 + m;
+            const bool va = pa < Pn, vb = pb < Pn;
+            const float* Ga = G + (va ? pa : 0) * kGStride;
+            const float* Gb = G + (vb ? pb : 0) * kGStride;
+            for (int step = 0; step < kBlock / 16; ++step) {
+                const int k0 = step * 16 + kb * 8;
+                const int w = k0 >> 6;
+                const float da = va ? delta_f[w][pa] : 0.0f;
+                const float db = vb ? delta_f[w][pb] : 0.0f;
+                bf16x8 ahi, alo, bhi, blo;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float swk = sw_l[k0 + e];
+                    const float ua = va ? swk * (Ga[k0 + e] + da) : 0.0f;
+                    const float ub = vb ? swk * (Gb[k0 + e] + db) : 0.0f;
+                    const uint16_t ha = bf16_bits(ua), hb = bf16_bits(ub);
+                    const uint16_t la = bf16_bits(ua - bf16_value(ha));
+                    const uint16_t lb = bf16_bits(ub - bf16_value(hb));
+                    ahi[e] = __builtin_bit_cast(__bf16, ha);
+                    alo[e] = __builtin_bit_cast(__bf16, la);
+                    bhi[e] = __builtin_bit_cast(__bf16, hb);
+                    blo[e] = __builtin_bit_cast(__bf16, lb);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const double w_ok = block_sum(sw, red);
+    const double sum = block_sum(swo, red);
+    const size_t tuple = size_t(blockIdx.x);
+    // C tile out: lane holds column n = lane % 32 of rows 8 (e / 4) + 4 (lane / 32) + e % 4
+    {
+        float* C = partC + tuple * size_t(Pn) * Pn;
+        const int col = Jt * 32 + (lane & 31);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int row = I * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+            if (row < Pn && col < Pn) C[size_t(row) * Pn + col] = acc[e];
+        }
+    }
+    if (tid < Pn) {
+        partST[tuple * 2 * Pn + tid] = Sacc[tid];
+        partST[tuple * 2 * Pn + Pn + tid] = Tacc[tid];
+    }
+    if (tid == 0) {
+        const bool valid = w_ok > 0.0;
+        // piston - o, the OPL shift of Tr
+        partT[tuple] = TupleNormal{w_ok, valid ? sum / w_ok - o_shift : 0.0, t.scale * t.scale};
+    }
+}
+
+// a = sum_t s_t^2 (C_t - S_t S_t^T / W_t), g = sum_t s_t^2 (Tr_t - (piston_t - o_t) S_t), tuples
+// in (field, wavelength) order; one workgroup per run.
+__global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
+    int T, int Pn, const float* partC, const double* partST, const TupleNormal* partT, double* a,
+    double* g) {
+    const int v = blockIdx.x;
+    for (int e = threadIdx.x; e < Pn * Pn; e += blockDim.x) {
+        const int q = e / Pn, p = e - q * Pn;
+        double accv = 0.0;
+        for (int t = 0; t < T; ++t) {
+            const size_t tuple = size_t(v) * T + t;
+            const TupleNormal tn = partT[tuple];
+            if (!(tn.w_ok > 0.0)) continue;
+            const double* S = partST + tuple * 2 * Pn;
+            const double c = double(partC[tuple * size_t(Pn) * Pn + size_t(q) * Pn + p]);
+            accv += tn.s2 * (c - S[p] * S[q] / tn.w_ok);
+        }
+        a[size_t(v) * Pn * Pn + e] = accv;
+    }
+    for (int p = threadIdx.x; p < Pn; p += blockDim.x) {
+        double accv = 0.0;
+        for (int t = 0; t < T; ++t) {
+            const size_t tuple = size_t(v) * T + t;
+            const TupleNormal tn = partT[tuple];
+            if (!(tn.w_ok > 0.0)) continue;
+            const double* S = partST + tuple * 2 * Pn;
+            accv += tn.s2 * (S[Pn + p] - tn.piston * S[p]);
+        }
+        g[size_t(v) * Pn + p] = accv;
+    }
+}
+
 // ---- Gram: A = [J r0]^T [J r0] per run -----------------------------------------------------
 
 // 16 x 16 output tile per workgroup over the upper triangle of the (P+1) x (P+1) matrix;
 // rows staged through LDS in chunks of 64. Each element is one FMA chain over the rows in
 // ascending order starting from 0 (gram_dot, normal_equations.hpp:19), products of the stored
-// J values formed in double: for double J this is bitwise the host definition.
-template <class TJ>
-__global__ __launch_bounds__(256) void gram_kernel(const TJ* J, const double* r0, int runs, int P,
+// J values: bitwise the host definition.
+__global__ __launch_bounds__(256) void gram_kernel(const double* J, const double* r0, int runs, int P,
                                                    int64_t rows, double* a, double* g) {
     constexpr int kT = 16, kC = 64;
     __shared__ double As[kT][kC + 1];
@@ -637,7 +930,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const TJ* J, const double* r0
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int p = tp * kT + tx; // row of A tile (column of J)
     const int q = tq * kT + ty;
-    const TJ* Jr = J + size_t(run) * P * rows;
+    const double* Jr = J + size_t(run) * P * rows;
     const double* rr = r0 + size_t(run) * rows;
     double acc = 0.0;
     for (int64_t r0c = 0; r0c < rows; r0c += kC) {
@@ -647,8 +940,8 @@ __global__ __launch_bounds__(256) void gram_kernel(const TJ* J, const double* r0
             const int pc = tp * kT + cc, qc = tq * kT + cc;
             double va = 0.0, vb = 0.0;
             if (row < rows) {
-                va = pc < P ? double(Jr[size_t(pc) * rows + row]) : pc == P ? rr[row] : 0.0;
-                vb = qc < P ? double(Jr[size_t(qc) * rows + row]) : qc == P ? rr[row] : 0.0;
+                va = pc < P ? Jr[size_t(pc) * rows + row] : pc == P ? rr[row] : 0.0;
+                vb = qc < P ? Jr[size_t(qc) * rows + row] : qc == P ? rr[row] : 0.0;
             }
             As[cc][rw] = va;
             Bs[cc][rw] = vb;
@@ -904,7 +1197,7 @@ int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
 }
 
 int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_settings* settings,
-                                const mko_variables* vars, int precision, void* jacobian,
+                                const mko_variables* vars, int precision, double* jacobian,
                                 double* r0, uint8_t* status, void* stream) {
     MKO_TRY({
         const TraceParams P = make_params(*batch, *settings);
@@ -913,34 +1206,71 @@ int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_setti
         if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
         const int rpl = rays_per_lane(P);
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        if (precision == MKO_PRECISION_FP64) {
-            double* Jd = static_cast<double*>(jacobian);
-            if (P.soft) launch_adjoint_rpl<true, double, double, double>(rpl, P, m, Jd, r0, status, s);
-            else launch_adjoint_rpl<false, double, double, double>(rpl, P, m, Jd, r0, status, s);
-        } else if (precision == MKO_PRECISION_MIXED) {
-            float* Jf = static_cast<float*>(jacobian);
-            if (P.soft) launch_adjoint_rpl<true, float, float, float>(rpl, P, m, Jf, r0, status, s);
-            else launch_adjoint_rpl<false, float, float, float>(rpl, P, m, Jf, r0, status, s);
-        } else {
+        // MKO_PRECISION_MIXED computes the stored Jacobian in FP64 as well: on this hardware
+        // the reduced-precision stage is the MFMA normal equations of the fused LM path.
+        if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED)
             throw std::invalid_argument("unknown precision");
-        }
+        double* Jd = static_cast<double*>(jacobian);
+        if (P.soft) launch_adjoint_rpl<true, double, double, double>(rpl, P, m, Jd, r0, status, s);
+        else launch_adjoint_rpl<false, double, double, double>(rpl, P, m, Jd, r0, status, s);
         check_hip(hipGetLastError(), "adjoint launch");
     })
 }
 
-int mko_gram_device(const void* jacobian, const double* r0, int runs, int variables,
-                    int64_t rows, int precision, double* a, double* g, void* stream) {
+size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, int variables) {
+    const size_t tuples = size_t(variants) * size_t(fields) * size_t(wavelengths);
+    const size_t P = size_t(variables);
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return up(tuples * P * P * sizeof(float)) + up(tuples * 2 * P * sizeof(double)) +
+           up(tuples * sizeof(TupleNormal));
+}
+
+int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_settings* settings,
+                              const mko_variables* vars, double* a, double* g, void* workspace,
+                              size_t workspace_bytes, void* stream) {
+    MKO_TRY({
+        const TraceParams P = make_params(*batch, *settings);
+        const ParamMap m = make_map(*vars, batch->surfaces);
+        if (P.soft) throw std::invalid_argument("fused adjoint normal equations need hard failure mode");
+        if (m.P > 64) throw std::invalid_argument("fused adjoint normal equations: at most 64 variables");
+        const int rpl = rays_per_lane(P);
+        if (rpl > 2) throw std::invalid_argument("fused adjoint normal equations: at most 512 rays per tuple");
+        if (P.V == 0) return 0;
+        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
+        const size_t need = mko_adjoint_normal_workspace(P.V, P.F, P.L, m.P);
+        if (workspace_bytes < need) throw std::invalid_argument("workspace too small");
+        const size_t tuples = size_t(P.V) * P.T;
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        char* w = static_cast<char*>(workspace);
+        float* partC = reinterpret_cast<float*>(w);
+        double* partST = reinterpret_cast<double*>(w + up(tuples * m.P * m.P * sizeof(float)));
+        TupleNormal* partT = reinterpret_cast<TupleNormal*>(
+            w + up(tuples * m.P * m.P * sizeof(float)) + up(tuples * 2 * m.P * sizeof(double)));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t lds = size_t(m.P) * kGStride * sizeof(float);
+        const dim3 grid{unsigned(tuples)}, block{unsigned(kBlock)};
+        if (rpl == 1)
+            hipLaunchKernelGGL((adjoint_normal_kernel<double, double, 1>), grid, block, lds, s, P, m,
+                               partC, partST, partT);
+        else
+            hipLaunchKernelGGL((adjoint_normal_kernel<double, double, 2>), grid, block, lds, s, P, m,
+                               partC, partST, partT);
+        check_hip(hipGetLastError(), "adjoint normal launch");
+        hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
+                           m.P, partC, partST, partT, a, g);
+        check_hip(hipGetLastError(), "adjoint normal finalize launch");
+    })
+}
+
+int mko_gram_device(const double* jacobian, const double* r0, int runs, int variables,
+                    int64_t rows, double* a, double* g, void* stream) {
     MKO_TRY({
         if (runs <= 0) return 0;
         const int T = (variables + 1 + 15) / 16;
         const dim3 grid(unsigned(runs) * unsigned(T * (T + 1) / 2));
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        if (precision == MKO_PRECISION_FP64)
-            hipLaunchKernelGGL(gram_kernel<double>, grid, dim3(256), 0, s,
-                               static_cast<const double*>(jacobian), r0, runs, variables, rows, a, g);
-        else
-            hipLaunchKernelGGL(gram_kernel<float>, grid, dim3(256), 0, s,
-                               static_cast<const float*>(jacobian), r0, runs, variables, rows, a, g);
+        hipLaunchKernelGGL(gram_kernel, grid, dim3(256), 0, s, jacobian, r0, runs, variables, rows,
+                           a, g);
         check_hip(hipGetLastError(), "gram launch");
     })
 }

@@ -23,7 +23,8 @@ _lib = None
 # every symbol include/mikroptim_hip.h declares
 EXPORTS = [
     "mko_last_error", "mko_residuals_device", "mko_merit_device", "mko_point_batch_device",
-    "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_gram_device",
+    "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
+    "mko_adjoint_normal_device", "mko_gram_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
     "mko_lm_results", "mko_lm_destroy",
@@ -57,10 +58,13 @@ def library():
                                  C.c_int),
         "mko_merit_device": ([bp, sp, _dp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp,
                               C.POINTER(C.c_int), C.c_void_p], C.c_int),
-        "mko_gram_device": ([C.c_void_p, _dp, C.c_int, C.c_int, C.c_int64, C.c_int, _dp, _dp,
+        "mko_gram_device": ([C.c_void_p, _dp, C.c_int, C.c_int, C.c_int64, _dp, _dp,
                              C.c_void_p], C.c_int),
         "mko_adjoint_jacobian_device": ([bp, sp, vp, C.c_int, C.c_void_p, _dp,
                                          C.POINTER(C.c_uint8), C.c_void_p], C.c_int),
+        "mko_adjoint_normal_workspace": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
+        "mko_adjoint_normal_device": ([bp, sp, vp, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_size_t, C.c_void_p], C.c_int),
         "mko_engine_create": ([C.c_int, C.c_size_t], C.c_void_p),
         "mko_engine_destroy": ([C.c_void_p], None),
         "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
@@ -194,12 +198,14 @@ class TraceEngine:
 
     def jacobian(self, base, settings, variables, x0, method="adjoint", precision="fp64",
                  want_jacobian=True):
-        """(J [runs][P][R] or None, r0 [runs][R], A [runs][P][P], g [runs][P])."""
+        """(J [runs][P][R] or None, r0 [runs][R] or None, A [runs][P][P], g [runs][P]).
+        want_jacobian=False asks for the normal equations only (the LM's use): J and r0 are
+        then not returned, and the mixed-precision adjoint forms A and g without storing J."""
         x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
         runs, P = x0.shape
         R = trace_rows(base, settings)
         J = np.zeros((runs, P, R)) if want_jacobian else None
-        r0 = np.zeros((runs, R))
+        r0 = np.zeros((runs, R)) if want_jacobian else None
         a = np.zeros((runs, P, P))
         g = np.zeros((runs, P))
         b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
