@@ -107,8 +107,9 @@ void mko_engine_destroy(mko_engine* engine);
 
 /* Device time per kernel family, measured with HIP events on the launch stream around each
  * launch while enabled (AdjointJacobianTiming, include/mikroptim/cuda/adjoint_jacobian.hpp:39):
- * ms / launches [5] = trace (RESIDUALS and MERIT), adjoint, gram, fd difference, point batch.
- * Returns the totals since the previous call, resets them and sets the enable flag. */
+ * ms / launches [6] = trace (RESIDUALS and MERIT), adjoint (stored-J or fused normal
+ * equations), gram, fd difference, point batch, LM control (device-side step solves and
+ * updates). Returns the totals since the previous call, resets them and sets the enable flag. */
 int mko_engine_kernel_timing(mko_engine* engine, int enable, double* ms, int64_t* launches);
 
 /* TraceEngine::run(batch, TraceMode::Residuals, ...): host batch in, host results out
@@ -133,6 +134,21 @@ int mko_engine_jacobian(mko_engine* engine, const mko_batch* base,
                         const mko_residual_settings* settings, const mko_variables* vars,
                         int runs, const double* x0, int method, int precision, double* jacobian,
                         double* r0, double* a, double* g);
+
+/* GpuLmBackend::normal_equations / merit (src/cuda/levenberg_marquardt.cpp:70-248): the two
+ * pure virtuals of mko::LmBackend (include/mikroptim/optim/levenberg_marquardt.hpp:171-182) for
+ * `count` points x [count][P] of the runs `runs` [count] (ascending run indices into base when
+ * base has one variant per run; NULL = 0 .. count-1; ignored for a one-variant base).
+ * a [count][P][P] (a[j*P*P + q*P + p]), g [count][P]; merit [count], failed / hard [count].
+ * A run's results depend only on its index and point. */
+int mko_engine_normal_equations(mko_engine* engine, const mko_batch* base,
+                                const mko_residual_settings* settings, const mko_variables* vars,
+                                int count, const int32_t* runs, const double* x, int method,
+                                int precision, double* a, double* g);
+int mko_engine_point_merit(mko_engine* engine, const mko_batch* base,
+                           const mko_residual_settings* settings, const mko_variables* vars,
+                           int count, const int32_t* runs, const double* x, double* merit,
+                           int64_t* failed, int64_t* hard_failed);
 
 /* Batched Levenberg-Marquardt on the GPU backend (optimize_lm_gpu,
  * include/mikroptim/cuda/levenberg_marquardt.hpp:91; control logic of

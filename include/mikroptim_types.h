@@ -120,6 +120,10 @@ typedef struct mko_lm_options {
     int32_t jacobian_method; /* MKO_JACOBIAN_* */
     double fail_weight_factor; /* 10 */
     double max_fail_weight;    /* 1e14 */
+    /* 0: the per-cycle control (step solves, acceptance, convergence tests) runs on the device
+     * for problems of at most 64 variables; 1: always on the host. Both take the same steps.
+     * No counterpart in LmOptions (the reference's control always runs on the host). */
+    int32_t host_control;
 } mko_lm_options;
 
 /* Per-run LM result (LmRunResult, include/mikroptim/optim/levenberg_marquardt.hpp:130). */

@@ -30,6 +30,8 @@
 #include <vector>
 
 #include "error.hpp"
+#include "internal.hpp"
+#include "lm_device.hpp"
 #include "mikroptim_hip.h"
 
 namespace mko_host {
@@ -197,6 +199,44 @@ struct DeviceBatch {
     double* mut(const double* p) { return const_cast<double*>(p); }
 };
 
+// Host batch made of selected variants of another batch (run templates of an LM subset).
+struct GatheredBatch {
+    std::vector<double> curv, thick, index, pupil, fa, ft, rc;
+    mko_batch view{};
+    const mko_batch& gather(const mko_batch& base, const int* ids, size_t n) {
+        const Strides st(base);
+        curv.resize(n * st.S);
+        thick.resize(n * st.S);
+        index.resize(n * st.SL);
+        pupil.resize(n);
+        fa.resize(n * st.F);
+        ft.resize(n * st.F);
+        rc.resize(st.refc ? n * st.F : 0);
+        for (size_t j = 0; j < n; ++j) {
+            const int id = ids[j];
+            if (id < 0 || id >= base.variants) throw std::out_of_range("run index outside the base batch");
+            const size_t v = size_t(id);
+            std::copy_n(base.curvature + v * st.S, st.S, curv.data() + j * st.S);
+            std::copy_n(base.thickness + v * st.S, st.S, thick.data() + j * st.S);
+            std::copy_n(base.index + v * st.SL, st.SL, index.data() + j * st.SL);
+            pupil[j] = base.pupil_radius[v];
+            std::copy_n(base.field_angle + v * st.F, st.F, fa.data() + j * st.F);
+            std::copy_n(base.field_target + v * st.F, st.F, ft.data() + j * st.F);
+            if (st.refc) std::copy_n(base.reference_curvature + v * st.F, st.F, rc.data() + j * st.F);
+        }
+        view = base;
+        view.variants = int(n);
+        view.curvature = curv.data();
+        view.thickness = thick.data();
+        view.index = index.data();
+        view.pupil_radius = pupil.data();
+        view.field_angle = fa.data();
+        view.field_target = ft.data();
+        view.reference_curvature = st.refc ? rc.data() : nullptr;
+        return view;
+    }
+};
+
 // ---- FD pattern (central scheme): fd_pattern, src/optim/variant_generator.cpp:177-237 -------
 
 constexpr double kFdStepFactor = 6.0554544523933395e-06; // 2^(-52/3)
@@ -243,41 +283,75 @@ void fd_pattern(const mko_variables& v, const double* x0, Pattern& out, size_t a
 // ---- kernel timing (AdjointJacobianTiming-like, include/mikroptim/cuda/adjoint_jacobian.hpp:39) --
 
 enum KernelFamily { kTrace = 0, kAdjoint = 1, kGram = 2, kFdDifference = 3, kPointBatch = 4,
-                    kFamilies = 5 };
+                    kLmControl = 5, kFamilies = 6 };
 
 // Accumulates the device time of each kernel family with HIP events recorded on the launch
-// stream (the null stream) around each launch, when enabled.
+// stream (the null stream) around each launch, when enabled. Recording does not block the host;
+// the events are resolved when the totals are read (collect()).
 class KernelTimer {
 public:
     bool enabled = false;
     double ms[kFamilies] = {};
     int64_t launches[kFamilies] = {};
+    ~KernelTimer() { drop(); }
     template <class F>
     void run(KernelFamily fam, F&& launch) {
         if (!enabled) {
             launch();
             return;
         }
-        hipEvent_t a, b;
-        hip_check(hipEventCreate(&a), "hipEventCreate");
-        hip_check(hipEventCreate(&b), "hipEventCreate");
+        hipEvent_t a = event(), b = event();
         hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
         launch();
         hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
-        hip_check(hipEventSynchronize(b), "hipEventSynchronize");
-        float t = 0.0f;
-        hip_check(hipEventElapsedTime(&t, a, b), "hipEventElapsedTime");
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        ms[fam] += t;
-        ++launches[fam];
+        pending_.push_back(Pending{fam, a, b});
+    }
+    void collect() {
+        for (const Pending& p : pending_) {
+            hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
+            float t = 0.0f;
+            hip_check(hipEventElapsedTime(&t, p.a, p.b), "hipEventElapsedTime");
+            ms[p.fam] += t;
+            ++launches[p.fam];
+            pool_.push_back(p.a);
+            pool_.push_back(p.b);
+        }
+        pending_.clear();
     }
     void reset() {
+        collect();
         for (int i = 0; i < kFamilies; ++i) {
             ms[i] = 0.0;
             launches[i] = 0;
         }
     }
+
+private:
+    struct Pending {
+        KernelFamily fam;
+        hipEvent_t a, b;
+    };
+    hipEvent_t event() {
+        if (!pool_.empty()) {
+            hipEvent_t e = pool_.back();
+            pool_.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        return e;
+    }
+    void drop() {
+        for (const Pending& p : pending_) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+        pending_.clear();
+        pool_.clear();
+    }
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
 };
 
 // ---- Engine --------------------------------------------------------------------------------
@@ -386,8 +460,7 @@ public:
         // the fused kernel and J is never formed (hard failure mode, <= 64 variables, <= 512
         // rays per tuple).
         const bool fused = !fd && precision == MKO_PRECISION_MIXED && !J && !r0 && (a || g) &&
-                           s.failure == MKO_FAILURE_HARD && P <= 64 &&
-                           int64_t(s.rings) * s.spokes <= 2 * 256;
+                           fused_ok(s, P);
         const int per_run = fd ? 2 * P + 1 : 1;
         const size_t jbytes = 8;
         const size_t T = size_t(base.fields) * base.wavelengths;
@@ -525,6 +598,134 @@ public:
             copy_out(hard + first, d_hard_, size_t(count) * 8);
         }
         hip_check(hipDeviceSynchronize(), "point merit");
+    }
+
+    // ---- device-resident variants for the LM control on the device -------------------------
+
+    // Variable types / surfaces for the point batches (host copy kept for the adjoint map).
+    void set_variables(const mko_variables& v) {
+        DeviceGuard guard(device_);
+        upload_vars(v);
+    }
+
+    // Normal equations a [count][P][P], g [count][P] (device) of `count` points xd [count][P]
+    // (device) on the device template tmpl (one variant, or one per run selected by tidx).
+    // Variables must have been set with set_variables; bounds lower / upper / typical [P] on the
+    // device (FD pattern).
+    void jacobian_dev(const mko_batch& tmpl, const int32_t* tidx, int count, const double* xd,
+                      const mko_residual_settings& s, int P, const double* lower,
+                      const double* upper, const double* typical, int method, int precision,
+                      double* ad, double* gd) {
+        DeviceGuard guard(device_);
+        if (count <= 0) return;
+        const int64_t R = trace_rows_of(tmpl, s), rays = rays_of(tmpl, s);
+        const bool fd = method == MKO_JACOBIAN_FD;
+        const bool fused = !fd && precision == MKO_PRECISION_MIXED && fused_ok(s, P);
+        const int per_run = fd ? 2 * P + 1 : 1;
+        const size_t T = size_t(tmpl.fields) * tmpl.wavelengths;
+        const size_t per = size_t(per_run) * (Strides(tmpl).doubles() * 8 +
+                                              (fd ? size_t(R) * 8 + size_t(rays) : 0)) +
+                           (fused ? T * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
+                                  : size_t(P) * R * 8 + size_t(R) * 8) +
+                           size_t(P) * 48 + 64;
+        const int chunk = tranche_size(count, per);
+        for (int first = 0; first < count; first += chunk) {
+            const int c = std::min(chunk, count - first);
+            const double* x = xd + size_t(first) * P;
+            double *dplus = nullptr, *dminus = nullptr, *dsp = nullptr, *dsm = nullptr;
+            uint8_t* dsten = nullptr;
+            if (fd) {
+                dplus = plus_.as<double>(size_t(c) * P);
+                dminus = minus_.as<double>(size_t(c) * P);
+                dsp = sp_.as<double>(size_t(c) * P);
+                dsm = sm_.as<double>(size_t(c) * P);
+                dsten = sten_.as<uint8_t>(size_t(c) * P);
+                timer.run(kLmControl, [&] {
+                    mko_lmdev::launch_fd_pattern(c, P, x, lower, upper, typical, dplus, dminus, dsp,
+                                                 dsm, dsten, nullptr);
+                    hip_check(hipGetLastError(), "fd pattern launch");
+                });
+            }
+            work_.allocate(tmpl, c * per_run);
+            timer.run(kPointBatch, [&] {
+                dev_check(mko_internal::point_batch(
+                    &tmpl, tidx ? tidx + first : nullptr, 0, c, per_run, &dvars_, x, dplus, dminus,
+                    work_.mut(work_.view.curvature), work_.mut(work_.view.thickness),
+                    work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
+                    work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
+                    work_.mut(work_.view.reference_curvature), nullptr));
+            });
+            double* a_out = ad + size_t(first) * P * P;
+            double* g_out = gd + size_t(first) * P;
+            if (fused) {
+                const size_t wbytes =
+                    mko_adjoint_normal_workspace(c, tmpl.fields, tmpl.wavelengths, P);
+                void* ws = jac_.reserve(wbytes);
+                timer.run(kAdjoint, [&] {
+                    dev_check(mko_adjoint_normal_device(&work_.view, &s, &vars_host_dev_, a_out,
+                                                        g_out, ws, wbytes, nullptr));
+                });
+                continue;
+            }
+            double* dJ = jac_.as<double>(size_t(c) * P * R);
+            double* dr0 = r0_.as<double>(size_t(c) * R);
+            if (fd) {
+                double* fres = fdres_.as<double>(size_t(c) * per_run * R);
+                uint8_t* fst = fdst_.as<uint8_t>(size_t(c) * per_run * rays);
+                timer.run(kTrace, [&] {
+                    dev_check(mko_residuals_device(&work_.view, &s, fres, R, nullptr, fst, nullptr));
+                });
+                int64_t* dch = ch_.as<int64_t>(size_t(c) * P);
+                timer.run(kFdDifference, [&] {
+                    dev_check(mko_fd_difference_device(fres, R, fst, rays, c, P, R, dsp, dsm, dsten,
+                                                       dJ, dr0, dch, nullptr));
+                });
+            } else {
+                timer.run(kAdjoint, [&] {
+                    dev_check(mko_adjoint_jacobian_device(&work_.view, &s, &vars_host_dev_,
+                                                          precision, dJ, dr0, nullptr, nullptr));
+                });
+            }
+            timer.run(kGram, [&] {
+                dev_check(mko_gram_device(dJ, dr0, c, P, R, a_out, g_out, nullptr));
+            });
+        }
+    }
+
+    // MERIT of `count` points xd [count][P] (device) on the device template: merit, failed,
+    // hard-failed counts into device arrays.
+    void merit_dev(const mko_batch& tmpl, const int32_t* tidx, int count, const double* xd,
+                   const mko_residual_settings& s, int P, double* md, int64_t* fd,
+                   int64_t* hd) {
+        DeviceGuard guard(device_);
+        if (count <= 0) return;
+        const size_t T = size_t(tmpl.fields) * tmpl.wavelengths;
+        const size_t per = Strides(tmpl).doubles() * 8 + T * 16 + 64 + size_t(P) * 8;
+        const int chunk = tranche_size(count, per);
+        for (int first = 0; first < count; first += chunk) {
+            const int c = std::min(chunk, count - first);
+            work_.allocate(tmpl, c);
+            timer.run(kPointBatch, [&] {
+                dev_check(mko_internal::point_batch(
+                    &tmpl, tidx ? tidx + first : nullptr, 0, c, 1, &dvars_,
+                    xd + size_t(first) * P, nullptr, nullptr, work_.mut(work_.view.curvature),
+                    work_.mut(work_.view.thickness), work_.mut(work_.view.index),
+                    work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
+                    work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
+                    nullptr));
+            });
+            double* part = part_.as<double>(size_t(c) * T);
+            int* pc = partc_.as<int>(size_t(c) * T * 2);
+            timer.run(kTrace, [&] {
+                dev_check(mko_merit_device(&work_.view, &s, nullptr, md + first, fd + first,
+                                           hd + first, part, pc, nullptr));
+            });
+        }
+    }
+
+    // The mixed-precision normal equations come from the fused kernel under these conditions.
+    static bool fused_ok(const mko_residual_settings& s, int P) {
+        return s.failure == MKO_FAILURE_HARD && P <= 64 && int64_t(s.rings) * s.spokes <= 512;
     }
 
     int device() const { return device_; }
@@ -751,6 +952,8 @@ public:
         x0_.assign(x0, x0 + size_t(runs) * P_);
         weight_.assign(size_t(runs), s.fail_weight);
         rounds_.assign(size_t(runs), 1);
+        device_ = !o.host_control && P_ <= size_t(mko_lmdev::kMaxVariables);
+        if (device_) dev_setup();
         start_round(std::vector<int>(), x0_);
     }
 
@@ -767,7 +970,8 @@ public:
         return true;
     }
 
-    void results(double* x, mko_lm_run* out, int* cycles) const {
+    void results(double* x, mko_lm_run* out, int* cycles) {
+        if (device_ && !done_) dev_download_state();
         const std::vector<Run>& src = (done_ || !ids_.empty()) ? final_ : state_;
         for (int i = 0; i < runs_; ++i) {
             const Run& r = src[size_t(i)];
@@ -792,6 +996,10 @@ public:
 private:
     // ---- one pass of levenberg_marquardt over the runs `ids` (empty = all) ----
     void start_round(const std::vector<int>& ids, const std::vector<double>& xs) {
+        if (device_) {
+            dev_start_round(ids, xs);
+            return;
+        }
         ids_ = ids;
         const int n = ids.empty() ? runs_ : int(ids.size());
         state_.assign(size_t(n), Run());
@@ -824,38 +1032,11 @@ private:
         for (int i : list) global.push_back(ids_.empty() ? i : ids_[size_t(i)]);
         if (base_.variants == 1) return base_;
         gather(global);
-        return gathered_;
+        return *gathered_;
     }
 
     void gather(const std::vector<int>& global) {
-        const Strides st(base_);
-        const size_t n = global.size();
-        g_curv_.resize(n * st.S);
-        g_thick_.resize(n * st.S);
-        g_index_.resize(n * st.SL);
-        g_pupil_.resize(n);
-        g_fa_.resize(n * st.F);
-        g_ft_.resize(n * st.F);
-        g_rc_.resize(st.refc ? n * st.F : 0);
-        for (size_t j = 0; j < n; ++j) {
-            const size_t v = size_t(global[j]);
-            std::copy_n(base_.curvature + v * st.S, st.S, g_curv_.data() + j * st.S);
-            std::copy_n(base_.thickness + v * st.S, st.S, g_thick_.data() + j * st.S);
-            std::copy_n(base_.index + v * st.SL, st.SL, g_index_.data() + j * st.SL);
-            g_pupil_[j] = base_.pupil_radius[v];
-            std::copy_n(base_.field_angle + v * st.F, st.F, g_fa_.data() + j * st.F);
-            std::copy_n(base_.field_target + v * st.F, st.F, g_ft_.data() + j * st.F);
-            if (st.refc) std::copy_n(base_.reference_curvature + v * st.F, st.F, g_rc_.data() + j * st.F);
-        }
-        gathered_ = base_;
-        gathered_.variants = int(n);
-        gathered_.curvature = g_curv_.data();
-        gathered_.thickness = g_thick_.data();
-        gathered_.index = g_index_.data();
-        gathered_.pupil_radius = g_pupil_.data();
-        gathered_.field_angle = g_fa_.data();
-        gathered_.field_target = g_ft_.data();
-        gathered_.reference_curvature = st.refc ? g_rc_.data() : nullptr;
+        gathered_ = &gathered_batch_.gather(base_, global.data(), global.size());
     }
 
     void gather_x(const std::vector<int>& list, bool trial) {
@@ -879,7 +1060,9 @@ private:
                          fl_.data(), hf_.data());
     }
 
-    bool round_cycle() {
+    bool round_cycle() { return device_ ? dev_round_cycle() : host_round_cycle(); }
+
+    bool host_round_cycle() {
         const int n = int(state_.size());
         std::vector<int> list;
         for (int i = 0; i < n; ++i) {
@@ -993,6 +1176,7 @@ private:
     }
 
     void finish_round() {
+        if (device_) dev_download_state();
         if (ids_.empty()) {
             final_ = state_;
             return;
@@ -1039,6 +1223,197 @@ private:
         return true;
     }
 
+    // ---- LM control on the device (lm_device.hip): per-run state in device memory -----------
+
+    void dev_setup() {
+        validate_values(base_, 0, base_.variants);
+        eng_.set_variables(vars_);
+        dev_tmpl_.upload(base_, 0, base_.variants, nullptr);
+        double* bnd = bounds_d_.as<double>(3 * P_);
+        hip_check(hipMemcpy(bnd, vars_.lower, P_ * 8, hipMemcpyHostToDevice), "bounds");
+        hip_check(hipMemcpy(bnd + P_, vars_.upper, P_ * 8, hipMemcpyHostToDevice), "bounds");
+        hip_check(hipMemcpy(bnd + 2 * P_, vars_.typical, P_ * 8, hipMemcpyHostToDevice), "bounds");
+        dopt_.max_iterations = o_.max_iterations;
+        dopt_.stall_window = o_.stall_window;
+        dopt_.reject_new_failures = o_.reject_new_failures;
+        dopt_.ftol = o_.ftol;
+        dopt_.xtol = o_.xtol;
+        dopt_.gtol = o_.gtol;
+        dopt_.stall_tolerance = o_.stall_tolerance;
+        dopt_.initial_damping = o_.initial_damping;
+        dopt_.max_damping = o_.max_damping;
+        dopt_.diag_floor = o_.diag_floor;
+    }
+
+    // State arrays of n runs in one allocation.
+    void dev_allocate(int n) {
+        const size_t N = size_t(n), P = P_;
+        const int H = o_.stall_window > 0 ? o_.stall_window + 1 : 1;
+        const size_t doubles = N * (4 * P + P * P + 7 + size_t(H));
+        const size_t bytes = doubles * 8 + N * 2 * 8 + N * 5 * 4 + N * 4 + 1024;
+        char* p = static_cast<char*>(state_mem_.reserve(bytes));
+        auto take = [&](size_t b) {
+            char* q = p;
+            p += (b + 15) & ~size_t(15);
+            return q;
+        };
+        mko_lmdev::State& S = ds_;
+        S.n = n;
+        S.P = int(P);
+        S.H = H;
+        S.x = reinterpret_cast<double*>(take(N * P * 8));
+        S.trial = reinterpret_cast<double*>(take(N * P * 8));
+        S.g = reinterpret_cast<double*>(take(N * P * 8));
+        S.d = reinterpret_cast<double*>(take(N * P * 8));
+        S.a = reinterpret_cast<double*>(take(N * P * P * 8));
+        S.merit = reinterpret_cast<double*>(take(N * 8));
+        S.mu = reinterpret_cast<double*>(take(N * 8));
+        S.nu = reinterpret_cast<double*>(take(N * 8));
+        S.step_norm = reinterpret_cast<double*>(take(N * 8));
+        S.predicted = reinterpret_cast<double*>(take(N * 8));
+        S.gradient_norm = reinterpret_cast<double*>(take(N * 8));
+        S.model_decrease = reinterpret_cast<double*>(take(N * 8));
+        S.failed = reinterpret_cast<int64_t*>(take(N * 8));
+        S.hard = reinterpret_cast<int64_t*>(take(N * 8));
+        S.status = reinterpret_cast<int32_t*>(take(N * 4));
+        S.iterations = reinterpret_cast<int32_t*>(take(N * 4));
+        S.accepted = reinterpret_cast<int32_t*>(take(N * 4));
+        S.jacobians = reinterpret_cast<int32_t*>(take(N * 4));
+        S.history_len = reinterpret_cast<int32_t*>(take(N * 4));
+        S.need_jacobian = reinterpret_cast<uint8_t*>(take(N));
+        S.rejected = reinterpret_cast<uint8_t*>(take(N));
+        S.solved = reinterpret_cast<uint8_t*>(take(N));
+        S.evaluate = reinterpret_cast<uint8_t*>(take(N));
+        S.history = reinterpret_cast<double*>(take(N * size_t(H) * 8));
+        const double* bnd = static_cast<const double*>(bounds_d_.as<double>(3 * P));
+        S.lower = bnd;
+        S.upper = bnd + P;
+        S.typical = bnd + 2 * P;
+        lists_ = lists_mem_.as<int32_t>(4 * N + 8);
+        counts_ = lists_ + 4 * N; // [0] Jacobian, [1] cycle, [2] evaluate
+    }
+
+    // template index per run of the round (global run ids) when runs have their own template
+    const int32_t* dev_round_index() const { return base_.variants == 1 ? nullptr : round_index_; }
+
+    int dev_count(int which) {
+        int32_t c = 0;
+        hip_check(hipMemcpy(&c, counts_ + which, 4, hipMemcpyDeviceToHost), "LM count");
+        return c;
+    }
+
+    void dev_start_round(const std::vector<int>& ids, const std::vector<double>& xs) {
+        ids_ = ids;
+        const int n = ids.empty() ? runs_ : int(ids.size());
+        state_.assign(size_t(n), Run());
+        dev_allocate(n);
+        hip_check(hipMemcpy(ds_.x, xs.data(), size_t(n) * P_ * 8, hipMemcpyHostToDevice), "x");
+        std::vector<int32_t> index(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) index[size_t(i)] = ids.empty() ? i : ids[size_t(i)];
+        round_index_ = round_mem_.as<int32_t>(size_t(n));
+        hip_check(hipMemcpy(round_index_, index.data(), size_t(n) * 4, hipMemcpyHostToDevice),
+                  "round index");
+        eng_.timer.run(kLmControl, [&] { mko_lmdev::launch_init(ds_, dopt_, nullptr); });
+        double* fm = fm_.as<double>(size_t(n));
+        int64_t* ff = ff_.as<int64_t>(size_t(n));
+        int64_t* fh = fh_.as<int64_t>(size_t(n));
+        eng_.merit_dev(dev_tmpl_.view, dev_round_index(), n, ds_.x, settings_, int(P_), fm, ff, fh);
+        eng_.timer.run(kLmControl, [&] { mko_lmdev::launch_start(ds_, fm, ff, fh, nullptr); });
+        hip_check(hipGetLastError(), "LM start");
+    }
+
+    bool dev_round_cycle() {
+        const int n = ds_.n;
+        int32_t* list_j = lists_;
+        int32_t* list_c = lists_ + n;
+        int32_t* list_e = lists_ + 2 * n;
+        int32_t* pos = lists_ + 3 * n;
+        const int32_t* rix = dev_round_index();
+        double* xs = xs_d_.as<double>(size_t(n) * P_);
+        int32_t* tidx = rix ? tidx_d_.as<int32_t>(size_t(n)) : nullptr;
+        eng_.timer.run(kLmControl, [&] {
+            mko_lmdev::launch_select(ds_, mko_lmdev::kListJacobian, o_.max_iterations, nullptr,
+                                     nullptr, list_j, counts_ + 0, nullptr, nullptr);
+        });
+        const int cj = dev_count(0);
+        if (cj > 0) {
+            eng_.timer.run(kLmControl, [&] {
+                mko_lmdev::launch_gather(ds_, list_j, cj, false, rix, xs, tidx, nullptr);
+            });
+            double* ac = ac_.as<double>(size_t(cj) * P_ * P_);
+            double* gc = gc_.as<double>(size_t(cj) * P_);
+            eng_.jacobian_dev(dev_tmpl_.view, tidx, cj, xs, settings_, int(P_), ds_.lower,
+                              ds_.upper, ds_.typical, o_.jacobian_method, precision_, ac, gc);
+            eng_.timer.run(kLmControl, [&] {
+                mko_lmdev::launch_prepare(ds_, dopt_, list_j, cj, ac, gc, nullptr);
+            });
+        }
+        eng_.timer.run(kLmControl, [&] {
+            mko_lmdev::launch_select(ds_, mko_lmdev::kListCycle, o_.max_iterations, nullptr,
+                                     nullptr, list_c, counts_ + 1, nullptr, nullptr);
+        });
+        const int cc = dev_count(1);
+        if (cc == 0) return false;
+        ++cycles_;
+        eng_.timer.run(kLmControl, [&] {
+            mko_lmdev::launch_step(ds_, dopt_, list_c, cc, nullptr);
+            mko_lmdev::launch_select(ds_, mko_lmdev::kListEvaluate, o_.max_iterations, list_c,
+                                     counts_ + 1, list_e, counts_ + 2, pos, nullptr);
+        });
+        const int ce = dev_count(2);
+        double* fm = fm_.as<double>(size_t(std::max(ce, 1)));
+        int64_t* ff = ff_.as<int64_t>(size_t(std::max(ce, 1)));
+        int64_t* fh = fh_.as<int64_t>(size_t(std::max(ce, 1)));
+        if (ce > 0) {
+            eng_.timer.run(kLmControl, [&] {
+                mko_lmdev::launch_gather(ds_, list_e, ce, true, rix, xs, tidx, nullptr);
+            });
+            eng_.merit_dev(dev_tmpl_.view, tidx, ce, xs, settings_, int(P_), fm, ff, fh);
+        }
+        eng_.timer.run(kLmControl, [&] {
+            mko_lmdev::launch_update(ds_, dopt_, list_c, cc, pos, fm, ff, fh, nullptr);
+        });
+        hip_check(hipGetLastError(), "LM cycle");
+        return true;
+    }
+
+    // Device state -> state_ (the fields finish_round and results read).
+    void dev_download_state() {
+        const size_t n = size_t(ds_.n);
+        if (n == 0) return;
+        std::vector<double> x(n * P_), merit(n), mu(n), gn(n), md(n);
+        std::vector<int64_t> failed(n), hard(n);
+        std::vector<int32_t> status(n), it(n), acc(n), jac(n);
+        auto get = [](void* h, const void* d, size_t b) {
+            hip_check(hipMemcpy(h, d, b, hipMemcpyDeviceToHost), "LM state");
+        };
+        get(x.data(), ds_.x, n * P_ * 8);
+        get(merit.data(), ds_.merit, n * 8);
+        get(mu.data(), ds_.mu, n * 8);
+        get(gn.data(), ds_.gradient_norm, n * 8);
+        get(md.data(), ds_.model_decrease, n * 8);
+        get(failed.data(), ds_.failed, n * 8);
+        get(hard.data(), ds_.hard, n * 8);
+        get(status.data(), ds_.status, n * 4);
+        get(it.data(), ds_.iterations, n * 4);
+        get(acc.data(), ds_.accepted, n * 4);
+        get(jac.data(), ds_.jacobians, n * 4);
+        for (size_t i = 0; i < n; ++i) {
+            Run& r = state_[i];
+            r.x.assign(x.begin() + std::ptrdiff_t(i * P_), x.begin() + std::ptrdiff_t((i + 1) * P_));
+            r.merit = merit[i];
+            r.mu = mu[i];
+            r.gradient_norm = gn[i];
+            r.model_decrease = md[i];
+            r.failed = failed[i];
+            r.hard = hard[i];
+            r.status = status[i];
+            r.iterations = it[i];
+            r.accepted = acc[i];
+            r.jacobians = jac[i];
+        }
+    }
+
     Engine& eng_;
     mko_batch base_;
     mko_residual_settings settings_;
@@ -1055,10 +1430,18 @@ private:
     std::vector<int> rounds_;
     int cycles_ = 0;
     bool done_ = false;
+    bool device_ = false;
+    mko_lmdev::State ds_{};
+    mko_lmdev::Options dopt_{};
+    DeviceBatch dev_tmpl_;
+    Buffer state_mem_, lists_mem_, round_mem_, bounds_d_, xs_d_, tidx_d_, ac_, gc_, fm_, ff_, fh_;
+    int32_t* lists_ = nullptr;
+    int32_t* counts_ = nullptr;
+    int32_t* round_index_ = nullptr;
     std::vector<double> xs_, a_, g_, f_;
     std::vector<int64_t> fl_, hf_;
-    mko_batch gathered_{};
-    std::vector<double> g_curv_, g_thick_, g_index_, g_pupil_, g_fa_, g_ft_, g_rc_;
+    GatheredBatch gathered_batch_;
+    const mko_batch* gathered_ = nullptr;
 };
 
 } // namespace mko_host
@@ -1101,6 +1484,7 @@ void mko_engine_destroy(mko_engine* e) { delete e; }
 int mko_engine_kernel_timing(mko_engine* e, int enable, double* ms, int64_t* launches) {
     MKO_HOST_TRY({
         KernelTimer& t = e->engine.timer;
+        t.collect();
         if (ms)
             for (int i = 0; i < kFamilies; ++i) ms[i] = t.ms[i];
         if (launches)
@@ -1146,6 +1530,47 @@ int mko_lm_results(mko_lm* lm, double* x, mko_lm_run* out, int* cycles) {
 }
 
 void mko_lm_destroy(mko_lm* lm) { delete lm; }
+
+int mko_engine_normal_equations(mko_engine* e, const mko_batch* base,
+                                const mko_residual_settings* s, const mko_variables* v, int count,
+                                const int32_t* runs, const double* x, int method, int precision,
+                                double* a, double* g) {
+    MKO_HOST_TRY({
+        if (count < 0) throw std::invalid_argument("negative run count");
+        if (count == 0) return 0;
+        check_batch(*base);
+        GatheredBatch gb;
+        const mko_batch* b = base;
+        if (base->variants != 1) {
+            if (runs) b = &gb.gather(*base, runs, size_t(count));
+            else if (base->variants != count)
+                throw std::invalid_argument("base must have one variant, one per run, or runs given");
+        }
+        e->engine.jacobian(*b, *s, *v, count, x, method, precision, nullptr, nullptr, a, g);
+        return 0;
+    })
+}
+
+int mko_engine_point_merit(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
+                           const mko_variables* v, int count, const int32_t* runs,
+                           const double* x, double* merit, int64_t* failed, int64_t* hard) {
+    MKO_HOST_TRY({
+        if (count < 0) throw std::invalid_argument("negative run count");
+        if (count == 0) return 0;
+        check_batch(*base);
+        if (v->count < 1) throw std::invalid_argument("no variables");
+        GatheredBatch gb;
+        const mko_batch* b = base;
+        if (base->variants != 1) {
+            if (runs) b = &gb.gather(*base, runs, size_t(count));
+            else if (base->variants != count)
+                throw std::invalid_argument("base must have one variant, one per run, or runs given");
+        }
+        validate_values(*b, 0, b->variants);
+        e->engine.point_merit(*b, *s, *v, count, x, merit, failed, hard);
+        return 0;
+    })
+}
 
 int mko_optimize_lm(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
                     const mko_variables* v, int runs, const double* x0, const mko_lm_options* o,

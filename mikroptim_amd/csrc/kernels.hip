@@ -26,6 +26,7 @@
 
 #include "error.hpp"
 #include "gauss_legendre_table.h"
+#include "internal.hpp"
 #include "mikroptim_hip.h"
 #include "trace_device.hpp"
 
@@ -381,14 +382,15 @@ struct ConstBatchPtrs {
 };
 
 __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, int per_run,
-                                   int template_per_run, int S, int L, int F, int P,
-                                   const int* var_type, const int* var_surface, const double* x0,
-                                   const double* plus, const double* minus) {
+                                   const int32_t* template_index, int template_per_run, int S,
+                                   int L, int F, int P, const int* var_type,
+                                   const int* var_surface, const double* x0, const double* plus,
+                                   const double* minus) {
     const int vv = blockIdx.x; // destination variant
     const int run = vv / per_run;
     const int j = vv - run * per_run;
     if (run >= runs) return;
-    const int from = template_per_run ? run : 0;
+    const int from = template_index ? template_index[run] : template_per_run ? run : 0;
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
         dst.curvature[size_t(vv) * S + i] = src.curvature[size_t(from) * S + i];
         dst.thickness[size_t(vv) * S + i] = src.thickness[size_t(from) * S + i];
@@ -697,8 +699,9 @@ __device__ __forceinline__ void backward_to_lds(const TraceParams& P, const Surf
 // One workgroup per (run, tuple), 256 lanes; tuples of at most 512 rays in halves of 256 rays.
 // Per half: forward trace in double (states recorded in RR), reverse sweep in RB, gradients
 // staged in LDS as a [P][256] block (pilot-relative floats); then the shifted moments of the
-// half: C on MFMA (v_mfma_f32_32x32x16_bf16, u split into bf16 hi + lo, hi*hi + hi*lo + lo*hi,
-// each of the 4 waves one 32 x 32 tile of the padded 64 x 64 C), S and Tr on VALU in double.
+// half: C on MFMA (v_mfma_f32_32x32x16_bf16, u split into bf16 hi + lo, hi*hi + hi*lo + lo*hi;
+// waves 0, 1, 3 form the 32 x 32 tiles (0,0), (0,1), (1,1) of the padded 64 x 64 C, the tile
+// below the diagonal is its mirror), S and Tr on VALU in double.
 template <class RB, class RR, int kHalves>
 __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, ParamMap map,
                                                                 float* partC, double* partST,
@@ -730,6 +733,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
     const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
     const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
     const int I = wave >> 1, Jt = wave & 1; // C tile of this wave
+    const bool tile_needed = I <= Jt && I * 32 < Pn && Jt * 32 < Pn;
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
@@ -817,8 +821,9 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
                 Tacc[p] += tp;
             }
         }
-        // C += u u^T on MFMA: operand lane = (m = lane % 32, kb = lane / 32), 8 rays per operand
-        {
+        // C += u u^T on MFMA: operand lane = (m = lane % 32, kb = lane / 32), 8 rays per operand.
+        // Only the tiles on and above the diagonal are formed (C is symmetric).
+        if (tile_needed) {
             const int m = lane & 31, kb = lane >> 5;
             const int pa = I * 32 + m, pb = Jt * 32 + m;
             const bool va = pa < Pn, vb = pb < Pn;
@@ -854,7 +859,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
     const double sum = block_sum(swo, red);
     const size_t tuple = size_t(blockIdx.x);
     // C tile out: lane holds column n = lane % 32 of rows 8 (e / 4) + 4 (lane / 32) + e % 4
-    {
+    if (tile_needed) {
         float* C = partC + tuple * size_t(Pn) * Pn;
         const int col = Jt * 32 + (lane & 31);
 #pragma unroll
@@ -888,7 +893,8 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
             const TupleNormal tn = partT[tuple];
             if (!(tn.w_ok > 0.0)) continue;
             const double* S = partST + tuple * 2 * Pn;
-            const double c = double(partC[tuple * size_t(Pn) * Pn + size_t(q) * Pn + p]);
+            const int lo = q < p ? q : p, hi = q < p ? p : q; // C is stored above the diagonal
+            const double c = double(partC[tuple * size_t(Pn) * Pn + size_t(lo) * Pn + hi]);
             accv += tn.s2 * (c - S[p] * S[q] / tn.w_ok);
         }
         a[size_t(v) * Pn * Pn + e] = accv;
@@ -1126,6 +1132,31 @@ void launch_adjoint_rpl(int rpl, const TraceParams& P, const ParamMap& m, TJ* J,
 
 } // namespace
 
+namespace mko_internal {
+
+int point_batch(const mko_batch* templ, const int32_t* template_index, int template_per_run,
+                int runs, int per_run, const mko_variables* vars_device, const double* x0,
+                const double* plus, const double* minus, double* curvature, double* thickness,
+                double* index, double* pupil_radius, double* field_angle, double* field_target,
+                double* reference_curvature, void* stream) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        const ConstBatchPtrs src{templ->curvature, templ->thickness, templ->index,
+                                 templ->pupil_radius, templ->field_angle, templ->field_target,
+                                 templ->reference_curvature};
+        const BatchPtrs dst{curvature, thickness, index, pupil_radius, field_angle, field_target,
+                            reference_curvature};
+        hipLaunchKernelGGL(point_batch_kernel, dim3(unsigned(runs) * unsigned(per_run)), dim3(64),
+                           0, static_cast<hipStream_t>(stream), src, dst, runs, per_run,
+                           template_index, template_per_run, templ->surfaces, templ->wavelengths,
+                           templ->fields, vars_device->count, vars_device->type,
+                           vars_device->batch_surface, x0, plus, minus);
+        check_hip(hipGetLastError(), "point batch launch");
+    })
+}
+
+} // namespace mko_internal
+
 extern "C" {
 
 const char* mko_last_error(void) { return mko_detail::last_error.c_str(); }
@@ -1165,20 +1196,9 @@ int mko_point_batch_device(const mko_batch* templ, int template_per_run, int run
                            double* thickness, double* index, double* pupil_radius,
                            double* field_angle, double* field_target,
                            double* reference_curvature, void* stream) {
-    MKO_TRY({
-        if (runs <= 0) return 0;
-        const ConstBatchPtrs src{templ->curvature, templ->thickness, templ->index,
-                                 templ->pupil_radius, templ->field_angle, templ->field_target,
-                                 templ->reference_curvature};
-        const BatchPtrs dst{curvature, thickness, index, pupil_radius, field_angle, field_target,
-                            reference_curvature};
-        hipLaunchKernelGGL(point_batch_kernel, dim3(unsigned(runs) * unsigned(per_run)), dim3(64),
-                           0, static_cast<hipStream_t>(stream), src, dst, runs, per_run,
-                           template_per_run, templ->surfaces, templ->wavelengths, templ->fields,
-                           vars_device->count, vars_device->type, vars_device->batch_surface, x0,
-                           plus, minus);
-        check_hip(hipGetLastError(), "point batch launch");
-    })
+    return mko_internal::point_batch(templ, nullptr, template_per_run, runs, per_run, vars_device,
+                                     x0, plus, minus, curvature, thickness, index, pupil_radius,
+                                     field_angle, field_target, reference_curvature, stream);
 }
 
 int mko_fd_difference_device(const double* fd_residuals, int64_t stride,

@@ -1,0 +1,517 @@
+// Device side of the batched Levenberg-Marquardt control (levenberg_marquardt,
+// src/optim/levenberg_marquardt.cpp:332-572): the per-run step solve, acceptance and convergence
+// tests of one LM cycle run on the GPU, one wavefront (64 lanes) per run, so a cycle needs no
+// host round trip of the normal equations or the trial points.
+//
+// Every floating-point operation is the one the host control performs, in the same order:
+// products and sums without contraction (fp contract off below), correctly rounded division and
+// square root (__ddiv_rn, __dsqrt_rn), std::max / std::min as (a < b) ? b : a / (b < a) ? b : a.
+// Given the same normal equations and merits, device and host control take bit-identical
+// trajectories. Sums keep the host order: the Cholesky factor and the forward substitution run
+// lane-parallel over rows with the inner sums in ascending order per lane; the back substitution
+// and the step model (lin, quad) run on lane 0.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+
+#include "lm_device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mko_lmdev {
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kMaxP = kMaxVariables;
+constexpr double kModelDamping = 1e-12;
+constexpr double kFdStepFactor = 6.0554544523933395e-06; // 2^(-52/3)
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double scale_of(double x, double typical) { return dmax(fabs(x), typical); }
+
+__device__ __forceinline__ bool blocks(const State& S, int k, double x, double dir) {
+    if (S.lower[k] == S.upper[k]) return true;
+    return (x <= S.lower[k] && dir < 0.0) || (x >= S.upper[k] && dir > 0.0);
+}
+
+struct SolveLds {
+    double m[kMaxP * kMaxP]; // lower triangle of the free block, column-major m[j * f + i]
+    double rhs[kMaxP];
+    double delta[kMaxP];
+    double step[kMaxP];
+    double aq[kMaxP];
+    double ratio[kMaxP];
+    int idx[kMaxP];
+    uint8_t fixed[kMaxP];
+    double diag;
+    int f;
+};
+
+// solve_step: (A_FF + mu D_FF) delta_F = -g_F on the free variables with the active set; writes
+// trial, step_norm, predicted and solved of the run (trial, step_norm and predicted only when
+// the factorisation succeeds, like the host).
+// a, g: the run's normal equations (row-major a[q * P + p]).
+__device__ void solve_step(const State& S, int run, const double* a, const double* g, double mu,
+                           SolveLds& L) {
+    const int n = S.P;
+    const int lane = threadIdx.x;
+    const double* x = S.x + size_t(run) * n;
+    const double* d = S.d + size_t(run) * n;
+    if (lane < n) L.fixed[lane] = blocks(S, lane, x[lane], -g[lane]) ? 1 : 0;
+    __syncthreads();
+    for (int round = 0; round <= n; ++round) {
+        if (lane == 0) {
+            int f = 0;
+            for (int k = 0; k < n; ++k)
+                if (!L.fixed[k]) L.idx[f++] = k;
+            L.f = f;
+        }
+        if (lane < n) L.delta[lane] = 0.0;
+        __syncthreads();
+        const int f = L.f;
+        if (f == 0) break;
+        for (int j = 0; j < f; ++j) {
+            const int i = lane;
+            if (i >= j && i < f) {
+                double v = a[size_t(L.idx[j]) * n + L.idx[i]];
+                if (i == j) v += mu * d[L.idx[j]];
+                L.m[j * f + i] = v;
+            }
+        }
+        if (lane < f) L.rhs[lane] = -g[L.idx[lane]];
+        __syncthreads();
+        // Cholesky, column j: every lane i >= j forms its sum over k < j in ascending order
+        for (int j = 0; j < f; ++j) {
+            const int i = lane;
+            double s = 0.0;
+            if (i >= j && i < f) {
+                s = L.m[j * f + i];
+                for (int k = 0; k < j; ++k) s -= L.m[k * f + i] * L.m[k * f + j];
+            }
+            if (i == j) L.diag = s;
+            __syncthreads();
+            const double diag = L.diag;
+            if (!(diag > 0.0) || !isfinite(diag)) {
+                if (lane == 0) S.solved[run] = 0;
+                return;
+            }
+            const double l = __dsqrt_rn(diag);
+            if (i == j) L.m[j * f + j] = l;
+            else if (i > j && i < f) L.m[j * f + i] = __ddiv_rn(s, l);
+            __syncthreads();
+        }
+        // forward substitution: lane i subtracts l(i, k) b_k for k = 0, 1, ... as b_k is final
+        {
+            double acc = lane < f ? L.rhs[lane] : 0.0;
+            for (int k = 0; k < f; ++k) {
+                if (lane == k) L.rhs[k] = __ddiv_rn(acc, L.m[k * f + k]);
+                __syncthreads();
+                if (lane > k && lane < f) acc -= L.m[k * f + lane] * L.rhs[k];
+            }
+        }
+        // back substitution on lane 0 (each sum in ascending k, as on the host)
+        if (lane == 0) {
+            for (int i = f - 1; i >= 0; --i) {
+                double s = L.rhs[i];
+                for (int k = i + 1; k < f; ++k) s -= L.m[i * f + k] * L.rhs[k];
+                L.rhs[i] = __ddiv_rn(s, L.m[i * f + i]);
+            }
+        }
+        __syncthreads();
+        if (lane < f) L.delta[L.idx[lane]] = L.rhs[lane];
+        __syncthreads();
+        bool newly_fixed = false;
+        if (lane < f) {
+            const int k = L.idx[lane];
+            if (blocks(S, k, x[k], L.delta[k])) {
+                L.fixed[k] = 1;
+                newly_fixed = true;
+            }
+        }
+        const bool changed = __any(newly_fixed);
+        __syncthreads();
+        if (!changed) break;
+    }
+    double* trial = S.trial + size_t(run) * n;
+    if (lane < n) {
+        const int k = lane;
+        const double t = dmin(dmax(x[k] + L.delta[k], S.lower[k]), S.upper[k]);
+        trial[k] = t;
+        L.step[k] = t - x[k];
+        L.ratio[k] = __ddiv_rn(fabs(L.step[k]), scale_of(x[k], S.typical[k]));
+    }
+    __syncthreads();
+    if (lane < n) {
+        const int q = lane;
+        double aq = 0.0;
+        for (int p = 0; p < n; ++p) aq += a[size_t(q) * n + p] * L.step[p];
+        L.aq[q] = aq;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double nrm = 0.0;
+        for (int k = 0; k < n; ++k) nrm = dmax(nrm, L.ratio[k]);
+        double lin = 0.0, quad = 0.0;
+        for (int q = 0; q < n; ++q) {
+            lin += L.step[q] * g[q];
+            quad += L.step[q] * L.aq[q];
+        }
+        S.step_norm[run] = nrm;
+        S.predicted[run] = -(2.0 * lin + quad);
+        S.solved[run] = 1;
+    }
+    __syncthreads();
+}
+
+// prepare_step: scaling diagonal D (Marquardt with floor) and the scaled gradient norm.
+__device__ void prepare_step(const State& S, const Options& o, int run, const double* a,
+                             const double* g) {
+    const int n = S.P;
+    const double* x = S.x + size_t(run) * n;
+    double* d = S.d + size_t(run) * n;
+    double smax = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double sk = scale_of(x[k], S.typical[k]);
+        smax = dmax(smax, a[size_t(k) * n + k] * sk * sk);
+    }
+    const double merit = S.merit[run];
+    double gmax = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double sk = scale_of(x[k], S.typical[k]);
+        const double akk = a[size_t(k) * n + k];
+        const double floor = smax > 0.0 ? __ddiv_rn(o.diag_floor * smax, sk * sk)
+                                        : __ddiv_rn(1.0, sk * sk);
+        d[k] = dmax(akk, floor);
+        if (!blocks(S, k, x[k], -g[k])) {
+            const double den = __dsqrt_rn(akk * merit);
+            if (den > 0.0) gmax = dmax(gmax, __ddiv_rn(fabs(g[k]), den));
+        }
+    }
+    S.gradient_norm[run] = gmax;
+}
+
+__global__ __launch_bounds__(256) void init_kernel(State S, Options o) {
+    const int run = blockIdx.x * blockDim.x + threadIdx.x;
+    if (run >= S.n) return;
+    const int n = S.P;
+    for (int k = 0; k < n; ++k) {
+        S.trial[size_t(run) * n + k] = S.x[size_t(run) * n + k];
+        S.g[size_t(run) * n + k] = 0.0;
+        S.d[size_t(run) * n + k] = 0.0;
+    }
+    S.mu[run] = o.initial_damping;
+    S.nu[run] = 2.0;
+    S.step_norm[run] = 0.0;
+    S.predicted[run] = 0.0;
+    S.gradient_norm[run] = 0.0;
+    S.model_decrease[run] = __builtin_nan("");
+    S.failed[run] = 0;
+    S.hard[run] = 0;
+    S.status[run] = MKO_LM_RUNNING;
+    S.iterations[run] = 0;
+    S.accepted[run] = 0;
+    S.jacobians[run] = 0;
+    S.history_len[run] = 0;
+    S.need_jacobian[run] = 1;
+    S.rejected[run] = 0;
+    S.solved[run] = 0;
+    S.evaluate[run] = 0;
+}
+
+__device__ __forceinline__ void push_history(const State& S, int run, double merit) {
+    const int c = S.history_len[run];
+    S.history[size_t(run) * S.H + (c % S.H)] = merit;
+    S.history_len[run] = c + 1;
+}
+
+__global__ __launch_bounds__(256) void start_kernel(State S, const double* merit,
+                                                    const int64_t* failed, const int64_t* hard) {
+    const int run = blockIdx.x * blockDim.x + threadIdx.x;
+    if (run >= S.n) return;
+    const double m = merit[run];
+    S.merit[run] = m;
+    S.failed[run] = failed[run];
+    S.hard[run] = hard[run];
+    push_history(S, run, m);
+    if (!isfinite(m)) S.status[run] = MKO_LM_NUMERICAL_ERROR;
+    else if (m == 0.0) S.status[run] = MKO_LM_CONVERGED_MERIT;
+}
+
+// Ordered compaction in one workgroup of 1024 threads (chunks of 1024 runs, block scan).
+__global__ __launch_bounds__(1024) void select_kernel(State S, int kind, int max_iterations,
+                                                      const int32_t* in, const int32_t* in_count,
+                                                      int32_t* list, int32_t* count, int32_t* pos) {
+    __shared__ int32_t wave_total[16];
+    __shared__ int32_t base_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int total = kind == kListEvaluate ? *in_count : S.n;
+    if (tid == 0) base_s = 0;
+    __syncthreads();
+    for (int start = 0; start < total; start += 1024) {
+        const int e = start + tid;
+        bool pick = false;
+        int run = -1;
+        if (e < total) {
+            run = kind == kListEvaluate ? in[e] : e;
+            const int st = S.status[run];
+            if (kind == kListJacobian) {
+                pick = st == MKO_LM_RUNNING && S.need_jacobian[run] &&
+                       S.iterations[run] < max_iterations;
+            } else if (kind == kListCycle) {
+                if (st == MKO_LM_RUNNING) {
+                    if (S.iterations[run] >= max_iterations) S.status[run] = MKO_LM_MAX_ITERATIONS;
+                    else pick = true;
+                }
+            } else {
+                pick = S.evaluate[run] != 0;
+            }
+        }
+        const uint64_t mask = __ballot(pick);
+        const int before = __popcll(mask & ((uint64_t(1) << lane) - 1));
+        if (lane == 0) wave_total[wave] = __popcll(mask);
+        __syncthreads();
+        int offset = base_s;
+        for (int w = 0; w < wave; ++w) offset += wave_total[w];
+        if (pick) list[offset + before] = run;
+        if (pos && run >= 0) pos[run] = pick ? offset + before : -1;
+        __syncthreads();
+        if (tid == 0) {
+            int sum = 0;
+            for (int w = 0; w < 16; ++w) sum += wave_total[w];
+            base_s += sum;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *count = base_s;
+}
+
+__global__ __launch_bounds__(64) void gather_kernel(State S, const int32_t* list, int count,
+                                                    int trial, const int32_t* round_index,
+                                                    double* xs, int32_t* tidx) {
+    const int j = blockIdx.x;
+    if (j >= count) return;
+    const int run = list[j];
+    const double* src = (trial ? S.trial : S.x) + size_t(run) * S.P;
+    for (int k = threadIdx.x; k < S.P; k += 64) xs[size_t(j) * S.P + k] = src[k];
+    if (threadIdx.x == 0 && tidx) tidx[j] = round_index ? round_index[run] : run;
+}
+
+// One wavefront per listed run (round_cycle, the runs that received new normal equations).
+__global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, const int32_t* list,
+                                                         int count, const double* a_c,
+                                                         const double* g_c) {
+    __shared__ SolveLds L;
+    __shared__ int finite_s;
+    const int j = blockIdx.x;
+    if (j >= count) return;
+    const int run = list[j];
+    const int n = S.P, lane = threadIdx.x;
+    bool finite = true;
+    for (int e = lane; e < n * n; e += kLanes) {
+        const double v = a_c[size_t(j) * n * n + e];
+        S.a[size_t(run) * n * n + e] = v;
+        finite = finite && isfinite(v);
+    }
+    if (lane < n) {
+        const double v = g_c[size_t(j) * n + lane];
+        S.g[size_t(run) * n + lane] = v;
+        finite = finite && isfinite(v);
+    }
+    const bool all_finite = __all(finite);
+    if (lane == 0) {
+        S.need_jacobian[run] = 0;
+        S.jacobians[run] += 1;
+        finite_s = all_finite ? 1 : 0;
+        if (!all_finite) S.status[run] = MKO_LM_NUMERICAL_ERROR;
+    }
+    __syncthreads();
+    if (!finite_s) return;
+    // the compact inputs are read-only in this kernel; the state copies serve later kernels
+    const double* a = a_c + size_t(j) * n * n;
+    const double* g = g_c + size_t(j) * n;
+    if (lane == 0) prepare_step(S, o, run, a, g);
+    __syncthreads();
+    solve_step(S, run, a, g, kModelDamping, L);
+    if (lane == 0) {
+        const bool solved = S.solved[run] != 0;
+        const double merit = S.merit[run];
+        const double predicted = S.predicted[run];
+        S.model_decrease[run] = solved ? __ddiv_rn(predicted, merit) : __builtin_nan("");
+        if (S.gradient_norm[run] <= o.gtol) S.status[run] = MKO_LM_CONVERGED_GRADIENT;
+        else if (solved && predicted >= 0.0 && predicted <= o.ftol * merit)
+            S.status[run] = MKO_LM_CONVERGED_MERIT;
+        else if (solved && S.step_norm[run] <= o.xtol) S.status[run] = MKO_LM_CONVERGED_STEP;
+    }
+}
+
+__global__ __launch_bounds__(kLanes) void step_kernel(State S, Options o, const int32_t* list,
+                                                      int count) {
+    __shared__ SolveLds L;
+    const int j = blockIdx.x;
+    if (j >= count) return;
+    const int run = list[j];
+    solve_step(S, run, S.a + size_t(run) * S.P * S.P, S.g + size_t(run) * S.P, S.mu[run], L);
+    if (threadIdx.x == 0) {
+        const bool solved = S.solved[run] != 0;
+        S.evaluate[run] = 0;
+        if (solved && S.rejected[run] && S.step_norm[run] <= o.xtol)
+            S.status[run] = MKO_LM_CONVERGED_STEP;
+        else if (solved)
+            S.evaluate[run] = 1;
+    }
+}
+
+__global__ __launch_bounds__(256) void update_kernel(State S, Options o, const int32_t* list,
+                                                     int count, const int32_t* pos,
+                                                     const double* f, const int64_t* failed,
+                                                     const int64_t* hard) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const int run = list[j];
+    if (S.status[run] == MKO_LM_CONVERGED_STEP) return;
+    S.iterations[run] += 1;
+    bool accept = false;
+    const double merit = S.merit[run];
+    if (S.solved[run]) {
+        const int e = pos[run];
+        const double f_new = f[e];
+        const int64_t failed_new = failed[e], hard_new = hard[e];
+        const double predicted = S.predicted[run];
+        const double rho = __ddiv_rn(merit - f_new, predicted);
+        accept = isfinite(f_new) && predicted > 0.0 && f_new < merit &&
+                 !(o.reject_new_failures && hard_new > S.hard[run]);
+        if (accept) {
+            const int n = S.P;
+            for (int k = 0; k < n; ++k) S.x[size_t(run) * n + k] = S.trial[size_t(run) * n + k];
+            S.merit[run] = f_new;
+            S.failed[run] = failed_new;
+            S.hard[run] = hard_new;
+            const double t = 2.0 * rho - 1.0;
+            S.mu[run] = S.mu[run] * dmax(1.0 / 3.0, 1.0 - t * t * t);
+            S.nu[run] = 2.0;
+            S.need_jacobian[run] = 1;
+            S.rejected[run] = 0;
+            S.accepted[run] += 1;
+            if (f_new == 0.0) S.status[run] = MKO_LM_CONVERGED_MERIT;
+        }
+    }
+    if (!accept) {
+        S.mu[run] = S.mu[run] * S.nu[run];
+        S.nu[run] = S.nu[run] * 2.0;
+        S.rejected[run] = 1;
+        if (S.mu[run] > o.max_damping) S.status[run] = MKO_LM_DAMPING_LIMIT;
+    }
+    const double now = S.merit[run];
+    push_history(S, run, now);
+    if (o.stall_window > 0 && S.status[run] == MKO_LM_RUNNING) {
+        const int w = o.stall_window;
+        const int size = S.history_len[run];
+        if (size > w) {
+            const double old = S.history[size_t(run) * S.H + ((size - 1 - w) % S.H)];
+            if (old - now <= o.stall_tolerance * old) S.status[run] = MKO_LM_CONVERGED_STALL;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void fd_pattern_kernel(int count, int P, const double* xs,
+                                                        const double* lower, const double* upper,
+                                                        const double* typical, double* plus,
+                                                        double* minus, double* step_plus,
+                                                        double* step_minus, uint8_t* stencil) {
+    const int j = blockIdx.x;
+    if (j >= count) return;
+    for (int k = threadIdx.x; k < P; k += 64) {
+        const size_t at = size_t(j) * P + k;
+        const double x = xs[at];
+        const double lo = lower[k], hi = upper[k];
+        const double h = kFdStepFactor * dmax(fabs(x), typical[k]);
+        const double up = x + h, down = x - h;
+        double xa, xb;
+        uint8_t sten = MKO_STENCIL_CENTRAL;
+        if (up <= hi && down >= lo) {
+            xa = up;
+            xb = down;
+        } else {
+            const bool towards_upper = hi - x >= x - lo;
+            const double far = towards_upper ? x + 2.0 * h : x - 2.0 * h;
+            if (towards_upper ? far <= hi : far >= lo) {
+                sten = MKO_STENCIL_ONE_SIDED;
+                xa = towards_upper ? up : down;
+                xb = far;
+            } else {
+                sten = MKO_STENCIL_TRUNCATED;
+                xa = up <= hi ? up : hi;
+                xb = down >= lo ? down : lo;
+            }
+        }
+        plus[at] = xa;
+        minus[at] = xb;
+        step_plus[at] = xa - x;
+        step_minus[at] = xb - x;
+        stencil[at] = sten;
+    }
+}
+
+inline hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
+inline unsigned blocks_of(int n, int per) { return unsigned((n + per - 1) / per); }
+
+} // namespace
+
+void launch_init(const State& s, const Options& o, void* stream) {
+    if (s.n > 0) hipLaunchKernelGGL(init_kernel, dim3(blocks_of(s.n, 256)), dim3(256), 0, st(stream), s, o);
+}
+
+void launch_start(const State& s, const double* merit, const int64_t* failed, const int64_t* hard,
+                  void* stream) {
+    if (s.n > 0)
+        hipLaunchKernelGGL(start_kernel, dim3(blocks_of(s.n, 256)), dim3(256), 0, st(stream), s,
+                           merit, failed, hard);
+}
+
+void launch_select(const State& s, int kind, int max_iterations, const int32_t* in,
+                   const int32_t* in_count, int32_t* list, int32_t* count, int32_t* pos,
+                   void* stream) {
+    hipLaunchKernelGGL(select_kernel, dim3(1), dim3(1024), 0, st(stream), s, kind, max_iterations,
+                       in, in_count, list, count, pos);
+}
+
+void launch_gather(const State& s, const int32_t* list, int count, bool trial,
+                   const int32_t* round_index, double* xs, int32_t* tidx, void* stream) {
+    if (count > 0)
+        hipLaunchKernelGGL(gather_kernel, dim3(unsigned(count)), dim3(64), 0, st(stream), s, list,
+                           count, trial ? 1 : 0, round_index, xs, tidx);
+}
+
+void launch_prepare(const State& s, const Options& o, const int32_t* list, int count,
+                    const double* a_c, const double* g_c, void* stream) {
+    if (count > 0)
+        hipLaunchKernelGGL(prepare_kernel, dim3(unsigned(count)), dim3(kLanes), 0, st(stream), s, o,
+                           list, count, a_c, g_c);
+}
+
+void launch_step(const State& s, const Options& o, const int32_t* list, int count, void* stream) {
+    if (count > 0)
+        hipLaunchKernelGGL(step_kernel, dim3(unsigned(count)), dim3(kLanes), 0, st(stream), s, o,
+                           list, count);
+}
+
+void launch_update(const State& s, const Options& o, const int32_t* list, int count,
+                   const int32_t* pos, const double* f, const int64_t* failed,
+                   const int64_t* hard, void* stream) {
+    if (count > 0)
+        hipLaunchKernelGGL(update_kernel, dim3(blocks_of(count, 256)), dim3(256), 0, st(stream), s,
+                           o, list, count, pos, f, failed, hard);
+}
+
+void launch_fd_pattern(int count, int P, const double* xs, const double* lower,
+                       const double* upper, const double* typical, double* plus, double* minus,
+                       double* step_plus, double* step_minus, uint8_t* stencil, void* stream) {
+    if (count > 0)
+        hipLaunchKernelGGL(fd_pattern_kernel, dim3(unsigned(count)), dim3(64), 0, st(stream), count,
+                           P, xs, lower, upper, typical, plus, minus, step_plus, step_minus,
+                           stencil);
+}
+
+} // namespace mko_lmdev

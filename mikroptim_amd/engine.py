@@ -26,7 +26,8 @@ EXPORTS = [
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
     "mko_adjoint_normal_device", "mko_gram_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
-    "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
+    "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
+    "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
     "mko_lm_results", "mko_lm_destroy",
 ]
 
@@ -73,6 +74,10 @@ def library():
         "mko_engine_jacobian": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
                                  _dp, _dp], C.c_int),
         "mko_engine_kernel_timing": ([C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
+        "mko_engine_normal_equations": ([C.c_void_p, bp, sp, vp, C.c_int, C.POINTER(C.c_int32),
+                                         _dp, C.c_int, C.c_int, _dp, _dp], C.c_int),
+        "mko_engine_point_merit": ([C.c_void_p, bp, sp, vp, C.c_int, C.POINTER(C.c_int32), _dp,
+                                    _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
         "mko_optimize_lm": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions),
                              C.c_int, _dp, C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
         "mko_lm_create": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions),
@@ -161,13 +166,13 @@ class TraceEngine:
     def __del__(self):
         self.close()
 
-    KERNEL_FAMILIES = ("trace", "adjoint", "gram", "fd_difference", "point_batch")
+    KERNEL_FAMILIES = ("trace", "adjoint", "gram", "fd_difference", "point_batch", "lm_control")
 
     def kernel_timing(self, enable=True):
         """Device ms and launch counts per kernel family since the last call (HIP events on the
         launch stream); then resets and enables/disables timing."""
-        ms = np.zeros(5)
-        n = np.zeros(5, dtype=np.int64)
+        ms = np.zeros(len(self.KERNEL_FAMILIES))
+        n = np.zeros(len(self.KERNEL_FAMILIES), dtype=np.int64)
         _check(library().mko_engine_kernel_timing(self._h, int(enable), abi.ptr(ms),
                                                   abi.ptr(n, C.c_int64)))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_FAMILIES)}
@@ -213,6 +218,35 @@ class TraceEngine:
                                              abi.ptr(x0), _METHOD[method], _PRECISION[precision],
                                              abi.ptr(J), abi.ptr(r0), abi.ptr(a), abi.ptr(g)))
         return J, r0, a, g
+
+    def normal_equations(self, base, settings, variables, x, runs=None, method="adjoint",
+                         precision="fp64"):
+        """LmBackend::normal_equations: (A [n][P][P], g [n][P]) at the points x [n][P] of the
+        runs `runs` (indices into a per-run base; None = 0..n-1)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        n, P = x.shape
+        r = None if runs is None else np.ascontiguousarray(runs, dtype=np.int32)
+        a = np.zeros((n, P, P))
+        g = np.zeros((n, P))
+        b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
+        _check(library().mko_engine_normal_equations(
+            self._h, C.byref(b), C.byref(s), C.byref(v), n, abi.ptr(r, C.c_int32), abi.ptr(x),
+            _METHOD[method], _PRECISION[precision], abi.ptr(a), abi.ptr(g)))
+        return a, g
+
+    def point_merit(self, base, settings, variables, x, runs=None):
+        """LmBackend::merit: (merit, failed, hard_failed) [n] at the points x [n][P]."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        n, P = x.shape
+        r = None if runs is None else np.ascontiguousarray(runs, dtype=np.int32)
+        m = np.zeros(n)
+        f = np.zeros(n, dtype=np.int64)
+        h = np.zeros(n, dtype=np.int64)
+        b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
+        _check(library().mko_engine_point_merit(
+            self._h, C.byref(b), C.byref(s), C.byref(v), n, abi.ptr(r, C.c_int32), abi.ptr(x),
+            abi.ptr(m), abi.ptr(f, C.c_int64), abi.ptr(h, C.c_int64)))
+        return m, f, h
 
     def optimize(self, base, settings, variables, x0, options=None, precision="fp64"):
         options = options or abi.lm_options()
