@@ -28,7 +28,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--fd-steps", type=int, default=3)
+    ap.add_argument("--fd-steps", type=int, default=None,
+                    help="timed FD cycles (default: --steps, after the same --warmup)")
     ap.add_argument("--variants", type=int, default=4096)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -42,6 +43,8 @@ def sync():
 
 
 def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps, threads):
+    """Times `steps` LM cycles after `warmup` cycles. Returns the timed cycle count, seconds,
+    device time per kernel family, the results and the Jacobians computed in the timed cycles."""
     from mikroptim_amd import abi
 
     opt = abi.lm_options(max_iterations=warmup + steps + 5, threads=threads,
@@ -50,6 +53,7 @@ def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps,
     for _ in range(warmup):
         sess.cycle()
     sync()
+    jac0 = int(sum(r.jacobians for r in sess.results().runs))
     eng.kernel_timing(True)
     t0 = time.perf_counter()
     done = 0
@@ -62,7 +66,8 @@ def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps,
     kt = eng.kernel_timing(False)
     res = sess.results()
     sess.close()
-    return done, dt, kt, res
+    jacobians = int(sum(r.jacobians for r in res.runs)) - jac0
+    return done, dt, kt, res, jacobians
 
 
 def cpu_baseline(batch, variables, settings, x_gpu_final, eng, sample_runs=64, threads=16):
@@ -113,12 +118,13 @@ def main():
     x0 = variables.extract(batch)
     eng = TraceEngine(0)
 
-    n_adj, t_adj, kt_adj, res_adj = run_lm(eng, base, settings, variables, x0,
+    n_adj, t_adj, kt_adj, res_adj, jac_adj = run_lm(eng, base, settings, variables, x0,
                                            abi.JACOBIAN_ADJOINT, "mixed", args.warmup,
                                            args.steps, args.threads)
     it_adj = n_adj / t_adj
-    n_fd, t_fd, kt_fd, res_fd = run_lm(eng, base, settings, variables, x0, abi.JACOBIAN_FD,
-                                       "fp64", 1, args.fd_steps, args.threads)
+    fd_steps = args.steps if args.fd_steps is None else args.fd_steps
+    n_fd, t_fd, kt_fd, res_fd, jac_fd = run_lm(eng, base, settings, variables, x0, abi.JACOBIAN_FD,
+                                       "fp64", args.warmup, fd_steps, args.threads)
     it_fd = n_fd / t_fd
 
     # roofline of the dominant kernel of the measured path (the adjoint kernel): algorithmic
@@ -129,20 +135,30 @@ def main():
     flop_fwd = 59 * (S - 1) + 60
     flop_adj = flop_fwd + 210 * (S - 1) + 6 * P
     adj_ms, adj_n = kt_adj["adjoint"]
-    runs_per_launch = args.variants  # every Jacobian call covers the runs needing one
     achieved = None
+    runs_per_launch = jac_adj / adj_n if adj_n else 0.0
     if adj_n:
-        # average runs per launch from the LM statistics is not exposed; the per-launch FLOP
-        # uses the runs that needed a Jacobian in the timed cycles
-        jac_runs = int(sum(r.jacobians for r in res_adj.runs))
-        per_launch_runs = jac_runs / max(1, (args.warmup + n_adj))
-        achieved = per_launch_runs * rays * (flop_fwd + flop_adj) / (adj_ms / adj_n * 1e-3) / 1e12
-        runs_per_launch = per_launch_runs
-    roof = {"bound": "valu", "kernel": "adjoint_kernel", "achieved": achieved,
-            "peak": 78.6, "unit": "TFLOP/s",
+        achieved = runs_per_launch * rays * (flop_fwd + flop_adj) / (adj_ms / adj_n * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "adjoint_normal_kernel (fused adjoint + normal equations)",
+            "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
             "frac": (achieved / 78.6) if achieved else None, "traffic": None,
             "avg_launch_ms": (adj_ms / adj_n) if adj_n else None,
-            "runs_per_launch": runs_per_launch}
+            "runs_per_launch": runs_per_launch,
+            "flop_per_ray": flop_fwd + flop_adj}
+
+    # per-result error of the mixed-precision normal equations against the FP64 stored-J path
+    # (itself within 1e-10 of the FP64 oracle) at 8 final points
+    xs = res_adj.x[:8]
+    _, _, am, gm = eng.jacobian(base, settings, variables, xs, method="adjoint",
+                                precision="mixed", want_jacobian=False)
+    _, _, ad, gd = eng.jacobian(base, settings, variables, xs, method="adjoint", precision="fp64")
+    mixed_err = {
+        "A_rel_frobenius": float(np.max(np.linalg.norm(am - ad, axis=(1, 2)) /
+                                        np.linalg.norm(ad, axis=(1, 2)))),
+        "A_diag_rel": float(np.max(np.abs(np.diagonal(am, axis1=1, axis2=2) /
+                                          np.diagonal(ad, axis1=1, axis2=2) - 1))),
+        "g_rel": float(np.max(np.linalg.norm(gm - gd, axis=1) / np.linalg.norm(gd, axis=1))),
+        "runs": 8}
 
     out = {
         "metric": "LM iter/s @4096 variants: adjoint+mixed vs FP64 FD; err <=1e-4 waves RMS",
@@ -155,6 +171,8 @@ def main():
                                "hard failure", "variants": args.variants, "variables": P,
                    "rays_per_variant": rays},
         "fd_fp64_iter_per_s": it_fd, "fd_steps": n_fd,
+        "jacobians_in_timed_cycles": {"adjoint_mixed": jac_adj, "fd_fp64": jac_fd},
+        "mixed_normal_equations_error_vs_fp64": mixed_err,
         "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
         "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
         "kernel_ms_fd_path": {k: v[0] for k, v in kt_fd.items()},
