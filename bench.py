@@ -146,6 +146,19 @@ def main():
             "runs_per_launch": runs_per_launch,
             "flop_per_ray": flop_fwd + flop_adj}
 
+    # forward trace throughput of the two number types (MERIT of all variants at the starts):
+    # FP64 is the trace of every mode; df64 is the reference's second number type
+    import copy
+    trace_rate = {}
+    for name, prec in (("fp64", abi.PRECISION_FP64), ("df64", abi.PRECISION_DF64)):
+        st = copy.copy(settings)
+        st.precision = prec
+        eng.merit(batch, st)
+        eng.kernel_timing(True)
+        eng.merit(batch, st)
+        ms, n = eng.kernel_timing(False)["trace"]
+        trace_rate[name] = batch.variants * rays / (ms * 1e-3) if ms > 0 else None
+
     # per-result error of the mixed-precision normal equations against the FP64 stored-J path
     # (itself within 1e-10 of the FP64 oracle) at 8 final points
     xs = res_adj.x[:8]
@@ -173,6 +186,7 @@ def main():
         "fd_fp64_iter_per_s": it_fd, "fd_steps": n_fd,
         "jacobians_in_timed_cycles": {"adjoint_mixed": jac_adj, "fd_fp64": jac_fd},
         "mixed_normal_equations_error_vs_fp64": mixed_err,
+        "merit_trace_rays_per_s": trace_rate,
         "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
         "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
         "kernel_ms_fd_path": {k: v[0] for k, v in kt_fd.items()},

@@ -27,6 +27,10 @@ const char* mko_last_error(void);
 
 /* ---- device layer ------------------------------------------------------------------------ */
 
+/* The RESIDUALS and MERIT entry points trace in settings->precision: MKO_PRECISION_FP64, or
+ * MKO_PRECISION_DF64 (Precision::Df64 of the reference: the ray trace in df64, OPL converted to
+ * double for the piston and the residuals); other values are rejected. */
+
 /* RESIDUALS mode: residual rows r_k (and, in soft mode, penalty rows after the ray rows) of
  * every variant at residuals[v * stride ...], OPD [V][rays] and status [V][rays] (both may be
  * NULL). Replaces residuals_kernel + DeviceResiduals::run (src/cuda/residuals.cu:58, :240;
@@ -95,6 +99,13 @@ int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_setting
  * (src/cuda/normal_equations.cu:48, :155). */
 int mko_gram_device(const double* jacobian, const double* r0, int runs, int variables,
                     int64_t rows, double* a, double* g, void* stream);
+
+/* df64 (double-float) arithmetic check: out[i] = a[i] op b[i] over n pairs (hi, lo) of floats in
+ * device memory ([n][2] each); op 0 add, 1 sub, 2 mul, 3 mul by the float b.hi, 4 div, 5 sqrt(a).
+ * The operations of the MKO_PRECISION_DF64 trace, for comparison with a host implementation
+ * (df64_check_kernel, src/cuda/df64_check.cu:17; CudaDf64.DeviceMatchesHostBitwise). */
+int mko_df64_ops_device(int op, int64_t n, const float* a, const float* b, float* out,
+                        void* stream);
 
 /* ---- host layer ----------------------------------------------------------------------------- */
 

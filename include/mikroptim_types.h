@@ -60,7 +60,7 @@ enum {
 /* Arithmetic of the device path. FP64: every kernel in double. MIXED: forward trace (OPL,
  * residuals, merit) in double, reverse sweep of the adjoint in float, JtJ on bf16x3 MFMA with
  * float accumulation, Jtr and the LM step in double. */
-enum { MKO_PRECISION_FP64 = 0, MKO_PRECISION_MIXED = 1 };
+enum { MKO_PRECISION_FP64 = 0, MKO_PRECISION_MIXED = 1, MKO_PRECISION_DF64 = 2 };
 
 /* One batch of V variants of one topology. `type`, `pupil_mode`, `wavelength_weight` and
  * `field_weight` are batch-wide and always live in host memory; the per-variant arrays are host
@@ -93,6 +93,9 @@ typedef struct mko_residual_settings {
     int32_t spokes; /* M (axis fields: 1) */
     int32_t failure; /* MKO_FAILURE_* */
     double fail_weight; /* w_fail [waves^2] */
+    /* number type of the RESIDUALS / MERIT traces (TraceSettings::precision): MKO_PRECISION_FP64
+     * or MKO_PRECISION_DF64. The adjoint entry points take their precision as an argument. */
+    int32_t precision;
 } mko_residual_settings;
 
 /* P variables. Curvature k -> curvature[v][batch_surface[k]]; thickness and working distance

@@ -18,7 +18,7 @@ STENCIL_CENTRAL, STENCIL_ONE_SIDED, STENCIL_TRUNCATED = 0, 1, 2
 JACOBIAN_FD, JACOBIAN_ADJOINT = 0, 1
 (LM_RUNNING, LM_CONVERGED_MERIT, LM_CONVERGED_STEP, LM_CONVERGED_GRADIENT, LM_CONVERGED_STALL,
  LM_MAX_ITERATIONS, LM_DAMPING_LIMIT, LM_NUMERICAL_ERROR) = range(8)
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_DF64 = 0, 1, 2
 
 LM_STATUS_NAMES = ["running", "converged_merit", "converged_step", "converged_gradient",
                    "converged_stall", "max_iterations", "damping_limit", "numerical_error"]
@@ -42,7 +42,7 @@ class Batch(C.Structure):
 
 class ResidualSettings(C.Structure):
     _fields_ = [("rings", C.c_int32), ("spokes", C.c_int32), ("failure", C.c_int32),
-                ("fail_weight", C.c_double)]
+                ("fail_weight", C.c_double), ("precision", C.c_int32)]
 
 
 class Variables(C.Structure):

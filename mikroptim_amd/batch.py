@@ -119,9 +119,11 @@ class ResidualSettings:
     spokes: int = 8
     failure: int = abi.FAILURE_SOFT
     fail_weight: float = 1e6
+    precision: int = abi.PRECISION_FP64  # number type of the RESIDUALS / MERIT traces
 
     def c_struct(self):
-        return abi.ResidualSettings(self.rings, self.spokes, self.failure, self.fail_weight)
+        return abi.ResidualSettings(self.rings, self.spokes, self.failure, self.fail_weight,
+                                    self.precision)
 
 
 def layout(batch, settings):

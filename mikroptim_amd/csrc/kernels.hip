@@ -54,7 +54,7 @@ struct TraceParams {
     const double* reference_curvature;
     int V, S, L, F, T;
     int source, ref_kind;
-    int rings, spokes, soft;
+    int rings, spokes, soft, precision;
     double fail_weight;
     double lambda_ref_nm;
     double src_pt[3];
@@ -198,29 +198,44 @@ struct Record {
 };
 
 struct NoRecord {
-    __device__ __forceinline__ void put(int, const Ray&, uint8_t) {}
+    template <class RR>
+    __device__ __forceinline__ void put(int, const RR&, uint8_t) {}
     uint8_t image_branch;
 };
 
-// trace_to_reference_sphere (reference_sphere.hpp:183-222) over the LDS table.
-// Returns the final status; soft.penalty holds the accumulated penalty.
-template <bool kSoft, class Rec>
-__device__ __forceinline__ uint8_t trace_ray(const TraceParams& P, const SurfLds& L, Ray& r,
-                                             V3<double> M, double kappa, Soft& soft, Rec& rec) {
+// trace_to_reference_sphere (reference_sphere.hpp:183-222) over the LDS table, in the number
+// type R of the ray (double, or df64 for MKO_PRECISION_DF64). Returns the final status;
+// soft.penalty holds the accumulated penalty.
+template <bool kSoft, class R, class Rec>
+__device__ __forceinline__ uint8_t trace_ray(const TraceParams& P, const SurfLds& L,
+                                             mkod::RayT<R>& r, V3<double> M, double kappa,
+                                             Soft& soft, Rec& rec) {
+    using mkod::num;
     rec.put(0, r, mkod::kBranchRegular);
     for (int s = 1; s < P.S; ++s) {
         uint8_t branch = 0;
-        const uint8_t st = mkod::surface_step<kSoft>(r, L.c[s], L.t[s - 1], L.n[s - 1], L.mu[s],
-                                                     soft, branch);
+        const uint8_t st = mkod::surface_step<kSoft>(r, num<R>(L.c[s]), num<R>(L.t[s - 1]),
+                                                     num<R>(L.n[s - 1]), num<R>(L.mu[s]), soft,
+                                                     branch);
         if (st != MKO_STATUS_OK) return st;
         rec.put(s, r, branch);
     }
-    if (!(r.d.z > 0.0)) return MKO_STATUS_BACKWARD;
+    if (!(r.d.z > num<R>(0.0))) return MKO_STATUS_BACKWARD;
     uint8_t rb = 0;
-    const uint8_t rst = mkod::reference_step<kSoft>(r, M, kappa, P.ref_kind, L.n[P.S - 1], soft, rb);
+    const uint8_t rst = mkod::reference_step<kSoft>(
+        r, V3<R>{num<R>(M.x), num<R>(M.y), num<R>(M.z)}, num<R>(kappa), P.ref_kind,
+        num<R>(L.n[P.S - 1]), soft, rb);
     if (!mkod::in_piston(rst)) return rst;
     rec.image_branch = rb;
     return kSoft ? soft.first : uint8_t(MKO_STATUS_OK);
+}
+
+// The start ray in the trace's number type (computed in double, then converted).
+template <class R>
+__device__ __forceinline__ mkod::RayT<R> ray_as(const Ray& r) {
+    using mkod::num;
+    return mkod::RayT<R>{V3<R>{num<R>(r.p.x), num<R>(r.p.y), num<R>(r.p.z)},
+                         V3<R>{num<R>(r.d.x), num<R>(r.d.y), num<R>(r.d.z)}, num<R>(r.opl)};
 }
 
 // ---- deterministic block reductions -----------------------------------------------------------
@@ -248,8 +263,9 @@ __device__ __forceinline__ T block_sum(T v, T* lds) {
 // ---- RESIDUALS / MERIT ----------------------------------------------------------------------
 
 // Mode 0: RESIDUALS (residual rows, penalty rows, OPD, status). Mode 1: MERIT partials per
-// (variant, tuple): sum r^2 + sum p^2, failed and hard-failed counts.
-template <int kMode, bool kSoft, int kRpl>
+// (variant, tuple): sum r^2 + sum p^2, failed and hard-failed counts. R: number type of the ray
+// trace (double or df64); OPL, piston and residuals are formed in double.
+template <int kMode, bool kSoft, int kRpl, class R>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* residuals,
                                                        double* opd, uint8_t* status,
                                                        double* partial, int* pfailed,
@@ -277,15 +293,15 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
         pen[i] = 0.0;
         if (k < t.n) {
             const Sample s = sample_of(P, t, k);
-            Ray r = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
+            mkod::RayT<R> r = ray_as<R>(source_ray(P, t, s, a, sin_t, cos_t, L.n[0]));
             Soft soft{0.0, MKO_STATUS_OK};
             NoRecord nr;
             st[i] = trace_ray<kSoft>(P, L, r, M, kappa, soft, nr);
-            opl[i] = r.opl;
+            opl[i] = mkod::to_double(r.opl);
             pen[i] = soft.penalty;
             if (mkod::in_piston(st[i])) {
                 sw += s.weight;
-                swo += s.weight * r.opl;
+                swo += s.weight * opl[i];
             }
         }
     }
@@ -667,42 +683,71 @@ struct LdsSink {
     }
 };
 
-template <class RB, class RR>
+// Last state of a forward trace (the state on surface S-1, which the reference-sphere step starts
+// from); the reverse sweep reconstructs the earlier states with mkod::inverse_step.
+struct LastState {
+    V3<double> p, d;
+    uint8_t image_branch;
+    int count;
+    __device__ __forceinline__ void put(int s, const Ray& r, uint8_t) {
+        p = r.p;
+        d = r.d;
+        count = s + 1;
+    }
+};
+
+// backward_ray (adjoint_backward.hpp:496-580) for a ray that reached the image in hard mode, with
+// the states on surfaces S-2 ... 1 reconstructed from the last state by inverse steps and the
+// state on surface 0 regenerated from the source ray (no per-ray record in memory).
+template <class RB>
 __device__ __forceinline__ void backward_to_lds(const TraceParams& P, const SurfLds& L,
-                                                const ParamMap& map, const Record<RR>& rec,
-                                                V3<double> M, double kappa, const LdsSink& sink) {
+                                                const ParamMap& map, const LastState& last,
+                                                const Ray& start, V3<double> M, double kappa,
+                                                const LdsSink& sink) {
     const int S = P.S;
     mkod::RayAdj<RB> adj{{RB(0), RB(0), RB(0)}, {RB(0), RB(0), RB(0)}};
     {
-        const V3<RB> p{RB(rec.p[S - 1][0]), RB(rec.p[S - 1][1]), RB(rec.p[S - 1][2])};
-        const V3<RB> d{RB(rec.d[S - 1][0]), RB(rec.d[S - 1][1]), RB(rec.d[S - 1][2])};
+        const V3<RB> p{RB(last.p.x), RB(last.p.y), RB(last.p.z)};
+        const V3<RB> d{RB(last.d.x), RB(last.d.y), RB(last.d.z)};
         RB nib = RB(0);
         const RB mz = mkod::reference_step_backward<RB>(p, d, V3<RB>{RB(M.x), RB(M.y), RB(M.z)},
                                                          RB(kappa), P.ref_kind, RB(L.n[S - 1]),
-                                                         rec.image_branch, RB(1), RB(0), adj, nib);
+                                                         last.image_branch, RB(1), RB(0), adj, nib);
         sink.write(map.thick_col[S - 1], double(mz));
     }
-    uint8_t next_branch = rec.branch[S - 1];
+    V3<double> q = last.p, d1 = last.d;
     for (int s = S - 2; s >= 0; --s) {
-        const V3<RB> p{RB(rec.p[s][0]), RB(rec.p[s][1]), RB(rec.p[s][2])};
-        const V3<RB> d{RB(rec.d[s][0]), RB(rec.d[s][1]), RB(rec.d[s][2])};
         const int j = s + 1;
+        V3<double> ps, ds;
+        if (s == 0) {
+            ps = start.p;
+            ds = start.d;
+        } else {
+            const mkod::RayState<double> st =
+                mkod::inverse_step(q, d1, L.c[s], L.t[s], L.c[j], L.n[s], L.n[j]);
+            ps = st.p;
+            ds = st.d;
+        }
+        const V3<RB> p{RB(ps.x), RB(ps.y), RB(ps.z)};
+        const V3<RB> d{RB(ds.x), RB(ds.y), RB(ds.z)};
         RB tb = RB(0), cb = RB(0), nbb = RB(0), nab = RB(0);
         mkod::refracting_step_backward<RB>(p, d, RB(L.t[s]), RB(L.c[j]), RB(L.n[s]), RB(L.n[j]),
-                                           next_branch, RB(1), RB(0), adj, tb, cb, nbb, nab);
+                                           mkod::kBranchRegular, RB(1), RB(0), adj, tb, cb, nbb,
+                                           nab);
         if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], double(cb));
         sink.write(map.thick_col[s], double(tb));
-        next_branch = rec.branch[s];
+        q = ps;
+        d1 = ds;
     }
 }
 
 // One workgroup per (run, tuple), 256 lanes; tuples of at most 512 rays in halves of 256 rays.
-// Per half: forward trace in double (states recorded in RR), reverse sweep in RB, gradients
+// Per half: forward trace in double, reverse sweep in RB over reconstructed states, gradients
 // staged in LDS as a [P][256] block (pilot-relative floats); then the shifted moments of the
 // half: C on MFMA (v_mfma_f32_32x32x16_bf16, u split into bf16 hi + lo, hi*hi + hi*lo + lo*hi;
 // waves 0, 1, 3 form the 32 x 32 tiles (0,0), (0,1), (1,1) of the padded 64 x 64 C, the tile
 // below the diagonal is its mirror), S and Tr on VALU in double.
-template <class RB, class RR, int kHalves>
+template <class RB, int kHalves>
 __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, ParamMap map,
                                                                 float* partC, double* partST,
                                                                 TupleNormal* partT) {
@@ -738,7 +783,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
     double sw = 0.0, swo = 0.0;
-    Record<RR> rec;
+    LastState last;
     for (int h = 0; h < kHalves; ++h) {
         const int k = tid + h * kBlock;
         for (int p = 0; p < Pn; ++p) G[p * kGStride + tid] = 0.0f;
@@ -748,14 +793,15 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
         double opl = 0.0;
         bool ok = false;
         Sample smp{};
-        Ray r;
+        Ray r, start;
         if (k < t.n) {
             smp = sample_of(P, t, k);
             r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+            start = r;
             Soft soft{0.0, MKO_STATUS_OK};
-            rec.count = 0;
-            rec.image_branch = 0;
-            ok = trace_ray<false>(P, L, r, M, kappa, soft, rec) == MKO_STATUS_OK;
+            last.count = 0;
+            last.image_branch = 0;
+            ok = trace_ray<false>(P, L, r, M, kappa, soft, last) == MKO_STATUS_OK;
         }
         const uint64_t okmask = __ballot(ok);
         if (ok) {
@@ -764,7 +810,8 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
             wk = float(smp.weight);
             opl = r.opl;
             const bool leader = lane == int(__builtin_ctzll(okmask));
-            backward_to_lds<RB, RR>(P, L, map, rec, M, kappa, LdsSink{G, pilot[wave], tid, leader});
+            backward_to_lds<RB>(P, L, map, last, start, M, kappa,
+                                LdsSink{G, pilot[wave], tid, leader});
         }
         w_l[tid] = wk;
         sw_l[tid] = __builtin_sqrtf(wk);
@@ -988,6 +1035,8 @@ TraceParams make_params(const mko_batch& b, const mko_residual_settings& s) {
         throw std::invalid_argument("unknown failure mode");
     if (!(s.fail_weight >= 0.0) || !std::isfinite(s.fail_weight))
         throw std::invalid_argument("fail_weight must be finite and >= 0");
+    if (s.precision != MKO_PRECISION_FP64 && s.precision != MKO_PRECISION_DF64)
+        throw std::invalid_argument("trace precision must be MKO_PRECISION_FP64 or MKO_PRECISION_DF64");
     if (b.reference_sphere != MKO_REF_LAST_SURFACE && b.reference_curvature == nullptr)
         throw std::invalid_argument("image reference sphere needs reference_curvature");
     if (b.source == MKO_SOURCE_POINT && b.fields != 1)
@@ -1010,6 +1059,7 @@ TraceParams make_params(const mko_batch& b, const mko_residual_settings& s) {
     P.rings = s.rings;
     P.spokes = s.spokes;
     P.soft = s.failure == MKO_FAILURE_SOFT;
+    P.precision = s.precision;
     P.fail_weight = s.fail_weight;
     P.lambda_ref_nm = b.reference_wavelength_nm;
     for (int i = 0; i < 3; ++i) P.src_pt[i] = b.source_point[i];
@@ -1052,13 +1102,13 @@ int rays_per_lane(const TraceParams& P) {
     return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : 8;
 }
 
-template <int kMode, bool kSoft>
+template <int kMode, bool kSoft, class R>
 void launch_trace_rpl(int rpl, const TraceParams& P, double* res, double* opd, uint8_t* st,
                       double* part, int* pf, int* ph, hipStream_t stream) {
     const dim3 grid(unsigned(P.V) * unsigned(P.T));
-#define MKO_LAUNCH(R)                                                                            \
-    hipLaunchKernelGGL((trace_kernel<kMode, kSoft, R>), grid, dim3(kBlock), 0, stream, P, res,    \
-                       opd, st, part, pf, ph)
+#define MKO_LAUNCH(RPL)                                                                          \
+    hipLaunchKernelGGL((trace_kernel<kMode, kSoft, RPL, R>), grid, dim3(kBlock), 0, stream, P,    \
+                       res, opd, st, part, pf, ph)
     switch (rpl) {
     case 1: MKO_LAUNCH(1); break;
     case 2: MKO_LAUNCH(2); break;
@@ -1072,12 +1122,21 @@ void launch_trace(int mode, const TraceParams& P, double* res, double* opd, uint
                   double* part, int* pf, int* ph, hipStream_t stream) {
     if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large for one launch");
     const int rpl = rays_per_lane(P);
-    if (mode == 0) {
-        if (P.soft) launch_trace_rpl<0, true>(rpl, P, res, opd, st, part, pf, ph, stream);
-        else launch_trace_rpl<0, false>(rpl, P, res, opd, st, part, pf, ph, stream);
+    if (P.precision == MKO_PRECISION_DF64) {
+        using mkod::df64;
+        if (mode == 0) {
+            if (P.soft) launch_trace_rpl<0, true, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
+            else launch_trace_rpl<0, false, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
+        } else {
+            if (P.soft) launch_trace_rpl<1, true, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
+            else launch_trace_rpl<1, false, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
+        }
+    } else if (mode == 0) {
+        if (P.soft) launch_trace_rpl<0, true, double>(rpl, P, res, opd, st, part, pf, ph, stream);
+        else launch_trace_rpl<0, false, double>(rpl, P, res, opd, st, part, pf, ph, stream);
     } else {
-        if (P.soft) launch_trace_rpl<1, true>(rpl, P, res, opd, st, part, pf, ph, stream);
-        else launch_trace_rpl<1, false>(rpl, P, res, opd, st, part, pf, ph, stream);
+        if (P.soft) launch_trace_rpl<1, true, double>(rpl, P, res, opd, st, part, pf, ph, stream);
+        else launch_trace_rpl<1, false, double>(rpl, P, res, opd, st, part, pf, ph, stream);
     }
     check_hip(hipGetLastError(), "trace kernel launch");
 }
@@ -1129,6 +1188,27 @@ void launch_adjoint_rpl(int rpl, const TraceParams& P, const ParamMap& m, TJ* J,
         mko_detail::last_error = e.what();                                                                        \
         return MKO_ERR_DEVICE;                                                                     \
     }
+
+// Check kernel of the df64 arithmetic (df64_check_kernel, src/cuda/df64_check.cu:17): one
+// operation per element, for the bitwise comparison with the host restatement.
+__global__ void df64_ops_kernel(int op, int64_t n, const float* a, const float* b, float* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    using mkod::df64;
+    const df64 x{a[2 * i], a[2 * i + 1]};
+    const df64 y = op == 5 ? df64{0.0f, 0.0f} : df64{b[2 * i], b[2 * i + 1]};
+    df64 z;
+    switch (op) {
+    case 0: z = x + y; break;
+    case 1: z = x - y; break;
+    case 2: z = x * y; break;
+    case 3: z = mkod::mul_fp(x, y.hi); break;
+    case 4: z = x / y; break;
+    default: z = mkod::sqrt(x); break;
+    }
+    out[2 * i] = z.hi;
+    out[2 * i + 1] = z.lo;
+}
 
 } // namespace
 
@@ -1270,15 +1350,27 @@ int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_setting
         const size_t lds = size_t(m.P) * kGStride * sizeof(float);
         const dim3 grid{unsigned(tuples)}, block{unsigned(kBlock)};
         if (rpl == 1)
-            hipLaunchKernelGGL((adjoint_normal_kernel<double, double, 1>), grid, block, lds, s, P, m,
+            hipLaunchKernelGGL((adjoint_normal_kernel<double, 1>), grid, block, lds, s, P, m,
                                partC, partST, partT);
         else
-            hipLaunchKernelGGL((adjoint_normal_kernel<double, double, 2>), grid, block, lds, s, P, m,
+            hipLaunchKernelGGL((adjoint_normal_kernel<double, 2>), grid, block, lds, s, P, m,
                                partC, partST, partT);
         check_hip(hipGetLastError(), "adjoint normal launch");
         hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
                            m.P, partC, partST, partT, a, g);
         check_hip(hipGetLastError(), "adjoint normal finalize launch");
+    })
+}
+
+int mko_df64_ops_device(int op, int64_t n, const float* a, const float* b, float* out,
+                        void* stream) {
+    MKO_TRY({
+        if (op < 0 || op > 5) throw std::invalid_argument("df64 op outside [0, 5]");
+        if (n < 0) throw std::invalid_argument("negative count");
+        if (n == 0) return 0;
+        hipLaunchKernelGGL(df64_ops_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), op, n, a, b, out);
+        check_hip(hipGetLastError(), "df64 ops launch");
     })
 }
 

@@ -13,9 +13,23 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "df64.hpp"
 #include "mikroptim_types.h"
 
 namespace mkod {
+
+// Number-type helpers of the forward trace: R = double (every mode) or df64 (the RESIDUALS /
+// MERIT traces with MKO_PRECISION_DF64). For double they are the identity.
+template <class R>
+__device__ __forceinline__ R num(double x) {
+    return x;
+}
+template <>
+__device__ __forceinline__ df64 num<df64>(double x) {
+    return df64_from(x);
+}
+__device__ __forceinline__ double to_double(double x) { return x; }
+__device__ __forceinline__ double to_double(df64 x) { return df64_to(x); }
 
 template <class R>
 struct V3 {
@@ -51,10 +65,12 @@ __device__ __forceinline__ V3<R> cross(V3<R> a, V3<R> b) {
 
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ df64 dsqrt(df64 x) { return sqrt(x); }
 template <class R>
 __device__ __forceinline__ R rnorm(V3<R> a) {
-    return R(1) / dsqrt(dot(a, a));
+    return num<R>(1.0) / dsqrt(dot(a, a));
 }
+__device__ __forceinline__ float rnorm(V3<float> a) { return 1.0f / dsqrt(dot(a, a)); }
 
 constexpr uint8_t kBranchRegular = 0, kBranchSoftMissed = 1, kBranchSoftTIR = 2;
 
@@ -75,52 +91,55 @@ struct Soft {
     }
 };
 
-struct Ray {
-    V3<double> p, d;
-    double opl;
+template <class R>
+struct RayT {
+    V3<R> p, d;
+    R opl;
 };
+using Ray = RayT<double>;
 
 // One surface step (propagate_surface_local with transfer, sphere intersection and refraction).
 // c = effective curvature (0 for stop and plane), t = vertex distance from the previous surface,
 // nb = medium before, mu = nb / n_after. Returns a TraceStatus (Ok also for soft continuations,
 // whose kind goes to `soft` and whose branch bits go to `branch`).
-template <bool kSoft>
-__device__ __forceinline__ uint8_t surface_step(Ray& r, double c, double t, double nb, double mu,
-                                                Soft& soft, uint8_t& branch) {
-    if (!(r.d.z > 0.0)) return MKO_STATUS_BACKWARD;
+template <bool kSoft, class R>
+__device__ __forceinline__ uint8_t surface_step(RayT<R>& r, R c, R t, R nb, R mu, Soft& soft,
+                                                uint8_t& branch) {
+    const R zero = num<R>(0.0), one = num<R>(1.0);
+    if (!(r.d.z > zero)) return MKO_STATUS_BACKWARD;
     // transfer_to_tangent_plane
     r.p.z -= t;
-    const double s0 = -r.p.z / r.d.z;
+    const R s0 = -r.p.z / r.d.z;
     r.p.x += s0 * r.d.x;
     r.p.y += s0 * r.d.y;
-    r.p.z = 0.0;
+    r.p.z = zero;
     // intersect_sphere
-    const double H = c * (r.p.x * r.p.x + r.p.y * r.p.y);
-    const double B = r.d.z - c * (r.p.x * r.d.x + r.p.y * r.d.y);
-    const double D = B * B - c * H;
-    double s;
+    const R H = c * (r.p.x * r.p.x + r.p.y * r.p.y);
+    const R B = r.d.z - c * (r.p.x * r.d.x + r.p.y * r.d.y);
+    const R D = B * B - c * H;
+    R s;
     bool missed = false;
-    if (D < 0.0) {
-        if (!kSoft || (!(c > 0.0) && !(c < 0.0))) return MKO_STATUS_MISSED;
-        soft.add(-D, MKO_STATUS_SOFT_MISSED);
+    if (D < zero) {
+        if (!kSoft || (!(c > zero) && !(c < zero))) return MKO_STATUS_MISSED;
+        soft.add(-to_double(D), MKO_STATUS_SOFT_MISSED);
         s = B / c;
         missed = true;
     } else {
-        const double denom = B + dsqrt(D);
-        if (!(denom > 0.0)) return MKO_STATUS_MISSED;
+        const R denom = B + dsqrt(D);
+        if (!(denom > zero)) return MKO_STATUS_MISSED;
         s = H / denom;
     }
     r.p = r.p + s * r.d;
     r.opl += nb * (s0 + s);
     // refract_sphere
-    V3<double> N{-c * r.p.x, -c * r.p.y, 1.0 - c * r.p.z};
+    V3<R> N{-c * r.p.x, -c * r.p.y, one - c * r.p.z};
     if (kSoft && missed) N = rnorm(N) * N;
-    const double cosI = dot(r.d, N);
-    const double k = 1.0 - mu * mu * (1.0 - cosI * cosI);
-    if (k < 0.0) {
+    const R cosI = dot(r.d, N);
+    const R k = one - mu * mu * (one - cosI * cosI);
+    if (k < zero) {
         if (!kSoft) return MKO_STATUS_TIR;
-        soft.add(-k, MKO_STATUS_SOFT_TIR);
-        const V3<double> tau = r.d - cosI * N;
+        soft.add(-to_double(k), MKO_STATUS_SOFT_TIR);
+        const V3<R> tau = r.d - cosI * N;
         r.d = rnorm(tau) * tau;
         branch = uint8_t((missed ? kBranchSoftMissed : 0) | kBranchSoftTIR);
         return MKO_STATUS_OK;
@@ -131,49 +150,50 @@ __device__ __forceinline__ uint8_t surface_step(Ray& r, double c, double t, doub
 }
 
 // reference_sphere_step (kind last_surface) / image_sphere_step (image kinds).
-template <bool kSoft>
-__device__ __forceinline__ uint8_t reference_step(Ray& r, V3<double> M, double kappa, int kind,
-                                                  double n_image, Soft& soft, uint8_t& branch) {
-    const V3<double> q = r.p - M;
-    const double b = -dot(q, r.d);
+template <bool kSoft, class R>
+__device__ __forceinline__ uint8_t reference_step(RayT<R>& r, V3<R> M, R kappa, int kind,
+                                                  R n_image, Soft& soft, uint8_t& branch) {
+    const R zero = num<R>(0.0), one = num<R>(1.0);
+    const V3<R> q = r.p - M;
+    const R b = -dot(q, r.d);
     branch = kBranchRegular;
     if (kind == MKO_REF_LAST_SURFACE) {
-        const double g = dot(r.p, r.p - 2.0 * M);
-        const double disc = b * b - g;
-        if (disc < 0.0) {
-            const double radius_sq = dot(M, M);
-            if (!kSoft || !(radius_sq > 0.0)) return MKO_STATUS_MISSED;
-            soft.add(-disc / radius_sq, MKO_STATUS_SOFT_MISSED);
+        const R g = dot(r.p, r.p - num<R>(2.0) * M);
+        const R disc = b * b - g;
+        if (disc < zero) {
+            const R radius_sq = dot(M, M);
+            if (!kSoft || !(radius_sq > zero)) return MKO_STATUS_MISSED;
+            soft.add(-to_double(disc) / to_double(radius_sq), MKO_STATUS_SOFT_MISSED);
             r.p = r.p + b * r.d;
             r.opl += n_image * b;
             branch = kBranchSoftMissed;
             return MKO_STATUS_SOFT_MISSED;
         }
-        const double root = dsqrt(disc);
-        const bool real_image = M.z >= 0.0;
-        const double denom = real_image ? b + root : b - root;
-        if (!(real_image ? denom > 0.0 : denom < 0.0)) return MKO_STATUS_MISSED;
-        const double s = g / denom;
+        const R root = dsqrt(disc);
+        const bool real_image = M.z >= zero;
+        const R denom = real_image ? b + root : b - root;
+        if (!(real_image ? denom > zero : denom < zero)) return MKO_STATUS_MISSED;
+        const R s = g / denom;
         r.p = r.p + s * r.d;
         r.opl += n_image * s;
         return MKO_STATUS_OK;
     }
-    const V3<double> cq = cross(q, r.d);
-    const double t = -dot(cq, cq);
-    const double disc = 1.0 + kappa * kappa * t;
-    double step;
+    const V3<R> cq = cross(q, r.d);
+    const R t = -dot(cq, cq);
+    const R disc = one + kappa * kappa * t;
+    R step;
     uint8_t st = MKO_STATUS_OK;
-    if (disc >= 0.0) {
-        step = b - kappa * t / (dsqrt(disc) + 1.0);
-    } else if (kSoft && disc < 0.0) {
+    if (disc >= zero) {
+        step = b - kappa * t / (dsqrt(disc) + one);
+    } else if (kSoft && disc < zero) {
         step = b - kappa * t;
         st = MKO_STATUS_SOFT_MISSED;
     } else {
         return MKO_STATUS_MISSED;
     }
-    if (!(step >= 0.0) && !(step < 0.0)) return MKO_STATUS_MISSED;
+    if (!(step >= zero) && !(step < zero)) return MKO_STATUS_MISSED;
     if (st == MKO_STATUS_SOFT_MISSED) {
-        soft.add(-disc, MKO_STATUS_SOFT_MISSED);
+        soft.add(-to_double(disc), MKO_STATUS_SOFT_MISSED);
         branch = kBranchSoftMissed;
     }
     r.p = r.p + b * r.d;
@@ -374,6 +394,48 @@ __device__ __forceinline__ R reference_step_backward(V3<R> p, V3<R> d, V3<R> M, 
     adj.d = db;
     center_adj += (R(-1)) * qb;
     return center_adj.z;
+}
+
+// ---- state reconstruction for the reverse sweep (no per-ray record) -----------------------------
+//
+// A regular (hard-mode) surface step maps the ray state on surface s (position in the frame of s,
+// direction in medium n_s) to the state on surface j = s + 1 and is invertible: refraction is
+// reversible (the tangential direction component scales by n_j / n_s, the normal component stays
+// positive), and the segment in gap s is recovered by intersecting the reversed ray with surface
+// s. inverse_step returns the state on s from the state (q, d1) on j; it is the forward transfer
+// and intersection of surface_step in the frame mirrored at z = 0 (z -> -z, curvature -> -c),
+// applied to the reversed ray, so it uses the same Spencer-Murty root.
+
+template <class R>
+struct RayState {
+    V3<R> p, d;
+};
+
+// State on surface s (curvature c_s, vertex t_s before surface j) from the state on surface j
+// (curvature c_j); n_s, n_j: media before and after surface j.
+__device__ __forceinline__ RayState<double> inverse_step(V3<double> q, V3<double> d1, double c_s,
+                                                         double t_s, double c_j, double n_s,
+                                                         double n_j) {
+    // inverse refraction at j: d = mu' d1 + (sqrt(k') - mu' cos') N, mu' = n_j / n_s
+    const V3<double> N{-c_j * q.x, -c_j * q.y, 1.0 - c_j * q.z};
+    const double mu = n_j / n_s;
+    const double cosI = dot(d1, N);
+    const double k = 1.0 - mu * mu * (1.0 - cosI * cosI);
+    const V3<double> d = mu * d1 + (dsqrt(k) - mu * cosI) * N;
+    // reversed ray in the mirrored frame of s: origin (q.x, q.y, -(q.z + t_s)), direction
+    // (-d.x, -d.y, d.z); transfer to the tangent plane z = 0 and intersect curvature -c_s
+    const double s0 = (q.z + t_s) / d.z;
+    const double x = q.x - s0 * d.x;
+    const double y = q.y - s0 * d.y;
+    const double cm = -c_s;
+    const double H = cm * (x * x + y * y);
+    const double B = d.z + cm * (x * d.x + y * d.y);
+    const double D = B * B - cm * H;
+    const double s = H / (B + dsqrt(D));
+    RayState<double> out;
+    out.p = V3<double>{x - s * d.x, y - s * d.y, -(s * d.z)};
+    out.d = d;
+    return out;
 }
 
 } // namespace mkod

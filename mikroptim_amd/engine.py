@@ -24,7 +24,7 @@ _lib = None
 EXPORTS = [
     "mko_last_error", "mko_residuals_device", "mko_merit_device", "mko_point_batch_device",
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
-    "mko_adjoint_normal_device", "mko_gram_device",
+    "mko_adjoint_normal_device", "mko_gram_device", "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
     "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
@@ -66,6 +66,8 @@ def library():
         "mko_adjoint_normal_workspace": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
         "mko_adjoint_normal_device": ([bp, sp, vp, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_size_t, C.c_void_p], C.c_int),
+        "mko_df64_ops_device": ([C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.c_void_p], C.c_int),
         "mko_engine_create": ([C.c_int, C.c_size_t], C.c_void_p),
         "mko_engine_destroy": ([C.c_void_p], None),
         "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),

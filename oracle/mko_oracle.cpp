@@ -22,6 +22,7 @@
 #include "gauss_legendre_table.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1560,9 +1561,105 @@ using namespace orc;
         return -1;                                                                                 \
     }
 
+// ---- df64 (double-float) arithmetic: include/mikroptim/math/df64.hpp:91-271 -------------------
+// Every float operation rounded on its own (this file is built with -ffp-contract=off); the
+// products that the algorithms fuse use fmaf.
+namespace orc_df64 {
+
+struct D {
+    float hi, lo;
+};
+inline void two_sum(float a, float b, float& s, float& e) { // df64.hpp:91-95
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+inline void fast_two_sum(float a, float b, float& s, float& e) { // df64.hpp:97-100
+    s = a + b;
+    e = b - (s - a);
+}
+inline void two_prod(float a, float b, float& p, float& e) { // df64.hpp:101-104
+    p = a * b;
+    e = std::fmaf(a, b, -p);
+}
+inline D add(D a, D b) { // AccurateDWPlusDW, df64.hpp:160-170
+    float sh, sl, th, tl, vh, vl, zh, zl;
+    two_sum(a.hi, b.hi, sh, sl);
+    two_sum(a.lo, b.lo, th, tl);
+    const float c = sl + th;
+    fast_two_sum(sh, c, vh, vl);
+    const float w = tl + vl;
+    fast_two_sum(vh, w, zh, zl);
+    return {zh, zl};
+}
+inline D neg(D a) { return {-a.hi, -a.lo}; }
+inline D mul(D a, D b) { // DWTimesDW3, df64.hpp:177-187
+    float ch, cl1, zh, zl;
+    two_prod(a.hi, b.hi, ch, cl1);
+    const float tl0 = a.lo * b.lo;
+    const float tl1 = std::fmaf(a.hi, b.lo, tl0);
+    const float cl2 = std::fmaf(a.lo, b.hi, tl1);
+    const float cl3 = cl1 + cl2;
+    fast_two_sum(ch, cl3, zh, zl);
+    return {zh, zl};
+}
+inline D mul_fp(D a, float b) { // DWTimesFP3, df64.hpp:190-196
+    float ch, cl1, zh, zl;
+    two_prod(a.hi, b, ch, cl1);
+    const float cl3 = std::fmaf(a.lo, b, cl1);
+    fast_two_sum(ch, cl3, zh, zl);
+    return {zh, zl};
+}
+inline D div(D a, D b) { // DWDivDW2, df64.hpp:204-215
+    const float th = a.hi / b.hi;
+    const D r = mul_fp(b, th);
+    const float pi_h = a.hi - r.hi;
+    const float d_l = a.lo - r.lo;
+    const float d = pi_h + d_l;
+    const float tl = d / b.hi;
+    float zh, zl;
+    fast_two_sum(th, tl, zh, zl);
+    return {zh, zl};
+}
+inline D sqrt(D a) { // SQRTDWtoDW, df64.hpp:259-271
+    if (!(a.hi > 0.0f) || a.hi > FLT_MAX) return {std::sqrt(a.hi), 0.0f};
+    const float sh = std::sqrt(a.hi);
+    const float rho1 = std::fmaf(-sh, sh, a.hi);
+    const float rho2 = a.lo + rho1;
+    const float sl = rho2 / (sh + sh);
+    float zh, zl;
+    fast_two_sum(sh, sl, zh, zl);
+    return {zh, zl};
+}
+
+} // namespace orc_df64
+
 extern "C" {
 
 const char* orc_last_error(void) { return orc::g_error.c_str(); }
+
+// Element-wise df64 operation over n pairs (hi, lo): op 0 a + b, 1 a - b, 2 a * b, 3 a * b.hi
+// (DWTimesFP3), 4 a / b, 5 sqrt(a). a, b, out: [n][2] floats.
+int orc_df64_ops(int op, int64_t n, const float* a, const float* b, float* out) {
+    if (op < 0 || op > 5 || n < 0) return -1;
+    using namespace orc_df64;
+    for (int64_t i = 0; i < n; ++i) {
+        const D x{a[2 * i], a[2 * i + 1]};
+        const D y = op == 5 ? D{0.0f, 0.0f} : D{b[2 * i], b[2 * i + 1]};
+        D z{};
+        switch (op) {
+        case 0: z = add(x, y); break;
+        case 1: z = add(x, neg(y)); break;
+        case 2: z = mul(x, y); break;
+        case 3: z = mul_fp(x, y.hi); break;
+        case 4: z = div(x, y); break;
+        default: z = sqrt(x); break;
+        }
+        out[2 * i] = z.hi;
+        out[2 * i + 1] = z.lo;
+    }
+    return 0;
+}
 
 int orc_gauss_legendre(int n, double* u, double* w) {
     ORC_TRY({

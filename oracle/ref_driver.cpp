@@ -8,6 +8,7 @@
 #include "mikroptim_types.h"
 
 #include "mikroptim/batch/batch_merit.hpp"
+#include "mikroptim/math/df64.hpp"
 #include "mikroptim/batch/batch_residuals.hpp"
 #include "mikroptim/batch/system_batch.hpp"
 #include "mikroptim/model/system.hpp"
@@ -314,6 +315,29 @@ int ref_optimize_lm(const mko_batch* base, const mko_residual_settings* s,
         }
         if (cycles_out) *cycles_out = r.cycles;
     })
+}
+
+// The reference's own df64 operations (include/mikroptim/math/df64.hpp), same layout and op
+// codes as orc_df64_ops.
+int ref_df64_ops(int op, int64_t n, const float* a, const float* b, float* out) {
+    if (op < 0 || op > 5 || n < 0) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        const mko::df64 x = mko::df64::from_parts(a[2 * i], a[2 * i + 1]);
+        const mko::df64 y = op == 5 ? mko::df64::from_parts(0.0f, 0.0f)
+                                    : mko::df64::from_parts(b[2 * i], b[2 * i + 1]);
+        mko::df64 z{};
+        switch (op) {
+        case 0: z = x + y; break;
+        case 1: z = x - y; break;
+        case 2: z = x * y; break;
+        case 3: z = x * y.hi; break;
+        case 4: z = x / y; break;
+        default: z = mko::msqrt(x); break;
+        }
+        out[2 * i] = z.hi;
+        out[2 * i + 1] = z.lo;
+    }
+    return 0;
 }
 
 } // extern "C"

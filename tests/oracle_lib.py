@@ -33,6 +33,7 @@ def _declare(lib, prefix):
         "splitmix64": ([C.c_uint64], C.c_uint64),
         "variant_seed": ([C.c_uint64, C.c_uint64], C.c_uint64),
         "unit_uniform": ([C.c_uint64], C.c_double),
+        "df64_ops": ([C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     }
     if prefix == "orc_":
         sig.update({
@@ -221,3 +222,25 @@ def reference():
             build()
         _cache["ref"] = Lib(REF_SO, "ref_") if os.path.exists(REF_SO) else None
     return _cache["ref"]
+
+
+DF64_OPS = {"add": 0, "sub": 1, "mul": 2, "mul_fp": 3, "div": 4, "sqrt": 5}
+
+
+def df64_split(x):
+    """float pairs (hi, lo) of doubles x: hi = float(x), lo = float(x - hi) -> [n][2] float32."""
+    x = np.asarray(x, dtype=np.float64)
+    hi = x.astype(np.float32)
+    lo = (x - hi.astype(np.float64)).astype(np.float32)
+    return np.ascontiguousarray(np.stack([hi, lo], axis=-1))
+
+
+def df64_ops(lib, op, a, b=None):
+    """Element-wise df64 operation of the oracle or reference driver over [n][2] float pairs."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.zeros_like(a) if b is None else np.ascontiguousarray(b, dtype=np.float32)
+    out = np.zeros_like(a)
+    fn = getattr(lib.lib, lib.prefix + "df64_ops")
+    rc = fn(DF64_OPS[op], a.shape[0], a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    assert rc == 0
+    return out

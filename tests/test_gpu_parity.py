@@ -358,3 +358,58 @@ def test_autograd_residuals_backward_is_adjoint_vjp(engine, singlet):
             rr, _, _ = engine.residuals(pp, s)
             fd = (rr[0] @ wn - rr[1] @ wn) / (2 * h)
             assert abs(gx[i, k] - fd) <= 1e-5 * max(1.0, abs(fd))
+
+
+# ---- df64 -----------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "mul_fp", "div", "sqrt"])
+def test_df64_device_matches_oracle_bitwise(op):
+    """The device df64 operations against the oracle's host restatement, bit for bit
+    (CudaDf64.DeviceMatchesHostBitwise)."""
+    import torch
+    from mikroptim_amd.engine import library
+    from oracle_lib import DF64_OPS, df64_ops, df64_split
+
+    rng = np.random.default_rng(41)
+    n = 100000
+    x = rng.uniform(0.5, 2.0, n) * np.exp2(rng.integers(-20, 21, n).astype(np.float64))
+    y = rng.uniform(-2.0, 2.0, n) * np.exp2(rng.integers(-20, 21, n).astype(np.float64))
+    a, b = df64_split(x), df64_split(y)
+    if op == "mul_fp":
+        b[:, 1] = 0.0
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = torch.empty_like(ta)
+    rc = library().mko_df64_ops_device(DF64_OPS[op], n, ta.data_ptr(), tb.data_ptr(),
+                                       out.data_ptr(), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    dev = out.cpu().numpy()
+    host = df64_ops(oracle(), op, a, b)
+    assert np.array_equal(dev.view(np.uint32), host.view(np.uint32))
+
+
+DF64_OPD_TOL = 1e-9        # mm, kDf64VsFp64 (tests/tolerances.hpp:182)
+DF64_OPD_REL = 1e-10       # kDf64VsFp64Relative (:189)
+
+
+@pytest.mark.parametrize("which", ["singlet_soft", "singlet_hard", "objective"])
+def test_df64_trace_matches_fp64_oracle(engine, singlet, objective, which):
+    """RESIDUALS traced in df64 against the FP64 oracle: equal statuses, OPD within
+    max(1e-9 mm, 1e-10 |OPD|) per ray; MERIT in df64 equals the sum of squares of the df64
+    residuals (kMeritVsResidualSumDf64 = 2e-13)."""
+    if which == "objective":
+        b, _ = objective
+        s = systems.objective_settings(abi.FAILURE_HARD)
+    else:
+        b, _ = singlet
+        s = systems.singlet_settings(abi.FAILURE_SOFT if which == "singlet_soft"
+                                     else abi.FAILURE_HARD)
+    s.precision = abi.PRECISION_DF64
+    res, opd, st = engine.residuals(b, s)
+    s64 = systems.ResidualSettings(s.rings, s.spokes, s.failure, s.fail_weight)
+    _, opdo, sto = oracle().residuals(b, s64, threads=8)
+    assert np.array_equal(st, sto)
+    tol = np.maximum(DF64_OPD_TOL, DF64_OPD_REL * np.abs(opdo))
+    assert np.all(np.abs(opd - opdo) <= tol)
+    m, _, _ = engine.merit(b, s)
+    assert np.max(np.abs(m - (res ** 2).sum(1)) / m) <= 2e-13

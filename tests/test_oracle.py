@@ -425,3 +425,72 @@ def test_lm_solves_linear_least_squares():
         assert abs(runs[i].merit - f_star) <= 1e-9 * f_star
         assert np.max(np.abs(x[i] - want)) <= 1e-5
         assert abi.LM_STATUS_NAMES[runs[i].status].startswith("converged")
+
+
+# ---- df64 (double-float) arithmetic of the MKO_PRECISION_DF64 trace --------------------------
+
+def _df64_inputs(n, seed, positive=False):
+    from oracle_lib import df64_split
+    rng = np.random.default_rng(seed)
+    m = rng.uniform(0.5, 1.0, n) * (1.0 if positive else rng.choice([-1.0, 1.0], n))
+    x = m * np.exp2(rng.integers(-20, 21, n).astype(np.float64))
+    x = x * (1.0 + rng.uniform(-2.0 ** -30, 2.0 ** -30, n))  # use the low word too
+    return df64_split(x)
+
+
+def _as_double(p):
+    return p[..., 0].astype(np.float64) + p[..., 1].astype(np.float64)
+
+
+U2 = 2.0 ** -48
+# relative bounds of the published algorithms plus the rounding of the double reference
+# (tests/tolerances.hpp:194-197 of the reference: kDf64{Add,Mul,Div,Sqrt}Relative)
+DF64_BOUNDS = {"add": (3 + 13 * 2.0 ** -24 + 0.0625) * U2, "sub": (3 + 13 * 2.0 ** -24 + 0.0625) * U2,
+               "mul": (4 + 0.0625) * U2, "mul_fp": (2 + 0.0625) * U2,
+               "div": (15 + 56 * 2.0 ** -24 + 0.0625) * U2, "sqrt": (3.125 + 0.0625) * U2}
+
+
+@pytest.mark.parametrize("op", sorted(DF64_BOUNDS))
+def test_df64_oracle_within_published_error_bounds(op):
+    """AccurateDWPlusDW (3u^2 + 13u^3), DWTimesDW3 (4u^2), DWTimesFP3 (2u^2), DWDivDW2
+    (15u^2 + 56u^3), SQRTDWtoDW (25/8 u^2) against the correctly rounded double result of the
+    same (exactly representable) operands. A dropped correction term fails these bounds."""
+    from oracle_lib import df64_ops
+    a = _df64_inputs(200000, 21, positive=op == "sqrt")
+    b = _df64_inputs(200000, 22)
+    if op == "mul_fp":
+        b[:, 1] = 0.0
+    z = _as_double(df64_ops(oracle(), op, a, b))
+    x, y = _as_double(a), _as_double(b)
+    exact = {"add": x + y, "sub": x - y, "mul": x * y, "mul_fp": x * y, "div": x / y,
+             "sqrt": np.sqrt(np.abs(x))}[op]
+    rel = np.abs(z - exact) / np.abs(exact)
+    assert rel.max() <= DF64_BOUNDS[op], rel.max()
+    # normalised results: |lo| <= ulp(hi) / 2
+    r = df64_ops(oracle(), op, a, b)
+    assert np.all(np.abs(r[:, 1]) <= np.spacing(np.abs(r[:, 0])) / 2)
+
+
+def test_df64_cancellation_stays_relative():
+    """x - (x + tiny): AccurateDWPlusDW keeps the relative bound under cancellation."""
+    from oracle_lib import df64_ops, df64_split
+    rng = np.random.default_rng(23)
+    x = rng.uniform(1.0, 2.0, 50000)
+    d = x * rng.uniform(1e-9, 1e-6, 50000)
+    a, b = df64_split(x), df64_split(x + d)
+    z = _as_double(df64_ops(oracle(), "sub", b, a))
+    exact = _as_double(b) - _as_double(a)
+    assert np.max(np.abs(z - exact) / np.abs(exact)) <= DF64_BOUNDS["sub"]
+
+
+@pytest.mark.skipif(reference() is None, reason="reference CPU build absent")
+@pytest.mark.parametrize("op", sorted(DF64_BOUNDS))
+def test_df64_oracle_matches_reference_bitwise(op):
+    """The oracle's df64 restatement against the reference's own df64.hpp, bit for bit."""
+    from oracle_lib import df64_ops
+    a = _df64_inputs(100000, 31, positive=op == "sqrt")
+    b = _df64_inputs(100000, 32)
+    if op == "mul_fp":
+        b[:, 1] = 0.0
+    assert np.array_equal(df64_ops(oracle(), op, a, b).view(np.uint32),
+                          df64_ops(reference(), op, a, b).view(np.uint32))
