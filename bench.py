@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--variants", type=int, default=4096)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with three 1 GiB device copies (scale for rocprofv3 byte counters)")
     return ap.parse_args()
 
 
@@ -68,6 +70,22 @@ def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps,
     sess.close()
     jacobians = int(sum(r.jacobians for r in res.runs)) - jac0
     return done, dt, kt, res, jacobians
+
+
+def profiled_traffic(runs_per_launch):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters on a 1 GiB copy, as
+    bytes per run), times the runs of this launch; None without the summary."""
+    import re
+    path = os.path.join(ROOT, "profiles", "adjoint_normal_kernel_objective_4096x48.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    f = re.search(r"fetch_bytes_per_run\s*=\s*([0-9.e+]+)", text)
+    w = re.search(r"write_bytes_per_run\s*=\s*([0-9.e+]+)", text)
+    if not (f and w):
+        return None
+    return (float(f.group(1)) + float(w.group(1))) * runs_per_launch
 
 
 def cpu_baseline(batch, variables, settings, x_gpu_final, eng, sample_runs=64, threads=16):
@@ -141,7 +159,8 @@ def main():
         achieved = runs_per_launch * rays * (flop_fwd + flop_adj) / (adj_ms / adj_n * 1e-3) / 1e12
     roof = {"bound": "valu", "kernel": "adjoint_normal_kernel (fused adjoint + normal equations)",
             "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
-            "frac": (achieved / 78.6) if achieved else None, "traffic": None,
+            "frac": (achieved / 78.6) if achieved else None,
+            "traffic": profiled_traffic(runs_per_launch),
             "avg_launch_ms": (adj_ms / adj_n) if adj_n else None,
             "runs_per_launch": runs_per_launch,
             "flop_per_ray": flop_fwd + flop_adj}
@@ -200,6 +219,14 @@ def main():
         out["opd_err_waves_rms"] = err
         out["opd_status_equal"] = same_status
     print(json.dumps(out), flush=True)
+    if args.copy_calibration:
+        import torch
+
+        x = torch.empty(256 * 2 ** 20, dtype=torch.float32, device="cuda")  # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        sync()
 
 
 if __name__ == "__main__":

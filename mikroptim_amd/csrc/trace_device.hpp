@@ -1,5 +1,5 @@
 // Per-ray device math of the trace and of its reverse (adjoint) sweep, templated on the number
-// type R (double for every forward trace; double or float for the reverse sweep).
+// type R (double or df64 for the forward trace; double for the reverse sweep).
 //
 // Formulas and branch rules follow the reference's shared math core:
 //   surface steps        include/mikroptim/trace/surface_ops.hpp:122-203
@@ -203,6 +203,51 @@ __device__ __forceinline__ uint8_t reference_step(RayT<R>& r, V3<R> M, R kappa, 
 
 // ---- reverse sweep ------------------------------------------------------------------------------
 
+// Division, reciprocal and square root of the reverse sweep and of the state reconstruction:
+// hardware reciprocal / reciprocal square root refined by Newton steps (a few ulp) instead of
+// the correctly rounded IEEE sequences with their scaling and fix-up steps. The sweep only needs
+// ~1e-15 relative accuracy (its results are compared to 1e-10 per Jacobian column); the forward
+// trace keeps the IEEE operations.
+#ifndef MKO_REVERSE_FAST_MATH
+#define MKO_REVERSE_FAST_MATH 1
+#endif
+__device__ __forceinline__ double rev_rcp(double b) {
+#if MKO_REVERSE_FAST_MATH
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    return __builtin_fma(r, e, r);
+#else
+    return 1.0 / b;
+#endif
+}
+__device__ __forceinline__ double rev_div(double a, double b) {
+#if MKO_REVERSE_FAST_MATH
+    const double r = rev_rcp(b);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+#else
+    return a / b;
+#endif
+}
+__device__ __forceinline__ double rev_sqrt(double x) {
+#if MKO_REVERSE_FAST_MATH
+    if (!(x > 0.0)) return __builtin_sqrt(x);
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+#else
+    return __builtin_sqrt(x);
+#endif
+}
+
 template <class R>
 struct RayAdj {
     V3<R> p, d;
@@ -217,7 +262,8 @@ __device__ __forceinline__ void refracting_step_backward(V3<R> p, V3<R> d, R t, 
                                                          R seed_pen, RayAdj<R>& adj, R& tb,
                                                          R& cb_out, R& nbb, R& nab) {
     const R z = p.z - t;
-    const R s0 = -z / d.z;
+    const R rdz = rev_rcp(d.z);
+    const R s0 = -z * rdz;
     const R x = p.x + s0 * d.x;
     const R y = p.y + s0 * d.y;
     const R rr = x * x + y * y;
@@ -232,8 +278,8 @@ __device__ __forceinline__ void refracting_step_backward(V3<R> p, V3<R> d, R t, 
     if (missed) {
         s = B / c;
     } else {
-        root_d = dsqrt(D);
-        s = H / (B + root_d);
+        root_d = rev_sqrt(D);
+        s = rev_div(H, B + root_d);
     }
     const V3<R> q{x + s * d.x, y + s * d.y, s * d.z};
     const V3<R> n_raw{-c * q.x, -c * q.y, R(1) - c * q.z};
@@ -243,7 +289,7 @@ __device__ __forceinline__ void refracting_step_backward(V3<R> p, V3<R> d, R t, 
         inv_norm = rnorm(n_raw);
         N = inv_norm * n_raw;
     }
-    const R mu = n_before / n_after;
+    const R mu = rev_div(n_before, n_after);
     const R cos_i = dot(d, N);
     const R sin2 = R(1) - cos_i * cos_i;
     const R k = R(1) - mu * mu * sin2;
@@ -263,13 +309,13 @@ __device__ __forceinline__ void refracting_step_backward(V3<R> p, V3<R> d, R t, 
         nb += (-cos_i) * taub;
         kb += -seed_pen;
     } else {
-        const R root_k = dsqrt(k);
+        const R root_k = rev_sqrt(k);
         const R a = root_k - mu * cos_i;
         in.d += mu * d1b;
         nb += a * d1b;
         const R ab = dot(d1b, N);
         mub += dot(d1b, d);
-        kb += ab / (R(2) * root_k);
+        kb += rev_div(ab, R(2) * root_k);
         mub += -ab * cos_i;
         cosb += -ab * mu;
     }
@@ -299,7 +345,7 @@ __device__ __forceinline__ void refracting_step_backward(V3<R> p, V3<R> d, R t, 
         cb += -db * H;
         hb += -db * c;
     } else {
-        const R inv = R(1) / (R(2) * root_d);
+        const R inv = rev_rcp(R(2) * root_d);
         hb += sb * inv;
         bb += -sb * s * R(2) * inv;
         cb += sb * s * s * inv;
@@ -319,7 +365,7 @@ __device__ __forceinline__ void refracting_step_backward(V3<R> p, V3<R> d, R t, 
     s0b += xb * d.x + yb * d.y;
     in.d.x += xb * s0;
     in.d.y += yb * s0;
-    const R qq = s0b / d.z;
+    const R qq = s0b * rdz;
     tb += qq;
     in.p.z += -qq;
     in.d.z += -qq * s0;
@@ -418,20 +464,20 @@ __device__ __forceinline__ RayState<double> inverse_step(V3<double> q, V3<double
                                                          double n_j) {
     // inverse refraction at j: d = mu' d1 + (sqrt(k') - mu' cos') N, mu' = n_j / n_s
     const V3<double> N{-c_j * q.x, -c_j * q.y, 1.0 - c_j * q.z};
-    const double mu = n_j / n_s;
+    const double mu = rev_div(n_j, n_s);
     const double cosI = dot(d1, N);
     const double k = 1.0 - mu * mu * (1.0 - cosI * cosI);
-    const V3<double> d = mu * d1 + (dsqrt(k) - mu * cosI) * N;
+    const V3<double> d = mu * d1 + (rev_sqrt(k) - mu * cosI) * N;
     // reversed ray in the mirrored frame of s: origin (q.x, q.y, -(q.z + t_s)), direction
     // (-d.x, -d.y, d.z); transfer to the tangent plane z = 0 and intersect curvature -c_s
-    const double s0 = (q.z + t_s) / d.z;
+    const double s0 = rev_div(q.z + t_s, d.z);
     const double x = q.x - s0 * d.x;
     const double y = q.y - s0 * d.y;
     const double cm = -c_s;
     const double H = cm * (x * x + y * y);
     const double B = d.z + cm * (x * d.x + y * d.y);
     const double D = B * B - cm * H;
-    const double s = H / (B + dsqrt(D));
+    const double s = rev_div(H, B + rev_sqrt(D));
     RayState<double> out;
     out.p = V3<double>{x - s * d.x, y - s * d.y, -(s * d.z)};
     out.d = d;
