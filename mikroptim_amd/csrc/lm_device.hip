@@ -7,9 +7,9 @@
 // products and sums without contraction (fp contract off below), correctly rounded division and
 // square root (__ddiv_rn, __dsqrt_rn), std::max / std::min as (a < b) ? b : a / (b < a) ? b : a.
 // Given the same normal equations and merits, device and host control take bit-identical
-// trajectories. Sums keep the host order: the Cholesky factor and the forward substitution run
-// lane-parallel over rows with the inner sums in ascending order per lane; the back substitution
-// and the step model (lin, quad) run on lane 0.
+// trajectories. Sums keep the host order: the Cholesky factor and the two substitutions run
+// lane-parallel over rows, each lane summing in the host's order; the step model (lin, quad)
+// runs on lane 0; maxima (order-independent) are wave reductions.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -34,6 +34,12 @@ __device__ __forceinline__ double scale_of(double x, double typical) { return dm
 __device__ __forceinline__ bool blocks(const State& S, int k, double x, double dir) {
     if (S.lower[k] == S.upper[k]) return true;
     return (x <= S.lower[k] && dir < 0.0) || (x >= S.upper[k] && dir > 0.0);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = dmax(v, __shfl_xor(v, off, 64));
+    return v;
 }
 
 struct SolveLds {
@@ -111,12 +117,15 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 if (lane > k && lane < f) acc -= L.m[k * f + lane] * L.rhs[k];
             }
         }
-        // back substitution on lane 0 (each sum in ascending k, as on the host)
-        if (lane == 0) {
-            for (int i = f - 1; i >= 0; --i) {
-                double s = L.rhs[i];
-                for (int k = i + 1; k < f; ++k) s -= L.m[i * f + k] * L.rhs[k];
-                L.rhs[i] = __ddiv_rn(s, L.m[i * f + i]);
+        // back substitution: lane i subtracts l(k, i) x_k for k = f-1, f-2, ... as x_k is final
+        // (the host sums in the same descending order)
+        {
+            __syncthreads();
+            double acc = lane < f ? L.rhs[lane] : 0.0;
+            for (int k = f - 1; k >= 0; --k) {
+                if (lane == k) L.rhs[k] = __ddiv_rn(acc, L.m[k * f + k]);
+                __syncthreads();
+                if (lane < k) acc -= L.m[lane * f + k] * L.rhs[k];
             }
         }
         __syncthreads();
@@ -150,9 +159,8 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         L.aq[q] = aq;
     }
     __syncthreads();
+    const double nrm = wave_max(lane < n ? L.ratio[lane] : 0.0);
     if (lane == 0) {
-        double nrm = 0.0;
-        for (int k = 0; k < n; ++k) nrm = dmax(nrm, L.ratio[k]);
         double lin = 0.0, quad = 0.0;
         for (int q = 0; q < n; ++q) {
             lin += L.step[q] * g[q];
@@ -165,31 +173,34 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
     __syncthreads();
 }
 
-// prepare_step: scaling diagonal D (Marquardt with floor) and the scaled gradient norm.
+// prepare_step: scaling diagonal D (Marquardt with floor) and the scaled gradient norm. The
+// per-variable terms are formed lane-parallel; the two maxima are order-independent (finite
+// values), so they equal the host's sequential maxima.
 __device__ void prepare_step(const State& S, const Options& o, int run, const double* a,
                              const double* g) {
     const int n = S.P;
+    const int k = threadIdx.x;
     const double* x = S.x + size_t(run) * n;
-    double* d = S.d + size_t(run) * n;
-    double smax = 0.0;
-    for (int k = 0; k < n; ++k) {
-        const double sk = scale_of(x[k], S.typical[k]);
-        smax = dmax(smax, a[size_t(k) * n + k] * sk * sk);
+    double sk = 1.0, akk = 0.0, term = 0.0;
+    if (k < n) {
+        sk = scale_of(x[k], S.typical[k]);
+        akk = a[size_t(k) * n + k];
+        term = akk * sk * sk;
     }
+    const double smax = wave_max(dmax(0.0, term));
     const double merit = S.merit[run];
-    double gmax = 0.0;
-    for (int k = 0; k < n; ++k) {
-        const double sk = scale_of(x[k], S.typical[k]);
-        const double akk = a[size_t(k) * n + k];
+    double ratio = 0.0;
+    if (k < n) {
         const double floor = smax > 0.0 ? __ddiv_rn(o.diag_floor * smax, sk * sk)
                                         : __ddiv_rn(1.0, sk * sk);
-        d[k] = dmax(akk, floor);
+        S.d[size_t(run) * n + k] = dmax(akk, floor);
         if (!blocks(S, k, x[k], -g[k])) {
             const double den = __dsqrt_rn(akk * merit);
-            if (den > 0.0) gmax = dmax(gmax, __ddiv_rn(fabs(g[k]), den));
+            if (den > 0.0) ratio = __ddiv_rn(fabs(g[k]), den);
         }
     }
-    S.gradient_norm[run] = gmax;
+    const double gmax = wave_max(dmax(0.0, ratio));
+    if (k == 0) S.gradient_norm[run] = gmax;
 }
 
 __global__ __launch_bounds__(256) void init_kernel(State S, Options o) {
@@ -331,7 +342,7 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
     // the compact inputs are read-only in this kernel; the state copies serve later kernels
     const double* a = a_c + size_t(j) * n * n;
     const double* g = g_c + size_t(j) * n;
-    if (lane == 0) prepare_step(S, o, run, a, g);
+    prepare_step(S, o, run, a, g);
     __syncthreads();
     solve_step(S, run, a, g, kModelDamping, L);
     if (lane == 0) {

@@ -81,133 +81,11 @@ __device__ __forceinline__ bool is_soft(uint8_t s) {
     return s == MKO_STATUS_SOFT_MISSED || s == MKO_STATUS_SOFT_TIR;
 }
 
-// Accumulator of soft ray failure (SoftFailure, surface_ops.hpp:94).
-struct Soft {
-    double penalty;
-    uint8_t first;
-    __device__ __forceinline__ void add(double amount, uint8_t kind) {
-        penalty += amount;
-        if (first == MKO_STATUS_OK) first = kind;
-    }
-};
-
-template <class R>
-struct RayT {
-    V3<R> p, d;
-    R opl;
-};
-using Ray = RayT<double>;
-
-// One surface step (propagate_surface_local with transfer, sphere intersection and refraction).
-// c = effective curvature (0 for stop and plane), t = vertex distance from the previous surface,
-// nb = medium before, mu = nb / n_after. Returns a TraceStatus (Ok also for soft continuations,
-// whose kind goes to `soft` and whose branch bits go to `branch`).
-template <bool kSoft, class R>
-__device__ __forceinline__ uint8_t surface_step(RayT<R>& r, R c, R t, R nb, R mu, Soft& soft,
-                                                uint8_t& branch) {
-    const R zero = num<R>(0.0), one = num<R>(1.0);
-    if (!(r.d.z > zero)) return MKO_STATUS_BACKWARD;
-    // transfer_to_tangent_plane
-    r.p.z -= t;
-    const R s0 = -r.p.z / r.d.z;
-    r.p.x += s0 * r.d.x;
-    r.p.y += s0 * r.d.y;
-    r.p.z = zero;
-    // intersect_sphere
-    const R H = c * (r.p.x * r.p.x + r.p.y * r.p.y);
-    const R B = r.d.z - c * (r.p.x * r.d.x + r.p.y * r.d.y);
-    const R D = B * B - c * H;
-    R s;
-    bool missed = false;
-    if (D < zero) {
-        if (!kSoft || (!(c > zero) && !(c < zero))) return MKO_STATUS_MISSED;
-        soft.add(-to_double(D), MKO_STATUS_SOFT_MISSED);
-        s = B / c;
-        missed = true;
-    } else {
-        const R denom = B + dsqrt(D);
-        if (!(denom > zero)) return MKO_STATUS_MISSED;
-        s = H / denom;
-    }
-    r.p = r.p + s * r.d;
-    r.opl += nb * (s0 + s);
-    // refract_sphere
-    V3<R> N{-c * r.p.x, -c * r.p.y, one - c * r.p.z};
-    if (kSoft && missed) N = rnorm(N) * N;
-    const R cosI = dot(r.d, N);
-    const R k = one - mu * mu * (one - cosI * cosI);
-    if (k < zero) {
-        if (!kSoft) return MKO_STATUS_TIR;
-        soft.add(-to_double(k), MKO_STATUS_SOFT_TIR);
-        const V3<R> tau = r.d - cosI * N;
-        r.d = rnorm(tau) * tau;
-        branch = uint8_t((missed ? kBranchSoftMissed : 0) | kBranchSoftTIR);
-        return MKO_STATUS_OK;
-    }
-    r.d = mu * r.d + (dsqrt(k) - mu * cosI) * N;
-    branch = missed ? kBranchSoftMissed : kBranchRegular;
-    return MKO_STATUS_OK;
-}
-
-// reference_sphere_step (kind last_surface) / image_sphere_step (image kinds).
-template <bool kSoft, class R>
-__device__ __forceinline__ uint8_t reference_step(RayT<R>& r, V3<R> M, R kappa, int kind,
-                                                  R n_image, Soft& soft, uint8_t& branch) {
-    const R zero = num<R>(0.0), one = num<R>(1.0);
-    const V3<R> q = r.p - M;
-    const R b = -dot(q, r.d);
-    branch = kBranchRegular;
-    if (kind == MKO_REF_LAST_SURFACE) {
-        const R g = dot(r.p, r.p - num<R>(2.0) * M);
-        const R disc = b * b - g;
-        if (disc < zero) {
-            const R radius_sq = dot(M, M);
-            if (!kSoft || !(radius_sq > zero)) return MKO_STATUS_MISSED;
-            soft.add(-to_double(disc) / to_double(radius_sq), MKO_STATUS_SOFT_MISSED);
-            r.p = r.p + b * r.d;
-            r.opl += n_image * b;
-            branch = kBranchSoftMissed;
-            return MKO_STATUS_SOFT_MISSED;
-        }
-        const R root = dsqrt(disc);
-        const bool real_image = M.z >= zero;
-        const R denom = real_image ? b + root : b - root;
-        if (!(real_image ? denom > zero : denom < zero)) return MKO_STATUS_MISSED;
-        const R s = g / denom;
-        r.p = r.p + s * r.d;
-        r.opl += n_image * s;
-        return MKO_STATUS_OK;
-    }
-    const V3<R> cq = cross(q, r.d);
-    const R t = -dot(cq, cq);
-    const R disc = one + kappa * kappa * t;
-    R step;
-    uint8_t st = MKO_STATUS_OK;
-    if (disc >= zero) {
-        step = b - kappa * t / (dsqrt(disc) + one);
-    } else if (kSoft && disc < zero) {
-        step = b - kappa * t;
-        st = MKO_STATUS_SOFT_MISSED;
-    } else {
-        return MKO_STATUS_MISSED;
-    }
-    if (!(step >= zero) && !(step < zero)) return MKO_STATUS_MISSED;
-    if (st == MKO_STATUS_SOFT_MISSED) {
-        soft.add(-to_double(disc), MKO_STATUS_SOFT_MISSED);
-        branch = kBranchSoftMissed;
-    }
-    r.p = r.p + b * r.d;
-    r.opl += n_image * step;
-    return st;
-}
-
-// ---- reverse sweep ------------------------------------------------------------------------------
-
-// Division, reciprocal and square root of the reverse sweep and of the state reconstruction:
-// hardware reciprocal / reciprocal square root refined by Newton steps (a few ulp) instead of
-// the correctly rounded IEEE sequences with their scaling and fix-up steps. The sweep only needs
-// ~1e-15 relative accuracy (its results are compared to 1e-10 per Jacobian column); the forward
-// trace keeps the IEEE operations.
+// Division, reciprocal and square root of the reverse sweep, of the state reconstruction and of
+// the fused kernel's forward trace: hardware reciprocal / reciprocal square root refined by
+// Newton steps (a few ulp) instead of the correctly rounded IEEE sequences with their scaling and
+// fix-up steps. These paths need ~1e-15 relative accuracy (Jacobian columns are compared to
+// 1e-10, OPD to 1e-12 mm per ray); the df64 trace keeps its own operations.
 #ifndef MKO_REVERSE_FAST_MATH
 #define MKO_REVERSE_FAST_MATH 1
 #endif
@@ -252,6 +130,140 @@ template <class R>
 struct RayAdj {
     V3<R> p, d;
 };
+
+// Forward-trace division / square root: IEEE, or the Newton-refined forms above (kFast, double).
+template <bool kFast, class R>
+__device__ __forceinline__ R fdiv(R a, R b) {
+    if constexpr (kFast) return rev_div(a, b);
+    else return a / b;
+}
+template <bool kFast, class R>
+__device__ __forceinline__ R fsqrt(R x) {
+    if constexpr (kFast) return rev_sqrt(x);
+    else return dsqrt(x);
+}
+
+// Accumulator of soft ray failure (SoftFailure, surface_ops.hpp:94).
+struct Soft {
+    double penalty;
+    uint8_t first;
+    __device__ __forceinline__ void add(double amount, uint8_t kind) {
+        penalty += amount;
+        if (first == MKO_STATUS_OK) first = kind;
+    }
+};
+
+template <class R>
+struct RayT {
+    V3<R> p, d;
+    R opl;
+};
+using Ray = RayT<double>;
+
+// One surface step (propagate_surface_local with transfer, sphere intersection and refraction).
+// c = effective curvature (0 for stop and plane), t = vertex distance from the previous surface,
+// nb = medium before, mu = nb / n_after. Returns a TraceStatus (Ok also for soft continuations,
+// whose kind goes to `soft` and whose branch bits go to `branch`).
+template <bool kSoft, bool kFast = false, class R>
+__device__ __forceinline__ uint8_t surface_step(RayT<R>& r, R c, R t, R nb, R mu, Soft& soft,
+                                                uint8_t& branch) {
+    const R zero = num<R>(0.0), one = num<R>(1.0);
+    if (!(r.d.z > zero)) return MKO_STATUS_BACKWARD;
+    // transfer_to_tangent_plane
+    r.p.z -= t;
+    const R s0 = fdiv<kFast>(-r.p.z, r.d.z);
+    r.p.x += s0 * r.d.x;
+    r.p.y += s0 * r.d.y;
+    r.p.z = zero;
+    // intersect_sphere
+    const R H = c * (r.p.x * r.p.x + r.p.y * r.p.y);
+    const R B = r.d.z - c * (r.p.x * r.d.x + r.p.y * r.d.y);
+    const R D = B * B - c * H;
+    R s;
+    bool missed = false;
+    if (D < zero) {
+        if (!kSoft || (!(c > zero) && !(c < zero))) return MKO_STATUS_MISSED;
+        soft.add(-to_double(D), MKO_STATUS_SOFT_MISSED);
+        s = B / c;
+        missed = true;
+    } else {
+        const R denom = B + fsqrt<kFast>(D);
+        if (!(denom > zero)) return MKO_STATUS_MISSED;
+        s = fdiv<kFast>(H, denom);
+    }
+    r.p = r.p + s * r.d;
+    r.opl += nb * (s0 + s);
+    // refract_sphere
+    V3<R> N{-c * r.p.x, -c * r.p.y, one - c * r.p.z};
+    if (kSoft && missed) N = rnorm(N) * N;
+    const R cosI = dot(r.d, N);
+    const R k = one - mu * mu * (one - cosI * cosI);
+    if (k < zero) {
+        if (!kSoft) return MKO_STATUS_TIR;
+        soft.add(-to_double(k), MKO_STATUS_SOFT_TIR);
+        const V3<R> tau = r.d - cosI * N;
+        r.d = rnorm(tau) * tau;
+        branch = uint8_t((missed ? kBranchSoftMissed : 0) | kBranchSoftTIR);
+        return MKO_STATUS_OK;
+    }
+    r.d = mu * r.d + (fsqrt<kFast>(k) - mu * cosI) * N;
+    branch = missed ? kBranchSoftMissed : kBranchRegular;
+    return MKO_STATUS_OK;
+}
+
+// reference_sphere_step (kind last_surface) / image_sphere_step (image kinds).
+template <bool kSoft, bool kFast = false, class R>
+__device__ __forceinline__ uint8_t reference_step(RayT<R>& r, V3<R> M, R kappa, int kind,
+                                                  R n_image, Soft& soft, uint8_t& branch) {
+    const R zero = num<R>(0.0), one = num<R>(1.0);
+    const V3<R> q = r.p - M;
+    const R b = -dot(q, r.d);
+    branch = kBranchRegular;
+    if (kind == MKO_REF_LAST_SURFACE) {
+        const R g = dot(r.p, r.p - num<R>(2.0) * M);
+        const R disc = b * b - g;
+        if (disc < zero) {
+            const R radius_sq = dot(M, M);
+            if (!kSoft || !(radius_sq > zero)) return MKO_STATUS_MISSED;
+            soft.add(-to_double(disc) / to_double(radius_sq), MKO_STATUS_SOFT_MISSED);
+            r.p = r.p + b * r.d;
+            r.opl += n_image * b;
+            branch = kBranchSoftMissed;
+            return MKO_STATUS_SOFT_MISSED;
+        }
+        const R root = fsqrt<kFast>(disc);
+        const bool real_image = M.z >= zero;
+        const R denom = real_image ? b + root : b - root;
+        if (!(real_image ? denom > zero : denom < zero)) return MKO_STATUS_MISSED;
+        const R s = fdiv<kFast>(g, denom);
+        r.p = r.p + s * r.d;
+        r.opl += n_image * s;
+        return MKO_STATUS_OK;
+    }
+    const V3<R> cq = cross(q, r.d);
+    const R t = -dot(cq, cq);
+    const R disc = one + kappa * kappa * t;
+    R step;
+    uint8_t st = MKO_STATUS_OK;
+    if (disc >= zero) {
+        step = b - fdiv<kFast>(kappa * t, fsqrt<kFast>(disc) + one);
+    } else if (kSoft && disc < zero) {
+        step = b - kappa * t;
+        st = MKO_STATUS_SOFT_MISSED;
+    } else {
+        return MKO_STATUS_MISSED;
+    }
+    if (!(step >= zero) && !(step < zero)) return MKO_STATUS_MISSED;
+    if (st == MKO_STATUS_SOFT_MISSED) {
+        soft.add(-to_double(disc), MKO_STATUS_SOFT_MISSED);
+        branch = kBranchSoftMissed;
+    }
+    r.p = r.p + b * r.d;
+    r.opl += n_image * step;
+    return st;
+}
+
+// ---- reverse sweep ------------------------------------------------------------------------------
 
 // Reverse of one stop/plane/sphere step (refracting_step_backward). (p, d): state before the
 // surface; adj: adjoint of the state after it on entry, of the state before it on return.
