@@ -615,7 +615,7 @@ public:
     void jacobian_dev(const mko_batch& tmpl, const int32_t* tidx, int count, const double* xd,
                       const mko_residual_settings& s, int P, const double* lower,
                       const double* upper, const double* typical, int method, int precision,
-                      double* ad, double* gd) {
+                      double* ad, double* gd, const mko_internal::RayStates* states = nullptr) {
         DeviceGuard guard(device_);
         if (count <= 0) return;
         const int64_t R = trace_rows_of(tmpl, s), rays = rays_of(tmpl, s);
@@ -661,9 +661,16 @@ public:
                 const size_t wbytes =
                     mko_adjoint_normal_workspace(c, tmpl.fields, tmpl.wavelengths, P);
                 void* ws = jac_.reserve(wbytes);
+                mko_internal::RayStates st{};
+                if (states) {
+                    st = *states;
+                    if (st.slot) st.slot += first;
+                    else st.slot_offset += first;
+                }
                 timer.run(kAdjoint, [&] {
-                    dev_check(mko_adjoint_normal_device(&work_.view, &s, &vars_host_dev_, a_out,
-                                                        g_out, ws, wbytes, nullptr));
+                    dev_check(mko_internal::adjoint_normal_states(&work_.view, &s, &vars_host_dev_,
+                                                                  a_out, g_out, ws, wbytes, st,
+                                                                  nullptr));
                 });
                 continue;
             }
@@ -696,7 +703,7 @@ public:
     // hard-failed counts into device arrays.
     void merit_dev(const mko_batch& tmpl, const int32_t* tidx, int count, const double* xd,
                    const mko_residual_settings& s, int P, double* md, int64_t* fd,
-                   int64_t* hd) {
+                   int64_t* hd, const mko_internal::RayStates* states = nullptr) {
         DeviceGuard guard(device_);
         if (count <= 0) return;
         const size_t T = size_t(tmpl.fields) * tmpl.wavelengths;
@@ -717,8 +724,16 @@ public:
             double* part = part_.as<double>(size_t(c) * T);
             int* pc = partc_.as<int>(size_t(c) * T * 2);
             timer.run(kTrace, [&] {
-                dev_check(mko_merit_device(&work_.view, &s, nullptr, md + first, fd + first,
-                                           hd + first, part, pc, nullptr));
+                if (states) {
+                    mko_internal::RayStates st = *states;
+                    if (st.slot) st.slot += first;
+                    else st.slot_offset += first;
+                    dev_check(mko_internal::merit_states(&work_.view, &s, md + first, fd + first,
+                                                         hd + first, part, pc, st, nullptr));
+                } else {
+                    dev_check(mko_merit_device(&work_.view, &s, nullptr, md + first, fd + first,
+                                               hd + first, part, pc, nullptr));
+                }
             });
         }
     }
@@ -1247,6 +1262,25 @@ private:
         dopt_.diag_floor = o_.diag_floor;
     }
 
+    // The LM's MERIT traces keep the final ray states for the fused mixed-precision adjoint at
+    // the same points (hard failure, FP64 trace, fused normal equations), if they fit in half
+    // of the engine's memory budget.
+    void dev_allocate_states(int n) {
+        use_states_ = false;
+        if (o_.jacobian_method != MKO_JACOBIAN_ADJOINT || precision_ != MKO_PRECISION_MIXED ||
+            !Engine::fused_ok(settings_, int(P_)) || settings_.precision != MKO_PRECISION_FP64)
+            return;
+        const int64_t rays = rays_of(base_, settings_);
+        const size_t bytes = size_t(n) * size_t(rays) * 7 * sizeof(double);
+        if (bytes > eng_.budget() / 2) return;
+        states_.data = states_mem_.as<double>(size_t(n) * size_t(rays) * 7);
+        states_.slot = nullptr;
+        states_.slot_offset = 0;
+        states_.slots = n;
+        states_.rays = rays;
+        use_states_ = true;
+    }
+
     // State arrays of n runs in one allocation.
     void dev_allocate(int n) {
         const size_t N = size_t(n), P = P_;
@@ -1309,6 +1343,7 @@ private:
         const int n = ids.empty() ? runs_ : int(ids.size());
         state_.assign(size_t(n), Run());
         dev_allocate(n);
+        dev_allocate_states(n);
         hip_check(hipMemcpy(ds_.x, xs.data(), size_t(n) * P_ * 8, hipMemcpyHostToDevice), "x");
         std::vector<int32_t> index(static_cast<size_t>(n));
         for (int i = 0; i < n; ++i) index[size_t(i)] = ids.empty() ? i : ids[size_t(i)];
@@ -1319,7 +1354,8 @@ private:
         double* fm = fm_.as<double>(size_t(n));
         int64_t* ff = ff_.as<int64_t>(size_t(n));
         int64_t* fh = fh_.as<int64_t>(size_t(n));
-        eng_.merit_dev(dev_tmpl_.view, dev_round_index(), n, ds_.x, settings_, int(P_), fm, ff, fh);
+        eng_.merit_dev(dev_tmpl_.view, dev_round_index(), n, ds_.x, settings_, int(P_), fm, ff, fh,
+                       use_states_ ? &states_ : nullptr);
         eng_.timer.run(kLmControl, [&] { mko_lmdev::launch_start(ds_, fm, ff, fh, nullptr); });
         hip_check(hipGetLastError(), "LM start");
     }
@@ -1344,8 +1380,11 @@ private:
             });
             double* ac = ac_.as<double>(size_t(cj) * P_ * P_);
             double* gc = gc_.as<double>(size_t(cj) * P_);
+            mko_internal::RayStates st = states_;
+            st.slot = list_j;
             eng_.jacobian_dev(dev_tmpl_.view, tidx, cj, xs, settings_, int(P_), ds_.lower,
-                              ds_.upper, ds_.typical, o_.jacobian_method, precision_, ac, gc);
+                              ds_.upper, ds_.typical, o_.jacobian_method, precision_, ac, gc,
+                              use_states_ ? &st : nullptr);
             eng_.timer.run(kLmControl, [&] {
                 mko_lmdev::launch_prepare(ds_, dopt_, list_j, cj, ac, gc, nullptr);
             });
@@ -1370,7 +1409,10 @@ private:
             eng_.timer.run(kLmControl, [&] {
                 mko_lmdev::launch_gather(ds_, list_e, ce, true, rix, xs, tidx, nullptr);
             });
-            eng_.merit_dev(dev_tmpl_.view, tidx, ce, xs, settings_, int(P_), fm, ff, fh);
+            mko_internal::RayStates st = states_;
+            st.slot = list_e;
+            eng_.merit_dev(dev_tmpl_.view, tidx, ce, xs, settings_, int(P_), fm, ff, fh,
+                           use_states_ ? &st : nullptr);
         }
         eng_.timer.run(kLmControl, [&] {
             mko_lmdev::launch_update(ds_, dopt_, list_c, cc, pos, fm, ff, fh, nullptr);
@@ -1437,6 +1479,9 @@ private:
     mko_lmdev::Options dopt_{};
     DeviceBatch dev_tmpl_;
     Buffer state_mem_, lists_mem_, round_mem_, bounds_d_, xs_d_, tidx_d_, ac_, gc_, fm_, ff_, fh_;
+    bool use_states_ = false;
+    mko_internal::RayStates states_{};
+    Buffer states_mem_;
     int32_t* lists_ = nullptr;
     int32_t* counts_ = nullptr;
     int32_t* round_index_ = nullptr;

@@ -149,10 +149,11 @@ __device__ __forceinline__ Sample sample_of(const TraceParams& P, const Tuple& t
 
 // Surface table of (variant, wavelength) in LDS.
 struct SurfLds {
-    double c[MKO_MAX_SURFACES];  // effective curvature (0 for stop/plane)
-    double t[MKO_MAX_SURFACES];  // thickness[s]
-    double n[MKO_MAX_SURFACES];  // index[s][l]
-    double mu[MKO_MAX_SURFACES]; // n[s-1] / n[s]
+    double c[MKO_MAX_SURFACES];   // effective curvature (0 for stop/plane)
+    double t[MKO_MAX_SURFACES];   // thickness[s]
+    double n[MKO_MAX_SURFACES];   // index[s][l]
+    double mu[MKO_MAX_SURFACES];  // n[s-1] / n[s]
+    double imu[MKO_MAX_SURFACES]; // n[s] / n[s-1]
 };
 
 __device__ __forceinline__ void stage_surfaces(const TraceParams& P, const Tuple& t, SurfLds& L) {
@@ -165,7 +166,10 @@ __device__ __forceinline__ void stage_surfaces(const TraceParams& P, const Tuple
         L.n[s] = P.index[vs * P.L + t.l];
     }
     __syncthreads();
-    if (s >= 1 && s < P.S) L.mu[s] = __ddiv_rn(L.n[s - 1], L.n[s]);
+    if (s >= 1 && s < P.S) {
+        L.mu[s] = __ddiv_rn(L.n[s - 1], L.n[s]);
+        L.imu[s] = __ddiv_rn(L.n[s], L.n[s - 1]);
+    }
     __syncthreads();
 }
 
@@ -211,6 +215,19 @@ struct NoRecord {
     template <class RR>
     __device__ __forceinline__ void put(int, const RR&, uint8_t) {}
     uint8_t image_branch;
+};
+
+// Last state of a forward trace (the state on surface S-1, which the reference-sphere step starts
+// from); the reverse sweep reconstructs the earlier states with mkod::inverse_step.
+struct LastState {
+    V3<double> p, d;
+    uint8_t image_branch;
+    int count;
+    __device__ __forceinline__ void put(int s, const Ray& r, uint8_t) {
+        p = r.p;
+        d = r.d;
+        count = s + 1;
+    }
 };
 
 // trace_to_reference_sphere (reference_sphere.hpp:183-222) over the LDS table, in the number
@@ -275,11 +292,18 @@ __device__ __forceinline__ T block_sum(T v, T* lds) {
 // Mode 0: RESIDUALS (residual rows, penalty rows, OPD, status). Mode 1: MERIT partials per
 // (variant, tuple): sum r^2 + sum p^2, failed and hard-failed counts. R: number type of the ray
 // trace (double or df64); OPL, piston and residuals are formed in double.
-template <int kMode, bool kSoft, int kRpl, class R>
+__device__ __forceinline__ int64_t state_index(const mko_internal::RayStates& rs, int v, int c,
+                                               int64_t row) {
+    const int64_t slot = rs.slot ? rs.slot[v] : int64_t(rs.slot_offset) + v;
+    return (int64_t(c) * rs.slots + slot) * rs.rays + row;
+}
+
+template <int kMode, bool kSoft, int kRpl, class R, bool kStates = false>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* residuals,
                                                        double* opd, uint8_t* status,
                                                        double* partial, int* pfailed,
-                                                       int* phard) {
+                                                       int* phard,
+                                                       mko_internal::RayStates states) {
     __shared__ SurfLds L;
     __shared__ double red[kWaves];
     __shared__ int ired[kWaves];
@@ -305,9 +329,28 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
             const Sample s = sample_of(P, t, k);
             mkod::RayT<R> r = ray_as<R>(source_ray(P, t, s, a, sin_t, cos_t, L.n[0]));
             Soft soft{0.0, MKO_STATUS_OK};
-            NoRecord nr;
-            st[i] = trace_ray<kSoft, kTraceFastMath && std::is_same_v<R, double>>(P, L, r, M, kappa,
-                                                                                soft, nr);
+            constexpr bool kFast = kTraceFastMath && std::is_same_v<R, double>;
+            if constexpr (kStates) {
+                LastState last;
+                last.count = 0;
+                last.image_branch = 0;
+                st[i] = trace_ray<kSoft, kFast>(P, L, r, M, kappa, soft, last);
+                const int64_t row = t.offset + k;
+                const bool okr = st[i] == MKO_STATUS_OK;
+                if (okr) {
+                    states.data[state_index(states, t.v, 0, row)] = last.p.x;
+                    states.data[state_index(states, t.v, 1, row)] = last.p.y;
+                    states.data[state_index(states, t.v, 2, row)] = last.p.z;
+                    states.data[state_index(states, t.v, 3, row)] = last.d.x;
+                    states.data[state_index(states, t.v, 4, row)] = last.d.y;
+                    states.data[state_index(states, t.v, 5, row)] = last.d.z;
+                }
+                states.data[state_index(states, t.v, 6, row)] =
+                    okr ? mkod::to_double(r.opl) : __builtin_nan("");
+            } else {
+                NoRecord nr;
+                st[i] = trace_ray<kSoft, kFast>(P, L, r, M, kappa, soft, nr);
+            }
             opl[i] = mkod::to_double(r.opl);
             pen[i] = soft.penalty;
             if (mkod::in_piston(st[i])) {
@@ -698,19 +741,6 @@ struct LdsSink {
     }
 };
 
-// Last state of a forward trace (the state on surface S-1, which the reference-sphere step starts
-// from); the reverse sweep reconstructs the earlier states with mkod::inverse_step.
-struct LastState {
-    V3<double> p, d;
-    uint8_t image_branch;
-    int count;
-    __device__ __forceinline__ void put(int s, const Ray& r, uint8_t) {
-        p = r.p;
-        d = r.d;
-        count = s + 1;
-    }
-};
-
 // backward_ray (adjoint_backward.hpp:496-580) for a ray that reached the image in hard mode, with
 // the states on surfaces S-2 ... 1 reconstructed from the last state by inverse steps and the
 // state on surface 0 regenerated from the source ray (no per-ray record in memory).
@@ -731,28 +761,24 @@ __device__ __forceinline__ void backward_to_lds(const TraceParams& P, const Surf
         sink.write(map.thick_col[S - 1], double(mz));
     }
     V3<double> q = last.p, d1 = last.d;
-    for (int s = S - 2; s >= 0; --s) {
+    for (int s = S - 2; s >= 1; --s) {
         const int j = s + 1;
-        V3<double> ps, ds;
-        if (s == 0) {
-            ps = start.p;
-            ds = start.d;
-        } else {
-            const mkod::RayState<double> st =
-                mkod::inverse_step(q, d1, L.c[s], L.t[s], L.c[j], L.n[s], L.n[j]);
-            ps = st.p;
-            ds = st.d;
-        }
-        const V3<RB> p{RB(ps.x), RB(ps.y), RB(ps.z)};
-        const V3<RB> d{RB(ds.x), RB(ds.y), RB(ds.z)};
+        double tb = 0.0, cb = 0.0;
+        const mkod::RayState<double> st = mkod::fused_step_backward(
+            q, d1, L.c[j], L.c[s], L.t[s], L.n[s], L.mu[j], L.imu[j], adj, tb, cb);
+        if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], cb);
+        sink.write(map.thick_col[s], tb);
+        q = st.p;
+        d1 = st.d;
+    }
+    // surface 1 from the regenerated source ray (exact state on surface 0)
+    {
         RB tb = RB(0), cb = RB(0), nbb = RB(0), nab = RB(0);
-        mkod::refracting_step_backward<RB>(p, d, RB(L.t[s]), RB(L.c[j]), RB(L.n[s]), RB(L.n[j]),
-                                           mkod::kBranchRegular, RB(1), RB(0), adj, tb, cb, nbb,
-                                           nab);
-        if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], double(cb));
-        sink.write(map.thick_col[s], double(tb));
-        q = ps;
-        d1 = ds;
+        mkod::refracting_step_backward<RB>(start.p, start.d, RB(L.t[0]), RB(L.c[1]), RB(L.n[0]),
+                                           RB(L.n[1]), mkod::kBranchRegular, RB(1), RB(0), adj, tb,
+                                           cb, nbb, nab);
+        if ((P.sphere_mask >> 1) & 1ull) sink.write(map.curv_col[1], double(cb));
+        sink.write(map.thick_col[0], double(tb));
     }
 }
 
@@ -762,10 +788,13 @@ __device__ __forceinline__ void backward_to_lds(const TraceParams& P, const Surf
 // half: C on MFMA (v_mfma_f32_32x32x16_bf16, u split into bf16 hi + lo, hi*hi + hi*lo + lo*hi;
 // waves 0, 1, 3 form the 32 x 32 tiles (0,0), (0,1), (1,1) of the padded 64 x 64 C, the tile
 // below the diagonal is its mirror), S and Tr on VALU in double.
-template <class RB, int kHalves>
+// kStates: the last state and OPL of each ray come from a MERIT trace of the same point
+// (mko_internal::RayStates) instead of a forward trace here.
+template <class RB, int kHalves, bool kStates = false>
 __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, ParamMap map,
                                                                 float* partC, double* partST,
-                                                                TupleNormal* partT) {
+                                                                TupleNormal* partT,
+                                                                mko_internal::RayStates states) {
     extern __shared__ float G[]; // [P][kGStride]
     __shared__ SurfLds L;
     __shared__ double red[kWaves];
@@ -813,10 +842,25 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
             smp = sample_of(P, t, k);
             r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
             start = r;
-            Soft soft{0.0, MKO_STATUS_OK};
             last.count = 0;
             last.image_branch = 0;
-            ok = trace_ray<false, true>(P, L, r, M, kappa, soft, last) == MKO_STATUS_OK;
+            if constexpr (kStates) {
+                const int64_t row = t.offset + k;
+                r.opl = states.data[state_index(states, t.v, 6, row)];
+                ok = !__builtin_isnan(r.opl);
+                if (ok) {
+                    last.p = V3<double>{states.data[state_index(states, t.v, 0, row)],
+                                        states.data[state_index(states, t.v, 1, row)],
+                                        states.data[state_index(states, t.v, 2, row)]};
+                    last.d = V3<double>{states.data[state_index(states, t.v, 3, row)],
+                                        states.data[state_index(states, t.v, 4, row)],
+                                        states.data[state_index(states, t.v, 5, row)]};
+                    last.count = P.S;
+                }
+            } else {
+                Soft soft{0.0, MKO_STATUS_OK};
+                ok = trace_ray<false, true>(P, L, r, M, kappa, soft, last) == MKO_STATUS_OK;
+            }
         }
         const uint64_t okmask = __ballot(ok);
         if (ok) {
@@ -959,36 +1003,73 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
 }
 
 // a = sum_t s_t^2 (C_t - S_t S_t^T / W_t), g = sum_t s_t^2 (Tr_t - (piston_t - o_t) S_t), tuples
-// in (field, wavelength) order; one workgroup per run.
+// in (field, wavelength) order; one workgroup per run. Each thread owns up to 8 entries of the
+// upper triangle (mirrored on output); the tuples' S, Tr and factors are staged through LDS in
+// chunks of 8 tuples.
+constexpr int kFinalizeChunk = 8;
+constexpr int kFinalizeEntries = (kMaxVariablesFused * (kMaxVariablesFused + 1) / 2 + 255) / 256;
+
 __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
     int T, int Pn, const float* partC, const double* partST, const TupleNormal* partT, double* a,
     double* g) {
+    __shared__ double S_l[kFinalizeChunk][2 * kMaxVariablesFused];
+    __shared__ double f1[kFinalizeChunk], f2[kFinalizeChunk], pist[kFinalizeChunk];
     const int v = blockIdx.x;
-    for (int e = threadIdx.x; e < Pn * Pn; e += blockDim.x) {
-        const int q = e / Pn, p = e - q * Pn;
-        double accv = 0.0;
-        for (int t = 0; t < T; ++t) {
-            const size_t tuple = size_t(v) * T + t;
-            const TupleNormal tn = partT[tuple];
-            if (!(tn.w_ok > 0.0)) continue;
-            const double* S = partST + tuple * 2 * Pn;
-            const int lo = q < p ? q : p, hi = q < p ? p : q; // C is stored above the diagonal
-            const double c = double(partC[tuple * size_t(Pn) * Pn + size_t(lo) * Pn + hi]);
-            accv += tn.s2 * (c - S[p] * S[q] / tn.w_ok);
+    const int tid = threadIdx.x;
+    const int nu = Pn * (Pn + 1) / 2;
+    int eq[kFinalizeEntries], ep[kFinalizeEntries];
+    double acc[kFinalizeEntries];
+#pragma unroll
+    for (int i = 0; i < kFinalizeEntries; ++i) {
+        acc[i] = 0.0;
+        // upper-triangle index u -> (q, p), q <= p, row-major over q
+        const int u = tid + i * 256;
+        int q = 0, rem = u;
+        while (q < Pn && rem >= Pn - q) {
+            rem -= Pn - q;
+            ++q;
         }
-        a[size_t(v) * Pn * Pn + e] = accv;
+        eq[i] = q;
+        ep[i] = q + rem;
     }
-    for (int p = threadIdx.x; p < Pn; p += blockDim.x) {
-        double accv = 0.0;
-        for (int t = 0; t < T; ++t) {
-            const size_t tuple = size_t(v) * T + t;
-            const TupleNormal tn = partT[tuple];
-            if (!(tn.w_ok > 0.0)) continue;
-            const double* S = partST + tuple * 2 * Pn;
-            accv += tn.s2 * (S[Pn + p] - tn.piston * S[p]);
+    double gacc = 0.0;
+    for (int t0 = 0; t0 < T; t0 += kFinalizeChunk) {
+        const int nt = min(kFinalizeChunk, T - t0);
+        for (int e = tid; e < nt * 2 * Pn; e += 256) {
+            const int tt = e / (2 * Pn), i = e - tt * 2 * Pn;
+            S_l[tt][i] = partST[(size_t(v) * T + t0 + tt) * 2 * Pn + i];
         }
-        g[size_t(v) * Pn + p] = accv;
+        if (tid < nt) {
+            const TupleNormal tn = partT[size_t(v) * T + t0 + tid];
+            const bool valid = tn.w_ok > 0.0;
+            f1[tid] = valid ? tn.s2 : 0.0;
+            f2[tid] = valid ? tn.s2 / tn.w_ok : 0.0;
+            pist[tid] = valid ? tn.piston : 0.0;
+        }
+        __syncthreads();
+        for (int tt = 0; tt < nt; ++tt) {
+            const float* C = partC + (size_t(v) * T + t0 + tt) * size_t(Pn) * Pn;
+#pragma unroll
+            for (int i = 0; i < kFinalizeEntries; ++i) {
+                if (tid + i * 256 < nu) {
+                    const int q = eq[i], p = ep[i];
+                    const double c = double(C[size_t(q) * Pn + p]);
+                    acc[i] += f1[tt] * c - f2[tt] * (S_l[tt][p] * S_l[tt][q]);
+                }
+            }
+            if (tid < Pn) gacc += f1[tt] * (S_l[tt][Pn + tid] - pist[tt] * S_l[tt][tid]);
+        }
+        __syncthreads();
     }
+    double* A = a + size_t(v) * Pn * Pn;
+#pragma unroll
+    for (int i = 0; i < kFinalizeEntries; ++i) {
+        if (tid + i * 256 < nu) {
+            A[size_t(eq[i]) * Pn + ep[i]] = acc[i];
+            A[size_t(ep[i]) * Pn + eq[i]] = acc[i];
+        }
+    }
+    if (tid < Pn) g[size_t(v) * Pn + tid] = gacc;
 }
 
 // ---- Gram: A = [J r0]^T [J r0] per run -----------------------------------------------------
@@ -1140,7 +1221,7 @@ void launch_trace_rpl(int rpl, const TraceParams& P, double* res, double* opd, u
     const dim3 grid(unsigned(P.V) * unsigned(P.T));
 #define MKO_LAUNCH(RPL)                                                                          \
     hipLaunchKernelGGL((trace_kernel<kMode, kSoft, RPL, R>), grid, dim3(kBlock), 0, stream, P,    \
-                       res, opd, st, part, pf, ph)
+                       res, opd, st, part, pf, ph, mko_internal::RayStates{})
     switch (rpl) {
     case 1: MKO_LAUNCH(1); break;
     case 2: MKO_LAUNCH(2); break;
@@ -1267,6 +1348,93 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
     })
 }
 
+int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* settings,
+                          const mko_variables* vars, double* a, double* g, void* workspace,
+                          size_t workspace_bytes, const RayStates& states, void* stream) {
+    MKO_TRY({
+        const TraceParams P = make_params(*batch, *settings);
+        const ParamMap m = make_map(*vars, batch->surfaces);
+        if (P.soft) throw std::invalid_argument("fused adjoint normal equations need hard failure mode");
+        if (m.P > 64) throw std::invalid_argument("fused adjoint normal equations: at most 64 variables");
+        const int rpl = rays_per_lane(P);
+        if (rpl > 2) throw std::invalid_argument("fused adjoint normal equations: at most 512 rays per tuple");
+        if (P.V == 0) return 0;
+        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
+        const size_t need = mko_adjoint_normal_workspace(P.V, P.F, P.L, m.P);
+        if (workspace_bytes < need) throw std::invalid_argument("workspace too small");
+        const size_t tuples = size_t(P.V) * P.T;
+        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+        char* w = static_cast<char*>(workspace);
+        float* partC = reinterpret_cast<float*>(w);
+        double* partST = reinterpret_cast<double*>(w + up(tuples * m.P * m.P * sizeof(float)));
+        TupleNormal* partT = reinterpret_cast<TupleNormal*>(
+            w + up(tuples * m.P * m.P * sizeof(float)) + up(tuples * 2 * m.P * sizeof(double)));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t lds = size_t(m.P) * kGStride * sizeof(float);
+        const dim3 grid{unsigned(tuples)}, block{unsigned(kBlock)};
+        if (states.data) {
+            if (P.precision != MKO_PRECISION_FP64)
+                throw std::invalid_argument("stored ray states need the FP64 trace");
+            if (states.rays != P.rays) throw std::invalid_argument("ray states of another layout");
+            if (rpl == 1)
+                hipLaunchKernelGGL((adjoint_normal_kernel<double, 1, true>), grid, block, lds, s, P,
+                                   m, partC, partST, partT, states);
+            else
+                hipLaunchKernelGGL((adjoint_normal_kernel<double, 2, true>), grid, block, lds, s, P,
+                                   m, partC, partST, partT, states);
+        } else if (rpl == 1) {
+            hipLaunchKernelGGL((adjoint_normal_kernel<double, 1>), grid, block, lds, s, P, m, partC,
+                               partST, partT, states);
+        } else {
+            hipLaunchKernelGGL((adjoint_normal_kernel<double, 2>), grid, block, lds, s, P, m, partC,
+                               partST, partT, states);
+        }
+        check_hip(hipGetLastError(), "adjoint normal launch");
+        hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
+                           m.P, partC, partST, partT, a, g);
+        check_hip(hipGetLastError(), "adjoint normal finalize launch");
+    })
+}
+
+int merit_states(const mko_batch* batch, const mko_residual_settings* settings, double* merit,
+                 int64_t* failed, int64_t* hard_failed, double* partial, int* partial_counts,
+                 const RayStates& states, void* stream) {
+    MKO_TRY({
+        const TraceParams P = make_params(*batch, *settings);
+        if (P.soft || P.precision != MKO_PRECISION_FP64)
+            throw std::invalid_argument("ray states need hard failure mode and the FP64 trace");
+        if (states.rays != P.rays) throw std::invalid_argument("ray states of another layout");
+        if (P.V == 0) return 0;
+        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        int* pf = partial_counts;
+        int* ph = partial_counts + size_t(P.V) * P.T;
+        const dim3 grid(unsigned(P.V) * unsigned(P.T));
+        switch (rays_per_lane(P)) {
+        case 1:
+            hipLaunchKernelGGL((trace_kernel<1, false, 1, double, true>), grid, dim3(kBlock), 0, s,
+                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
+            break;
+        case 2:
+            hipLaunchKernelGGL((trace_kernel<1, false, 2, double, true>), grid, dim3(kBlock), 0, s,
+                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
+            break;
+        case 4:
+            hipLaunchKernelGGL((trace_kernel<1, false, 4, double, true>), grid, dim3(kBlock), 0, s,
+                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
+            break;
+        default:
+            hipLaunchKernelGGL((trace_kernel<1, false, 8, double, true>), grid, dim3(kBlock), 0, s,
+                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
+            break;
+        }
+        check_hip(hipGetLastError(), "merit launch");
+        hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
+                           P.T, partial, pf, ph, nullptr, merit, failed, hard_failed);
+        check_hip(hipGetLastError(), "merit finalize launch");
+    })
+}
+
 } // namespace mko_internal
 
 extern "C" {
@@ -1360,38 +1528,8 @@ size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, i
 int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_settings* settings,
                               const mko_variables* vars, double* a, double* g, void* workspace,
                               size_t workspace_bytes, void* stream) {
-    MKO_TRY({
-        const TraceParams P = make_params(*batch, *settings);
-        const ParamMap m = make_map(*vars, batch->surfaces);
-        if (P.soft) throw std::invalid_argument("fused adjoint normal equations need hard failure mode");
-        if (m.P > 64) throw std::invalid_argument("fused adjoint normal equations: at most 64 variables");
-        const int rpl = rays_per_lane(P);
-        if (rpl > 2) throw std::invalid_argument("fused adjoint normal equations: at most 512 rays per tuple");
-        if (P.V == 0) return 0;
-        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
-        const size_t need = mko_adjoint_normal_workspace(P.V, P.F, P.L, m.P);
-        if (workspace_bytes < need) throw std::invalid_argument("workspace too small");
-        const size_t tuples = size_t(P.V) * P.T;
-        auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-        char* w = static_cast<char*>(workspace);
-        float* partC = reinterpret_cast<float*>(w);
-        double* partST = reinterpret_cast<double*>(w + up(tuples * m.P * m.P * sizeof(float)));
-        TupleNormal* partT = reinterpret_cast<TupleNormal*>(
-            w + up(tuples * m.P * m.P * sizeof(float)) + up(tuples * 2 * m.P * sizeof(double)));
-        const hipStream_t s = static_cast<hipStream_t>(stream);
-        const size_t lds = size_t(m.P) * kGStride * sizeof(float);
-        const dim3 grid{unsigned(tuples)}, block{unsigned(kBlock)};
-        if (rpl == 1)
-            hipLaunchKernelGGL((adjoint_normal_kernel<double, 1>), grid, block, lds, s, P, m,
-                               partC, partST, partT);
-        else
-            hipLaunchKernelGGL((adjoint_normal_kernel<double, 2>), grid, block, lds, s, P, m,
-                               partC, partST, partT);
-        check_hip(hipGetLastError(), "adjoint normal launch");
-        hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
-                           m.P, partC, partST, partT, a, g);
-        check_hip(hipGetLastError(), "adjoint normal finalize launch");
-    })
+    return mko_internal::adjoint_normal_states(batch, settings, vars, a, g, workspace,
+                                               workspace_bytes, mko_internal::RayStates{}, stream);
 }
 
 int mko_df64_ops_device(int op, int64_t n, const float* a, const float* b, float* out,

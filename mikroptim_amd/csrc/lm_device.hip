@@ -7,9 +7,10 @@
 // products and sums without contraction (fp contract off below), correctly rounded division and
 // square root (__ddiv_rn, __dsqrt_rn), std::max / std::min as (a < b) ? b : a / (b < a) ? b : a.
 // Given the same normal equations and merits, device and host control take bit-identical
-// trajectories. Sums keep the host order: the Cholesky factor and the two substitutions run
-// lane-parallel over rows, each lane summing in the host's order; the step model (lin, quad)
-// runs on lane 0; maxima (order-independent) are wave reductions.
+// trajectories. Sums keep the host order: the Cholesky factor (right-looking) and the two
+// substitutions run lane-parallel over rows, each entry receiving its subtractions in the host's
+// order; the step model (lin, quad) runs on lane 0; maxima (order-independent) are wave
+// reductions.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -88,15 +89,13 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         }
         if (lane < f) L.rhs[lane] = -g[L.idx[lane]];
         __syncthreads();
-        // Cholesky, column j: every lane i >= j forms its sum over k < j in ascending order
+        // Cholesky, right-looking: after column j is final, every lane i > j subtracts
+        // l(i, j) l(k, j) from its entries (k, i), j < k <= i. Each entry receives the same
+        // subtractions in the same (ascending j) order as in the host's left-looking loop, so the
+        // factor is bitwise the host's; the updates of a lane are independent of each other.
         for (int j = 0; j < f; ++j) {
             const int i = lane;
-            double s = 0.0;
-            if (i >= j && i < f) {
-                s = L.m[j * f + i];
-                for (int k = 0; k < j; ++k) s -= L.m[k * f + i] * L.m[k * f + j];
-            }
-            if (i == j) L.diag = s;
+            if (i == j) L.diag = L.m[j * f + j];
             __syncthreads();
             const double diag = L.diag;
             if (!(diag > 0.0) || !isfinite(diag)) {
@@ -105,9 +104,14 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             }
             const double l = __dsqrt_rn(diag);
             if (i == j) L.m[j * f + j] = l;
-            else if (i > j && i < f) L.m[j * f + i] = __ddiv_rn(s, l);
+            else if (i > j && i < f) L.m[j * f + i] = __ddiv_rn(L.m[j * f + i], l);
             __syncthreads();
+            if (i > j && i < f) {
+                const double lij = L.m[j * f + i];
+                for (int k = j + 1; k <= i; ++k) L.m[k * f + i] -= lij * L.m[j * f + k];
+            }
         }
+        __syncthreads();
         // forward substitution: lane i subtracts l(i, k) b_k for k = 0, 1, ... as b_k is final
         {
             double acc = lane < f ? L.rhs[lane] : 0.0;

@@ -496,4 +496,84 @@ __device__ __forceinline__ RayState<double> inverse_step(V3<double> q, V3<double
     return out;
 }
 
+// One reverse step of a regular (hard-mode) ray for the fused kernel: reconstructs the state on
+// surface s from the state (q, d1) on surface j = s + 1 and propagates the adjoint (seed: OPL)
+// through the surface-j step, sharing the quantities both need. With N the unit normal at q:
+// d1 . N = sqrt(k) of the forward refraction, and the incidence cosine d . N equals the
+// intersection's sqrt(D) (B - c s = sqrt(D) on the sphere), so the reverse step needs two square
+// roots instead of four. mu = n_s / n_j, imu = n_j / n_s. Formulas: inverse_step above and
+// refracting_step_backward (adjoint_backward.hpp:119-241) specialised to the regular branch.
+__device__ __forceinline__ RayState<double> fused_step_backward(
+    V3<double> q, V3<double> d1, double c, double cs, double t, double n_s, double mu, double imu,
+    RayAdj<double>& adj, double& tb, double& cb_out) {
+    // reconstruction
+    const V3<double> N{-c * q.x, -c * q.y, 1.0 - c * q.z};
+    const double root_k = dot(d1, N);
+    const double kp = 1.0 - imu * imu * (1.0 - root_k * root_k);
+    const double cos_i = rev_sqrt(kp); // = sqrt(D) of the forward intersection
+    const V3<double> d = imu * d1 + (cos_i - imu * root_k) * N;
+    const double rdz = rev_rcp(d.z);
+    const double s = q.z * rdz;
+    const double x = q.x - s * d.x;
+    const double y = q.y - s * d.y;
+    const double s0p = (q.z + t) * rdz;
+    const double xp = q.x - s0p * d.x;
+    const double yp = q.y - s0p * d.y;
+    const double cm = -cs;
+    const double Hp = cm * (xp * xp + yp * yp);
+    const double Bp = d.z + cm * (xp * d.x + yp * d.y);
+    const double Dp = Bp * Bp - cm * Hp;
+    const double sp = rev_div(Hp, Bp + rev_sqrt(Dp));
+    RayState<double> prev;
+    prev.p = V3<double>{xp - sp * d.x, yp - sp * d.y, -(sp * d.z)};
+    prev.d = d;
+    const double s0 = t * rdz + sp;
+    // adjoint of the step from s to j
+    const double rr = x * x + y * y;
+    const double xd = x * d.x + y * d.y;
+    const V3<double> qb_out = adj.p;
+    const V3<double> d1b = adj.d;
+    RayAdj<double> in{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    const double a = root_k - mu * cos_i;
+    in.d += mu * d1b;
+    V3<double> nb = a * d1b;
+    const double ab = dot(d1b, N);
+    const double kb = rev_div(ab, 2.0 * root_k);
+    const double cosb = -ab * mu + kb * 2.0 * mu * mu * cos_i;
+    in.d += cosb * N;
+    nb += cosb * d;
+    const V3<double> qb = qb_out + (-c) * nb;
+    double cb = -dot(q, nb);
+    double xb = qb.x;
+    double yb = qb.y;
+    const double sb = dot(qb, d) + n_s;
+    in.d += s * qb;
+    double s0b = n_s;
+    const double inv = rev_rcp(2.0 * cos_i);
+    const double hb = sb * inv;
+    const double bb = -sb * s * 2.0 * inv;
+    cb += sb * s * s * inv;
+    cb += hb * rr;
+    xb += hb * 2.0 * c * x;
+    yb += hb * 2.0 * c * y;
+    in.d.z += bb;
+    cb += -bb * xd;
+    xb += -bb * c * d.x;
+    yb += -bb * c * d.y;
+    in.d.x += -bb * c * x;
+    in.d.y += -bb * c * y;
+    in.p.x += xb;
+    in.p.y += yb;
+    s0b += xb * d.x + yb * d.y;
+    in.d.x += xb * s0;
+    in.d.y += yb * s0;
+    const double qq = s0b * rdz;
+    tb += qq;
+    in.p.z += -qq;
+    in.d.z += -qq * s0;
+    cb_out += cb;
+    adj = in;
+    return prev;
+}
+
 } // namespace mkod
