@@ -828,6 +828,21 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
     for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
     double sw = 0.0, swo = 0.0;
     LastState last;
+    // stored states of both halves, loaded up front so that their latency hides behind the
+    // surface staging and the first half's sweep
+    double pre[kStates ? kHalves : 1][7];
+    if constexpr (kStates) {
+#pragma unroll
+        for (int h = 0; h < kHalves; ++h) {
+            const int k = tid + h * kBlock;
+            pre[h][6] = __builtin_nan("");
+            if (k < t.n) {
+                const int64_t row = t.offset + k;
+#pragma unroll
+                for (int c = 0; c < 7; ++c) pre[h][c] = states.data[state_index(states, t.v, c, row)];
+            }
+        }
+    }
     for (int h = 0; h < kHalves; ++h) {
         const int k = tid + h * kBlock;
         for (int p = 0; p < Pn; ++p) G[p * kGStride + tid] = 0.0f;
@@ -845,18 +860,11 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
             last.count = 0;
             last.image_branch = 0;
             if constexpr (kStates) {
-                const int64_t row = t.offset + k;
-                r.opl = states.data[state_index(states, t.v, 6, row)];
+                r.opl = pre[h][6];
                 ok = !__builtin_isnan(r.opl);
-                if (ok) {
-                    last.p = V3<double>{states.data[state_index(states, t.v, 0, row)],
-                                        states.data[state_index(states, t.v, 1, row)],
-                                        states.data[state_index(states, t.v, 2, row)]};
-                    last.d = V3<double>{states.data[state_index(states, t.v, 3, row)],
-                                        states.data[state_index(states, t.v, 4, row)],
-                                        states.data[state_index(states, t.v, 5, row)]};
-                    last.count = P.S;
-                }
+                last.p = V3<double>{pre[h][0], pre[h][1], pre[h][2]};
+                last.d = V3<double>{pre[h][3], pre[h][4], pre[h][5]};
+                last.count = P.S;
             } else {
                 Soft soft{0.0, MKO_STATUS_OK};
                 ok = trace_ray<false, true>(P, L, r, M, kappa, soft, last) == MKO_STATUS_OK;

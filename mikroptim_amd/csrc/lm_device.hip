@@ -43,8 +43,10 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
+// Packed lower triangle of an f x f matrix, column-major: element (row i, column j), i >= j.
+__device__ __forceinline__ int tri(int j, int i, int f) { return j * f - (j * (j - 1)) / 2 + (i - j); }
+
 struct SolveLds {
-    double m[kMaxP * kMaxP]; // lower triangle of the free block, column-major m[j * f + i]
     double rhs[kMaxP];
     double delta[kMaxP];
     double step[kMaxP];
@@ -61,7 +63,7 @@ struct SolveLds {
 // the factorisation succeeds, like the host).
 // a, g: the run's normal equations (row-major a[q * P + p]).
 __device__ void solve_step(const State& S, int run, const double* a, const double* g, double mu,
-                           SolveLds& L) {
+                           SolveLds& L, double* m) {
     const int n = S.P;
     const int lane = threadIdx.x;
     const double* x = S.x + size_t(run) * n;
@@ -84,7 +86,7 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             if (i >= j && i < f) {
                 double v = a[size_t(L.idx[j]) * n + L.idx[i]];
                 if (i == j) v += mu * d[L.idx[j]];
-                L.m[j * f + i] = v;
+                m[tri(j, i, f)] = v;
             }
         }
         if (lane < f) L.rhs[lane] = -g[L.idx[lane]];
@@ -95,7 +97,7 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         // factor is bitwise the host's; the updates of a lane are independent of each other.
         for (int j = 0; j < f; ++j) {
             const int i = lane;
-            if (i == j) L.diag = L.m[j * f + j];
+            if (i == j) L.diag = m[tri(j, j, f)];
             __syncthreads();
             const double diag = L.diag;
             if (!(diag > 0.0) || !isfinite(diag)) {
@@ -103,12 +105,12 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 return;
             }
             const double l = __dsqrt_rn(diag);
-            if (i == j) L.m[j * f + j] = l;
-            else if (i > j && i < f) L.m[j * f + i] = __ddiv_rn(L.m[j * f + i], l);
+            if (i == j) m[tri(j, j, f)] = l;
+            else if (i > j && i < f) m[tri(j, i, f)] = __ddiv_rn(m[tri(j, i, f)], l);
             __syncthreads();
             if (i > j && i < f) {
-                const double lij = L.m[j * f + i];
-                for (int k = j + 1; k <= i; ++k) L.m[k * f + i] -= lij * L.m[j * f + k];
+                const double lij = m[tri(j, i, f)];
+                for (int k = j + 1; k <= i; ++k) m[tri(k, i, f)] -= lij * m[tri(j, k, f)];
             }
         }
         __syncthreads();
@@ -116,9 +118,9 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         {
             double acc = lane < f ? L.rhs[lane] : 0.0;
             for (int k = 0; k < f; ++k) {
-                if (lane == k) L.rhs[k] = __ddiv_rn(acc, L.m[k * f + k]);
+                if (lane == k) L.rhs[k] = __ddiv_rn(acc, m[tri(k, k, f)]);
                 __syncthreads();
-                if (lane > k && lane < f) acc -= L.m[k * f + lane] * L.rhs[k];
+                if (lane > k && lane < f) acc -= m[tri(k, lane, f)] * L.rhs[k];
             }
         }
         // back substitution: lane i subtracts l(k, i) x_k for k = f-1, f-2, ... as x_k is final
@@ -127,9 +129,9 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             __syncthreads();
             double acc = lane < f ? L.rhs[lane] : 0.0;
             for (int k = f - 1; k >= 0; --k) {
-                if (lane == k) L.rhs[k] = __ddiv_rn(acc, L.m[k * f + k]);
+                if (lane == k) L.rhs[k] = __ddiv_rn(acc, m[tri(k, k, f)]);
                 __syncthreads();
-                if (lane < k) acc -= L.m[lane * f + k] * L.rhs[k];
+                if (lane < k) acc -= m[tri(lane, k, f)] * L.rhs[k];
             }
         }
         __syncthreads();
@@ -319,6 +321,7 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
                                                          const double* g_c) {
     __shared__ SolveLds L;
     __shared__ int finite_s;
+    extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles
     const int j = blockIdx.x;
     if (j >= count) return;
     const int run = list[j];
@@ -348,7 +351,7 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
     const double* g = g_c + size_t(j) * n;
     prepare_step(S, o, run, a, g);
     __syncthreads();
-    solve_step(S, run, a, g, kModelDamping, L);
+    solve_step(S, run, a, g, kModelDamping, L, m_tri);
     if (lane == 0) {
         const bool solved = S.solved[run] != 0;
         const double merit = S.merit[run];
@@ -364,10 +367,12 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
 __global__ __launch_bounds__(kLanes) void step_kernel(State S, Options o, const int32_t* list,
                                                       int count) {
     __shared__ SolveLds L;
+    extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles
     const int j = blockIdx.x;
     if (j >= count) return;
     const int run = list[j];
-    solve_step(S, run, S.a + size_t(run) * S.P * S.P, S.g + size_t(run) * S.P, S.mu[run], L);
+    solve_step(S, run, S.a + size_t(run) * S.P * S.P, S.g + size_t(run) * S.P, S.mu[run], L,
+               m_tri);
     if (threadIdx.x == 0) {
         const bool solved = S.solved[run] != 0;
         S.evaluate[run] = 0;
@@ -470,6 +475,7 @@ __global__ __launch_bounds__(64) void fd_pattern_kernel(int count, int P, const 
 }
 
 inline hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
+inline size_t tri_bytes(int P) { return size_t(P) * size_t(P + 1) / 2 * sizeof(double); }
 inline unsigned blocks_of(int n, int per) { return unsigned((n + per - 1) / per); }
 
 } // namespace
@@ -502,14 +508,14 @@ void launch_gather(const State& s, const int32_t* list, int count, bool trial,
 void launch_prepare(const State& s, const Options& o, const int32_t* list, int count,
                     const double* a_c, const double* g_c, void* stream) {
     if (count > 0)
-        hipLaunchKernelGGL(prepare_kernel, dim3(unsigned(count)), dim3(kLanes), 0, st(stream), s, o,
-                           list, count, a_c, g_c);
+        hipLaunchKernelGGL(prepare_kernel, dim3(unsigned(count)), dim3(kLanes), tri_bytes(s.P),
+                           st(stream), s, o, list, count, a_c, g_c);
 }
 
 void launch_step(const State& s, const Options& o, const int32_t* list, int count, void* stream) {
     if (count > 0)
-        hipLaunchKernelGGL(step_kernel, dim3(unsigned(count)), dim3(kLanes), 0, st(stream), s, o,
-                           list, count);
+        hipLaunchKernelGGL(step_kernel, dim3(unsigned(count)), dim3(kLanes), tri_bytes(s.P),
+                           st(stream), s, o, list, count);
 }
 
 void launch_update(const State& s, const Options& o, const int32_t* list, int count,
