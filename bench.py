@@ -145,25 +145,33 @@ def main():
                                        "fp64", args.warmup, fd_steps, args.threads)
     it_fd = n_fd / t_fd
 
-    # roofline of the dominant kernel of the measured path (the adjoint kernel): algorithmic
-    # FLOP per launch = rays * (FLOP_fwd + FLOP_adj), FLOP_fwd = 59 (S-1) + 60,
-    # FLOP_adj = FLOP_fwd + 210 (S-1) + 6 P (counting div/sqrt as 1, FMA as 2)
+    # roofline of the dominant kernel of the measured path, the fused adjoint + normal equations.
+    # Algorithmic FLOP (div/sqrt counted as 1, FMA as 2): FLOP_fwd = 59 (S-1) + 60 per ray for a
+    # forward trace, FLOP_adj = FLOP_fwd + 210 (S-1) + 6 P per ray for an adjoint Jacobian (its
+    # recording forward pass plus the reverse sweep). In the LM the fused kernel starts from the
+    # ray states that the MERIT trace of the same point stored, so its work is the reverse part,
+    # 210 (S-1) + 6 P per ray, plus the MFMA products (3 x 2 P^2 per ray, bf16). The MERIT trace
+    # (FLOP_fwd per ray) is reported beside it.
     S, P = batch.surfaces, variables.size
     rays = 17920
     flop_fwd = 59 * (S - 1) + 60
-    flop_adj = flop_fwd + 210 * (S - 1) + 6 * P
+    flop_rev = 210 * (S - 1) + 6 * P
     adj_ms, adj_n = kt_adj["adjoint"]
     achieved = None
     runs_per_launch = jac_adj / adj_n if adj_n else 0.0
     if adj_n:
-        achieved = runs_per_launch * rays * (flop_fwd + flop_adj) / (adj_ms / adj_n * 1e-3) / 1e12
-    roof = {"bound": "valu", "kernel": "adjoint_normal_kernel (fused adjoint + normal equations)",
+        achieved = runs_per_launch * rays * flop_rev / (adj_ms / adj_n * 1e-3) / 1e12
+    # compute-bound (FP64 vector ALU): MI355X's FP64 vector and FP64 matrix peaks are the same
+    # 78.6 TFLOP/s, so the compute roof is that figure
+    roof = {"bound": "mfma", "roof": "FP64 compute (vector ALU), 78.6 TFLOP/s",
+            "kernel": "adjoint_normal_kernel (fused adjoint + normal equations)",
             "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
             "frac": (achieved / 78.6) if achieved else None,
             "traffic": profiled_traffic(runs_per_launch),
             "avg_launch_ms": (adj_ms / adj_n) if adj_n else None,
             "runs_per_launch": runs_per_launch,
-            "flop_per_ray": flop_fwd + flop_adj}
+            "flop_per_ray": flop_rev,
+            "mfma_bf16_flop_per_ray": 6 * P * P}
 
     # forward trace throughput of the two number types (MERIT of all variants at the starts):
     # FP64 is the trace of every mode; df64 is the reference's second number type
@@ -177,6 +185,12 @@ def main():
         eng.merit(batch, st)
         ms, n = eng.kernel_timing(False)["trace"]
         trace_rate[name] = batch.variants * rays / (ms * 1e-3) if ms > 0 else None
+    merit_roof = None
+    if trace_rate.get("fp64"):
+        ach = trace_rate["fp64"] * flop_fwd / 1e12
+        merit_roof = {"kernel": "trace_kernel (MERIT, FP64, all 4096 start points)",
+                      "achieved": ach, "peak": 78.6, "unit": "TFLOP/s", "frac": ach / 78.6,
+                      "flop_per_ray": flop_fwd}
 
     # per-result error of the mixed-precision normal equations against the FP64 stored-J path
     # (itself within 1e-10 of the FP64 oracle) at 8 final points
@@ -212,6 +226,7 @@ def main():
         "final_mean_merit": {"adjoint_mixed": float(np.mean(res_adj.merit)),
                              "fd_fp64": float(np.mean(res_fd.merit))},
         "roofline": roof,
+        "merit_trace": merit_roof,
     }
     if not args.no_cpu_baseline:
         base_cpu, err, same_status = cpu_baseline(batch, variables, settings, res_adj.x, eng)

@@ -860,10 +860,20 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
             last.count = 0;
             last.image_branch = 0;
             if constexpr (kStates) {
-                r.opl = pre[h][6];
+                // select this half's states with constant indices (a runtime index into pre
+                // would place the array in private memory)
+                double st[7];
+#pragma unroll
+                for (int c = 0; c < 7; ++c) st[c] = pre[0][c];
+#pragma unroll
+                for (int hh = 1; hh < kHalves; ++hh)
+                    if (hh == h)
+#pragma unroll
+                        for (int c = 0; c < 7; ++c) st[c] = pre[hh][c];
+                r.opl = st[6];
                 ok = !__builtin_isnan(r.opl);
-                last.p = V3<double>{pre[h][0], pre[h][1], pre[h][2]};
-                last.d = V3<double>{pre[h][3], pre[h][4], pre[h][5]};
+                last.p = V3<double>{st[0], st[1], st[2]};
+                last.d = V3<double>{st[3], st[4], st[5]};
                 last.count = P.S;
             } else {
                 Soft soft{0.0, MKO_STATUS_OK};

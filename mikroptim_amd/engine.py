@@ -263,6 +263,13 @@ class TraceEngine:
                                          abi.ptr(x), res, C.byref(cyc)))
         return LmResult(x, list(res), cyc.value)
 
+    def optimize_groups(self, groups, options=None, precision="fp64"):
+        """Uneven batches: runs of different topologies (surface, field, wavelength or variable
+        counts), one batched LM per topology group; groups = [(base, settings, variables, x0)].
+        Returns the LmResult of every group, in order. (The reference's batches hold a single
+        topology, include/mikroptim/batch/system_batch.hpp:26.)"""
+        return [self.optimize(b, s, v, x0, options, precision) for b, s, v, x0 in groups]
+
     def lm(self, base, settings, variables, x0, options=None, precision="fp64"):
         return LmSession(self, base, settings, variables, x0, options or abi.lm_options(),
                          precision)

@@ -240,3 +240,96 @@ def objective_variables(batch):
 
 def objective_settings(failure=abi.FAILURE_HARD):
     return ResidualSettings(rings=16, spokes=32, failure=failure)
+
+
+# ---- uneven batches: random topologies (one LM per topology group) ----------------------------
+
+UNEVEN_GLASSES = ((1.5168, 64.2), (1.6204, 60.3), (1.5750, 41.5), (1.7200, 50.3),
+                  (1.6477, 33.8), (1.4875, 70.4), (1.8052, 25.4), (1.5891, 61.3))
+
+
+@functools.lru_cache(maxsize=None)
+def _uneven_glass(i):
+    return optics.synthetic_glass(*UNEVEN_GLASSES[i])
+
+
+def uneven_topology(index, seed=5, variants=4, jitter=0.01):
+    """One random topology of the uneven workload (seeded by (seed, index)): 8 ... 32 refracting
+    spherical surfaces behind the stop forming weak positive elements (a cemented first element
+    when the count is odd), 3 ... 5 fields up to 1 deg, 3 ... 7 wavelengths 450 ... 650 nm,
+    10 ... 96 variables (curvatures, then inner thicknesses, then the working distance; at most
+    2 (S - 1)); image plane at the paraxial focus, pupil radius 0.08 f. `variants` copies with
+    the variables jittered by N(0, jitter) (Irwin-Hall), the first unperturbed. Returns
+    (batch, variables)."""
+    g = np.random.default_rng([seed, index])
+    n_refr = int(g.integers(8, 33))
+    F = int(g.integers(3, 6))
+    L = int(g.integers(3, 8))
+    P_draw = int(g.integers(10, 97))
+    # media after each refracting surface: alternate glass / air, the first element cemented
+    # (two glasses) when n_refr is odd, so that the last medium is air
+    media = []
+    k = 1
+    if n_refr % 2 == 1:
+        media += [_uneven_glass(int(g.integers(len(UNEVEN_GLASSES)))),
+                  _uneven_glass(int(g.integers(len(UNEVEN_GLASSES)))), None]
+        k = 4
+    while len(media) < n_refr:
+        media += [_uneven_glass(int(g.integers(len(UNEVEN_GLASSES)))), None]
+    c = [0.0]
+    t = [float(g.uniform(1.0, 3.0))]
+    glasses = [None] + media
+    for s in range(1, n_refr + 1):
+        glass_after = glasses[s] is not None
+        glass_before = glasses[s - 1] is not None
+        if glass_after and not glass_before:
+            c.append(float(g.uniform(0.004, 0.012)))      # front of a positive element
+        elif glass_before and not glass_after:
+            c.append(-float(g.uniform(0.002, 0.010)))     # back
+        else:
+            c.append(float(g.uniform(-0.01, 0.01)))       # cemented interface
+        t.append(float(g.uniform(2.0, 4.0)) if glass_after else float(g.uniform(0.5, 2.0)))
+    scale = 1.0
+    for _ in range(20):
+        cs = [ci * scale for ci in c]
+        f, bfd = _paraxial(cs, t, glasses)
+        if f > 0 and bfd > 2.0:
+            break
+        scale *= 0.7
+    c = cs
+    t[-1] = bfd
+    wl = np.linspace(450.0, 650.0, L)
+    fields_deg = np.linspace(0.0, 1.0, F)
+    types = [abi.SURFACE_STOP] + [abi.SURFACE_SPHERE] * n_refr
+    b = SystemBatch(types, wl, np.ones(L), np.ones(F),
+                    [abi.PUPIL_AXIS if a == 0.0 else abi.PUPIL_HALF for a in fields_deg],
+                    variants, LAMBDA_REF)
+    b.curvature[:] = c
+    b.thickness[:] = t
+    b.index[:] = _index_table(glasses, wl)
+    b.pupil_radius[:] = 0.08 * f
+    theta = np.radians(fields_deg)
+    b.field_angle[:] = theta
+    b.field_target[:] = f * np.tan(theta)
+    kinds = ([(abi.VAR_CURVATURE, s) for s in range(1, n_refr + 1)] +
+             [(abi.VAR_THICKNESS, s) for s in range(1, n_refr)] +
+             [(abi.VAR_WORKING_DISTANCE, n_refr)])
+    kinds = kinds[:min(P_draw, len(kinds))]
+    inf = math.inf
+    variables = VariableSet(
+        type=np.array([k for k, _ in kinds], dtype=np.int32),
+        batch_surface=np.array([s for _, s in kinds], dtype=np.int32),
+        lower=np.array([-0.2 if k == abi.VAR_CURVATURE else 0.05 for k, _ in kinds]),
+        upper=np.array([0.2 if k == abi.VAR_CURVATURE else 1000.0 for k, _ in kinds]),
+        typical=np.array([0.01 if k == abi.VAR_CURVATURE else 1.0 for k, _ in kinds]),
+        names=[("c[%d]" if k == abi.VAR_CURVATURE else "t[%d]") % s for k, s in kinds])
+    x0 = variables.extract(b, 0)
+    xs = rng.perturbed_vectors(variables, x0, variants, seed=seed * 1000003 + index,
+                               relative=(jitter, jitter, jitter), distribution="irwin_hall",
+                               first_is_base=True)
+    variables.apply(b, xs)
+    return b, variables
+
+
+def uneven_settings(failure=abi.FAILURE_HARD):
+    return ResidualSettings(rings=6, spokes=8, failure=failure)

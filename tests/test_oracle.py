@@ -494,3 +494,21 @@ def test_df64_oracle_matches_reference_bitwise(op):
         b[:, 1] = 0.0
     assert np.array_equal(df64_ops(oracle(), op, a, b).view(np.uint32),
                           df64_ops(reference(), op, a, b).view(np.uint32))
+
+
+def test_uneven_topologies_cover_the_ranges_and_trace_cleanly():
+    """The uneven workload's random topologies: 8 ... 32 refracting surfaces, 3 ... 5 fields,
+    3 ... 7 wavelengths, 10 ... 2 (S - 1) variables; at the seeded start points no ray fails
+    (oracle, hard mode) and the merit is finite."""
+    seen = set()
+    for i in range(12):
+        b, v = systems.uneven_topology(i, variants=2)
+        assert 9 <= b.surfaces <= 33 and 3 <= b.fields <= 5 and 3 <= b.wavelengths <= 7
+        assert 10 <= v.size <= min(96, 2 * (b.surfaces - 1))
+        s = systems.uneven_settings()
+        _, _, st = oracle().residuals(b, s, threads=8)
+        assert (st == 0).all()
+        m, _, _ = oracle().merit(b, s, threads=8)
+        assert np.all(np.isfinite(m)) and np.all(m > 0)
+        seen.add((b.surfaces, b.fields, b.wavelengths, v.size))
+    assert len(seen) == 12  # distinct topologies
