@@ -86,7 +86,63 @@ typedef struct mko_batch {
     const double* field_angle;         /* [V][F]  rad */
     const double* field_target;        /* [V][F]  ideal image height h_f */
     const double* reference_curvature; /* [V][F]  kappa (image references only) or NULL */
+    /* [V][S] index of the medium after surface s at lambda_ref (SystemBatchView::paraxial_index),
+     * or NULL; needed only by the paraxial constraint rows (focal length, paraxial focus,
+     * Petzval). */
+    const double* paraxial_index;
 } mko_batch;
+
+/* Constraint rows appended to every variant's residual vector (ExtraResiduals /
+ * TraceExtraResiduals, include/mikroptim/optim/extra_residuals.hpp:23,
+ * trace_extra_residuals.hpp:38). Types and modes keep the numbering of ConstraintType /
+ * ConstraintMode / EdgeHeight (include/mikroptim/model/constraints.hpp:28,76,69); supported
+ * are the paraxial types (paraxial_residuals.hpp:67) and the mechanical types
+ * (mechanical_constraints.hpp:38). */
+enum {
+    MKO_CON_FOCAL_LENGTH = 0,
+    MKO_CON_PARFOCAL_LENGTH = 1,
+    MKO_CON_WORKING_DISTANCE = 2,
+    MKO_CON_PARAXIAL_FOCUS = 3,
+    MKO_CON_PETZVAL = 4,
+    MKO_CON_MAX_SEMI_DIAMETER = 7,
+    MKO_CON_MIN_CENTER_THICKNESS = 8,
+    MKO_CON_MAX_CENTER_THICKNESS = 9,
+    MKO_CON_MIN_EDGE_THICKNESS = 10,
+    MKO_CON_MAX_INCIDENCE_ANGLE = 11
+};
+enum { MKO_MODE_EQUAL = 0, MKO_MODE_MIN = 1, MKO_MODE_MAX = 2 };
+enum { MKO_EDGE_SEMI_DIAMETER = 0, MKO_EDGE_RAYS = 1 };
+
+/* One resolved constraint (ParaxialConstraint, paraxial_residuals.hpp:67; MechanicalConstraint,
+ * mechanical_constraints.hpp:38). Surfaces are table (batch) surfaces.
+ *   paraxial:   r = sqrt(w) * mode(q) with q = f_s (1 - f_s phi) | offset + sum_{j>=surface}
+ *               t_j - L_s | t_{S-1} - w_s | a (A + t_{S-1} C / n'_L) | |kappa_P| - kappa_max
+ *               (target = f_s, L_s, w_s, 0, kappa_max; mode Min: min(q - shift, 0), Max:
+ *               max(q + shift, 0))
+ *   mechanical: r = sqrt(w) * max(0, g + shift), g = value - limit (max semi-diameter, max
+ *               center thickness) or limit - value (others); target = limit (mm, or
+ *               cos(theta_max) for the incidence angle), offset = clearance. */
+typedef struct mko_constraint {
+    int32_t type;         /* MKO_CON_* */
+    int32_t mode;         /* MKO_MODE_* (paraxial types) */
+    double weight;        /* w >= 0 */
+    double target;        /* target value or limit */
+    int32_t surface;      /* parfocal reference surface / mechanical surface m(i) */
+    int32_t next_surface; /* thickness types: surface m(i + 1) of the gap's far side */
+    double offset;        /* parfocal offset / mechanical clearance [mm] */
+    double semi_diameter, next_semi_diameter; /* edge thickness at fixed heights [mm] */
+    int32_t edge_height;  /* MKO_EDGE_* (min edge thickness) */
+    double shift;         /* augmented-Lagrangian shift theta >= 0 (0: plain penalty) */
+} mko_constraint;
+
+/* The rows of a residual vector after the ray rows: first the items that need no trace
+ * extrema (E rows, in item order), then the items that do (E_t rows: max semi-diameter, max
+ * incidence angle, min edge thickness at ray heights; in item order). R = trace rows + E + E_t.
+ * (host memory) */
+typedef struct mko_extra_rows {
+    int32_t count;
+    const mko_constraint* items; /* [count] */
+} mko_extra_rows;
 
 typedef struct mko_residual_settings {
     int32_t rings;  /* N, 1 ... 16 */
@@ -96,6 +152,8 @@ typedef struct mko_residual_settings {
     /* number type of the RESIDUALS / MERIT traces (TraceSettings::precision): MKO_PRECISION_FP64
      * or MKO_PRECISION_DF64. The adjoint entry points take their precision as an argument. */
     int32_t precision;
+    /* constraint rows after the ray rows (FdJacobianOptions::extra / trace_extra), or NULL */
+    const mko_extra_rows* extra;
 } mko_residual_settings;
 
 /* P variables. Curvature k -> curvature[v][batch_surface[k]]; thickness and working distance

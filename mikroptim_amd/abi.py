@@ -19,6 +19,13 @@ JACOBIAN_FD, JACOBIAN_ADJOINT = 0, 1
 (LM_RUNNING, LM_CONVERGED_MERIT, LM_CONVERGED_STEP, LM_CONVERGED_GRADIENT, LM_CONVERGED_STALL,
  LM_MAX_ITERATIONS, LM_DAMPING_LIMIT, LM_NUMERICAL_ERROR) = range(8)
 PRECISION_FP64, PRECISION_MIXED, PRECISION_DF64 = 0, 1, 2
+# ConstraintType / ConstraintMode / EdgeHeight (include/mikroptim/model/constraints.hpp:28,76,69)
+(CON_FOCAL_LENGTH, CON_PARFOCAL_LENGTH, CON_WORKING_DISTANCE, CON_PARAXIAL_FOCUS,
+ CON_PETZVAL) = range(5)
+(CON_MAX_SEMI_DIAMETER, CON_MIN_CENTER_THICKNESS, CON_MAX_CENTER_THICKNESS,
+ CON_MIN_EDGE_THICKNESS, CON_MAX_INCIDENCE_ANGLE) = range(7, 12)
+MODE_EQUAL, MODE_MIN, MODE_MAX = 0, 1, 2
+EDGE_SEMI_DIAMETER, EDGE_RAYS = 0, 1
 
 LM_STATUS_NAMES = ["running", "converged_merit", "converged_step", "converged_gradient",
                    "converged_stall", "max_iterations", "damping_limit", "numerical_error"]
@@ -37,12 +44,27 @@ class Batch(C.Structure):
         ("type", _u8p), ("pupil_mode", _u8p), ("wavelength_weight", _dp), ("field_weight", _dp),
         ("curvature", _dp), ("thickness", _dp), ("index", _dp), ("pupil_radius", _dp),
         ("field_angle", _dp), ("field_target", _dp), ("reference_curvature", _dp),
+        ("paraxial_index", _dp),
     ]
+
+
+class Constraint(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("mode", C.c_int32), ("weight", C.c_double), ("target", C.c_double),
+        ("surface", C.c_int32), ("next_surface", C.c_int32), ("offset", C.c_double),
+        ("semi_diameter", C.c_double), ("next_semi_diameter", C.c_double),
+        ("edge_height", C.c_int32), ("shift", C.c_double),
+    ]
+
+
+class ExtraRows(C.Structure):
+    _fields_ = [("count", C.c_int32), ("items", C.POINTER(Constraint))]
 
 
 class ResidualSettings(C.Structure):
     _fields_ = [("rings", C.c_int32), ("spokes", C.c_int32), ("failure", C.c_int32),
-                ("fail_weight", C.c_double), ("precision", C.c_int32)]
+                ("fail_weight", C.c_double), ("precision", C.c_int32),
+                ("extra", C.POINTER(ExtraRows))]
 
 
 class Variables(C.Structure):

@@ -44,6 +44,8 @@ class SystemBatch:
         self.field_target = np.zeros((V, F))
         self.reference_curvature = (np.zeros((V, F)) if reference_sphere != abi.REF_LAST_SURFACE
                                     else None)
+        # [V][S] medium index after each surface at lambda_ref (paraxial constraint rows)
+        self.paraxial_index = None
 
     # ---- shape --------------------------------------------------------------------------
     @property
@@ -63,7 +65,7 @@ class SystemBatch:
         return len(self.field_weight)
 
     PER_VARIANT = ("curvature", "thickness", "index", "pupil_radius", "field_angle",
-                   "field_target", "reference_curvature")
+                   "field_target", "reference_curvature", "paraxial_index")
 
     def take(self, rows):
         """New batch with variants `rows` of this one (shared fields copied)."""
@@ -113,17 +115,114 @@ class SystemBatch:
 
 
 @dataclass
+class Constraint:
+    """One constraint row (ParaxialConstraint, include/mikroptim/optim/paraxial_residuals.hpp:67;
+    MechanicalConstraint, include/mikroptim/optim/mechanical_constraints.hpp:38); surfaces are
+    batch (table) surfaces, `target` the target value or limit, `offset` the parfocal offset or
+    the mechanical clearance."""
+    type: int
+    weight: float = 1.0
+    target: float = 0.0
+    mode: int = abi.MODE_EQUAL
+    surface: int = 0
+    next_surface: int = 0
+    offset: float = 0.0
+    semi_diameter: float = 0.0
+    next_semi_diameter: float = 0.0
+    edge_height: int = abi.EDGE_SEMI_DIAMETER
+    shift: float = 0.0
+
+    @property
+    def needs_extrema(self):
+        return self.type in (abi.CON_MAX_SEMI_DIAMETER, abi.CON_MAX_INCIDENCE_ANGLE) or (
+            self.type == abi.CON_MIN_EDGE_THICKNESS and self.edge_height == abi.EDGE_RAYS)
+
+    def c_struct(self):
+        return abi.Constraint(self.type, self.mode, self.weight, self.target, self.surface,
+                              self.next_surface, self.offset, self.semi_diameter,
+                              self.next_semi_diameter, self.edge_height, self.shift)
+
+
+def focal_length(target, weight=1.0):
+    return Constraint(abi.CON_FOCAL_LENGTH, weight, target)
+
+
+def working_distance(target, weight=1.0, mode=abi.MODE_EQUAL, shift=0.0):
+    return Constraint(abi.CON_WORKING_DISTANCE, weight, target, mode, shift=shift)
+
+
+def parfocal_length(target, surface=0, offset=0.0, weight=1.0, mode=abi.MODE_EQUAL):
+    return Constraint(abi.CON_PARFOCAL_LENGTH, weight, target, mode, surface=surface,
+                      offset=offset)
+
+
+def paraxial_focus(weight=1.0):
+    return Constraint(abi.CON_PARAXIAL_FOCUS, weight, 0.0)
+
+
+def petzval(kappa_max, weight=1.0, shift=0.0):
+    return Constraint(abi.CON_PETZVAL, weight, kappa_max, abi.MODE_MAX, shift=shift)
+
+
+def max_semi_diameter(surface, limit, weight=1.0, clearance=0.0, shift=0.0):
+    return Constraint(abi.CON_MAX_SEMI_DIAMETER, weight, limit, surface=surface,
+                      next_surface=surface, offset=clearance, shift=shift)
+
+
+def center_thickness(surface, next_surface, limit, weight=1.0, upper=False, shift=0.0):
+    return Constraint(abi.CON_MAX_CENTER_THICKNESS if upper else abi.CON_MIN_CENTER_THICKNESS,
+                      weight, limit, surface=surface, next_surface=next_surface, shift=shift)
+
+
+def min_edge_thickness(surface, next_surface, limit, weight=1.0, clearance=0.0,
+                       semi_diameters=None, shift=0.0):
+    """Edge thickness at the larger of two fixed semi-diameters, or (None) at the larger of the
+    two surfaces' largest ray heights."""
+    c = Constraint(abi.CON_MIN_EDGE_THICKNESS, weight, limit, surface=surface,
+                   next_surface=next_surface, offset=clearance, shift=shift,
+                   edge_height=abi.EDGE_RAYS if semi_diameters is None else abi.EDGE_SEMI_DIAMETER)
+    if semi_diameters is not None:
+        c.semi_diameter, c.next_semi_diameter = map(float, semi_diameters)
+    return c
+
+
+def max_incidence_angle(surface, angle_deg, weight=1.0, shift=0.0):
+    """limit = cos(theta_max) (make_mechanical_constraints)."""
+    return Constraint(abi.CON_MAX_INCIDENCE_ANGLE, weight, float(np.cos(np.deg2rad(angle_deg))),
+                      surface=surface, next_surface=surface, shift=shift)
+
+
+@dataclass
 class ResidualSettings:
-    """ResidualSettings, include/mikroptim/batch/batch_residuals.hpp:24 (reference defaults)."""
+    """ResidualSettings, include/mikroptim/batch/batch_residuals.hpp:24 (reference defaults),
+    plus the constraint rows appended to every residual vector (FdJacobianOptions::extra /
+    trace_extra)."""
     rings: int = 10
     spokes: int = 8
     failure: int = abi.FAILURE_SOFT
     fail_weight: float = 1e6
     precision: int = abi.PRECISION_FP64  # number type of the RESIDUALS / MERIT traces
+    constraints: list = field(default_factory=list)
+
+    def row_constraints(self):
+        """Constraints in row order: those without extrema, then those with."""
+        return ([c for c in self.constraints if not c.needs_extrema]
+                + [c for c in self.constraints if c.needs_extrema])
+
+    @property
+    def extra_rows(self):
+        return len(self.constraints)
 
     def c_struct(self):
-        return abi.ResidualSettings(self.rings, self.spokes, self.failure, self.fail_weight,
-                                    self.precision)
+        s = abi.ResidualSettings(self.rings, self.spokes, self.failure, self.fail_weight,
+                                 self.precision, None)
+        if self.constraints:
+            items = (abi.Constraint * len(self.constraints))(
+                *[c.c_struct() for c in self.constraints])
+            rows = abi.ExtraRows(len(self.constraints), items)
+            s.extra = C.pointer(rows)
+            s._keep = (items, rows)
+        return s
 
 
 def layout(batch, settings):
@@ -146,6 +245,11 @@ def rays_per_variant(batch, settings):
 def trace_rows(batch, settings):
     rays = rays_per_variant(batch, settings)
     return rays * (2 if settings.failure == abi.FAILURE_SOFT else 1)
+
+
+def residual_rows(batch, settings):
+    """Rows of a variant's residual vector: trace rows + E + E_t."""
+    return trace_rows(batch, settings) + settings.extra_rows
 
 
 @dataclass

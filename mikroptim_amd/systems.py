@@ -25,6 +25,7 @@ import math
 import numpy as np
 
 from . import abi, optics, rng
+from . import batch
 from .batch import ResidualSettings, SystemBatch, VariableSet
 
 LAMBDA_REF = 587.56
@@ -64,6 +65,7 @@ def singlet(variants=1024, seed=1, perturb=0.05):
     b.curvature[:] = c
     b.thickness[:] = t
     b.index[:] = _index_table(glasses, F_C_D)
+    b.paraxial_index = np.tile(_index_table(glasses, [LAMBDA_REF])[:, 0], (variants, 1))
     b.pupil_radius[:] = 5.0
     theta = np.radians(fields_deg)
     b.field_angle[:] = theta
@@ -89,6 +91,21 @@ def singlet_variables():
 
 def singlet_settings(failure=abi.FAILURE_HARD):
     return ResidualSettings(rings=6, spokes=12, failure=failure)
+
+
+def singlet_constraints():
+    """Constraint rows of every kind on the singlet: focal length 100 mm, working distance >= 95,
+    parfocal length, paraxial focus, Petzval bound, semi-diameter of the first lens surface,
+    center and edge thickness of the lens (fixed heights and ray heights), incidence angle at
+    the second lens surface."""
+    return [batch.focal_length(100.0, 0.03), batch.working_distance(95.0, 0.1, abi.MODE_MIN),
+            batch.parfocal_length(150.0, 1, 2.0, 0.1), batch.paraxial_focus(0.5),
+            batch.petzval(0.001, 100.0),
+            batch.max_semi_diameter(1, 4.0, 2.0, clearance=0.5),
+            batch.center_thickness(1, 2, 6.0, 1.0),
+            batch.min_edge_thickness(1, 2, 2.0, 1.0, semi_diameters=(6.0, 6.0)),
+            batch.max_incidence_angle(2, 5.0, 3.0),
+            batch.min_edge_thickness(1, 2, 4.7, 1.0, clearance=0.2)]
 
 
 # ---- the 24-surface objective ------------------------------------------------------------------
@@ -198,6 +215,7 @@ def objective(variants=4096, seed=2, jitter=0.01, fields=5, wavelengths=7, max_f
     b.curvature[:] = c
     b.thickness[:] = t
     b.index[:] = _index_table(media, wl)
+    b.paraxial_index = np.tile(_index_table(media, [LAMBDA_REF])[:, 0], (variants, 1))
     b.pupil_radius[:] = a
     theta = np.radians(fields_deg)
     b.field_angle[:] = theta
@@ -307,6 +325,7 @@ def uneven_topology(index, seed=5, variants=4, jitter=0.01):
     b.curvature[:] = c
     b.thickness[:] = t
     b.index[:] = _index_table(glasses, wl)
+    b.paraxial_index = np.tile(_index_table(glasses, [LAMBDA_REF])[:, 0], (variants, 1))
     b.pupil_radius[:] = 0.08 * f
     theta = np.radians(fields_deg)
     b.field_angle[:] = theta

@@ -415,11 +415,18 @@ struct Layout {
     std::vector<int64_t> field_offset;
     std::vector<int> samples;
     int64_t rays = 0;
+    // constraint rows in row order: the items without extrema (E), then those with (E_t)
+    std::vector<mko_constraint> items;
+    int E = 0, Et = 0;
     bool soft() const { return settings.failure == MKO_FAILURE_SOFT; }
     int64_t penalty_rows() const { return soft() ? rays : 0; }
     int64_t trace_rows() const { return rays + penalty_rows(); }
+    int64_t rows() const { return trace_rows() + E + Et; }
     int64_t offset(int f, int l) const { return field_offset[f] + int64_t(l) * samples[f]; }
 };
+
+void check_constraint(const mko_constraint& c, int k, const Batch& B);
+bool con_needs_extrema(const mko_constraint& c);
 
 inline Layout make_layout(const Batch& B, const mko_residual_settings& s) {
     if (s.failure != MKO_FAILURE_SOFT && s.failure != MKO_FAILURE_HARD)
@@ -440,6 +447,15 @@ inline Layout make_layout(const Batch& B, const mko_residual_settings& s) {
         offset += int64_t(lay.L) * n;
     }
     lay.rays = offset;
+    if (s.extra && s.extra->count > 0) {
+        if (!s.extra->items) throw std::invalid_argument("constraint rows without items");
+        for (int k = 0; k < s.extra->count; ++k) check_constraint(s.extra->items[k], k, B);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int k = 0; k < s.extra->count; ++k)
+                if (con_needs_extrema(s.extra->items[k]) == (pass == 1))
+                    lay.items.push_back(s.extra->items[k]);
+        for (const mko_constraint& c : lay.items) (con_needs_extrema(c) ? lay.Et : lay.E) += 1;
+    }
     return lay;
 }
 
@@ -546,6 +562,280 @@ void variant_merit(const Batch& B, int v, const Layout& lay, double& merit, int6
     for (uint8_t s : st) {
         failed += s != Ok ? 1 : 0;
         hard_failed += is_hard(Status(s)) ? 1 : 0;
+    }
+}
+
+// ---- surface extrema: src/batch/batch_extrema.cpp:17-66, include/mikroptim/trace/
+// surface_extrema.hpp:47-110 -----------------------------------------------------------------
+// Per table surface s of variant v, over the rays with status Ok (hard trace) of all fields,
+// wavelengths and samples: the largest height^2 x^2 + y^2 at the intersection (the start point
+// for s = 0) and the smallest cos I = d_before . N, N = (-c x, -c y, 1 - c z), c the effective
+// curvature (0 for stop and plane); neutral 0 and 1 without a valid ray; out[2s] = sqrt(max
+// height^2), out[2s+1] = min cos I.
+inline double extremum_max(double acc, double x) { return x > acc ? x : acc; }
+inline double extremum_min(double acc, double x) { return x < acc ? x : acc; }
+
+void variant_extrema(const Batch& B, int v, const Layout& lay, double* out) {
+    const Table tb = B.table(v);
+    const int S = B.S();
+    std::vector<double> h2(size_t(S), 0.0), ci(size_t(S), 1.0);
+    Record rec;
+    for (int f = 0; f < lay.F; ++f) {
+        const Quadrature& q = lay.quad[f];
+        const RefSphere ref = reference_of(B, tb, v, f);
+        for (int l = 0; l < lay.L; ++l) {
+            for (int k = 0; k < lay.samples[f]; ++k) {
+                Ray ray = source_ray(B, v, f, l, sample(q, k));
+                if (trace_to_reference(tb, l, ray, ref, nullptr, &rec) != Ok) continue;
+                for (int j = 0; j < S; ++j) {
+                    const Vec3 p = rec.p[j];
+                    const double c =
+                        j == 0 ? 0.0 : (tb.type[j] == MKO_SURFACE_SPHERE ? tb.c[j] : 0.0);
+                    const Vec3 d = j == 0 ? rec.d[0] : rec.d[j - 1];
+                    const Vec3 n{-c * p.x, -c * p.y, 1.0 - c * p.z};
+                    h2[size_t(j)] = extremum_max(h2[size_t(j)], p.x * p.x + p.y * p.y);
+                    ci[size_t(j)] = extremum_min(ci[size_t(j)], dot(d, n));
+                }
+            }
+        }
+    }
+    for (int j = 0; j < S; ++j) {
+        out[2 * j] = std::sqrt(h2[size_t(j)]);
+        out[2 * j + 1] = ci[size_t(j)];
+    }
+}
+
+// ---- paraxial y-nu trace at lambda_ref: compute_paraxial, src/paraxial/paraxial.cpp:98-175;
+// paraxial_trace / petzval_sum, include/mikroptim/paraxial/ynu_trace.hpp:53-93 -------------------
+struct Paraxial {
+    double A = 1.0, B = 0.0, C = 0.0, D = 1.0, power = 0.0, petzval_curvature = 0.0;
+};
+
+Paraxial batch_paraxial(const Batch& B, int v) {
+    const int S = B.S();
+    if (!B.b.paraxial_index) throw std::invalid_argument("paraxial constraint without paraxial_index");
+    const double* c = B.b.curvature + size_t(v) * S;
+    const double* t = B.b.thickness + size_t(v) * S;
+    const double* n = B.b.paraxial_index + size_t(v) * S;
+    for (int j = 0; j < S; ++j) {
+        if (!std::isfinite(c[j]) || !std::isfinite(t[j]) || !std::isfinite(n[j]))
+            throw std::invalid_argument("paraxial trace: value not finite");
+        if (!(n[j] > 0.0)) throw std::invalid_argument("paraxial trace: index must be > 0");
+    }
+    if (!(std::isfinite(B.b.pupil_radius[v]) && B.b.pupil_radius[v] > 0.0))
+        throw std::invalid_argument("paraxial trace: stop radius must be > 0");
+    const auto trace = [&](double y, double u) {
+        double nb = 1.0;
+        for (int j = 0; j < S; ++j) {
+            if (j > 0) y += t[j - 1] * u;
+            const double phi = c[j] * (n[j] - nb);
+            u = (nb * u - y * phi) / n[j];
+            nb = n[j];
+        }
+        return std::pair<double, double>{y, u};
+    };
+    const double n_image = n[S - 1];
+    const auto marginal = trace(1.0, 0.0);
+    const auto chief = trace(0.0, 1.0);
+    Paraxial p;
+    p.A = marginal.first;
+    p.C = n_image * marginal.second;
+    p.B = chief.first;
+    p.D = n_image * chief.second;
+    p.power = -p.C;
+    if (p.C == 0.0) p.power = 0.0;
+    double ps = 0.0, nb = 1.0;
+    for (int j = 0; j < S; ++j) {
+        ps += c[j] * (n[j] - nb) / (nb * n[j]);
+        nb = n[j];
+    }
+    p.petzval_curvature = -n_image * ps;
+    return p;
+}
+
+// ---- constraint rows: paraxial_residual, src/optim/paraxial_residuals.cpp:257-371;
+// mechanical_residual, src/optim/mechanical_constraints.cpp:58-281 -----------------------------
+inline bool con_paraxial(int type) { return type >= MKO_CON_FOCAL_LENGTH && type <= MKO_CON_PETZVAL; }
+inline bool con_mechanical(int type) {
+    return type >= MKO_CON_MAX_SEMI_DIAMETER && type <= MKO_CON_MAX_INCIDENCE_ANGLE;
+}
+inline bool con_gap(int type) {
+    return type == MKO_CON_MIN_CENTER_THICKNESS || type == MKO_CON_MAX_CENTER_THICKNESS ||
+           type == MKO_CON_MIN_EDGE_THICKNESS;
+}
+bool con_needs_extrema(const mko_constraint& c) {
+    return c.type == MKO_CON_MAX_SEMI_DIAMETER || c.type == MKO_CON_MAX_INCIDENCE_ANGLE ||
+           (c.type == MKO_CON_MIN_EDGE_THICKNESS && c.edge_height == MKO_EDGE_RAYS);
+}
+inline bool con_needs_paraxial(const mko_constraint& c) {
+    return c.type == MKO_CON_FOCAL_LENGTH || c.type == MKO_CON_PARAXIAL_FOCUS ||
+           c.type == MKO_CON_PETZVAL;
+}
+
+// check_paraxial_constraints (paraxial_residuals.cpp:152-255), check_mechanical_constraints
+// (mechanical_constraints.cpp:148-203)
+void check_constraint(const mko_constraint& c, int k, const Batch& B) {
+    const auto fail = [k](const std::string& what) {
+        throw std::invalid_argument("constraint " + std::to_string(k) + ": " + what);
+    };
+    const int S = B.S();
+    if (!(std::isfinite(c.weight) && c.weight >= 0.0)) fail("weight must be finite and >= 0");
+    if (!(std::isfinite(c.shift) && c.shift >= 0.0)) fail("shift must be finite and >= 0");
+    if (con_paraxial(c.type)) {
+        if (c.mode < MKO_MODE_EQUAL || c.mode > MKO_MODE_MAX) fail("unknown mode");
+        if (!std::isfinite(c.target)) fail("target must be finite");
+        if (c.shift != 0.0 && c.mode == MKO_MODE_EQUAL) fail("shift only for one-sided modes");
+        switch (c.type) {
+        case MKO_CON_FOCAL_LENGTH:
+            if (c.target == 0.0) fail("target focal length must not be 0");
+            if (c.mode != MKO_MODE_EQUAL) fail("defined as an equation only");
+            break;
+        case MKO_CON_PARAXIAL_FOCUS:
+            if (c.mode != MKO_MODE_EQUAL) fail("defined as an equation only");
+            if (B.b.source != MKO_SOURCE_COLLIMATED) fail("needs a collimated source");
+            break;
+        case MKO_CON_PARFOCAL_LENGTH:
+            if (c.surface < 0 || c.surface >= S) fail("reference surface out of range");
+            if (!std::isfinite(c.offset)) fail("offset must be finite");
+            break;
+        case MKO_CON_PETZVAL:
+            if (c.mode != MKO_MODE_MAX) fail("defined as an upper bound only");
+            if (c.target < 0.0) fail("kappa_max must be >= 0");
+            break;
+        default:
+            break;
+        }
+        if (con_needs_paraxial(c) && !B.b.paraxial_index) fail("batch has no paraxial_index");
+        return;
+    }
+    if (!con_mechanical(c.type)) fail("unsupported constraint type");
+    if (c.surface < 0 || c.surface >= S) fail("surface out of range");
+    if (con_gap(c.type)) {
+        if (c.next_surface < 0 || c.next_surface >= S) fail("next surface out of range");
+        if (c.next_surface <= c.surface) fail("next surface not behind the surface");
+    }
+    if (!std::isfinite(c.target)) fail("limit must be finite");
+    if (c.type == MKO_CON_MAX_SEMI_DIAMETER && !(c.target > 0.0)) fail("semi-diameter must be > 0");
+    if (c.type == MKO_CON_MAX_INCIDENCE_ANGLE && !(c.target > 0.0 && c.target < 1.0))
+        fail("cos theta_max must be in (0, 1)");
+    if (!(std::isfinite(c.offset) && c.offset >= 0.0)) fail("clearance must be finite and >= 0");
+    if (c.type == MKO_CON_MIN_EDGE_THICKNESS) {
+        if (c.edge_height != MKO_EDGE_SEMI_DIAMETER && c.edge_height != MKO_EDGE_RAYS)
+            fail("unknown edge height");
+        if (c.edge_height == MKO_EDGE_SEMI_DIAMETER &&
+            !(std::isfinite(c.semi_diameter) && c.semi_diameter > 0.0 &&
+              std::isfinite(c.next_semi_diameter) && c.next_semi_diameter > 0.0))
+            fail("semi-diameters must be finite and > 0");
+    }
+}
+
+// sag / edge_thickness, mechanical_constraints.cpp:58-69
+inline double sag(double curvature, double height) {
+    const double h2 = height * height;
+    const double a = std::abs(curvature * height);
+    if (!(a < 1.0)) return curvature * h2;
+    return curvature * h2 / (1.0 + std::sqrt((1.0 - a) * (1.0 + a)));
+}
+
+double constraint_row(const mko_constraint& c, const Batch& B, int v, const Paraxial& p,
+                      const double* ext) {
+    const int S = B.S();
+    const double* t = B.b.thickness + size_t(v) * S;
+    if (con_paraxial(c.type)) {
+        double q = 0.0;
+        switch (c.type) {
+        case MKO_CON_FOCAL_LENGTH: q = c.target * (1.0 - c.target * p.power); break;
+        case MKO_CON_WORKING_DISTANCE: q = t[S - 1] - c.target; break;
+        case MKO_CON_PARFOCAL_LENGTH: {
+            double length = c.offset;
+            for (int j = c.surface; j < S; ++j) length += t[j];
+            q = length - c.target;
+            break;
+        }
+        case MKO_CON_PARAXIAL_FOCUS:
+            q = B.b.pupil_radius[v] *
+                (p.A + t[S - 1] * p.C / B.b.paraxial_index[size_t(v) * S + size_t(S - 1)]);
+            break;
+        default: q = std::abs(p.petzval_curvature) - c.target; break;
+        }
+        if (c.shift != 0.0) {
+            if (c.mode == MKO_MODE_MAX) q += c.shift;
+            else if (c.mode == MKO_MODE_MIN) q -= c.shift;
+        }
+        const double m = c.mode == MKO_MODE_MIN ? std::min(q, 0.0)
+                         : c.mode == MKO_MODE_MAX ? std::max(q, 0.0)
+                                                  : q;
+        return std::sqrt(c.weight) * m;
+    }
+    const auto center = [&] {
+        double sum = 0.0;
+        for (int j = c.surface; j < c.next_surface; ++j) sum += t[j];
+        return sum;
+    };
+    const auto curv = [&](int j) {
+        return B.b.type[j] == MKO_SURFACE_SPHERE ? B.b.curvature[size_t(v) * S + size_t(j)] : 0.0;
+    };
+    double value = 0.0;
+    switch (c.type) {
+    case MKO_CON_MAX_SEMI_DIAMETER: value = ext[2 * c.surface] + c.offset; break;
+    case MKO_CON_MIN_CENTER_THICKNESS:
+    case MKO_CON_MAX_CENTER_THICKNESS: value = center(); break;
+    case MKO_CON_MIN_EDGE_THICKNESS: {
+        const double h = (c.edge_height == MKO_EDGE_RAYS
+                              ? std::max(ext[2 * c.surface], ext[2 * c.next_surface])
+                              : std::max(c.semi_diameter, c.next_semi_diameter)) +
+                         c.offset;
+        value = center() - sag(curv(c.surface), h) + sag(curv(c.next_surface), h);
+        break;
+    }
+    default: value = ext[2 * c.surface + 1]; break;
+    }
+    double g = (c.type == MKO_CON_MAX_SEMI_DIAMETER || c.type == MKO_CON_MAX_CENTER_THICKNESS)
+                   ? value - c.target
+                   : c.target - value;
+    if (c.shift != 0.0) g += c.shift;
+    return std::sqrt(c.weight) * std::max(g, 0.0);
+}
+
+// The E + E_t constraint rows of variant v (ext: its extrema [S][2], needed when E_t > 0);
+// untraced_only: the first E rows only.
+void variant_rows(const Batch& B, int v, const Layout& lay, const double* ext, double* out,
+                  bool untraced_only = false) {
+    Paraxial p;
+    bool need_p = false;
+    const int n = untraced_only ? lay.E : lay.E + lay.Et;
+    for (int k = 0; k < n; ++k) need_p = need_p || con_needs_paraxial(lay.items[size_t(k)]);
+    if (need_p) p = batch_paraxial(B, v);
+    for (int k = 0; k < n; ++k) out[k] = constraint_row(lay.items[size_t(k)], B, v, p, ext);
+}
+
+// rows of variant v into out (extrema from its own trace when E_t > 0)
+void variant_rows_traced(const Batch& B, int v, const Layout& lay, double* out) {
+    std::vector<double> ext;
+    if (lay.Et > 0) {
+        ext.resize(size_t(B.S()) * 2);
+        variant_extrema(B, v, lay, ext.data());
+    }
+    variant_rows(B, v, lay, ext.data(), out);
+}
+
+// merit with constraint rows, CpuLmBackend::merit (src/optim/levenberg_marquardt.cpp:866-888):
+// (ray merit + sum e^2) + sum t^2, each sum ascending from 0
+void variant_merit_rows(const Batch& B, int v, const Layout& lay, double& merit, int64_t& failed,
+                        int64_t& hard_failed) {
+    variant_merit(B, v, lay, merit, failed, hard_failed);
+    if (lay.E + lay.Et == 0) return;
+    std::vector<double> rows(size_t(lay.E + lay.Et));
+    variant_rows_traced(B, v, lay, rows.data());
+    if (lay.E > 0) {
+        double e = 0.0;
+        for (int k = 0; k < lay.E; ++k) e += rows[size_t(k)] * rows[size_t(k)];
+        merit = merit + e;
+    }
+    if (lay.Et > 0) {
+        double tsum = 0.0;
+        for (int k = lay.E; k < lay.E + lay.Et; ++k) tsum += rows[size_t(k)] * rows[size_t(k)];
+        merit += tsum;
     }
 }
 
@@ -724,7 +1014,7 @@ inline void perturbed_vector(const Vars& vars, const double* x0, const double* r
 // A batch whose per-variant arrays this oracle owns (FD batches, point batches).
 struct OwnedBatch {
     mko_batch b{};
-    std::vector<double> curvature, thickness, index, pupil, fangle, ftarget, refcurv;
+    std::vector<double> curvature, thickness, index, pupil, fangle, ftarget, refcurv, pindex;
     // copies variant `from` of `src` into variant `to`
     void init(const mko_batch& src, int V) {
         b = src;
@@ -737,6 +1027,8 @@ struct OwnedBatch {
         fangle.assign(size_t(V) * F, 0.0);
         ftarget.assign(size_t(V) * F, 0.0);
         if (src.reference_curvature) refcurv.assign(size_t(V) * F, 0.0);
+        if (src.paraxial_index) pindex.assign(size_t(V) * S, 0.0);
+        b.paraxial_index = src.paraxial_index ? pindex.data() : nullptr;
         b.curvature = curvature.data();
         b.thickness = thickness.data();
         b.index = index.data();
@@ -755,16 +1047,22 @@ struct OwnedBatch {
         std::copy_n(src.field_target + from * F, F, ftarget.data() + to * F);
         if (src.reference_curvature)
             std::copy_n(src.reference_curvature + from * F, F, refcurv.data() + to * F);
+        if (src.paraxial_index)
+            std::copy_n(src.paraxial_index + from * S, S, pindex.data() + to * S);
     }
 };
 
+// RESIDUALS of every variant, rows [V][trace rows + E + E_t] (CpuLmBackend::host_residuals,
+// src/optim/levenberg_marquardt.cpp:812-864: constraint rows from the variant's own extrema)
 void residuals_all(const Batch& B, const Layout& lay, double* residuals, double* opd,
                    uint8_t* status, int threads) {
-    const size_t R = size_t(lay.trace_rows());
+    const size_t R = size_t(lay.rows());
     const size_t rays = size_t(lay.rays);
     parallel_for(0, B.b.variants, threads, [&](int v) {
         variant_residuals(B, v, lay, residuals + size_t(v) * R, opd + size_t(v) * rays,
                           status + size_t(v) * rays);
+        if (lay.E + lay.Et > 0)
+            variant_rows_traced(B, v, lay, residuals + size_t(v) * R + size_t(lay.trace_rows()));
     });
 }
 
@@ -796,7 +1094,7 @@ void fd_jacobian(const Batch& base, const Layout& lay, const Vars& vars, int run
         }
     }
     const Batch FB{fd.b};
-    const size_t R = size_t(lay.trace_rows());
+    const size_t R = size_t(lay.rows());
     const size_t rays = size_t(lay.rays);
     std::vector<double> res(size_t(runs) * n * R), opd(size_t(runs) * n * rays);
     std::vector<uint8_t> st(size_t(runs) * n * rays);
@@ -1077,6 +1375,46 @@ inline void backward_ray(const Table& tb, int l, const Record& rec, bool reached
 }
 
 // adjoint_jacobian_cpu, src/optim/adjoint_jacobian.cpp:370-481 (no extra rows)
+// Constraint rows of the adjoint Jacobian by central differences, extra_rows_untraced /
+// extra_rows_cpu (src/optim/adjoint_jacobian.cpp:245-337): per run, x0 and the 2P points of
+// fd_pattern applied to a one-variant copy, rows of each (rows with extrema from the copy's own
+// trace), fd_derivative per row into J[run][p][trace rows + e] and r0.
+void extra_rows(const Batch& base, const Layout& lay, const Vars& vars, int runs, const double* x0,
+                double* J, double* r0, int threads) {
+    const int P = vars.P;
+    const int S = base.S();
+    const int64_t R = lay.rows(), T = lay.trace_rows();
+    const int E = lay.E + lay.Et;
+    parallel_for(0, runs, threads, [&](int i) {
+        OwnedBatch one;
+        one.init(base.b, 1);
+        one.copy_variant(base.b, base.b.variants == 1 ? 0 : i, 0);
+        const Batch OB{one.b};
+        const double* xi = x0 + size_t(i) * P;
+        const FdPattern pat = fd_pattern(vars, xi);
+        std::vector<double> y(xi, xi + P);
+        std::vector<double> e0(static_cast<size_t>(E)), ea(static_cast<size_t>(E)),
+            eb(static_cast<size_t>(E));
+        const auto rows_at = [&](std::vector<double>& out) {
+            apply(vars, S, one.curvature.data(), one.thickness.data(), 0, y.data());
+            variant_rows_traced(OB, 0, lay, out.data());
+        };
+        rows_at(e0);
+        for (int e = 0; e < E; ++e) r0[size_t(i) * R + size_t(T + e)] = e0[size_t(e)];
+        for (int k = 0; k < P; ++k) {
+            y[k] = pat.plus[k];
+            rows_at(ea);
+            y[k] = pat.minus[k];
+            rows_at(eb);
+            y[k] = xi[k];
+            double* col = J + (size_t(i) * P + size_t(k)) * R + size_t(T);
+            for (int e = 0; e < E; ++e)
+                col[e] = fd_derivative(pat.stencil[k], e0[size_t(e)], ea[size_t(e)], eb[size_t(e)],
+                                       pat.step_plus[k], pat.step_minus[k]);
+        }
+    });
+}
+
 void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, int runs,
                       const double* x0, double* J, double* r0, uint8_t* status_out, int threads) {
     const int P = vars.P;
@@ -1099,7 +1437,7 @@ void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, in
         }
     }
     const int64_t rays = lay.rays;
-    const int64_t R = lay.trace_rows();
+    const int64_t R = lay.rows();
     std::fill(J, J + size_t(runs) * P * R, 0.0);
     std::vector<uint8_t> status(size_t(runs) * rays);
     parallel_for(0, runs, threads, [&](int v) {
@@ -1164,6 +1502,7 @@ void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, in
         }
     });
     if (status_out) std::copy(status.begin(), status.end(), status_out);
+    if (lay.E + lay.Et > 0) extra_rows(base, lay, vars, runs, x0, J, r0, threads);
 }
 
 // normal_equations_host, include/mikroptim/optim/normal_equations.hpp:19-56
@@ -1498,7 +1837,7 @@ std::vector<RunState> levenberg_marquardt(Backend& backend, int P, const Bounds&
     return state;
 }
 
-// CpuLmBackend, src/optim/levenberg_marquardt.cpp:678-866 (no extra residuals)
+// CpuLmBackend, src/optim/levenberg_marquardt.cpp:678-888 (constraint rows from the settings)
 Backend cpu_backend(const Batch& base, const Layout& lay, const Vars& vars, int method,
                     int threads) {
     Backend be;
@@ -1515,7 +1854,7 @@ Backend cpu_backend(const Batch& base, const Layout& lay, const Vars& vars, int 
             tb = &tmpl.b;
         }
         const Batch TB{*tb};
-        const int64_t R = lay.trace_rows();
+        const int64_t R = lay.rows();
         std::vector<double> J(size_t(count) * P * R), r0(size_t(count) * R);
         if (method == MKO_JACOBIAN_ADJOINT)
             adjoint_jacobian(TB, lay, vars, count, x.data(), J.data(), r0.data(), nullptr, threads);
@@ -1540,7 +1879,7 @@ Backend cpu_backend(const Batch& base, const Layout& lay, const Vars& vars, int 
         }
         const Batch PB{pb.b};
         parallel_for(0, count, threads, [&](int v) {
-            variant_merit(PB, v, lay, merit[size_t(v)], failed[size_t(v)], hard[size_t(v)]);
+            variant_merit_rows(PB, v, lay, merit[size_t(v)], failed[size_t(v)], hard[size_t(v)]);
         });
     };
     return be;
@@ -1733,9 +2072,62 @@ int orc_merit(const mko_batch* b, const mko_residual_settings* s, double* merit,
         const Batch B{*b};
         const Layout lay = make_layout(B, *s);
         parallel_for(0, b->variants, threads, [&](int v) {
-            variant_merit(B, v, lay, merit[v], failed[v], hard_failed[v]);
+            variant_merit_rows(B, v, lay, merit[v], failed[v], hard_failed[v]);
         });
     })
+}
+
+// Surface extrema [V][S][2] (surface_extrema_cpu, src/batch/batch_extrema.cpp:75-92).
+int orc_surface_extrema(const mko_batch* b, const mko_residual_settings* s, double* extrema,
+                        int threads) {
+    ORC_TRY({
+        const Batch B{*b};
+        const Layout lay = make_layout(B, *s);
+        const size_t S = size_t(b->surfaces);
+        parallel_for(0, b->variants, threads,
+                     [&](int v) { variant_extrema(B, v, lay, extrema + size_t(v) * S * 2); });
+    })
+}
+
+// Paraxial data of variant v at lambda_ref: out = {A, B, C, D, power, petzval_curvature}
+// (batch_paraxial, src/batch/system_batch.cpp:965; compute_paraxial, src/paraxial/paraxial.cpp:98).
+int orc_paraxial(const mko_batch* b, int v, double* out) {
+    ORC_TRY({
+        const Paraxial p = batch_paraxial(Batch{*b}, v);
+        out[0] = p.A;
+        out[1] = p.B;
+        out[2] = p.C;
+        out[3] = p.D;
+        out[4] = p.power;
+        out[5] = p.petzval_curvature;
+    })
+}
+
+// Rows of the constraints in s->extra for every variant [V][E + E_t] (row order: untraced,
+// then traced); extrema [V][S][2] as given (NULL: from each variant's own trace).
+int orc_constraint_rows(const mko_batch* b, const mko_residual_settings* s, const double* extrema,
+                        double* rows, int threads) {
+    ORC_TRY({
+        const Batch B{*b};
+        const Layout lay = make_layout(B, *s);
+        const size_t E = size_t(lay.E + lay.Et), S = size_t(b->surfaces);
+        parallel_for(0, b->variants, threads, [&](int v) {
+            if (extrema)
+                variant_rows(B, v, lay, extrema + size_t(v) * S * 2, rows + size_t(v) * E);
+            else
+                variant_rows_traced(B, v, lay, rows + size_t(v) * E);
+        });
+    })
+}
+
+// Rows per variant of the residual vector: trace rows + E + E_t.
+int64_t orc_residual_rows(const mko_batch* b, const mko_residual_settings* s) {
+    try {
+        return make_layout(Batch{*b}, *s).rows();
+    } catch (const std::exception& e) {
+        orc::g_error = e.what();
+        return -1;
+    }
 }
 
 int orc_fd_pattern(const mko_variables* vars, int surfaces, const double* x0, double* nominal,

@@ -14,8 +14,12 @@
 #include "mikroptim/model/system.hpp"
 #include "mikroptim/optim/adjoint_jacobian.hpp"
 #include "mikroptim/optim/fd_jacobian.hpp"
+#include "mikroptim/batch/batch_extrema.hpp"
 #include "mikroptim/optim/levenberg_marquardt.hpp"
+#include "mikroptim/optim/mechanical_constraints.hpp"
 #include "mikroptim/optim/normal_equations.hpp"
+#include "mikroptim/optim/paraxial_residuals.hpp"
+#include "mikroptim/optim/trace_extra_residuals.hpp"
 #include "mikroptim/optim/variable_set.hpp"
 #include "mikroptim/optim/variant_generator.hpp"
 #include "mikroptim/pupil/quadrature.hpp"
@@ -64,7 +68,71 @@ mko::SystemBatchHost batch_of(const mko_batch& b) {
     std::copy_n(b.field_target, V * F, h.field_target_data().begin());
     if (b.reference_sphere != MKO_REF_LAST_SURFACE)
         std::copy_n(b.reference_curvature, V * F, h.reference_curvature_data().begin());
+    if (b.paraxial_index) std::copy_n(b.paraxial_index, V * S, h.paraxial_index_data().begin());
     return h;
+}
+
+bool con_gap(int type) {
+    return type == MKO_CON_MIN_CENTER_THICKNESS || type == MKO_CON_MAX_CENTER_THICKNESS ||
+           type == MKO_CON_MIN_EDGE_THICKNESS;
+}
+
+bool needs_extrema(const mko_constraint& c) {
+    return c.type == MKO_CON_MAX_SEMI_DIAMETER || c.type == MKO_CON_MAX_INCIDENCE_ANGLE ||
+           (c.type == MKO_CON_MIN_EDGE_THICKNESS && c.edge_height == MKO_EDGE_RAYS);
+}
+
+// The constraint items as the reference's own row sources: one ExtraResiduals per untraced item
+// (make_paraxial_residuals / make_mechanical_residuals(...).geometric), concatenated in item
+// order, and the traced mechanical items in one TraceExtraResiduals.
+struct Extras {
+    mko::ExtraResiduals extra;
+    mko::TraceExtraResiduals trace_extra;
+};
+Extras extras_of(const mko_residual_settings& s) {
+    Extras out;
+    if (!s.extra || s.extra->count == 0) return out;
+    std::vector<mko::ExtraResiduals> parts;
+    mko::MechanicalConstraints traced;
+    for (int k = 0; k < s.extra->count; ++k) {
+        const mko_constraint& c = s.extra->items[k];
+        if (c.type <= MKO_CON_PETZVAL) {
+            mko::ParaxialConstraint pc;
+            pc.type = static_cast<mko::ConstraintType>(c.type);
+            pc.mode = static_cast<mko::ConstraintMode>(c.mode);
+            pc.weight = c.weight;
+            pc.target = c.target;
+            pc.reference_surface = c.surface;
+            pc.offset = c.offset;
+            pc.shift = c.shift;
+            mko::ParaxialConstraints one;
+            one.items.push_back(pc);
+            parts.push_back(mko::make_paraxial_residuals(one));
+            continue;
+        }
+        mko::MechanicalConstraint mc;
+        mc.type = static_cast<mko::ConstraintType>(c.type);
+        mc.file_surface = c.surface;
+        mc.surface = c.surface;
+        mc.next_surface = con_gap(c.type) ? c.next_surface : c.surface;
+        mc.weight = c.weight;
+        mc.limit = c.target;
+        mc.clearance = c.offset;
+        mc.height = static_cast<mko::EdgeHeight>(c.edge_height);
+        mc.semi_diameter = c.semi_diameter;
+        mc.next_semi_diameter = c.next_semi_diameter;
+        mc.shift = c.shift;
+        if (needs_extrema(c)) {
+            traced.items.push_back(mc);
+        } else {
+            mko::MechanicalConstraints one;
+            one.items.push_back(mc);
+            parts.push_back(mko::make_mechanical_residuals(one).geometric);
+        }
+    }
+    out.extra = mko::concat_extra_residuals(std::move(parts));
+    if (!traced.empty()) out.trace_extra = mko::make_mechanical_residuals(traced).traced;
+    return out;
 }
 
 mko::ResidualSettings settings_of(const mko_residual_settings& s) {
@@ -134,6 +202,9 @@ mko::LmOptions lm_options_of(const mko_lm_options& o, const mko_residual_setting
     r.reject_new_failures = o.reject_new_failures != 0;
     r.threads = o.threads;
     r.jacobian.quadrature = settings_of(s);
+    const Extras x = extras_of(s);
+    r.jacobian.extra = x.extra;
+    r.jacobian.trace_extra = x.trace_extra;
     r.jacobian_method = o.jacobian_method == MKO_JACOBIAN_ADJOINT
                             ? mko::JacobianMethod::Adjoint
                             : mko::JacobianMethod::FiniteDifferences;
@@ -177,9 +248,72 @@ int ref_residuals(const mko_batch* b, const mko_residual_settings* s, double* re
         const mko::SystemBatchHost h = batch_of(*b);
         const mko::ResidualLayout layout(h.topology(), settings_of(*s));
         const mko::ResidualResult r = mko::residuals_cpu(h, layout);
-        std::copy(r.residuals.begin(), r.residuals.end(), residuals);
         std::copy(r.opd.begin(), r.opd.end(), opd);
         std::copy(r.status.begin(), r.status.end(), status);
+        // constraint rows after the ray rows, as CpuLmBackend::host_residuals
+        // (src/optim/levenberg_marquardt.cpp:812-864)
+        const Extras x = extras_of(*s);
+        mko::FdJacobianOptions fo;
+        fo.quadrature = settings_of(*s);
+        fo.extra = x.extra;
+        fo.trace_extra = x.trace_extra;
+        const int E = fo.extra_count();
+        const size_t tr = size_t(r.stride), R = tr + size_t(E);
+        std::vector<double> ev;
+        if (E > 0) {
+            const mko::SurfaceExtremaResult ext = x.trace_extra.count > 0
+                                                      ? mko::surface_extrema_cpu(h, layout)
+                                                      : mko::SurfaceExtremaResult{};
+            ev = mko::fd_extra_residuals(fo, h, ext);
+        }
+        for (size_t v = 0; v < size_t(h.variants()); ++v) {
+            std::copy_n(r.residuals.begin() + std::ptrdiff_t(v * tr), tr, residuals + v * R);
+            for (int e = 0; e < E; ++e) residuals[v * R + tr + size_t(e)] = ev[v * size_t(E) + size_t(e)];
+        }
+    })
+}
+
+int ref_surface_extrema(const mko_batch* b, const mko_residual_settings* s, double* extrema) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::ResidualLayout layout(h.topology(), settings_of(*s));
+        const mko::SurfaceExtremaResult r = mko::surface_extrema_cpu(h, layout);
+        std::copy(r.values.begin(), r.values.end(), extrema);
+    })
+}
+
+int ref_paraxial(const mko_batch* b, int v, double* out) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::ParaxialData p = mko::batch_paraxial(h, v);
+        out[0] = p.abcd[0];
+        out[1] = p.abcd[1];
+        out[2] = p.abcd[2];
+        out[3] = p.abcd[3];
+        out[4] = p.power;
+        out[5] = p.petzval_curvature;
+    })
+}
+
+// Rows [V][E + E_t] with the given extrema [V][S][2] (fd_extra_residuals,
+// src/optim/fd_jacobian.cpp:164-180).
+int ref_constraint_rows(const mko_batch* b, const mko_residual_settings* s, const double* extrema,
+                        double* rows) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const Extras x = extras_of(*s);
+        mko::FdJacobianOptions fo;
+        fo.quadrature = settings_of(*s);
+        fo.extra = x.extra;
+        fo.trace_extra = x.trace_extra;
+        mko::SurfaceExtremaResult ext;
+        if (x.trace_extra.count > 0) {
+            ext.variants = h.variants();
+            ext.surfaces = h.surfaces();
+            ext.values.assign(extrema, extrema + size_t(ext.variants) * size_t(ext.surfaces) * 2);
+        }
+        const std::vector<double> ev = mko::fd_extra_residuals(fo, h, ext);
+        std::copy(ev.begin(), ev.end(), rows);
     })
 }
 
@@ -188,10 +322,19 @@ int ref_merit(const mko_batch* b, const mko_residual_settings* s, double* merit,
     REF_TRY({
         const mko::SystemBatchHost h = batch_of(*b);
         const mko::ResidualLayout layout(h.topology(), settings_of(*s));
-        const mko::MeritResult r = mko::merit_cpu(h, layout, false);
-        std::copy(r.merit.begin(), r.merit.end(), merit);
-        std::copy(r.failed.begin(), r.failed.end(), failed);
-        std::copy(r.hard_failed.begin(), r.hard_failed.end(), hard_failed);
+        // (ray merit + sum e^2) + sum t^2 as CpuLmBackend::merit
+        // (src/optim/levenberg_marquardt.cpp:866-888)
+        const Extras x = extras_of(*s);
+        const std::vector<double> extra = mko::extra_merit_values(x.extra, h, 1);
+        for (int v = 0; v < h.variants(); ++v) {
+            const mko::MeritResult m = mko::merit_cpu(h, v, layout, x.trace_extra.count > 0);
+            merit[v] = extra.empty() ? m.merit[0] : m.merit[0] + extra[size_t(v)];
+            if (x.trace_extra.count > 0)
+                merit[v] += mko::trace_extra_merit(x.trace_extra, h, v,
+                                                   mko::variant_extrema(m.extrema, 0));
+            failed[v] = m.failed[0];
+            hard_failed[v] = m.hard_failed[0];
+        }
     })
 }
 
@@ -222,6 +365,9 @@ int ref_fd_jacobian(const mko_batch* base, const mko_residual_settings* s,
         const mko::VariableSet set(description_of(*base, *vars), h.topology());
         mko::FdJacobianOptions o;
         o.quadrature = settings_of(*s);
+        const Extras x = extras_of(*s);
+        o.extra = x.extra;
+        o.trace_extra = x.trace_extra;
         const mko::FdJacobian jac = mko::fd_jacobian_cpu(
             set, h, std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o);
         std::copy(jac.jacobian.begin(), jac.jacobian.end(), J);
@@ -239,6 +385,9 @@ int ref_adjoint_jacobian(const mko_batch* base, const mko_residual_settings* s,
         mko::AdjointJacobianOptions o;
         o.quadrature = settings_of(*s);
         o.threads = threads;
+        const Extras x = extras_of(*s);
+        o.extra = x.extra;
+        o.trace_extra = x.trace_extra;
         const mko::AdjointJacobian jac = mko::adjoint_jacobian_cpu(
             set, h, std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o);
         std::copy(jac.jacobian.begin(), jac.jacobian.end(), J);

@@ -38,6 +38,26 @@ def main():
     s.rings, s.spokes = 3, 4
     res, opd, st = ref.residuals(ob, s)
     out.update({"objective_coarse_residuals": res, "objective_coarse_status": st})
+    out["objective_paraxial_v1"] = ref.paraxial(ob, 1)
+    # constraint rows (ExtraResiduals / TraceExtraResiduals) on the singlet
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    out["singlet_extrema"] = ref.surface_extrema(b, s)
+    s.constraints = systems.singlet_constraints()
+    out["singlet_rows"] = ref.constraint_rows(b, s)
+    out["singlet_rows_merit"] = ref.merit(b, s)[0]
+    soft = systems.singlet_settings(abi.FAILURE_SOFT)
+    soft.constraints = systems.singlet_constraints()
+    x0 = v.extract(b)[:2]
+    T = ref.residuals(b.take([0]), systems.singlet_settings(abi.FAILURE_SOFT))[0].shape[1]
+    for key, fn in (("fd", ref.fd_jacobian), ("adjoint", ref.adjoint_jacobian)):
+        J, r0, _ = fn(b.take([0, 1]), soft, v, x0)
+        out[f"singlet_{key}_rows_J"] = J[:, :, T:]
+        out[f"singlet_{key}_rows_r0"] = r0[:, T:]
+    lb, lv = systems.singlet(8, seed=3)
+    for key, method in (("fd", abi.JACOBIAN_FD), ("adjoint", abi.JACOBIAN_ADJOINT)):
+        opt = abi.lm_options(max_iterations=8, threads=2, jacobian_method=method)
+        x, _, _ = ref.optimize_lm(lb.take([0]), s, lv, lv.extract(lb)[:2], opt)
+        out[f"singlet_rows_lm_{key}_x"] = x
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "reference_outputs.npz"), **out)
     print("wrote", sorted(out))
 

@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 
 from mikroptim_amd import abi
-from mikroptim_amd.batch import rays_per_variant, trace_rows
+from mikroptim_amd.batch import rays_per_variant, residual_rows
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
@@ -34,6 +34,7 @@ def _declare(lib, prefix):
         "variant_seed": ([C.c_uint64, C.c_uint64], C.c_uint64),
         "unit_uniform": ([C.c_uint64], C.c_double),
         "df64_ops": ([C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+        "paraxial": ([C.POINTER(abi.Batch), C.c_int, _dp], C.c_int),
     }
     if prefix == "orc_":
         sig.update({
@@ -54,6 +55,8 @@ def _declare(lib, prefix):
             "trace_ray": ([bp, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_int,
                            _dp, _dp, _dp, _dp], C.c_int),
             "mt19937_64": ([C.c_uint64, C.c_uint64], C.c_uint64),
+            "surface_extrema": ([bp, sp, _dp, C.c_int], C.c_int),
+            "constraint_rows": ([bp, sp, _dp, _dp, C.c_int], C.c_int),
         })
     else:
         sig.update({
@@ -67,6 +70,8 @@ def _declare(lib, prefix):
                                   C.c_uint64, _dp], C.c_int),
             "optimize_lm": ([bp, sp, vp, C.c_int, _dp, C.POINTER(abi.LmOptions), _dp,
                              C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
+            "surface_extrema": ([bp, sp, _dp], C.c_int),
+            "constraint_rows": ([bp, sp, _dp, _dp], C.c_int),
         })
     for name, (args, res) in sig.items():
         f = getattr(lib, p + name)
@@ -91,7 +96,7 @@ class Lib:
 
     def residuals(self, batch, settings, threads=0):
         V = batch.variants
-        R = trace_rows(batch, settings)
+        R = residual_rows(batch, settings)
         rays = rays_per_variant(batch, settings)
         res = np.zeros((V, R))
         opd = np.zeros((V, rays))
@@ -118,7 +123,7 @@ class Lib:
     def fd_jacobian(self, base, settings, variables, x0, threads=0):
         x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
         runs, P = x0.shape
-        R = trace_rows(base, settings)
+        R = residual_rows(base, settings)
         J = np.zeros((runs, P, R))
         r0 = np.zeros((runs, R))
         ch = np.zeros((runs, P), dtype=np.int64)
@@ -133,7 +138,7 @@ class Lib:
     def adjoint_jacobian(self, base, settings, variables, x0, threads=0):
         x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
         runs, P = x0.shape
-        R = trace_rows(base, settings)
+        R = residual_rows(base, settings)
         rays = rays_per_variant(base, settings)
         J = np.zeros((runs, P, R))
         r0 = np.zeros((runs, R))
@@ -143,6 +148,35 @@ class Lib:
             C.byref(variables.c_struct()), runs, abi.ptr(x0), abi.ptr(J), abi.ptr(r0),
             abi.ptr(st, C.c_uint8), threads))
         return J, r0, st
+
+    def surface_extrema(self, batch, settings, threads=0):
+        """[V][S][2]: largest ray height and smallest cos I per surface over the valid rays."""
+        out = np.zeros((batch.variants, batch.surfaces, 2))
+        args = [C.byref(batch.c_struct()), C.byref(settings.c_struct()), abi.ptr(out)]
+        if self.prefix == "orc_":
+            args.append(threads)
+        self.check(self.fn("surface_extrema")(*args))
+        return out
+
+    def paraxial(self, batch, v):
+        """(A, B, C, D, power, petzval_curvature) of variant v at lambda_ref."""
+        out = np.zeros(6)
+        self.check(self.fn("paraxial")(C.byref(batch.c_struct()), v, abi.ptr(out)))
+        return out
+
+    def constraint_rows(self, batch, settings, extrema=None, threads=0):
+        """[V][E + E_t] constraint rows (row order: without extrema, then with); extrema
+        [V][S][2] as given (the oracle computes them when None)."""
+        if extrema is None and self.prefix == "ref_":
+            extrema = self.surface_extrema(batch, settings)
+        ext = None if extrema is None else np.ascontiguousarray(extrema, dtype=np.float64)
+        out = np.zeros((batch.variants, settings.extra_rows))
+        args = [C.byref(batch.c_struct()), C.byref(settings.c_struct()), abi.ptr(ext),
+                abi.ptr(out)]
+        if self.prefix == "orc_":
+            args.append(threads)
+        self.check(self.fn("constraint_rows")(*args))
+        return out
 
     def normal_equations(self, J, r):
         J = np.ascontiguousarray(J, dtype=np.float64)
