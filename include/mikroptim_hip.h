@@ -53,13 +53,14 @@ int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settin
  * pattern (write_fd_variants, src/optim/variant_generator.cpp:240-262), per_run = 1 the point
  * batch of LM merit / adjoint calls (lm_point_batch, src/optim/levenberg_marquardt.cpp:634).
  * vars_device: variable type / surface arrays in device memory; x0, plus, minus [runs][P] in
- * device memory; outputs are the per-variant arrays of the destination batch. */
+ * device memory; outputs are the per-variant arrays of the destination batch (paraxial_index
+ * NULL: not copied). */
 int mko_point_batch_device(const mko_batch* templ, int template_per_run, int runs, int per_run,
                            const mko_variables* vars_device, const double* x0,
                            const double* plus, const double* minus, double* curvature,
                            double* thickness, double* index, double* pupil_radius,
                            double* field_angle, double* field_target,
-                           double* reference_curvature, void* stream);
+                           double* reference_curvature, double* paraxial_index, void* stream);
 
 /* FD Jacobian from the residuals of a central FD batch (runs * (2P + 1) variants, rows per
  * variant at stride): J [runs][P][rows] by fd_derivative, r0 [runs][rows], status changes
@@ -123,21 +124,39 @@ void mko_engine_destroy(mko_engine* engine);
  * updates). Returns the totals since the previous call, resets them and sets the enable flag. */
 int mko_engine_kernel_timing(mko_engine* engine, int enable, double* ms, int64_t* launches);
 
+/* Constraint rows (settings->extra, see mikroptim_types.h): every host-layer entry point below
+ * appends them to each variant's residual vector, R = trace_rows + E + E_t rows, as the
+ * reference's FdJacobianOptions::extra / trace_extra do: residuals, J and r0 carry R rows, the
+ * merit is (ray merit + sum e^2) + sum t^2 (CpuLmBackend::merit, src/optim/levenberg_marquardt.cpp:866),
+ * a and g include the rows. Rows that need extrema take them from the same trace (every FD
+ * variant from its own); the adjoint paths differentiate the rows by central differences of
+ * the FD batch (extra_rows_cpu, src/optim/adjoint_jacobian.cpp:245-337). The device-layer
+ * entry points above reject settings with constraint rows. */
+
 /* TraceEngine::run(batch, TraceMode::Residuals, ...): host batch in, host results out
- * (residuals [V][trace_rows], opd / status [V][rays], either may be NULL). */
+ * (residuals [V][R], opd / status [V][rays], either may be NULL). */
 int mko_engine_residuals(mko_engine* engine, const mko_batch* batch,
                          const mko_residual_settings* settings, double* residuals, double* opd,
                          uint8_t* status);
 
-/* TraceEngine::run(batch, TraceMode::Merit, ...) with MeritRunOptions::extra_merit. */
+/* TraceEngine::run(batch, TraceMode::Merit, ...) with MeritRunOptions::extra_merit (added to
+ * the constraint rows' sum e^2 when both are present). */
 int mko_engine_merit(mko_engine* engine, const mko_batch* batch,
                      const mko_residual_settings* settings, const double* extra_merit,
                      double* merit, int64_t* failed, int64_t* hard_failed);
 
+/* Max reductions of the MERIT trace (TraceSettings::extrema, SurfaceExtremaResult,
+ * include/mikroptim/batch/batch_extrema.hpp:29): extrema [V][S][2] = {largest ray height
+ * sqrt(x^2 + y^2) at surface s, smallest cos I = d . N before the refraction} over the rays with
+ * status Ok of all fields, wavelengths and samples; surface 0: the start state; neutral 0 / 1
+ * without a valid ray. */
+int mko_engine_surface_extrema(mko_engine* engine, const mko_batch* batch,
+                               const mko_residual_settings* settings, double* extrema);
+
 /* DeviceFdJacobian::compute / DeviceAdjointJacobian::compute + download
  * (include/mikroptim/cuda/fd_jacobian.hpp:58, adjoint_jacobian.hpp:70): base has one variant
  * (shared template) or one per run; x0 [runs][P]. Outputs (host, any may be NULL):
- * J [runs][P][trace_rows] double, r0 [runs][trace_rows], a [runs][P][P], g [runs][P].
+ * J [runs][P][R] double, r0 [runs][R], a [runs][P][P], g [runs][P].
  * method ADJOINT with precision MKO_PRECISION_MIXED, J and r0 NULL, hard failure mode, <= 64
  * variables and <= 512 rays per tuple takes a and g from mko_adjoint_normal_device; every other
  * combination computes J in FP64 and a, g with mko_gram_device. */

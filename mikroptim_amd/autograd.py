@@ -10,15 +10,16 @@ placed on x's device."""
 import numpy as np
 import torch
 
-from .batch import trace_rows
+from .batch import residual_rows
 
 
 class TraceResiduals(torch.autograd.Function):
     """r = TraceResiduals.apply(x, engine, base, settings, variables)
 
     x: [runs][P] float64 tensor of variable values; base: SystemBatch with one variant (shared
-    template) or one per run; returns r [runs][trace_rows] float64. backward: grad_x[i] =
-    J_i^T grad_r[i] with the FP64 adjoint Jacobian J_i [P][R] at x_i."""
+    template) or one per run; returns r [runs][R] float64 (trace rows, then the settings'
+    constraint rows). backward: grad_x[i] = J_i^T grad_r[i] with the FP64 adjoint Jacobian
+    J_i [P][R] at x_i."""
 
     @staticmethod
     def forward(ctx, x, engine, base, settings, variables):
@@ -43,7 +44,7 @@ class TraceResiduals(torch.autograd.Function):
         J, _, _, _ = engine.jacobian(base, settings, variables, xn, method="adjoint",
                                      precision="fp64")
         gr = grad_r.detach().to("cpu", torch.float64).numpy()
-        assert gr.shape == (xn.shape[0], trace_rows(base, settings))
+        assert gr.shape == (xn.shape[0], residual_rows(base, settings))
         gx = np.matmul(J, gr[:, :, None])[:, :, 0]
         return torch.from_numpy(gx).to(x.device), None, None, None, None
 

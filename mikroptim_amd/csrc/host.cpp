@@ -102,11 +102,12 @@ inline int64_t trace_rows_of(const mko_batch& b, const mko_residual_settings& s)
 // Per-variant arrays of a batch: element counts per variant.
 struct Strides {
     size_t S, SL, F;
-    bool refc;
+    bool refc, pidx;
     explicit Strides(const mko_batch& b)
         : S(size_t(b.surfaces)), SL(size_t(b.surfaces) * size_t(b.wavelengths)),
-          F(size_t(b.fields)), refc(b.reference_sphere != MKO_REF_LAST_SURFACE) {}
-    size_t doubles() const { return 2 * S + SL + 1 + 2 * F + (refc ? F : 0); }
+          F(size_t(b.fields)), refc(b.reference_sphere != MKO_REF_LAST_SURFACE),
+          pidx(b.paraxial_index != nullptr) {}
+    size_t doubles() const { return 2 * S + SL + 1 + 2 * F + (refc ? F : 0) + (pidx ? S : 0); }
 };
 
 void check_batch(const mko_batch& b) {
@@ -143,6 +144,120 @@ void validate_values(const mko_batch& b, int first, int count) {
     }
 }
 
+// ---- constraint rows (ExtraResiduals / TraceExtraResiduals) -----------------------------------
+
+inline bool con_paraxial(int type) { return type >= MKO_CON_FOCAL_LENGTH && type <= MKO_CON_PETZVAL; }
+inline bool con_mechanical(int type) {
+    return type >= MKO_CON_MAX_SEMI_DIAMETER && type <= MKO_CON_MAX_INCIDENCE_ANGLE;
+}
+inline bool con_gap(int type) {
+    return type == MKO_CON_MIN_CENTER_THICKNESS || type == MKO_CON_MAX_CENTER_THICKNESS ||
+           type == MKO_CON_MIN_EDGE_THICKNESS;
+}
+inline bool con_needs_extrema(const mko_constraint& c) {
+    return c.type == MKO_CON_MAX_SEMI_DIAMETER || c.type == MKO_CON_MAX_INCIDENCE_ANGLE ||
+           (c.type == MKO_CON_MIN_EDGE_THICKNESS && c.edge_height == MKO_EDGE_RAYS);
+}
+inline bool con_needs_paraxial(const mko_constraint& c) {
+    return c.type == MKO_CON_FOCAL_LENGTH || c.type == MKO_CON_PARAXIAL_FOCUS ||
+           c.type == MKO_CON_PETZVAL;
+}
+
+// check_paraxial_constraints (src/optim/paraxial_residuals.cpp:152-255) and
+// check_mechanical_constraints (src/optim/mechanical_constraints.cpp:148-203) for one item.
+void check_constraint(const mko_constraint& c, int k, const mko_batch& b) {
+    const auto fail = [k](const std::string& what) {
+        throw std::invalid_argument("constraint " + std::to_string(k) + ": " + what);
+    };
+    const int S = b.surfaces;
+    if (!(std::isfinite(c.weight) && c.weight >= 0.0)) fail("weight must be finite and >= 0");
+    if (!(std::isfinite(c.shift) && c.shift >= 0.0)) fail("shift must be finite and >= 0");
+    if (con_paraxial(c.type)) {
+        if (c.mode < MKO_MODE_EQUAL || c.mode > MKO_MODE_MAX) fail("unknown mode");
+        if (!std::isfinite(c.target)) fail("target must be finite");
+        if (c.shift != 0.0 && c.mode == MKO_MODE_EQUAL) fail("shift only for one-sided modes");
+        switch (c.type) {
+        case MKO_CON_FOCAL_LENGTH:
+            if (c.target == 0.0) fail("target focal length must not be 0");
+            if (c.mode != MKO_MODE_EQUAL) fail("defined as an equation only");
+            break;
+        case MKO_CON_PARAXIAL_FOCUS:
+            if (c.mode != MKO_MODE_EQUAL) fail("defined as an equation only");
+            if (b.source != MKO_SOURCE_COLLIMATED) fail("needs a collimated source");
+            break;
+        case MKO_CON_PARFOCAL_LENGTH:
+            if (c.surface < 0 || c.surface >= S) fail("reference surface out of range");
+            if (!std::isfinite(c.offset)) fail("offset must be finite");
+            break;
+        case MKO_CON_PETZVAL:
+            if (c.mode != MKO_MODE_MAX) fail("defined as an upper bound only");
+            if (c.target < 0.0) fail("kappa_max must be >= 0");
+            break;
+        default:
+            break;
+        }
+        if (con_needs_paraxial(c) && !b.paraxial_index) fail("batch has no paraxial_index");
+        return;
+    }
+    if (!con_mechanical(c.type)) fail("unsupported constraint type");
+    if (c.surface < 0 || c.surface >= S) fail("surface out of range");
+    if (con_gap(c.type)) {
+        if (c.next_surface < 0 || c.next_surface >= S) fail("next surface out of range");
+        if (c.next_surface <= c.surface) fail("next surface not behind the surface");
+    }
+    if (!std::isfinite(c.target)) fail("limit must be finite");
+    if (c.type == MKO_CON_MAX_SEMI_DIAMETER && !(c.target > 0.0)) fail("semi-diameter must be > 0");
+    if (c.type == MKO_CON_MAX_INCIDENCE_ANGLE && !(c.target > 0.0 && c.target < 1.0))
+        fail("cos theta_max must be in (0, 1)");
+    if (!(std::isfinite(c.offset) && c.offset >= 0.0)) fail("clearance must be finite and >= 0");
+    if (c.type == MKO_CON_MIN_EDGE_THICKNESS) {
+        if (c.edge_height != MKO_EDGE_SEMI_DIAMETER && c.edge_height != MKO_EDGE_RAYS)
+            fail("unknown edge height");
+        if (c.edge_height == MKO_EDGE_SEMI_DIAMETER &&
+            !(std::isfinite(c.semi_diameter) && c.semi_diameter > 0.0 &&
+              std::isfinite(c.next_semi_diameter) && c.next_semi_diameter > 0.0))
+            fail("semi-diameters must be finite and > 0");
+    }
+}
+
+// The constraint rows of a call: items in row order (without extrema, then with), on the host
+// and (after upload) on the device.
+struct RowPlan {
+    std::vector<mko_constraint> items;
+    int E = 0, Et = 0;
+    bool paraxial = false;
+    const mko_constraint* dev = nullptr;
+    int count() const { return E + Et; }
+};
+
+RowPlan plan_rows(const mko_batch& b, const mko_residual_settings& s) {
+    RowPlan r;
+    if (!s.extra || s.extra->count <= 0) return r;
+    if (!s.extra->items) throw std::invalid_argument("constraint rows without items");
+    for (int k = 0; k < s.extra->count; ++k) check_constraint(s.extra->items[k], k, b);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int k = 0; k < s.extra->count; ++k)
+            if (con_needs_extrema(s.extra->items[k]) == (pass == 1))
+                r.items.push_back(s.extra->items[k]);
+    for (const mko_constraint& c : r.items) {
+        (con_needs_extrema(c) ? r.Et : r.E) += 1;
+        r.paraxial = r.paraxial || con_needs_paraxial(c);
+    }
+    return r;
+}
+
+// compute_paraxial's checks (src/paraxial/paraxial.cpp:104-129) on the paraxial index of a
+// host batch that has paraxial rows.
+void validate_paraxial(const mko_batch& b, int first, int count) {
+    for (int v = first; v < first + count; ++v)
+        for (int s = 0; s < b.surfaces; ++s) {
+            const double n = b.paraxial_index[size_t(v) * b.surfaces + s];
+            if (!(n > 0.0) || !std::isfinite(n))
+                throw std::invalid_argument("variant " + std::to_string(v) +
+                                            ": paraxial index not finite and > 0");
+        }
+}
+
 // Device copy of (a range of) a host batch in one allocation.
 struct DeviceBatch {
     Buffer mem;
@@ -172,6 +287,7 @@ struct DeviceBatch {
         put(h.field_angle, st.F, &view.field_angle);
         put(h.field_target, st.F, &view.field_target);
         put(st.refc ? h.reference_curvature : nullptr, st.F, &view.reference_curvature);
+        put(h.paraxial_index, st.S, &view.paraxial_index);
     }
     // Uninitialised device batch of `count` variants shaped like h (filled by a kernel).
     void allocate(const mko_batch& h, int count) {
@@ -195,13 +311,14 @@ struct DeviceBatch {
         take(st.F, &view.field_angle);
         take(st.F, &view.field_target);
         take(st.F, &view.reference_curvature, st.refc);
+        take(st.S, &view.paraxial_index, st.pidx);
     }
     double* mut(const double* p) { return const_cast<double*>(p); }
 };
 
 // Host batch made of selected variants of another batch (run templates of an LM subset).
 struct GatheredBatch {
-    std::vector<double> curv, thick, index, pupil, fa, ft, rc;
+    std::vector<double> curv, thick, index, pupil, fa, ft, rc, pi;
     mko_batch view{};
     const mko_batch& gather(const mko_batch& base, const int* ids, size_t n) {
         const Strides st(base);
@@ -212,6 +329,7 @@ struct GatheredBatch {
         fa.resize(n * st.F);
         ft.resize(n * st.F);
         rc.resize(st.refc ? n * st.F : 0);
+        pi.resize(st.pidx ? n * st.S : 0);
         for (size_t j = 0; j < n; ++j) {
             const int id = ids[j];
             if (id < 0 || id >= base.variants) throw std::out_of_range("run index outside the base batch");
@@ -223,6 +341,7 @@ struct GatheredBatch {
             std::copy_n(base.field_angle + v * st.F, st.F, fa.data() + j * st.F);
             std::copy_n(base.field_target + v * st.F, st.F, ft.data() + j * st.F);
             if (st.refc) std::copy_n(base.reference_curvature + v * st.F, st.F, rc.data() + j * st.F);
+            if (st.pidx) std::copy_n(base.paraxial_index + v * st.S, st.S, pi.data() + j * st.S);
         }
         view = base;
         view.variants = int(n);
@@ -233,6 +352,7 @@ struct GatheredBatch {
         view.field_angle = fa.data();
         view.field_target = ft.data();
         view.reference_curvature = st.refc ? rc.data() : nullptr;
+        view.paraxial_index = st.pidx ? pi.data() : nullptr;
         return view;
     }
 };
@@ -381,8 +501,13 @@ public:
         DeviceGuard g(device_);
         check_batch(b);
         validate_values(b, 0, b.variants);
-        const int64_t R = trace_rows_of(b, s), rays = rays_of(b, s);
-        const size_t per = Strides(b).doubles() * 8 + size_t(R) * 8 + size_t(rays) * 9;
+        RowPlan rp = plan_rows(b, s);
+        if (rp.paraxial) validate_paraxial(b, 0, b.variants);
+        upload_rows(rp);
+        const int64_t T = trace_rows_of(b, s), R = T + rp.count(), rays = rays_of(b, s);
+        const size_t tuples = size_t(b.fields) * b.wavelengths, S = size_t(b.surfaces);
+        const size_t per = Strides(b).doubles() * 8 + size_t(R) * 8 + size_t(rays) * 9 +
+                           (rp.Et ? (tuples + 1) * S * 16 : 0);
         const int tranche = tranche_size(b.variants, per);
         for (int first = 0; first < b.variants; first += tranche) {
             const int count = std::min(tranche, b.variants - first);
@@ -390,9 +515,15 @@ public:
             double* dres = out_res_.as<double>(size_t(count) * R);
             double* dopd = opd ? out_opd_.as<double>(size_t(count) * rays) : nullptr;
             uint8_t* dst = status ? out_st_.as<uint8_t>(size_t(count) * rays) : nullptr;
+            double* ep = rp.Et ? ext_part_.as<double>(size_t(count) * tuples * S * 2) : nullptr;
+            double* ex = rp.Et ? ext_.as<double>(size_t(count) * S * 2) : nullptr;
             timer.run(kTrace, [&] {
-                dev_check(mko_residuals_device(&batch_.view, &s, dres, R, dopd, dst, nullptr));
+                dev_check(mko_internal::trace_extrema(&batch_.view, &s, 0, dres, R, dopd, dst,
+                                                      nullptr, nullptr, nullptr, ep, ex, nullptr));
             });
+            if (rp.count())
+                dev_check(mko_internal::constraint_rows(&batch_.view, rp.dev, rp.count(),
+                                                        rp.paraxial, ex, dres, R, T, nullptr));
             if (res) copy_out(res + size_t(first) * R, dres, size_t(count) * R * 8);
             if (opd) copy_out(opd + size_t(first) * rays, dopd, size_t(count) * rays * 8);
             if (status) copy_out(status + size_t(first) * rays, dst, size_t(count) * rays);
@@ -400,18 +531,45 @@ public:
         hip_check(hipDeviceSynchronize(), "residuals");
     }
 
+    // Surface extrema [V][S][2] of a host batch (the MERIT trace's max reductions).
+    void surface_extrema(const mko_batch& b, const mko_residual_settings& s, double* out) {
+        DeviceGuard g(device_);
+        check_batch(b);
+        validate_values(b, 0, b.variants);
+        const size_t T = size_t(b.fields) * b.wavelengths, S = size_t(b.surfaces);
+        const size_t per = Strides(b).doubles() * 8 + T * 16 + (T + 1) * S * 16;
+        const int tranche = tranche_size(b.variants, per);
+        for (int first = 0; first < b.variants; first += tranche) {
+            const int count = std::min(tranche, b.variants - first);
+            batch_.upload(b, first, count, nullptr);
+            double* part = part_.as<double>(size_t(count) * T);
+            int* pc = partc_.as<int>(size_t(count) * T * 2);
+            double* ep = ext_part_.as<double>(size_t(count) * T * S * 2);
+            double* ex = ext_.as<double>(size_t(count) * S * 2);
+            timer.run(kTrace, [&] {
+                dev_check(mko_internal::trace_extrema(&batch_.view, &s, 1, nullptr, 0, nullptr,
+                                                      nullptr, part, pc, nullptr, ep, ex, nullptr));
+            });
+            copy_out(out + size_t(first) * S * 2, ex, size_t(count) * S * 2 * 8);
+        }
+        hip_check(hipDeviceSynchronize(), "surface extrema");
+    }
+
     void merit(const mko_batch& b, const mko_residual_settings& s, const double* extra,
                double* merit, int64_t* failed, int64_t* hard) {
         DeviceGuard g(device_);
         check_batch(b);
         validate_values(b, 0, b.variants);
+        RowPlan rp = plan_rows(b, s);
+        if (rp.paraxial) validate_paraxial(b, 0, b.variants);
+        upload_rows(rp);
         const size_t T = size_t(b.fields) * b.wavelengths;
-        const size_t per = Strides(b).doubles() * 8 + T * 16 + 32;
+        const size_t per = Strides(b).doubles() * 8 + T * 16 + 32 + rows_bytes(rp, b);
         const int tranche = tranche_size(b.variants, per);
         for (int first = 0; first < b.variants; first += tranche) {
             const int count = std::min(tranche, b.variants - first);
             batch_.upload(b, first, count, nullptr);
-            merit_device(batch_.view, s, extra ? extra + first : nullptr, count);
+            merit_device(batch_.view, s, rp, extra ? extra + first : nullptr, count);
             copy_out(merit + first, d_merit_, size_t(count) * 8);
             copy_out(failed + first, d_failed_, size_t(count) * 8);
             copy_out(hard + first, d_hard_, size_t(count) * 8);
@@ -419,27 +577,51 @@ public:
         hip_check(hipDeviceSynchronize(), "merit");
     }
 
-    // merit of a device-resident batch into d_merit_/d_failed_/d_hard_
-    void merit_device(const mko_batch& dev, const mko_residual_settings& s, const double* extra_host,
-                      int count) {
-        const size_t T = size_t(dev.fields) * dev.wavelengths;
+    // MERIT of a device-resident batch: (ray merit + sum e^2 [+ extra]) + sum t^2 into md / fd /
+    // hd (NULL: d_merit_ / d_failed_ / d_hard_); states: the LM's final ray states.
+    void merit_device(const mko_batch& dev, const mko_residual_settings& s, const RowPlan& rp,
+                      const double* extra_host, int count,
+                      const mko_internal::RayStates* states = nullptr, double* md = nullptr,
+                      int64_t* fd = nullptr, int64_t* hd = nullptr) {
+        const size_t T = size_t(dev.fields) * dev.wavelengths, S = size_t(dev.surfaces);
         double* part = part_.as<double>(size_t(count) * T);
         int* pc = partc_.as<int>(size_t(count) * T * 2);
-        d_merit_ = out_merit_.as<double>(size_t(count));
-        d_failed_ = out_failed_.as<int64_t>(size_t(count));
-        d_hard_ = out_hard_.as<int64_t>(size_t(count));
+        if (!md) {
+            md = d_merit_ = out_merit_.as<double>(size_t(count));
+            fd = d_failed_ = out_failed_.as<int64_t>(size_t(count));
+            hd = d_hard_ = out_hard_.as<int64_t>(size_t(count));
+        }
         const double* dextra = nullptr;
         if (extra_host) {
             double* e = extra_.as<double>(size_t(count));
             hip_check(hipMemcpy(e, extra_host, size_t(count) * 8, hipMemcpyHostToDevice), "extra");
             dextra = e;
         }
+        double* ep = rp.Et ? ext_part_.as<double>(size_t(count) * T * S * 2) : nullptr;
+        double* ex = rp.Et ? ext_.as<double>(size_t(count) * S * 2) : nullptr;
         timer.run(kTrace, [&] {
-            dev_check(mko_merit_device(&dev, &s, dextra, d_merit_, d_failed_, d_hard_, part, pc, nullptr));
+            dev_check(mko_internal::trace_extrema(&dev, &s, 1, nullptr, 0, nullptr, nullptr, part,
+                                                  pc, states, ep, ex, nullptr));
+            const double* esum = dextra;
+            const double* tsum = nullptr;
+            if (rp.count()) {
+                double* rows = rows_.as<double>(size_t(count) * rp.count());
+                dev_check(mko_internal::constraint_rows(&dev, rp.dev, rp.count(), rp.paraxial, ex,
+                                                        rows, rp.count(), 0, nullptr));
+                double* e = (rp.E || dextra) ? esum_.as<double>(size_t(count)) : nullptr;
+                double* t = rp.Et ? tsum_.as<double>(size_t(count)) : nullptr;
+                dev_check(mko_internal::rows_merit(count, rows, rp.count(), 0, rp.E, rp.Et, dextra,
+                                                   e, t, nullptr));
+                esum = e;
+                tsum = t;
+            }
+            dev_check(mko_internal::merit_finalize(count, int(T), part, pc, esum, tsum, md, fd, hd,
+                                                   nullptr));
         });
     }
 
-    // J / r0 / A / g for `runs` points; base has 1 variant or one per run.
+    // J / r0 / A / g for `runs` points; base has 1 variant or one per run. Rows per run:
+    // trace rows + constraint rows.
     void jacobian(const mko_batch& base, const mko_residual_settings& s, const mko_variables& vars,
                   int runs, const double* x0, int method, int precision, double* J, double* r0,
                   double* a, double* g) {
@@ -453,26 +635,31 @@ public:
         if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED)
             throw std::invalid_argument("unknown precision");
         validate_values(base, 0, base.variants);
+        RowPlan rp = plan_rows(base, s);
+        if (rp.paraxial) validate_paraxial(base, 0, base.variants);
+        upload_rows(rp);
         const int P = vars.count;
-        const int64_t R = trace_rows_of(base, s), rays = rays_of(base, s);
+        const int64_t T = trace_rows_of(base, s), R = T + rp.count(), rays = rays_of(base, s);
         const bool fd = method == MKO_JACOBIAN_FD;
         // The LM consumes only J^T J and J^T r0: for the mixed-precision adjoint those come from
         // the fused kernel and J is never formed (hard failure mode, <= 64 variables, <= 512
         // rays per tuple).
         const bool fused = !fd && precision == MKO_PRECISION_MIXED && !J && !r0 && (a || g) &&
                            fused_ok(s, P);
+        const bool pattern = fd || rp.count() > 0;
         const int per_run = fd ? 2 * P + 1 : 1;
-        const size_t jbytes = 8;
-        const size_t T = size_t(base.fields) * base.wavelengths;
+        const size_t Tt = size_t(base.fields) * base.wavelengths, S = size_t(base.surfaces);
         const size_t per = size_t(per_run) * (Strides(base).doubles() * 8 +
-                                              (fd ? size_t(R) * 8 + size_t(rays) : 0)) +
-                           (fused ? T * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
-                                  : size_t(P) * R * jbytes + size_t(R) * 8 + size_t(rays)) +
-                           size_t(P) * (P + 1) * 8 + 64 * size_t(P);
+                                              (fd ? size_t(R) * 8 + size_t(rays) : 0) +
+                                              (fd && rp.Et ? (Tt + 1) * S * 16 : 0)) +
+                           (fused ? Tt * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
+                                  : size_t(P) * R * 8 + size_t(R) * 8 + size_t(rays)) +
+                           size_t(P) * (P + 1) * 8 + 64 * size_t(P) +
+                           (!fd ? rows_fd_bytes(rp, base, P) : 0);
         const int chunk = tranche_size(runs, per);
         upload_vars(vars);
         Pattern pat;
-        if (fd) {
+        if (pattern) {
             pat.plus.resize(size_t(runs) * P);
             pat.minus.resize(size_t(runs) * P);
             pat.step_plus.resize(size_t(runs) * P);
@@ -490,22 +677,29 @@ public:
             double* dx = x_.as<double>(size_t(count) * P);
             hip_check(hipMemcpy(dx, x0 + size_t(first) * P, size_t(count) * P * 8,
                                 hipMemcpyHostToDevice), "x0");
-            double *dplus = nullptr, *dminus = nullptr;
-            if (fd) {
-                dplus = plus_.as<double>(size_t(count) * P);
-                dminus = minus_.as<double>(size_t(count) * P);
-                hip_check(hipMemcpy(dplus, pat.plus.data() + size_t(first) * P, size_t(count) * P * 8,
-                                    hipMemcpyHostToDevice), "pattern");
-                hip_check(hipMemcpy(dminus, pat.minus.data() + size_t(first) * P, size_t(count) * P * 8,
-                                    hipMemcpyHostToDevice), "pattern");
+            double *dplus = nullptr, *dminus = nullptr, *dsp = nullptr, *dsm = nullptr;
+            uint8_t* dsten = nullptr;
+            if (pattern) {
+                const size_t o = size_t(first) * P, n = size_t(count) * P;
+                dplus = plus_.as<double>(n);
+                dminus = minus_.as<double>(n);
+                dsp = sp_.as<double>(n);
+                dsm = sm_.as<double>(n);
+                dsten = sten_.as<uint8_t>(n);
+                hip_check(hipMemcpy(dplus, pat.plus.data() + o, n * 8, hipMemcpyHostToDevice), "pattern");
+                hip_check(hipMemcpy(dminus, pat.minus.data() + o, n * 8, hipMemcpyHostToDevice), "pattern");
+                hip_check(hipMemcpy(dsp, pat.step_plus.data() + o, n * 8, hipMemcpyHostToDevice), "steps");
+                hip_check(hipMemcpy(dsm, pat.step_minus.data() + o, n * 8, hipMemcpyHostToDevice), "steps");
+                hip_check(hipMemcpy(dsten, pat.stencil.data() + o, n, hipMemcpyHostToDevice), "stencil");
             }
             work_.allocate(base, count * per_run);
-            dev_check(mko_point_batch_device(
-                &tmpl_.view, base.variants != 1, count, per_run, &dvars_, dx, dplus, dminus,
-                work_.mut(work_.view.curvature), work_.mut(work_.view.thickness),
-                work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
-                work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
-                work_.mut(work_.view.reference_curvature), nullptr));
+            dev_check(mko_internal::point_batch(
+                &tmpl_.view, nullptr, base.variants != 1, count, per_run, &dvars_, dx,
+                fd ? dplus : nullptr, fd ? dminus : nullptr, work_.mut(work_.view.curvature),
+                work_.mut(work_.view.thickness), work_.mut(work_.view.index),
+                work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
+                work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
+                work_.mut(work_.view.paraxial_index), nullptr));
             if (fused) {
                 double* da = a_.as<double>(size_t(count) * P * P);
                 double* dg = g_.as<double>(size_t(count) * P);
@@ -513,50 +707,42 @@ public:
                     mko_adjoint_normal_workspace(count, base.fields, base.wavelengths, P);
                 void* ws = jac_.reserve(wbytes);
                 timer.run(kAdjoint, [&] {
-                    dev_check(mko_adjoint_normal_device(&work_.view, &s, &vars_host_dev_, da, dg,
-                                                        ws, wbytes, nullptr));
+                    dev_check(mko_internal::adjoint_normal_states(&work_.view, &s, &vars_host_dev_,
+                                                                  da, dg, ws, wbytes,
+                                                                  mko_internal::RayStates{},
+                                                                  nullptr));
                 });
+                if (rp.count()) {
+                    const int E = rp.count();
+                    double* Je = je_.as<double>(size_t(count) * P * E);
+                    double* re = re_.as<double>(size_t(count) * E);
+                    rows_fd(tmpl_.view, nullptr, base.variants != 1, count, dx, dplus, dminus, dsp,
+                            dsm, dsten, s, rp, P, Je, re, E, 0);
+                    dev_check(mko_internal::extra_normal(count, P, E, Je, re, da, dg, nullptr));
+                }
                 if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
                 if (g) copy_out(g + size_t(first) * P, dg, size_t(count) * P * 8);
                 hip_check(hipDeviceSynchronize(), "jacobian");
                 continue;
             }
-            void* dJ = jac_.reserve(size_t(count) * P * R * jbytes);
+            double* dJ = jac_.as<double>(size_t(count) * P * R);
             double* dr0 = r0_.as<double>(size_t(count) * R);
             if (fd) {
-                double* fres = fdres_.as<double>(size_t(count) * per_run * R);
-                uint8_t* fst = fdst_.as<uint8_t>(size_t(count) * per_run * rays);
-                timer.run(kTrace, [&] {
-                    dev_check(mko_residuals_device(&work_.view, &s, fres, R, nullptr, fst, nullptr));
-                });
-                double* dsp = sp_.as<double>(size_t(count) * P);
-                double* dsm = sm_.as<double>(size_t(count) * P);
-                uint8_t* dsten = sten_.as<uint8_t>(size_t(count) * P);
-                int64_t* dch = ch_.as<int64_t>(size_t(count) * P);
-                hip_check(hipMemcpy(dsp, pat.step_plus.data() + size_t(first) * P, size_t(count) * P * 8,
-                                    hipMemcpyHostToDevice), "steps");
-                hip_check(hipMemcpy(dsm, pat.step_minus.data() + size_t(first) * P, size_t(count) * P * 8,
-                                    hipMemcpyHostToDevice), "steps");
-                hip_check(hipMemcpy(dsten, pat.stencil.data() + size_t(first) * P, size_t(count) * P,
-                                    hipMemcpyHostToDevice), "stencil");
-                timer.run(kFdDifference, [&] {
-                    dev_check(mko_fd_difference_device(fres, R, fst, rays, count, P, R, dsp, dsm,
-                                                       dsten, static_cast<double*>(dJ), dr0, dch,
-                                                       nullptr));
-                });
+                fd_rows(work_.view, s, rp, count, P, dsp, dsm, dsten, T, R, rays, dJ, dr0);
             } else {
                 timer.run(kAdjoint, [&] {
-                    dev_check(mko_adjoint_jacobian_device(&work_.view, &s, &vars_host_dev_,
-                                                          precision, static_cast<double*>(dJ), dr0,
-                                                          nullptr, nullptr));
+                    dev_check(mko_internal::adjoint_jacobian(&work_.view, &s, &vars_host_dev_, dJ,
+                                                             dr0, nullptr, R, nullptr));
                 });
+                if (rp.count())
+                    rows_fd(tmpl_.view, nullptr, base.variants != 1, count, dx, dplus, dminus, dsp,
+                            dsm, dsten, s, rp, P, dJ, dr0, R, T);
             }
             if (a || g) {
                 double* da = a_.as<double>(size_t(count) * P * P);
                 double* dg = g_.as<double>(size_t(count) * P);
                 timer.run(kGram, [&] {
-                    dev_check(mko_gram_device(static_cast<const double*>(dJ), dr0, count, P, R, da,
-                                              dg, nullptr));
+                    dev_check(mko_gram_device(dJ, dr0, count, P, R, da, dg, nullptr));
                 });
                 if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
                 if (g) copy_out(g + size_t(first) * P, dg, size_t(count) * P * 8);
@@ -574,8 +760,12 @@ public:
         const int P = vars.count;
         for (size_t i = 0; i < size_t(runs) * P; ++i)
             if (!std::isfinite(x[i])) throw std::invalid_argument("x not finite");
+        RowPlan rp = plan_rows(base, s);
+        if (rp.paraxial) validate_paraxial(base, 0, base.variants);
+        upload_rows(rp);
         const size_t T = size_t(base.fields) * base.wavelengths;
-        const size_t per = Strides(base).doubles() * 8 + T * 16 + 64 + size_t(P) * 8;
+        const size_t per =
+            Strides(base).doubles() * 8 + T * 16 + 64 + size_t(P) * 8 + rows_bytes(rp, base);
         const int chunk = tranche_size(runs, per);
         upload_vars(vars);
         if (base.variants == 1) tmpl_.upload(base, 0, 1, nullptr);
@@ -586,13 +776,14 @@ public:
             hip_check(hipMemcpy(dx, x + size_t(first) * P, size_t(count) * P * 8,
                                 hipMemcpyHostToDevice), "x");
             work_.allocate(base, count);
-            dev_check(mko_point_batch_device(
-                &tmpl_.view, base.variants != 1, count, 1, &dvars_, dx, nullptr, nullptr,
+            dev_check(mko_internal::point_batch(
+                &tmpl_.view, nullptr, base.variants != 1, count, 1, &dvars_, dx, nullptr, nullptr,
                 work_.mut(work_.view.curvature), work_.mut(work_.view.thickness),
                 work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
                 work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
-                work_.mut(work_.view.reference_curvature), nullptr));
-            merit_device(work_.view, s, nullptr, count);
+                work_.mut(work_.view.reference_curvature), work_.mut(work_.view.paraxial_index),
+                nullptr));
+            merit_device(work_.view, s, rp, nullptr, count);
             copy_out(merit + first, d_merit_, size_t(count) * 8);
             copy_out(failed + first, d_failed_, size_t(count) * 8);
             copy_out(hard + first, d_hard_, size_t(count) * 8);
@@ -618,23 +809,27 @@ public:
                       double* ad, double* gd, const mko_internal::RayStates* states = nullptr) {
         DeviceGuard guard(device_);
         if (count <= 0) return;
-        const int64_t R = trace_rows_of(tmpl, s), rays = rays_of(tmpl, s);
+        RowPlan rp = plan_rows(tmpl, s);
+        upload_rows(rp);
+        const int64_t T = trace_rows_of(tmpl, s), R = T + rp.count(), rays = rays_of(tmpl, s);
         const bool fd = method == MKO_JACOBIAN_FD;
         const bool fused = !fd && precision == MKO_PRECISION_MIXED && fused_ok(s, P);
+        const bool pattern = fd || rp.count() > 0;
         const int per_run = fd ? 2 * P + 1 : 1;
-        const size_t T = size_t(tmpl.fields) * tmpl.wavelengths;
+        const size_t Tt = size_t(tmpl.fields) * tmpl.wavelengths, S = size_t(tmpl.surfaces);
         const size_t per = size_t(per_run) * (Strides(tmpl).doubles() * 8 +
-                                              (fd ? size_t(R) * 8 + size_t(rays) : 0)) +
-                           (fused ? T * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
+                                              (fd ? size_t(R) * 8 + size_t(rays) : 0) +
+                                              (fd && rp.Et ? (Tt + 1) * S * 16 : 0)) +
+                           (fused ? Tt * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
                                   : size_t(P) * R * 8 + size_t(R) * 8) +
-                           size_t(P) * 48 + 64;
+                           size_t(P) * 48 + 64 + (!fd ? rows_fd_bytes(rp, tmpl, P) : 0);
         const int chunk = tranche_size(count, per);
         for (int first = 0; first < count; first += chunk) {
             const int c = std::min(chunk, count - first);
             const double* x = xd + size_t(first) * P;
             double *dplus = nullptr, *dminus = nullptr, *dsp = nullptr, *dsm = nullptr;
             uint8_t* dsten = nullptr;
-            if (fd) {
+            if (pattern) {
                 dplus = plus_.as<double>(size_t(c) * P);
                 dminus = minus_.as<double>(size_t(c) * P);
                 dsp = sp_.as<double>(size_t(c) * P);
@@ -646,14 +841,16 @@ public:
                     hip_check(hipGetLastError(), "fd pattern launch");
                 });
             }
+            const int32_t* ti = tidx ? tidx + first : nullptr;
             work_.allocate(tmpl, c * per_run);
             timer.run(kPointBatch, [&] {
                 dev_check(mko_internal::point_batch(
-                    &tmpl, tidx ? tidx + first : nullptr, 0, c, per_run, &dvars_, x, dplus, dminus,
-                    work_.mut(work_.view.curvature), work_.mut(work_.view.thickness),
-                    work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
-                    work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
-                    work_.mut(work_.view.reference_curvature), nullptr));
+                    &tmpl, ti, 0, c, per_run, &dvars_, x, fd ? dplus : nullptr,
+                    fd ? dminus : nullptr, work_.mut(work_.view.curvature),
+                    work_.mut(work_.view.thickness), work_.mut(work_.view.index),
+                    work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
+                    work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
+                    work_.mut(work_.view.paraxial_index), nullptr));
             });
             double* a_out = ad + size_t(first) * P * P;
             double* g_out = gd + size_t(first) * P;
@@ -672,26 +869,28 @@ public:
                                                                   a_out, g_out, ws, wbytes, st,
                                                                   nullptr));
                 });
+                if (rp.count()) {
+                    const int E = rp.count();
+                    double* Je = je_.as<double>(size_t(c) * P * E);
+                    double* re = re_.as<double>(size_t(c) * E);
+                    rows_fd(tmpl, ti, 0, c, x, dplus, dminus, dsp, dsm, dsten, s, rp, P, Je, re, E,
+                            0);
+                    dev_check(mko_internal::extra_normal(c, P, E, Je, re, a_out, g_out, nullptr));
+                }
                 continue;
             }
             double* dJ = jac_.as<double>(size_t(c) * P * R);
             double* dr0 = r0_.as<double>(size_t(c) * R);
             if (fd) {
-                double* fres = fdres_.as<double>(size_t(c) * per_run * R);
-                uint8_t* fst = fdst_.as<uint8_t>(size_t(c) * per_run * rays);
-                timer.run(kTrace, [&] {
-                    dev_check(mko_residuals_device(&work_.view, &s, fres, R, nullptr, fst, nullptr));
-                });
-                int64_t* dch = ch_.as<int64_t>(size_t(c) * P);
-                timer.run(kFdDifference, [&] {
-                    dev_check(mko_fd_difference_device(fres, R, fst, rays, c, P, R, dsp, dsm, dsten,
-                                                       dJ, dr0, dch, nullptr));
-                });
+                fd_rows(work_.view, s, rp, c, P, dsp, dsm, dsten, T, R, rays, dJ, dr0);
             } else {
                 timer.run(kAdjoint, [&] {
-                    dev_check(mko_adjoint_jacobian_device(&work_.view, &s, &vars_host_dev_,
-                                                          precision, dJ, dr0, nullptr, nullptr));
+                    dev_check(mko_internal::adjoint_jacobian(&work_.view, &s, &vars_host_dev_, dJ,
+                                                             dr0, nullptr, R, nullptr));
                 });
+                if (rp.count())
+                    rows_fd(tmpl, ti, 0, c, x, dplus, dminus, dsp, dsm, dsten, s, rp, P, dJ, dr0, R,
+                            T);
             }
             timer.run(kGram, [&] {
                 dev_check(mko_gram_device(dJ, dr0, c, P, R, a_out, g_out, nullptr));
@@ -706,8 +905,11 @@ public:
                    int64_t* hd, const mko_internal::RayStates* states = nullptr) {
         DeviceGuard guard(device_);
         if (count <= 0) return;
+        RowPlan rp = plan_rows(tmpl, s);
+        upload_rows(rp);
         const size_t T = size_t(tmpl.fields) * tmpl.wavelengths;
-        const size_t per = Strides(tmpl).doubles() * 8 + T * 16 + 64 + size_t(P) * 8;
+        const size_t per =
+            Strides(tmpl).doubles() * 8 + T * 16 + 64 + size_t(P) * 8 + rows_bytes(rp, tmpl);
         const int chunk = tranche_size(count, per);
         for (int first = 0; first < count; first += chunk) {
             const int c = std::min(chunk, count - first);
@@ -719,22 +921,16 @@ public:
                     work_.mut(work_.view.thickness), work_.mut(work_.view.index),
                     work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
                     work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
-                    nullptr));
+                    work_.mut(work_.view.paraxial_index), nullptr));
             });
-            double* part = part_.as<double>(size_t(c) * T);
-            int* pc = partc_.as<int>(size_t(c) * T * 2);
-            timer.run(kTrace, [&] {
-                if (states) {
-                    mko_internal::RayStates st = *states;
-                    if (st.slot) st.slot += first;
-                    else st.slot_offset += first;
-                    dev_check(mko_internal::merit_states(&work_.view, &s, md + first, fd + first,
-                                                         hd + first, part, pc, st, nullptr));
-                } else {
-                    dev_check(mko_merit_device(&work_.view, &s, nullptr, md + first, fd + first,
-                                               hd + first, part, pc, nullptr));
-                }
-            });
+            mko_internal::RayStates st{};
+            if (states) {
+                st = *states;
+                if (st.slot) st.slot += first;
+                else st.slot_offset += first;
+            }
+            merit_device(work_.view, s, rp, nullptr, c, states ? &st : nullptr, md + first,
+                         fd + first, hd + first);
         }
     }
 
@@ -747,6 +943,103 @@ public:
     KernelTimer timer;
 
 private:
+    // Device copy of the plan's items (kept while the items do not change).
+    void upload_rows(RowPlan& rp) {
+        if (rp.count() == 0) return;
+        const size_t bytes = rp.items.size() * sizeof(mko_constraint);
+        const char* src = reinterpret_cast<const char*>(rp.items.data());
+        void* d = items_.reserve(bytes);
+        if (items_host_.size() != bytes || std::memcmp(items_host_.data(), src, bytes) != 0) {
+            hip_check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "constraint items");
+            items_host_.assign(src, src + bytes);
+        }
+        rp.dev = static_cast<const mko_constraint*>(d);
+    }
+    // device bytes per variant of a MERIT / RESIDUALS call for its constraint rows
+    static size_t rows_bytes(const RowPlan& rp, const mko_batch& b) {
+        if (rp.count() == 0) return 0;
+        const size_t T = size_t(b.fields) * b.wavelengths, S = size_t(b.surfaces);
+        return size_t(rp.count()) * 8 + 16 + (rp.Et ? (T + 1) * S * 16 : 0);
+    }
+    // device bytes per run of rows_fd
+    static size_t rows_fd_bytes(const RowPlan& rp, const mko_batch& b, int P) {
+        if (rp.count() == 0) return 0;
+        const size_t T = size_t(b.fields) * b.wavelengths, S = size_t(b.surfaces);
+        return size_t(2 * P + 1) * (Strides(b).doubles() * 8 + size_t(rp.count()) * 8 +
+                                    (rp.Et ? T * 16 + (T + 1) * S * 16 : 0)) +
+               size_t(P + 1) * rp.count() * 8;
+    }
+
+    // FD Jacobian of the central FD batch fdb (count runs x (2P + 1) variants): RESIDUALS of
+    // every variant with its constraint rows (traced rows from the variant's own extrema) at
+    // stride R, then fd_derivative of all R rows (fd_jacobian_cpu, src/optim/fd_jacobian.cpp:182).
+    void fd_rows(const mko_batch& fdb, const mko_residual_settings& s, const RowPlan& rp,
+                 int count, int P, const double* dsp, const double* dsm, const uint8_t* dsten,
+                 int64_t T, int64_t R, int64_t rays, double* dJ, double* dr0) {
+        const size_t n = size_t(count) * (2 * P + 1);
+        const size_t Tt = size_t(fdb.fields) * fdb.wavelengths, S = size_t(fdb.surfaces);
+        double* fres = fdres_.as<double>(n * R);
+        uint8_t* fst = fdst_.as<uint8_t>(n * rays);
+        double* ep = rp.Et ? ext_part_.as<double>(n * Tt * S * 2) : nullptr;
+        double* ex = rp.Et ? ext_.as<double>(n * S * 2) : nullptr;
+        timer.run(kTrace, [&] {
+            dev_check(mko_internal::trace_extrema(&fdb, &s, 0, fres, R, nullptr, fst, nullptr,
+                                                  nullptr, nullptr, ep, ex, nullptr));
+            if (rp.count())
+                dev_check(mko_internal::constraint_rows(&fdb, rp.dev, rp.count(), rp.paraxial, ex,
+                                                        fres, R, T, nullptr));
+        });
+        int64_t* dch = ch_.as<int64_t>(size_t(count) * P);
+        timer.run(kFdDifference, [&] {
+            dev_check(mko_fd_difference_device(fres, R, fst, rays, count, P, R, dsp, dsm, dsten,
+                                               dJ, dr0, dch, nullptr));
+        });
+    }
+
+    // Jacobian of the constraint rows by central differences for the adjoint paths
+    // (extra_rows_untraced / extra_rows_cpu, src/optim/adjoint_jacobian.cpp:245-337): the FD
+    // batch of the runs' points (2P + 1 variants per run), the rows of every variant (traced rows
+    // from each variant's own MERIT extrema), fd_derivative per row into
+    // J[(i * P + p) * jstride + joffset + e] and r0[i * jstride + joffset + e].
+    void rows_fd(const mko_batch& tmpl, const int32_t* tidx, int template_per_run, int count,
+                 const double* x, const double* plus, const double* minus, const double* sp,
+                 const double* sm, const uint8_t* sten, const mko_residual_settings& s,
+                 const RowPlan& rp, int P, double* J, double* r0, int64_t jstride,
+                 int64_t joffset) {
+        const int per_run = 2 * P + 1;
+        const size_t n = size_t(count) * per_run;
+        const int E = rp.count();
+        rowfd_.allocate(tmpl, int(n));
+        timer.run(kPointBatch, [&] {
+            dev_check(mko_internal::point_batch(
+                &tmpl, tidx, template_per_run, count, per_run, &dvars_, x, plus, minus,
+                rowfd_.mut(rowfd_.view.curvature), rowfd_.mut(rowfd_.view.thickness),
+                rowfd_.mut(rowfd_.view.index), rowfd_.mut(rowfd_.view.pupil_radius),
+                rowfd_.mut(rowfd_.view.field_angle), rowfd_.mut(rowfd_.view.field_target),
+                rowfd_.mut(rowfd_.view.reference_curvature),
+                rowfd_.mut(rowfd_.view.paraxial_index), nullptr));
+        });
+        double* ex = nullptr;
+        if (rp.Et) {
+            const size_t Tt = size_t(tmpl.fields) * tmpl.wavelengths, S = size_t(tmpl.surfaces);
+            double* part = part_.as<double>(n * Tt);
+            int* pc = partc_.as<int>(n * Tt * 2);
+            double* ep = ext_part_.as<double>(n * Tt * S * 2);
+            ex = ext_.as<double>(n * S * 2);
+            timer.run(kTrace, [&] {
+                dev_check(mko_internal::trace_extrema(&rowfd_.view, &s, 1, nullptr, 0, nullptr,
+                                                      nullptr, part, pc, nullptr, ep, ex, nullptr));
+            });
+        }
+        double* rows = rows_.as<double>(n * E);
+        dev_check(mko_internal::constraint_rows(&rowfd_.view, rp.dev, E, rp.paraxial, ex, rows, E,
+                                                0, nullptr));
+        timer.run(kFdDifference, [&] {
+            dev_check(mko_internal::fd_difference(rows, E, nullptr, 0, count, P, E, sp, sm, sten, J,
+                                                  r0, nullptr, jstride, joffset, nullptr));
+        });
+    }
+
     int tranche_size(int items, size_t per_item) const {
         if (items <= 0) return 1;
         const size_t b = budget();
@@ -770,8 +1063,10 @@ private:
 
     int device_;
     size_t limit_;
-    DeviceBatch batch_, tmpl_, work_;
+    DeviceBatch batch_, tmpl_, work_, rowfd_;
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
+    Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
+    std::vector<char> items_host_;
     Buffer x_, plus_, minus_, jac_, r0_, fdres_, fdst_, sp_, sm_, sten_, ch_, a_, g_, vtype_, vsurf_;
     double* d_merit_ = nullptr;
     int64_t* d_failed_ = nullptr;
@@ -964,6 +1259,16 @@ public:
             const size_t k = i % P_;
             if (!std::isfinite(x0[i]) || x0[i] < v.lower[k] || x0[i] > v.upper[k])
                 throw std::invalid_argument("start point not finite or outside the bounds");
+        }
+        // constraint rows: checked here, items owned by the session
+        const RowPlan rp = plan_rows(base, s);
+        if (rp.paraxial) validate_paraxial(base, 0, base.variants);
+        if (s.extra && s.extra->count > 0) {
+            items_own_.assign(s.extra->items, s.extra->items + s.extra->count);
+            extra_own_ = mko_extra_rows{int32_t(items_own_.size()), items_own_.data()};
+            settings_.extra = &extra_own_;
+        } else {
+            settings_.extra = nullptr;
         }
         bounds_ = Bounds{v.lower, v.upper, v.typical};
         x0_.assign(x0, x0 + size_t(runs) * P_);
@@ -1461,6 +1766,8 @@ private:
     Engine& eng_;
     mko_batch base_;
     mko_residual_settings settings_;
+    std::vector<mko_constraint> items_own_;
+    mko_extra_rows extra_own_{};
     mko_variables vars_;
     int runs_;
     mko_lm_options o_;
@@ -1545,6 +1852,11 @@ int mko_engine_kernel_timing(mko_engine* e, int enable, double* ms, int64_t* lau
 int mko_engine_residuals(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                          double* res, double* opd, uint8_t* st) {
     MKO_HOST_TRY(e->engine.residuals(*b, *s, res, opd, st); return 0;)
+}
+
+int mko_engine_surface_extrema(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
+                               double* extrema) {
+    MKO_HOST_TRY(e->engine.surface_extrema(*b, *s, extrema); return 0;)
 }
 
 int mko_engine_merit(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,

@@ -37,6 +37,49 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                 int runs, int per_run, const mko_variables* vars_device, const double* x0,
                 const double* plus, const double* minus, double* curvature, double* thickness,
                 double* index, double* pupil_radius, double* field_angle, double* field_target,
-                double* reference_curvature, void* stream);
+                double* reference_curvature, double* paraxial_index, void* stream);
+
+// RESIDUALS (mode 0: rows at `stride` >= trace rows) or MERIT partials (mode 1: partial [V*T],
+// partial_counts [2*V*T], finalized by merit_finalize) of a device batch, optionally with the
+// LM's final ray states (mode 1) and the surface extrema: ext_partial [V*T][S][2] workspace,
+// extrema [V][S][2] = {largest ray height, smallest cos I} over the rays with status Ok
+// (surface_extrema_cpu, src/batch/batch_extrema.cpp:75). ext_partial == NULL: no extrema.
+int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
+                  double* residuals, int64_t stride, double* opd, uint8_t* status,
+                  double* partial, int* partial_counts, const RayStates* states,
+                  double* ext_partial, double* extrema, void* stream);
+
+// merit[v] = (sum of the tuple partials + extra[v]) + traced[v] (extra / traced may be NULL).
+int merit_finalize(int V, int T, const double* partial, const int* partial_counts,
+                   const double* extra, const double* traced, double* merit, int64_t* failed,
+                   int64_t* hard_failed, void* stream);
+
+// Constraint rows of a device batch: rows[v * stride + offset + k] for the `count` items
+// (device memory, row order) with the extrema [V][S][2] of the traced items.
+int constraint_rows(const mko_batch* batch, const mko_constraint* items, int count,
+                    int need_paraxial, const double* extrema, double* rows, int64_t stride,
+                    int64_t offset, void* stream);
+
+// e_sum[v] = (extra ? extra[v] + : ) sum of the first E rows squared, t_sum[v] = sum of the
+// next Et rows squared (either output may be NULL).
+int rows_merit(int V, const double* rows, int64_t stride, int64_t offset, int E, int Et,
+               const double* extra, double* e_sum, double* t_sum, void* stream);
+
+// a [runs][P][P] += Je^T Je, g [runs][P] += Je^T re for the constraint rows' Jacobian
+// Je [runs][P][E] and values re [runs][E] (fused path).
+int extra_normal(int runs, int P, int E, const double* Je, const double* re, double* a, double* g,
+                 void* stream);
+
+// mko_adjoint_jacobian_device (FP64) with J / r0 rows at `stride` (0: trace rows).
+int adjoint_jacobian(const mko_batch* batch, const mko_residual_settings* settings,
+                     const mko_variables* vars, double* jacobian, double* r0, uint8_t* status,
+                     int64_t stride, void* stream);
+
+// mko_fd_difference_device with J column (i, p) at jacobian + (i * P + p) * jstride + joffset,
+// r0 of run i at r0 + i * jstride + joffset; status / status_changes may be NULL.
+int fd_difference(const double* rows, int64_t stride, const uint8_t* status, int64_t rays,
+                  int runs, int variables, int64_t count, const double* step_plus,
+                  const double* step_minus, const uint8_t* stencil, double* jacobian, double* r0,
+                  int64_t* status_changes, int64_t jstride, int64_t joffset, void* stream);
 
 } // namespace mko_internal

@@ -29,6 +29,7 @@
 #include "gauss_legendre_table.h"
 #include "internal.hpp"
 #include "mikroptim_hip.h"
+#include "constraints.hpp"
 #include "trace_device.hpp"
 
 namespace {
@@ -265,6 +266,82 @@ __device__ __forceinline__ mkod::RayT<R> ray_as(const Ray& r) {
                          V3<R>{num<R>(r.d.x), num<R>(r.d.y), num<R>(r.d.z)}, num<R>(r.opl)};
 }
 
+// ---- surface extrema (SurfaceExtrema, include/mikroptim/trace/surface_extrema.hpp:47-110) -----
+
+// height^2 = x^2 + y^2 at the intersection and cos I = d_before . N, N = (-c x, -c y, 1 - c z),
+// each product and sum rounded on its own (RaySurfaceRecord::surface).
+__device__ __forceinline__ void surface_extremum(double x, double y, double z, double c, double dx,
+                                                 double dy, double dz, double& h, double& ci) {
+#pragma clang fp contract(off)
+    h = x * x + y * y;
+    ci = dx * (-c * x) + dy * (-c * y) + dz * (1.0 - c * z);
+}
+
+// trace_ray with the surface extrema of the wave's rays: at every surface s (0 = start state),
+// the largest height^2 and the smallest cos I over the wave's lanes whose ray is `alive` (has
+// passed every surface so far) are reduced across the wave; lane s keeps surface s's values in
+// eh / ec (S <= 64 = wave width), replacing them (merge = false) or folding them in (merge =
+// true). Every lane of the wave calls this with the same P.S (uniform control flow around the
+// cross-lane reductions); lanes without a ray pass alive = false.
+template <bool kSoft, bool kFast, class R, class Rec>
+__device__ __forceinline__ uint8_t trace_ray_ext(const TraceParams& P, const SurfLds& L,
+                                                 mkod::RayT<R>& r, V3<double> M, double kappa,
+                                                 Soft& soft, bool alive, int lane, double& eh,
+                                                 double& ec, bool merge, Rec& rec) {
+    using mkod::num;
+    using mkod::to_double;
+    auto reduce = [&](int s, double h, double c) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double oh = __shfl_xor(h, off, 64);
+            const double oc = __shfl_xor(c, off, 64);
+            h = oh > h ? oh : h;
+            c = oc < c ? oc : c;
+        }
+        if (lane == s) {
+            eh = merge ? (h > eh ? h : eh) : h;
+            ec = merge ? (c < ec ? c : ec) : c;
+        }
+    };
+    uint8_t st = MKO_STATUS_OK;
+    {
+        double h = 0.0, c = 1.0;
+        if (alive)
+            surface_extremum(to_double(r.p.x), to_double(r.p.y), to_double(r.p.z), 0.0,
+                             to_double(r.d.x), to_double(r.d.y), to_double(r.d.z), h, c);
+        reduce(0, h, c);
+    }
+    if (alive) rec.put(0, r, mkod::kBranchRegular);
+    for (int s = 1; s < P.S; ++s) {
+        double h = 0.0, c = 1.0;
+        if (alive) {
+            const double dx = to_double(r.d.x), dy = to_double(r.d.y), dz = to_double(r.d.z);
+            uint8_t branch = 0;
+            const uint8_t sst = mkod::surface_step<kSoft, kFast>(
+                r, num<R>(L.c[s]), num<R>(L.t[s - 1]), num<R>(L.n[s - 1]), num<R>(L.mu[s]), soft,
+                branch);
+            if (sst != MKO_STATUS_OK) {
+                alive = false;
+                st = sst;
+            } else {
+                rec.put(s, r, branch);
+                surface_extremum(to_double(r.p.x), to_double(r.p.y), to_double(r.p.z), L.c[s], dx,
+                                 dy, dz, h, c);
+            }
+        }
+        reduce(s, h, c);
+    }
+    if (!alive) return st;
+    if (!(r.d.z > num<R>(0.0))) return MKO_STATUS_BACKWARD;
+    uint8_t rb = 0;
+    const uint8_t rst = mkod::reference_step<kSoft, kFast>(
+        r, V3<R>{num<R>(M.x), num<R>(M.y), num<R>(M.z)}, num<R>(kappa), P.ref_kind,
+        num<R>(L.n[P.S - 1]), soft, rb);
+    if (!mkod::in_piston(rst)) return rst;
+    rec.image_branch = rb;
+    return kSoft ? soft.first : uint8_t(MKO_STATUS_OK);
+}
+
 // ---- deterministic block reductions -----------------------------------------------------------
 
 template <class T>
@@ -298,12 +375,16 @@ __device__ __forceinline__ int64_t state_index(const mko_internal::RayStates& rs
     return (int64_t(c) * rs.slots + slot) * rs.rays + row;
 }
 
-template <int kMode, bool kSoft, int kRpl, class R, bool kStates = false>
+// kExt: also the surface extrema of the tuple's rays with status Ok, ext_partial[block][S][2]
+// (largest height^2, smallest cos I); a wave that holds a ray with another status traces its
+// valid rays a second time for them.
+template <int kMode, bool kSoft, int kRpl, class R, bool kStates = false, bool kExt = false>
 __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* residuals,
                                                        double* opd, uint8_t* status,
                                                        double* partial, int* pfailed,
                                                        int* phard,
-                                                       mko_internal::RayStates states) {
+                                                       mko_internal::RayStates states,
+                                                       double* ext_partial) {
     __shared__ SurfLds L;
     __shared__ double red[kWaves];
     __shared__ int ired[kWaves];
@@ -319,12 +400,70 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     double opl[kRpl], pen[kRpl];
     uint8_t st[kRpl];
     double sw = 0.0, swo = 0.0;
+    const int lane = threadIdx.x & 63;
+    double eh = 0.0, ec = 1.0; // surface `lane`'s extrema over this wave's valid rays
 #pragma unroll
     for (int i = 0; i < kRpl; ++i) {
         const int k = threadIdx.x + i * kBlock;
         st[i] = MKO_STATUS_UNSUPPORTED;
         opl[i] = 0.0;
         pen[i] = 0.0;
+        if constexpr (kExt) {
+            constexpr bool kFast = kTraceFastMath && std::is_same_v<R, double>;
+            const bool have = k < t.n;
+            Sample s{};
+            Ray r0{};
+            if (have) {
+                s = sample_of(P, t, k);
+                r0 = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
+            }
+            mkod::RayT<R> r = ray_as<R>(r0);
+            Soft soft{0.0, MKO_STATUS_OK};
+            double pend_h = 0.0, pend_c = 1.0;
+            std::conditional_t<kStates, LastState, NoRecord> rec{};
+            const uint8_t sti = trace_ray_ext<kSoft, kFast>(P, L, r, M, kappa, soft, have, lane,
+                                                            pend_h, pend_c, false, rec);
+            const uint64_t have_mask = __ballot(have);
+            const uint64_t ok_mask = __ballot(have && sti == MKO_STATUS_OK);
+            if (ok_mask == have_mask) {
+                if (lane < P.S) {
+                    eh = pend_h > eh ? pend_h : eh;
+                    ec = pend_c < ec ? pend_c : ec;
+                }
+            } else {
+                // some ray of the wave did not end with status Ok: its earlier surfaces must not
+                // count, so the wave's valid rays are traced again and folded in directly
+                mkod::RayT<R> rr = ray_as<R>(r0);
+                Soft s2{0.0, MKO_STATUS_OK};
+                NoRecord nr;
+                trace_ray_ext<kSoft, kFast>(P, L, rr, M, kappa, s2, have && sti == MKO_STATUS_OK,
+                                            lane, eh, ec, true, nr);
+            }
+            if (have) {
+                st[i] = sti;
+                if constexpr (kStates) {
+                    const int64_t row = t.offset + k;
+                    const bool okr = sti == MKO_STATUS_OK;
+                    if (okr) {
+                        states.data[state_index(states, t.v, 0, row)] = rec.p.x;
+                        states.data[state_index(states, t.v, 1, row)] = rec.p.y;
+                        states.data[state_index(states, t.v, 2, row)] = rec.p.z;
+                        states.data[state_index(states, t.v, 3, row)] = rec.d.x;
+                        states.data[state_index(states, t.v, 4, row)] = rec.d.y;
+                        states.data[state_index(states, t.v, 5, row)] = rec.d.z;
+                    }
+                    states.data[state_index(states, t.v, 6, row)] =
+                        okr ? mkod::to_double(r.opl) : __builtin_nan("");
+                }
+                opl[i] = mkod::to_double(r.opl);
+                pen[i] = soft.penalty;
+                if (mkod::in_piston(sti)) {
+                    sw += s.weight;
+                    swo += s.weight * opl[i];
+                }
+            }
+            continue;
+        }
         if (k < t.n) {
             const Sample s = sample_of(P, t, k);
             mkod::RayT<R> r = ray_as<R>(source_ray(P, t, s, a, sin_t, cos_t, L.n[0]));
@@ -407,12 +546,107 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
             phard[idx] = th;
         }
     }
+    if constexpr (kExt) {
+        __shared__ double ext_w[kWaves][MKO_MAX_SURFACES][2];
+        const int wave = threadIdx.x >> 6;
+        if (lane < P.S) {
+            ext_w[wave][lane][0] = eh;
+            ext_w[wave][lane][1] = ec;
+        }
+        __syncthreads();
+        if (int(threadIdx.x) < P.S) {
+            double h = ext_w[0][threadIdx.x][0], c = ext_w[0][threadIdx.x][1];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) {
+                const double hw = ext_w[w][threadIdx.x][0], cw = ext_w[w][threadIdx.x][1];
+                h = hw > h ? hw : h;
+                c = cw < c ? cw : c;
+            }
+            double* out = ext_partial + (size_t(blockIdx.x) * P.S + threadIdx.x) * 2;
+            out[0] = h;
+            out[1] = c;
+        }
+    }
+}
+
+// Extrema of every variant from the tuple partials: out[v][s] = {sqrt(max height^2), min cos I}
+// over the variant's tuples (extrema_finalize_kernel of the reference, wavefront_block.hpp:414).
+__global__ void extrema_finalize_kernel(int V, int T, int S, const double* partial, double* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(V) * S) return;
+    const int64_t v = i / S, s = i - v * S;
+    double h = 0.0, c = 1.0;
+    for (int t = 0; t < T; ++t) {
+        const double* p = partial + ((v * T + t) * S + s) * 2;
+        h = p[0] > h ? p[0] : h;
+        c = p[1] < c ? p[1] : c;
+    }
+    out[i * 2] = __dsqrt_rn(h);
+    out[i * 2 + 1] = c;
+}
+
+// Constraint rows of every variant (one thread per variant), rows[v * stride + offset + k] for
+// the `count` items in row order; extrema [V][S][2] for the items that need them.
+__global__ void constraint_rows_kernel(int V, int S, const double* curvature,
+                                       const double* thickness, const double* paraxial_index,
+                                       const double* pupil_radius, uint64_t sphere_mask,
+                                       const mko_constraint* items, int count, int need_paraxial,
+                                       const double* extrema, double* rows, int64_t stride,
+                                       int64_t offset) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const mkoc::VariantRows vr{curvature + size_t(v) * S, thickness + size_t(v) * S,
+                               paraxial_index ? paraxial_index + size_t(v) * S : nullptr,
+                               sphere_mask, pupil_radius[v], S};
+    mkoc::Paraxial p{1.0, 0.0, 0.0, 0.0};
+    if (need_paraxial) p = mkoc::paraxial(vr.c, vr.t, vr.n, S);
+    const double* ext = extrema ? extrema + size_t(v) * S * 2 : nullptr;
+    double* out = rows + size_t(v) * stride + offset;
+    for (int k = 0; k < count; ++k) out[k] = mkoc::row(items[k], vr, p, ext);
+}
+
+// Normal equations of the constraint rows added to a run's A [P][P] and g [P] (the fused
+// mixed-precision path forms no J): A(p, q) += sum_e Je(p, e) Je(q, e), g(p) += sum_e Je(p, e)
+// re(e), one workgroup per run, Je [runs][P][E], re [runs][E].
+__global__ void extra_normal_kernel(int P, int E, const double* Je, const double* re, double* a,
+                                    double* g) {
+    const int run = blockIdx.x;
+    const double* J = Je + size_t(run) * P * E;
+    const double* r = re + size_t(run) * E;
+    for (int i = threadIdx.x; i < P * (P + 1); i += blockDim.x) {
+        const int p = i / (P + 1), q = i - p * (P + 1);
+        const double* u = J + size_t(p) * E;
+        const double* w = q < P ? J + size_t(q) * E : r;
+        double acc = 0.0;
+        for (int e = 0; e < E; ++e) acc = fma(u[e], w[e], acc);
+        if (q < P) a[(size_t(run) * P + p) * P + q] += acc;
+        else g[size_t(run) * P + p] += acc;
+    }
+}
+
+// Sums of squares of the constraint rows per variant (extra_merit_values,
+// src/optim/levenberg_marquardt.cpp:659; trace_extra_merit, src/optim/trace_extra_residuals.cpp:96):
+// e[v] = (extra ? extra[v] + : ) sum_{k < E} r_k^2, t[v] = sum_{E <= k < E + Et} r_k^2, ascending.
+__global__ void rows_merit_kernel(int V, const double* rows, int64_t stride, int64_t offset, int E,
+                                  int Et, const double* extra, double* e_sum, double* t_sum) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double* r = rows + size_t(v) * stride + offset;
+    double e = 0.0;
+    for (int k = 0; k < E; ++k) e = e + r[k] * r[k];
+    if (e_sum) e_sum[v] = extra ? extra[v] + e : e;
+    double t = 0.0;
+    for (int k = E; k < E + Et; ++k) t = t + r[k] * r[k];
+    if (t_sum) t_sum[v] = t;
 }
 
 // merit[v] = sum over tuples in (f, l) order + extra[v] (merit_finalize_kernel, merit.cu:207)
 __global__ void merit_finalize_kernel(int V, int T, const double* partial, const int* pfailed,
-                                      const int* phard, const double* extra, double* merit,
-                                      int64_t* failed, int64_t* hard) {
+                                      const int* phard, const double* extra,
+                                      const double* traced, double* merit, int64_t* failed,
+                                      int64_t* hard) {
+#pragma clang fp contract(off)
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     double s = 0.0;
@@ -422,7 +656,10 @@ __global__ void merit_finalize_kernel(int V, int T, const double* partial, const
         f += pfailed[size_t(v) * T + t];
         h += phard[size_t(v) * T + t];
     }
-    merit[v] = extra ? s + extra[v] : s;
+    // (ray merit + sum e^2) + sum t^2 (CpuLmBackend::merit, levenberg_marquardt.cpp:866-888)
+    double m = extra ? s + extra[v] : s;
+    if (traced) m = m + traced[v];
+    merit[v] = m;
     failed[v] = f;
     hard[v] = h;
 }
@@ -440,6 +677,7 @@ struct BatchPtrs {
     double* field_angle;
     double* field_target;
     double* reference_curvature;
+    double* paraxial_index;
 };
 struct ConstBatchPtrs {
     const double* curvature;
@@ -449,6 +687,7 @@ struct ConstBatchPtrs {
     const double* field_angle;
     const double* field_target;
     const double* reference_curvature;
+    const double* paraxial_index;
 };
 
 __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, int per_run,
@@ -464,6 +703,8 @@ __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, 
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
         dst.curvature[size_t(vv) * S + i] = src.curvature[size_t(from) * S + i];
         dst.thickness[size_t(vv) * S + i] = src.thickness[size_t(from) * S + i];
+        if (src.paraxial_index && dst.paraxial_index)
+            dst.paraxial_index[size_t(vv) * S + i] = src.paraxial_index[size_t(from) * S + i];
     }
     for (int i = threadIdx.x; i < S * L; i += blockDim.x)
         dst.index[size_t(vv) * S * L + i] = src.index[size_t(from) * S * L + i];
@@ -493,7 +734,7 @@ __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, 
 __global__ __launch_bounds__(kBlock) void fd_difference_kernel(
     const double* res, int64_t stride, const uint8_t* st, int64_t rays, int runs, int P,
     int64_t rows, const double* step_plus, const double* step_minus, const uint8_t* stencil,
-    double* J, double* r0, int64_t* changes) {
+    double* J, double* r0, int64_t* changes, int64_t jstride, int64_t joffset) {
     __shared__ int ired[kWaves];
     const int i = blockIdx.x / (P + 1);
     const int k = blockIdx.x - i * (P + 1);
@@ -501,7 +742,8 @@ __global__ __launch_bounds__(kBlock) void fd_difference_kernel(
     const size_t v0 = size_t(i) * n;
     const double* r_0 = res + v0 * stride;
     if (k == P) {
-        for (int64_t r = threadIdx.x; r < rows; r += kBlock) r0[size_t(i) * rows + r] = r_0[r];
+        for (int64_t r = threadIdx.x; r < rows; r += kBlock)
+            r0[size_t(i) * jstride + joffset + r] = r_0[r];
         return;
     }
     const double* ra = res + (v0 + 2 * k + 1) * stride;
@@ -509,7 +751,7 @@ __global__ __launch_bounds__(kBlock) void fd_difference_kernel(
     const double da = step_plus[size_t(i) * P + k];
     const double db = step_minus[size_t(i) * P + k];
     const uint8_t sten = stencil[size_t(i) * P + k];
-    double* col = J + (size_t(i) * P + k) * rows;
+    double* col = J + (size_t(i) * P + k) * jstride + joffset;
     for (int64_t r = threadIdx.x; r < rows; r += kBlock) {
         const double a0 = r_0[r], aa = ra[r], ab = rb[r];
         double v;
@@ -521,6 +763,7 @@ __global__ __launch_bounds__(kBlock) void fd_difference_kernel(
         }
         col[r] = v;
     }
+    if (!st) return;
     int c = 0;
     const uint8_t* s0 = st + v0 * rays;
     const uint8_t* sa = st + (v0 + 2 * k + 1) * rays;
@@ -1153,6 +1396,13 @@ void check_hip(hipError_t e, const char* what) {
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// The raw device entry points trace ray rows only; constraint rows are the host layer's
+// (mko_engine_*), which evaluates them next to the traces.
+void no_rows(const mko_residual_settings& s) {
+    if (s.extra && s.extra->count > 0)
+        throw std::invalid_argument("constraint rows are evaluated by the mko_engine_* entry points");
+}
+
 TraceParams make_params(const mko_batch& b, const mko_residual_settings& s) {
     if (b.surfaces < 2 || b.surfaces > MKO_MAX_SURFACES)
         throw std::invalid_argument("surfaces outside [2, 64]");
@@ -1233,13 +1483,14 @@ int rays_per_lane(const TraceParams& P) {
     return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : 8;
 }
 
-template <int kMode, bool kSoft, class R>
+template <int kMode, bool kSoft, class R, bool kStates, bool kExt>
 void launch_trace_rpl(int rpl, const TraceParams& P, double* res, double* opd, uint8_t* st,
-                      double* part, int* pf, int* ph, hipStream_t stream) {
+                      double* part, int* pf, int* ph, const mko_internal::RayStates& states,
+                      double* ext, hipStream_t stream) {
     const dim3 grid(unsigned(P.V) * unsigned(P.T));
 #define MKO_LAUNCH(RPL)                                                                          \
-    hipLaunchKernelGGL((trace_kernel<kMode, kSoft, RPL, R>), grid, dim3(kBlock), 0, stream, P,    \
-                       res, opd, st, part, pf, ph, mko_internal::RayStates{})
+    hipLaunchKernelGGL((trace_kernel<kMode, kSoft, RPL, R, kStates, kExt>), grid, dim3(kBlock),   \
+                       0, stream, P, res, opd, st, part, pf, ph, states, ext)
     switch (rpl) {
     case 1: MKO_LAUNCH(1); break;
     case 2: MKO_LAUNCH(2); break;
@@ -1249,25 +1500,42 @@ void launch_trace_rpl(int rpl, const TraceParams& P, double* res, double* opd, u
 #undef MKO_LAUNCH
 }
 
+template <int kMode, bool kSoft, class R>
+void launch_trace_ext(int rpl, const TraceParams& P, double* res, double* opd, uint8_t* st,
+                      double* part, int* pf, int* ph, double* ext, hipStream_t stream) {
+    const mko_internal::RayStates none{};
+    if (ext) launch_trace_rpl<kMode, kSoft, R, false, true>(rpl, P, res, opd, st, part, pf, ph, none, ext, stream);
+    else launch_trace_rpl<kMode, kSoft, R, false, false>(rpl, P, res, opd, st, part, pf, ph, none, nullptr, stream);
+}
+
+template <class R>
+void launch_trace_r(int mode, int rpl, const TraceParams& P, double* res, double* opd, uint8_t* st,
+                    double* part, int* pf, int* ph, double* ext, hipStream_t stream) {
+    if (mode == 0) {
+        if (P.soft) launch_trace_ext<0, true, R>(rpl, P, res, opd, st, part, pf, ph, ext, stream);
+        else launch_trace_ext<0, false, R>(rpl, P, res, opd, st, part, pf, ph, ext, stream);
+    } else {
+        if (P.soft) launch_trace_ext<1, true, R>(rpl, P, res, opd, st, part, pf, ph, ext, stream);
+        else launch_trace_ext<1, false, R>(rpl, P, res, opd, st, part, pf, ph, ext, stream);
+    }
+}
+
+// RESIDUALS (mode 0) or MERIT partials (mode 1); ext: extrema partials [V*T][S][2] or NULL;
+// states: the LM MERIT's final ray states (mode 1, hard failure, FP64 trace) or NULL.
 void launch_trace(int mode, const TraceParams& P, double* res, double* opd, uint8_t* st,
-                  double* part, int* pf, int* ph, hipStream_t stream) {
+                  double* part, int* pf, int* ph, double* ext, hipStream_t stream,
+                  const mko_internal::RayStates* states = nullptr) {
     if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large for one launch");
     const int rpl = rays_per_lane(P);
-    if (P.precision == MKO_PRECISION_DF64) {
-        using mkod::df64;
-        if (mode == 0) {
-            if (P.soft) launch_trace_rpl<0, true, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
-            else launch_trace_rpl<0, false, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
-        } else {
-            if (P.soft) launch_trace_rpl<1, true, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
-            else launch_trace_rpl<1, false, df64>(rpl, P, res, opd, st, part, pf, ph, stream);
-        }
-    } else if (mode == 0) {
-        if (P.soft) launch_trace_rpl<0, true, double>(rpl, P, res, opd, st, part, pf, ph, stream);
-        else launch_trace_rpl<0, false, double>(rpl, P, res, opd, st, part, pf, ph, stream);
+    if (states) {
+        if (mode != 1 || P.soft || P.precision != MKO_PRECISION_FP64)
+            throw std::invalid_argument("ray states need the MERIT trace, hard failure mode and FP64");
+        if (ext) launch_trace_rpl<1, false, double, true, true>(rpl, P, res, opd, st, part, pf, ph, *states, ext, stream);
+        else launch_trace_rpl<1, false, double, true, false>(rpl, P, res, opd, st, part, pf, ph, *states, nullptr, stream);
+    } else if (P.precision == MKO_PRECISION_DF64) {
+        launch_trace_r<mkod::df64>(mode, rpl, P, res, opd, st, part, pf, ph, ext, stream);
     } else {
-        if (P.soft) launch_trace_rpl<1, true, double>(rpl, P, res, opd, st, part, pf, ph, stream);
-        else launch_trace_rpl<1, false, double>(rpl, P, res, opd, st, part, pf, ph, stream);
+        launch_trace_r<double>(mode, rpl, P, res, opd, st, part, pf, ph, ext, stream);
     }
     check_hip(hipGetLastError(), "trace kernel launch");
 }
@@ -1349,14 +1617,14 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                 int runs, int per_run, const mko_variables* vars_device, const double* x0,
                 const double* plus, const double* minus, double* curvature, double* thickness,
                 double* index, double* pupil_radius, double* field_angle, double* field_target,
-                double* reference_curvature, void* stream) {
+                double* reference_curvature, double* paraxial_index, void* stream) {
     MKO_TRY({
         if (runs <= 0) return 0;
         const ConstBatchPtrs src{templ->curvature, templ->thickness, templ->index,
                                  templ->pupil_radius, templ->field_angle, templ->field_target,
-                                 templ->reference_curvature};
+                                 templ->reference_curvature, templ->paraxial_index};
         const BatchPtrs dst{curvature, thickness, index, pupil_radius, field_angle, field_target,
-                            reference_curvature};
+                            reference_curvature, paraxial_index};
         hipLaunchKernelGGL(point_batch_kernel, dim3(unsigned(runs) * unsigned(per_run)), dim3(64),
                            0, static_cast<hipStream_t>(stream), src, dst, runs, per_run,
                            template_index, template_per_run, templ->surfaces, templ->wavelengths,
@@ -1423,33 +1691,91 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
             throw std::invalid_argument("ray states need hard failure mode and the FP64 trace");
         if (states.rays != P.rays) throw std::invalid_argument("ray states of another layout");
         if (P.V == 0) return 0;
-        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
         const hipStream_t s = static_cast<hipStream_t>(stream);
         int* pf = partial_counts;
         int* ph = partial_counts + size_t(P.V) * P.T;
-        const dim3 grid(unsigned(P.V) * unsigned(P.T));
-        switch (rays_per_lane(P)) {
-        case 1:
-            hipLaunchKernelGGL((trace_kernel<1, false, 1, double, true>), grid, dim3(kBlock), 0, s,
-                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
-            break;
-        case 2:
-            hipLaunchKernelGGL((trace_kernel<1, false, 2, double, true>), grid, dim3(kBlock), 0, s,
-                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
-            break;
-        case 4:
-            hipLaunchKernelGGL((trace_kernel<1, false, 4, double, true>), grid, dim3(kBlock), 0, s,
-                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
-            break;
-        default:
-            hipLaunchKernelGGL((trace_kernel<1, false, 8, double, true>), grid, dim3(kBlock), 0, s,
-                               P, nullptr, nullptr, nullptr, partial, pf, ph, states);
-            break;
-        }
-        check_hip(hipGetLastError(), "merit launch");
+        launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, nullptr, s, &states);
         hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
-                           P.T, partial, pf, ph, nullptr, merit, failed, hard_failed);
+                           P.T, partial, pf, ph, nullptr, nullptr, merit, failed, hard_failed);
         check_hip(hipGetLastError(), "merit finalize launch");
+    })
+}
+
+int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
+                  double* residuals, int64_t stride, double* opd, uint8_t* status,
+                  double* partial, int* partial_counts, const RayStates* states,
+                  double* ext_partial, double* extrema, void* stream) {
+    MKO_TRY({
+        TraceParams P = make_params(*batch, *settings);
+        if (mode == 0) {
+            if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
+            P.stride = stride;
+        }
+        if (states && states->rays != P.rays) throw std::invalid_argument("ray states of another layout");
+        if (P.V == 0) return 0;
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        int* pf = mode == 1 ? partial_counts : nullptr;
+        int* ph = mode == 1 ? partial_counts + size_t(P.V) * P.T : nullptr;
+        launch_trace(mode, P, residuals, opd, status, partial, pf, ph, ext_partial, s, states);
+        if (ext_partial) {
+            const int64_t n = int64_t(P.V) * P.S;
+            hipLaunchKernelGGL(extrema_finalize_kernel, dim3(unsigned((n + 255) / 256)), dim3(256),
+                               0, s, P.V, P.T, P.S, ext_partial, extrema);
+            check_hip(hipGetLastError(), "extrema finalize launch");
+        }
+    })
+}
+
+int merit_finalize(int V, int T, const double* partial, const int* partial_counts,
+                   const double* extra, const double* traced, double* merit, int64_t* failed,
+                   int64_t* hard_failed, void* stream) {
+    MKO_TRY({
+        if (V <= 0) return 0;
+        hipLaunchKernelGGL(merit_finalize_kernel, dim3((V + 255) / 256), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), V, T, partial, partial_counts,
+                           partial_counts + size_t(V) * T, extra, traced, merit, failed,
+                           hard_failed);
+        check_hip(hipGetLastError(), "merit finalize launch");
+    })
+}
+
+int constraint_rows(const mko_batch* batch, const mko_constraint* items, int count,
+                    int need_paraxial, const double* extrema, double* rows, int64_t stride,
+                    int64_t offset, void* stream) {
+    MKO_TRY({
+        if (batch->variants <= 0 || count <= 0) return 0;
+        uint64_t mask = 0;
+        for (int i = 0; i < batch->surfaces; ++i)
+            if (batch->type[i] == MKO_SURFACE_SPHERE) mask |= 1ull << i;
+        if (need_paraxial && !batch->paraxial_index)
+            throw std::invalid_argument("paraxial constraint rows need the batch's paraxial_index");
+        hipLaunchKernelGGL(constraint_rows_kernel, dim3((batch->variants + 127) / 128), dim3(128),
+                           0, static_cast<hipStream_t>(stream), batch->variants, batch->surfaces,
+                           batch->curvature, batch->thickness, batch->paraxial_index,
+                           batch->pupil_radius, mask, items, count, need_paraxial, extrema, rows,
+                           stride, offset);
+        check_hip(hipGetLastError(), "constraint rows launch");
+    })
+}
+
+int extra_normal(int runs, int P, int E, const double* Je, const double* re, double* a, double* g,
+                 void* stream) {
+    MKO_TRY({
+        if (runs <= 0 || E <= 0) return 0;
+        hipLaunchKernelGGL(extra_normal_kernel, dim3(unsigned(runs)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), P, E, Je, re, a, g);
+        check_hip(hipGetLastError(), "extra normal launch");
+    })
+}
+
+int rows_merit(int V, const double* rows, int64_t stride, int64_t offset, int E, int Et,
+               const double* extra, double* e_sum, double* t_sum, void* stream) {
+    MKO_TRY({
+        if (V <= 0) return 0;
+        hipLaunchKernelGGL(rows_merit_kernel, dim3((V + 255) / 256), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), V, rows, stride, offset, E, Et,
+                           extra, e_sum, t_sum);
+        check_hip(hipGetLastError(), "rows merit launch");
     })
 }
 
@@ -1463,11 +1789,12 @@ int mko_residuals_device(const mko_batch* batch, const mko_residual_settings* se
                          double* residuals, int64_t stride, double* opd, uint8_t* status,
                          void* stream) {
     MKO_TRY({
+        no_rows(*settings);
         TraceParams P = make_params(*batch, *settings);
         if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
         P.stride = stride;
         if (P.V == 0) return 0;
-        launch_trace(0, P, residuals, opd, status, nullptr, nullptr, nullptr,
+        launch_trace(0, P, residuals, opd, status, nullptr, nullptr, nullptr, nullptr,
                      static_cast<hipStream_t>(stream));
     })
 }
@@ -1476,14 +1803,15 @@ int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settin
                      const double* extra_merit, double* merit, int64_t* failed,
                      int64_t* hard_failed, double* partial, int* partial_counts, void* stream) {
     MKO_TRY({
+        no_rows(*settings);
         const TraceParams P = make_params(*batch, *settings);
         if (P.V == 0) return 0;
         const hipStream_t s = static_cast<hipStream_t>(stream);
         int* pf = partial_counts;
         int* ph = partial_counts + size_t(P.V) * P.T;
-        launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, s);
+        launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, nullptr, s);
         hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
-                           P.T, partial, pf, ph, extra_merit, merit, failed, hard_failed);
+                           P.T, partial, pf, ph, extra_merit, nullptr, merit, failed, hard_failed);
         check_hip(hipGetLastError(), "merit finalize launch");
     })
 }
@@ -1493,10 +1821,11 @@ int mko_point_batch_device(const mko_batch* templ, int template_per_run, int run
                            const double* plus, const double* minus, double* curvature,
                            double* thickness, double* index, double* pupil_radius,
                            double* field_angle, double* field_target,
-                           double* reference_curvature, void* stream) {
+                           double* reference_curvature, double* paraxial_index, void* stream) {
     return mko_internal::point_batch(templ, nullptr, template_per_run, runs, per_run, vars_device,
                                      x0, plus, minus, curvature, thickness, index, pupil_radius,
-                                     field_angle, field_target, reference_curvature, stream);
+                                     field_angle, field_target, reference_curvature,
+                                     paraxial_index, stream);
 }
 
 int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
@@ -1509,7 +1838,7 @@ int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
         hipLaunchKernelGGL(fd_difference_kernel, dim3(unsigned(runs) * unsigned(variables + 1)),
                            dim3(kBlock), 0, static_cast<hipStream_t>(stream), fd_residuals,
                            stride, fd_status, rays, runs, variables, rows, step_plus, step_minus,
-                           stencil, jacobian, r0, status_changes);
+                           stencil, jacobian, r0, status_changes, rows, int64_t(0));
         check_hip(hipGetLastError(), "fd difference launch");
     })
 }
@@ -1518,7 +1847,28 @@ int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_setti
                                 const mko_variables* vars, int precision, double* jacobian,
                                 double* r0, uint8_t* status, void* stream) {
     MKO_TRY({
-        const TraceParams P = make_params(*batch, *settings);
+        no_rows(*settings);
+        if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED)
+            throw std::invalid_argument("unknown precision");
+        return mko_internal::adjoint_jacobian(batch, settings, vars, jacobian, r0, status, 0,
+                                              stream);
+    })
+}
+
+} // extern "C"
+
+namespace mko_internal {
+
+int adjoint_jacobian(const mko_batch* batch, const mko_residual_settings* settings,
+                     const mko_variables* vars, double* jacobian, double* r0, uint8_t* status,
+                     int64_t stride, void* stream) {
+    MKO_TRY({
+        TraceParams P = make_params(*batch, *settings);
+        if (stride > 0) {
+            if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
+            P.stride = stride;
+        }
+        const int precision = MKO_PRECISION_FP64;
         const ParamMap m = make_map(*vars, batch->surfaces);
         if (P.V == 0) return 0;
         if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
@@ -1535,6 +1885,24 @@ int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_setti
     })
 }
 
+int fd_difference(const double* rows, int64_t stride, const uint8_t* status, int64_t rays,
+                  int runs, int variables, int64_t count, const double* step_plus,
+                  const double* step_minus, const uint8_t* stencil, double* jacobian, double* r0,
+                  int64_t* status_changes, int64_t jstride, int64_t joffset, void* stream) {
+    MKO_TRY({
+        if (runs <= 0 || count <= 0) return 0;
+        hipLaunchKernelGGL(fd_difference_kernel, dim3(unsigned(runs) * unsigned(variables + 1)),
+                           dim3(kBlock), 0, static_cast<hipStream_t>(stream), rows, stride,
+                           status, rays, runs, variables, count, step_plus, step_minus, stencil,
+                           jacobian, r0, status_changes, jstride, joffset);
+        check_hip(hipGetLastError(), "fd difference launch");
+    })
+}
+
+} // namespace mko_internal
+
+extern "C" {
+
 size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, int variables) {
     const size_t tuples = size_t(variants) * size_t(fields) * size_t(wavelengths);
     const size_t P = size_t(variables);
@@ -1546,6 +1914,10 @@ size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, i
 int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_settings* settings,
                               const mko_variables* vars, double* a, double* g, void* workspace,
                               size_t workspace_bytes, void* stream) {
+    if (settings && settings->extra && settings->extra->count > 0) {
+        mko_detail::last_error = "constraint rows are evaluated by the mko_engine_* entry points";
+        return MKO_ERR_INVALID;
+    }
     return mko_internal::adjoint_normal_states(batch, settings, vars, a, g, workspace,
                                                workspace_bytes, mko_internal::RayStates{}, stream);
 }

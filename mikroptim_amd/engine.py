@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import abi
-from .batch import rays_per_variant, trace_rows
+from .batch import rays_per_variant, residual_rows
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmikroptim_hip.so")
@@ -26,6 +26,7 @@ EXPORTS = [
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
     "mko_adjoint_normal_device", "mko_gram_device", "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
+    "mko_engine_surface_extrema",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
     "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
     "mko_lm_results", "mko_lm_destroy",
@@ -73,6 +74,7 @@ def library():
         "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
         "mko_engine_merit": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)], C.c_int),
+        "mko_engine_surface_extrema": ([C.c_void_p, bp, sp, _dp], C.c_int),
         "mko_engine_jacobian": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
                                  _dp, _dp], C.c_int),
         "mko_engine_kernel_timing": ([C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
@@ -180,8 +182,10 @@ class TraceEngine:
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_FAMILIES)}
 
     def residuals(self, batch, settings):
+        """(residuals [V][R], OPD [V][rays], status [V][rays]); R = trace rows + the
+        settings' constraint rows."""
         V = batch.variants
-        R = trace_rows(batch, settings)
+        R = residual_rows(batch, settings)
         rays = rays_per_variant(batch, settings)
         res = np.zeros((V, R))
         opd = np.zeros((V, rays))
@@ -190,6 +194,15 @@ class TraceEngine:
         _check(library().mko_engine_residuals(self._h, C.byref(b), C.byref(s), abi.ptr(res),
                                               abi.ptr(opd), abi.ptr(st, C.c_uint8)))
         return res, opd, st
+
+    def surface_extrema(self, batch, settings):
+        """[V][S][2]: largest ray height and smallest cos I per surface over the rays with
+        status Ok (SurfaceExtremaResult)."""
+        out = np.zeros((batch.variants, batch.surfaces, 2))
+        b, s = batch.c_struct(), settings.c_struct()
+        _check(library().mko_engine_surface_extrema(self._h, C.byref(b), C.byref(s),
+                                                    abi.ptr(out)))
+        return out
 
     def merit(self, batch, settings, extra=None):
         V = batch.variants
@@ -210,7 +223,7 @@ class TraceEngine:
         then not returned, and the mixed-precision adjoint forms A and g without storing J."""
         x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
         runs, P = x0.shape
-        R = trace_rows(base, settings)
+        R = residual_rows(base, settings)
         J = np.zeros((runs, P, R)) if want_jacobian else None
         r0 = np.zeros((runs, R)) if want_jacobian else None
         a = np.zeros((runs, P, P))

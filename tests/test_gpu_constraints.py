@@ -1,0 +1,235 @@
+"""Parity of the constraint rows on the HIP path (C-ABI of libmikroptim_hip.so) with the CPU
+oracle: surface extrema of the MERIT / RESIDUALS traces, the rows in residuals, merit, FD and
+adjoint Jacobians, the fused mixed-precision normal equations and the LM.
+
+Rows that need no extrema are computed on the device with the oracle's operation order and
+correctly rounded division / square root, so they are compared bit for bit (values and their
+central differences). Rows from extrema inherit the trace's tolerance (1e-12 mm per ray
+height); their central differences divide that by the FD step (~6e-6 relative)."""
+
+import numpy as np
+import pytest
+
+from mikroptim_amd import abi, systems
+from mikroptim_amd import batch as bt
+from oracle_lib import oracle
+
+pytestmark = pytest.mark.gpu
+
+H_TOL = 1e-12    # mm, largest ray height (kCpuGpuFp64 per ray)
+C_TOL = 1e-13    # smallest cos I
+
+
+def _singlet(n=6, seed=7, failure=abi.FAILURE_HARD):
+    b, v = systems.singlet(n, seed=seed)
+    s = systems.singlet_settings(failure)
+    s.constraints = systems.singlet_constraints()
+    return b, v, s
+
+
+def _objective(n=3):
+    b, v = systems.objective(variants=16, seed=2)
+    b = b.take(np.arange(n))
+    s = systems.objective_settings(abi.FAILURE_HARD)
+    s.rings, s.spokes = 4, 6
+    ext = oracle().surface_extrema(b, s, threads=8)
+    S = b.surfaces
+    # limits just inside the current extrema, so the traced rows are active
+    s.constraints = [
+        bt.focal_length(float(1.0 / oracle().paraxial(b, 0)[4]) * 1.01, 0.5),
+        bt.working_distance(0.65, 4.0, abi.MODE_MIN),
+        bt.max_semi_diameter(S - 3, float(ext[0, S - 3, 0]) * 0.98, 3.0, clearance=0.05),
+        bt.max_incidence_angle(S - 4, float(np.degrees(np.arccos(ext[0, S - 4, 1]))) - 1.0, 50.0),
+        bt.center_thickness(3, 4, float(b.thickness[0, 3]) * 1.05, 2.0),
+        bt.min_edge_thickness(S - 5, S - 4, 0.3, 1.0, clearance=0.1),
+        bt.petzval(0.0, 1.0),
+    ]
+    return b, v, s
+
+
+def _split(s):
+    E = sum(not c.needs_extrema for c in s.constraints)
+    return E, len(s.constraints)
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+# ---- surface extrema -----------------------------------------------------------------------
+
+@pytest.mark.parametrize("failure", [abi.FAILURE_HARD, abi.FAILURE_SOFT])
+def test_extrema_match_oracle(engine, failure):
+    b, _ = systems.singlet(8, seed=5)
+    b.pupil_radius[3] = 1e4    # no valid ray: neutral extrema
+    b.pupil_radius[5] = 110.0  # outer rings fail
+    s = systems.singlet_settings(failure)
+    e = engine.surface_extrema(b, s)
+    eo = oracle().surface_extrema(b, s)
+    assert np.all(e[3, :, 0] == 0.0) and np.all(e[3, :, 1] == 1.0)
+    assert np.max(np.abs(e[..., 0] - eo[..., 0])) <= H_TOL
+    assert np.max(np.abs(e[..., 1] - eo[..., 1])) <= C_TOL
+
+
+def test_extrema_objective_and_df64(engine):
+    b, _, s = _objective(3)
+    e = engine.surface_extrema(b, s)
+    eo = oracle().surface_extrema(b, s, threads=8)
+    assert np.max(np.abs(e[..., 0] - eo[..., 0])) <= H_TOL
+    assert np.max(np.abs(e[..., 1] - eo[..., 1])) <= C_TOL
+    s.precision = abi.PRECISION_DF64
+    ed = engine.surface_extrema(b, s)
+    assert np.max(np.abs(ed[..., 0] - eo[..., 0])) <= 1e-9
+    assert np.max(np.abs(ed[..., 1] - eo[..., 1])) <= 1e-10
+
+
+# ---- rows in residuals and merit ---------------------------------------------------------------
+
+@pytest.mark.parametrize("failure", [abi.FAILURE_HARD, abi.FAILURE_SOFT])
+def test_residual_rows_match_oracle(engine, failure):
+    b, _, s = _singlet(8, failure=failure)
+    plain = systems.singlet_settings(failure)
+    res, opd, st = engine.residuals(b, s)
+    res0, opd0, st0 = engine.residuals(b, plain)
+    T = res0.shape[1]
+    assert np.array_equal(res[:, :T], res0) and np.array_equal(st, st0)
+    rows = res[:, T:]
+    ro = oracle().constraint_rows(b, s)
+    E, _ = _split(s)
+    assert np.array_equal(rows[:, :E], ro[:, :E])          # paraxial / geometric rows
+    assert np.max(np.abs(rows[:, E:] - ro[:, E:])) <= 1e-10  # rows from extrema
+
+
+@pytest.mark.parametrize("failure", [abi.FAILURE_HARD, abi.FAILURE_SOFT])
+def test_merit_with_rows(engine, failure):
+    b, _, s = _singlet(8, failure=failure)
+    m, f, h = engine.merit(b, s)
+    mo, fo, ho = oracle().merit(b, s)
+    assert np.array_equal(f, fo) and np.array_equal(h, ho)
+    assert _rel(m, mo) <= 1e-9
+    res, _, _ = engine.residuals(b, s)
+    assert _rel(m, (res ** 2).sum(1)) <= 1e-13
+    extra = np.linspace(0.5, 2.0, b.variants)
+    me, _, _ = engine.merit(b, s, extra=extra)
+    assert _rel(me, m + extra) <= 1e-14
+
+
+# ---- rows in the Jacobians ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("method", ["fd", "adjoint"])
+@pytest.mark.parametrize("failure", [abi.FAILURE_HARD, abi.FAILURE_SOFT])
+def test_jacobian_rows_match_oracle(engine, method, failure):
+    b, v, s = _singlet(6, failure=failure)
+    x0 = v.extract(b)[:3]
+    base = b.take([0])
+    J, r0, a, g = engine.jacobian(base, s, v, x0, method=method)
+    fn = oracle().fd_jacobian if method == "fd" else oracle().adjoint_jacobian
+    Jo, r0o, _ = fn(base, s, v, x0)
+    E, n = _split(s)
+    T = J.shape[2] - n
+    assert np.array_equal(J[:, :, T:T + E], Jo[:, :, T:T + E])
+    assert np.array_equal(r0[:, T:T + E], r0o[:, T:T + E])
+    assert np.max(np.abs(r0[:, T + E:] - r0o[:, T + E:])) <= 1e-10
+    # derivatives of rows from extrema: value differences / FD step
+    err = np.abs(J[:, :, T + E:] - Jo[:, :, T + E:])
+    assert err.max() <= 1e-5 * max(1.0, np.abs(Jo[:, :, T + E:]).max())
+    ao, go = oracle().normal_equations(J, r0)
+    assert np.array_equal(a, ao) and np.array_equal(g, go)
+
+
+def test_objective_jacobian_rows(engine):
+    b, v, s = _objective(2)
+    x0 = v.extract(b)[:2]
+    base = b.take([0])
+    for method in ("fd", "adjoint"):
+        J, r0, _, _ = engine.jacobian(base, s, v, x0, method=method)
+        fn = oracle().fd_jacobian if method == "fd" else oracle().adjoint_jacobian
+        Jo, r0o, _ = fn(base, s, v, x0, threads=8)
+        E, n = _split(s)
+        T = J.shape[2] - n
+        assert np.array_equal(J[:, :, T:T + E], Jo[:, :, T:T + E])
+        rows_t = J[:, :, T + E:]
+        scale = max(1.0, np.abs(Jo[:, :, T + E:]).max())
+        assert np.abs(rows_t - Jo[:, :, T + E:]).max() <= 1e-5 * scale
+        assert np.abs(r0[:, T:] - r0o[:, T:]).max() <= 1e-9
+        assert np.count_nonzero(r0o[:, T + E:]) > 0  # the traced rows are active
+
+
+def test_mixed_normal_equations_with_rows(engine):
+    """Fused mixed-precision A, g (no J formed; rows added by their own normal equations) against
+    the FP64 normal equations of the stored J with the same rows."""
+    b, v, s = _objective(2)
+    x0 = v.extract(b)[:2]
+    base = b.take([0])
+    _, _, a, g = engine.jacobian(base, s, v, x0, method="adjoint", precision="fp64")
+    _, _, am, gm = engine.jacobian(base, s, v, x0, method="adjoint", precision="mixed",
+                                   want_jacobian=False)
+    for i in range(2):
+        assert np.linalg.norm(am[i] - a[i]) <= 1e-5 * np.linalg.norm(a[i])
+        assert np.linalg.norm(gm[i] - g[i]) <= 1e-7 * np.linalg.norm(g[i])
+
+
+# ---- LM with rows ------------------------------------------------------------------------------
+
+def _same_lm(a, b):
+    assert np.array_equal(a.x, b.x)
+    assert np.array_equal(a.merit, b.merit)
+    assert a.status == b.status
+    assert np.array_equal(a.iterations, b.iterations)
+    assert a.cycles == b.cycles
+
+
+@pytest.mark.parametrize("method", [abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT])
+def test_lm_with_rows_device_equals_host_control(engine, method):
+    b, v, s = _singlet(6)
+    x0 = v.extract(b)[:6]
+    kw = dict(max_iterations=10, threads=4, jacobian_method=method)
+    rd = engine.optimize(b.take([0]), s, v, x0, abi.lm_options(**kw))
+    rh = engine.optimize(b.take([0]), s, v, x0, abi.lm_options(host_control=True, **kw))
+    _same_lm(rd, rh)
+
+
+def test_lm_with_rows_follows_oracle(engine):
+    b, v, s = _singlet(4, seed=3)
+    x0 = v.extract(b)[:3]
+    base = b.take([0])
+    for method in (abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT):
+        opt = abi.lm_options(max_iterations=10, threads=4, jacobian_method=method)
+        res = engine.optimize(base, s, v, x0, opt)
+        _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, opt)
+        mo = np.array([r.merit for r in runs_o])
+        assert np.allclose(res.merit, mo, rtol=1e-3)
+        pts = base.take(np.zeros(3, dtype=np.int64))
+        v.apply(pts, x0)
+        m0, _, _ = engine.merit(pts, s)
+        assert np.all(res.merit < m0)
+
+
+def test_lm_with_rows_mixed_objective(engine):
+    b, v, s = _objective(2)
+    x0 = v.extract(b)[:2]
+    base = b.take([0])
+    pts = base.take(np.zeros(2, dtype=np.int64))
+    v.apply(pts, x0)
+    m0, _, _ = engine.merit(pts, s)
+    kw = dict(max_iterations=3, threads=4, jacobian_method=abi.JACOBIAN_ADJOINT)
+    rd = engine.optimize(base, s, v, x0, abi.lm_options(**kw), precision="mixed")
+    rh = engine.optimize(base, s, v, x0, abi.lm_options(host_control=True, **kw),
+                         precision="mixed")
+    _same_lm(rd, rh)
+    assert np.all(rd.merit < m0)
+
+
+def test_device_entry_points_reject_rows(engine):
+    """The raw device entry points trace ray rows only and say so (no kernel is launched)."""
+    import ctypes as C
+
+    from mikroptim_amd.engine import library
+
+    b, _, s = _singlet(1)
+    cb, cs = b.c_struct(), s.c_struct()
+    rc = library().mko_residuals_device(C.byref(cb), C.byref(cs), None, 0, None, None, None)
+    assert rc != 0 and b"constraint rows" in library().mko_last_error()
+    rc = library().mko_merit_device(C.byref(cb), C.byref(cs), None, None, None, None, None, None,
+                                    None)
+    assert rc != 0
