@@ -192,6 +192,28 @@ def main():
                       "achieved": ach, "peak": 78.6, "unit": "TFLOP/s", "frac": ach / 78.6,
                       "flop_per_ray": flop_fwd}
 
+    # constraint rows (paraxial and mechanical, the reference's extra residuals) on the same
+    # workload: LM cycles/s with the rows that need no extrema, and with rows from the surface
+    # extrema as well (their columns come from central differences of each run's FD batch,
+    # traced with extrema, as in the reference); and the MERIT trace rate with extrema
+    ext0 = eng.surface_extrema(base, settings)
+    rows = {}
+    for name, ext in (("without_extrema", None), ("with_extrema", ext0)):
+        st = copy.copy(settings)
+        st.constraints = systems.objective_constraints(base, ext)
+        steps = args.steps if ext is None else max(1, min(args.steps, 3))
+        n_r, t_r, kt_r, res_r, jac_r = run_lm(eng, base, st, variables, x0, abi.JACOBIAN_ADJOINT,
+                                              "mixed", 1, steps, args.threads)
+        rows[name] = {"lm_iter_per_s": n_r / t_r, "steps": n_r, "rows": len(st.constraints),
+                      "jacobians": jac_r,
+                      "kernel_ms": {k: v[0] for k, v in kt_r.items()}}
+    st = copy.copy(settings)
+    eng.surface_extrema(batch, st)
+    eng.kernel_timing(True)
+    eng.surface_extrema(batch, st)
+    ms, _ = eng.kernel_timing(False)["trace"]
+    rows["merit_trace_with_extrema_rays_per_s"] = batch.variants * rays / (ms * 1e-3) if ms else None
+
     # per-result error of the mixed-precision normal equations against the FP64 stored-J path
     # (itself within 1e-10 of the FP64 oracle) at 8 final points
     xs = res_adj.x[:8]
@@ -220,6 +242,7 @@ def main():
         "jacobians_in_timed_cycles": {"adjoint_mixed": jac_adj, "fd_fp64": jac_fd},
         "mixed_normal_equations_error_vs_fp64": mixed_err,
         "merit_trace_rays_per_s": trace_rate,
+        "constraint_rows": rows,
         "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
         "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
         "kernel_ms_fd_path": {k: v[0] for k, v in kt_fd.items()},

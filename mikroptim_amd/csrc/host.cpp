@@ -226,6 +226,7 @@ struct RowPlan {
     std::vector<mko_constraint> items;
     int E = 0, Et = 0;
     bool paraxial = false;
+    uint64_t surfaces = 0; // surfaces whose extrema the rows read
     const mko_constraint* dev = nullptr;
     int count() const { return E + Et; }
 };
@@ -242,6 +243,10 @@ RowPlan plan_rows(const mko_batch& b, const mko_residual_settings& s) {
     for (const mko_constraint& c : r.items) {
         (con_needs_extrema(c) ? r.Et : r.E) += 1;
         r.paraxial = r.paraxial || con_needs_paraxial(c);
+        if (con_needs_extrema(c)) {
+            r.surfaces |= 1ull << c.surface;
+            if (c.type == MKO_CON_MIN_EDGE_THICKNESS) r.surfaces |= 1ull << c.next_surface;
+        }
     }
     return r;
 }
@@ -519,7 +524,8 @@ public:
             double* ex = rp.Et ? ext_.as<double>(size_t(count) * S * 2) : nullptr;
             timer.run(kTrace, [&] {
                 dev_check(mko_internal::trace_extrema(&batch_.view, &s, 0, dres, R, dopd, dst,
-                                                      nullptr, nullptr, nullptr, ep, ex, nullptr));
+                                                      nullptr, nullptr, nullptr, ep, ex,
+                                                      rp.surfaces, nullptr));
             });
             if (rp.count())
                 dev_check(mko_internal::constraint_rows(&batch_.view, rp.dev, rp.count(),
@@ -548,7 +554,8 @@ public:
             double* ex = ext_.as<double>(size_t(count) * S * 2);
             timer.run(kTrace, [&] {
                 dev_check(mko_internal::trace_extrema(&batch_.view, &s, 1, nullptr, 0, nullptr,
-                                                      nullptr, part, pc, nullptr, ep, ex, nullptr));
+                                                      nullptr, part, pc, nullptr, ep, ex, ~0ull,
+                                                      nullptr));
             });
             copy_out(out + size_t(first) * S * 2, ex, size_t(count) * S * 2 * 8);
         }
@@ -601,7 +608,7 @@ public:
         double* ex = rp.Et ? ext_.as<double>(size_t(count) * S * 2) : nullptr;
         timer.run(kTrace, [&] {
             dev_check(mko_internal::trace_extrema(&dev, &s, 1, nullptr, 0, nullptr, nullptr, part,
-                                                  pc, states, ep, ex, nullptr));
+                                                  pc, states, ep, ex, rp.surfaces, nullptr));
             const double* esum = dextra;
             const double* tsum = nullptr;
             if (rp.count()) {
@@ -984,7 +991,8 @@ private:
         double* ex = rp.Et ? ext_.as<double>(n * S * 2) : nullptr;
         timer.run(kTrace, [&] {
             dev_check(mko_internal::trace_extrema(&fdb, &s, 0, fres, R, nullptr, fst, nullptr,
-                                                  nullptr, nullptr, ep, ex, nullptr));
+                                                  nullptr, nullptr, ep, ex, rp.surfaces,
+                                                  nullptr));
             if (rp.count())
                 dev_check(mko_internal::constraint_rows(&fdb, rp.dev, rp.count(), rp.paraxial, ex,
                                                         fres, R, T, nullptr));
@@ -1028,7 +1036,8 @@ private:
             ex = ext_.as<double>(n * S * 2);
             timer.run(kTrace, [&] {
                 dev_check(mko_internal::trace_extrema(&rowfd_.view, &s, 1, nullptr, 0, nullptr,
-                                                      nullptr, part, pc, nullptr, ep, ex, nullptr));
+                                                      nullptr, part, pc, nullptr, ep, ex,
+                                                      rp.surfaces, nullptr));
             });
         }
         double* rows = rows_.as<double>(n * E);

@@ -44,10 +44,11 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
 // LM's final ray states (mode 1) and the surface extrema: ext_partial [V*T][S][2] workspace,
 // extrema [V][S][2] = {largest ray height, smallest cos I} over the rays with status Ok
 // (surface_extrema_cpu, src/batch/batch_extrema.cpp:75). ext_partial == NULL: no extrema.
+// `surfaces`: bit s set = surface s is reduced (the others stay at the neutral 0 / 1).
 int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
                   double* residuals, int64_t stride, double* opd, uint8_t* status,
                   double* partial, int* partial_counts, const RayStates* states,
-                  double* ext_partial, double* extrema, void* stream);
+                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream);
 
 // merit[v] = (sum of the tuple partials + extra[v]) + traced[v] (extra / traced may be NULL).
 int merit_finalize(int V, int T, const double* partial, const int* partial_counts,

@@ -71,6 +71,7 @@ struct TraceParams {
     double src_pt[3];
     double src_na;
     uint64_t sphere_mask;
+    uint64_t ext_mask; // surfaces whose extrema a kExt trace reduces
     int64_t rays, trace_rows, stride;
     double gl_u[MKO_MAX_RINGS], gl_w[MKO_MAX_RINGS];
     double field_weight[kMaxFields];
@@ -277,7 +278,8 @@ __device__ __forceinline__ void surface_extremum(double x, double y, double z, d
     ci = dx * (-c * x) + dy * (-c * y) + dz * (1.0 - c * z);
 }
 
-// trace_ray with the surface extrema of the wave's rays: at every surface s (0 = start state),
+// trace_ray with the surface extrema of the wave's rays: at every surface s in P.ext_mask (0 =
+// start state; the other surfaces keep the neutral 0 / 1),
 // the largest height^2 and the smallest cos I over the wave's lanes whose ray is `alive` (has
 // passed every surface so far) are reduced across the wave; lane s keeps surface s's values in
 // eh / ec (S <= 64 = wave width), replacing them (merge = false) or folding them in (merge =
@@ -309,7 +311,7 @@ __device__ __forceinline__ uint8_t trace_ray_ext(const TraceParams& P, const Sur
         if (alive)
             surface_extremum(to_double(r.p.x), to_double(r.p.y), to_double(r.p.z), 0.0,
                              to_double(r.d.x), to_double(r.d.y), to_double(r.d.z), h, c);
-        reduce(0, h, c);
+        if (P.ext_mask & 1ull) reduce(0, h, c);
     }
     if (alive) rec.put(0, r, mkod::kBranchRegular);
     for (int s = 1; s < P.S; ++s) {
@@ -329,7 +331,7 @@ __device__ __forceinline__ uint8_t trace_ray_ext(const TraceParams& P, const Sur
                                  dy, dz, h, c);
             }
         }
-        reduce(s, h, c);
+        if ((P.ext_mask >> s) & 1ull) reduce(s, h, c); // uniform: P.ext_mask is a kernel argument
     }
     if (!alive) return st;
     if (!(r.d.z > num<R>(0.0))) return MKO_STATUS_BACKWARD;
@@ -1704,9 +1706,10 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
 int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
                   double* residuals, int64_t stride, double* opd, uint8_t* status,
                   double* partial, int* partial_counts, const RayStates* states,
-                  double* ext_partial, double* extrema, void* stream) {
+                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream) {
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
+        P.ext_mask = surfaces;
         if (mode == 0) {
             if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
             P.stride = stride;

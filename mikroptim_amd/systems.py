@@ -25,7 +25,7 @@ import math
 import numpy as np
 
 from . import abi, optics, rng
-from . import batch
+from . import batch as batch_mod
 from .batch import ResidualSettings, SystemBatch, VariableSet
 
 LAMBDA_REF = 587.56
@@ -98,14 +98,15 @@ def singlet_constraints():
     parfocal length, paraxial focus, Petzval bound, semi-diameter of the first lens surface,
     center and edge thickness of the lens (fixed heights and ray heights), incidence angle at
     the second lens surface."""
-    return [batch.focal_length(100.0, 0.03), batch.working_distance(95.0, 0.1, abi.MODE_MIN),
-            batch.parfocal_length(150.0, 1, 2.0, 0.1), batch.paraxial_focus(0.5),
-            batch.petzval(0.001, 100.0),
-            batch.max_semi_diameter(1, 4.0, 2.0, clearance=0.5),
-            batch.center_thickness(1, 2, 6.0, 1.0),
-            batch.min_edge_thickness(1, 2, 2.0, 1.0, semi_diameters=(6.0, 6.0)),
-            batch.max_incidence_angle(2, 5.0, 3.0),
-            batch.min_edge_thickness(1, 2, 4.7, 1.0, clearance=0.2)]
+    m = batch_mod
+    return [m.focal_length(100.0, 0.03), m.working_distance(95.0, 0.1, abi.MODE_MIN),
+            m.parfocal_length(150.0, 1, 2.0, 0.1), m.paraxial_focus(0.5),
+            m.petzval(0.001, 100.0),
+            m.max_semi_diameter(1, 4.0, 2.0, clearance=0.5),
+            m.center_thickness(1, 2, 6.0, 1.0),
+            m.min_edge_thickness(1, 2, 2.0, 1.0, semi_diameters=(6.0, 6.0)),
+            m.max_incidence_angle(2, 5.0, 3.0),
+            m.min_edge_thickness(1, 2, 4.7, 1.0, clearance=0.2)]
 
 
 # ---- the 24-surface objective ------------------------------------------------------------------
@@ -258,6 +259,28 @@ def objective_variables(batch):
 
 def objective_settings(failure=abi.FAILURE_HARD):
     return ResidualSettings(rings=16, spokes=32, failure=failure)
+
+
+def objective_constraints(batch, extrema=None):
+    """Constraint rows for the objective, relative to variant 0: focal length 1 % above its
+    paraxial value, working distance >= 0.65 mm, the first element's center thickness >= 105 %
+    of its value, Petzval curvature bound 0 (active); with `extrema` ([V][S][2], the surface
+    extrema of the batch) also the rows from ray heights and angles: semi-diameter of surface
+    S-3 at 98 % of its largest ray height (+0.05 mm clearance), incidence angle at surface S-4
+    one degree below its largest, edge thickness of the element S-5..S-4 at ray heights."""
+    S = batch.surfaces
+    f, _ = optics.focal_and_bfd(batch.curvature[0], batch.thickness[0], batch.paraxial_index[0])
+    out = [batch_mod.focal_length(1.01 * f, 0.5),
+           batch_mod.working_distance(0.65, 4.0, abi.MODE_MIN),
+           batch_mod.center_thickness(1, 2, 1.05 * float(batch.thickness[0, 1]), 2.0),
+           batch_mod.petzval(0.0, 1.0)]
+    if extrema is not None:
+        angle = float(np.degrees(np.arccos(extrema[0, S - 4, 1])))
+        out += [batch_mod.max_semi_diameter(S - 3, 0.98 * float(extrema[0, S - 3, 0]), 3.0,
+                                            clearance=0.05),
+                batch_mod.max_incidence_angle(S - 4, angle - 1.0, 50.0),
+                batch_mod.min_edge_thickness(S - 5, S - 4, 0.3, 1.0, clearance=0.1)]
+    return out
 
 
 # ---- uneven batches: random topologies (one LM per topology group) ----------------------------

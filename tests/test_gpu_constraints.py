@@ -11,7 +11,6 @@ import numpy as np
 import pytest
 
 from mikroptim_amd import abi, systems
-from mikroptim_amd import batch as bt
 from oracle_lib import oracle
 
 pytestmark = pytest.mark.gpu
@@ -32,18 +31,8 @@ def _objective(n=3):
     b = b.take(np.arange(n))
     s = systems.objective_settings(abi.FAILURE_HARD)
     s.rings, s.spokes = 4, 6
-    ext = oracle().surface_extrema(b, s, threads=8)
-    S = b.surfaces
-    # limits just inside the current extrema, so the traced rows are active
-    s.constraints = [
-        bt.focal_length(float(1.0 / oracle().paraxial(b, 0)[4]) * 1.01, 0.5),
-        bt.working_distance(0.65, 4.0, abi.MODE_MIN),
-        bt.max_semi_diameter(S - 3, float(ext[0, S - 3, 0]) * 0.98, 3.0, clearance=0.05),
-        bt.max_incidence_angle(S - 4, float(np.degrees(np.arccos(ext[0, S - 4, 1]))) - 1.0, 50.0),
-        bt.center_thickness(3, 4, float(b.thickness[0, 3]) * 1.05, 2.0),
-        bt.min_edge_thickness(S - 5, S - 4, 0.3, 1.0, clearance=0.1),
-        bt.petzval(0.0, 1.0),
-    ]
+    # limits just inside the variant-0 extrema, so the rows from extrema are active
+    s.constraints = systems.objective_constraints(b, oracle().surface_extrema(b, s, threads=8))
     return b, v, s
 
 
