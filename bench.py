@@ -185,6 +185,12 @@ def main():
         eng.merit(batch, st)
         ms, n = eng.kernel_timing(False)["trace"]
         trace_rate[name] = batch.variants * rays / (ms * 1e-3) if ms > 0 else None
+    # FP32 prefilter (float trace with boundary margins) of all variants, against the FP64 MERIT
+    eng.prefilter(batch, settings)
+    eng.kernel_timing(True)
+    eng.prefilter(batch, settings)
+    ms, _ = eng.kernel_timing(False)["trace"]
+    trace_rate["prefilter_fp32"] = batch.variants * rays / (ms * 1e-3) if ms > 0 else None
     merit_roof = None
     if trace_rate.get("fp64"):
         ach = trace_rate["fp64"] * flop_fwd / 1e12

@@ -180,6 +180,23 @@ int mko_engine_point_merit(mko_engine* engine, const mko_batch* base,
                            int count, const int32_t* runs, const double* x, double* merit,
                            int64_t* failed, int64_t* hard_failed);
 
+/* FP32 prefilter (TraceEngine::prefilter_variants, include/mikroptim/cuda/trace_engine.hpp;
+ * prefilter_kernel src/cuda/prefilter.cu:87; prefilter_decide src/batch/batch_prefilter.cpp:236):
+ * every ray traced in float with its boundary margin, per-variant stats, then the conservative
+ * decision in double. extra_merit [V] (FP64, e.g. constraint rows' sum e^2) or NULL. The
+ * values only select variants: they never enter a merit, residual or Jacobian. */
+int mko_engine_prefilter(mko_engine* engine, const mko_batch* batch,
+                         const mko_residual_settings* settings,
+                         const mko_prefilter_options* options, const double* extra_merit,
+                         mko_prefilter_result* result);
+
+/* The decision alone (host, no device work) from the stats already in `result` (merit,
+ * optical_length, failed, uncertain_ok, uncertain_failed): fills merit_lower / merit_upper /
+ * reject / reference_* / kept (prefilter_decide, src/batch/batch_prefilter.cpp:236). */
+int mko_prefilter_decide(const mko_batch* batch, const mko_residual_settings* settings,
+                         const mko_prefilter_options* options, const double* extra_merit,
+                         mko_prefilter_result* result);
+
 /* Batched Levenberg-Marquardt on the GPU backend (optimize_lm_gpu,
  * include/mikroptim/cuda/levenberg_marquardt.hpp:91; control logic of
  * src/optim/levenberg_marquardt.cpp:332-572, outer penalty rounds :891). precision applies to

@@ -187,6 +187,41 @@ typedef struct mko_lm_options {
     int32_t host_control;
 } mko_lm_options;
 
+/* FP32 prefilter of multistart variants (PrefilterCriteria / PrefilterMargins,
+ * include/mikroptim/batch/batch_prefilter.hpp:32,46; defaults in brackets). A variant is
+ * rejected only when the criteria also hold for its FP64 values. */
+typedef struct mko_prefilter_options {
+    double max_failed_fraction; /* (a) failed rays > fraction * rays per variant [0.5] */
+    double max_merit;           /* (b) merit above this [+inf = off] */
+    double max_merit_factor;    /* (c) merit above factor * best merit [+inf = off] */
+    double boundary;            /* eta: margin under which a ray's decision is uncertain [0.15] */
+    double opl_error_factor;    /* kappa of the OPD error model [32] */
+    double merit_relative;      /* m of the OPD error model, in [0, 1) [0.01] */
+} mko_prefilter_options;
+
+/* PrefilterReason bits (batch_prefilter.hpp:87) */
+enum {
+    MKO_PREFILTER_KEEP = 0,
+    MKO_PREFILTER_FAILED_RAYS = 1,
+    MKO_PREFILTER_MERIT_ABSOLUTE = 2,
+    MKO_PREFILTER_MERIT_RELATIVE = 4
+};
+
+/* PrefilterStats + PrefilterResult (batch_prefilter.hpp:61,96): caller-owned arrays [V]. */
+typedef struct mko_prefilter_result {
+    double* merit;            /* F32: sum r^2 over the certain-valid rays (float trace) */
+    double* optical_length;   /* largest optical length of a certain-valid ray [mm] */
+    int64_t* failed;          /* certainly failed rays */
+    int64_t* uncertain_ok;    /* valid in float, near the boundary */
+    int64_t* uncertain_failed; /* failed in float, near the boundary */
+    double* merit_lower;      /* bounds of the FP64 merit (with extra_merit) */
+    double* merit_upper;
+    uint8_t* reject;          /* MKO_PREFILTER_* bits, 0 = keep */
+    double reference_merit;   /* upper bound of the best merit for (c), +inf if none */
+    int32_t reference_variant; /* -1 if none */
+    int32_t kept;
+} mko_prefilter_result;
+
 /* Per-run LM result (LmRunResult, include/mikroptim/optim/levenberg_marquardt.hpp:130). */
 typedef struct mko_lm_run {
     double merit;

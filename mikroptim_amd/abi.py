@@ -83,6 +83,33 @@ class LmOptions(C.Structure):
     ]
 
 
+PREFILTER_KEEP, PREFILTER_FAILED_RAYS, PREFILTER_MERIT_ABSOLUTE, PREFILTER_MERIT_RELATIVE = (
+    0, 1, 2, 4)
+
+
+class PrefilterOptions(C.Structure):
+    _fields_ = [("max_failed_fraction", C.c_double), ("max_merit", C.c_double),
+                ("max_merit_factor", C.c_double), ("boundary", C.c_double),
+                ("opl_error_factor", C.c_double), ("merit_relative", C.c_double)]
+
+
+def prefilter_options(max_failed_fraction=0.5, max_merit=float("inf"),
+                      max_merit_factor=float("inf"), boundary=0.15, opl_error_factor=32.0,
+                      merit_relative=0.01):
+    """PrefilterCriteria + PrefilterMargins with the reference defaults
+    (include/mikroptim/batch/batch_prefilter.hpp:32-56)."""
+    return PrefilterOptions(max_failed_fraction, max_merit, max_merit_factor, boundary,
+                            opl_error_factor, merit_relative)
+
+
+class PrefilterResult(C.Structure):
+    _fields_ = [("merit", _dp), ("optical_length", _dp), ("failed", C.POINTER(C.c_int64)),
+                ("uncertain_ok", C.POINTER(C.c_int64)), ("uncertain_failed", C.POINTER(C.c_int64)),
+                ("merit_lower", _dp), ("merit_upper", _dp), ("reject", _u8p),
+                ("reference_merit", C.c_double), ("reference_variant", C.c_int32),
+                ("kept", C.c_int32)]
+
+
 class LmRun(C.Structure):
     _fields_ = [
         ("merit", C.c_double), ("failed_rays", C.c_int64), ("hard_failed_rays", C.c_int64),

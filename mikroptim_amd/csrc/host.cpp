@@ -481,6 +481,93 @@ private:
 
 // ---- Engine --------------------------------------------------------------------------------
 
+// ---- FP32 prefilter: the conservative decision (prefilter_decide,
+// src/batch/batch_prefilter.cpp:236-329) --------------------------------------------------------
+
+// check_prefilter_options (src/batch/batch_prefilter.cpp:40-52)
+void check_prefilter_options(const mko_prefilter_options& o) {
+    if (!(o.max_failed_fraction >= 0.0)) throw std::invalid_argument("max_failed_fraction must be >= 0");
+    if (!(o.max_merit >= 0.0)) throw std::invalid_argument("max_merit must be >= 0");
+    if (!(o.max_merit_factor >= 0.0)) throw std::invalid_argument("max_merit_factor must be >= 0");
+    if (!(o.boundary >= 0.0) || !std::isfinite(o.boundary))
+        throw std::invalid_argument("boundary margin must be finite and >= 0");
+    if (!(o.opl_error_factor >= 0.0) || !std::isfinite(o.opl_error_factor))
+        throw std::invalid_argument("opl_error_factor must be finite and >= 0");
+    if (!(o.merit_relative >= 0.0 && o.merit_relative < 1.0))
+        throw std::invalid_argument("merit_relative must be in [0, 1)");
+}
+
+// residual_scale (include/mikroptim/merit/ray_residual.hpp:132): sqrt(w_f w_l) / lambda_ref[mm]
+double residual_scale_of(const mko_batch& b, int f, int l) {
+    const double wf = b.field_weight[f], wl = b.wavelength_weight[l];
+    if (!(wf > 0.0) || !(wl > 0.0)) return 0.0;
+    double sf = 0.0, sl = 0.0;
+    for (int i = 0; i < b.fields; ++i) sf += b.field_weight[i];
+    for (int i = 0; i < b.wavelengths; ++i) sl += b.wavelength_weight[i];
+    const double w = (wf / sf) * (wl / sl);
+    if (!(w > 0.0)) return 0.0;
+    return std::sqrt(w) / (b.reference_wavelength_nm * 1e-6);
+}
+
+void prefilter_decide(const mko_batch& b, const mko_residual_settings& s,
+                      const mko_prefilter_options& o, const double* extra,
+                      mko_prefilter_result& r) {
+    check_prefilter_options(o);
+    constexpr double kU = 0x1p-24; // float unit roundoff
+    double scale_sq = 0.0;
+    for (int f = 0; f < b.fields; ++f)
+        for (int l = 0; l < b.wavelengths; ++l) {
+            const double sc = residual_scale_of(b, f, l);
+            scale_sq += sc * sc;
+        }
+    const double scale_norm = std::sqrt(scale_sq);
+    const int64_t R = rays_of(b, s);
+    const bool soft = s.failure == MKO_FAILURE_SOFT;
+    const int V = b.variants;
+    const double inf = std::numeric_limits<double>::infinity();
+    auto too_many = [&](int64_t failed) { return double(failed) > o.max_failed_fraction * double(R); };
+    for (int v = 0; v < V; ++v) {
+        const double m = r.merit[v];
+        const double root = std::sqrt(m < 0.0 ? 0.0 : m);
+        const double delta = o.opl_error_factor * kU * r.optical_length[v] * scale_norm;
+        const double rd = root - delta;
+        const double lower = (0.0 < rd ? rd : 0.0) / (1.0 + o.merit_relative);
+        const double upper = (root + delta) / (1.0 - o.merit_relative);
+        const double e = extra ? extra[v] : 0.0;
+        r.merit_lower[v] = lower * lower + e;
+        const bool finite = std::isfinite(root) && std::isfinite(delta);
+        const bool penalties = soft && r.failed[v] > 0;
+        const int64_t uncertain = r.uncertain_ok[v] + r.uncertain_failed[v];
+        r.merit_upper[v] = uncertain == 0 && finite && !penalties ? upper * upper + e : inf;
+        if (!finite) r.merit_lower[v] = e;
+        uint8_t rej = MKO_PREFILTER_KEEP;
+        if (too_many(r.failed[v])) rej |= MKO_PREFILTER_FAILED_RAYS;
+        if (r.merit_lower[v] > o.max_merit) rej |= MKO_PREFILTER_MERIT_ABSOLUTE;
+        r.reject[v] = rej;
+    }
+    r.reference_merit = inf;
+    r.reference_variant = -1;
+    if (std::isfinite(o.max_merit_factor)) {
+        for (int v = 0; v < V; ++v) {
+            const int64_t uncertain = r.uncertain_ok[v] + r.uncertain_failed[v];
+            if (uncertain != 0 || (r.reject[v] & MKO_PREFILTER_FAILED_RAYS) != 0 || r.failed[v] >= R)
+                continue;
+            if (r.merit_upper[v] < r.reference_merit) {
+                r.reference_merit = r.merit_upper[v];
+                r.reference_variant = v;
+            }
+        }
+        if (r.reference_variant >= 0) {
+            const double limit = o.max_merit_factor * r.reference_merit;
+            for (int v = 0; v < V; ++v)
+                if (r.merit_lower[v] > limit) r.reject[v] |= MKO_PREFILTER_MERIT_RELATIVE;
+        }
+    }
+    int kept = 0;
+    for (int v = 0; v < V; ++v) kept += r.reject[v] == MKO_PREFILTER_KEEP;
+    r.kept = kept;
+}
+
 class Engine {
 public:
     Engine(int device, size_t memory_limit) : device_(device), limit_(memory_limit) {
@@ -535,6 +622,40 @@ public:
             if (status) copy_out(status + size_t(first) * rays, dst, size_t(count) * rays);
         }
         hip_check(hipDeviceSynchronize(), "residuals");
+    }
+
+    // FP32 prefilter of a host batch: stats on the device (tranches), then the decision.
+    void prefilter(const mko_batch& b, const mko_residual_settings& s,
+                   const mko_prefilter_options& o, const double* extra, mko_prefilter_result& r) {
+        DeviceGuard g(device_);
+        check_batch(b);
+        check_prefilter_options(o);
+        validate_values(b, 0, b.variants);
+        if (!r.merit || !r.optical_length || !r.failed || !r.uncertain_ok || !r.uncertain_failed ||
+            !r.merit_lower || !r.merit_upper || !r.reject)
+            throw std::invalid_argument("prefilter result arrays missing");
+        const size_t T = size_t(b.fields) * b.wavelengths;
+        const size_t per = Strides(b).doubles() * 8 + T * 24 + 40;
+        const int tranche = tranche_size(b.variants, per);
+        for (int first = 0; first < b.variants; first += tranche) {
+            const int count = std::min(tranche, b.variants - first);
+            batch_.upload(b, first, count, nullptr);
+            void* ws = part_.reserve(size_t(count) * T * 24);
+            double* dm = out_merit_.as<double>(size_t(count) * 2);
+            int64_t* dc = out_failed_.as<int64_t>(size_t(count) * 3);
+            timer.run(kTrace, [&] {
+                dev_check(mko_internal::prefilter_stats(&batch_.view, &s, o.boundary, ws, dm,
+                                                        dm + count, dc, dc + count,
+                                                        dc + 2 * size_t(count), nullptr));
+            });
+            copy_out(r.merit + first, dm, size_t(count) * 8);
+            copy_out(r.optical_length + first, dm + count, size_t(count) * 8);
+            copy_out(r.failed + first, dc, size_t(count) * 8);
+            copy_out(r.uncertain_ok + first, dc + count, size_t(count) * 8);
+            copy_out(r.uncertain_failed + first, dc + 2 * size_t(count), size_t(count) * 8);
+        }
+        hip_check(hipDeviceSynchronize(), "prefilter");
+        prefilter_decide(b, s, o, extra, r);
     }
 
     // Surface extrema [V][S][2] of a host batch (the MERIT trace's max reductions).
@@ -1861,6 +1982,25 @@ int mko_engine_kernel_timing(mko_engine* e, int enable, double* ms, int64_t* lau
 int mko_engine_residuals(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                          double* res, double* opd, uint8_t* st) {
     MKO_HOST_TRY(e->engine.residuals(*b, *s, res, opd, st); return 0;)
+}
+
+int mko_engine_prefilter(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
+                         const mko_prefilter_options* o, const double* extra,
+                         mko_prefilter_result* r) {
+    MKO_HOST_TRY(e->engine.prefilter(*b, *s, *o, extra, *r); return 0;)
+}
+
+int mko_prefilter_decide(const mko_batch* b, const mko_residual_settings* s,
+                         const mko_prefilter_options* o, const double* extra,
+                         mko_prefilter_result* r) {
+    MKO_HOST_TRY({
+        check_batch(*b);
+        if (!r->merit || !r->optical_length || !r->failed || !r->uncertain_ok ||
+            !r->uncertain_failed || !r->merit_lower || !r->merit_upper || !r->reject)
+            throw std::invalid_argument("prefilter result arrays missing");
+        prefilter_decide(*b, *s, *o, extra, *r);
+        return 0;
+    })
 }
 
 int mko_engine_surface_extrema(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,

@@ -50,6 +50,12 @@ int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings,
                   double* partial, int* partial_counts, const RayStates* states,
                   double* ext_partial, double* extrema, uint64_t surfaces, void* stream);
 
+// FP32 prefilter stats of a device batch [V] (prefilter_kernel + second stage); workspace of
+// 24 bytes per (variant, tuple).
+int prefilter_stats(const mko_batch* batch, const mko_residual_settings* settings, double boundary,
+                    void* workspace, double* merit, double* optical, int64_t* failed,
+                    int64_t* uncertain_ok, int64_t* uncertain_failed, void* stream);
+
 // merit[v] = (sum of the tuple partials + extra[v]) + traced[v] (extra / traced may be NULL).
 int merit_finalize(int V, int T, const double* partial, const int* partial_counts,
                    const double* extra, const double* traced, double* merit, int64_t* failed,

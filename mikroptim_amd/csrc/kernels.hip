@@ -666,6 +666,283 @@ __global__ void merit_finalize_kernel(int V, int T, const double* partial, const
     hard[v] = h;
 }
 
+// ---- FP32 prefilter (prefilter_kernel, src/cuda/prefilter.cu:87; per ray prefilter_ray,
+// include/mikroptim/merit/prefilter_ray.hpp:171-249) ---------------------------------------------
+// The only float path: its values select multistart variants and never enter a merit, residual
+// or Jacobian. Each ray is traced in float (hard failure) and carries its boundary margin: the
+// smallest distance of the tracer's checks from their failure boundary, as a cosine (d_z before a
+// transfer, cos I = sqrt(D) at the sphere, cos I' = sqrt(k) after refraction, the reference
+// sphere's discriminant and denominator); for a failed ray the size of the violated check.
+
+struct PfLds {
+    float c[MKO_MAX_SURFACES], t[MKO_MAX_SURFACES], n[MKO_MAX_SURFACES];
+};
+
+__device__ __forceinline__ float pf_min_margin(float acc, float x) {
+    if (x < acc) return x;
+    return x >= acc ? acc : 0.0f; // NaN: 0 (uncertain)
+}
+__device__ __forceinline__ float pf_fail_linear(float x) { return -x > 0.0f ? -x : 0.0f; }
+__device__ __forceinline__ float pf_fail_root(float q) {
+    return -q > 0.0f ? __builtin_sqrtf(-q) : 0.0f;
+}
+
+struct PfRay {
+    uint8_t status;
+    float opl, margin;
+};
+
+__device__ __forceinline__ PfRay pf_trace(const PfLds& L, int S, V3<float> p, V3<float> d,
+                                          float opl, V3<float> M, int kind, float kappa) {
+    using mkod::cross;
+    using mkod::dot;
+    PfRay out{MKO_STATUS_OK, 0.0f, 0.0f};
+    float margin = 1.0f;
+    for (int s = 1; s < S; ++s) {
+        const float c = L.c[s], nb = L.n[s - 1], na = L.n[s];
+        if (!(d.z > 0.0f)) {
+            out.status = MKO_STATUS_BACKWARD;
+            out.margin = pf_fail_linear(d.z);
+            return out;
+        }
+        const V3<float> db = d;
+        p.z -= L.t[s - 1];
+        const float s0 = -p.z / d.z;
+        p.x += s0 * d.x;
+        p.y += s0 * d.y;
+        p.z = 0.0f;
+        const float H = c * (p.x * p.x + p.y * p.y);
+        const float B = d.z - c * (p.x * d.x + p.y * d.y);
+        const float D = B * B - c * H;
+        const float denom = D < 0.0f ? 0.0f : B + __builtin_sqrtf(D);
+        if (D < 0.0f || !(denom > 0.0f)) {
+            const float a = pf_fail_root(D), b = pf_fail_linear(B);
+            out.status = MKO_STATUS_MISSED;
+            out.margin = a > b ? a : b;
+            return out;
+        }
+        const float sh = H / denom;
+        p = p + sh * d;
+        opl += nb * (s0 + sh);
+        const float mu = nb / na;
+        const V3<float> N{-c * p.x, -c * p.y, 1.0f - c * p.z};
+        const float cosI = dot(d, N);
+        const float k = 1.0f - mu * mu * (1.0f - cosI * cosI);
+        if (k < 0.0f) {
+            out.status = MKO_STATUS_TIR;
+            out.margin = pf_fail_root(k);
+            return out;
+        }
+        d = mu * d + (__builtin_sqrtf(k) - mu * cosI) * N;
+        margin = pf_min_margin(margin, db.z);
+        margin = pf_min_margin(margin, dot(db, N));
+        margin = pf_min_margin(margin, dot(d, N));
+    }
+    if (!(d.z > 0.0f)) {
+        out.status = MKO_STATUS_BACKWARD;
+        out.margin = pf_fail_linear(d.z);
+        return out;
+    }
+    margin = pf_min_margin(margin, d.z);
+    const float n_image = L.n[S - 1];
+    if (kind != MKO_REF_LAST_SURFACE) {
+        const V3<float> q = p - M;
+        const V3<float> ci = cross(q, d);
+        const float t = -dot(ci, ci);
+        const float disc = 1.0f + kappa * kappa * t;
+        const float b = -dot(q, d);
+        float step = 0.0f;
+        const bool ok = disc >= 0.0f;
+        if (ok) step = b - kappa * t / (__builtin_sqrtf(disc) + 1.0f);
+        if (!ok || (!(step >= 0.0f) && !(step < 0.0f))) {
+            out.status = MKO_STATUS_MISSED;
+            out.margin = disc < 0.0f ? pf_fail_root(disc) : 0.0f;
+            return out;
+        }
+        out.opl = opl + n_image * step;
+        out.margin = pf_min_margin(margin, __builtin_sqrtf(disc));
+        return out;
+    }
+    const float r2 = dot(M, M);
+    const float inv_r = r2 > 0.0f ? 1.0f / __builtin_sqrtf(r2) : 1.0f;
+    const V3<float> q = p - M;
+    const float bq = -dot(q, d);
+    const float g = dot(p, p - 2.0f * M);
+    const float disc = bq * bq - g;
+    const float sigma = M.z >= 0.0f ? 1.0f : -1.0f;
+    const float root = disc > 0.0f ? __builtin_sqrtf(disc) : 0.0f;
+    const float den = sigma * (bq + sigma * root) * inv_r;
+    bool ok = !(disc < 0.0f);
+    float s_ref = 0.0f;
+    if (ok) {
+        const float rt = __builtin_sqrtf(disc);
+        const bool real_image = M.z >= 0.0f;
+        const float dn = real_image ? bq + rt : bq - rt;
+        ok = real_image ? dn > 0.0f : dn < 0.0f;
+        if (ok) s_ref = g / dn;
+    }
+    if (!ok) {
+        const float a = pf_fail_root(disc * inv_r * inv_r);
+        const float c = disc >= 0.0f ? pf_fail_linear(den) : 0.0f;
+        out.status = MKO_STATUS_MISSED;
+        out.margin = a > c ? a : c;
+        return out;
+    }
+    margin = pf_min_margin(margin, root * inv_r);
+    margin = pf_min_margin(margin, den);
+    out.opl = opl + n_image * s_ref;
+    out.margin = margin;
+    return out;
+}
+
+__device__ __forceinline__ float block_max_f(float v, float* lds) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[wave] = v;
+    __syncthreads();
+    float m = lds[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) m = lds[w] > m ? lds[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// One workgroup per (variant, field, wavelength) tuple: the tuple's float surface table in LDS,
+// OPL relative to the axial optical length (prefilter_opl_offset), classes 0 certain valid /
+// 1 certain failed / 2, 3 uncertain (margin <= eta), the piston over the certain-valid rays and
+// the tuple's sum of squared residuals; partials per tuple: merit (as double), largest optical
+// length, counts [3] (certain failed, uncertain valid, uncertain failed).
+template <int kRpl>
+__global__ __launch_bounds__(kBlock) void prefilter_kernel(TraceParams P, float eta,
+                                                           double* pmerit, float* plength,
+                                                           int* pcounts) {
+    __shared__ PfLds L;
+    __shared__ float fred[kWaves];
+    __shared__ int ired[kWaves];
+    const Tuple t = tuple_of(P, blockIdx.x);
+    const int tid = threadIdx.x;
+    if (tid < P.S) {
+        const size_t vs = size_t(t.v) * P.S + tid;
+        L.c[tid] = ((P.sphere_mask >> tid) & 1ull) ? float(P.curvature[vs]) : 0.0f;
+        L.t[tid] = float(P.thickness[vs]);
+        L.n[tid] = float(P.index[vs * P.L + t.l]);
+    }
+    __syncthreads();
+    float offset = 0.0f;
+    for (int s = 0; s + 1 < P.S; ++s) offset += L.n[s] * L.t[s];
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const float a = float(P.pupil_radius[t.v]);
+    const float th = float(P.field_angle[vf]);
+    const float sth = sinf(th), cth = cosf(th);
+    const V3<float> M{0.0f, float(P.field_target[vf]), L.t[P.S - 1]};
+    const float kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? float(P.reference_curvature[vf]) : 0.0f;
+    const float pi = float(kPi);
+    float opl[kRpl], wts[kRpl];
+    uint8_t cls[kRpl];
+    float wok = 0.0f, wopl = 0.0f, optical = 0.0f;
+    int nf = 0, nuo = 0, nuf = 0;
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+        const int k = tid + i * kBlock;
+        cls[i] = 4;
+        opl[i] = 0.0f;
+        wts[i] = 0.0f;
+        if (k >= t.n) continue;
+        const int ri = k / t.M, j = k - ri * t.M;
+        const float rho = __builtin_sqrtf(float(P.gl_u[ri]));
+        const float w = float(P.gl_w[ri]) / float(t.M);
+        const uint8_t mode = P.pupil_mode[t.f];
+        const float phi = mode == MKO_PUPIL_HALF   ? float(2 * j + 1) * pi / float(2 * t.M)
+                          : mode == MKO_PUPIL_FULL ? float(2 * j + 1) * pi / float(t.M)
+                                                   : 0.0f;
+        const float sp = sinf(phi), cp = cosf(phi);
+        V3<float> p, d;
+        float o;
+        if (P.source == MKO_SOURCE_POINT) {
+            const float sin_max = float(P.src_na) / L.n[0];
+            const float sr = sin_max * rho;
+            p = V3<float>{float(P.src_pt[0]), float(P.src_pt[1]), float(P.src_pt[2])};
+            d = V3<float>{sr * sp, sr * cp, __builtin_sqrtf((1.0f - sr) * (1.0f + sr))};
+            o = 0.0f;
+        } else {
+            const float r = a * rho;
+            p = V3<float>{r * sp, r * cp, 0.0f};
+            d = V3<float>{0.0f, sth, cth};
+            o = mkod::dot(p, d);
+        }
+        const PfRay rr = pf_trace(L, P.S, p, d, o - offset, M, P.ref_kind, kappa);
+        const bool ok = rr.status == MKO_STATUS_OK;
+        const bool certain = rr.margin > eta;
+        const uint8_t c = uint8_t(ok ? (certain ? 0 : 2) : (certain ? 1 : 3));
+        cls[i] = c;
+        opl[i] = rr.opl;
+        wts[i] = w;
+        if (c == 0) {
+            wok += w;
+            wopl += w * rr.opl;
+            const float len = fabsf(offset) + fabsf(rr.opl);
+            optical = len > optical ? len : optical;
+        }
+        nf += c == 1;
+        nuo += c == 2;
+        nuf += c == 3;
+    }
+    const float w_ok = block_sum(wok, fred);
+    const float sum_o = block_sum(wopl, fred);
+    const bool valid = w_ok > 0.0f;
+    const float piston = valid ? sum_o / w_ok : 0.0f;
+    const float scale = float(t.scale);
+    float r2 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kRpl; ++i) {
+        const float e = (valid && cls[i] == 0) ? opl[i] - piston : 0.0f;
+        const float r = scale * __builtin_sqrtf(wts[i]) * e;
+        r2 += r * r;
+    }
+    const float tr2 = block_sum(r2, fred);
+    const float tl = block_max_f(optical, fred);
+    const int tf = block_sum(nf, ired);
+    const int tuo = block_sum(nuo, ired);
+    const int tuf = block_sum(nuf, ired);
+    if (tid == 0) {
+        pmerit[blockIdx.x] = double(tr2);
+        plength[blockIdx.x] = tl;
+        pcounts[size_t(blockIdx.x) * 3] = tf;
+        pcounts[size_t(blockIdx.x) * 3 + 1] = tuo;
+        pcounts[size_t(blockIdx.x) * 3 + 2] = tuf;
+    }
+}
+
+// Per variant: merit = sum of the tuple partials (double, tuple order), optical length = max,
+// counts summed (the second stage of the reference's prefilter).
+__global__ void prefilter_finalize_kernel(int V, int T, const double* pmerit,
+                                          const float* plength, const int* pcounts,
+                                          double* merit, double* optical, int64_t* failed,
+                                          int64_t* uncertain_ok, int64_t* uncertain_failed) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    double m = 0.0;
+    float len = 0.0f;
+    int64_t f = 0, uo = 0, uf = 0;
+    for (int t = 0; t < T; ++t) {
+        const size_t i = size_t(v) * T + t;
+        m += pmerit[i];
+        len = plength[i] > len ? plength[i] : len;
+        f += pcounts[i * 3];
+        uo += pcounts[i * 3 + 1];
+        uf += pcounts[i * 3 + 2];
+    }
+    merit[v] = m;
+    optical[v] = double(len);
+    failed[v] = f;
+    uncertain_ok[v] = uo;
+    uncertain_failed[v] = uf;
+}
+
 // ---- FD batch and FD difference -------------------------------------------------------------
 
 // Builds the 2P+1 FD variants of each run from its template variant: variant 0 = x0,
@@ -1726,6 +2003,37 @@ int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings,
                                0, s, P.V, P.T, P.S, ext_partial, extrema);
             check_hip(hipGetLastError(), "extrema finalize launch");
         }
+    })
+}
+
+int prefilter_stats(const mko_batch* batch, const mko_residual_settings* settings, double boundary,
+                    void* workspace, double* merit, double* optical, int64_t* failed,
+                    int64_t* uncertain_ok, int64_t* uncertain_failed, void* stream) {
+    MKO_TRY({
+        TraceParams P = make_params(*batch, *settings);
+        if (!(boundary >= 0.0) || !std::isfinite(boundary))
+            throw std::invalid_argument("prefilter boundary margin must be finite and >= 0");
+        if (P.V == 0) return 0;
+        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
+        const size_t tuples = size_t(P.V) * P.T;
+        char* w = static_cast<char*>(workspace);
+        double* pm = reinterpret_cast<double*>(w);
+        float* pl = reinterpret_cast<float*>(w + tuples * 8);
+        int* pc = reinterpret_cast<int*>(w + tuples * 12);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        const dim3 grid{unsigned(tuples)}, block{unsigned(kBlock)};
+        const float eta = float(boundary);
+        switch (rays_per_lane(P)) {
+        case 1: hipLaunchKernelGGL((prefilter_kernel<1>), grid, block, 0, s, P, eta, pm, pl, pc); break;
+        case 2: hipLaunchKernelGGL((prefilter_kernel<2>), grid, block, 0, s, P, eta, pm, pl, pc); break;
+        case 4: hipLaunchKernelGGL((prefilter_kernel<4>), grid, block, 0, s, P, eta, pm, pl, pc); break;
+        default: hipLaunchKernelGGL((prefilter_kernel<8>), grid, block, 0, s, P, eta, pm, pl, pc); break;
+        }
+        check_hip(hipGetLastError(), "prefilter launch");
+        hipLaunchKernelGGL(prefilter_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s,
+                           P.V, P.T, pm, pl, pc, merit, optical, failed, uncertain_ok,
+                           uncertain_failed);
+        check_hip(hipGetLastError(), "prefilter finalize launch");
     })
 }
 

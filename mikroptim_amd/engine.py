@@ -26,7 +26,7 @@ EXPORTS = [
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
     "mko_adjoint_normal_device", "mko_gram_device", "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
-    "mko_engine_surface_extrema",
+    "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
     "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
     "mko_lm_results", "mko_lm_destroy",
@@ -75,6 +75,10 @@ def library():
         "mko_engine_merit": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)], C.c_int),
         "mko_engine_surface_extrema": ([C.c_void_p, bp, sp, _dp], C.c_int),
+        "mko_engine_prefilter": ([C.c_void_p, bp, sp, C.POINTER(abi.PrefilterOptions), _dp,
+                                  C.POINTER(abi.PrefilterResult)], C.c_int),
+        "mko_prefilter_decide": ([bp, sp, C.POINTER(abi.PrefilterOptions), _dp,
+                                  C.POINTER(abi.PrefilterResult)], C.c_int),
         "mko_engine_jacobian": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.c_int, C.c_int, _dp, _dp,
                                  _dp, _dp], C.c_int),
         "mko_engine_kernel_timing": ([C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)], C.c_int),
@@ -105,6 +109,49 @@ def _check(rc):
 
 _PRECISION = {"fp64": abi.PRECISION_FP64, "mixed": abi.PRECISION_MIXED}
 _METHOD = {"fd": abi.JACOBIAN_FD, "adjoint": abi.JACOBIAN_ADJOINT}
+
+
+class Prefilter:
+    """PrefilterResult: per-variant stats of the float trace (merit, optical_length, failed,
+    uncertain_ok, uncertain_failed), FP64 merit bounds, reject bits (0 = keep), the reference
+    merit / variant of criterion (c) and the number kept."""
+
+    def __init__(self, V):
+        self.merit = np.zeros(V)
+        self.optical_length = np.zeros(V)
+        self.failed = np.zeros(V, dtype=np.int64)
+        self.uncertain_ok = np.zeros(V, dtype=np.int64)
+        self.uncertain_failed = np.zeros(V, dtype=np.int64)
+        self.merit_lower = np.zeros(V)
+        self.merit_upper = np.zeros(V)
+        self.reject = np.zeros(V, dtype=np.uint8)
+        self.reference_merit = float("inf")
+        self.reference_variant = -1
+        self.kept = 0
+
+    def c_struct(self):
+        return abi.PrefilterResult(
+            abi.ptr(self.merit), abi.ptr(self.optical_length),
+            abi.ptr(self.failed, C.c_int64), abi.ptr(self.uncertain_ok, C.c_int64),
+            abi.ptr(self.uncertain_failed, C.c_int64), abi.ptr(self.merit_lower),
+            abi.ptr(self.merit_upper), abi.ptr(self.reject, C.c_uint8), float("inf"), -1, 0)
+
+    def take(self, r):
+        self.reference_merit = r.reference_merit
+        self.reference_variant = r.reference_variant
+        self.kept = r.kept
+        return self
+
+
+def prefilter_decide(batch, settings, stats, options=None, extra=None):
+    """prefilter_decide on given stats (a Prefilter whose stats are filled): host only."""
+    options = options or abi.prefilter_options()
+    ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float64)
+    r = stats.c_struct()
+    b, s = batch.c_struct(), settings.c_struct()
+    _check(library().mko_prefilter_decide(C.byref(b), C.byref(s), C.byref(options), abi.ptr(ex),
+                                          C.byref(r)))
+    return stats.take(r)
 
 
 class LmResult:
@@ -203,6 +250,18 @@ class TraceEngine:
         _check(library().mko_engine_surface_extrema(self._h, C.byref(b), C.byref(s),
                                                     abi.ptr(out)))
         return out
+
+    def prefilter(self, batch, settings, options=None, extra=None):
+        """TraceEngine::prefilter_variants: the FP32 prefilter of every variant (float
+        trace with boundary margins) and its conservative decision."""
+        options = options or abi.prefilter_options()
+        out = Prefilter(batch.variants)
+        ex = None if extra is None else np.ascontiguousarray(extra, dtype=np.float64)
+        r = out.c_struct()
+        b, s = batch.c_struct(), settings.c_struct()
+        _check(library().mko_engine_prefilter(self._h, C.byref(b), C.byref(s), C.byref(options),
+                                              abi.ptr(ex), C.byref(r)))
+        return out.take(r)
 
     def merit(self, batch, settings, extra=None):
         V = batch.variants

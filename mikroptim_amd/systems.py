@@ -78,6 +78,25 @@ def singlet(variants=1024, seed=1, perturb=0.05):
     return b, variables
 
 
+def singlet_multistart(variants=1000, seed=11):
+    """Multistart variants of the singlet for the prefilter: curvatures x(1 + 0.5 xi), lens
+    thickness and working distance x(1 + 0.3 xi) (xi uniform, perturbed_vector seeding), pupil
+    radius x 2^eta (eta uniform in [-1, 1]) and every seventh variant extreme (pupil x 4 ... 12),
+    so that the batch holds failed rays, near-boundary rays and merits over many decades."""
+    b, v = singlet(variants, seed=seed, perturb=0.0)
+    x0 = v.extract(b, 0)
+    xs = rng.perturbed_vectors(v, x0, variants, seed=seed, relative=(0.5, 0.3, 0.3),
+                               first_is_base=True)
+    xs = v.clamp(xs)
+    v.apply(b, xs)
+    g = np.random.default_rng(seed)
+    scale = 2.0 ** g.uniform(-1.0, 1.0, variants)
+    scale[::7] = g.uniform(4.0, 12.0, len(scale[::7]))
+    scale[0] = 1.0
+    b.pupil_radius[:] = b.pupil_radius * scale
+    return b, v
+
+
 def singlet_variables():
     """Both curvatures, the lens thickness and the working distance."""
     inf = math.inf

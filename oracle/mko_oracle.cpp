@@ -1885,6 +1885,266 @@ Backend cpu_backend(const Batch& base, const Layout& lay, const Vars& vars, int 
     return be;
 }
 
+// ---- FP32 prefilter: include/mikroptim/merit/prefilter_ray.hpp, src/batch/batch_prefilter.cpp --
+// The float ray of the prefilter (hard failure) with its boundary margin (the distance of the
+// tracer's checks from their failure boundary, as a cosine), per-variant stats in float summed
+// per tuple in double, and the conservative decision in double.
+namespace pf {
+
+struct V3f {
+    float x, y, z;
+};
+inline V3f operator+(V3f a, V3f b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3f operator-(V3f a, V3f b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3f operator*(float s, V3f a) { return {s * a.x, s * a.y, s * a.z}; }
+inline float dot(V3f a, V3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3f cross(V3f a, V3f b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+struct RayF {
+    V3f p, d;
+    float opl;
+};
+
+// prefilter_min_margin / prefilter_failure_linear / prefilter_failure_root (prefilter_ray.hpp:36-56)
+inline float min_margin(float acc, float x) {
+    if (x < acc) return x;
+    return x >= acc ? acc : 0.0f;
+}
+inline float failure_linear(float x) { return -x > 0.0f ? -x : 0.0f; }
+inline float failure_root(float q) { return -q > 0.0f ? std::sqrt(-q) : 0.0f; }
+
+struct TableF {
+    int S;
+    const uint8_t* type;
+    float c[MKO_MAX_SURFACES], t[MKO_MAX_SURFACES], n[MKO_MAX_SURFACES]; // one wavelength
+};
+
+inline float curv_of(const TableF& tb, int s) {
+    return tb.type[s] == MKO_SURFACE_SPHERE ? tb.c[s] : 0.0f;
+}
+
+struct RayResult {
+    Status status;
+    float opl, margin;
+};
+
+// prefilter_ray (prefilter_ray.hpp:171-249) over the float surface table of one wavelength; the
+// trace steps are those of surface_ops.hpp:122-203 in float, hard failure.
+RayResult prefilter_ray(const TableF& tb, RayF ray, V3f center, int kind, float kappa) {
+    RayResult out{Ok, 0.0f, 0.0f};
+    float margin = 1.0f;
+    for (int s = 1; s < tb.S; ++s) {
+        const float c = curv_of(tb, s);
+        const float nb = tb.n[s - 1], na = tb.n[s];
+        if (!(ray.d.z > 0.0f)) {
+            out.status = Backward;
+            out.margin = failure_linear(ray.d.z);
+            return out;
+        }
+        const V3f d_before = ray.d;
+        // transfer_to_tangent_plane
+        ray.p.z -= tb.t[s - 1];
+        const float s0 = -ray.p.z / ray.d.z;
+        ray.p = ray.p + s0 * ray.d;
+        ray.p.z = 0.0f;
+        // intersect_sphere
+        const float H = c * (ray.p.x * ray.p.x + ray.p.y * ray.p.y);
+        const float B = ray.d.z - c * (ray.p.x * ray.d.x + ray.p.y * ray.d.y);
+        const float D = B * B - c * H;
+        const float denom = D < 0.0f ? 0.0f : B + std::sqrt(D);
+        if (D < 0.0f || !(denom > 0.0f)) {
+            const float a = failure_root(D), b = failure_linear(B);
+            out.status = Missed;
+            out.margin = a > b ? a : b;
+            return out;
+        }
+        const float sh = H / denom;
+        ray.p = ray.p + sh * ray.d;
+        ray.opl += nb * (s0 + sh);
+        // refract_sphere
+        const float mu = nb / na;
+        const V3f N{-c * ray.p.x, -c * ray.p.y, 1.0f - c * ray.p.z};
+        const float cosI = dot(ray.d, N);
+        const float k = 1.0f - mu * mu * (1.0f - cosI * cosI);
+        if (k < 0.0f) {
+            out.status = TIR;
+            out.margin = failure_root(k);
+            return out;
+        }
+        ray.d = mu * ray.d + (std::sqrt(k) - mu * cosI) * N;
+        // PrefilterMarginObserver (prefilter_ray.hpp:68-81)
+        margin = min_margin(margin, d_before.z);
+        margin = min_margin(margin, dot(d_before, N));
+        margin = min_margin(margin, dot(ray.d, N));
+    }
+    if (!(ray.d.z > 0.0f)) {
+        out.status = Backward;
+        out.margin = failure_linear(ray.d.z);
+        return out;
+    }
+    margin = min_margin(margin, ray.d.z);
+    const float n_image = tb.n[tb.S - 1];
+    if (kind != MKO_REF_LAST_SURFACE) {
+        // image_sphere_step (reference_sphere.hpp:110-143), hard
+        const V3f q = ray.p - center;
+        const V3f ci = cross(q, ray.d);
+        const float disc_i = 1.0f + kappa * kappa * (-dot(ci, ci));
+        const float b = -dot(q, ray.d);
+        const float t = -dot(ci, ci);
+        const float disc = 1.0f + kappa * kappa * t;
+        if (!(disc >= 0.0f)) {
+            out.status = Missed;
+            out.margin = disc_i < 0.0f ? failure_root(disc_i) : 0.0f;
+            return out;
+        }
+        const float step = b - kappa * t / (std::sqrt(disc) + 1.0f);
+        if (!(step >= 0.0f) && !(step < 0.0f)) {
+            out.status = Missed;
+            out.margin = disc_i < 0.0f ? failure_root(disc_i) : 0.0f;
+            return out;
+        }
+        ray.opl += n_image * step;
+        out.opl = ray.opl;
+        out.margin = min_margin(margin, std::sqrt(disc_i));
+        return out;
+    }
+    // the reference sphere's checks (reference_sphere_step, reference_sphere.hpp:56-90)
+    const float r2 = dot(center, center);
+    const float inv_r = r2 > 0.0f ? 1.0f / std::sqrt(r2) : 1.0f;
+    const V3f q = ray.p - center;
+    const float bq = -dot(q, ray.d);
+    const float g = dot(ray.p, ray.p - 2.0f * center);
+    const float disc = bq * bq - g;
+    const float sigma = center.z >= 0.0f ? 1.0f : -1.0f;
+    const float root = disc > 0.0f ? std::sqrt(disc) : 0.0f;
+    const float den = sigma * (bq + sigma * root) * inv_r;
+    bool ok = !(disc < 0.0f);
+    float s_ref = 0.0f;
+    if (ok) {
+        const float rt = std::sqrt(disc);
+        const bool real_image = center.z >= 0.0f;
+        const float dn = real_image ? bq + rt : bq - rt;
+        ok = real_image ? dn > 0.0f : dn < 0.0f;
+        if (ok) s_ref = g / dn;
+    }
+    if (!ok) {
+        const float a = failure_root(disc * inv_r * inv_r);
+        const float c = disc >= 0.0f ? failure_linear(den) : 0.0f;
+        out.status = Missed;
+        out.margin = a > c ? a : c;
+        return out;
+    }
+    ray.opl += n_image * s_ref;
+    margin = min_margin(margin, root * inv_r);
+    margin = min_margin(margin, den);
+    out.opl = ray.opl;
+    out.margin = margin;
+    return out;
+}
+
+struct Stats {
+    double merit = 0.0, optical = 0.0;
+    int64_t failed = 0, uncertain_ok = 0, uncertain_failed = 0;
+};
+
+// variant_stats (src/batch/batch_prefilter.cpp:151-222): float quadrature, float source ray,
+// classes 0 certain valid / 1 certain failed / 2, 3 uncertain, piston over the certain-valid
+// rays, tuple sums in float added in double.
+Stats variant_stats(const Batch& B, int v, const Layout& lay, double boundary) {
+    const int S = B.S(), L = B.L();
+    const float eta = float(boundary);
+    Stats out;
+    double merit = 0.0;
+    float optical = 0.0f;
+    TableF tb;
+    tb.S = S;
+    tb.type = B.b.type;
+    for (int f = 0; f < lay.F; ++f) {
+        const Quadrature& qd = lay.quad[f];
+        const int n = lay.samples[f];
+        const size_t nn = static_cast<size_t>(n);
+        std::vector<float> opl(nn), wts(nn), rho(nn), phi(nn);
+        std::vector<uint8_t> cls(nn);
+        for (int k = 0; k < n; ++k) {
+            // sample() of PupilQuadrature<float> (quadrature.hpp:74-95)
+            const int i = k / qd.spokes, j = k - i * qd.spokes;
+            const float pi = float(3.14159265358979323846264338327950288);
+            rho[size_t(k)] = std::sqrt(float(qd.u[i]));
+            wts[size_t(k)] = float(qd.w[i]) / float(qd.spokes);
+            phi[size_t(k)] = qd.mode == MKO_PUPIL_HALF   ? float(2 * j + 1) * pi / float(2 * qd.spokes)
+                             : qd.mode == MKO_PUPIL_FULL ? float(2 * j + 1) * pi / float(qd.spokes)
+                                                         : 0.0f;
+        }
+        for (int l = 0; l < L; ++l) {
+            for (int s = 0; s < S; ++s) {
+                tb.c[s] = float(B.b.curvature[size_t(v) * S + s]);
+                tb.t[s] = float(B.b.thickness[size_t(v) * S + s]);
+                tb.n[s] = float(B.b.index[(size_t(v) * S + s) * L + l]);
+            }
+            // prefilter_opl_offset (prefilter_ray.hpp:264-272)
+            float offset = 0.0f;
+            for (int s = 0; s + 1 < S; ++s) offset += tb.n[s] * tb.t[s];
+            const size_t vf = size_t(v) * B.F() + f;
+            const V3f center{0.0f, float(B.b.field_target[vf]), tb.t[S - 1]};
+            const float kappa =
+                B.b.reference_sphere != MKO_REF_LAST_SURFACE ? float(B.b.reference_curvature[vf]) : 0.0f;
+            float weight_ok = 0.0f, weighted_opl = 0.0f;
+            for (int k = 0; k < n; ++k) {
+                RayF ray;
+                if (B.b.source == MKO_SOURCE_POINT) {
+                    const float sin_max = float(B.b.source_na) / tb.n[0];
+                    const float sr = sin_max * rho[size_t(k)];
+                    ray.p = V3f{float(B.b.source_point[0]), float(B.b.source_point[1]),
+                                float(B.b.source_point[2])};
+                    ray.d = V3f{sr * std::sin(phi[size_t(k)]), sr * std::cos(phi[size_t(k)]),
+                                std::sqrt((1.0f - sr) * (1.0f + sr))};
+                    ray.opl = 0.0f;
+                } else {
+                    const float a = float(B.b.pupil_radius[v]);
+                    const float th = float(B.b.field_angle[vf]);
+                    const float r = a * rho[size_t(k)];
+                    ray.p = V3f{r * std::sin(phi[size_t(k)]), r * std::cos(phi[size_t(k)]), 0.0f};
+                    ray.d = V3f{0.0f, std::sin(th), std::cos(th)};
+                    ray.opl = dot(ray.p, ray.d);
+                }
+                ray.opl -= offset;
+                const RayResult rr = prefilter_ray(tb, ray, center, B.b.reference_sphere, kappa);
+                const bool ok = rr.status == Ok;
+                const bool certain = rr.margin > eta;
+                const uint8_t c = uint8_t(ok ? (certain ? 0 : 2) : (certain ? 1 : 3));
+                cls[size_t(k)] = c;
+                opl[size_t(k)] = rr.opl;
+                if (c == 0) {
+                    weight_ok += wts[size_t(k)];
+                    weighted_opl += wts[size_t(k)] * rr.opl;
+                    const float len = std::abs(offset) + std::abs(rr.opl);
+                    optical = len > optical ? len : optical;
+                }
+                out.failed += c == 1;
+                out.uncertain_ok += c == 2;
+                out.uncertain_failed += c == 3;
+            }
+            const bool valid = weight_ok > 0.0f;
+            const float piston = valid ? weighted_opl / weight_ok : 0.0f;
+            const float scale = float(residual_scale(B, f, l));
+            float sum = 0.0f;
+            for (int k = 0; k < n; ++k) {
+                float e = 0.0f;
+                if (valid && cls[size_t(k)] == 0) e = opl[size_t(k)] - piston;
+                const float r = scale * std::sqrt(wts[size_t(k)]) * e;
+                sum += r * r;
+            }
+            merit += double(sum);
+        }
+    }
+    out.merit = merit;
+    out.optical = double(optical);
+    return out;
+}
+
+} // namespace pf
+
 thread_local std::string g_error;
 
 } // namespace orc
@@ -2128,6 +2388,27 @@ int64_t orc_residual_rows(const mko_batch* b, const mko_residual_settings* s) {
         orc::g_error = e.what();
         return -1;
     }
+}
+
+// Prefilter stats per variant (prefilter_stats_cpu, src/batch/batch_prefilter.cpp:384):
+// merit F32, optical length, certain-failed, uncertain-valid, uncertain-failed counts.
+int orc_prefilter_stats(const mko_batch* b, const mko_residual_settings* s, double boundary,
+                        double* merit, double* optical, int64_t* failed, int64_t* uncertain_ok,
+                        int64_t* uncertain_failed, int threads) {
+    ORC_TRY({
+        if (!(boundary >= 0.0) || !std::isfinite(boundary))
+            throw std::invalid_argument("boundary margin must be finite and >= 0");
+        const Batch B{*b};
+        const Layout lay = make_layout(B, *s);
+        parallel_for(0, b->variants, threads, [&](int v) {
+            const pf::Stats st = pf::variant_stats(B, v, lay, boundary);
+            merit[v] = st.merit;
+            optical[v] = st.optical;
+            failed[v] = st.failed;
+            uncertain_ok[v] = st.uncertain_ok;
+            uncertain_failed[v] = st.uncertain_failed;
+        });
+    })
 }
 
 int orc_fd_pattern(const mko_variables* vars, int surfaces, const double* x0, double* nominal,

@@ -15,6 +15,7 @@
 #include "mikroptim/optim/adjoint_jacobian.hpp"
 #include "mikroptim/optim/fd_jacobian.hpp"
 #include "mikroptim/batch/batch_extrema.hpp"
+#include "mikroptim/batch/batch_prefilter.hpp"
 #include "mikroptim/optim/levenberg_marquardt.hpp"
 #include "mikroptim/optim/mechanical_constraints.hpp"
 #include "mikroptim/optim/normal_equations.hpp"
@@ -463,6 +464,60 @@ int ref_optimize_lm(const mko_batch* base, const mko_residual_settings* s,
             out.penalty_rounds = q.penalty_rounds;
         }
         if (cycles_out) *cycles_out = r.cycles;
+    })
+}
+
+// prefilter_stats_cpu (src/batch/batch_prefilter.cpp:384).
+int ref_prefilter_stats(const mko_batch* b, const mko_residual_settings* s, double boundary,
+                        double* merit, double* optical, int64_t* failed, int64_t* uncertain_ok,
+                        int64_t* uncertain_failed) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::ResidualLayout layout(h.topology(), settings_of(*s));
+        const mko::PrefilterStats st = mko::prefilter_stats_cpu(h, layout, boundary);
+        std::copy(st.merit.begin(), st.merit.end(), merit);
+        std::copy(st.optical_length.begin(), st.optical_length.end(), optical);
+        std::copy(st.failed.begin(), st.failed.end(), failed);
+        std::copy(st.uncertain_ok.begin(), st.uncertain_ok.end(), uncertain_ok);
+        std::copy(st.uncertain_failed.begin(), st.uncertain_failed.end(), uncertain_failed);
+    })
+}
+
+// prefilter_decide (src/batch/batch_prefilter.cpp:236) on given stats; options[6] =
+// {max_failed_fraction, max_merit, max_merit_factor, boundary, opl_error_factor,
+// merit_relative}; extra [V] or NULL.
+int ref_prefilter_decide(const mko_batch* b, const mko_residual_settings* s, const double* options,
+                         const double* merit, const double* optical, const int64_t* failed,
+                         const int64_t* uncertain_ok, const int64_t* uncertain_failed,
+                         const double* extra, double* lower, double* upper, uint8_t* reject,
+                         double* reference_merit, int* reference_variant) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*b);
+        const mko::ResidualLayout layout(h.topology(), settings_of(*s));
+        const size_t V = size_t(b->variants);
+        mko::PrefilterStats st;
+        st.variants = b->variants;
+        st.rays_per_variant = layout.rays_per_variant();
+        st.merit.assign(merit, merit + V);
+        st.optical_length.assign(optical, optical + V);
+        st.failed.assign(failed, failed + V);
+        st.uncertain_ok.assign(uncertain_ok, uncertain_ok + V);
+        st.uncertain_failed.assign(uncertain_failed, uncertain_failed + V);
+        mko::PrefilterOptions o;
+        o.criteria.max_failed_fraction = options[0];
+        o.criteria.max_merit = options[1];
+        o.criteria.max_merit_factor = options[2];
+        o.margins.boundary = options[3];
+        o.margins.opl_error_factor = options[4];
+        o.margins.merit_relative = options[5];
+        const std::vector<double> ex = extra ? std::vector<double>(extra, extra + V)
+                                             : std::vector<double>();
+        const mko::PrefilterResult r = mko::prefilter_decide(std::move(st), h, layout, o, ex);
+        std::copy(r.merit_lower.begin(), r.merit_lower.end(), lower);
+        std::copy(r.merit_upper.begin(), r.merit_upper.end(), upper);
+        std::copy(r.reject.begin(), r.reject.end(), reject);
+        *reference_merit = r.reference_merit;
+        *reference_variant = r.reference_variant;
     })
 }
 

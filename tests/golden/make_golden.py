@@ -53,6 +53,13 @@ def main():
         J, r0, _ = fn(b.take([0, 1]), soft, v, x0)
         out[f"singlet_{key}_rows_J"] = J[:, :, T:]
         out[f"singlet_{key}_rows_r0"] = r0[:, T:]
+    # FP32 prefilter stats (prefilter_stats_cpu) of the multistart singlet batch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_prefilter import stats  # noqa: E402
+    mb, _ = systems.singlet_multistart(150, seed=11)
+    st = stats(ref, mb, systems.singlet_settings(abi.FAILURE_HARD))
+    for name in ("merit", "optical_length", "failed", "uncertain_ok", "uncertain_failed"):
+        out[f"prefilter_multistart_{name}"] = getattr(st, name)
     lb, lv = systems.singlet(8, seed=3)
     for key, method in (("fd", abi.JACOBIAN_FD), ("adjoint", abi.JACOBIAN_ADJOINT)):
         opt = abi.lm_options(max_iterations=8, threads=2, jacobian_method=method)
