@@ -8,6 +8,7 @@ curvature / thickness / working-distance part of VariableSet
 """
 
 import ctypes as C
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -188,8 +189,9 @@ def min_edge_thickness(surface, next_surface, limit, weight=1.0, clearance=0.0,
 
 def max_incidence_angle(surface, angle_deg, weight=1.0, shift=0.0):
     """limit = cos(theta_max) (make_mechanical_constraints)."""
-    return Constraint(abi.CON_MAX_INCIDENCE_ANGLE, weight, float(np.cos(np.deg2rad(angle_deg))),
-                      surface=surface, next_surface=surface, shift=shift)
+    limit = math.cos(angle_deg * 3.14159265358979323846 / 180.0)
+    return Constraint(abi.CON_MAX_INCIDENCE_ANGLE, weight, limit, surface=surface,
+                      next_surface=surface, shift=shift)
 
 
 @dataclass

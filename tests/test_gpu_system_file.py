@@ -1,0 +1,106 @@
+"""The reference's example system files on the HIP path (C-ABI of libmikroptim_hip.so): merit,
+constraint rows and the LM against the documented results of `mikroptim optimize` and the oracle.
+
+The documented values are CPU results of the reference; its own GPU run of the same file ends
+at 1.557334138e-6 against the CPU's 1.557334134e-6 (docs/cli.md:812), so the LM results here
+are held to the reference's CPU-GPU differences: merit within 1e-8 relative, same status. Rows
+that come from surface extrema follow CudaMechanical.TraceResidualsFromEngineMatchCpu
+(tests/test_cuda_mechanical.cpp:68): tolerance sqrt(w) * 2 * max(1e-12, 5e-12) per row."""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from mikroptim_amd import abi, rng
+from mikroptim_amd import system_file as sf
+from oracle_lib import oracle
+
+pytestmark = pytest.mark.gpu
+
+SYSTEMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "data", "systems")
+BIEGUNG = os.path.join(SYSTEMS, "einzellinse_bk7_biegung.json")
+MECHANIK = os.path.join(SYSTEMS, "zweilinser_mechanik.json")
+SOFT = os.path.join(SYSTEMS, "weiches_versagen_start.json")
+
+
+def test_biegung_merit_and_rows(engine):
+    ld = sf.load(BIEGUNG)
+    m, f, h = engine.merit(ld.batch, ld.settings)
+    assert abs(m[0] / 0.031345155357940434 - 1.0) <= 1e-12 and f[0] == 0
+    res, _, _ = engine.residuals(ld.batch, ld.settings)
+    assert res[0, -1] == 0.17689736616623206        # paraxial row: bit for bit
+
+
+@pytest.mark.parametrize("method", [abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT])
+def test_biegung_lm_reaches_documented_optimum(engine, method):
+    ld = sf.load(BIEGUNG)
+    b, v, s = ld.batch, ld.variables, ld.settings
+    x0 = v.extract(b, 0)[None, :]
+    r = engine.optimize(b, s, v, x0, abi.lm_options(jacobian_method=method))
+    assert r.status[0].startswith("converged")
+    assert abs(r.merit[0] / 1.557334133980554e-06 - 1.0) <= 1e-8
+    assert abs(r.x[0, 0] - 0.01672853972858829) <= 1e-6 * 0.0167
+
+
+def test_biegung_multistart_ends_in_one_optimum(engine):
+    ld = sf.load(BIEGUNG)
+    b, v, s = ld.batch, ld.variables, ld.settings
+    starts = rng.perturbed_vectors(v, v.extract(b, 0), 8, seed=1, relative=(0.2,) * 3,
+                                   first_is_base=True)
+    pts = b.take(np.zeros(8, dtype=np.int64))
+    v.apply(pts, starts)
+    m0, _, _ = engine.merit(pts, s)
+    doc = [3.134515536e-02, 1.544867972e+00, 5.204384846e+00, 4.067522266e+00,
+           1.154977668e+00, 4.200103488e+00, 7.160670488e+00, 2.021973972e+00]
+    assert np.all(np.abs(m0 / doc - 1.0) <= 1e-9)
+    r = engine.optimize(b, s, v, starts, abi.lm_options())
+    assert all(x.startswith("converged") for x in r.status)
+    assert np.all(np.abs(r.merit / 1.557334134e-06 - 1.0) <= 1e-8)
+
+
+def test_mechanical_rows_match_oracle(engine):
+    ld = sf.load(MECHANIK)
+    b, v, s = ld.batch, ld.variables, ld.settings
+    x = rng.perturbed_vectors(v, v.extract(b, 0), 64, seed=314, relative=(0.02,) * 3,
+                              first_is_base=True)
+    pts = b.take(np.zeros(64, dtype=np.int64))
+    v.apply(pts, x)
+    res, _, _ = engine.residuals(pts, s)
+    E = len(s.constraints)
+    rg = res[:, -E:]
+    rc = oracle().constraint_rows(pts, s, threads=8)
+    tol = math.sqrt(1e6) * 2.0 * max(1e-12, 1e-12 * 5.0)
+    assert np.max(np.abs(rg - rc)) <= tol
+    assert np.count_nonzero(rc[:, 1:]) > 20
+
+
+def test_soft_failure_file_becomes_failure_free(engine):
+    ld = sf.load(SOFT)
+    b, v, s = ld.batch, ld.variables, ld.settings
+    r = engine.optimize(b, s, v, v.extract(b, 0)[None, :], abi.lm_options())
+    run = r.runs[0]
+    assert r.status[0] == "converged_merit" and run.failed_rays == 0
+    assert run.penalty_rounds == 4 and run.fail_weight == 1e9
+    assert abs(run.merit / 1878.482892413148 - 1.0) <= 1e-6
+
+
+def test_optimize_report(engine):
+    rep = sf.optimize(BIEGUNG, engine)
+    assert rep["format"] == "mikroptim.optimize/1" and rep["backend"] == "hip"
+    assert rep["system"] == "Einzellinse N-BK7, Biegung"
+    assert [x["name"] for x in rep["variables"]] == ["c[1]", "c[2]", "w"]
+    c = rep["constraints"][0]
+    assert (c["item"], c["type"], c["mode"], c["weight"], c["target"]) == (
+        0, "focal_length", "equal", 0.029, 100.0)
+    assert c["value_start"] == 101.04968094724019
+    assert c["residual_start"] == 0.17689736616623206
+    assert abs(c["value_result"] - 100.00000160346728) <= 1e-6
+    res = rep["result"]
+    assert res["converged"] and abs(res["start_merit"] / 0.031345155357940434 - 1.0) <= 1e-12
+    assert abs(res["merit"] / 1.557334133980554e-06 - 1.0) <= 1e-8
+    assert rep["settings"]["perturbation"] is None and rep["runs"][0]["run"] == 0
+    rep = sf.optimize(MECHANIK, engine, max_iterations=5, runs=2, seed=3)
+    assert len(rep["mechanical_constraints"]) == 13 and rep["best_run"] in (0, 1)
+    assert rep["settings"]["perturbation"] == 0.05 and rep["settings"]["seed"] == 3
