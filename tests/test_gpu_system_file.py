@@ -41,7 +41,9 @@ def test_biegung_lm_reaches_documented_optimum(engine, method):
     r = engine.optimize(b, s, v, x0, abi.lm_options(jacobian_method=method))
     assert r.status[0].startswith("converged")
     assert abs(r.merit[0] / 1.557334133980554e-06 - 1.0) <= 1e-8
-    assert abs(r.x[0, 0] - 0.01672853972858829) <= 1e-6 * 0.0167
+    # flat valley: the reference's own GPU run ends with a shape factor 5.5e-5 relative off the
+    # CPU's (q = 0.720768 against 0.720808, docs/cli.md:812)
+    assert abs(r.x[0, 0] - 0.01672853972858829) <= 1e-4 * 0.0167
 
 
 def test_biegung_multistart_ends_in_one_optimum(engine):
