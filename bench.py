@@ -67,6 +67,7 @@ def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps,
     dt = time.perf_counter() - t0
     kt = eng.kernel_timing(False)
     res = sess.results()
+    run_lm.graph_cycles = sess.graph_cycles()
     sess.close()
     jacobians = int(sum(r.jacobians for r in res.runs)) - jac0
     return done, dt, kt, res, jacobians
@@ -140,6 +141,7 @@ def main():
                                            abi.JACOBIAN_ADJOINT, "mixed", args.warmup,
                                            args.steps, args.threads)
     it_adj = n_adj / t_adj
+    graph_adj = run_lm.graph_cycles
     fd_steps = args.steps if args.fd_steps is None else args.fd_steps
     n_fd, t_fd, kt_fd, res_fd, jac_fd = run_lm(eng, base, settings, variables, x0, abi.JACOBIAN_FD,
                                        "fp64", args.warmup, fd_steps, args.threads)
@@ -246,6 +248,8 @@ def main():
                    "rays_per_variant": rays},
         "fd_fp64_iter_per_s": it_fd, "fd_steps": n_fd,
         "jacobians_in_timed_cycles": {"adjoint_mixed": jac_adj, "fd_fp64": jac_fd},
+        # cycles of the adjoint + mixed LM session that ran as replays of its captured hipGraph
+        "lm_graph_cycles_adjoint_mixed": graph_adj,
         "mixed_normal_equations_error_vs_fp64": mixed_err,
         "merit_trace_rays_per_s": trace_rate,
         "constraint_rows": rows,

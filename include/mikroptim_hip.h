@@ -216,6 +216,10 @@ mko_lm* mko_lm_create(mko_engine* engine, const mko_batch* base,
 int mko_lm_cycle(mko_lm* lm);
 int mko_lm_results(mko_lm* lm, double* x_out, mko_lm_run* run_out, int* cycles_out);
 void mko_lm_destroy(mko_lm* lm);
+/* Number of the session's cycles that ran as one replay of a captured hipGraph (device control
+ * with the fused mixed-precision adjoint, see mko_lm_options.host_control); no counterpart in the
+ * reference. */
+int mko_lm_graph_cycles(const mko_lm* lm);
 
 #ifdef __cplusplus
 }

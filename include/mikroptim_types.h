@@ -182,8 +182,11 @@ typedef struct mko_lm_options {
     double fail_weight_factor; /* 10 */
     double max_fail_weight;    /* 1e14 */
     /* 0: the per-cycle control (step solves, acceptance, convergence tests) runs on the device
-     * for problems of at most 64 variables; 1: always on the host. Both take the same steps.
-     * No counterpart in LmOptions (the reference's control always runs on the host). */
+     * for problems of at most 64 variables, and with the fused mixed-precision adjoint (hard
+     * failure, no constraint rows) each cycle is one replay of a hipGraph captured per round;
+     * 1: control always on the host; 2: device control launched kernel by kernel (no graph).
+     * All take the same steps. No counterpart in LmOptions (the reference's control always runs
+     * on the host). */
     int32_t host_control;
 } mko_lm_options;
 

@@ -123,12 +123,14 @@ class LmRun(C.Structure):
 def lm_options(max_iterations=100, ftol=1e-10, xtol=1e-10, gtol=1e-10, stall_window=0,
                stall_tolerance=1e-3, initial_damping=1e-3, max_damping=1e16, diag_floor=1e-12,
                reject_new_failures=True, threads=0, jacobian_method=JACOBIAN_FD,
-               fail_weight_factor=10.0, max_fail_weight=1e14, host_control=False):
+               fail_weight_factor=10.0, max_fail_weight=1e14, host_control=False, graph=True):
     """LmOptions with the reference defaults (include/mikroptim/optim/levenberg_marquardt.hpp:85);
-    host_control selects the host-side LM control of this library (default: device)."""
+    host_control selects the host-side LM control of this library (default: device); graph=False
+    launches the device cycle kernel by kernel instead of replaying a captured hipGraph."""
+    mode = 1 if host_control else (0 if graph else 2)
     return LmOptions(max_iterations, ftol, xtol, gtol, stall_window, stall_tolerance,
                      initial_damping, max_damping, diag_floor, int(reject_new_failures), threads,
-                     jacobian_method, fail_weight_factor, max_fail_weight, int(host_control))
+                     jacobian_method, fail_weight_factor, max_fail_weight, mode)
 
 
 def ptr(array, ctype=C.c_double):

@@ -431,6 +431,11 @@ public:
         hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
         pending_.push_back(Pending{fam, a, b});
     }
+    // time of a launch measured elsewhere (events recorded inside a captured graph)
+    void add(KernelFamily fam, float t) {
+        ms[fam] += t;
+        ++launches[fam];
+    }
     void collect() {
         for (const Pending& p : pending_) {
             hip_check(hipEventSynchronize(p.b), "hipEventSynchronize");
@@ -1366,6 +1371,7 @@ void solve_step(Run& s, const Bounds& b, size_t n, double mu) {
 
 class Lm {
 public:
+    ~Lm() { graph_release(); }
     Lm(Engine& eng, const mko_batch& base, const mko_residual_settings& s, const mko_variables& v,
        int runs, const double* x0, const mko_lm_options& o, int precision)
         : eng_(eng), base_(base), settings_(s), vars_(v), runs_(runs), o_(o), precision_(precision) {
@@ -1404,7 +1410,8 @@ public:
         x0_.assign(x0, x0 + size_t(runs) * P_);
         weight_.assign(size_t(runs), s.fail_weight);
         rounds_.assign(size_t(runs), 1);
-        device_ = !o.host_control && P_ <= size_t(mko_lmdev::kMaxVariables);
+        device_ = o.host_control != 1 && P_ <= size_t(mko_lmdev::kMaxVariables);
+        graph_wanted_ = o.host_control == 0;
         if (device_) dev_setup();
         start_round(std::vector<int>(), x0_);
     }
@@ -1421,6 +1428,8 @@ public:
         }
         return true;
     }
+
+    int graph_cycles() const { return graph_cycles_; }
 
     void results(double* x, mko_lm_run* out, int* cycles) {
         if (device_ && !done_) dev_download_state();
@@ -1512,7 +1521,10 @@ private:
                          fl_.data(), hf_.data());
     }
 
-    bool round_cycle() { return device_ ? dev_round_cycle() : host_round_cycle(); }
+    bool round_cycle() {
+        if (!device_) return host_round_cycle();
+        return graph_eligible() ? graph_round_cycle() : dev_round_cycle();
+    }
 
     bool host_round_cycle() {
         const int n = int(state_.size());
@@ -1774,6 +1786,7 @@ private:
     }
 
     void dev_start_round(const std::vector<int>& ids, const std::vector<double>& xs) {
+        graph_reset(); // the round's state arrays may move
         ids_ = ids;
         const int n = ids.empty() ? runs_ : int(ids.size());
         state_.assign(size_t(n), Run());
@@ -1856,6 +1869,213 @@ private:
         return true;
     }
 
+    // ---- one LM cycle as a captured hipGraph ------------------------------------------------
+    struct GraphPart {
+        KernelFamily fam;
+        int which; // list whose count > 0 makes the part a timed launch (-1: always)
+        std::function<void()> launch;
+    };
+    struct GraphTimed {
+        KernelFamily fam;
+        int which;
+        hipGraphExec_t exec;
+        hipEvent_t a, b;
+    };
+    // Device control with the fused mixed-precision adjoint on stored ray states and no
+    // constraint rows (the batched LM of the headline workload): the cycle of dev_round_cycle
+    // without its three host reads of list lengths. Every launch is sized for the round's n runs
+    // and its kernels read the list length from device memory (`live` / `dcount`), so one
+    // capture per round serves every cycle; the host waits once per cycle for the three counts.
+    // The graph owns its work buffers (nothing the engine reallocates between cycles) and is
+    // rebuilt when a round starts. Same kernels, same inputs per run: the results are those of
+    // the eager cycle bit for bit.
+    bool graph_eligible() const {
+        return graph_wanted_ && !graph_failed_ && use_states_ && settings_.extra == nullptr &&
+               precision_ == MKO_PRECISION_MIXED && o_.jacobian_method == MKO_JACOBIAN_ADJOINT &&
+               Engine::fused_ok(settings_, int(P_));
+    }
+
+    void graph_reset() {
+        if (gexec_) (void)hipGraphExecDestroy(gexec_);
+        gexec_ = nullptr;
+        for (GraphTimed& t : gparts_) {
+            if (t.exec) (void)hipGraphExecDestroy(t.exec);
+            if (t.a) (void)hipEventDestroy(t.a);
+            if (t.b) (void)hipEventDestroy(t.b);
+        }
+        gparts_.clear();
+    }
+
+    void graph_release() {
+        graph_reset();
+        if (gs_) (void)hipStreamDestroy(gs_);
+        if (counts_host_) (void)hipHostFree(counts_host_);
+        gs_ = nullptr;
+        counts_host_ = nullptr;
+    }
+
+    // Captures `parts` of the cycle, in order, into one executable graph on gs_ (NULL on failure:
+    // nothing captured, nothing run).
+    hipGraphExec_t graph_capture_parts(const std::vector<const GraphPart*>& parts) {
+        hipGraph_t g = nullptr;
+        bool ok = true;
+        try {
+            hip_check(hipStreamBeginCapture(gs_, hipStreamCaptureModeRelaxed), "begin capture");
+            for (const GraphPart* p : parts) p->launch();
+            hip_check(hipGetLastError(), "graph cycle launch");
+        } catch (const std::exception&) {
+            ok = false;
+        }
+        const hipError_t end = hipStreamEndCapture(gs_, &g);
+        ok = ok && end == hipSuccess && g != nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (ok) ok = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess;
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            (void)hipGetLastError(); // clear the capture's error state
+            return nullptr;
+        }
+        return exec;
+    }
+
+    // The cycle as one graph (gexec_) and, for the engine's kernel timing, as one graph per part
+    // (HIP events recorded between their launches on gs_); false on failure.
+    bool graph_capture() {
+        const int n = ds_.n;
+        const mko_batch& tmpl = dev_tmpl_.view;
+        if (!gs_) hip_check(hipStreamCreateWithFlags(&gs_, hipStreamNonBlocking), "graph stream");
+        if (!counts_host_)
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&counts_host_), 16, hipHostMallocDefault),
+                      "pinned counts");
+        // buffers of n runs, allocated before the capture (within half of the memory budget;
+        // else the eager cycle, which tranches)
+        const size_t P = P_, N = size_t(n);
+        const size_t T = size_t(tmpl.fields) * tmpl.wavelengths;
+        const size_t wbytes = mko_adjoint_normal_workspace(n, tmpl.fields, tmpl.wavelengths, int(P));
+        const size_t need =
+            wbytes + N * (Strides(tmpl).doubles() * 8 + P * P * 8 + P * 16 + T * 16 + 64);
+        if (need > eng_.budget() / 2) return false;
+        gwork_.allocate(tmpl, n);
+        void* ws = gws_.reserve(wbytes);
+        double* ac = gac_.as<double>(N * P * P);
+        double* gc = ggc_.as<double>(N * P);
+        double* xs = gxs_.as<double>(N * P);
+        const int32_t* rix = dev_round_index();
+        int32_t* tidx = rix ? gtidx_.as<int32_t>(N) : nullptr;
+        double* fm = gfm_.as<double>(N);
+        int64_t* ff = gff_.as<int64_t>(N);
+        int64_t* fh = gfh_.as<int64_t>(N);
+        double* part = gpart_.as<double>(N * T);
+        int* pc = gpc_.as<int>(N * T * 2);
+        int32_t* vt = gvt_.as<int32_t>(P);
+        int32_t* vs = gvs_.as<int32_t>(P);
+        hip_check(hipMemcpy(vt, vars_.type, P * 4, hipMemcpyHostToDevice), "graph vars");
+        hip_check(hipMemcpy(vs, vars_.batch_surface, P * 4, hipMemcpyHostToDevice), "graph vars");
+        mko_variables dv = vars_;
+        dv.type = vt;
+        dv.batch_surface = vs;
+        // the round's start (null stream) is complete before the first replay on gs_
+        hip_check(hipStreamSynchronize(nullptr), "graph start");
+        int32_t* list_j = lists_;
+        int32_t* list_c = lists_ + n;
+        int32_t* list_e = lists_ + 2 * n;
+        int32_t* pos = lists_ + 3 * n;
+        const int32_t *cj = counts_ + 0, *cc = counts_ + 1, *ce = counts_ + 2;
+        mko_internal::RayStates st_j = states_, st_e = states_;
+        st_j.slot = list_j;
+        st_e.slot = list_e;
+        auto point_batch = [this, &tmpl, tidx, n, dv, xs](const int32_t* live) {
+            dev_check(mko_internal::point_batch(
+                &tmpl, tidx, 0, n, 1, &dv, xs, nullptr, nullptr, gwork_.mut(gwork_.view.curvature),
+                gwork_.mut(gwork_.view.thickness), gwork_.mut(gwork_.view.index),
+                gwork_.mut(gwork_.view.pupil_radius), gwork_.mut(gwork_.view.field_angle),
+                gwork_.mut(gwork_.view.field_target), gwork_.mut(gwork_.view.reference_curvature),
+                gwork_.mut(gwork_.view.paraxial_index), gs_, live));
+        };
+        // the parts of the cycle in order; `which`: the list whose count > 0 makes the part a
+        // timed launch, as in the eager cycle (-1: always)
+        const std::vector<GraphPart> parts = {
+            {kLmControl, -1, [&] {
+                 mko_lmdev::launch_select(ds_, mko_lmdev::kListJacobian, o_.max_iterations,
+                                          nullptr, nullptr, list_j, counts_ + 0, nullptr, gs_);
+                 mko_lmdev::launch_gather(ds_, list_j, n, false, rix, xs, tidx, gs_, cj);
+             }},
+            {kPointBatch, 0, [&] { point_batch(cj); }},
+            {kAdjoint, 0, [&] {
+                 dev_check(mko_internal::adjoint_normal_states(&gwork_.view, &settings_, &vars_,
+                                                               ac, gc, ws, wbytes, st_j, gs_, cj));
+             }},
+            {kLmControl, -1, [&] {
+                 mko_lmdev::launch_prepare(ds_, dopt_, list_j, n, ac, gc, gs_, cj);
+                 mko_lmdev::launch_select(ds_, mko_lmdev::kListCycle, o_.max_iterations, nullptr,
+                                          nullptr, list_c, counts_ + 1, nullptr, gs_);
+                 mko_lmdev::launch_step(ds_, dopt_, list_c, n, gs_, cc);
+                 mko_lmdev::launch_select(ds_, mko_lmdev::kListEvaluate, o_.max_iterations,
+                                          list_c, counts_ + 1, list_e, counts_ + 2, pos, gs_);
+                 mko_lmdev::launch_gather(ds_, list_e, n, true, rix, xs, tidx, gs_, ce);
+             }},
+            {kPointBatch, 2, [&] { point_batch(ce); }},
+            {kTrace, 2, [&] {
+                 dev_check(mko_internal::merit_states(&gwork_.view, &settings_, fm, ff, fh, part,
+                                                      pc, st_e, gs_, ce));
+             }},
+            {kLmControl, -1, [&] {
+                 mko_lmdev::launch_update(ds_, dopt_, list_c, n, pos, fm, ff, fh, gs_, cc);
+             }},
+        };
+        std::vector<const GraphPart*> all;
+        for (const GraphPart& p : parts) all.push_back(&p);
+        gexec_ = graph_capture_parts(all);
+        if (!gexec_) return false;
+        for (const GraphPart& p : parts) {
+            GraphTimed t{p.fam, p.which, graph_capture_parts({&p}), nullptr, nullptr};
+            if (!t.exec) {
+                graph_reset();
+                return false;
+            }
+            hip_check(hipEventCreate(&t.a), "graph event");
+            hip_check(hipEventCreate(&t.b), "graph event");
+            gparts_.push_back(t);
+        }
+        return true;
+    }
+
+    bool graph_round_cycle() {
+        if (!gexec_ && !graph_capture()) {
+            graph_reset();
+            graph_failed_ = true; // eager cycles for the rest of this LM
+            return dev_round_cycle();
+        }
+        const bool timed = eng_.timer.enabled;
+        if (timed) {
+            for (const GraphTimed& t : gparts_) {
+                hip_check(hipEventRecord(t.a, gs_), "hipEventRecord");
+                hip_check(hipGraphLaunch(t.exec, gs_), "graph launch");
+                hip_check(hipEventRecord(t.b, gs_), "hipEventRecord");
+            }
+        } else {
+            hip_check(hipGraphLaunch(gexec_, gs_), "graph launch");
+        }
+        hip_check(hipMemcpyAsync(counts_host_, counts_, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 gs_),
+                  "LM counts");
+        hip_check(hipStreamSynchronize(gs_), "LM cycle");
+        ++graph_cycles_;
+        // per family the launches that had work, as the eager cycle times them
+        if (timed) {
+            for (const GraphTimed& t : gparts_) {
+                if (t.which >= 0 && counts_host_[t.which] == 0) continue;
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, t.a, t.b), "graph timing");
+                eng_.timer.add(t.fam, ms);
+            }
+        }
+        if (counts_host_[1] == 0) return false;
+        ++cycles_;
+        return true;
+    }
+
     // Device state -> state_ (the fields finish_round and results read).
     void dev_download_state() {
         const size_t n = size_t(ds_.n);
@@ -1926,6 +2146,16 @@ private:
     std::vector<int64_t> fl_, hf_;
     GatheredBatch gathered_batch_;
     const mko_batch* gathered_ = nullptr;
+    // captured cycle
+    bool graph_wanted_ = false;
+    bool graph_failed_ = false;
+    int graph_cycles_ = 0;
+    hipStream_t gs_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    int32_t* counts_host_ = nullptr;
+    std::vector<GraphTimed> gparts_;
+    DeviceBatch gwork_;
+    Buffer gws_, gac_, ggc_, gxs_, gtidx_, gfm_, gff_, gfh_, gpart_, gpc_, gvt_, gvs_;
 };
 
 } // namespace mko_host
@@ -2038,6 +2268,8 @@ int mko_lm_results(mko_lm* lm, double* x, mko_lm_run* out, int* cycles) {
 }
 
 void mko_lm_destroy(mko_lm* lm) { delete lm; }
+
+int mko_lm_graph_cycles(const mko_lm* lm) { return lm ? lm->lm.graph_cycles() : 0; }
 
 int mko_engine_normal_equations(mko_engine* e, const mko_batch* base,
                                 const mko_residual_settings* s, const mko_variables* v, int count,

@@ -21,15 +21,18 @@ struct RayStates {
 };
 
 // MERIT (mko_merit_device) that also writes the final ray states (hard failure mode, FP64).
+// `live` (device, NULL: all variants): only variants v < *live are traced and finalized; the
+// launch is sized for the batch, so a captured graph serves any count the device sets.
 int merit_states(const mko_batch* batch, const mko_residual_settings* settings, double* merit,
                  int64_t* failed, int64_t* hard_failed, double* partial, int* partial_counts,
-                 const RayStates& states, void* stream);
+                 const RayStates& states, void* stream, const int32_t* live = nullptr);
 
 // mko_adjoint_normal_device starting each ray's reverse sweep from the stored final state
 // instead of tracing it forward (the states must come from a MERIT trace of the same points).
 int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* settings,
                           const mko_variables* vars, double* a, double* g, void* workspace,
-                          size_t workspace_bytes, const RayStates& states, void* stream);
+                          size_t workspace_bytes, const RayStates& states, void* stream,
+                          const int32_t* live = nullptr);
 
 // mko_point_batch_device with an optional device array template_index [runs]: run i copies
 // template variant template_index[i] (NULL: as mko_point_batch_device).
@@ -37,7 +40,8 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                 int runs, int per_run, const mko_variables* vars_device, const double* x0,
                 const double* plus, const double* minus, double* curvature, double* thickness,
                 double* index, double* pupil_radius, double* field_angle, double* field_target,
-                double* reference_curvature, double* paraxial_index, void* stream);
+                double* reference_curvature, double* paraxial_index, void* stream,
+                const int32_t* live = nullptr);
 
 // RESIDUALS (mode 0: rows at `stride` >= trace rows) or MERIT partials (mode 1: partial [V*T],
 // partial_counts [2*V*T], finalized by merit_finalize) of a device batch, optionally with the

@@ -72,6 +72,9 @@ struct TraceParams {
     double src_na;
     uint64_t sphere_mask;
     uint64_t ext_mask; // surfaces whose extrema a kExt trace reduces
+    // device count of the variants to trace (NULL: all V): blocks of variants >= *live exit at
+    // once, so that a launch sized for V serves a captured LM cycle whose count the device sets
+    const int32_t* live;
     int64_t rays, trace_rows, stride;
     double gl_u[MKO_MAX_RINGS], gl_w[MKO_MAX_RINGS];
     double field_weight[kMaxFields];
@@ -390,6 +393,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     __shared__ SurfLds L;
     __shared__ double red[kWaves];
     __shared__ int ired[kWaves];
+    if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
     const Tuple t = tuple_of(P, blockIdx.x);
     stage_surfaces(P, t, L);
     const size_t vf = size_t(t.v) * P.F + t.f;
@@ -647,10 +651,10 @@ __global__ void rows_merit_kernel(int V, const double* rows, int64_t stride, int
 __global__ void merit_finalize_kernel(int V, int T, const double* partial, const int* pfailed,
                                       const int* phard, const double* extra,
                                       const double* traced, double* merit, int64_t* failed,
-                                      int64_t* hard) {
+                                      int64_t* hard, const int32_t* live) {
 #pragma clang fp contract(off)
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
+    if (v >= V || (live && v >= *live)) return;
     double s = 0.0;
     int64_t f = 0, h = 0;
     for (int t = 0; t < T; ++t) {
@@ -973,11 +977,11 @@ __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, 
                                    const int32_t* template_index, int template_per_run, int S,
                                    int L, int F, int P, const int* var_type,
                                    const int* var_surface, const double* x0, const double* plus,
-                                   const double* minus) {
+                                   const double* minus, const int32_t* live) {
     const int vv = blockIdx.x; // destination variant
     const int run = vv / per_run;
     const int j = vv - run * per_run;
-    if (run >= runs) return;
+    if (run >= runs || (live && run >= *live)) return;
     const int from = template_index ? template_index[run] : template_per_run ? run : 0;
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
         dst.curvature[size_t(vv) * S + i] = src.curvature[size_t(from) * S + i];
@@ -1328,6 +1332,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
     __shared__ float delta_f[kWaves][64];
     __shared__ double Sacc[64], Tacc[64];
     __shared__ double o_shift;
+    if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
     const Tuple t = tuple_of(P, blockIdx.x);
     stage_surfaces(P, t, L);
     const int Pn = map.P;
@@ -1551,10 +1556,11 @@ constexpr int kFinalizeEntries = (kMaxVariablesFused * (kMaxVariablesFused + 1) 
 
 __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
     int T, int Pn, const float* partC, const double* partST, const TupleNormal* partT, double* a,
-    double* g) {
+    double* g, const int32_t* live) {
     __shared__ double S_l[kFinalizeChunk][2 * kMaxVariablesFused];
     __shared__ double f1[kFinalizeChunk], f2[kFinalizeChunk], pist[kFinalizeChunk];
     const int v = blockIdx.x;
+    if (live && v >= *live) return;
     const int tid = threadIdx.x;
     const int nu = Pn * (Pn + 1) / 2;
     int eq[kFinalizeEntries], ep[kFinalizeEntries];
@@ -1896,7 +1902,8 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                 int runs, int per_run, const mko_variables* vars_device, const double* x0,
                 const double* plus, const double* minus, double* curvature, double* thickness,
                 double* index, double* pupil_radius, double* field_angle, double* field_target,
-                double* reference_curvature, double* paraxial_index, void* stream) {
+                double* reference_curvature, double* paraxial_index, void* stream,
+                const int32_t* live) {
     MKO_TRY({
         if (runs <= 0) return 0;
         const ConstBatchPtrs src{templ->curvature, templ->thickness, templ->index,
@@ -1908,16 +1915,18 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                            0, static_cast<hipStream_t>(stream), src, dst, runs, per_run,
                            template_index, template_per_run, templ->surfaces, templ->wavelengths,
                            templ->fields, vars_device->count, vars_device->type,
-                           vars_device->batch_surface, x0, plus, minus);
+                           vars_device->batch_surface, x0, plus, minus, live);
         check_hip(hipGetLastError(), "point batch launch");
     })
 }
 
 int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* settings,
                           const mko_variables* vars, double* a, double* g, void* workspace,
-                          size_t workspace_bytes, const RayStates& states, void* stream) {
+                          size_t workspace_bytes, const RayStates& states, void* stream,
+                          const int32_t* live) {
     MKO_TRY({
-        const TraceParams P = make_params(*batch, *settings);
+        TraceParams P = make_params(*batch, *settings);
+        P.live = live;
         const ParamMap m = make_map(*vars, batch->surfaces);
         if (P.soft) throw std::invalid_argument("fused adjoint normal equations need hard failure mode");
         if (m.P > 64) throw std::invalid_argument("fused adjoint normal equations: at most 64 variables");
@@ -1956,16 +1965,17 @@ int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* s
         }
         check_hip(hipGetLastError(), "adjoint normal launch");
         hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
-                           m.P, partC, partST, partT, a, g);
+                           m.P, partC, partST, partT, a, g, live);
         check_hip(hipGetLastError(), "adjoint normal finalize launch");
     })
 }
 
 int merit_states(const mko_batch* batch, const mko_residual_settings* settings, double* merit,
                  int64_t* failed, int64_t* hard_failed, double* partial, int* partial_counts,
-                 const RayStates& states, void* stream) {
+                 const RayStates& states, void* stream, const int32_t* live) {
     MKO_TRY({
-        const TraceParams P = make_params(*batch, *settings);
+        TraceParams P = make_params(*batch, *settings);
+        P.live = live;
         if (P.soft || P.precision != MKO_PRECISION_FP64)
             throw std::invalid_argument("ray states need hard failure mode and the FP64 trace");
         if (states.rays != P.rays) throw std::invalid_argument("ray states of another layout");
@@ -1975,7 +1985,7 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
         int* ph = partial_counts + size_t(P.V) * P.T;
         launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, nullptr, s, &states);
         hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
-                           P.T, partial, pf, ph, nullptr, nullptr, merit, failed, hard_failed);
+                           P.T, partial, pf, ph, nullptr, nullptr, merit, failed, hard_failed, live);
         check_hip(hipGetLastError(), "merit finalize launch");
     })
 }
@@ -2045,7 +2055,7 @@ int merit_finalize(int V, int T, const double* partial, const int* partial_count
         hipLaunchKernelGGL(merit_finalize_kernel, dim3((V + 255) / 256), dim3(256), 0,
                            static_cast<hipStream_t>(stream), V, T, partial, partial_counts,
                            partial_counts + size_t(V) * T, extra, traced, merit, failed,
-                           hard_failed);
+                           hard_failed, nullptr);
         check_hip(hipGetLastError(), "merit finalize launch");
     })
 }
@@ -2122,7 +2132,8 @@ int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settin
         int* ph = partial_counts + size_t(P.V) * P.T;
         launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, nullptr, s);
         hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
-                           P.T, partial, pf, ph, extra_merit, nullptr, merit, failed, hard_failed);
+                           P.T, partial, pf, ph, extra_merit, nullptr, merit, failed, hard_failed,
+                           nullptr);
         check_hip(hipGetLastError(), "merit finalize launch");
     })
 }

@@ -305,10 +305,11 @@ __global__ __launch_bounds__(1024) void select_kernel(State S, int kind, int max
 }
 
 __global__ __launch_bounds__(64) void gather_kernel(State S, const int32_t* list, int count,
-                                                    int trial, const int32_t* round_index,
-                                                    double* xs, int32_t* tidx) {
+                                                    const int32_t* dcount, int trial,
+                                                    const int32_t* round_index, double* xs,
+                                                    int32_t* tidx) {
     const int j = blockIdx.x;
-    if (j >= count) return;
+    if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
     const double* src = (trial ? S.trial : S.x) + size_t(run) * S.P;
     for (int k = threadIdx.x; k < S.P; k += 64) xs[size_t(j) * S.P + k] = src[k];
@@ -317,13 +318,14 @@ __global__ __launch_bounds__(64) void gather_kernel(State S, const int32_t* list
 
 // One wavefront per listed run (round_cycle, the runs that received new normal equations).
 __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, const int32_t* list,
-                                                         int count, const double* a_c,
+                                                         int count, const int32_t* dcount,
+                                                         const double* a_c,
                                                          const double* g_c) {
     __shared__ SolveLds L;
     __shared__ int finite_s;
     extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles
     const int j = blockIdx.x;
-    if (j >= count) return;
+    if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
     const int n = S.P, lane = threadIdx.x;
     bool finite = true;
@@ -365,11 +367,11 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
 }
 
 __global__ __launch_bounds__(kLanes) void step_kernel(State S, Options o, const int32_t* list,
-                                                      int count) {
+                                                      int count, const int32_t* dcount) {
     __shared__ SolveLds L;
     extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles
     const int j = blockIdx.x;
-    if (j >= count) return;
+    if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
     solve_step(S, run, S.a + size_t(run) * S.P * S.P, S.g + size_t(run) * S.P, S.mu[run], L,
                m_tri);
@@ -384,11 +386,11 @@ __global__ __launch_bounds__(kLanes) void step_kernel(State S, Options o, const 
 }
 
 __global__ __launch_bounds__(256) void update_kernel(State S, Options o, const int32_t* list,
-                                                     int count, const int32_t* pos,
-                                                     const double* f, const int64_t* failed,
-                                                     const int64_t* hard) {
+                                                     int count, const int32_t* dcount,
+                                                     const int32_t* pos, const double* f,
+                                                     const int64_t* failed, const int64_t* hard) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= count) return;
+    if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
     if (S.status[run] == MKO_LM_CONVERGED_STEP) return;
     S.iterations[run] += 1;
@@ -499,31 +501,33 @@ void launch_select(const State& s, int kind, int max_iterations, const int32_t* 
 }
 
 void launch_gather(const State& s, const int32_t* list, int count, bool trial,
-                   const int32_t* round_index, double* xs, int32_t* tidx, void* stream) {
+                   const int32_t* round_index, double* xs, int32_t* tidx, void* stream,
+                   const int32_t* dcount) {
     if (count > 0)
         hipLaunchKernelGGL(gather_kernel, dim3(unsigned(count)), dim3(64), 0, st(stream), s, list,
-                           count, trial ? 1 : 0, round_index, xs, tidx);
+                           count, dcount, trial ? 1 : 0, round_index, xs, tidx);
 }
 
 void launch_prepare(const State& s, const Options& o, const int32_t* list, int count,
-                    const double* a_c, const double* g_c, void* stream) {
+                    const double* a_c, const double* g_c, void* stream, const int32_t* dcount) {
     if (count > 0)
         hipLaunchKernelGGL(prepare_kernel, dim3(unsigned(count)), dim3(kLanes), tri_bytes(s.P),
-                           st(stream), s, o, list, count, a_c, g_c);
+                           st(stream), s, o, list, count, dcount, a_c, g_c);
 }
 
-void launch_step(const State& s, const Options& o, const int32_t* list, int count, void* stream) {
+void launch_step(const State& s, const Options& o, const int32_t* list, int count, void* stream,
+                 const int32_t* dcount) {
     if (count > 0)
         hipLaunchKernelGGL(step_kernel, dim3(unsigned(count)), dim3(kLanes), tri_bytes(s.P),
-                           st(stream), s, o, list, count);
+                           st(stream), s, o, list, count, dcount);
 }
 
 void launch_update(const State& s, const Options& o, const int32_t* list, int count,
                    const int32_t* pos, const double* f, const int64_t* failed,
-                   const int64_t* hard, void* stream) {
+                   const int64_t* hard, void* stream, const int32_t* dcount) {
     if (count > 0)
         hipLaunchKernelGGL(update_kernel, dim3(blocks_of(count, 256)), dim3(256), 0, st(stream), s,
-                           o, list, count, pos, f, failed, hard);
+                           o, list, count, dcount, pos, f, failed, hard);
 }
 
 void launch_fd_pattern(int count, int P, const double* xs, const double* lower,

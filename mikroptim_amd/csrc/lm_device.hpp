@@ -32,6 +32,10 @@ struct Options {
 
 enum ListKind { kListJacobian = 0, kListCycle = 1, kListEvaluate = 2 };
 
+// gather / prepare / step / update take the list length as `count`, or, with `dcount` (device,
+// NULL: unused), as *dcount <= count: `count` then only sizes the launch, so that a captured
+// cycle works for whatever list lengths the device selects.
+
 // Sets every run to the start of a round at x (already in state.x).
 void launch_init(const State& s, const Options& o, void* stream);
 // After the merit of x: merit / counts, first history entry, NUMERICAL_ERROR / CONVERGED_MERIT.
@@ -47,18 +51,21 @@ void launch_select(const State& s, int kind, int max_iterations, const int32_t* 
                    void* stream);
 // xs[j] = (trial ? state.trial : state.x)[list[j]], tidx[j] = round_index[list[j]].
 void launch_gather(const State& s, const int32_t* list, int count, bool trial,
-                   const int32_t* round_index, double* xs, int32_t* tidx, void* stream);
+                   const int32_t* round_index, double* xs, int32_t* tidx, void* stream,
+                   const int32_t* dcount = nullptr);
 // New normal equations a_c [count][P][P], g_c [count][P] of the listed runs: copy, finite check,
 // prepare_step, the model step (damping 1e-12) and the gradient / merit / step convergence tests.
 void launch_prepare(const State& s, const Options& o, const int32_t* list, int count,
-                    const double* a_c, const double* g_c, void* stream);
+                    const double* a_c, const double* g_c, void* stream,
+                    const int32_t* dcount = nullptr);
 // Damped step of the listed runs (solve_step with mu), CONVERGED_STEP test, evaluate flag.
-void launch_step(const State& s, const Options& o, const int32_t* list, int count, void* stream);
+void launch_step(const State& s, const Options& o, const int32_t* list, int count, void* stream,
+                 const int32_t* dcount = nullptr);
 // Accept / reject of the listed runs from the merit of their trial points (f, failed, hard at
 // pos[run]), damping update, merit history and stall test.
 void launch_update(const State& s, const Options& o, const int32_t* list, int count,
                    const int32_t* pos, const double* f, const int64_t* failed,
-                   const int64_t* hard, void* stream);
+                   const int64_t* hard, void* stream, const int32_t* dcount = nullptr);
 // FD pattern (fd_pattern, src/optim/variant_generator.cpp:177-237) of `count` points on the
 // device: plus, minus, step_plus, step_minus [count][P], stencil [count][P].
 void launch_fd_pattern(int count, int P, const double* xs, const double* lower,

@@ -29,7 +29,7 @@ EXPORTS = [
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
     "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
-    "mko_lm_results", "mko_lm_destroy",
+    "mko_lm_results", "mko_lm_destroy", "mko_lm_graph_cycles",
 ]
 
 
@@ -93,6 +93,7 @@ def library():
         "mko_lm_cycle": ([C.c_void_p], C.c_int),
         "mko_lm_results": ([C.c_void_p, _dp, C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
         "mko_lm_destroy": ([C.c_void_p], None),
+        "mko_lm_graph_cycles": ([C.c_void_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -184,6 +185,10 @@ class LmSession:
         if rc < 0:
             _check(rc)
         return rc == 1
+
+    def graph_cycles(self):
+        """Cycles that ran as a replay of the captured hipGraph."""
+        return library().mko_lm_graph_cycles(self._h)
 
     def results(self):
         x = np.zeros((self.runs, self.P))
