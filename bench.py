@@ -35,6 +35,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 1 GiB device copies (scale for rocprofv3 byte counters)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the device LM cycles kernel by kernel instead of replaying the "
+                         "captured hipGraph (for profiles: launches then cover the active runs "
+                         "only, so per-dispatch counters divide by the grid's runs)")
     return ap.parse_args()
 
 
@@ -44,13 +48,14 @@ def sync():
     torch.cuda.synchronize()
 
 
-def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps, threads):
+def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps, threads,
+           graph=True):
     """Times `steps` LM cycles after `warmup` cycles. Returns the timed cycle count, seconds,
     device time per kernel family, the results and the Jacobians computed in the timed cycles."""
     from mikroptim_amd import abi
 
     opt = abi.lm_options(max_iterations=warmup + steps + 5, threads=threads,
-                         jacobian_method=method, reject_new_failures=True)
+                         jacobian_method=method, reject_new_failures=True, graph=graph)
     sess = eng.lm(base, settings, variables, x0, opt, precision)
     for _ in range(warmup):
         sess.cycle()
@@ -139,7 +144,7 @@ def main():
 
     n_adj, t_adj, kt_adj, res_adj, jac_adj = run_lm(eng, base, settings, variables, x0,
                                            abi.JACOBIAN_ADJOINT, "mixed", args.warmup,
-                                           args.steps, args.threads)
+                                           args.steps, args.threads, graph=not args.no_graph)
     it_adj = n_adj / t_adj
     graph_adj = run_lm.graph_cycles
     fd_steps = args.steps if args.fd_steps is None else args.fd_steps
