@@ -394,3 +394,100 @@ def uneven_topology(index, seed=5, variants=4, jitter=0.01):
 
 def uneven_settings(failure=abi.FAILURE_HARD):
     return ResidualSettings(rings=6, spokes=8, failure=failure)
+
+
+# ---- workloads at the limits of the path ---------------------------------------------------
+
+def point_singlet(point=(0.0, 2.0, -300.0), na=0.05, variants=4, seed=7, image_distance=120.0):
+    """The singlet lit by a point source (one field; pupil Axis on the axis, Half for a point in
+    the y-z plane, Full otherwise, as topology_of picks it), image plane at `image_distance`.
+    Mirrors the reference's point-source adjoint test system (tests/adjoint_test_systems.hpp:
+    76-95: N-BK7 singlet, source (0, 2, -300) mm, NA 0.05, image plane at 120 mm)."""
+    nbk7 = (optics.NBK7_B, optics.NBK7_C)
+    glasses = [None, nbk7, None]
+    x, y, _ = point
+    mode = (abi.PUPIL_AXIS if y == 0.0 else abi.PUPIL_HALF) if x == 0.0 else abi.PUPIL_FULL
+    b = SystemBatch([abi.SURFACE_STOP, abi.SURFACE_SPHERE, abi.SURFACE_SPHERE], F_C_D,
+                    [1.0, 1.0, 1.0], [1.0], [mode], variants, LAMBDA_REF,
+                    source=abi.SOURCE_POINT)
+    b.curvature[:] = [0.0, 0.02, -0.005]
+    b.thickness[:] = [5.0, 5.0, image_distance]
+    b.index[:] = _index_table(glasses, F_C_D)
+    b.paraxial_index = np.tile(_index_table(glasses, [LAMBDA_REF])[:, 0], (variants, 1))
+    b.pupil_radius[:] = 5.0
+    b.source_point[:] = point
+    b.source_na = na
+    v = singlet_variables()
+    v.upper[3] = 500.0
+    xs = rng.perturbed_vectors(v, v.extract(b, 0), variants, seed=seed,
+                               relative=(0.05, 0.0, 0.0), first_is_base=True)
+    v.apply(b, xs)
+    return b, v
+
+
+def long_relay(variants=2, seed=3, fields_deg=(0.0, 0.5), pupil_radius=1.0):
+    """64 surfaces (the most a batch holds): the stop and 63 weak spheres, alternately into a
+    glass of constant index 1.5 and into air (surfaces 1 ... 62), the last one in air; lens and
+    air gaps 1 mm, image plane 50 mm behind. 64 variables (the most the fused normal equations
+    and the device LM control take): curvatures of surfaces 1 ... 32 and thicknesses 0 ... 31.
+    Per variant the curvatures are jittered by 5 % (seeded)."""
+    S = abi.MAX_SURFACES
+    types = [abi.SURFACE_STOP] + [abi.SURFACE_SPHERE] * (S - 1)
+    fields_deg = tuple(fields_deg)
+    modes = [abi.PUPIL_AXIS if a == 0.0 else abi.PUPIL_HALF for a in fields_deg]
+    b = SystemBatch(types, F_C_D, [1.0, 1.0, 1.0], [1.0] * len(fields_deg), modes, variants,
+                    LAMBDA_REF)
+    c = [0.0] + [0.004 * (1 if (s // 2) % 2 == 0 else -1) * (1 if s % 2 else -1)
+                 for s in range(1, S)]
+    t = [1.0] * (S - 1) + [50.0]
+    n_after = [1.0] + [1.5 if (s % 2 == 1 and s < S - 1) else 1.0 for s in range(1, S)]
+    b.curvature[:] = c
+    b.thickness[:] = t
+    b.index[:] = np.array(n_after)[:, None]
+    b.paraxial_index = np.tile(np.array(n_after), (variants, 1))
+    b.pupil_radius[:] = pupil_radius
+    theta = np.radians(fields_deg)
+    b.field_angle[:] = theta
+    b.field_target[:] = 0.0
+    P = 64
+    v = VariableSet(
+        type=np.array([abi.VAR_CURVATURE] * 32 + [abi.VAR_THICKNESS] * 32, dtype=np.int32),
+        batch_surface=np.array(list(range(1, 33)) + list(range(0, 32)), dtype=np.int32),
+        lower=np.array([-0.05] * 32 + [0.2] * 32), upper=np.array([0.05] * 32 + [5.0] * 32),
+        typical=np.array([0.01] * 32 + [1.0] * 32),
+        names=[f"c[{s}]" for s in range(1, 33)] + [f"t[{s}]" for s in range(32)])
+    assert v.size == P
+    xs = rng.perturbed_vectors(v, v.extract(b, 0), variants, seed=seed,
+                               relative=(0.05, 0.0, 0.0), first_is_base=True)
+    v.apply(b, xs)
+    return b, v
+
+
+def many_tuples(variants=3, fields=32, wavelengths=32):
+    """The singlet with the most fields and wavelengths a batch holds (32 each): field angles
+    0 ... 5 deg, wavelengths 450 ... 700 nm, uneven weights."""
+    nbk7 = (optics.NBK7_B, optics.NBK7_C)
+    glasses = [None, nbk7, None]
+    wl = list(np.linspace(450.0, 700.0, wavelengths))
+    ang = list(np.linspace(0.0, 5.0, fields))
+    b = SystemBatch([abi.SURFACE_STOP, abi.SURFACE_SPHERE, abi.SURFACE_SPHERE], wl,
+                    list(np.linspace(0.5, 1.5, wavelengths)), list(np.linspace(1.0, 0.2, fields)),
+                    [abi.PUPIL_AXIS if a == 0.0 else abi.PUPIL_HALF for a in ang], variants,
+                    LAMBDA_REF)
+    c = [0.0, 0.02, -0.005]
+    t = [5.0, 5.0, 0.0]
+    f, bfd = _paraxial(c, t, glasses)
+    t[2] = bfd
+    b.curvature[:] = c
+    b.thickness[:] = t
+    b.index[:] = _index_table(glasses, wl)
+    b.paraxial_index = np.tile(_index_table(glasses, [LAMBDA_REF])[:, 0], (variants, 1))
+    b.pupil_radius[:] = 5.0
+    theta = np.radians(ang)
+    b.field_angle[:] = theta
+    b.field_target[:] = f * np.tan(theta)
+    v = singlet_variables()
+    xs = rng.perturbed_vectors(v, v.extract(b, 0), variants, seed=9,
+                               relative=(0.05, 0.0, 0.0), first_is_base=True)
+    v.apply(b, xs)
+    return b, v

@@ -512,3 +512,35 @@ def test_uneven_topologies_cover_the_ranges_and_trace_cleanly():
         assert np.all(np.isfinite(m)) and np.all(m > 0)
         seen.add((b.surfaces, b.fields, b.wavelengths, v.size))
     assert len(seen) == 12  # distinct topologies
+
+
+LIMIT_WORKLOADS = {
+    "point_half": lambda: (systems.point_singlet(), systems.singlet_settings(abi.FAILURE_SOFT)),
+    "point_full": lambda: (systems.point_singlet(point=(1.0, 1.5, -300.0)),
+                           systems.singlet_settings()),
+    "point_axis": lambda: (systems.point_singlet(point=(0.0, 0.0, -300.0)),
+                           systems.singlet_settings()),
+    "relay_64_surfaces": lambda: (systems.long_relay(),
+                                  systems.ResidualSettings(rings=16, spokes=32,
+                                                           failure=abi.FAILURE_HARD)),
+    "fields_wavelengths_32": lambda: (systems.many_tuples(),
+                                      systems.ResidualSettings(rings=2, spokes=3,
+                                                               failure=abi.FAILURE_SOFT)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LIMIT_WORKLOADS))
+def test_oracle_equals_reference_at_the_limits(name):
+    """Point sources (Axis / Half / Full pupils), 64 surfaces with 64 variables, 32 fields x 32
+    wavelengths: residuals, merit, adjoint and FD Jacobians bit for bit."""
+    ref = _need_ref()
+    (b, v), s = LIMIT_WORKLOADS[name]()
+    for a, c in zip(oracle().residuals(b, s, threads=8), ref.residuals(b, s)):
+        assert np.array_equal(a, c)
+    for a, c in zip(oracle().merit(b, s, threads=8), ref.merit(b, s)):
+        assert np.array_equal(a, c)
+    base, x0 = b.take([0]), v.extract(b)[:2]
+    assert np.array_equal(oracle().adjoint_jacobian(base, s, v, x0, threads=8)[0],
+                          ref.adjoint_jacobian(base, s, v, x0)[0])
+    assert np.array_equal(oracle().fd_jacobian(base, s, v, x0, threads=8)[0],
+                          ref.fd_jacobian(base, s, v, x0)[0])
