@@ -41,7 +41,7 @@ enum {
     MKO_STATUS_SOFT_TIR = 6
 };
 /* VariableType (include/mikroptim/model/variables.hpp:17), the kinds this build supports */
-enum { MKO_VAR_CURVATURE = 0, MKO_VAR_THICKNESS = 1, MKO_VAR_WORKING_DISTANCE = 2 };
+enum { MKO_VAR_CURVATURE = 0, MKO_VAR_THICKNESS = 1, MKO_VAR_WORKING_DISTANCE = 2, MKO_VAR_GLASS = 3 };
 /* FdStencil (include/mikroptim/optim/fd_formula.hpp:15) */
 enum { MKO_STENCIL_CENTRAL = 0, MKO_STENCIL_ONE_SIDED = 1, MKO_STENCIL_TRUNCATED = 2 };
 /* JacobianMethod (include/mikroptim/optim/levenberg_marquardt.hpp:73) */
@@ -90,6 +90,14 @@ typedef struct mko_batch {
      * or NULL; needed only by the paraxial constraint rows (focal length, paraxial focus,
      * Petzval). */
     const double* paraxial_index;
+    /* [L] wavelengths in nm (host), or NULL; needed with glass variables. */
+    const double* wavelength_nm;
+    /* [V][S][4] Buchdahl coefficients (n0, nu1, nu2, nu3) of the medium after surface s
+     * (SystemBatchHost::buchdahl_coeff, include/mikroptim/batch/system_batch.hpp), or NULL;
+     * needed with glass variables. Setting a glass variable rewrites its surface's index[v][s][l]
+     * and paraxial_index[v][s] as buchdahl_index (glass/dispersion.hpp:93):
+     * n = n0 + w (nu1 + w (nu2 + w nu3)), w = (um - 0.546074) / (1 + 2.5 (um - 0.546074)). */
+    const double* buchdahl;
 } mko_batch;
 
 /* Constraint rows appended to every variant's residual vector (ExtraResiduals /
@@ -157,8 +165,10 @@ typedef struct mko_residual_settings {
 } mko_residual_settings;
 
 /* P variables. Curvature k -> curvature[v][batch_surface[k]]; thickness and working distance
- * -> thickness[v][batch_surface[k]] (working distance: batch_surface = S - 1). Bounds may be
- * +-inf; typical > 0. (host memory) */
+ * -> thickness[v][batch_surface[k]] (working distance: batch_surface = S - 1); glass k ->
+ * buchdahl[v][batch_surface[k]][component[k]], the medium after the surface, components
+ * n0, nu1, nu2 as three consecutive variables (resolve_variables, src/model/variables.cpp:
+ * 420-432). Bounds may be +-inf; typical > 0. (host memory) */
 typedef struct mko_variables {
     int32_t count;
     const int32_t* type;          /* [P] MKO_VAR_* */
@@ -166,6 +176,7 @@ typedef struct mko_variables {
     const double* lower;          /* [P] */
     const double* upper;          /* [P] */
     const double* typical;        /* [P] */
+    const int32_t* component;     /* [P] glass variables' component 0 / 1 / 2, or NULL */
 } mko_variables;
 
 typedef struct mko_lm_options {

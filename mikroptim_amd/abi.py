@@ -13,7 +13,7 @@ REF_LAST_SURFACE, REF_IMAGE_EXIT_PUPIL, REF_IMAGE_INFINITE = 0, 1, 2
 FAILURE_SOFT, FAILURE_HARD = 0, 1
 (STATUS_OK, STATUS_MISSED, STATUS_TIR, STATUS_BACKWARD, STATUS_UNSUPPORTED, STATUS_SOFT_MISSED,
  STATUS_SOFT_TIR) = range(7)
-VAR_CURVATURE, VAR_THICKNESS, VAR_WORKING_DISTANCE = 0, 1, 2
+VAR_CURVATURE, VAR_THICKNESS, VAR_WORKING_DISTANCE, VAR_GLASS = 0, 1, 2, 3
 STENCIL_CENTRAL, STENCIL_ONE_SIDED, STENCIL_TRUNCATED = 0, 1, 2
 JACOBIAN_FD, JACOBIAN_ADJOINT = 0, 1
 (LM_RUNNING, LM_CONVERGED_MERIT, LM_CONVERGED_STEP, LM_CONVERGED_GRADIENT, LM_CONVERGED_STALL,
@@ -44,7 +44,7 @@ class Batch(C.Structure):
         ("type", _u8p), ("pupil_mode", _u8p), ("wavelength_weight", _dp), ("field_weight", _dp),
         ("curvature", _dp), ("thickness", _dp), ("index", _dp), ("pupil_radius", _dp),
         ("field_angle", _dp), ("field_target", _dp), ("reference_curvature", _dp),
-        ("paraxial_index", _dp),
+        ("paraxial_index", _dp), ("wavelength_nm", _dp), ("buchdahl", _dp),
     ]
 
 
@@ -69,7 +69,7 @@ class ResidualSettings(C.Structure):
 
 class Variables(C.Structure):
     _fields_ = [("count", C.c_int32), ("type", _i32p), ("batch_surface", _i32p),
-                ("lower", _dp), ("upper", _dp), ("typical", _dp)]
+                ("lower", _dp), ("upper", _dp), ("typical", _dp), ("component", _i32p)]
 
 
 class LmOptions(C.Structure):

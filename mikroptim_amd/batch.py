@@ -47,6 +47,9 @@ class SystemBatch:
                                     else None)
         # [V][S] medium index after each surface at lambda_ref (paraxial constraint rows)
         self.paraxial_index = None
+        # [V][S][4] Buchdahl coefficients (n0, nu1, nu2, nu3) of the medium after each surface,
+        # needed with glass variables
+        self.buchdahl = None
 
     # ---- shape --------------------------------------------------------------------------
     @property
@@ -66,7 +69,7 @@ class SystemBatch:
         return len(self.field_weight)
 
     PER_VARIANT = ("curvature", "thickness", "index", "pupil_radius", "field_angle",
-                   "field_target", "reference_curvature", "paraxial_index")
+                   "field_target", "reference_curvature", "paraxial_index", "buchdahl")
 
     def take(self, rows):
         """New batch with variants `rows` of this one (shared fields copied)."""
@@ -105,6 +108,7 @@ class SystemBatch:
         b.pupil_mode = abi.ptr(self.pupil_mode, C.c_uint8)
         b.wavelength_weight = abi.ptr(self.wavelength_weight)
         b.field_weight = abi.ptr(self.field_weight)
+        b.wavelength_nm = abi.ptr(self.wavelength_nm)
         for name in self.PER_VARIANT:
             if arrays is not None:
                 value = arrays.get(name)
@@ -254,19 +258,49 @@ def residual_rows(batch, settings):
     return trace_rows(batch, settings) + settings.extra_rows
 
 
+BUCHDAHL_LAMBDA0_UM = 0.546074
+BUCHDAHL_ALPHA = 2.5
+
+
+def buchdahl_omega(lambda_nm):
+    """buchdahl_omega of nm_to_um(lambda) (include/mikroptim/glass/dispersion.hpp:18-89)."""
+    um = lambda_nm / 1000.0
+    den = 1.0 + BUCHDAHL_ALPHA * (um - BUCHDAHL_LAMBDA0_UM)
+    if not den > 0.0:
+        raise ValueError(f"Buchdahl coordinate undefined at {lambda_nm} nm (needs > 146.074 nm)")
+    return (um - BUCHDAHL_LAMBDA0_UM) / den
+
+
+def buchdahl_index(coeff, lambda_nm):
+    """n = n0 + w (nu1 + w (nu2 + w nu3)) (buchdahl_index, dispersion.hpp:93)."""
+    w = buchdahl_omega(lambda_nm)
+    return coeff[0] + w * (coeff[1] + w * (coeff[2] + w * coeff[3]))
+
+
+# typical sizes of the glass components (kDefaultTypicalGlass*, model/variables.hpp:77-79)
+GLASS_TYPICAL = (1.0, 0.01, 0.001)
+
+
 @dataclass
 class VariableSet:
-    """Variables x <-> batch elements (curvature[v][s], thickness[v][s])."""
+    """Variables x <-> batch elements (curvature[v][s], thickness[v][s]; glass: the Buchdahl
+    coefficient buchdahl[v][s][component], which also sets the surface's index and paraxial
+    index, VariableSet::apply, src/optim/variable_set.cpp:237-285)."""
     type: np.ndarray
     batch_surface: np.ndarray
     lower: np.ndarray
     upper: np.ndarray
     typical: np.ndarray
     names: list = field(default_factory=list)
+    component: np.ndarray = None  # [P] glass component 0 / 1 / 2 (None without glass)
 
     @property
     def size(self):
         return len(self.type)
+
+    @property
+    def has_glass(self):
+        return bool(np.any(np.asarray(self.type) == abi.VAR_GLASS))
 
     def c_struct(self):
         self.type = np.ascontiguousarray(self.type, dtype=np.int32)
@@ -274,28 +308,57 @@ class VariableSet:
         self.lower = np.ascontiguousarray(self.lower, dtype=np.float64)
         self.upper = np.ascontiguousarray(self.upper, dtype=np.float64)
         self.typical = np.ascontiguousarray(self.typical, dtype=np.float64)
+        comp = None
+        if self.component is not None:
+            self.component = np.ascontiguousarray(self.component, dtype=np.int32)
+            comp = abi.ptr(self.component, C.c_int32)
         return abi.Variables(self.size, abi.ptr(self.type, C.c_int32),
                              abi.ptr(self.batch_surface, C.c_int32), abi.ptr(self.lower),
-                             abi.ptr(self.upper), abi.ptr(self.typical))
+                             abi.ptr(self.upper), abi.ptr(self.typical), comp)
+
+    def _column(self, batch, k):
+        t, s = int(self.type[k]), int(self.batch_surface[k])
+        if t == abi.VAR_GLASS:
+            if batch.buchdahl is None:
+                raise ValueError("glass variables need the batch's Buchdahl coefficients")
+            return batch.buchdahl[:, s, int(self.component[k])]
+        return (batch.curvature if t == abi.VAR_CURVATURE else batch.thickness)[:, s]
 
     def extract(self, batch, v=None):
         """x of variant v (all variants if v is None): VariableSet::extract."""
-        rows = slice(None) if v is None else v
-        cols = []
-        for t, s in zip(self.type, self.batch_surface):
-            src = batch.curvature if t == abi.VAR_CURVATURE else batch.thickness
-            cols.append(src[rows, s])
-        return np.stack(cols, axis=-1)
+        cols = [self._column(batch, k) for k in range(self.size)]
+        x = np.stack(cols, axis=-1)
+        return x if v is None else x[v]
+
+    def glass_surfaces(self):
+        return sorted({int(s) for t, s in zip(self.type, self.batch_surface)
+                       if t == abi.VAR_GLASS})
 
     def apply(self, batch, x, variants=None):
-        """Writes x ([P] or [n, P]) into the given variants (all if None): VariableSet::apply."""
+        """Writes x ([P] or [n, P]) into the given variants (all if None): VariableSet::apply;
+        glass surfaces then get index[v][s][l] and paraxial_index[v][s] from their Buchdahl
+        coefficients."""
         x = np.atleast_2d(np.asarray(x, dtype=np.float64))
         rows = np.arange(batch.variants) if variants is None else np.asarray(variants)
         if not np.all(np.isfinite(x)):
             raise ValueError("variable value not finite")
+        glass = self.glass_surfaces()
+        if glass:
+            for nm in list(batch.wavelength_nm) + [batch.reference_wavelength_nm]:
+                buchdahl_omega(float(nm))
         for k, (t, s) in enumerate(zip(self.type, self.batch_surface)):
-            dst = batch.curvature if t == abi.VAR_CURVATURE else batch.thickness
-            dst[rows, s] = x[:, k]
+            if t == abi.VAR_GLASS:
+                batch.buchdahl[rows, s, int(self.component[k])] = x[:, k]
+            else:
+                dst = batch.curvature if t == abi.VAR_CURVATURE else batch.thickness
+                dst[rows, s] = x[:, k]
+        for s in glass:
+            for v in np.atleast_1d(rows):
+                c = [float(z) for z in batch.buchdahl[v, s]]
+                for l, nm in enumerate(batch.wavelength_nm):
+                    batch.index[v, s, l] = buchdahl_index(c, float(nm))
+                if batch.paraxial_index is not None:
+                    batch.paraxial_index[v, s] = buchdahl_index(c, batch.reference_wavelength_nm)
 
     def clamp(self, x):
         return np.minimum(np.maximum(x, self.lower), self.upper)

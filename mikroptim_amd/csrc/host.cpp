@@ -102,12 +102,14 @@ inline int64_t trace_rows_of(const mko_batch& b, const mko_residual_settings& s)
 // Per-variant arrays of a batch: element counts per variant.
 struct Strides {
     size_t S, SL, F;
-    bool refc, pidx;
+    bool refc, pidx, bd;
     explicit Strides(const mko_batch& b)
         : S(size_t(b.surfaces)), SL(size_t(b.surfaces) * size_t(b.wavelengths)),
           F(size_t(b.fields)), refc(b.reference_sphere != MKO_REF_LAST_SURFACE),
-          pidx(b.paraxial_index != nullptr) {}
-    size_t doubles() const { return 2 * S + SL + 1 + 2 * F + (refc ? F : 0) + (pidx ? S : 0); }
+          pidx(b.paraxial_index != nullptr), bd(b.buchdahl != nullptr) {}
+    size_t doubles() const {
+        return 2 * S + SL + 1 + 2 * F + (refc ? F : 0) + (pidx ? S : 0) + (bd ? 4 * S : 0);
+    }
 };
 
 void check_batch(const mko_batch& b) {
@@ -119,6 +121,29 @@ void check_batch(const mko_batch& b) {
         throw std::invalid_argument("batch-wide arrays missing");
     if (b.reference_sphere != MKO_REF_LAST_SURFACE && !b.reference_curvature)
         throw std::invalid_argument("image reference sphere needs reference_curvature");
+}
+
+// Glass variables (VariableSet::apply, src/optim/variable_set.cpp:244-258): components 0 ... 2,
+// the batch's Buchdahl coefficients and wavelengths, a defined Buchdahl coordinate at every
+// wavelength and at lambda_ref.
+void check_glass(const mko_batch& b, const mko_variables& v) {
+    bool glass = false;
+    for (int k = 0; k < v.count; ++k) {
+        if (v.type[k] != MKO_VAR_GLASS) continue;
+        glass = true;
+        if (!v.component || v.component[k] < 0 || v.component[k] > 2)
+            throw std::invalid_argument("glass variables need components 0 ... 2");
+    }
+    if (!glass) return;
+    if (!b.buchdahl || !b.wavelength_nm)
+        throw std::invalid_argument("glass variables need the batch's Buchdahl coefficients and "
+                                    "wavelengths");
+    for (int l = 0; l <= b.wavelengths; ++l) {
+        const double nm = l < b.wavelengths ? b.wavelength_nm[l] : b.reference_wavelength_nm;
+        if (!(1.0 + 2.5 * (nm / 1000.0 - 0.546074) > 0.0))
+            throw std::invalid_argument("glass variables: Buchdahl coordinate undefined at a "
+                                        "wavelength (needs lambda > 146.074 nm)");
+    }
 }
 
 // validate_values (src/batch/system_batch.cpp:558): finite values, pupil radius > 0, indices > 0.
@@ -293,6 +318,7 @@ struct DeviceBatch {
         put(h.field_target, st.F, &view.field_target);
         put(st.refc ? h.reference_curvature : nullptr, st.F, &view.reference_curvature);
         put(h.paraxial_index, st.S, &view.paraxial_index);
+        put(h.buchdahl, 4 * st.S, &view.buchdahl);
     }
     // Uninitialised device batch of `count` variants shaped like h (filled by a kernel).
     void allocate(const mko_batch& h, int count) {
@@ -317,13 +343,14 @@ struct DeviceBatch {
         take(st.F, &view.field_target);
         take(st.F, &view.reference_curvature, st.refc);
         take(st.S, &view.paraxial_index, st.pidx);
+        take(4 * st.S, &view.buchdahl, st.bd);
     }
     double* mut(const double* p) { return const_cast<double*>(p); }
 };
 
 // Host batch made of selected variants of another batch (run templates of an LM subset).
 struct GatheredBatch {
-    std::vector<double> curv, thick, index, pupil, fa, ft, rc, pi;
+    std::vector<double> curv, thick, index, pupil, fa, ft, rc, pi, bd;
     mko_batch view{};
     const mko_batch& gather(const mko_batch& base, const int* ids, size_t n) {
         const Strides st(base);
@@ -335,6 +362,7 @@ struct GatheredBatch {
         ft.resize(n * st.F);
         rc.resize(st.refc ? n * st.F : 0);
         pi.resize(st.pidx ? n * st.S : 0);
+        bd.resize(st.bd ? n * 4 * st.S : 0);
         for (size_t j = 0; j < n; ++j) {
             const int id = ids[j];
             if (id < 0 || id >= base.variants) throw std::out_of_range("run index outside the base batch");
@@ -347,6 +375,7 @@ struct GatheredBatch {
             std::copy_n(base.field_target + v * st.F, st.F, ft.data() + j * st.F);
             if (st.refc) std::copy_n(base.reference_curvature + v * st.F, st.F, rc.data() + j * st.F);
             if (st.pidx) std::copy_n(base.paraxial_index + v * st.S, st.S, pi.data() + j * st.S);
+            if (st.bd) std::copy_n(base.buchdahl + v * 4 * st.S, 4 * st.S, bd.data() + j * 4 * st.S);
         }
         view = base;
         view.variants = int(n);
@@ -358,6 +387,7 @@ struct GatheredBatch {
         view.field_target = ft.data();
         view.reference_curvature = st.refc ? rc.data() : nullptr;
         view.paraxial_index = st.pidx ? pi.data() : nullptr;
+        view.buchdahl = st.bd ? bd.data() : nullptr;
         return view;
     }
 };
@@ -778,7 +808,7 @@ public:
         // the fused kernel and J is never formed (hard failure mode, <= 64 variables, <= 512
         // rays per tuple).
         const bool fused = !fd && precision == MKO_PRECISION_MIXED && !J && !r0 && (a || g) &&
-                           fused_ok(s, P);
+                           fused_ok(s, vars);
         const bool pattern = fd || rp.count() > 0;
         const int per_run = fd ? 2 * P + 1 : 1;
         const size_t Tt = size_t(base.fields) * base.wavelengths, S = size_t(base.surfaces);
@@ -790,6 +820,7 @@ public:
                            size_t(P) * (P + 1) * 8 + 64 * size_t(P) +
                            (!fd ? rows_fd_bytes(rp, base, P) : 0);
         const int chunk = tranche_size(runs, per);
+        check_glass(base, vars);
         upload_vars(vars);
         Pattern pat;
         if (pattern) {
@@ -832,7 +863,8 @@ public:
                 work_.mut(work_.view.thickness), work_.mut(work_.view.index),
                 work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
                 work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
-                work_.mut(work_.view.paraxial_index), nullptr));
+                work_.mut(work_.view.paraxial_index),
+                    work_.mut(work_.view.buchdahl), nullptr));
             if (fused) {
                 double* da = a_.as<double>(size_t(count) * P * P);
                 double* dg = g_.as<double>(size_t(count) * P);
@@ -900,6 +932,7 @@ public:
         const size_t per =
             Strides(base).doubles() * 8 + T * 16 + 64 + size_t(P) * 8 + rows_bytes(rp, base);
         const int chunk = tranche_size(runs, per);
+        check_glass(base, vars);
         upload_vars(vars);
         if (base.variants == 1) tmpl_.upload(base, 0, 1, nullptr);
         for (int first = 0; first < runs; first += chunk) {
@@ -915,7 +948,7 @@ public:
                 work_.mut(work_.view.index), work_.mut(work_.view.pupil_radius),
                 work_.mut(work_.view.field_angle), work_.mut(work_.view.field_target),
                 work_.mut(work_.view.reference_curvature), work_.mut(work_.view.paraxial_index),
-                nullptr));
+                work_.mut(work_.view.buchdahl), nullptr));
             merit_device(work_.view, s, rp, nullptr, count);
             copy_out(merit + first, d_merit_, size_t(count) * 8);
             copy_out(failed + first, d_failed_, size_t(count) * 8);
@@ -946,7 +979,7 @@ public:
         upload_rows(rp);
         const int64_t T = trace_rows_of(tmpl, s), R = T + rp.count(), rays = rays_of(tmpl, s);
         const bool fd = method == MKO_JACOBIAN_FD;
-        const bool fused = !fd && precision == MKO_PRECISION_MIXED && fused_ok(s, P);
+        const bool fused = !fd && precision == MKO_PRECISION_MIXED && fused_ok(s, vars_host_dev_);
         const bool pattern = fd || rp.count() > 0;
         const int per_run = fd ? 2 * P + 1 : 1;
         const size_t Tt = size_t(tmpl.fields) * tmpl.wavelengths, S = size_t(tmpl.surfaces);
@@ -983,7 +1016,8 @@ public:
                     work_.mut(work_.view.thickness), work_.mut(work_.view.index),
                     work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
                     work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
-                    work_.mut(work_.view.paraxial_index), nullptr));
+                    work_.mut(work_.view.paraxial_index),
+                    work_.mut(work_.view.buchdahl), nullptr));
             });
             double* a_out = ad + size_t(first) * P * P;
             double* g_out = gd + size_t(first) * P;
@@ -1054,7 +1088,8 @@ public:
                     work_.mut(work_.view.thickness), work_.mut(work_.view.index),
                     work_.mut(work_.view.pupil_radius), work_.mut(work_.view.field_angle),
                     work_.mut(work_.view.field_target), work_.mut(work_.view.reference_curvature),
-                    work_.mut(work_.view.paraxial_index), nullptr));
+                    work_.mut(work_.view.paraxial_index),
+                    work_.mut(work_.view.buchdahl), nullptr));
             });
             mko_internal::RayStates st{};
             if (states) {
@@ -1068,8 +1103,11 @@ public:
     }
 
     // The mixed-precision normal equations come from the fused kernel under these conditions.
-    static bool fused_ok(const mko_residual_settings& s, int P) {
-        return s.failure == MKO_FAILURE_HARD && P <= 64 && int64_t(s.rings) * s.spokes <= 512;
+    static bool fused_ok(const mko_residual_settings& s, const mko_variables& v) {
+        bool glass = false;
+        for (int k = 0; k < v.count; ++k) glass = glass || v.type[k] == MKO_VAR_GLASS;
+        return s.failure == MKO_FAILURE_HARD && v.count <= 64 &&
+               int64_t(s.rings) * s.spokes <= 512 && !glass;
     }
 
     int device() const { return device_; }
@@ -1151,7 +1189,8 @@ private:
                 rowfd_.mut(rowfd_.view.index), rowfd_.mut(rowfd_.view.pupil_radius),
                 rowfd_.mut(rowfd_.view.field_angle), rowfd_.mut(rowfd_.view.field_target),
                 rowfd_.mut(rowfd_.view.reference_curvature),
-                rowfd_.mut(rowfd_.view.paraxial_index), nullptr));
+                rowfd_.mut(rowfd_.view.paraxial_index),
+                rowfd_.mut(rowfd_.view.buchdahl), nullptr));
         });
         double* ex = nullptr;
         if (rp.Et) {
@@ -1193,6 +1232,12 @@ private:
         dvars_ = v;
         dvars_.type = t;
         dvars_.batch_surface = s;
+        dvars_.component = nullptr;
+        if (v.component) {
+            int32_t* c = vcomp_.as<int32_t>(size_t(v.count));
+            hip_check(hipMemcpy(c, v.component, size_t(v.count) * 4, hipMemcpyHostToDevice), "vars");
+            dvars_.component = c;
+        }
         vars_host_dev_ = v; // the adjoint entry point reads type/surface on the host
     }
 
@@ -1202,7 +1247,8 @@ private:
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
     Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
     std::vector<char> items_host_;
-    Buffer x_, plus_, minus_, jac_, r0_, fdres_, fdst_, sp_, sm_, sten_, ch_, a_, g_, vtype_, vsurf_;
+    Buffer x_, plus_, minus_, jac_, r0_, fdres_, fdst_, sp_, sm_, sten_, ch_, a_, g_, vtype_, vsurf_,
+        vcomp_;
     double* d_merit_ = nullptr;
     int64_t* d_failed_ = nullptr;
     int64_t* d_hard_ = nullptr;
@@ -1691,6 +1737,7 @@ private:
 
     void dev_setup() {
         validate_values(base_, 0, base_.variants);
+        check_glass(base_, vars_);
         eng_.set_variables(vars_);
         dev_tmpl_.upload(base_, 0, base_.variants, nullptr);
         double* bnd = bounds_d_.as<double>(3 * P_);
@@ -1715,7 +1762,7 @@ private:
     void dev_allocate_states(int n) {
         use_states_ = false;
         if (o_.jacobian_method != MKO_JACOBIAN_ADJOINT || precision_ != MKO_PRECISION_MIXED ||
-            !Engine::fused_ok(settings_, int(P_)) || settings_.precision != MKO_PRECISION_FP64)
+            !Engine::fused_ok(settings_, vars_) || settings_.precision != MKO_PRECISION_FP64)
             return;
         const int64_t rays = rays_of(base_, settings_);
         const size_t bytes = size_t(n) * size_t(rays) * 7 * sizeof(double);
@@ -1892,7 +1939,7 @@ private:
     bool graph_eligible() const {
         return graph_wanted_ && !graph_failed_ && use_states_ && settings_.extra == nullptr &&
                precision_ == MKO_PRECISION_MIXED && o_.jacobian_method == MKO_JACOBIAN_ADJOINT &&
-               Engine::fused_ok(settings_, int(P_));
+               Engine::fused_ok(settings_, vars_);
     }
 
     void graph_reset() {
@@ -1975,6 +2022,7 @@ private:
         mko_variables dv = vars_;
         dv.type = vt;
         dv.batch_surface = vs;
+        dv.component = nullptr; // no glass variables in the captured cycle
         // the round's start (null stream) is complete before the first replay on gs_
         hip_check(hipStreamSynchronize(nullptr), "graph start");
         int32_t* list_j = lists_;
@@ -1991,7 +2039,8 @@ private:
                 gwork_.mut(gwork_.view.thickness), gwork_.mut(gwork_.view.index),
                 gwork_.mut(gwork_.view.pupil_radius), gwork_.mut(gwork_.view.field_angle),
                 gwork_.mut(gwork_.view.field_target), gwork_.mut(gwork_.view.reference_curvature),
-                gwork_.mut(gwork_.view.paraxial_index), gs_, live));
+                gwork_.mut(gwork_.view.paraxial_index),
+                gwork_.mut(gwork_.view.buchdahl), gs_, live));
         };
         // the parts of the cycle in order; `which`: the list whose count > 0 makes the part a
         // timed launch, as in the eager cycle (-1: always)

@@ -40,8 +40,8 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                 int runs, int per_run, const mko_variables* vars_device, const double* x0,
                 const double* plus, const double* minus, double* curvature, double* thickness,
                 double* index, double* pupil_radius, double* field_angle, double* field_target,
-                double* reference_curvature, double* paraxial_index, void* stream,
-                const int32_t* live = nullptr);
+                double* reference_curvature, double* paraxial_index, double* buchdahl,
+                void* stream, const int32_t* live = nullptr);
 
 // RESIDUALS (mode 0: rows at `stride` >= trace rows) or MERIT partials (mode 1: partial [V*T],
 // partial_counts [2*V*T], finalized by merit_finalize) of a device batch, optionally with the

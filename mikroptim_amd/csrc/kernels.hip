@@ -87,6 +87,12 @@ struct ParamMap {
     int P;
     int8_t curv_col[MKO_MAX_SURFACES];
     int8_t thick_col[MKO_MAX_SURFACES];
+    // glass variables (AdjointParameterMap::glass / glass_slope, ray_jacobian.hpp:28-38): the
+    // variables of (n0, nu1, nu2) of the medium after surface s, and the Buchdahl coordinate
+    // w_l of each wavelength (dn/d(n0, nu1, nu2) = 1, w_l, w_l^2)
+    int glass;
+    int8_t glass_col[MKO_MAX_SURFACES][3];
+    double omega[kMaxWavelengths];
 };
 
 // ---- tuple geometry ------------------------------------------------------------------------
@@ -961,6 +967,7 @@ struct BatchPtrs {
     double* field_target;
     double* reference_curvature;
     double* paraxial_index;
+    double* buchdahl;
 };
 struct ConstBatchPtrs {
     const double* curvature;
@@ -971,13 +978,22 @@ struct ConstBatchPtrs {
     const double* field_target;
     const double* reference_curvature;
     const double* paraxial_index;
+    const double* buchdahl;
+};
+
+// Buchdahl coordinate w of each wavelength and (at [L]) of lambda_ref, computed on the host from
+// the batch's wavelengths (valid = 0: not available).
+struct OmegaTable {
+    double w[kMaxWavelengths + 1];
+    int valid;
 };
 
 __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, int per_run,
                                    const int32_t* template_index, int template_per_run, int S,
                                    int L, int F, int P, const int* var_type,
-                                   const int* var_surface, const double* x0, const double* plus,
-                                   const double* minus, const int32_t* live) {
+                                   const int* var_surface, const int* var_component,
+                                   const double* x0, const double* plus, const double* minus,
+                                   const int32_t* live, OmegaTable om) {
     const int vv = blockIdx.x; // destination variant
     const int run = vv / per_run;
     const int j = vv - run * per_run;
@@ -998,17 +1014,46 @@ __global__ void point_batch_kernel(ConstBatchPtrs src, BatchPtrs dst, int runs, 
             dst.reference_curvature[size_t(vv) * F + i] = src.reference_curvature[size_t(from) * F + i];
     }
     if (threadIdx.x == 0) dst.pupil_radius[vv] = src.pupil_radius[from];
+    if (src.buchdahl && dst.buchdahl)
+        for (int i = threadIdx.x; i < 4 * S; i += blockDim.x)
+            dst.buchdahl[size_t(vv) * 4 * S + i] = src.buchdahl[size_t(from) * 4 * S + i];
     __syncthreads();
     // variables: x0 of the run, component k replaced for FD variants
     const int kk = j == 0 ? -1 : (j - 1) / 2;
+    bool glass = false;
+    for (int k = 0; k < P; ++k) glass = glass || var_type[k] == MKO_VAR_GLASS;
     for (int k = threadIdx.x; k < P; k += blockDim.x) {
         double x = x0[size_t(run) * P + k];
         if (k == kk) x = (j & 1) ? plus[size_t(run) * P + k] : minus[size_t(run) * P + k];
         const int s = var_surface[k];
         if (var_type[k] == MKO_VAR_CURVATURE)
             dst.curvature[size_t(vv) * S + s] = x;
-        else
+        else if (var_type[k] == MKO_VAR_GLASS) {
+            if (dst.buchdahl && var_component)
+                dst.buchdahl[(size_t(vv) * S + s) * 4 + var_component[k]] = x;
+        } else
             dst.thickness[size_t(vv) * S + s] = x;
+    }
+    if (!glass) return;
+    __syncthreads();
+    // index of the glass-variable surfaces from their Buchdahl coefficients
+    // (VariableSet::apply, variable_set.cpp:271-284; buchdahl_index, dispersion.hpp:93);
+    // without coefficients or wavelengths the index becomes NaN (the trace then fails loudly)
+    for (int k = threadIdx.x; k < P; k += blockDim.x) {
+        if (var_type[k] != MKO_VAR_GLASS || !var_component || var_component[k] != 0) continue;
+        const int s = var_surface[k];
+        const bool ok = om.valid && dst.buchdahl != nullptr;
+        const double* c = ok ? dst.buchdahl + (size_t(vv) * S + s) * 4 : nullptr;
+        for (int l = 0; l <= L; ++l) {
+#pragma clang fp contract(off)
+            double n = __builtin_nan("");
+            if (ok) {
+                const double w = om.w[l];
+                n = c[0] + w * (c[1] + w * (c[2] + w * c[3]));
+            }
+            if (l < L) dst.index[(size_t(vv) * S + s) * L + l] = n;
+            else if (dst.paraxial_index) dst.paraxial_index[size_t(vv) * S + s] = n;
+        }
     }
 }
 
@@ -1078,11 +1123,21 @@ template <class RB, class RR, class TJ>
 __device__ __forceinline__ void backward_ray(const TraceParams& P, const SurfLds& L,
                                              const ParamMap& map, const Record<RR>& rec,
                                              bool reached_image, V3<double> M, double kappa,
-                                             RB seed_opl, RB seed_pen, const RowSink<TJ>& sink) {
+                                             RB seed_opl, RB seed_pen, const RowSink<TJ>& sink,
+                                             int l) {
     const int S = P.S;
     const int count = rec.count;
     mkod::RayAdj<RB> adj{{RB(0), RB(0), RB(0)}, {RB(0), RB(0), RB(0)}};
     uint8_t next_branch = 0;
+    RB n_carry = RB(0); // adjoint of index[j] from step j + 1 (or the reference sphere)
+    // JacobianRowWriter::index: the glass variables of the medium after surface j
+    const auto index_sink = [&](int j, RB value) {
+        if (!map.glass) return;
+        const double w = map.omega[l];
+        const double slope[3] = {1.0, w, w * w};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sink.write(map.glass_col[j][c], double(value) * slope[c]);
+    };
     for (int s = count - 1; s >= 0; --s) {
         const V3<RB> p{RB(rec.p[s][0]), RB(rec.p[s][1]), RB(rec.p[s][2])};
         const V3<RB> d{RB(rec.d[s][0]), RB(rec.d[s][1]), RB(rec.d[s][2])};
@@ -1093,6 +1148,7 @@ __device__ __forceinline__ void backward_ray(const TraceParams& P, const SurfLds
                     p, d, V3<RB>{RB(M.x), RB(M.y), RB(M.z)}, RB(kappa), P.ref_kind,
                     RB(L.n[S - 1]), rec.image_branch, seed_opl, seed_pen, adj, nib);
                 sink.write(map.thick_col[S - 1], double(mz));
+                n_carry = nib;
             }
         } else {
             const int j = s + 1;
@@ -1102,6 +1158,8 @@ __device__ __forceinline__ void backward_ray(const TraceParams& P, const SurfLds
                                                tb, cb, nbb, nab);
             if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], double(cb));
             sink.write(map.thick_col[s], double(tb));
+            index_sink(j, nab + n_carry);
+            n_carry = nbb;
         }
         next_branch = rec.branch[s];
     }
@@ -1159,11 +1217,11 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
             sw += s.weight;
             swo += s.weight * r.opl;
             backward_ray<RB, RR, TJ>(P, L, map, rec, true, M, kappa, RB(1), RB(0),
-                                     RowSink<TJ>{col0, R, row, 1.0});
+                                     RowSink<TJ>{col0, R, row, 1.0}, t.l);
         }
         if (kSoft && soft.first != MKO_STATUS_OK) {
             backward_ray<RB, RR, TJ>(P, L, map, rec, reached, M, kappa, RB(0), RB(1),
-                                     RowSink<TJ>{col0, R, P.rays + row, t.pscale * s.rootw});
+                                     RowSink<TJ>{col0, R, P.rays + row, t.pscale * s.rootw}, t.l);
         }
     }
     const double w_ok = block_sum(sw, red);
@@ -1825,22 +1883,64 @@ void launch_trace(int mode, const TraceParams& P, double* res, double* opd, uint
     check_hip(hipGetLastError(), "trace kernel launch");
 }
 
-ParamMap make_map(const mko_variables& vars, int S) {
+// buchdahl_omega (dispersion.hpp:18-89) of nm / 1000 on the host, contract off
+double buchdahl_omega_host(double lambda_nm, bool& ok) {
+#pragma clang fp contract(off)
+    const double um = lambda_nm / 1000.0;
+    const double den = 1.0 + 2.5 * (um - 0.546074);
+    ok = ok && den > 0.0;
+    return (um - 0.546074) / den;
+}
+
+OmegaTable omega_table(const mko_batch& b) {
+    OmegaTable t{};
+    t.valid = 0;
+    if (!b.wavelength_nm || b.wavelengths > kMaxWavelengths) return t;
+    bool ok = true;
+    for (int l = 0; l < b.wavelengths; ++l) t.w[l] = buchdahl_omega_host(b.wavelength_nm[l], ok);
+    t.w[b.wavelengths] = buchdahl_omega_host(b.reference_wavelength_nm, ok);
+    t.valid = ok ? 1 : 0;
+    return t;
+}
+
+// Column map of the variables (AdjointParameters, src/optim/adjoint_jacobian.cpp:79-127); glass
+// variables need the batch's wavelengths (host) for their slopes, and `allow_glass`.
+ParamMap make_map(const mko_variables& vars, const mko_batch& batch, bool allow_glass) {
+    const int S = batch.surfaces;
     ParamMap m{};
     m.P = vars.count;
     if (vars.count < 1 || vars.count > 127) throw std::invalid_argument("1 ... 127 variables");
     for (int s = 0; s < MKO_MAX_SURFACES; ++s) {
         m.curv_col[s] = -1;
         m.thick_col[s] = -1;
+        for (int c = 0; c < 3; ++c) m.glass_col[s][c] = -1;
     }
     for (int k = 0; k < vars.count; ++k) {
         const int s = vars.batch_surface[k];
         if (s < 0 || s >= S) throw std::invalid_argument("variable surface out of range");
-        int8_t& slot = vars.type[k] == MKO_VAR_CURVATURE ? m.curv_col[s] : m.thick_col[s];
-        if (vars.type[k] < MKO_VAR_CURVATURE || vars.type[k] > MKO_VAR_WORKING_DISTANCE)
+        if (vars.type[k] < MKO_VAR_CURVATURE || vars.type[k] > MKO_VAR_GLASS)
             throw std::invalid_argument("unsupported variable type");
-        if (slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
-        slot = int8_t(k);
+        int8_t* slot = nullptr;
+        if (vars.type[k] == MKO_VAR_GLASS) {
+            if (!allow_glass)
+                throw std::invalid_argument("glass variables need the FP64 adjoint (not the fused "
+                                            "normal equations)");
+            if (!vars.component || vars.component[k] < 0 || vars.component[k] > 2)
+                throw std::invalid_argument("glass variables need components 0 ... 2");
+            slot = &m.glass_col[s][vars.component[k]];
+            m.glass = 1;
+        } else {
+            slot = vars.type[k] == MKO_VAR_CURVATURE ? &m.curv_col[s] : &m.thick_col[s];
+        }
+        if (*slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
+        *slot = int8_t(k);
+    }
+    if (m.glass) {
+        const OmegaTable t = omega_table(batch);
+        if (!t.valid)
+            throw std::invalid_argument("glass variables need wavelength_nm with a defined Buchdahl "
+                                        "coordinate (lambda > 146.074 nm)");
+        for (int l = 0; l < batch.wavelengths; ++l) m.omega[l] = t.w[l];
     }
     return m;
 }
@@ -1902,20 +2002,23 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
                 int runs, int per_run, const mko_variables* vars_device, const double* x0,
                 const double* plus, const double* minus, double* curvature, double* thickness,
                 double* index, double* pupil_radius, double* field_angle, double* field_target,
-                double* reference_curvature, double* paraxial_index, void* stream,
-                const int32_t* live) {
+                double* reference_curvature, double* paraxial_index, double* buchdahl,
+                void* stream, const int32_t* live) {
     MKO_TRY({
         if (runs <= 0) return 0;
         const ConstBatchPtrs src{templ->curvature, templ->thickness, templ->index,
                                  templ->pupil_radius, templ->field_angle, templ->field_target,
-                                 templ->reference_curvature, templ->paraxial_index};
+                                 templ->reference_curvature, templ->paraxial_index,
+                                 templ->buchdahl};
         const BatchPtrs dst{curvature, thickness, index, pupil_radius, field_angle, field_target,
-                            reference_curvature, paraxial_index};
+                            reference_curvature, paraxial_index, buchdahl};
+        const OmegaTable om = omega_table(*templ);
         hipLaunchKernelGGL(point_batch_kernel, dim3(unsigned(runs) * unsigned(per_run)), dim3(64),
                            0, static_cast<hipStream_t>(stream), src, dst, runs, per_run,
                            template_index, template_per_run, templ->surfaces, templ->wavelengths,
                            templ->fields, vars_device->count, vars_device->type,
-                           vars_device->batch_surface, x0, plus, minus, live);
+                           vars_device->batch_surface, vars_device->component, x0, plus, minus,
+                           live, om);
         check_hip(hipGetLastError(), "point batch launch");
     })
 }
@@ -1927,7 +2030,7 @@ int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* s
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
         P.live = live;
-        const ParamMap m = make_map(*vars, batch->surfaces);
+        const ParamMap m = make_map(*vars, *batch, false);
         if (P.soft) throw std::invalid_argument("fused adjoint normal equations need hard failure mode");
         if (m.P > 64) throw std::invalid_argument("fused adjoint normal equations: at most 64 variables");
         const int rpl = rays_per_lane(P);
@@ -2147,7 +2250,7 @@ int mko_point_batch_device(const mko_batch* templ, int template_per_run, int run
     return mko_internal::point_batch(templ, nullptr, template_per_run, runs, per_run, vars_device,
                                      x0, plus, minus, curvature, thickness, index, pupil_radius,
                                      field_angle, field_target, reference_curvature,
-                                     paraxial_index, stream);
+                                     paraxial_index, nullptr, stream);
 }
 
 int mko_fd_difference_device(const double* fd_residuals, int64_t stride,
@@ -2191,7 +2294,7 @@ int adjoint_jacobian(const mko_batch* batch, const mko_residual_settings* settin
             P.stride = stride;
         }
         const int precision = MKO_PRECISION_FP64;
-        const ParamMap m = make_map(*vars, batch->surfaces);
+        const ParamMap m = make_map(*vars, *batch, true);
         if (P.V == 0) return 0;
         if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
         const int rpl = rays_per_lane(P);

@@ -116,9 +116,12 @@ def fma_exact(a, b, c):
 def perturbed_vectors(variables, x0, count, seed=0, relative=(0.01, 0.01, 0.01),
                       absolute=(0.0, 0.0, 0.0), distribution="uniform", first_is_base=False):
     """Vectors of variants j = 0 ... count-1 (perturbed_vector, variant_generator.cpp:95).
-    relative/absolute: per MKO_VAR_* type (curvature, thickness, working distance)."""
+    relative/absolute: per MKO_VAR_* type (curvature, thickness, working distance[, glass]);
+    the glass scale defaults to PerturbationOptions::glass (0.01 relative)."""
     x0 = np.asarray(x0, dtype=np.float64)
     P = len(x0)
+    relative = tuple(relative) + (0.01,) * (4 - len(relative))
+    absolute = tuple(absolute) + (0.0,) * (4 - len(absolute))
     gen = MT19937_64([variant_seed(seed, j) for j in range(count)])
     out = np.empty((count, P))
     s = [_fma(relative[t], abs(float(x)), absolute[t]) for t, x in zip(variables.type, x0)]

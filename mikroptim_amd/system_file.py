@@ -811,7 +811,7 @@ def optimize(path, engine, runs=1, max_iterations=100, jacobian="fd", precision=
     if v.size == 0:
         raise ValueError(f"{path}: no variables")
     x0 = v.extract(b, 0)
-    starts = (rng.perturbed_vectors(v, x0, runs, seed=seed, relative=(perturbation,) * 3,
+    starts = (rng.perturbed_vectors(v, x0, runs, seed=seed, relative=(perturbation,) * 4,
                                     first_is_base=True) if runs > 1 else x0[None, :])
     method = abi.JACOBIAN_ADJOINT if jacobian == "adjoint" else abi.JACOBIAN_FD
     opt = abi.lm_options(max_iterations=max_iterations, jacobian_method=method)

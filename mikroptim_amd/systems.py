@@ -491,3 +491,28 @@ def many_tuples(variants=3, fields=32, wavelengths=32):
                                relative=(0.05, 0.0, 0.0), first_is_base=True)
     v.apply(b, xs)
     return b, v
+
+
+def glass_singlet(variants=4, seed=13, coeff=(1.5187, -0.0812, 0.0198, 0.004)):
+    """The singlet with its lens medium a Buchdahl glass (n0, nu1, nu2, nu3 = `coeff`, close to
+    N-BK7) and glass variables on it: c[1], c[2], (n0, nu1, nu2) of the lens medium and the
+    working distance (P = 6). The index table comes from the coefficients (VariableSet::apply);
+    per variant the curvatures and n0 are jittered (seeded)."""
+    b, _ = singlet(variants, seed=seed, perturb=0.0)
+    b.buchdahl = np.zeros((variants, b.surfaces, 4))
+    b.buchdahl[:, 1, :] = coeff
+    inf = math.inf
+    v = VariableSet(
+        type=np.array([abi.VAR_CURVATURE, abi.VAR_CURVATURE, abi.VAR_GLASS, abi.VAR_GLASS,
+                       abi.VAR_GLASS, abi.VAR_WORKING_DISTANCE], dtype=np.int32),
+        batch_surface=np.array([1, 2, 1, 1, 1, 2], dtype=np.int32),
+        lower=np.array([-0.1, -0.1, -inf, -inf, -inf, 10.0]),
+        upper=np.array([0.1, 0.1, inf, inf, inf, 500.0]),
+        typical=np.array([0.01, 0.01, *batch_mod.GLASS_TYPICAL, 1.0]),
+        names=["c[1]", "c[2]", "n0[1]", "nu1[1]", "nu2[1]", "w"],
+        component=np.array([0, 0, 0, 1, 2, 0], dtype=np.int32))
+    x0 = v.extract(b, 0)
+    xs = rng.perturbed_vectors(v, x0, variants, seed=seed, relative=(0.05, 0.0, 0.0, 0.002),
+                               first_is_base=True)
+    v.apply(b, xs)
+    return b, v

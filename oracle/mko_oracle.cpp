@@ -873,37 +873,90 @@ void parallel_for(int first, int last, int threads, const std::function<void(int
 
 struct Vars {
     int P = 0;
-    std::vector<int> type, surface;
+    std::vector<int> type, surface, component;
     std::vector<double> lower, upper, typical;
+    std::vector<int> glass_surfaces; // surfaces with glass variables, ascending
+    bool glass() const { return !glass_surfaces.empty(); }
 };
 
 inline Vars make_vars(const mko_variables* v, int S) {
     Vars o;
     o.P = v->count;
     for (int k = 0; k < o.P; ++k) {
-        if (v->type[k] < MKO_VAR_CURVATURE || v->type[k] > MKO_VAR_WORKING_DISTANCE)
+        if (v->type[k] < MKO_VAR_CURVATURE || v->type[k] > MKO_VAR_GLASS)
             throw std::invalid_argument("unsupported variable type");
         if (v->batch_surface[k] < 0 || v->batch_surface[k] >= S)
             throw std::invalid_argument("variable surface out of range");
         if (!(v->typical[k] > 0.0)) throw std::invalid_argument("typical must be > 0");
+        int comp = 0;
+        if (v->type[k] == MKO_VAR_GLASS) {
+            if (!v->component) throw std::invalid_argument("glass variables need components");
+            comp = v->component[k];
+            if (comp < 0 || comp > 2) throw std::invalid_argument("glass component outside 0 ... 2");
+            if (std::find(o.glass_surfaces.begin(), o.glass_surfaces.end(), v->batch_surface[k]) ==
+                o.glass_surfaces.end())
+                o.glass_surfaces.push_back(v->batch_surface[k]);
+        }
         o.type.push_back(v->type[k]);
         o.surface.push_back(v->batch_surface[k]);
+        o.component.push_back(comp);
         o.lower.push_back(v->lower[k]);
         o.upper.push_back(v->upper[k]);
         o.typical.push_back(v->typical[k]);
     }
+    std::sort(o.glass_surfaces.begin(), o.glass_surfaces.end());
     return o;
 }
 
-// VariableSet::apply for curvature / thickness / working distance, src/optim/variable_set.cpp:237
-inline void apply(const Vars& vars, int S, double* curvature, double* thickness, int v,
-                  const double* x) {
+// nm_to_um, buchdahl_omega, buchdahl_index (include/mikroptim/glass/dispersion.hpp:18-95)
+constexpr double kBuchdahlLambda0Um = 0.546074, kBuchdahlAlpha = 2.5;
+inline double buchdahl_omega(double lambda_nm) {
+    const double um = lambda_nm / 1000.0;
+    const double den = 1.0 + kBuchdahlAlpha * (um - kBuchdahlLambda0Um);
+    if (!(den > 0.0)) throw std::invalid_argument("Buchdahl coordinate undefined (lambda <= 146.074 nm)");
+    return (um - kBuchdahlLambda0Um) / den;
+}
+inline double buchdahl_index(const double* c, double lambda_nm) {
+    const double w = buchdahl_omega(lambda_nm);
+    return c[0] + w * (c[1] + w * (c[2] + w * c[3]));
+}
+
+// The per-variant arrays VariableSet::apply writes.
+struct Target {
+    double* curvature;
+    double* thickness;
+    double* index;
+    double* buchdahl; // [V][S][4] or NULL
+    double* pindex;   // [V][S] or NULL
+};
+
+// VariableSet::apply, src/optim/variable_set.cpp:237-285: curvature / thickness / working
+// distance; glass components into the Buchdahl coefficients, then the glass surfaces' index at
+// every wavelength and at lambda_ref.
+inline void apply(const Vars& vars, const mko_batch& b, const Target& t, int v, const double* x) {
+    const size_t S = size_t(b.surfaces), L = size_t(b.wavelengths);
+    if (vars.glass()) {
+        if (!t.buchdahl || !b.wavelength_nm)
+            throw std::invalid_argument("glass variables need Buchdahl coefficients and wavelengths");
+        for (size_t l = 0; l < L; ++l) (void)buchdahl_omega(b.wavelength_nm[l]);
+        (void)buchdahl_omega(b.reference_wavelength_nm);
+    }
     for (int k = 0; k < vars.P; ++k) {
         if (!std::isfinite(x[k])) throw std::invalid_argument("variable value not finite");
+        const size_t s = size_t(vars.surface[k]);
         if (vars.type[k] == MKO_VAR_CURVATURE)
-            curvature[size_t(v) * S + vars.surface[k]] = x[k];
+            t.curvature[size_t(v) * S + s] = x[k];
+        else if (vars.type[k] == MKO_VAR_GLASS)
+            t.buchdahl[(size_t(v) * S + s) * 4 + size_t(vars.component[k])] = x[k];
         else
-            thickness[size_t(v) * S + vars.surface[k]] = x[k];
+            t.thickness[size_t(v) * S + s] = x[k];
+    }
+    for (const int si : vars.glass_surfaces) {
+        const size_t s = size_t(si);
+        const double* c = t.buchdahl + (size_t(v) * S + s) * 4;
+        for (size_t l = 0; l < L; ++l)
+            t.index[(size_t(v) * S + s) * L + l] = buchdahl_index(c, b.wavelength_nm[l]);
+        if (t.pindex) t.pindex[size_t(v) * S + s] = buchdahl_index(c, b.reference_wavelength_nm);
     }
 }
 
@@ -1014,7 +1067,7 @@ inline void perturbed_vector(const Vars& vars, const double* x0, const double* r
 // A batch whose per-variant arrays this oracle owns (FD batches, point batches).
 struct OwnedBatch {
     mko_batch b{};
-    std::vector<double> curvature, thickness, index, pupil, fangle, ftarget, refcurv, pindex;
+    std::vector<double> curvature, thickness, index, pupil, fangle, ftarget, refcurv, pindex, bd;
     // copies variant `from` of `src` into variant `to`
     void init(const mko_batch& src, int V) {
         b = src;
@@ -1029,6 +1082,8 @@ struct OwnedBatch {
         if (src.reference_curvature) refcurv.assign(size_t(V) * F, 0.0);
         if (src.paraxial_index) pindex.assign(size_t(V) * S, 0.0);
         b.paraxial_index = src.paraxial_index ? pindex.data() : nullptr;
+        if (src.buchdahl) bd.assign(size_t(V) * S * 4, 0.0);
+        b.buchdahl = src.buchdahl ? bd.data() : nullptr;
         b.curvature = curvature.data();
         b.thickness = thickness.data();
         b.index = index.data();
@@ -1049,6 +1104,11 @@ struct OwnedBatch {
             std::copy_n(src.reference_curvature + from * F, F, refcurv.data() + to * F);
         if (src.paraxial_index)
             std::copy_n(src.paraxial_index + from * S, S, pindex.data() + to * S);
+        if (src.buchdahl) std::copy_n(src.buchdahl + from * S * 4, S * 4, bd.data() + to * S * 4);
+    }
+    Target target() {
+        return Target{curvature.data(), thickness.data(), index.data(),
+                      bd.empty() ? nullptr : bd.data(), pindex.empty() ? nullptr : pindex.data()};
     }
 };
 
@@ -1084,12 +1144,12 @@ void fd_jacobian(const Batch& base, const Layout& lay, const Vars& vars, int run
         pats.push_back(fd_pattern(vars, xi));
         for (int j = 0; j < n; ++j) fd.copy_variant(base.b, tmpl, i * n + j);
         std::copy_n(xi, P, y.data());
-        apply(vars, S, fd.curvature.data(), fd.thickness.data(), i * n, y.data());
+        apply(vars, fd.b, fd.target(), i * n, y.data());
         for (int k = 0; k < P; ++k) {
             y[k] = pats[i].plus[k];
-            apply(vars, S, fd.curvature.data(), fd.thickness.data(), i * n + 2 * k + 1, y.data());
+            apply(vars, fd.b, fd.target(), i * n + 2 * k + 1, y.data());
             y[k] = pats[i].minus[k];
-            apply(vars, S, fd.curvature.data(), fd.thickness.data(), i * n + 2 * k + 2, y.data());
+            apply(vars, fd.b, fd.target(), i * n + 2 * k + 2, y.data());
             y[k] = xi[k];
         }
     }
@@ -1321,7 +1381,7 @@ inline void image_step_backward(const Vec3& p, const Vec3& d, const Vec3& center
 }
 
 // Writes the per-ray derivative of one seed into J rows (JacobianRowWriter,
-// include/mikroptim/merit/ray_jacobian.hpp:50-78; only curvature/thickness parameters exist).
+// include/mikroptim/merit/ray_jacobian.hpp:50-78; curvature, thickness and glass parameters).
 struct RowWriter {
     const int* curv_col; // [S] variable index or -1
     const int* thick_col;
@@ -1329,8 +1389,17 @@ struct RowWriter {
     int64_t rows;
     int64_t row;
     double factor;
+    const int* glass_col = nullptr; // [S][3] variable of (n0, nu1, nu2) of the medium after s
+    const double* slope = nullptr;  // [3] dn/d(n0, nu1, nu2) at this wavelength: 1, w, w^2
     void write(int p, double value) const {
         if (p >= 0) column0[int64_t(p) * rows + row] = factor * value;
+    }
+    void index(int s, double value) const {
+        if (!glass_col) return;
+        for (int c = 0; c < 3; ++c) {
+            const int p = glass_col[s * 3 + c];
+            if (p >= 0) write(p, value * slope[c]);
+        }
     }
 };
 
@@ -1339,6 +1408,7 @@ inline void backward_ray(const Table& tb, int l, const Record& rec, bool reached
                          const RefSphere& ref, bool point_source, Seed seed, const RowWriter& w) {
     const int S = tb.S, L = tb.L, count = rec.count;
     RayAdj adj{{0, 0, 0}, {0, 0, 0}};
+    double n_carry = 0.0; // adjoint of index[j] from step j + 1 (or the reference sphere)
     uint8_t next_branch = kBranchRegular;
     Vec3 d0{0, 0, 0};
     for (int s = count - 1; s >= 0; --s) {
@@ -1356,6 +1426,7 @@ inline void backward_ray(const Table& tb, int l, const Record& rec, bool reached
                     image_step_backward(p, d, ref.center, ref.kappa, n_image, rec.image_branch,
                                         seed, adj, center_adj, n_image_adj);
                 w.write(w.thick_col[S - 1], center_adj.z);
+                n_carry = n_image_adj;
             }
         } else {
             const int j = s + 1;
@@ -1366,12 +1437,15 @@ inline void backward_ray(const Table& tb, int l, const Record& rec, bool reached
                                      next_branch, seed, adj, par);
             if (kind == MKO_SURFACE_SPHERE) w.write(w.curv_col[j], par.curvature);
             w.write(w.thick_col[s], par.thickness);
+            w.index(j, par.n_after + n_carry);
+            n_carry = par.n_before;
         }
         next_branch = rec.branch[s];
         if (s == 0) d0 = d;
     }
+    // the point source's direction derivative has no parameter of this build (no decenter)
     (void)point_source;
-    (void)d0; // index derivatives (glass variables) are not part of this build
+    (void)d0;
 }
 
 // adjoint_jacobian_cpu, src/optim/adjoint_jacobian.cpp:370-481 (no extra rows)
@@ -1396,7 +1470,7 @@ void extra_rows(const Batch& base, const Layout& lay, const Vars& vars, int runs
         std::vector<double> e0(static_cast<size_t>(E)), ea(static_cast<size_t>(E)),
             eb(static_cast<size_t>(E));
         const auto rows_at = [&](std::vector<double>& out) {
-            apply(vars, S, one.curvature.data(), one.thickness.data(), 0, y.data());
+            apply(vars, one.b, one.target(), 0, y.data());
             variant_rows_traced(OB, 0, lay, out.data());
         };
         rows_at(e0);
@@ -1423,14 +1497,18 @@ void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, in
     ab.init(base.b, runs);
     for (int i = 0; i < runs; ++i) {
         ab.copy_variant(base.b, base.b.variants == 1 ? 0 : i, i);
-        apply(vars, S, ab.curvature.data(), ab.thickness.data(), i, x0 + size_t(i) * P);
+        apply(vars, ab.b, ab.target(), i, x0 + size_t(i) * P);
     }
     const Batch B{ab.b};
-    std::vector<int> curv_col(S, -1), thick_col(S, -1);
+    std::vector<int> curv_col(S, -1), thick_col(S, -1), glass_col(size_t(S) * 3, -1);
     for (int k = 0; k < P; ++k) {
         if (vars.type[k] == MKO_VAR_CURVATURE) {
             if (curv_col[vars.surface[k]] >= 0) throw std::invalid_argument("duplicate parameter");
             curv_col[vars.surface[k]] = k;
+        } else if (vars.type[k] == MKO_VAR_GLASS) {
+            int& c = glass_col[size_t(vars.surface[k]) * 3 + size_t(vars.component[k])];
+            if (c >= 0) throw std::invalid_argument("duplicate parameter");
+            c = k;
         } else {
             if (thick_col[vars.surface[k]] >= 0) throw std::invalid_argument("duplicate parameter");
             thick_col[vars.surface[k]] = k;
@@ -1447,6 +1525,16 @@ void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, in
     const bool soft_mode = lay.soft();
     const bool point_source = B.b.source == MKO_SOURCE_POINT;
     const int F = lay.F;
+    // dn(lambda_l)/d(n0, nu1, nu2) = (1, w_l, w_l^2) (AdjointParameters, adjoint_jacobian.cpp:
+    // 107-127)
+    std::vector<double> slope(size_t(lay.L) * 3, 0.0);
+    for (int l = 0; l < lay.L; ++l) {
+        const double w = vars.glass() ? buchdahl_omega(B.b.wavelength_nm[l]) : 0.0;
+        slope[size_t(l) * 3 + 0] = 1.0;
+        slope[size_t(l) * 3 + 1] = w;
+        slope[size_t(l) * 3 + 2] = w * w;
+    }
+    const int* gcol = vars.glass() ? glass_col.data() : nullptr;
     parallel_for(0, runs * F, threads, [&](int pair) {
         const int v = pair / F;
         const int f = pair % F;
@@ -1474,9 +1562,11 @@ void adjoint_jacobian(const Batch& base, const Layout& lay, const Vars& vars, in
                 weight[k] = s.weight;
                 root_weight[k] = residual_weight(s);
                 piston[k] = in_piston(st) ? 1 : 0;
-                const RowWriter opl_row{curv_col.data(), thick_col.data(), col0, R, offset + k, 1.0};
+                const double* sl = slope.data() + size_t(l) * 3;
+                const RowWriter opl_row{curv_col.data(), thick_col.data(), col0, R, offset + k,
+                                        1.0, gcol, sl};
                 const RowWriter pen_row{curv_col.data(), thick_col.data(), col0, R,
-                                        rays + offset + k, pscale * root_weight[k]};
+                                        rays + offset + k, pscale * root_weight[k], gcol, sl};
                 // ray_gradient_rows, include/mikroptim/merit/ray_jacobian.hpp:93-107
                 const bool reached = in_piston(st);
                 if (reached) backward_ray(tb, l, rec, true, ref, point_source, Seed{1.0, 0.0}, opl_row);
@@ -1874,8 +1964,7 @@ Backend cpu_backend(const Batch& base, const Layout& lay, const Vars& vars, int 
         pb.init(base.b, count);
         for (int j = 0; j < count; ++j) {
             pb.copy_variant(base.b, base.b.variants == 1 ? 0 : runs[size_t(j)], j);
-            apply(vars, base.S(), pb.curvature.data(), pb.thickness.data(), j,
-                  x.data() + size_t(j) * P);
+            apply(vars, pb.b, pb.target(), j, x.data() + size_t(j) * P);
         }
         const Batch PB{pb.b};
         parallel_for(0, count, threads, [&](int v) {
@@ -2461,8 +2550,11 @@ int orc_normal_equations(const double* J, const double* r, int runs, int P, int6
 // Writes x0 of each run into a copy of the base variant (VariableSet::apply).
 int orc_apply(mko_batch* b, const mko_variables* vars, int v, const double* x) {
     ORC_TRY({
-        apply(make_vars(vars, b->surfaces), b->surfaces, const_cast<double*>(b->curvature),
-              const_cast<double*>(b->thickness), v, x);
+        apply(make_vars(vars, b->surfaces), *b,
+              Target{const_cast<double*>(b->curvature), const_cast<double*>(b->thickness),
+                     const_cast<double*>(b->index), const_cast<double*>(b->buchdahl),
+                     const_cast<double*>(b->paraxial_index)},
+              v, x);
     })
 }
 

@@ -58,7 +58,8 @@ mko::SystemBatchHost batch_of(const mko_batch& b) {
     h.source_na = b.source_na;
     for (size_t l = 0; l < L; ++l) {
         h.wavelength_weight(int(l)) = b.wavelength_weight[l];
-        h.wavelength_nm(int(l)) = 500.0; // not used by the trace
+        // used only by glass variables (Buchdahl index of each wavelength)
+        h.wavelength_nm(int(l)) = b.wavelength_nm ? b.wavelength_nm[l] : 500.0;
     }
     for (size_t f = 0; f < F; ++f) h.field_weight(int(f)) = b.field_weight[f];
     std::copy_n(b.curvature, V * S, h.curvature_data().begin());
@@ -70,6 +71,7 @@ mko::SystemBatchHost batch_of(const mko_batch& b) {
     if (b.reference_sphere != MKO_REF_LAST_SURFACE)
         std::copy_n(b.reference_curvature, V * F, h.reference_curvature_data().begin());
     if (b.paraxial_index) std::copy_n(b.paraxial_index, V * S, h.paraxial_index_data().begin());
+    if (b.buchdahl) std::copy_n(b.buchdahl, V * S * 4, h.buchdahl_coeff_data().begin());
     return h;
 }
 
@@ -157,14 +159,35 @@ mko::SystemDescription description_of(const mko_batch& b, const mko_variables& v
                                                    : mko::SurfaceType::Sphere;
         sf.curvature = b.curvature[s];
         sf.thickness = b.thickness[s];
+        // glass-variable media as Buchdahl materials (variant 0's coefficients), as
+        // convert_glass_variables leaves them
+        if (b.buchdahl) {
+            for (int k = 0; k < v.count; ++k) {
+                if (v.type[k] == MKO_VAR_GLASS && v.batch_surface[k] == s) {
+                    const double* c = b.buchdahl + size_t(s) * 4;
+                    sf.material.kind = mko::MaterialKind::Buchdahl;
+                    sf.material.n0 = c[0];
+                    sf.material.nu = {c[1], c[2], c[3]};
+                }
+            }
+        }
         d.surfaces.push_back(sf);
     }
     mko::Variables vars;
     for (int k = 0; k < v.count; ++k) {
+        // a glass entry stands for its three components (resolve_variables expands it)
+        if (v.type[k] == MKO_VAR_GLASS && v.component[k] != 0) continue;
         mko::VariableSpec spec;
         spec.type = v.type[k] == MKO_VAR_CURVATURE   ? mko::VariableType::Curvature
                     : v.type[k] == MKO_VAR_THICKNESS ? mko::VariableType::Thickness
+                    : v.type[k] == MKO_VAR_GLASS     ? mko::VariableType::Glass
                                                      : mko::VariableType::WorkingDistance;
+        if (v.type[k] == MKO_VAR_GLASS) {
+            spec.selection = mko::SurfaceSelection::One;
+            spec.surfaces = {v.batch_surface[k]};
+            vars.items.push_back(spec);
+            continue;
+        }
         if (v.type[k] == MKO_VAR_WORKING_DISTANCE) {
             spec.selection = mko::SurfaceSelection::None;
         } else {

@@ -1,0 +1,61 @@
+"""Glass variables on the HIP path: the point batch sets the Buchdahl coefficients and the
+glass surfaces' indices on the device, the FD Jacobian differences those traces, the FP64 adjoint
+carries the index adjoints into the (n0, nu1, nu2) columns, the mixed precision falls back to
+FP64 (the fused reverse sweep has no index adjoints), and the LM runs with them, device control
+equal to host control bit for bit. Tolerances as in test_gpu_parity.py / test_gpu_limits.py."""
+
+import numpy as np
+import pytest
+
+from mikroptim_amd import abi, systems
+from oracle_lib import oracle
+from test_gpu_limits import _check_jacobians, _check_trace
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("failure", [abi.FAILURE_HARD, abi.FAILURE_SOFT])
+def test_glass_traces_and_jacobians_match_oracle(engine, failure):
+    b, v = systems.glass_singlet()
+    s = systems.singlet_settings(failure)
+    _check_trace(engine, b, s)
+    _check_jacobians(engine, b, v, s, runs=3)
+
+
+def test_glass_mixed_precision_is_fp64(engine):
+    b, v = systems.glass_singlet()
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    base, x0 = b.take([0]), v.extract(b)[:3]
+    _, _, am, gm = engine.jacobian(base, s, v, x0, method="adjoint", precision="mixed",
+                                   want_jacobian=False)
+    _, _, ad, gd = engine.jacobian(base, s, v, x0, method="adjoint", precision="fp64")
+    assert np.array_equal(am, ad) and np.array_equal(gm, gd)
+
+
+@pytest.mark.parametrize("method", [abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT])
+def test_glass_lm_device_equals_host(engine, method):
+    b, v = systems.glass_singlet()
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    base, x0 = b.take([0]), v.extract(b)[:3]
+    kw = dict(max_iterations=10, threads=4, jacobian_method=method)
+    for precision in ("fp64", "mixed"):
+        rd = engine.optimize(base, s, v, x0, abi.lm_options(**kw), precision=precision)
+        rh = engine.optimize(base, s, v, x0, abi.lm_options(host_control=True, **kw),
+                             precision=precision)
+        assert np.array_equal(rd.x, rh.x) and np.array_equal(rd.merit, rh.merit)
+        assert rd.status == rh.status and rd.cycles == rh.cycles
+
+
+def test_glass_lm_follows_oracle(engine):
+    b, v = systems.glass_singlet()
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    base, x0 = b.take([0]), v.extract(b)[:3]
+    pts = base.take(np.zeros(3, dtype=np.int64))
+    v.apply(pts, x0)
+    m0, _, _ = engine.merit(pts, s)
+    for method in (abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT):
+        opt = abi.lm_options(max_iterations=10, threads=4, jacobian_method=method)
+        res = engine.optimize(base, s, v, x0, opt)
+        _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, opt)
+        assert np.allclose(res.merit, [r.merit for r in runs_o], rtol=1e-3)
+        assert np.all(res.merit < m0)
