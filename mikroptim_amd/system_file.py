@@ -13,11 +13,13 @@ src/optim/mechanical_constraints.cpp:87-146) and the optimize output (docs/cli.m
 Supported is what this path computes: stop / plane / sphere surfaces without decenter or tilt,
 catalogue (Sellmeier, polynomial), constant and Buchdahl materials, collimated and point sources,
 the three reference spheres with the ideal image point, curvature / thickness / working-distance
-variables, paraxial (focal length, parfocal length, working distance, paraxial focus, Petzval) and
-mechanical constraints. Ghost plates, glass and ghost-coefficient variables, the ghost-penalty,
-sine-condition and distortion constraints, smoothed extrema and traced reference points are
-rejected with a FormatError naming the field; tolerances and a glass hull (a penalty on glass
-variables only) are ignored with a warning.
+variables, glass variables on Buchdahl and constant media (a constant medium becomes Buchdahl
+with n0 = n, as convert_glass_variables does), paraxial (focal length, parfocal length, working
+distance, paraxial focus, Petzval) and mechanical constraints. Ghost plates, ghost-coefficient
+variables, glass variables on catalogue glasses (they need the Buchdahl fit), the ghost-penalty,
+glass-hull (with glass variables), sine-condition and distortion constraints, smoothed extrema
+and traced reference points are rejected with a FormatError naming the field; tolerances and a
+glass hull without glass variables are ignored with a warning.
 """
 
 import json
@@ -248,9 +250,9 @@ def _material(m, path, origin):
     raise FormatError(path, "unknown material", origin)
 
 
-UNSUPPORTED_VARIABLES = ("ghost_coefficient", "glass")
+UNSUPPORTED_VARIABLES = ("ghost_coefficient",)
 VARIABLE_TYPES = {"curvature": abi.VAR_CURVATURE, "thickness": abi.VAR_THICKNESS,
-                  "working_distance": abi.VAR_WORKING_DISTANCE}
+                  "working_distance": abi.VAR_WORKING_DISTANCE, "glass": abi.VAR_GLASS}
 # kDefaultTypical* (include/mikroptim/model/variables.hpp:67-69)
 DEFAULT_TYPICAL = {"curvature": 0.01, "thickness": 1.0, "working_distance": 1.0}
 UNSUPPORTED_CONSTRAINTS = ("ghost_penalty", "sine_condition", "distortion")
@@ -397,6 +399,7 @@ class ResolvedVariable:
     typical: float
     name: str
     item: int
+    component: int = 0
 
 
 def _variables(sec, sysf, origin):
@@ -414,6 +417,12 @@ def _variables(sec, sysf, origin):
             raise FormatError(p + ".type", f"'{t}' variables are not part of this path", origin)
         if t not in VARIABLE_TYPES:
             raise FormatError(p + ".type", f"unknown variable type '{t}'", origin)
+        if t == "glass":
+            # three components with their own scales; the glass hull acts as a penalty, not as
+            # bounds (resolve_variables, src/model/variables.cpp:364-383)
+            for key in ("min", "max", "typical"):
+                if key in it:
+                    raise FormatError(f"{p}.{key}", "not allowed for glass", origin)
         lo = _num(it["min"], p + ".min", origin) if "min" in it else -math.inf
         hi = _num(it["max"], p + ".max", origin) if "max" in it else math.inf
         for key, x in (("min", lo), ("max", hi)):
@@ -424,7 +433,7 @@ def _variables(sec, sysf, origin):
             raise FormatError(p + ".max", f"upper bound {hi} must exceed the lower bound {lo}",
                               origin)
         typ = _num(it["typical"], p + ".typical", origin) if "typical" in it \
-            else DEFAULT_TYPICAL[t]
+            else DEFAULT_TYPICAL.get(t, 1.0)
         if not (math.isfinite(typ) and typ > 0):
             raise FormatError(p + ".typical", "typical size must be finite and > 0", origin)
         if t == "working_distance":
@@ -438,8 +447,13 @@ def _variables(sec, sysf, origin):
         elif "surface" in it:
             targets = [(_index_of(it["surface"], p + ".surface", origin), p + ".surface")]
         elif it["surfaces"] == "all":
-            targets = [(i, p + ".surfaces") for i in range(S)
-                       if (sysf.surfaces[i].type == "sphere" if t == "curvature" else i < S - 1)]
+            def take(i):
+                if t == "curvature":
+                    return sysf.surfaces[i].type == "sphere"
+                if t == "glass":
+                    return sysf.surfaces[i].material != "AIR"
+                return i < S - 1
+            targets = [(i, p + ".surfaces") for i in range(S) if take(i)]
             if not targets:
                 raise FormatError(p + ".surfaces", '"all" finds no surface', origin)
         else:
@@ -458,10 +472,28 @@ def _variables(sec, sysf, origin):
             if t == "thickness" and i == S - 1:
                 raise FormatError(tp, "the thickness of the last surface is the working "
                                   "distance", origin)
+            if t == "glass" and sysf.surfaces[i].material == "AIR":
+                raise FormatError(tp, f"glass variable needs a glass after surface {i}", origin)
             for prev in out:
                 if prev.type == vt and prev.surface == i:
                     raise FormatError(tp, f"variable twice: {t} of surface {i} is already in "
                                       f"variables.items[{prev.item}]", origin)
+            if t == "glass":
+                # convert_glass_variables (src/optim/glass_variables.cpp:26-62): a constant
+                # medium becomes Buchdahl (n0 = n, nu = 0); a catalogue glass needs the Buchdahl
+                # fit (fit_buchdahl), not part of this path
+                m = sysf.surfaces[i].material
+                if isinstance(m, tuple) and m[0] == "constant":
+                    sysf.surfaces[i].material = ("buchdahl", m[1], 0.0, 0.0, 0.0)
+                elif not (isinstance(m, tuple) and m[0] == "buchdahl"):
+                    raise FormatError(tp, f"glass variable on the catalogue glass '{m}' needs "
+                                      "its Buchdahl fit (not part of this path; give the medium "
+                                      "as a buchdahl material)", origin)
+                for c in range(3):
+                    name = ("n0", "nu1", "nu2")[c] + f"[{i}]"
+                    out.append(ResolvedVariable(vt, i, -math.inf, math.inf,
+                                                bt.GLASS_TYPICAL[c], name, k, c))
+                continue
             start = sysf.surfaces[i].curvature if t == "curvature" else sysf.surfaces[i].thickness
             if start < lo:
                 raise FormatError(p + ".min", f"start value {start} below the lower bound", origin)
@@ -474,11 +506,13 @@ def _variables(sec, sysf, origin):
 
 def variable_set(sysf):
     v = sysf.variables
+    glass = any(x.type == abi.VAR_GLASS for x in v)
     return bt.VariableSet(
         type=np.array([x.type for x in v], dtype=np.int32),
         batch_surface=np.array([x.surface for x in v], dtype=np.int32),
         lower=np.array([x.lower for x in v]), upper=np.array([x.upper for x in v]),
-        typical=np.array([x.typical for x in v]), names=[x.name for x in v])
+        typical=np.array([x.typical for x in v]), names=[x.name for x in v],
+        component=np.array([x.component for x in v], dtype=np.int32) if glass else None)
 
 
 # ---- constraints (make_paraxial_constraints / make_mechanical_constraints) --------------------------
@@ -549,8 +583,11 @@ def _constraints(sec, sysf, origin):
         if t in UNSUPPORTED_CONSTRAINTS:
             raise FormatError(p + ".type", f"'{t}' constraints are not part of this path", origin)
         if t == "glass_hull":
-            # a penalty on glass variables only (optimize_command.cpp:1530-1539); without them
-            # (and glass variables are not part of this path) the entry adds no row
+            # a penalty on glass variables only (optimize_command.cpp:1530-1539): without them
+            # the entry adds no row; its rows with glass variables are not part of this path
+            if any(x.type == abi.VAR_GLASS for x in sysf.variables):
+                raise FormatError(p + ".type", "glass_hull rows are not part of this path",
+                                  origin)
             sysf.warnings.append(f"{p}: glass_hull without glass variables adds no residual")
             continue
         if t not in PARAXIAL_TYPES and t not in MECHANICAL_TYPES:
@@ -716,6 +753,12 @@ def make_batch(sysf, variants=1):
         for l, nm in enumerate(sysf.wavelengths_nm):
             b.index[:, s, l] = material_index(sysf, surf.material, nm)
     npar = [material_index(sysf, s.material, sysf.reference_nm) for s in sysf.surfaces]
+    # Buchdahl coefficients of the Buchdahl media (build_batch, system_batch.cpp:890-903)
+    if any(isinstance(s.material, tuple) and s.material[0] == "buchdahl" for s in sysf.surfaces):
+        b.buchdahl = np.zeros((variants, S, 4))
+        for si, surf in enumerate(sysf.surfaces):
+            if isinstance(surf.material, tuple) and surf.material[0] == "buchdahl":
+                b.buchdahl[:, si, :] = surf.material[1:5]
     b.paraxial_index = np.tile(np.array(npar), (variants, 1))
     b.pupil_radius[:] = sysf.surfaces[0].semi_diameter
     par = _paraxial(c, t, npar)

@@ -288,7 +288,10 @@ def _set(path, value):
      "variables.items[1].surface"),
     (_set(["variables", "items", 1], {"type": "curvature", "surface": 1}),
      "variables.items[1].surface"),
-    (_set(["variables", "items", 1], {"type": "glass", "surface": 1}), "variables.items[1].type"),
+    (_set(["variables", "items", 1], {"type": "glass", "surface": 1}),     # catalogue glass
+     "variables.items[1].surface"),
+    (_set(["variables", "items", 1], {"type": "ghost_coefficient", "plate": 0}),
+     "variables.items[1].type"),
     (_set(["constraints", "items", 0, "mode"], "max"), "constraints.items[0].mode"),
     (_set(["constraints", "items", 0, "weight"], -1.0), "constraints.items[0].weight"),
     (_set(["constraints", "items", 0, "surface"], 1), "constraints.items[0].surface"),
@@ -400,3 +403,46 @@ def test_soft_failure_start_as_documented():
     assert "%.1e" % m[0] == "2.7e+04" and h[0] == 27
     _, runs, _ = oracle().optimize_lm(b, s, v, x0, abi.lm_options(threads=8))
     assert "%.0f" % runs[0].merit == "709" and runs[0].hard_failed_rays == 27
+
+
+def test_glass_variables_on_constant_and_buchdahl_media(tmp_path):
+    """Glass variables (resolve_variables expands one entry into n0, nu1, nu2; typical sizes 1,
+    0.01, 0.001; no bounds): a constant medium becomes Buchdahl with n0 = n (the index is
+    unchanged, so is the merit), the columns follow the oracle's FD; a catalogue glass needs the
+    Buchdahl fit and is refused."""
+    def edit(d):
+        d["variables"]["items"].append({"type": "glass", "surface": 1})
+    path = _edited(tmp_path, edit, PETZVAL)
+    ld = sf.load(path)
+    names = ld.variables.names
+    assert names[-3:] == ["n0[1]", "nu1[1]", "nu2[1]"]
+    assert list(ld.variables.typical[-3:]) == [1.0, 0.01, 0.001]
+    assert list(ld.variables.component[-3:]) == [0, 1, 2]
+    assert np.all(np.isinf(ld.variables.lower[-3:]))
+    assert list(ld.batch.buchdahl[0, 1]) == [1.8, 0.0, 0.0, 0.0]
+    plain = sf.load(PETZVAL)
+    assert np.array_equal(ld.batch.index, plain.batch.index)
+    m, _, _ = oracle().merit(ld.batch, ld.settings)
+    mp, _, _ = oracle().merit(plain.batch, plain.settings)
+    assert m[0] == mp[0]
+    b, v, s = ld.batch, ld.variables, ld.settings
+    s.failure = abi.FAILURE_HARD
+    x0 = v.extract(b)[:1]
+    Ja, _, _ = oracle().adjoint_jacobian(b, s, v, x0)
+    Jf, _, _ = oracle().fd_jacobian(b, s, v, x0)
+    # n0 (the file's one wavelength, 546.07 nm, sits at the Buchdahl origin: w = -2.6e-6, so the
+    # nu columns are too small for a central difference to resolve)
+    k = len(names) - 3
+    assert np.linalg.norm(Ja[0, k] - Jf[0, k]) <= 1e-4 * np.linalg.norm(Jf[0, k])
+
+    def catalogue(d):
+        d["variables"]["items"].append({"type": "glass", "surface": 3})   # CAF2
+    with pytest.raises(sf.FormatError) as e:
+        sf.read_system(_edited(tmp_path, catalogue, PETZVAL))
+    assert e.value.json_path == "variables.items[3].surface"
+
+    def bounded(d):
+        d["variables"]["items"].append({"type": "glass", "surface": 1, "min": 1.0})
+    with pytest.raises(sf.FormatError) as e:
+        sf.read_system(_edited(tmp_path, bounded, PETZVAL))
+    assert e.value.json_path == "variables.items[3].min"
