@@ -220,6 +220,14 @@ void mko_lm_destroy(mko_lm* lm);
  * with the fused mixed-precision adjoint, see mko_lm_options.host_control); no counterpart in the
  * reference. */
 int mko_lm_graph_cycles(const mko_lm* lm);
+/* Split form of mko_lm_cycle for LMs that run at once (for instance one per topology of an
+ * uneven batch): mko_lm_cycle_begin launches the session's captured cycle on its own stream and
+ * returns 1 without waiting (or runs a whole eager cycle and returns 2; 0: the LM is done, < 0:
+ * error); mko_lm_cycle_end waits for it and returns what mko_lm_cycle would (1 while runs are
+ * active). Begin every session's cycle, then end them: their graphs run concurrently. Results
+ * equal those of mko_lm_cycle bit for bit. No counterpart in the reference. */
+int mko_lm_cycle_begin(mko_lm* lm);
+int mko_lm_cycle_end(mko_lm* lm);
 
 #ifdef __cplusplus
 }

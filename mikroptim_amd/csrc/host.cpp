@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -960,9 +961,14 @@ public:
     // ---- device-resident variants for the LM control on the device -------------------------
 
     // Variable types / surfaces for the point batches (host copy kept for the adjoint map).
-    void set_variables(const mko_variables& v) {
+    // `owner` (an LM session's id, never reused) skips the upload when the engine holds that
+    // session's variables already; sessions that share the engine call this before each of their
+    // device cycles.
+    void set_variables(const mko_variables& v, uint64_t owner = 0) {
+        if (owner && owner == vars_owner_) return;
         DeviceGuard guard(device_);
         upload_vars(v);
+        vars_owner_ = owner;
     }
 
     // Normal equations a [count][P][P], g [count][P] (device) of `count` points xd [count][P]
@@ -1239,6 +1245,7 @@ private:
             dvars_.component = c;
         }
         vars_host_dev_ = v; // the adjoint entry point reads type/surface on the host
+        vars_owner_ = 0;
     }
 
     int device_;
@@ -1254,6 +1261,7 @@ private:
     int64_t* d_hard_ = nullptr;
     mko_variables dvars_{};
     mko_variables vars_host_dev_{};
+    uint64_t vars_owner_ = 0; // id of the LM session whose variables dvars_ holds
 };
 
 // ---- batched Levenberg-Marquardt (src/optim/levenberg_marquardt.cpp:37-572) --------------------
@@ -1476,6 +1484,37 @@ public:
     }
 
     int graph_cycles() const { return graph_cycles_; }
+
+    // Split form of cycle() for running several LMs at once: cycle_begin launches a captured
+    // cycle on this LM's own stream without waiting (1), or, when this LM's cycle is not a
+    // graph, runs cycle() to the end (2; 0: already done); cycle_end waits for a launched cycle
+    // and returns what cycle() would (true while runs are active). Between the two calls the
+    // caller may begin other LMs' cycles, which then run concurrently on the device.
+    int cycle_begin() {
+        if (done_) return 0;
+        if (device_ && graph_eligible() && graph_ready()) {
+            graph_launch();
+            pending_ = 1;
+            return 1;
+        }
+        sync_result_ = cycle();
+        pending_ = 2;
+        return 2;
+    }
+    bool cycle_end() {
+        const int p = pending_;
+        pending_ = 0;
+        if (p == 2) return sync_result_;
+        if (p != 1) return !done_;
+        if (!graph_finish()) {
+            finish_round();
+            if (!next_round()) {
+                done_ = true;
+                return false;
+            }
+        }
+        return true;
+    }
 
     void results(double* x, mko_lm_run* out, int* cycles) {
         if (device_ && !done_) dev_download_state();
@@ -1738,7 +1777,7 @@ private:
     void dev_setup() {
         validate_values(base_, 0, base_.variants);
         check_glass(base_, vars_);
-        eng_.set_variables(vars_);
+        eng_.set_variables(vars_, id_);
         dev_tmpl_.upload(base_, 0, base_.variants, nullptr);
         double* bnd = bounds_d_.as<double>(3 * P_);
         hip_check(hipMemcpy(bnd, vars_.lower, P_ * 8, hipMemcpyHostToDevice), "bounds");
@@ -1834,6 +1873,7 @@ private:
 
     void dev_start_round(const std::vector<int>& ids, const std::vector<double>& xs) {
         graph_reset(); // the round's state arrays may move
+        eng_.set_variables(vars_, id_); // another session may have used the engine
         ids_ = ids;
         const int n = ids.empty() ? runs_ : int(ids.size());
         state_.assign(size_t(n), Run());
@@ -1856,6 +1896,7 @@ private:
     }
 
     bool dev_round_cycle() {
+        eng_.set_variables(vars_, id_); // another session may have used the engine
         const int n = ds_.n;
         int32_t* list_j = lists_;
         int32_t* list_c = lists_ + n;
@@ -2090,29 +2131,39 @@ private:
         return true;
     }
 
-    bool graph_round_cycle() {
-        if (!gexec_ && !graph_capture()) {
-            graph_reset();
-            graph_failed_ = true; // eager cycles for the rest of this LM
-            return dev_round_cycle();
-        }
-        const bool timed = eng_.timer.enabled;
-        if (timed) {
+    // The captured cycle is ready (captured now if needed); false: eager cycles from here on.
+    bool graph_ready() {
+        if (gexec_) return true;
+        if (graph_capture()) return true;
+        graph_reset();
+        graph_failed_ = true; // eager cycles for the rest of this LM
+        return false;
+    }
+
+    // Launches the cycle (and the read of its three counts) on gs_ without waiting.
+    void graph_launch() {
+        if (eng_.timer.enabled) {
             for (const GraphTimed& t : gparts_) {
                 hip_check(hipEventRecord(t.a, gs_), "hipEventRecord");
                 hip_check(hipGraphLaunch(t.exec, gs_), "graph launch");
                 hip_check(hipEventRecord(t.b, gs_), "hipEventRecord");
             }
+            timed_launch_ = true;
         } else {
             hip_check(hipGraphLaunch(gexec_, gs_), "graph launch");
+            timed_launch_ = false;
         }
         hip_check(hipMemcpyAsync(counts_host_, counts_, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
                                  gs_),
                   "LM counts");
+    }
+
+    // Waits for the launched cycle; false when the round has no active run (as dev_round_cycle).
+    bool graph_finish() {
         hip_check(hipStreamSynchronize(gs_), "LM cycle");
         ++graph_cycles_;
         // per family the launches that had work, as the eager cycle times them
-        if (timed) {
+        if (timed_launch_) {
             for (const GraphTimed& t : gparts_) {
                 if (t.which >= 0 && counts_host_[t.which] == 0) continue;
                 float ms = 0.0f;
@@ -2123,6 +2174,12 @@ private:
         if (counts_host_[1] == 0) return false;
         ++cycles_;
         return true;
+    }
+
+    bool graph_round_cycle() {
+        if (!graph_ready()) return dev_round_cycle();
+        graph_launch();
+        return graph_finish();
     }
 
     // Device state -> state_ (the fields finish_round and results read).
@@ -2163,6 +2220,13 @@ private:
     }
 
     Engine& eng_;
+    // Session id for the engine's variable upload: unique per process, unlike the address of a
+    // session, which a later session may reuse after this one is closed.
+    const uint64_t id_ = next_id();
+    static uint64_t next_id() {
+        static std::atomic<uint64_t> n{0};
+        return ++n;
+    }
     mko_batch base_;
     mko_residual_settings settings_;
     std::vector<mko_constraint> items_own_;
@@ -2198,7 +2262,10 @@ private:
     // captured cycle
     bool graph_wanted_ = false;
     bool graph_failed_ = false;
+    bool timed_launch_ = false;
     int graph_cycles_ = 0;
+    int pending_ = 0;          // cycle_begin: 1 launched, 2 ran synchronously
+    bool sync_result_ = false;
     hipStream_t gs_ = nullptr;
     hipGraphExec_t gexec_ = nullptr;
     int32_t* counts_host_ = nullptr;
@@ -2319,6 +2386,10 @@ int mko_lm_results(mko_lm* lm, double* x, mko_lm_run* out, int* cycles) {
 void mko_lm_destroy(mko_lm* lm) { delete lm; }
 
 int mko_lm_graph_cycles(const mko_lm* lm) { return lm ? lm->lm.graph_cycles() : 0; }
+
+int mko_lm_cycle_begin(mko_lm* lm) { MKO_HOST_TRY(return lm->lm.cycle_begin();) }
+
+int mko_lm_cycle_end(mko_lm* lm) { MKO_HOST_TRY(return lm->lm.cycle_end() ? 1 : 0;) }
 
 int mko_engine_normal_equations(mko_engine* e, const mko_batch* base,
                                 const mko_residual_settings* s, const mko_variables* v, int count,

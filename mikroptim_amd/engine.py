@@ -29,7 +29,8 @@ EXPORTS = [
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
     "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
-    "mko_lm_results", "mko_lm_destroy", "mko_lm_graph_cycles",
+    "mko_lm_results", "mko_lm_destroy", "mko_lm_graph_cycles", "mko_lm_cycle_begin",
+    "mko_lm_cycle_end",
 ]
 
 
@@ -94,6 +95,8 @@ def library():
         "mko_lm_results": ([C.c_void_p, _dp, C.POINTER(abi.LmRun), C.POINTER(C.c_int)], C.c_int),
         "mko_lm_destroy": ([C.c_void_p], None),
         "mko_lm_graph_cycles": ([C.c_void_p], C.c_int),
+        "mko_lm_cycle_begin": ([C.c_void_p], C.c_int),
+        "mko_lm_cycle_end": ([C.c_void_p], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -182,6 +185,19 @@ class LmSession:
 
     def cycle(self):
         rc = library().mko_lm_cycle(self._h)
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def cycle_begin(self):
+        """Launches the next cycle without waiting (1), or runs it eagerly (2); 0 when done."""
+        rc = library().mko_lm_cycle_begin(self._h)
+        if rc < 0:
+            _check(rc)
+        return rc
+
+    def cycle_end(self):
+        rc = library().mko_lm_cycle_end(self._h)
         if rc < 0:
             _check(rc)
         return rc == 1
@@ -343,9 +359,23 @@ class TraceEngine:
     def optimize_groups(self, groups, options=None, precision="fp64"):
         """Uneven batches: runs of different topologies (surface, field, wavelength or variable
         counts), one batched LM per topology group; groups = [(base, settings, variables, x0)].
-        Returns the LmResult of every group, in order. (The reference's batches hold a single
-        topology, include/mikroptim/batch/system_batch.hpp:26.)"""
-        return [self.optimize(b, s, v, x0, options, precision) for b, s, v, x0 in groups]
+        Returns the LmResult of every group, in order. The groups advance cycle by cycle
+        together; groups whose cycle is a captured hipGraph (device control, fused mixed adjoint)
+        run concurrently on their own streams. Each group's result is that of optimize() on it,
+        bit for bit. (The reference's batches hold a single topology,
+        include/mikroptim/batch/system_batch.hpp:26.)"""
+        sessions = [self.lm(b, s, v, x0, options, precision) for b, s, v, x0 in groups]
+        try:
+            # every group's cycle is begun before any is waited for: the groups whose cycles are
+            # captured graphs run concurrently, each on its own stream
+            active = list(sessions)
+            while active:
+                begun = [(sess, sess.cycle_begin()) for sess in active]
+                active = [sess for sess, rc in begun if rc != 0 and sess.cycle_end()]
+            return [sess.results() for sess in sessions]
+        finally:
+            for sess in sessions:
+                sess.close()
 
     def lm(self, base, settings, variables, x0, options=None, precision="fp64"):
         return LmSession(self, base, settings, variables, x0, options or abi.lm_options(),

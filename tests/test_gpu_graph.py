@@ -109,3 +109,30 @@ def test_lms_outside_the_graph_run_eagerly(engine, case):
         pass
     assert sess.graph_cycles() == 0
     sess.close()
+
+
+def test_uneven_groups_concurrent_equal_sequential(engine):
+    """optimize_groups begins every group's cycle before waiting for any: the captured cycles of
+    the groups run concurrently on their own streams; each group's result equals optimize() on
+    it bit for bit, graph groups and eager groups (soft failure mode) alike."""
+    groups = []
+    for i in range(5):
+        b, v = systems.uneven_topology(i, variants=3)
+        # the last group in soft failure mode: outside the graph, cycled eagerly
+        failure = abi.FAILURE_SOFT if i == 4 else abi.FAILURE_HARD
+        groups.append((b.take([0]), systems.uneven_settings(failure), v, v.extract(b)))
+    opt = abi.lm_options(max_iterations=6, threads=4, jacobian_method=abi.JACOBIAN_ADJOINT)
+    together = engine.optimize_groups(groups, opt, precision="mixed")
+    for (base, s, v, x0), r in zip(groups, together):
+        alone = engine.optimize(base, s, v, x0, opt, precision="mixed")
+        assert np.array_equal(r.x, alone.x) and np.array_equal(r.merit, alone.merit)
+        assert r.status == alone.status and r.cycles == alone.cycles
+    # the eligible groups' cycles were graph replays
+    sessions = [engine.lm(b, s, v, x0, opt, precision="mixed") for b, s, v, x0 in groups]
+    active = list(sessions)
+    while active:
+        begun = [(sess, sess.cycle_begin()) for sess in active]
+        active = [sess for sess, rc in begun if rc != 0 and sess.cycle_end()]
+    for (base, s, v, x0), sess in zip(groups, sessions):
+        assert (sess.graph_cycles() > 0) == (s.failure == abi.FAILURE_HARD)
+        sess.close()
