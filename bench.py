@@ -78,6 +78,45 @@ def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps,
     return done, dt, kt, res, jacobians
 
 
+def uneven_groups(eng, topologies=64, per_topology=128, max_iterations=20, threads=16):
+    """The uneven workload: 8192 variants of 64 random topologies (8-32 refracting surfaces,
+    3-5 fields, 3-7 wavelengths, 10-64 variables, 6x8 Half pupil; 128 jittered variants each),
+    20 LM iterations (adjoint + mixed, device control), one batched LM per topology. Timed with
+    the topologies' cycles run concurrently (optimize_groups) and one topology after another."""
+    from mikroptim_amd import abi, systems
+
+    groups = []
+    for i in range(topologies):
+        b, v = systems.uneven_topology(i, variants=per_topology)
+        groups.append((b.take([0]), systems.uneven_settings(abi.FAILURE_HARD), v, v.extract(b)))
+    opt = abi.lm_options(max_iterations=max_iterations, threads=threads,
+                         jacobian_method=abi.JACOBIAN_ADJOINT)
+    eng.optimize_groups(groups[:4], opt, precision="mixed")
+    sync()
+    t0 = time.perf_counter()
+    together = eng.optimize_groups(groups, opt, precision="mixed")
+    sync()
+    t_together = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    alone = [eng.optimize(*g, opt, precision="mixed") for g in groups]
+    sync()
+    t_alone = time.perf_counter() - t0
+    same = all(np.array_equal(a.x, b.x) for a, b in zip(together, alone))
+    iters = int(sum(int(np.sum(r.iterations)) for r in together))
+    shapes = [(g[0].surfaces, g[2].size, g[0].fields, g[0].wavelengths)
+              for g in groups]
+    return {"variants": topologies * per_topology, "topologies": topologies,
+            "max_iterations": max_iterations,
+            "surfaces": [min(x[0] for x in shapes), max(x[0] for x in shapes)],
+            "variables": [min(x[1] for x in shapes), max(x[1] for x in shapes)],
+            "fields": [min(x[2] for x in shapes), max(x[2] for x in shapes)],
+            "wavelengths": [min(x[3] for x in shapes), max(x[3] for x in shapes)],
+            "run_iterations": iters, "seconds_concurrent": t_together,
+            "run_iter_per_s": iters / t_together, "seconds_one_after_another": t_alone,
+            "concurrency_speedup": t_alone / t_together, "results_equal": same,
+            "lm_cycles_max": max(r.cycles for r in together)}
+
+
 def profiled_traffic(runs_per_launch):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters on a 1 GiB copy, as
@@ -241,6 +280,8 @@ def main():
         "g_rel": float(np.max(np.linalg.norm(gm - gd, axis=1) / np.linalg.norm(gd, axis=1))),
         "runs": 8}
 
+    uneven = uneven_groups(eng, threads=args.threads)
+
     out = {
         "metric": "LM iter/s @4096 variants: adjoint+mixed vs FP64 FD; err <=1e-4 waves RMS",
         "value": it_adj, "unit": "LM iter/s", "n_gpus": 1, "steps": n_adj,
@@ -258,6 +299,7 @@ def main():
         "mixed_normal_equations_error_vs_fp64": mixed_err,
         "merit_trace_rays_per_s": trace_rate,
         "constraint_rows": rows,
+        "uneven_topologies": uneven,
         "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
         "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
         "kernel_ms_fd_path": {k: v[0] for k, v in kt_fd.items()},
