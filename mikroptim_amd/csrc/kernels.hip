@@ -210,6 +210,7 @@ struct Record {
     uint8_t branch[MKO_MAX_SURFACES];
     uint8_t image_branch;
     int count;
+    static constexpr bool kLastOnly = false; // every surface's state
     __device__ __forceinline__ void put(int s, const Ray& r, uint8_t b) {
         p[s][0] = RR(r.p.x);
         p[s][1] = RR(r.p.y);
@@ -223,17 +224,21 @@ struct Record {
 };
 
 struct NoRecord {
+    static constexpr bool kLastOnly = false;
     template <class RR>
     __device__ __forceinline__ void put(int, const RR&, uint8_t) {}
     uint8_t image_branch;
 };
 
 // Last state of a forward trace (the state on surface S-1, which the reference-sphere step starts
-// from); the reverse sweep reconstructs the earlier states with mkod::inverse_step.
+// from); the reverse sweep reconstructs the earlier states with mkod::inverse_step. The traces put
+// it once, after the last surface (kLastOnly): a ray that fails on the way leaves it unset, and
+// the loop carries no copy of the state.
 struct LastState {
     V3<double> p, d;
     uint8_t image_branch;
     int count;
+    static constexpr bool kLastOnly = true;
     __device__ __forceinline__ void put(int s, const Ray& r, uint8_t) {
         p = r.p;
         d = r.d;
@@ -249,15 +254,16 @@ __device__ __forceinline__ uint8_t trace_ray(const TraceParams& P, const SurfLds
                                              mkod::RayT<R>& r, V3<double> M, double kappa,
                                              Soft& soft, Rec& rec) {
     using mkod::num;
-    rec.put(0, r, mkod::kBranchRegular);
+    if constexpr (!Rec::kLastOnly) rec.put(0, r, mkod::kBranchRegular);
     for (int s = 1; s < P.S; ++s) {
         uint8_t branch = 0;
         const uint8_t st = mkod::surface_step<kSoft, kFast>(r, num<R>(L.c[s]), num<R>(L.t[s - 1]),
                                                      num<R>(L.n[s - 1]), num<R>(L.mu[s]), soft,
                                                      branch);
         if (st != MKO_STATUS_OK) return st;
-        rec.put(s, r, branch);
+        if constexpr (!Rec::kLastOnly) rec.put(s, r, branch);
     }
+    if constexpr (Rec::kLastOnly) rec.put(P.S - 1, r, mkod::kBranchRegular);
     if (!(r.d.z > num<R>(0.0))) return MKO_STATUS_BACKWARD;
     uint8_t rb = 0;
     const uint8_t rst = mkod::reference_step<kSoft, kFast>(
@@ -322,7 +328,8 @@ __device__ __forceinline__ uint8_t trace_ray_ext(const TraceParams& P, const Sur
                              to_double(r.d.x), to_double(r.d.y), to_double(r.d.z), h, c);
         if (P.ext_mask & 1ull) reduce(0, h, c);
     }
-    if (alive) rec.put(0, r, mkod::kBranchRegular);
+    if constexpr (!Rec::kLastOnly)
+        if (alive) rec.put(0, r, mkod::kBranchRegular);
     for (int s = 1; s < P.S; ++s) {
         double h = 0.0, c = 1.0;
         if (alive) {
@@ -335,7 +342,7 @@ __device__ __forceinline__ uint8_t trace_ray_ext(const TraceParams& P, const Sur
                 alive = false;
                 st = sst;
             } else {
-                rec.put(s, r, branch);
+                if constexpr (!Rec::kLastOnly) rec.put(s, r, branch);
                 surface_extremum(to_double(r.p.x), to_double(r.p.y), to_double(r.p.z), L.c[s], dx,
                                  dy, dz, h, c);
             }
@@ -343,6 +350,7 @@ __device__ __forceinline__ uint8_t trace_ray_ext(const TraceParams& P, const Sur
         if ((P.ext_mask >> s) & 1ull) reduce(s, h, c); // uniform: P.ext_mask is a kernel argument
     }
     if (!alive) return st;
+    if constexpr (Rec::kLastOnly) rec.put(P.S - 1, r, mkod::kBranchRegular);
     if (!(r.d.z > num<R>(0.0))) return MKO_STATUS_BACKWARD;
     uint8_t rb = 0;
     const uint8_t rst = mkod::reference_step<kSoft, kFast>(
@@ -1559,6 +1567,8 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
         if (tile_needed) {
             const int m = lane & 31, kb = lane >> 5;
             const int pa = I * 32 + m, pb = Jt * 32 + m;
+            // lanes of rows >= P (the padding of the 64 x 64 C) load nothing and take zeros: an
+            // unconditional load of a shared zero row measured 2 % slower
             const bool va = pa < Pn, vb = pb < Pn;
             const uint16_t* Ra = reinterpret_cast<const uint16_t*>(G + (va ? pa : 0) * kGStride);
             const uint16_t* Rb = reinterpret_cast<const uint16_t*>(G + (vb ? pb : 0) * kGStride);
