@@ -1332,11 +1332,11 @@ void cholesky_solve(const std::vector<double>& l, size_t n, std::vector<double>&
         for (size_t k = 0; k < i; ++k) s -= l[k * n + i] * b[k];
         b[i] = s / l[i * n + i];
     }
-    // back substitution, each sum in descending k (the order of the device control's
-    // lane-parallel column sweep)
+    // back substitution, each sum in ascending k (cholesky_solve,
+    // src/optim/levenberg_marquardt.cpp:94-112)
     for (size_t i = n; i-- > 0;) {
         double s = b[i];
-        for (size_t k = n; k-- > i + 1;) s -= l[i * n + k] * b[k];
+        for (size_t k = i + 1; k < n; ++k) s -= l[i * n + k] * b[k];
         b[i] = s / l[i * n + i];
     }
 }

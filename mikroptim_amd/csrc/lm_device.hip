@@ -7,10 +7,12 @@
 // products and sums without contraction (fp contract off below), correctly rounded division and
 // square root (__ddiv_rn, __dsqrt_rn), std::max / std::min as (a < b) ? b : a / (b < a) ? b : a.
 // Given the same normal equations and merits, device and host control take bit-identical
-// trajectories. Sums keep the host order: the Cholesky factor (right-looking) and the two
-// substitutions run lane-parallel over rows, each entry receiving its subtractions in the host's
-// order; the step model (lin, quad) runs on lane 0; maxima (order-independent) are wave
-// reductions.
+// trajectories. Sums keep the host order (that of the reference's cholesky / cholesky_solve,
+// src/optim/levenberg_marquardt.cpp:70-112): the Cholesky factor (right-looking) and the forward
+// substitution run lane-parallel over rows, each entry receiving its subtractions in the host's
+// order; the back substitution forms each row's products on the lanes and subtracts them in
+// ascending order; the step model (lin, quad) runs on lane 0; maxima (order-independent) are
+// wave reductions.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -27,6 +29,13 @@ constexpr int kLanes = 64;
 constexpr int kMaxP = kMaxVariables;
 constexpr double kModelDamping = 1e-12;
 constexpr double kFdStepFactor = 6.0554544523933395e-06; // 2^(-52/3)
+
+// Lane k's value of v, in every lane (k uniform).
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
 
 __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }
 __device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }
@@ -123,15 +132,18 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 if (lane > k && lane < f) acc -= m[tri(k, lane, f)] * L.rhs[k];
             }
         }
-        // back substitution: lane i subtracts l(k, i) x_k for k = f-1, f-2, ... as x_k is final
-        // (the host sums in the same descending order)
+        // back substitution, x_i = (b_i - sum_{k > i} l(k, i) x_k) / l(i, i) for i = f-1 ... 0:
+        // lane k > i forms the product l(k, i) x_k, and every lane subtracts the products from b_i
+        // in ascending k, as the host does
         {
             __syncthreads();
-            double acc = lane < f ? L.rhs[lane] : 0.0;
-            for (int k = f - 1; k >= 0; --k) {
-                if (lane == k) L.rhs[k] = __ddiv_rn(acc, m[tri(k, k, f)]);
+            for (int i = f - 1; i >= 0; --i) {
+                const double p = (lane > i && lane < f) ? m[tri(i, lane, f)] * L.rhs[lane] : 0.0;
+                double acc = L.rhs[i];
+                for (int k = i + 1; k < f; ++k) acc -= lane_value(p, k);
+                __syncthreads(); // every lane has read b_i
+                if (lane == i) L.rhs[i] = __ddiv_rn(acc, m[tri(i, i, f)]);
                 __syncthreads();
-                if (lane < k) acc -= m[tri(lane, k, f)] * L.rhs[k];
             }
         }
         __syncthreads();

@@ -224,6 +224,30 @@ class Lib:
             res, C.byref(cycles)))
         return x, list(res), cycles.value
 
+    def optimize_lm_hip_backend(self, engine, base, settings, variables, x0, options,
+                                precision):
+        """The reference's optimize_lm_rounds driving HipLmBackend, its LmBackend plug-in over the
+        HIP library's mko_engine_normal_equations / mko_engine_point_merit (reference build
+        only); `engine` is a mikroptim_amd TraceEngine."""
+        from mikroptim_amd import engine as E
+
+        lib = E.library()
+        calls = (C.c_void_p * 3)(*[C.cast(getattr(lib, n), C.c_void_p).value for n in (
+            "mko_engine_normal_equations", "mko_engine_point_merit", "mko_last_error")])
+        x0 = np.ascontiguousarray(np.atleast_2d(x0), dtype=np.float64)
+        runs, P = x0.shape
+        x = np.zeros((runs, P))
+        res = (abi.LmRun * runs)()
+        cycles = C.c_int(0)
+        f = self.fn("optimize_lm_hip_backend")
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                      C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
+        self.check(f(calls, engine._h, C.byref(b), C.byref(s), C.byref(v), runs, abi.ptr(x0),
+                     C.byref(options), E._PRECISION[precision], abi.ptr(x), res,
+                     C.byref(cycles)))
+        return x, list(res), cycles.value
+
     def gauss_legendre(self, n):
         u = np.zeros(n)
         w = np.zeros(n)
