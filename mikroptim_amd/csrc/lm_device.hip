@@ -61,6 +61,7 @@ struct SolveLds {
     double step[kMaxP];
     double aq[kMaxP];
     double ratio[kMaxP];
+    double col[kMaxP]; // column j of the Cholesky factor while rows below it are updated
     int idx[kMaxP];
     uint8_t fixed[kMaxP];
     double diag;
@@ -72,7 +73,7 @@ struct SolveLds {
 // the factorisation succeeds, like the host).
 // a, g: the run's normal equations (row-major a[q * P + p]).
 __device__ void solve_step(const State& S, int run, const double* a, const double* g, double mu,
-                           SolveLds& L, double* m) {
+                           SolveLds& L, double* __restrict__ m) {
     const int n = S.P;
     const int lane = threadIdx.x;
     const double* x = S.x + size_t(run) * n;
@@ -114,12 +115,19 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 return;
             }
             const double l = __dsqrt_rn(diag);
-            if (i == j) m[tri(j, j, f)] = l;
-            else if (i > j && i < f) m[tri(j, i, f)] = __ddiv_rn(m[tri(j, i, f)], l);
+            if (i == j) {
+                m[tri(j, j, f)] = l;
+            } else if (i > j && i < f) {
+                const double v = __ddiv_rn(m[tri(j, i, f)], l);
+                m[tri(j, i, f)] = v;
+                L.col[i] = v;
+            }
             __syncthreads();
+            // column j is read from L.col, apart from the entries being updated, so a lane's
+            // updates of its row do not wait on each other
             if (i > j && i < f) {
-                const double lij = m[tri(j, i, f)];
-                for (int k = j + 1; k <= i; ++k) m[tri(k, i, f)] -= lij * m[tri(j, k, f)];
+                const double lij = L.col[i];
+                for (int k = j + 1; k <= i; ++k) m[tri(k, i, f)] -= lij * L.col[k];
             }
         }
         __syncthreads();
