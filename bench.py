@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--variants", type=int, default=4096)
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="leave out the uneven-topology measurement (for short profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with three 1 GiB device copies (scale for rocprofv3 byte counters)")
     ap.add_argument("--no-graph", action="store_true",
@@ -280,7 +282,7 @@ def main():
         "g_rel": float(np.max(np.linalg.norm(gm - gd, axis=1) / np.linalg.norm(gd, axis=1))),
         "runs": 8}
 
-    uneven = uneven_groups(eng, threads=args.threads)
+    uneven = None if args.no_uneven else uneven_groups(eng, threads=args.threads)
 
     out = {
         "metric": "LM iter/s @4096 variants: adjoint+mixed vs FP64 FD; err <=1e-4 waves RMS",

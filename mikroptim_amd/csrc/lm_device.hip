@@ -141,18 +141,20 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             }
         }
         // back substitution, x_i = (b_i - sum_{k > i} l(k, i) x_k) / l(i, i) for i = f-1 ... 0:
-        // lane k > i forms the product l(k, i) x_k, and every lane subtracts the products from b_i
-        // in ascending k, as the host does
+        // lane k keeps b_k, then x_k, in a register and forms the product l(k, i) x_k; every lane
+        // subtracts the products from b_i in ascending k, as the host does, and divides, so x_i
+        // is the same in all lanes and lane i keeps it (no LDS round trip between rows)
         {
             __syncthreads();
+            double xk = lane < f ? L.rhs[lane] : 0.0;
             for (int i = f - 1; i >= 0; --i) {
-                const double p = (lane > i && lane < f) ? m[tri(i, lane, f)] * L.rhs[lane] : 0.0;
-                double acc = L.rhs[i];
+                const double p = (lane > i && lane < f) ? m[tri(i, lane, f)] * xk : 0.0;
+                double acc = lane_value(xk, i);
                 for (int k = i + 1; k < f; ++k) acc -= lane_value(p, k);
-                __syncthreads(); // every lane has read b_i
-                if (lane == i) L.rhs[i] = __ddiv_rn(acc, m[tri(i, i, f)]);
-                __syncthreads();
+                const double xi = __ddiv_rn(acc, m[tri(i, i, f)]);
+                if (lane == i) xk = xi;
             }
+            if (lane < f) L.rhs[lane] = xk;
         }
         __syncthreads();
         if (lane < f) L.delta[L.idx[lane]] = L.rhs[lane];
