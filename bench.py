@@ -284,6 +284,17 @@ def main():
 
     uneven = None if args.no_uneven else uneven_groups(eng, threads=args.threads)
 
+    # soft failure mode (the reference CLI's default) on the same workload: no ray fails, so
+    # every run's normal equations come from the fused kernel (its own forward trace; the LM's
+    # MERIT traces keep no ray states in soft mode)
+    st = copy.copy(settings)
+    st.failure = abi.FAILURE_SOFT
+    n_s, t_s, kt_s, res_s, jac_s = run_lm(eng, base, st, variables, x0, abi.JACOBIAN_ADJOINT,
+                                          "mixed", 1, max(1, min(args.steps, 5)), args.threads)
+    soft = {"lm_iter_per_s": n_s / t_s, "steps": n_s, "jacobians": jac_s,
+            "failed_rays_final": int(sum(r.failed_rays for r in res_s.runs)),
+            "kernel_ms": {k: v[0] for k, v in kt_s.items()}}
+
     out = {
         "metric": "LM iter/s @4096 variants: adjoint+mixed vs FP64 FD; err <=1e-4 waves RMS",
         "value": it_adj, "unit": "LM iter/s", "n_gpus": 1, "steps": n_adj,
@@ -302,6 +313,7 @@ def main():
         "merit_trace_rays_per_s": trace_rate,
         "constraint_rows": rows,
         "uneven_topologies": uneven,
+        "soft_failure_mode": soft,
         "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
         "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
         "kernel_ms_fd_path": {k: v[0] for k, v in kt_fd.items()},

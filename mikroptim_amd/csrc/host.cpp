@@ -799,6 +799,10 @@ public:
         if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED)
             throw std::invalid_argument("unknown precision");
         validate_values(base, 0, base.variants);
+        if (soft_split_applies(s, vars, method, precision) && !J && !r0 && (a || g)) {
+            soft_split(base, s, vars, runs, x0, a, g);
+            return;
+        }
         RowPlan rp = plan_rows(base, s);
         if (rp.paraxial) validate_paraxial(base, 0, base.variants);
         upload_rows(rp);
@@ -1109,6 +1113,49 @@ public:
     }
 
     // The mixed-precision normal equations come from the fused kernel under these conditions.
+    // Soft failure mode with the mixed-precision adjoint: a run whose point has no failed ray has
+    // no penalty residual and no penalty derivative, so its normal equations are those of hard
+    // mode, which the fused kernel forms; runs with failed rays take the FP64 path. The choice
+    // depends only on the run's own point (its failed-ray count).
+    static bool soft_split_applies(const mko_residual_settings& s, const mko_variables& v,
+                                   int method, int precision) {
+        if (s.failure != MKO_FAILURE_SOFT || method != MKO_JACOBIAN_ADJOINT ||
+            precision != MKO_PRECISION_MIXED)
+            return false;
+        mko_residual_settings hs = s;
+        hs.failure = MKO_FAILURE_HARD;
+        return fused_ok(hs, v);
+    }
+    void soft_split(const mko_batch& base, const mko_residual_settings& s, const mko_variables& vars,
+                    int runs, const double* x0, double* a, double* g) {
+        const size_t P = size_t(vars.count);
+        const size_t nr = size_t(runs);
+        std::vector<double> merit(nr);
+        std::vector<int64_t> failed(nr), hard(nr);
+        point_merit(base, s, vars, runs, x0, merit.data(), failed.data(), hard.data());
+        std::vector<int> clean, other;
+        for (int i = 0; i < runs; ++i) (failed[size_t(i)] == 0 ? clean : other).push_back(i);
+        mko_residual_settings hs = s;
+        hs.failure = MKO_FAILURE_HARD;
+        auto part = [&](const std::vector<int>& ids, const mko_residual_settings& ps, int prec) {
+            if (ids.empty()) return;
+            const size_t n = ids.size();
+            std::vector<double> xs(n * P), as(a ? n * P * P : 0), gs(g ? n * P : 0);
+            for (size_t j = 0; j < n; ++j) std::copy_n(x0 + size_t(ids[j]) * P, P, xs.data() + j * P);
+            GatheredBatch gb;
+            const mko_batch& b = base.variants == 1 ? base : gb.gather(base, ids.data(), n);
+            jacobian(b, ps, vars, int(n), xs.data(), MKO_JACOBIAN_ADJOINT, prec, nullptr, nullptr,
+                     a ? as.data() : nullptr, g ? gs.data() : nullptr);
+            for (size_t j = 0; j < n; ++j) {
+                const size_t i = size_t(ids[j]);
+                if (a) std::copy_n(as.data() + j * P * P, P * P, a + i * P * P);
+                if (g) std::copy_n(gs.data() + j * P, P, g + i * P);
+            }
+        };
+        part(clean, hs, MKO_PRECISION_MIXED);
+        part(other, s, MKO_PRECISION_FP64);
+    }
+
     static bool fused_ok(const mko_residual_settings& s, const mko_variables& v) {
         bool glass = false;
         for (int k = 0; k < v.count; ++k) glass = glass || v.type[k] == MKO_VAR_GLASS;
@@ -1793,6 +1840,7 @@ private:
         dopt_.initial_damping = o_.initial_damping;
         dopt_.max_damping = o_.max_damping;
         dopt_.diag_floor = o_.diag_floor;
+        soft_split_ = Engine::soft_split_applies(settings_, vars_, o_.jacobian_method, precision_);
     }
 
     // The LM's MERIT traces keep the final ray states for the fused mixed-precision adjoint at
@@ -1911,6 +1959,24 @@ private:
         });
         const int cj = dev_count(0);
         if (cj > 0) {
+            // soft mode, mixed adjoint: the runs whose point has no failed ray first (hard
+            // mode's normal equations, fused kernel), then the others (FP64), as
+            // Engine::soft_split orders the work
+            int clean = 0;
+            if (soft_split_) {
+                const size_t ncj = size_t(cj), nn = size_t(n);
+                std::vector<int32_t> lj(ncj);
+                std::vector<int64_t> fl(nn);
+                hip_check(hipMemcpy(lj.data(), list_j, size_t(cj) * 4, hipMemcpyDeviceToHost),
+                          "LM list");
+                hip_check(hipMemcpy(fl.data(), ds_.failed, size_t(n) * 8, hipMemcpyDeviceToHost),
+                          "LM failed rays");
+                const auto mid = std::stable_partition(
+                    lj.begin(), lj.end(), [&](int32_t r) { return fl[size_t(r)] == 0; });
+                clean = int(mid - lj.begin());
+                hip_check(hipMemcpy(list_j, lj.data(), size_t(cj) * 4, hipMemcpyHostToDevice),
+                          "LM list");
+            }
             eng_.timer.run(kLmControl, [&] {
                 mko_lmdev::launch_gather(ds_, list_j, cj, false, rix, xs, tidx, nullptr);
             });
@@ -1918,9 +1984,24 @@ private:
             double* gc = gc_.as<double>(size_t(cj) * P_);
             mko_internal::RayStates st = states_;
             st.slot = list_j;
-            eng_.jacobian_dev(dev_tmpl_.view, tidx, cj, xs, settings_, int(P_), ds_.lower,
-                              ds_.upper, ds_.typical, o_.jacobian_method, precision_, ac, gc,
-                              use_states_ ? &st : nullptr);
+            if (soft_split_) {
+                mko_residual_settings hs = settings_;
+                hs.failure = MKO_FAILURE_HARD;
+                if (clean > 0)
+                    eng_.jacobian_dev(dev_tmpl_.view, tidx, clean, xs, hs, int(P_), ds_.lower,
+                                      ds_.upper, ds_.typical, o_.jacobian_method,
+                                      MKO_PRECISION_MIXED, ac, gc, nullptr);
+                if (cj > clean)
+                    eng_.jacobian_dev(dev_tmpl_.view, tidx ? tidx + clean : nullptr, cj - clean,
+                                      xs + size_t(clean) * P_, settings_, int(P_), ds_.lower,
+                                      ds_.upper, ds_.typical, o_.jacobian_method,
+                                      MKO_PRECISION_FP64, ac + size_t(clean) * P_ * P_,
+                                      gc + size_t(clean) * P_, nullptr);
+            } else {
+                eng_.jacobian_dev(dev_tmpl_.view, tidx, cj, xs, settings_, int(P_), ds_.lower,
+                                  ds_.upper, ds_.typical, o_.jacobian_method, precision_, ac, gc,
+                                  use_states_ ? &st : nullptr);
+            }
             eng_.timer.run(kLmControl, [&] {
                 mko_lmdev::launch_prepare(ds_, dopt_, list_j, cj, ac, gc, nullptr);
             });
@@ -2250,6 +2331,7 @@ private:
     DeviceBatch dev_tmpl_;
     Buffer state_mem_, lists_mem_, round_mem_, bounds_d_, xs_d_, tidx_d_, ac_, gc_, fm_, ff_, fh_;
     bool use_states_ = false;
+    bool soft_split_ = false; // soft mode, mixed adjoint: failure-free runs on the fused kernel
     mko_internal::RayStates states_{};
     Buffer states_mem_;
     int32_t* lists_ = nullptr;

@@ -167,15 +167,18 @@ def test_adjoint_mixed_normal_equations_singlet(engine, singlet):
     assert ea <= MIXED_A_TOL and ed <= MIXED_DIAG_TOL and eg <= MIXED_G_TOL
 
 
-def test_adjoint_mixed_soft_mode_uses_fp64(engine, singlet):
-    """Soft failure mode is outside the fused kernel: MIXED then gives the FP64 result."""
+def test_adjoint_mixed_soft_mode_without_failures_is_hard_mode(engine, singlet):
+    """Soft failure mode at points without failed rays: the penalty rows and their derivatives
+    are zero, so MIXED gives hard mode's fused normal equations (runs with failed rays take the
+    FP64 path: tests/test_gpu_soft_mixed.py)."""
     b, v = singlet
     s = systems.singlet_settings(abi.FAILURE_SOFT)
     base, x0 = b.take([0]), v.extract(b)[:3]
     _, _, am, gm = engine.jacobian(base, s, v, x0, method="adjoint", precision="mixed",
                                    want_jacobian=False)
-    _, _, ad, gd = engine.jacobian(base, s, v, x0, method="adjoint", precision="fp64")
-    assert np.array_equal(am, ad) and np.array_equal(gm, gd)
+    _, _, ah, gh = engine.jacobian(base, systems.singlet_settings(abi.FAILURE_HARD), v, x0,
+                                   method="adjoint", precision="mixed", want_jacobian=False)
+    assert np.array_equal(am, ah) and np.array_equal(gm, gh)
 
 
 @pytest.mark.parametrize("failure", [abi.FAILURE_SOFT, abi.FAILURE_HARD])
