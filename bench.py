@@ -284,9 +284,9 @@ def main():
 
     uneven = None if args.no_uneven else uneven_groups(eng, threads=args.threads)
 
-    # soft failure mode (the reference CLI's default) on the same workload: no ray fails, so
-    # every run's normal equations come from the fused kernel (its own forward trace; the LM's
-    # MERIT traces keep no ray states in soft mode)
+    # soft failure mode (the reference CLI's default) on the same workload: the runs whose point
+    # has no failed ray take the fused kernel on the MERIT trace's stored ray states, the others
+    # the FP64 path
     st = copy.copy(settings)
     st.failure = abi.FAILURE_SOFT
     n_s, t_s, kt_s, res_s, jac_s = run_lm(eng, base, st, variables, x0, abi.JACOBIAN_ADJOINT,

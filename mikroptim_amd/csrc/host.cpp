@@ -1844,12 +1844,13 @@ private:
     }
 
     // The LM's MERIT traces keep the final ray states for the fused mixed-precision adjoint at
-    // the same points (hard failure, FP64 trace, fused normal equations), if they fit in half
-    // of the engine's memory budget.
+    // the same points (FP64 trace, fused normal equations: hard failure, or soft failure for the
+    // runs without failed rays), if they fit in half of the engine's memory budget.
     void dev_allocate_states(int n) {
         use_states_ = false;
         if (o_.jacobian_method != MKO_JACOBIAN_ADJOINT || precision_ != MKO_PRECISION_MIXED ||
-            !Engine::fused_ok(settings_, vars_) || settings_.precision != MKO_PRECISION_FP64)
+            !(Engine::fused_ok(settings_, vars_) || soft_split_) ||
+            settings_.precision != MKO_PRECISION_FP64)
             return;
         const int64_t rays = rays_of(base_, settings_);
         const size_t bytes = size_t(n) * size_t(rays) * 7 * sizeof(double);
@@ -1990,7 +1991,7 @@ private:
                 if (clean > 0)
                     eng_.jacobian_dev(dev_tmpl_.view, tidx, clean, xs, hs, int(P_), ds_.lower,
                                       ds_.upper, ds_.typical, o_.jacobian_method,
-                                      MKO_PRECISION_MIXED, ac, gc, nullptr);
+                                      MKO_PRECISION_MIXED, ac, gc, use_states_ ? &st : nullptr);
                 if (cj > clean)
                     eng_.jacobian_dev(dev_tmpl_.view, tidx ? tidx + clean : nullptr, cj - clean,
                                       xs + size_t(clean) * P_, settings_, int(P_), ds_.lower,

@@ -1874,17 +1874,24 @@ void launch_trace_r(int mode, int rpl, const TraceParams& P, double* res, double
 }
 
 // RESIDUALS (mode 0) or MERIT partials (mode 1); ext: extrema partials [V*T][S][2] or NULL;
-// states: the LM MERIT's final ray states (mode 1, hard failure, FP64 trace) or NULL.
+// states: the LM MERIT's final ray states (mode 1, FP64 trace; stored for the rays that end
+// with status Ok) or NULL.
 void launch_trace(int mode, const TraceParams& P, double* res, double* opd, uint8_t* st,
                   double* part, int* pf, int* ph, double* ext, hipStream_t stream,
                   const mko_internal::RayStates* states = nullptr) {
     if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large for one launch");
     const int rpl = rays_per_lane(P);
     if (states) {
-        if (mode != 1 || P.soft || P.precision != MKO_PRECISION_FP64)
-            throw std::invalid_argument("ray states need the MERIT trace, hard failure mode and FP64");
-        if (ext) launch_trace_rpl<1, false, double, true, true>(rpl, P, res, opd, st, part, pf, ph, *states, ext, stream);
-        else launch_trace_rpl<1, false, double, true, false>(rpl, P, res, opd, st, part, pf, ph, *states, nullptr, stream);
+        if (mode != 1 || P.precision != MKO_PRECISION_FP64)
+            throw std::invalid_argument("ray states need the MERIT trace and FP64");
+        if (P.soft) {
+            if (ext) launch_trace_rpl<1, true, double, true, true>(rpl, P, res, opd, st, part, pf, ph, *states, ext, stream);
+            else launch_trace_rpl<1, true, double, true, false>(rpl, P, res, opd, st, part, pf, ph, *states, nullptr, stream);
+        } else if (ext) {
+            launch_trace_rpl<1, false, double, true, true>(rpl, P, res, opd, st, part, pf, ph, *states, ext, stream);
+        } else {
+            launch_trace_rpl<1, false, double, true, false>(rpl, P, res, opd, st, part, pf, ph, *states, nullptr, stream);
+        }
     } else if (P.precision == MKO_PRECISION_DF64) {
         launch_trace_r<mkod::df64>(mode, rpl, P, res, opd, st, part, pf, ph, ext, stream);
     } else {
@@ -2089,8 +2096,8 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
         P.live = live;
-        if (P.soft || P.precision != MKO_PRECISION_FP64)
-            throw std::invalid_argument("ray states need hard failure mode and the FP64 trace");
+        if (P.precision != MKO_PRECISION_FP64)
+            throw std::invalid_argument("ray states need the FP64 trace");
         if (states.rays != P.rays) throw std::invalid_argument("ray states of another layout");
         if (P.V == 0) return 0;
         const hipStream_t s = static_cast<hipStream_t>(stream);
