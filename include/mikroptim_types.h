@@ -40,7 +40,7 @@ enum {
     MKO_STATUS_SOFT_MISSED = 5,
     MKO_STATUS_SOFT_TIR = 6
 };
-/* VariableType (include/mikroptim/model/variables.hpp:17), the kinds this build supports */
+/* VariableType (include/mikroptim/model/variables.hpp:17), the kinds the library supports */
 enum { MKO_VAR_CURVATURE = 0, MKO_VAR_THICKNESS = 1, MKO_VAR_WORKING_DISTANCE = 2, MKO_VAR_GLASS = 3 };
 /* FdStencil (include/mikroptim/optim/fd_formula.hpp:15) */
 enum { MKO_STENCIL_CENTRAL = 0, MKO_STENCIL_ONE_SIDED = 1, MKO_STENCIL_TRUNCATED = 2 };

@@ -1801,7 +1801,7 @@ TraceParams make_params(const mko_batch& b, const mko_residual_settings& s) {
     P.sphere_mask = 0;
     for (int i = 0; i < b.surfaces; ++i) {
         if (b.type[i] > MKO_SURFACE_SPHERE)
-            throw std::invalid_argument("unsupported surface type (ghost plates are not part of this build)");
+            throw std::invalid_argument("unsupported surface type (ghost plates are not supported)");
         if (b.type[i] == MKO_SURFACE_SPHERE) P.sphere_mask |= 1ull << i;
     }
     int64_t rays = 0;

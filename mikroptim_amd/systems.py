@@ -1,5 +1,5 @@
 """Synthetic systems and variable sets of the measured workloads (built in code from a seed; the
-reference ships none of these systems, and there is no system-file reader in this build).
+reference ships none of these systems; reference system files go through system_file.py).
 
 singlet(...)        stop + biconvex N-BK7 singlet (c1 = +0.02, c2 = -0.005 1/mm, t = 5 mm),
                     EPD 10 mm, fields 0 / 3.5 / 5 deg, 486.13 / 587.56 / 656.27 nm, image plane at

@@ -1443,7 +1443,7 @@ inline void backward_ray(const Table& tb, int l, const Record& rec, bool reached
         next_branch = rec.branch[s];
         if (s == 0) d0 = d;
     }
-    // the point source's direction derivative has no parameter of this build (no decenter)
+    // the point source's direction derivative belongs to no supported variable (no decenter)
     (void)point_source;
     (void)d0;
 }

@@ -213,7 +213,7 @@ def test_lm_adjoint_mixed_converges_like_fp64(engine, singlet, objective):
     flat valleys (the singlet's bending, the objective's ill-conditioned damped systems), so
     trajectories are not compared point by point. Required: every run's merit decreases, and on
     the singlet the mixed LM ends within 5 % of (or below) the FP64 LM's merit. Both evaluate
-    the merit in FP64, so accepted points are judged exactly."""
+    the merit in FP64, so accepted points are compared exactly."""
     b, v = singlet
     s = systems.singlet_settings(abi.FAILURE_HARD)
     x0 = v.extract(b)[:6]
