@@ -54,6 +54,8 @@ __device__ __forceinline__ double wave_max(double v) {
 
 // Packed lower triangle of an f x f matrix, column-major: element (row i, column j), i >= j.
 __device__ __forceinline__ int tri(int j, int i, int f) { return j * f - (j * (j - 1)) / 2 + (i - j); }
+// entries of the packed lower triangle of a P x P matrix
+__host__ __device__ __forceinline__ int tri_size(int P) { return P * (P + 1) / 2; }
 
 struct SolveLds {
     double rhs[kMaxP];
@@ -101,10 +103,27 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         }
         if (lane < f) L.rhs[lane] = -g[L.idx[lane]];
         __syncthreads();
-        // Cholesky, right-looking: after column j is final, every lane i > j subtracts
-        // l(i, j) l(k, j) from its entries (k, i), j < k <= i. Each entry receives the same
-        // subtractions in the same (ascending j) order as in the host's left-looking loop, so the
-        // factor is bitwise the host's; the updates of a lane are independent of each other.
+        // row and column of every packed entry (the trailing columns' entries are spread over the
+        // lanes below)
+        {
+            uint8_t* rowof = reinterpret_cast<uint8_t*>(m + tri_size(n));
+            uint8_t* colof = rowof + tri_size(n);
+            for (int c = 0; c < f; ++c)
+                if (lane >= c && lane < f) {
+                    const int e = tri(c, lane, f);
+                    rowof[e] = uint8_t(lane);
+                    colof[e] = uint8_t(c);
+                }
+        }
+        __syncthreads();
+        // Cholesky, right-looking: after column j is final, every entry (i, k), j < k <= i,
+        // receives l(i, j) l(k, j). Each entry receives the same subtractions in the same
+        // (ascending j) order as in the host's left-looking loop, so the factor is bitwise the
+        // host's. The trailing columns j+1 ... f-1 are one contiguous range of the packed layout,
+        // whose entries are spread evenly over the lanes.
+        const uint8_t* rowof = reinterpret_cast<const uint8_t*>(m + tri_size(n));
+        const uint8_t* colof = rowof + tri_size(n);
+        const int e_end = tri(f - 1, f - 1, f) + 1;
         for (int j = 0; j < f; ++j) {
             const int i = lane;
             if (i == j) L.diag = m[tri(j, j, f)];
@@ -123,12 +142,11 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 L.col[i] = v;
             }
             __syncthreads();
-            // column j is read from L.col, apart from the entries being updated, so a lane's
-            // updates of its row do not wait on each other
-            if (i > j && i < f) {
-                const double lij = L.col[i];
-                for (int k = j + 1; k <= i; ++k) m[tri(k, i, f)] -= lij * L.col[k];
+            if (j + 1 < f) {
+                for (int e = tri(j + 1, j + 1, f) + lane; e < e_end; e += kLanes)
+                    m[e] -= L.col[rowof[e]] * L.col[colof[e]];
             }
+            __syncthreads();
         }
         __syncthreads();
         // forward substitution: lane i subtracts l(i, k) b_k for k = 0, 1, ... as b_k is final
@@ -345,7 +363,8 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
                                                          const double* g_c) {
     __shared__ SolveLds L;
     __shared__ int finite_s;
-    extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles
+    extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles, then the
+                                      // row and column (uint8) of each entry
     const int j = blockIdx.x;
     if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
@@ -391,7 +410,8 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
 __global__ __launch_bounds__(kLanes) void step_kernel(State S, Options o, const int32_t* list,
                                                       int count, const int32_t* dcount) {
     __shared__ SolveLds L;
-    extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles
+    extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles, then the
+                                      // row and column (uint8) of each entry
     const int j = blockIdx.x;
     if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
@@ -499,7 +519,9 @@ __global__ __launch_bounds__(64) void fd_pattern_kernel(int count, int P, const 
 }
 
 inline hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
-inline size_t tri_bytes(int P) { return size_t(P) * size_t(P + 1) / 2 * sizeof(double); }
+inline size_t tri_bytes(int P) {
+    return size_t(tri_size(P)) * (sizeof(double) + 2) + 7; // + the entries' row and column bytes
+}
 inline unsigned blocks_of(int n, int per) { return unsigned((n + per - 1) / per); }
 
 } // namespace
