@@ -149,14 +149,16 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             __syncthreads();
         }
         __syncthreads();
-        // forward substitution: lane i subtracts l(i, k) b_k for k = 0, 1, ... as b_k is final
+        // forward substitution: lane i subtracts l(i, k) y_k for k = 0, 1, ... as y_k is final;
+        // y_k = (lane k's sum) / l(k, k) is formed in every lane alike, so no LDS round trip
         {
             double acc = lane < f ? L.rhs[lane] : 0.0;
             for (int k = 0; k < f; ++k) {
-                if (lane == k) L.rhs[k] = __ddiv_rn(acc, m[tri(k, k, f)]);
-                __syncthreads();
-                if (lane > k && lane < f) acc -= m[tri(k, lane, f)] * L.rhs[k];
+                const double yk = __ddiv_rn(lane_value(acc, k), m[tri(k, k, f)]);
+                if (lane == k) acc = yk;
+                else if (lane > k && lane < f) acc -= m[tri(k, lane, f)] * yk;
             }
+            if (lane < f) L.rhs[lane] = acc;
         }
         // back substitution, x_i = (b_i - sum_{k > i} l(k, i) x_k) / l(i, i) for i = f-1 ... 0:
         // lane k keeps b_k, then x_k, in a register and forms the product l(k, i) x_k; every lane
