@@ -8,7 +8,8 @@ equations for the runs that need a new Jacobian, the damped step solve, one tria
 Inputs are generated in host memory, uploaded once per backend call by the engine (inputs are
 the template system and the run points; the timed region includes that small H2D traffic).
 
-Prints one JSON line (see DESIGN.md for the fields). Single GPU only.
+Prints one JSON line: the metric and its roofline and CPU baseline, with the FD, MERIT-rate,
+constraint-row, soft-failure-mode and uneven-topology figures beside it. Single GPU only.
 """
 
 import argparse
@@ -172,7 +173,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards "
-              "by variant ranges with no inter-GPU traffic, see DESIGN.md", file=sys.stderr)
+              "by variant ranges with no inter-GPU traffic", file=sys.stderr)
         sys.exit(2)
     from mikroptim_amd import abi, systems
     from mikroptim_amd.engine import TraceEngine
