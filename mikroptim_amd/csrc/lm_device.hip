@@ -207,7 +207,9 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         L.aq[q] = aq;
     }
     __syncthreads();
-    const double nrm = wave_max(lane < n ? L.ratio[lane] : 0.0);
+    // dmax(0, .) per lane first: a NaN ratio drops out as in the host's sequential std::max,
+    // so the butterfly only sees ordered values and every lane ends with the same maximum
+    const double nrm = wave_max(dmax(0.0, lane < n ? L.ratio[lane] : 0.0));
     if (lane == 0) {
         double lin = 0.0, quad = 0.0;
         for (int q = 0; q < n; ++q) {

@@ -53,7 +53,7 @@ def test_residuals_objective(engine, objective):
     assert np.array_equal(st, sto)
     assert (st == 0).all()
     err = np.abs(opd - opdo)
-    assert err.max() <= max(OPD_TOL, 1e-10 * 0)  # absolute bound per ray
+    assert err.max() <= OPD_TOL  # absolute bound per ray
 
 
 @pytest.mark.parametrize("failure", [abi.FAILURE_SOFT, abi.FAILURE_HARD])
