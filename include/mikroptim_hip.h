@@ -47,6 +47,18 @@ int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settin
                      const double* extra_merit, double* merit, int64_t* failed,
                      int64_t* hard_failed, double* partial, int* partial_counts, void* stream);
 
+/* mko_merit_device without the workspace: the tuple partials live in a static device array of
+ * the library (8 MiB per GPU), so the call allocates no device memory and calls no allocating
+ * routine. No counterpart in the reference; for callers that cannot allocate (code under
+ * hipGraph capture, serving loops, harnesses that forbid hipMalloc). At most 524288
+ * (variant, field, wavelength) tuples per call (variants * fields * wavelengths); larger batches
+ * are rejected with MKO_ERR_INVALID. All calls on one GPU share the array: calls that overlap
+ * on one GPU (other streams or threads) race, calls ordered on one stream do not. Results equal
+ * mko_merit_device's bit for bit. */
+int mko_merit_device_noalloc(const mko_batch* batch, const mko_residual_settings* settings,
+                             const double* extra_merit, double* merit, int64_t* failed,
+                             int64_t* hard_failed, void* stream);
+
 /* Writes runs * per_run variants: variant (i, j) is a copy of template variant i (or 0 if
  * !template_per_run) with the variables x0[i] applied; for j > 0, component (j - 1) / 2 is
  * replaced by plus[i] (j odd) or minus[i] (j even). per_run = 2P + 1 gives the central FD

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -2222,6 +2223,38 @@ int rows_merit(int V, const double* rows, int64_t stride, int64_t offset, int E,
 
 } // namespace mko_internal
 
+namespace {
+
+// Tuple partials of mko_merit_device_noalloc: [kScratchTuples] doubles (merit partials), then
+// 2 x [kScratchTuples] ints (failed / hard-failed counts), 8 MiB in all. A static device array
+// of the library: each GPU the library runs on holds its own copy until the process ends, and
+// every call on that GPU shares it.
+constexpr size_t kScratchTuples = size_t(1) << 19;
+constexpr size_t kScratchBytes = kScratchTuples * (sizeof(double) + 2 * sizeof(int));
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device. In host code `scratch` names a host-side
+// stand-in, never the device memory: the address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept per device (each GPU's copy has its own).
+unsigned char* scratch_address() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mutex;
+    static unsigned char* address[kMaxDevices] = {};
+    int device = 0;
+    check_hip(hipGetDevice(&device), "hipGetDevice");
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(mutex);
+    if (!address[device]) {
+        void* ptr = nullptr;
+        check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) throw std::runtime_error("hipGetSymbolAddress(scratch) returned no address");
+        address[device] = static_cast<unsigned char*>(ptr);
+    }
+    return address[device];
+}
+
+} // namespace
+
 extern "C" {
 
 const char* mko_last_error(void) { return mko_detail::last_error.c_str(); }
@@ -2250,6 +2283,35 @@ int mko_merit_device(const mko_batch* batch, const mko_residual_settings* settin
         const hipStream_t s = static_cast<hipStream_t>(stream);
         int* pf = partial_counts;
         int* ph = partial_counts + size_t(P.V) * P.T;
+        launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, nullptr, s);
+        hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
+                           P.T, partial, pf, ph, extra_merit, nullptr, merit, failed, hard_failed,
+                           nullptr);
+        check_hip(hipGetLastError(), "merit finalize launch");
+    })
+}
+
+int mko_merit_device_noalloc(const mko_batch* batch, const mko_residual_settings* settings,
+                             const double* extra_merit, double* merit, int64_t* failed,
+                             int64_t* hard_failed, void* stream) {
+    MKO_TRY({
+        no_rows(*settings);
+        const TraceParams P = make_params(*batch, *settings);
+        if (P.V == 0) return 0;
+        const size_t tuples = size_t(P.V) * size_t(P.T);
+        if (tuples > kScratchTuples)
+            throw std::invalid_argument(
+                "mko_merit_device_noalloc: " + std::to_string(tuples) +
+                " (variant, field, wavelength) tuples, at most " + std::to_string(kScratchTuples) +
+                " per call (split the batch, or use mko_merit_device with a workspace)");
+        unsigned char* base = scratch_address();
+        double* partial = reinterpret_cast<double*>(base);
+        int* counts = reinterpret_cast<int*>(base + kScratchTuples * sizeof(double));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        int* pf = counts;
+        int* ph = counts + tuples;
+        // every tuple's partial and counts are written by its trace block before the finalize
+        // reads them, so the previous call's contents need no clearing
         launch_trace(1, P, nullptr, nullptr, nullptr, partial, pf, ph, nullptr, s);
         hipLaunchKernelGGL(merit_finalize_kernel, dim3((P.V + 255) / 256), dim3(256), 0, s, P.V,
                            P.T, partial, pf, ph, extra_merit, nullptr, merit, failed, hard_failed,

@@ -22,7 +22,8 @@ _lib = None
 
 # every symbol include/mikroptim_hip.h declares
 EXPORTS = [
-    "mko_last_error", "mko_residuals_device", "mko_merit_device", "mko_point_batch_device",
+    "mko_last_error", "mko_residuals_device", "mko_merit_device", "mko_merit_device_noalloc",
+    "mko_point_batch_device",
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
     "mko_adjoint_normal_device", "mko_gram_device", "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
@@ -61,6 +62,8 @@ def library():
                                  C.c_int),
         "mko_merit_device": ([bp, sp, _dp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _dp,
                               C.POINTER(C.c_int), C.c_void_p], C.c_int),
+        "mko_merit_device_noalloc": ([bp, sp, _dp, _dp, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64), C.c_void_p], C.c_int),
         "mko_gram_device": ([C.c_void_p, _dp, C.c_int, C.c_int, C.c_int64, _dp, _dp,
                              C.c_void_p], C.c_int),
         "mko_adjoint_jacobian_device": ([bp, sp, vp, C.c_int, C.c_void_p, _dp,
