@@ -32,8 +32,12 @@ def parse():
     ap.add_argument("--fd-steps", type=int, default=None,
                     help="timed FD cycles (default: --steps, after the same --warmup)")
     ap.add_argument("--variants", type=int, default=4096)
-    ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--threads", type=int, default=None,
+                    help="host threads of the CPU baseline and of the LM's host work (default: "
+                         "the process's CPU share, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--convergence-cycles", type=int, default=20,
+                    help="LM cycles of the time-to-merit comparison")
     ap.add_argument("--no-uneven", action="store_true",
                     help="leave out the uneven-topology measurement (for short profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -49,6 +53,60 @@ def sync():
     import torch
 
     torch.cuda.synchronize()
+
+
+def cpu_share():
+    """Host threads this process may use: its CPU affinity, capped by OMP_NUM_THREADS where that
+    is set (a GPU box's share of a shared host is set there, whatever the core count shows)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def convergence(eng, base, settings, variables, x0, method, precision, cycles, threads):
+    """Mean merit over the runs after each LM cycle and the device time of the cycles so far
+    (each cycle timed alone; the merit download between cycles is not counted)."""
+    from mikroptim_amd import abi
+
+    opt = abi.lm_options(max_iterations=cycles, threads=threads, jacobian_method=method,
+                         reject_new_failures=True)
+    sess = eng.lm(base, settings, variables, x0, opt, precision)
+    elapsed, trace = 0.0, []
+    for _ in range(cycles):
+        t0 = time.perf_counter()
+        alive = sess.cycle()
+        sync()
+        elapsed += time.perf_counter() - t0
+        trace.append((elapsed, float(np.mean(sess.results().merit))))
+        if not alive:
+            break
+    sess.close()
+    return trace
+
+
+def time_to_merit(traces):
+    """Cycles and seconds each LM needs to bring the mean merit to the larger of their final
+    mean merits (a level both reach)."""
+    target = max(t[-1][1] for t in traces.values())
+    out = {"target_mean_merit": target}
+    for name, t in traces.items():
+        k = next(i for i, (_, m) in enumerate(t) if m <= target)
+        out[name] = {"cycles": k + 1, "seconds": t[k][0], "final_mean_merit": t[-1][1],
+                     "mean_merit_per_cycle": [m for _, m in t]}
+    return out
 
 
 def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps, threads,
@@ -136,11 +194,13 @@ def profiled_traffic(runs_per_launch):
     return (float(f.group(1)) + float(w.group(1))) * runs_per_launch
 
 
-def cpu_baseline(batch, variables, settings, x_gpu_final, eng, sample_runs=64, threads=16):
-    """Reference CPU LM (the reference's own CpuLmBackend with its adjoint Jacobian, built from
-    source into oracle/_ref; the oracle's restatement if that build is absent) timed on a bounded
-    sample of runs for one LM cycle, scaled to 4096 runs. Also checks the GPU forward trace
-    against the FP64 oracle on the final points of a few runs (OPD error in waves)."""
+def cpu_baseline(batch, variables, settings, x_gpu_final, eng, threads, sample_runs=64,
+                 fd_sample_runs=16):
+    """Reference CPU LM (the reference's own CpuLmBackend, built from source into oracle/_ref; the
+    oracle's restatement if that build is absent) timed on bounded samples of runs for one LM
+    cycle each, scaled to the batch's runs: with its adjoint Jacobian (the value) and with its FD
+    Jacobian. Also checks the GPU forward trace against the FP64 oracle on the final points of a
+    few runs (OPD error in waves)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from mikroptim_amd import abi
     from oracle_lib import oracle, reference
@@ -149,12 +209,14 @@ def cpu_baseline(batch, variables, settings, x_gpu_final, eng, sample_runs=64, t
     if lib is None:
         lib, kind = oracle(), "port"
     base = batch.take([0])
-    x0 = variables.extract(batch)[:sample_runs]
-    opt = abi.lm_options(max_iterations=1, threads=threads, jacobian_method=abi.JACOBIAN_ADJOINT)
-    t0 = time.perf_counter()
-    lib.optimize_lm(base, settings, variables, x0, opt)
-    dt = time.perf_counter() - t0
-    it_s = 1.0 / (dt * batch.variants / sample_runs)
+    rates = {}
+    for method, runs in ((abi.JACOBIAN_ADJOINT, sample_runs), (abi.JACOBIAN_FD, fd_sample_runs)):
+        x0 = variables.extract(batch)[:runs]
+        opt = abi.lm_options(max_iterations=1, threads=threads, jacobian_method=method)
+        t0 = time.perf_counter()
+        lib.optimize_lm(base, settings, variables, x0, opt)
+        dt = time.perf_counter() - t0
+        rates[method] = (1.0 / (dt * batch.variants / runs), dt)
     # OPD error of the GPU forward trace vs the FP64 oracle at the GPU's final points
     pts = base.take(np.zeros(4, dtype=np.int64))
     variables.apply(pts, x_gpu_final[:4])
@@ -163,10 +225,48 @@ def cpu_baseline(batch, variables, settings, x_gpu_final, eng, sample_runs=64, t
     lam = batch.reference_wavelength_nm * 1e-6
     ok = (st_g == 0) & (st_o == 0)
     err = float(np.sqrt(np.mean(((opd_g - opd_o)[ok] / lam) ** 2)))
-    return {"value": it_s, "unit": "LM iter/s @4096 variants", "cores": threads, "kind": kind,
+    adj, fd = rates[abi.JACOBIAN_ADJOINT], rates[abi.JACOBIAN_FD]
+    return {"value": adj[0], "unit": "LM iter/s @4096 variants", "cores": threads, "kind": kind,
+            "cpu_model": cpu_model(), "host_cpus_visible": os.cpu_count(),
             "sample": f"{sample_runs} runs x 1 LM cycle (adjoint Jacobian, FP64, {threads} host "
                       f"threads), scaled to {batch.variants} runs",
-            "seconds": dt}, err, bool(np.array_equal(st_g, st_o))
+            "seconds": adj[1],
+            "fd_lm_iter_per_s": fd[0],
+            "fd_sample": f"{fd_sample_runs} runs x 1 LM cycle (FD Jacobian, FP64, {threads} host "
+                         f"threads), scaled to {batch.variants} runs",
+            "fd_seconds": fd[1]}, err, bool(np.array_equal(st_g, st_o))
+
+
+def singlet_rates(eng, threads):
+    """Config 1 (the singlet, 1024 variants x 648 rays, FP64 RESIDUALS): GPU trace-kernel rays/s
+    and the reference's residuals_cpu (one thread, as the reference runs it) and the oracle's
+    restatement on the host threads, in rays/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from mikroptim_amd import abi, systems
+    from mikroptim_amd.batch import rays_per_variant
+    from oracle_lib import oracle, reference
+
+    b, _ = systems.singlet(1024)
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    rays = b.variants * rays_per_variant(b, s)
+    eng.residuals(b, s)
+    eng.kernel_timing(True)
+    eng.residuals(b, s)
+    ms, _ = eng.kernel_timing(False)["trace"]
+    out = {"workload": "singlet, 1024 variants x 3 fields x 3 wavelengths x 6x12 Half pupil "
+                       "(648 rays/variant), FP64 RESIDUALS, hard failure",
+           "rays": rays, "gpu_rays_per_s": rays / (ms * 1e-3) if ms > 0 else None}
+    ref = reference()
+    if ref is not None:
+        t0 = time.perf_counter()
+        ref.residuals(b, s)
+        out["cpu_reference_rays_per_s"] = rays / (time.perf_counter() - t0)
+        out["cpu_reference_threads"] = 1
+    t0 = time.perf_counter()
+    oracle().residuals(b, s, threads=threads)
+    out["cpu_port_rays_per_s"] = rays / (time.perf_counter() - t0)
+    out["cpu_port_threads"] = threads
+    return out
 
 
 def main():
@@ -178,6 +278,8 @@ def main():
     from mikroptim_amd import abi, systems
     from mikroptim_amd.engine import TraceEngine
 
+    if args.threads is None:
+        args.threads = cpu_share()
     batch, variables = systems.objective(variants=args.variants, seed=2)
     settings = systems.objective_settings(abi.FAILURE_HARD)
     base = batch.take([0])
@@ -210,9 +312,9 @@ def main():
     runs_per_launch = jac_adj / adj_n if adj_n else 0.0
     if adj_n:
         achieved = runs_per_launch * rays * flop_rev / (adj_ms / adj_n * 1e-3) / 1e12
-    # compute-bound (FP64 vector ALU): MI355X's FP64 vector and FP64 matrix peaks are the same
-    # 78.6 TFLOP/s, so the compute roof is that figure
-    roof = {"bound": "mfma", "roof": "FP64 compute (vector ALU), 78.6 TFLOP/s",
+    # bound by FP64 arithmetic on the vector ALU (the MFMA share is 6 P^2 bf16 FLOP per ray, which
+    # MI355X's 2.5 PFLOP/s bf16 matrix rate makes negligible); roof: the FP64 vector peak
+    roof = {"bound": "valu", "roof": "FP64 vector ALU, 78.6 TFLOP/s",
             "kernel": "adjoint_normal_kernel (fused adjoint + normal equations)",
             "achieved": achieved, "peak": 78.6, "unit": "TFLOP/s",
             "frac": (achieved / 78.6) if achieved else None,
@@ -274,16 +376,34 @@ def main():
     xs = res_adj.x[:8]
     _, _, am, gm = eng.jacobian(base, settings, variables, xs, method="adjoint",
                                 precision="mixed", want_jacobian=False)
-    _, _, ad, gd = eng.jacobian(base, settings, variables, xs, method="adjoint", precision="fp64")
+    _J8, _r8, ad, gd = eng.jacobian(base, settings, variables, xs, method="adjoint",
+                                    precision="fp64")
+    nJ = np.linalg.norm(_J8.reshape(len(xs), -1), axis=1)
+    nr = np.linalg.norm(_r8, axis=1)
     mixed_err = {
         "A_rel_frobenius": float(np.max(np.linalg.norm(am - ad, axis=(1, 2)) /
                                         np.linalg.norm(ad, axis=(1, 2)))),
         "A_diag_rel": float(np.max(np.abs(np.diagonal(am, axis1=1, axis2=2) /
                                           np.diagonal(ad, axis1=1, axis2=2) - 1))),
         "g_rel": float(np.max(np.linalg.norm(gm - gd, axis=1) / np.linalg.norm(gd, axis=1))),
+        # against the absolute bounds of the split-bf16 products and the float staging:
+        # ||dA||_F <= (3 u_bf^2 + 514 u_f) ||J||_F^2, ||dg|| <= 4 u_f ||J||_F ||r||
+        "A_vs_bound": float(np.max(np.linalg.norm(am - ad, axis=(1, 2)) /
+                                   ((3 * 2.0 ** -16 + 514 * 2.0 ** -24) * nJ ** 2))),
+        "g_vs_bound": float(np.max(np.linalg.norm(gm - gd, axis=1) /
+                                   (4 * 2.0 ** -24 * nJ * nr))),
         "runs": 8}
 
     uneven = None if args.no_uneven else uneven_groups(eng, threads=args.threads)
+
+    # per-cycle progress of the two LMs on the same starts: cycles and device seconds each needs
+    # to reach a mean merit both reach (LM iter/s alone does not show what a cycle buys)
+    traces = {name: convergence(eng, base, settings, variables, x0, method, prec,
+                                args.convergence_cycles, args.threads)
+              for name, method, prec in (("adjoint_mixed", abi.JACOBIAN_ADJOINT, "mixed"),
+                                         ("fd_fp64", abi.JACOBIAN_FD, "fp64"))}
+    to_merit = time_to_merit(traces)
+    config1 = singlet_rates(eng, args.threads)
 
     # soft failure mode (the reference CLI's default) on the same workload: the runs whose point
     # has no failed ray take the fused kernel on the MERIT trace's stored ray states, the others
@@ -314,6 +434,8 @@ def main():
         "merit_trace_rays_per_s": trace_rate,
         "constraint_rows": rows,
         "uneven_topologies": uneven,
+        "time_to_merit": to_merit,
+        "singlet_config1": config1,
         "soft_failure_mode": soft,
         "speedup_vs_fd_fp64": it_adj / it_fd if it_fd > 0 else None,
         "kernel_ms_adjoint_path": {k: v[0] for k, v in kt_adj.items()},
@@ -324,7 +446,8 @@ def main():
         "merit_trace": merit_roof,
     }
     if not args.no_cpu_baseline:
-        base_cpu, err, same_status = cpu_baseline(batch, variables, settings, res_adj.x, eng)
+        base_cpu, err, same_status = cpu_baseline(batch, variables, settings, res_adj.x, eng,
+                                                  args.threads)
         out["cpu_baseline"] = base_cpu
         out["opd_err_waves_rms"] = err
         out["opd_status_equal"] = same_status
