@@ -141,9 +141,11 @@ def run_lm(eng, base, settings, variables, x0, method, precision, warmup, steps,
 
 def uneven_groups(eng, topologies=64, per_topology=128, max_iterations=20, threads=16):
     """The uneven workload: 8192 variants of 64 random topologies (8-32 refracting surfaces,
-    3-5 fields, 3-7 wavelengths, 10-64 variables, 6x8 Half pupil; 128 jittered variants each),
-    20 LM iterations (adjoint + mixed, device control), one batched LM per topology. Timed with
-    the topologies' cycles run concurrently (optimize_groups) and one topology after another."""
+    3-5 fields, 3-7 wavelengths, 10-96 variables drawn (curvatures, thicknesses, working
+    distance, then the glass media's Buchdahl coefficients), 6x8 Half pupil; 128 jittered
+    variants each), 20 LM iterations (adjoint + mixed, device control), one batched LM per
+    topology. Timed with the topologies' cycles run concurrently (optimize_groups) and one
+    topology after another."""
     from mikroptim_amd import abi, systems
 
     groups = []
@@ -170,6 +172,7 @@ def uneven_groups(eng, topologies=64, per_topology=128, max_iterations=20, threa
             "max_iterations": max_iterations,
             "surfaces": [min(x[0] for x in shapes), max(x[0] for x in shapes)],
             "variables": [min(x[1] for x in shapes), max(x[1] for x in shapes)],
+            "topologies_with_glass_variables": sum(int(g[2].has_glass) for g in groups),
             "fields": [min(x[2] for x in shapes), max(x[2] for x in shapes)],
             "wavelengths": [min(x[3] for x in shapes), max(x[3] for x in shapes)],
             "run_iterations": iters, "seconds_concurrent": t_together,

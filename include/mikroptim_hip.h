@@ -97,8 +97,8 @@ int mko_adjoint_jacobian_device(const mko_batch* batch, const mko_residual_setti
  * trace and reverse sweep in FP64; per-ray gradients staged in LDS as float differences to a
  * per-wave pilot ray; J^T J from split-bf16 products (hi*hi + hi*lo + lo*hi) on MFMA with float
  * accumulation per (field, wavelength) tuple and FP64 across tuples; J^T r0 in FP64. Hard failure
- * mode, at most 512 rays per tuple and 64 variables. Workspace: mko_adjoint_normal_workspace()
- * bytes of device memory. Replaces adjoint_jacobian_kernel + gram_kernel
+ * mode, at most 96 variables (curvature and thickness; glass variables need the LM path's stored
+ * ray states). Workspace: mko_adjoint_normal_workspace() bytes of device memory. Replaces adjoint_jacobian_kernel + gram_kernel
  * (src/cuda/adjoint_jacobian.cu:119, src/cuda/normal_equations.cu:48) on the LM path, where only
  * J^T J and J^T r0 are consumed (src/optim/levenberg_marquardt.cpp:427-447). */
 size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, int variables);
@@ -169,8 +169,8 @@ int mko_engine_surface_extrema(mko_engine* engine, const mko_batch* batch,
  * (include/mikroptim/cuda/fd_jacobian.hpp:58, adjoint_jacobian.hpp:70): base has one variant
  * (shared template) or one per run; x0 [runs][P]. Outputs (host, any may be NULL):
  * J [runs][P][R] double, r0 [runs][R], a [runs][P][P], g [runs][P].
- * method ADJOINT with precision MKO_PRECISION_MIXED, J and r0 NULL, <= 64 variables and <= 512
- * rays per tuple takes a and g from mko_adjoint_normal_device: in hard failure mode for every
+ * method ADJOINT with precision MKO_PRECISION_MIXED, J and r0 NULL, <= 96 variables and no glass
+ * variables takes a and g from mko_adjoint_normal_device: in hard failure mode for every
  * run, in soft mode for the runs whose point has no failed ray (their penalty rows and penalty
  * derivatives are zero, so their normal equations are hard mode's). Every other run or
  * combination computes J in FP64 and a, g with mko_gram_device. */

@@ -193,8 +193,9 @@ typedef struct mko_lm_options {
     double fail_weight_factor; /* 10 */
     double max_fail_weight;    /* 1e14 */
     /* 0: the per-cycle control (step solves, acceptance, convergence tests) runs on the device
-     * for problems of at most 64 variables, and with the fused mixed-precision adjoint (hard
-     * failure, no constraint rows) each cycle is one replay of a hipGraph captured per round;
+     * for problems of at most 127 variables, and with the fused mixed-precision adjoint (hard
+     * failure, at most 96 variables, no constraint rows) each cycle is one replay of a hipGraph
+     * captured per round;
      * 1: control always on the host; 2: device control launched kernel by kernel (no graph).
      * All take the same steps. No counterpart in LmOptions (the reference's control always runs
      * on the host). */

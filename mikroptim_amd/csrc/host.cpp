@@ -810,17 +810,21 @@ public:
         const int64_t T = trace_rows_of(base, s), R = T + rp.count(), rays = rays_of(base, s);
         const bool fd = method == MKO_JACOBIAN_FD;
         // The LM consumes only J^T J and J^T r0: for the mixed-precision adjoint those come from
-        // the fused kernel and J is never formed (hard failure mode, <= 64 variables, <= 512
-        // rays per tuple).
-        const bool fused = !fd && precision == MKO_PRECISION_MIXED && !J && !r0 && (a || g) &&
-                           fused_ok(s, vars);
+        // the fused kernel and J is never formed (hard failure mode, <= 96 variables).
+        // With glass variables the fused kernel needs stored ray states (its index adjoints are
+        // in the stored-state kernels): a MERIT trace of the same points keeps them first.
+        const bool glass_fused = !fd && precision == MKO_PRECISION_MIXED && !J && !r0 &&
+                                 (a || g) && !fused_ok(s, vars) && fused_ok(s, vars, true);
+        const bool fused = glass_fused || (!fd && precision == MKO_PRECISION_MIXED && !J && !r0 &&
+                                           (a || g) && fused_ok(s, vars));
         const bool pattern = fd || rp.count() > 0;
         const int per_run = fd ? 2 * P + 1 : 1;
         const size_t Tt = size_t(base.fields) * base.wavelengths, S = size_t(base.surfaces);
         const size_t per = size_t(per_run) * (Strides(base).doubles() * 8 +
                                               (fd ? size_t(R) * 8 + size_t(rays) : 0) +
                                               (fd && rp.Et ? (Tt + 1) * S * 16 : 0)) +
-                           (fused ? Tt * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
+                           (fused ? Tt * (size_t(P) * P * 4 + size_t(P) * 64 + 24) + 1024 +
+                                        (glass_fused ? size_t(rays) * 56 + Tt * 16 + 32 : 0)
                                   : size_t(P) * R * 8 + size_t(R) * 8 + size_t(rays)) +
                            size_t(P) * (P + 1) * 8 + 64 * size_t(P) +
                            (!fd ? rows_fd_bytes(rp, base, P) : 0);
@@ -876,10 +880,28 @@ public:
                 const size_t wbytes =
                     mko_adjoint_normal_workspace(count, base.fields, base.wavelengths, P);
                 void* ws = jac_.reserve(wbytes);
+                mko_internal::RayStates st{};
+                if (glass_fused) {
+                    const int64_t nrays = rays_of(base, s);
+                    st.data = states_.as<double>(size_t(count) * size_t(nrays) * 7);
+                    st.slot = nullptr;
+                    st.slot_offset = 0;
+                    st.slots = count;
+                    st.rays = nrays;
+                    mko_residual_settings sp = s;
+                    sp.precision = MKO_PRECISION_FP64; // the states come from the FP64 trace
+                    sp.extra = nullptr;
+                    timer.run(kTrace, [&] {
+                        dev_check(mko_internal::merit_states(
+                            &work_.view, &sp, smerit_.as<double>(size_t(count)),
+                            sfailed_.as<int64_t>(size_t(count)), shard_.as<int64_t>(size_t(count)),
+                            spart_.as<double>(size_t(count) * Tt),
+                            spc_.as<int>(size_t(count) * Tt * 2), st, nullptr));
+                    });
+                }
                 timer.run(kAdjoint, [&] {
                     dev_check(mko_internal::adjoint_normal_states(&work_.view, &s, &vars_host_dev_,
-                                                                  da, dg, ws, wbytes,
-                                                                  mko_internal::RayStates{},
+                                                                  da, dg, ws, wbytes, st,
                                                                   nullptr));
                 });
                 if (rp.count()) {
@@ -989,14 +1011,15 @@ public:
         upload_rows(rp);
         const int64_t T = trace_rows_of(tmpl, s), R = T + rp.count(), rays = rays_of(tmpl, s);
         const bool fd = method == MKO_JACOBIAN_FD;
-        const bool fused = !fd && precision == MKO_PRECISION_MIXED && fused_ok(s, vars_host_dev_);
+        const bool fused = !fd && precision == MKO_PRECISION_MIXED &&
+                           fused_ok(s, vars_host_dev_, states != nullptr);
         const bool pattern = fd || rp.count() > 0;
         const int per_run = fd ? 2 * P + 1 : 1;
         const size_t Tt = size_t(tmpl.fields) * tmpl.wavelengths, S = size_t(tmpl.surfaces);
         const size_t per = size_t(per_run) * (Strides(tmpl).doubles() * 8 +
                                               (fd ? size_t(R) * 8 + size_t(rays) : 0) +
                                               (fd && rp.Et ? (Tt + 1) * S * 16 : 0)) +
-                           (fused ? Tt * (size_t(P) * P * 4 + size_t(P) * 16 + 24) + 1024
+                           (fused ? Tt * (size_t(P) * P * 4 + size_t(P) * 64 + 24) + 1024
                                   : size_t(P) * R * 8 + size_t(R) * 8) +
                            size_t(P) * 48 + 64 + (!fd ? rows_fd_bytes(rp, tmpl, P) : 0);
         const int chunk = tranche_size(count, per);
@@ -1156,11 +1179,14 @@ public:
         part(other, s, MKO_PRECISION_FP64);
     }
 
-    static bool fused_ok(const mko_residual_settings& s, const mko_variables& v) {
+    // The fused mixed-precision normal equations apply: hard failure mode, at most 96 variables;
+    // glass variables only where the reverse sweeps start from an LM MERIT trace's stored ray
+    // states (`states`), whose kernels carry the index adjoints.
+    static bool fused_ok(const mko_residual_settings& s, const mko_variables& v,
+                         bool states = false) {
         bool glass = false;
         for (int k = 0; k < v.count; ++k) glass = glass || v.type[k] == MKO_VAR_GLASS;
-        return s.failure == MKO_FAILURE_HARD && v.count <= 64 &&
-               int64_t(s.rings) * s.spokes <= 512 && !glass;
+        return s.failure == MKO_FAILURE_HARD && v.count <= 96 && (!glass || states);
     }
 
     int device() const { return device_; }
@@ -1301,6 +1327,7 @@ private:
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
     Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
     std::vector<char> items_host_;
+    Buffer states_, smerit_, sfailed_, shard_, spart_, spc_; // ray states of the glass mixed path
     Buffer x_, plus_, minus_, jac_, r0_, fdres_, fdst_, sp_, sm_, sten_, ch_, a_, g_, vtype_, vsurf_,
         vcomp_;
     double* d_merit_ = nullptr;
@@ -1849,7 +1876,7 @@ private:
     void dev_allocate_states(int n) {
         use_states_ = false;
         if (o_.jacobian_method != MKO_JACOBIAN_ADJOINT || precision_ != MKO_PRECISION_MIXED ||
-            !(Engine::fused_ok(settings_, vars_) || soft_split_) ||
+            !(Engine::fused_ok(settings_, vars_, true) || soft_split_) ||
             settings_.precision != MKO_PRECISION_FP64)
             return;
         const int64_t rays = rays_of(base_, settings_);
@@ -2062,7 +2089,7 @@ private:
     bool graph_eligible() const {
         return graph_wanted_ && !graph_failed_ && use_states_ && settings_.extra == nullptr &&
                precision_ == MKO_PRECISION_MIXED && o_.jacobian_method == MKO_JACOBIAN_ADJOINT &&
-               Engine::fused_ok(settings_, vars_);
+               Engine::fused_ok(settings_, vars_, true);
     }
 
     void graph_reset() {
@@ -2145,7 +2172,12 @@ private:
         mko_variables dv = vars_;
         dv.type = vt;
         dv.batch_surface = vs;
-        dv.component = nullptr; // no glass variables in the captured cycle
+        dv.component = nullptr;
+        if (vars_.component) { // glass variables: their components, on the graph's own buffer
+            int32_t* vc = gvc_.as<int32_t>(P);
+            hip_check(hipMemcpy(vc, vars_.component, P * 4, hipMemcpyHostToDevice), "graph vars");
+            dv.component = vc;
+        }
         // the round's start (null stream) is complete before the first replay on gs_
         hip_check(hipStreamSynchronize(nullptr), "graph start");
         int32_t* list_j = lists_;
@@ -2354,7 +2386,7 @@ private:
     int32_t* counts_host_ = nullptr;
     std::vector<GraphTimed> gparts_;
     DeviceBatch gwork_;
-    Buffer gws_, gac_, ggc_, gxs_, gtidx_, gfm_, gff_, gfh_, gpart_, gpc_, gvt_, gvs_;
+    Buffer gws_, gac_, ggc_, gxs_, gtidx_, gfm_, gff_, gfh_, gpart_, gpc_, gvt_, gvs_, gvc_;
 };
 
 } // namespace mko_host

@@ -28,11 +28,13 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
                  const RayStates& states, void* stream, const int32_t* live = nullptr);
 
 // mko_adjoint_normal_device starting each ray's reverse sweep from the stored final state
-// instead of tracing it forward (the states must come from a MERIT trace of the same points).
+// instead of tracing it forward (the states must come from a MERIT trace of the same points);
+// with states, glass variables are taken as well. waves: workgroup size of the fused kernel in
+// wavefronts (1, 2 or 4; 0 chooses from the tuple size).
 int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* settings,
                           const mko_variables* vars, double* a, double* g, void* workspace,
                           size_t workspace_bytes, const RayStates& states, void* stream,
-                          const int32_t* live = nullptr);
+                          const int32_t* live = nullptr, int waves = 0);
 
 // mko_point_batch_device with an optional device array template_index [runs]: run i copies
 // template variant template_index[i] (NULL: as mko_point_batch_device).

@@ -1280,12 +1280,16 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
 // ---- ADJOINT fused with the normal equations (hard failure mode) ---------------------------
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-constexpr int kMaxVariablesFused = 64;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
+constexpr int kMaxVariablesFused = 96;
+constexpr int kFusedSlots = 4; // S / Tr slots per tuple in the workspace, one per wave (<= 4)
+#ifndef MKO_FUSED_WAVES_PER_EU
+#define MKO_FUSED_WAVES_PER_EU 3
+#endif
 
-// Per-(run, tuple) partial of the normal equations. With a per-tuple shift c (the weighted mean
-// gradient of the tuple's first 256 rays), o (their weighted mean OPL) and u_k = sqrt(W_k) (g_k - c)
-// over the rays in the piston:
+// Per-(run, tuple) partial of the normal equations. With a per-tuple shift c (near the tuple's
+// weighted mean gradient), o (near its weighted mean OPL) and u_k = sqrt(W_k) (g_k - c) over the
+// rays in the piston:
 //   C  = sum_k u_k u_k^T                  (P x P, MFMA, float)
 //   S  = sum_k W_k (g_k - c)              (P, double)
 //   Tr = sum_k W_k (g_k - c) (OPL_k - o)  (P, double)
@@ -1297,10 +1301,28 @@ struct TupleNormal {
     double w_ok, piston, s2;
 };
 
-// LDS row stride of the gradient block (floats during the sweep; afterwards each row holds 264
-// bf16 hi then 264 bf16 lo values in the same 1056 bytes, so every 8-ray MFMA operand is one
-// aligned 16-byte load).
-constexpr int kGStride = 264;
+// Gradient block of one chunk of rays in LDS: P rows of 64 * kNW rays, 4 bytes per ray (a float
+// during the sweep, then bf16 hi and lo). Rays sit in groups of 8 (32 bytes); the group's place in
+// row p is XOR-swizzled by p, so that the MFMA operand loads (32 rows at one ray group, 16 bytes
+// per lane) hit distinct banks. A group's floats and its bf16 values share the same 32 bytes (hi
+// in one half, lo in the other, the order alternating with bit 3 of the swizzled slot for the
+// same reason), so the 8 lanes of a group -- one wave -- convert in place without a barrier.
+template <int kNW>
+struct GradLayout {
+    static constexpr int kRow = 64 * kNW;
+    static constexpr int kRowBytes = 4 * kRow;
+    static constexpr int kMask = kRow / 8 >= 16 ? 15 : kRow / 8 - 1;
+    __device__ static __forceinline__ int slot(int group, int p) { return group ^ (p & kMask); }
+    __device__ static __forceinline__ int float_off(int p, int k) {
+        return p * kRowBytes + ((4 * k) ^ ((p & kMask) << 5));
+    }
+    __device__ static __forceinline__ int hi_off(int p, int s) {
+        return p * kRowBytes + 32 * s + (((s >> 3) & 1) << 4);
+    }
+    __device__ static __forceinline__ int lo_off(int p, int s) {
+        return p * kRowBytes + 32 * s + ((((s >> 3) & 1) ^ 1) << 4);
+    }
+};
 
 __device__ __forceinline__ uint16_t bf16_bits(float x) {
     // round to nearest even (finite inputs)
@@ -1321,294 +1343,467 @@ __device__ __forceinline__ double first_lane(double x) {
 // write column p together: the value of the wave's first such lane (pilot) goes to pilot[p] and
 // each lane stores its difference to it in float. The float then carries the variation of the
 // gradient over the pupil, not its (often much larger) common part.
-struct LdsSink {
-    float* G;      // [P][kGStride], column = lane of the block
-    double* pilot; // [64] of this wave
-    int col;
-    bool leader;   // first lane of the wave with a reached ray
+template <int kNW>
+struct SwzSink {
+    unsigned char* G; // gradient block
+    double* pilot;    // [P] of this wave
+    int k;            // ray of the chunk (thread)
+    bool leader;      // first lane of the wave with a reached ray
     __device__ __forceinline__ void write(int p, double value) const {
         if (p < 0) return;
         const double c = first_lane(value);
-        G[p * kGStride + col] = float(value - c);
+        *reinterpret_cast<float*>(G + GradLayout<kNW>::float_off(p, k)) = float(value - c);
         if (leader) pilot[p] = c;
     }
 };
 
 // backward_ray (adjoint_backward.hpp:496-580) for a ray that reached the image in hard mode, with
 // the states on surfaces S-2 ... 1 reconstructed from the last state by inverse steps and the
-// state on surface 0 regenerated from the source ray (no per-ray record in memory).
-template <class RB>
-__device__ __forceinline__ void backward_to_lds(const TraceParams& P, const SurfLds& L,
-                                                const ParamMap& map, const LastState& last,
-                                                const Ray& start, V3<double> M, double kappa,
-                                                const LdsSink& sink) {
+// state on surface 0 regenerated from the source ray (no per-ray record in memory). kGlass: the
+// index adjoints as well, into the glass variables' columns of each medium (n_carry as in
+// backward_ray; dn/d(n0, nu1, nu2) = 1, w_l, w_l^2).
+template <bool kGlass, class Sink>
+__device__ __forceinline__ void backward_fused(const TraceParams& P, const SurfLds& L,
+                                               const ParamMap& map, const LastState& last,
+                                               const Ray& start, V3<double> M, double kappa,
+                                               const Sink& sink, int l) {
     const int S = P.S;
-    mkod::RayAdj<RB> adj{{RB(0), RB(0), RB(0)}, {RB(0), RB(0), RB(0)}};
+    mkod::RayAdj<double> adj{{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double n_carry = 0.0;
+    const auto index_sink = [&](int j, double value) {
+        const double w = map.omega[l];
+        sink.write(map.glass_col[j][0], value);
+        sink.write(map.glass_col[j][1], value * w);
+        sink.write(map.glass_col[j][2], value * (w * w));
+    };
     {
-        const V3<RB> p{RB(last.p.x), RB(last.p.y), RB(last.p.z)};
-        const V3<RB> d{RB(last.d.x), RB(last.d.y), RB(last.d.z)};
-        RB nib = RB(0);
-        const RB mz = mkod::reference_step_backward<RB>(p, d, V3<RB>{RB(M.x), RB(M.y), RB(M.z)},
-                                                         RB(kappa), P.ref_kind, RB(L.n[S - 1]),
-                                                         last.image_branch, RB(1), RB(0), adj, nib);
-        sink.write(map.thick_col[S - 1], double(mz));
+        double nib = 0.0;
+        const double mz = mkod::reference_step_backward<double>(last.p, last.d, M, kappa,
+                                                                P.ref_kind, L.n[S - 1],
+                                                                last.image_branch, 1.0, 0.0, adj,
+                                                                nib);
+        sink.write(map.thick_col[S - 1], mz);
+        n_carry = nib;
     }
     V3<double> q = last.p, d1 = last.d;
     for (int s = S - 2; s >= 1; --s) {
         const int j = s + 1;
-        double tb = 0.0, cb = 0.0;
-        const mkod::RayState<double> st = mkod::fused_step_backward(
-            q, d1, L.c[j], L.c[s], L.t[s], L.n[s], L.mu[j], L.imu[j], adj, tb, cb);
+        double tb = 0.0, cb = 0.0, nbb = 0.0, nab = 0.0;
+        const mkod::RayState<double> st = mkod::fused_step_backward<kGlass>(
+            q, d1, L.c[j], L.c[s], L.t[s], L.n[s], L.mu[j], L.imu[j], adj, tb, cb, &nbb, &nab);
         if ((P.sphere_mask >> j) & 1ull) sink.write(map.curv_col[j], cb);
         sink.write(map.thick_col[s], tb);
+        if constexpr (kGlass) {
+            index_sink(j, nab + n_carry);
+            n_carry = nbb;
+        }
         q = st.p;
         d1 = st.d;
     }
     // surface 1 from the regenerated source ray (exact state on surface 0)
     {
-        RB tb = RB(0), cb = RB(0), nbb = RB(0), nab = RB(0);
-        mkod::refracting_step_backward<RB>(start.p, start.d, RB(L.t[0]), RB(L.c[1]), RB(L.n[0]),
-                                           RB(L.n[1]), mkod::kBranchRegular, RB(1), RB(0), adj, tb,
-                                           cb, nbb, nab);
-        if ((P.sphere_mask >> 1) & 1ull) sink.write(map.curv_col[1], double(cb));
-        sink.write(map.thick_col[0], double(tb));
+        double tb = 0.0, cb = 0.0, nbb = 0.0, nab = 0.0;
+        mkod::refracting_step_backward<double>(start.p, start.d, L.t[0], L.c[1], L.n[0], L.n[1],
+                                               mkod::kBranchRegular, 1.0, 0.0, adj, tb, cb, nbb,
+                                               nab);
+        if ((P.sphere_mask >> 1) & 1ull) sink.write(map.curv_col[1], cb);
+        sink.write(map.thick_col[0], tb);
+        if constexpr (kGlass) index_sink(1, nab + n_carry);
     }
 }
 
-// One workgroup per (run, tuple), 256 lanes; tuples of at most 512 rays in halves of 256 rays.
-// Per half: forward trace in double, reverse sweep in RB over reconstructed states, gradients
-// staged in LDS as a [P][256] block (pilot-relative floats); then the shifted moments of the
-// half: C on MFMA (v_mfma_f32_32x32x16_bf16, u split into bf16 hi + lo, hi*hi + hi*lo + lo*hi;
-// waves 0, 1, 3 form the 32 x 32 tiles (0,0), (0,1), (1,1) of the padded 64 x 64 C, the tile
-// below the diagonal is its mirror), S and Tr on VALU in double.
-// kStates: the last state and OPL of each ray come from a MERIT trace of the same point
-// (mko_internal::RayStates) instead of a forward trace here.
-template <class RB, int kHalves, bool kStates = false>
-__global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, ParamMap map,
-                                                                float* partC, double* partST,
-                                                                TupleNormal* partT,
-                                                                mko_internal::RayStates states) {
-    extern __shared__ float G[]; // [P][kGStride]
+// 64-bit lane moves of the transposed reduction below.
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// x, y -> the sums of the pairs (lane, lane ^ 32) (kHalf 32) or (lane, lane ^ 16) (kHalf 16):
+// lanes below the half keep x's parameter, lanes above y's (v_permlane32_swap /
+// v_permlane16_swap exchange exactly the halves that each lane does not keep).
+template <int kHalf>
+__device__ __forceinline__ double swap_sum(double x, double y) {
+    const unsigned xl = unsigned(__double2loint(x)), xh = unsigned(__double2hiint(x));
+    const unsigned yl = unsigned(__double2loint(y)), yh = unsigned(__double2hiint(y));
+    if constexpr (kHalf == 32) {
+        const auto lo = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
+        return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+    } else {
+        const auto lo = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
+        return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+    }
+}
+// Transposed wave sum of 16 values per lane: lane L returns the sum over the 64 lanes of value
+// number L >> 2 (so lanes 4q ... 4q + 3 hold value q). Each step pairs lanes that differ in one
+// more bit and keeps half of the values: bit 5 and bit 4 by the permlane swaps, bit 3 by the row
+// mirror (lane i <-> 15 - i), bit 2 by the half-row mirror (i <-> 7 - i); then bits 1 and 0 sum
+// plainly. A fixed tree: the result does not depend on anything but the 64 x 16 inputs.
+__device__ __forceinline__ double transpose_sum16(const double (&v)[16], int lane) {
+    double a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = swap_sum<32>(v[j], v[j + 8]);
+    double b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = swap_sum<16>(a[j], a[j + 4]);
+    const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
+    double c[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const double keep = b3 ? b[j + 2] : b[j];
+        const double send = b3 ? b[j] : b[j + 2];
+        c[j] = keep + dpp_f64<0x140>(send); // row_mirror
+    }
+    const double keep = b2 ? c[1] : c[0];
+    const double send = b2 ? c[0] : c[1];
+    double r = keep + dpp_f64<0x141>(send); // row_half_mirror
+    r += dpp_f64<0x4e>(r);                  // quad_perm 2,3,0,1 (lane ^ 2)
+    r += dpp_f64<0xb1>(r);                  // quad_perm 1,0,3,2 (lane ^ 1)
+    return r;
+}
+
+template <int kNW, class T>
+__device__ __forceinline__ T block_sum_n(T v, T* lds) {
+    v = wave_sum(v);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[wave] = v;
+    __syncthreads();
+    T total = lds[0];
+#pragma unroll
+    for (int w = 1; w < kNW; ++w) total += lds[w];
+    __syncthreads();
+    return total;
+}
+
+// C tile t of the upper triangle of a tg x tg grid of 32 x 32 tiles, row-major: (I, J), I <= J.
+__device__ __forceinline__ void tile_of(int t, int tg, int& I, int& J) {
+    I = 0;
+    while (t >= tg - I) {
+        t -= tg - I;
+        ++I;
+    }
+    J = I + t;
+}
+
+// C of one 32 x 32 tile over the chunk's rays: operand lane = (m = lane % 32, kb = lane / 32),
+// 8 rays per operand, hi*hi + hi*lo + lo*hi; on the diagonal the B operands are the A operands.
+template <int kNW>
+__device__ __forceinline__ void tile_products(const unsigned char* G, int Pn, int I, int J,
+                                              int lane, f32x16& acc) {
+    using Lay = GradLayout<kNW>;
+    const int m = lane & 31, kb = lane >> 5;
+    const int pa = I * 32 + m, pb = J * 32 + m;
+    // lanes of rows >= P (the padding of the tile) load nothing and take zeros
+    const bool va = pa < Pn, vb = pb < Pn;
+    bf16x8 zero;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) zero[e] = __builtin_bit_cast(__bf16, uint16_t(0));
+#pragma unroll 4
+    for (int step = 0; step < Lay::kRow / 16; ++step) {
+        const int grp = 2 * step + kb;
+        bf16x8 ahi = zero, alo = zero;
+        if (va) {
+            const int s = Lay::slot(grp, pa);
+            ahi = *reinterpret_cast<const bf16x8*>(G + Lay::hi_off(pa, s));
+            alo = *reinterpret_cast<const bf16x8*>(G + Lay::lo_off(pa, s));
+        }
+        bf16x8 bhi = ahi, blo = alo;
+        if (I != J) {
+            bhi = zero;
+            blo = zero;
+            if (vb) {
+                const int s = Lay::slot(grp, pb);
+                bhi = *reinterpret_cast<const bf16x8*>(G + Lay::hi_off(pb, s));
+                blo = *reinterpret_cast<const bf16x8*>(G + Lay::lo_off(pb, s));
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
+    }
+}
+
+// C tile out: lane holds column n = lane % 32 of rows 8 (e / 4) + 4 (lane / 32) + e % 4.
+__device__ __forceinline__ void store_tile(float* C, int Pn, int I, int J, int lane,
+                                           const f32x16& acc) {
+    const int col = J * 32 + (lane & 31);
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int row = I * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        if (row < Pn && col < Pn) C[size_t(row) * Pn + col] = acc[e];
+    }
+}
+
+// Dynamic LDS of adjoint_normal_kernel<kNW> for P variables: the gradient block, then the
+// waves' pilots [kNW][P] and the shift [P] (doubles).
+template <int kNW>
+constexpr size_t fused_lds_bytes(int Pn) {
+    return size_t(Pn) * GradLayout<kNW>::kRowBytes + size_t(kNW + 1) * Pn * sizeof(double);
+}
+
+// One workgroup of kNW waves per (run, tuple); the tuple's rays in chunks of 64 kNW, one ray per
+// thread. Per chunk: the forward trace in double (or, kStates, the last state and OPL of each ray
+// from a MERIT trace of the same point, mko_internal::RayStates), the reverse sweep over
+// reconstructed states into the LDS gradient block (pilot-relative floats); S and Tr of each
+// wave's rays in FP64 by transposed wave sums (per-lane accumulators, summed over the waves in
+// index order at the end); the block rewritten in place as bf16 hi / lo; C on MFMA
+// (v_mfma_f32_32x32x16_bf16, 3 products, the upper tiles of the 32 x 32 tile grid spread over
+// the waves; kPMax 96: up to 3 x 3 tiles). The shift c is the weight-averaged pilot of the waves
+// of the first chunk that holds a reached ray, o its weighted mean OPL. kStream (kNW 1, tuples of
+// at most 64 rays, one chunk): the wave's tiles are formed and stored one after another.
+template <int kNW, int kPMax, bool kStates, bool kGlass>
+__global__ __launch_bounds__(64 * kNW) __attribute__((amdgpu_waves_per_eu(MKO_FUSED_WAVES_PER_EU)))
+void adjoint_normal_kernel(TraceParams P, ParamMap map,
+                                                                  float* partC, double* partST,
+                                                                  TupleNormal* partT,
+                                                                  mko_internal::RayStates states) {
+    using Lay = GradLayout<kNW>;
+    constexpr int kThreads = 64 * kNW;
+    constexpr int kTileGrid = (kPMax + 31) / 32;
+    constexpr int kTilesAll = kTileGrid * (kTileGrid + 1) / 2;
+    constexpr bool kStream = kNW == 1;
+    constexpr int kTiles = kStream ? 1 : (kTilesAll + kNW - 1) / kNW;
+    constexpr int kGroups = (kPMax + 15) / 16;
+    extern __shared__ __align__(16) unsigned char lds[];
     __shared__ SurfLds L;
-    __shared__ double red[kWaves];
-    __shared__ double opl_l[kBlock];
-    __shared__ float w_l[kBlock], sw_l[kBlock];
-    __shared__ double pilot[kWaves][64];
-    __shared__ double shift[64];          // c
-    __shared__ double delta[kWaves][64];  // pilot - c
-    __shared__ float delta_f[kWaves][64];
-    __shared__ double Sacc[64], Tacc[64];
-    __shared__ double o_shift;
+    __shared__ double red[kNW];
+    __shared__ double wsum_l[kNW], osum_l[kNW];
     if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
     const Tuple t = tuple_of(P, blockIdx.x);
     stage_surfaces(P, t, L);
     const int Pn = map.P;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    if (tid < 64) {
-        Sacc[tid] = 0.0;
-        Tacc[tid] = 0.0;
-    }
+    unsigned char* G = lds;
+    double* pilot = reinterpret_cast<double*>(lds + size_t(Pn) * Lay::kRowBytes);
+    double* shift = pilot + kNW * Pn;
+    double* own_pilot = pilot + wave * Pn;
     const size_t vf = size_t(t.v) * P.F + t.f;
-    const double a = P.pupil_radius[t.v];
+    // block-uniform values in scalar registers (they stay live across the sweep)
+    const double a = first_lane(P.pupil_radius[t.v]);
     double sin_t, cos_t;
     sincos(P.field_angle[vf], &sin_t, &cos_t);
-    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
-    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
-    const int I = wave >> 1, Jt = wave & 1; // C tile of this wave
-    const bool tile_needed = I <= Jt && I * 32 < Pn && Jt * 32 < Pn;
-    f32x16 acc;
+    sin_t = first_lane(sin_t);
+    cos_t = first_lane(cos_t);
+    const V3<double> M{0.0, first_lane(P.field_target[vf]), first_lane(L.t[P.S - 1])};
+    const double kappa =
+        first_lane(P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0);
+    const int tg = (Pn + 31) / 32;
+    const int ntiles = tg * (tg + 1) / 2;
+    const size_t tuple = size_t(blockIdx.x);
+    f32x16 acc[kTiles];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
-    double sw = 0.0, swo = 0.0;
-    LastState last;
-    // stored states of both halves, loaded up front so that their latency hides behind the
-    // surface staging and the first half's sweep
-    double pre[kStates ? kHalves : 1][7];
-    if constexpr (kStates) {
+    for (int i = 0; i < kTiles; ++i)
 #pragma unroll
-        for (int h = 0; h < kHalves; ++h) {
-            const int k = tid + h * kBlock;
-            pre[h][6] = __builtin_nan("");
+        for (int e = 0; e < 16; ++e) acc[i][e] = 0.0f;
+    // this wave's slot of the tuple's S and Tr in the workspace ([tuple][4][2 P]): each chunk adds
+    // its sums (the first one stores), so no accumulator stays live across the sweeps
+    double* slot_st = partST + (tuple * kFusedSlots + wave) * 2 * Pn;
+    bool accumulated = false;
+    double sw = 0.0, swo = 0.0, o_shift = 0.0;
+    bool shift_ready = false;
+    const int nchunks = (t.n + kThreads - 1) / kThreads;
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int k = ch * kThreads + tid;
+        double st[7];
+        if constexpr (kStates) {
+            st[6] = __builtin_nan("");
             if (k < t.n) {
                 const int64_t row = t.offset + k;
 #pragma unroll
-                for (int c = 0; c < 7; ++c) pre[h][c] = states.data[state_index(states, t.v, c, row)];
+                for (int c = 0; c < 7; ++c) st[c] = states.data[state_index(states, t.v, c, row)];
             }
         }
-    }
-    for (int h = 0; h < kHalves; ++h) {
-        const int k = tid + h * kBlock;
-        for (int p = 0; p < Pn; ++p) G[p * kGStride + tid] = 0.0f;
-        if (lane < 64) pilot[wave][lane] = 0.0;
-        __syncthreads();
-        float wk = 0.0f;
-        double opl = 0.0;
+        // this thread's entries of the block and this wave's pilots start at zero
+        for (int p = 0; p < Pn; ++p) *reinterpret_cast<float*>(G + Lay::float_off(p, tid)) = 0.0f;
+        for (int p = lane; p < Pn; p += 64) own_pilot[p] = 0.0;
+        double wk = 0.0, opl = 0.0;
         bool ok = false;
         Sample smp{};
-        Ray r, start;
+        Ray start;
+        LastState last;
         if (k < t.n) {
             smp = sample_of(P, t, k);
-            r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+            Ray r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
             start = r;
             last.count = 0;
             last.image_branch = 0;
             if constexpr (kStates) {
-                // select this half's states with constant indices (a runtime index into pre
-                // would place the array in private memory)
-                double st[7];
-#pragma unroll
-                for (int c = 0; c < 7; ++c) st[c] = pre[0][c];
-#pragma unroll
-                for (int hh = 1; hh < kHalves; ++hh)
-                    if (hh == h)
-#pragma unroll
-                        for (int c = 0; c < 7; ++c) st[c] = pre[hh][c];
-                r.opl = st[6];
-                ok = !__builtin_isnan(r.opl);
+                opl = st[6];
+                ok = !__builtin_isnan(opl);
                 last.p = V3<double>{st[0], st[1], st[2]};
                 last.d = V3<double>{st[3], st[4], st[5]};
                 last.count = P.S;
             } else {
                 Soft soft{0.0, MKO_STATUS_OK};
                 ok = trace_ray<false, true>(P, L, r, M, kappa, soft, last) == MKO_STATUS_OK;
+                opl = r.opl;
             }
         }
         const uint64_t okmask = __ballot(ok);
         if (ok) {
-            sw += smp.weight;
-            swo += smp.weight * r.opl;
-            wk = float(smp.weight);
-            opl = r.opl;
+            wk = smp.weight;
+            sw += wk;
+            swo += wk * opl;
             const bool leader = lane == int(__builtin_ctzll(okmask));
-            backward_to_lds<RB>(P, L, map, last, start, M, kappa,
-                                LdsSink{G, pilot[wave], tid, leader});
-        }
-        w_l[tid] = wk;
-        sw_l[tid] = __builtin_sqrtf(wk);
-        opl_l[tid] = opl;
-        __syncthreads();
-        if (h == 0) {
-            // c = weighted mean gradient and o = weighted mean OPL of this half (wave w handles
-            // the parameters p = w mod 4; every wave forms the same sums in the same order)
-            double wsum = 0.0, osum = 0.0;
-            double wwave[kWaves];
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const double wi = double(w_l[w * 64 + lane]);
-                wwave[w] = wave_sum(wi);
-                wsum += wwave[w];
-                osum += wi * opl_l[w * 64 + lane];
-            }
-            osum = wave_sum(osum);
-            for (int p = wave; p < Pn; p += kWaves) {
-                double accp = 0.0;
-#pragma unroll
-                for (int w = 0; w < kWaves; ++w)
-                    accp += double(w_l[w * 64 + lane]) * double(G[p * kGStride + w * 64 + lane]);
-                accp = wave_sum(accp);
-#pragma unroll
-                for (int w = 0; w < kWaves; ++w) accp += wwave[w] * pilot[w][p];
-                if (lane == 0) shift[p] = wsum > 0.0 ? accp / wsum : 0.0;
-            }
-            if (tid == 0) o_shift = wsum > 0.0 ? osum / wsum : 0.0;
-            __syncthreads();
-        }
-        if (lane < 64) {
-            const double d = lane < Pn ? pilot[wave][lane] - shift[lane] : 0.0;
-            delta[wave][lane] = d;
-            delta_f[wave][lane] = float(d);
+            backward_fused<kGlass>(P, L, map, last, start, M, kappa,
+                                   SwzSink<kNW>{G, own_pilot, tid, leader}, t.l);
         }
         __syncthreads();
-        // S and Tr of this half, fixed order
-        const double o = o_shift;
-        for (int p = wave; p < Pn; p += kWaves) {
-            double sp = 0.0, tp = 0.0;
-#pragma unroll
-            for (int w = 0; w < kWaves; ++w) {
-                const int i = w * 64 + lane;
-                const double wi = double(w_l[i]);
-                const double du = wi * (double(G[p * kGStride + i]) + delta[w][p]);
-                sp += du;
-                tp += wi > 0.0 ? du * (opl_l[i] - o) : 0.0;
-            }
-            sp = wave_sum(sp);
-            tp = wave_sum(tp);
+        if (!shift_ready) {
+            // c = sum_w W_w pilot_w / W and o = sum W OPL / W over this chunk (every thread
+            // forms the same values from the waves' sums)
+            const double ww = wave_sum(wk), wo = wave_sum(wk * opl);
             if (lane == 0) {
-                Sacc[p] += sp;
-                Tacc[p] += tp;
-            }
-        }
-        // u = sqrt(W) (g - c) of the thread's own ray for every parameter, split once into bf16
-        // hi + lo. Row p of the block is rewritten in place as [hi x 264 | lo x 264] bf16 (the
-        // same 1056 bytes), 8 rows at a time: read the chunk, sync, write it.
-        for (int p0 = 0; p0 < Pn; p0 += 8) {
-            float u[8];
-#pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int p = p0 + r;
-                u[r] = p < Pn ? sw_l[tid] * (G[p * kGStride + tid] + delta_f[wave][p]) : 0.0f;
+                wsum_l[wave] = ww;
+                osum_l[wave] = wo;
             }
             __syncthreads();
+            double W = 0.0, WO = 0.0;
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int p = p0 + r;
-                if (p < Pn) {
-                    uint16_t* row = reinterpret_cast<uint16_t*>(G + p * kGStride);
-                    const uint16_t hi = bf16_bits(u[r]);
-                    row[tid] = hi;
-                    row[kGStride + tid] = bf16_bits(u[r] - bf16_value(hi));
+            for (int w = 0; w < kNW; ++w) {
+                W += wsum_l[w];
+                WO += osum_l[w];
+            }
+            if (!(W > 0.0)) continue; // no reached ray yet: nothing of this chunk counts
+            // c0: the weight-averaged pilot of the waves; then c = c0 + S0 / W with S0 the sum of
+            // W (g - c0) over the chunk, i.e. the chunk's weighted mean gradient (the waves' S0
+            // pass through their slots of the workspace, summed in wave order)
+            for (int p = tid; p < Pn; p += kThreads) {
+                double c = 0.0;
+#pragma unroll
+                for (int w = 0; w < kNW; ++w) c += wsum_l[w] * pilot[w * Pn + p];
+                shift[p] = c / W;
+            }
+            o_shift = WO / W;
+            shift_ready = true;
+            __syncthreads();
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+                if (g * 16 < Pn) {
+                    double v[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int p = g * 16 + j;
+                        v[j] = p < Pn ? wk * (double(*reinterpret_cast<const float*>(
+                                                  G + Lay::float_off(p, tid))) +
+                                              (own_pilot[p] - shift[p]))
+                                      : 0.0;
+                    }
+                    const double r = transpose_sum16(v, lane);
+                    const int pq = g * 16 + (lane >> 2);
+                    if ((lane & 3) == 0 && pq < Pn) slot_st[pq] = r;
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            for (int p = tid; p < Pn; p += kThreads) {
+                const double* st0 = partST + tuple * kFusedSlots * 2 * Pn + p;
+                double s0 = st0[0];
+#pragma unroll
+                for (int w = 1; w < kNW; ++w) s0 += st0[w * 2 * Pn];
+                shift[p] += s0 / W;
+            }
+            __syncthreads();
+        }
+        // this wave's pilots become its deltas pilot - c
+        for (int p = lane; p < Pn; p += 64) own_pilot[p] -= shift[p];
+        __syncthreads();
+        // S and Tr of this wave's rays: W (g - c) = W (G + delta) per parameter, then the same
+        // times (OPL - o); one sum at a time keeps 16 values live
+        const double ow = ok ? opl - o_shift : 0.0;
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            if (g * 16 < Pn) {
+#pragma unroll
+                for (int pass = 0; pass < 2; ++pass) {
+                    const double f = pass == 0 ? wk : wk * ow;
+                    double v[16];
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const int p = g * 16 + j;
+                        v[j] = p < Pn ? f * (double(*reinterpret_cast<const float*>(
+                                                 G + Lay::float_off(p, tid))) +
+                                             own_pilot[p])
+                                      : 0.0;
+                    }
+                    const double r = transpose_sum16(v, lane);
+                    // lanes 4q hold parameter 16 g + q: this wave's slot of the tuple's S / Tr
+                    const int pq = g * 16 + (lane >> 2);
+                    if ((lane & 3) == 0 && pq < Pn) {
+                        double* dst = slot_st + pass * Pn + pq;
+                        *dst = accumulated ? *dst + r : r;
+                    }
                 }
             }
         }
+        accumulated = true;
+        // u = sqrt(W) (g - c) of the thread's ray, split into bf16 hi + lo in place (the ray's
+        // 32-byte group belongs to 8 lanes of this wave: its floats are read before they are
+        // overwritten)
+        {
+            const float swk = __builtin_sqrtf(float(wk));
+            const int grp = tid >> 3, i = tid & 7;
+            for (int p = 0; p < Pn; ++p) {
+                const float u =
+                    swk * (*reinterpret_cast<const float*>(G + Lay::float_off(p, tid)) +
+                           float(own_pilot[p]));
+                const uint16_t hi = bf16_bits(u);
+                const uint16_t lo = bf16_bits(u - bf16_value(hi));
+                const int s = Lay::slot(grp, p);
+                *reinterpret_cast<uint16_t*>(G + Lay::hi_off(p, s) + 2 * i) = hi;
+                *reinterpret_cast<uint16_t*>(G + Lay::lo_off(p, s) + 2 * i) = lo;
+            }
+        }
         __syncthreads();
-        // C += u u^T on MFMA: operand lane = (m = lane % 32, kb = lane / 32), 8 rays per operand,
-        // hi*hi + hi*lo + lo*hi. Only the tiles on and above the diagonal are formed (C is
-        // symmetric); on the diagonal tiles the B operands are the A operands.
-        if (tile_needed) {
-            const int m = lane & 31, kb = lane >> 5;
-            const int pa = I * 32 + m, pb = Jt * 32 + m;
-            // lanes of rows >= P (the padding of the 64 x 64 C) load nothing and take zeros: an
-            // unconditional load of a shared zero row measured 2 % slower
-            const bool va = pa < Pn, vb = pb < Pn;
-            const uint16_t* Ra = reinterpret_cast<const uint16_t*>(G + (va ? pa : 0) * kGStride);
-            const uint16_t* Rb = reinterpret_cast<const uint16_t*>(G + (vb ? pb : 0) * kGStride);
-            bf16x8 zero;
+        if constexpr (kStream) {
+            float* C = partC + tuple * size_t(Pn) * Pn;
+            for (int tt = 0; tt < ntiles; ++tt) {
+                int I, J;
+                tile_of(tt, tg, I, J);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) zero[e] = __builtin_bit_cast(__bf16, uint16_t(0));
-            for (int step = 0; step < kBlock / 16; ++step) {
-                const int k0 = step * 16 + kb * 8;
-                const bf16x8 ahi = va ? *reinterpret_cast<const bf16x8*>(Ra + k0) : zero;
-                const bf16x8 alo = va ? *reinterpret_cast<const bf16x8*>(Ra + kGStride + k0) : zero;
-                bf16x8 bhi = ahi, blo = alo;
-                if (I != Jt) {
-                    bhi = vb ? *reinterpret_cast<const bf16x8*>(Rb + k0) : zero;
-                    blo = vb ? *reinterpret_cast<const bf16x8*>(Rb + kGStride + k0) : zero;
+                for (int e = 0; e < 16; ++e) acc[0][e] = 0.0f;
+                tile_products<kNW>(G, Pn, I, J, lane, acc[0]);
+                store_tile(C, Pn, I, J, lane, acc[0]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kTiles; ++i) {
+                const int tt = wave + i * kNW;
+                if (tt < ntiles) {
+                    int I, J;
+                    tile_of(tt, tg, I, J);
+                    tile_products<kNW>(G, Pn, I, J, lane, acc[i]);
                 }
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi, acc, 0, 0, 0);
             }
         }
         __syncthreads();
     }
-    const double w_ok = block_sum(sw, red);
-    const double sum = block_sum(swo, red);
-    const size_t tuple = size_t(blockIdx.x);
-    // C tile out: lane holds column n = lane % 32 of rows 8 (e / 4) + 4 (lane / 32) + e % 4
-    if (tile_needed) {
+    if constexpr (kStream) {
+        if (!shift_ready) {
+            // no reached ray in the tuple: C = 0
+            float* C = partC + tuple * size_t(Pn) * Pn;
+            for (int e = tid; e < Pn * Pn; e += kThreads) C[e] = 0.0f;
+        }
+    } else {
         float* C = partC + tuple * size_t(Pn) * Pn;
-        const int col = Jt * 32 + (lane & 31);
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int row = I * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-            if (row < Pn && col < Pn) C[size_t(row) * Pn + col] = acc[e];
+        for (int i = 0; i < kTiles; ++i) {
+            const int tt = wave + i * kNW;
+            if (tt < ntiles) {
+                int I, J;
+                tile_of(tt, tg, I, J);
+                store_tile(C, Pn, I, J, lane, acc[i]);
+            }
         }
     }
-    if (tid < Pn) {
-        partST[tuple * 2 * Pn + tid] = Sacc[tid];
-        partST[tuple * 2 * Pn + Pn + tid] = Tacc[tid];
-    }
+    const double w_ok = block_sum_n<kNW>(sw, red);
+    const double sum = block_sum_n<kNW>(swo, red);
+    // a tuple without a reached ray: its slots hold zeros (the finalize sums kNW slots)
+    if (!accumulated)
+        for (int e = lane; e < 2 * Pn; e += 64) slot_st[e] = 0.0;
     if (tid == 0) {
         const bool valid = w_ok > 0.0;
         // piston - o, the OPL shift of Tr
@@ -1617,15 +1812,15 @@ __global__ __launch_bounds__(kBlock) void adjoint_normal_kernel(TraceParams P, P
 }
 
 // a = sum_t s_t^2 (C_t - S_t S_t^T / W_t), g = sum_t s_t^2 (Tr_t - (piston_t - o_t) S_t), tuples
-// in (field, wavelength) order; one workgroup per run. Each thread owns up to 8 entries of the
+// in (field, wavelength) order; one workgroup per run. Each thread owns up to 19 entries of the
 // upper triangle (mirrored on output); the tuples' S, Tr and factors are staged through LDS in
 // chunks of 8 tuples.
 constexpr int kFinalizeChunk = 8;
 constexpr int kFinalizeEntries = (kMaxVariablesFused * (kMaxVariablesFused + 1) / 2 + 255) / 256;
 
 __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
-    int T, int Pn, const float* partC, const double* partST, const TupleNormal* partT, double* a,
-    double* g, const int32_t* live) {
+    int T, int Pn, int slots, const float* partC, const double* partST, const TupleNormal* partT,
+    double* a, double* g, const int32_t* live) {
     __shared__ double S_l[kFinalizeChunk][2 * kMaxVariablesFused];
     __shared__ double f1[kFinalizeChunk], f2[kFinalizeChunk], pist[kFinalizeChunk];
     const int v = blockIdx.x;
@@ -1651,8 +1846,12 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
     for (int t0 = 0; t0 < T; t0 += kFinalizeChunk) {
         const int nt = min(kFinalizeChunk, T - t0);
         for (int e = tid; e < nt * 2 * Pn; e += 256) {
+            // the waves' slots of S and Tr, summed in wave order
             const int tt = e / (2 * Pn), i = e - tt * 2 * Pn;
-            S_l[tt][i] = partST[(size_t(v) * T + t0 + tt) * 2 * Pn + i];
+            const double* st = partST + (size_t(v) * T + t0 + tt) * kFusedSlots * 2 * Pn + i;
+            double x = st[0];
+            for (int w = 1; w < slots; ++w) x += st[w * 2 * Pn];
+            S_l[tt][i] = x;
         }
         if (tid < nt) {
             const TupleNormal tn = partT[size_t(v) * T + t0 + tid];
@@ -2041,18 +2240,53 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
     })
 }
 
+template <int kNW, int kPMax, bool kStates, bool kGlass>
+void launch_fused(const TraceParams& P, const ParamMap& m, float* partC, double* partST,
+                  TupleNormal* partT, const RayStates& states, hipStream_t s) {
+    const size_t tuples = size_t(P.V) * P.T;
+    hipLaunchKernelGGL((adjoint_normal_kernel<kNW, kPMax, kStates, kGlass>), dim3(unsigned(tuples)),
+                       dim3(64 * kNW), fused_lds_bytes<kNW>(m.P), s, P, m, partC, partST, partT,
+                       states);
+}
+
+template <bool kStates, bool kGlass>
+void launch_fused_waves(int waves, const TraceParams& P, const ParamMap& m, float* partC,
+                        double* partST, TupleNormal* partT, const RayStates& states,
+                        hipStream_t s) {
+    const bool wide = m.P > 64;
+#define MKO_FUSED(NW, PM) launch_fused<NW, PM, kStates, kGlass>(P, m, partC, partST, partT, states, s)
+    switch (waves) {
+    case 1: wide ? MKO_FUSED(1, 96) : MKO_FUSED(1, 64); break;
+    case 2: wide ? MKO_FUSED(2, 96) : MKO_FUSED(2, 64); break;
+    default: wide ? MKO_FUSED(4, 96) : MKO_FUSED(4, 64); break;
+    }
+#undef MKO_FUSED
+}
+
+// Waves per workgroup of the fused kernel: 1 for tuples of at most 64 rays (one chunk, tiles
+// formed one after another), 2 up to 128, else 4 (chunks of 256 rays); `requested` (1, 2, 4)
+// overrides where it fits (1 needs tuples of at most 64 rays).
+int fused_waves(const TraceParams& P, int requested) {
+    int nmax = 0;
+    for (int f = 0; f < P.F; ++f)
+        nmax = std::max(nmax, P.rings * (P.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : P.spokes));
+    if (requested == 2 || requested == 4) return requested;
+    if (requested == 1 && nmax <= 64) return 1;
+    return nmax <= 64 ? 1 : nmax <= 128 ? 2 : 4;
+}
+
 int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* settings,
                           const mko_variables* vars, double* a, double* g, void* workspace,
                           size_t workspace_bytes, const RayStates& states, void* stream,
-                          const int32_t* live) {
+                          const int32_t* live, int waves) {
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
         P.live = live;
-        const ParamMap m = make_map(*vars, *batch, false);
+        // glass variables (index adjoints in the reverse sweep) on the LM path's stored states
+        const ParamMap m = make_map(*vars, *batch, states.data != nullptr);
         if (P.soft) throw std::invalid_argument("fused adjoint normal equations need hard failure mode");
-        if (m.P > 64) throw std::invalid_argument("fused adjoint normal equations: at most 64 variables");
-        const int rpl = rays_per_lane(P);
-        if (rpl > 2) throw std::invalid_argument("fused adjoint normal equations: at most 512 rays per tuple");
+        if (m.P > kMaxVariablesFused)
+            throw std::invalid_argument("fused adjoint normal equations: at most 96 variables");
         if (P.V == 0) return 0;
         if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
         const size_t need = mko_adjoint_normal_workspace(P.V, P.F, P.L, m.P);
@@ -2063,30 +2297,22 @@ int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* s
         float* partC = reinterpret_cast<float*>(w);
         double* partST = reinterpret_cast<double*>(w + up(tuples * m.P * m.P * sizeof(float)));
         TupleNormal* partT = reinterpret_cast<TupleNormal*>(
-            w + up(tuples * m.P * m.P * sizeof(float)) + up(tuples * 2 * m.P * sizeof(double)));
+            w + up(tuples * m.P * m.P * sizeof(float)) +
+            up(tuples * kFusedSlots * 2 * m.P * sizeof(double)));
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        const size_t lds = size_t(m.P) * kGStride * sizeof(float);
-        const dim3 grid{unsigned(tuples)}, block{unsigned(kBlock)};
+        const int nw = fused_waves(P, waves);
         if (states.data) {
             if (P.precision != MKO_PRECISION_FP64)
                 throw std::invalid_argument("stored ray states need the FP64 trace");
             if (states.rays != P.rays) throw std::invalid_argument("ray states of another layout");
-            if (rpl == 1)
-                hipLaunchKernelGGL((adjoint_normal_kernel<double, 1, true>), grid, block, lds, s, P,
-                                   m, partC, partST, partT, states);
-            else
-                hipLaunchKernelGGL((adjoint_normal_kernel<double, 2, true>), grid, block, lds, s, P,
-                                   m, partC, partST, partT, states);
-        } else if (rpl == 1) {
-            hipLaunchKernelGGL((adjoint_normal_kernel<double, 1>), grid, block, lds, s, P, m, partC,
-                               partST, partT, states);
+            if (m.glass) launch_fused_waves<true, true>(nw, P, m, partC, partST, partT, states, s);
+            else launch_fused_waves<true, false>(nw, P, m, partC, partST, partT, states, s);
         } else {
-            hipLaunchKernelGGL((adjoint_normal_kernel<double, 2>), grid, block, lds, s, P, m, partC,
-                               partST, partT, states);
+            launch_fused_waves<false, false>(nw, P, m, partC, partST, partT, states, s);
         }
         check_hip(hipGetLastError(), "adjoint normal launch");
         hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
-                           m.P, partC, partST, partT, a, g, live);
+                           m.P, nw, partC, partST, partT, a, g, live);
         check_hip(hipGetLastError(), "adjoint normal finalize launch");
     })
 }
@@ -2411,7 +2637,7 @@ size_t mko_adjoint_normal_workspace(int variants, int fields, int wavelengths, i
     const size_t tuples = size_t(variants) * size_t(fields) * size_t(wavelengths);
     const size_t P = size_t(variables);
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return up(tuples * P * P * sizeof(float)) + up(tuples * 2 * P * sizeof(double)) +
+    return up(tuples * P * P * sizeof(float)) + up(tuples * kFusedSlots * 2 * P * sizeof(double)) +
            up(tuples * sizeof(TupleNormal));
 }
 

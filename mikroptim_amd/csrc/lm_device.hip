@@ -26,7 +26,8 @@ namespace mko_lmdev {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kMaxP = kMaxVariables;
+constexpr int kMaxP = 128;            // two slots per lane
+constexpr int kSlots = kMaxP / kLanes; // 2: the selects below assume it
 constexpr double kModelDamping = 1e-12;
 constexpr double kFdStepFactor = 6.0554544523933395e-06; // 2^(-52/3)
 
@@ -73,14 +74,16 @@ struct SolveLds {
 // solve_step: (A_FF + mu D_FF) delta_F = -g_F on the free variables with the active set; writes
 // trial, step_norm, predicted and solved of the run (trial, step_norm and predicted only when
 // the factorisation succeeds, like the host).
-// a, g: the run's normal equations (row-major a[q * P + p]).
+// a, g: the run's normal equations (row-major a[q * P + p]). Variable k belongs to lane k % 64,
+// slot k / 64 (two slots: up to 128 variables); values a lane keeps per variable sit in
+// two-element arrays indexed by the slot.
 __device__ void solve_step(const State& S, int run, const double* a, const double* g, double mu,
                            SolveLds& L, double* __restrict__ m) {
     const int n = S.P;
     const int lane = threadIdx.x;
     const double* x = S.x + size_t(run) * n;
     const double* d = S.d + size_t(run) * n;
-    if (lane < n) L.fixed[lane] = blocks(S, lane, x[lane], -g[lane]) ? 1 : 0;
+    for (int k = lane; k < n; k += kLanes) L.fixed[k] = blocks(S, k, x[k], -g[k]) ? 1 : 0;
     __syncthreads();
     for (int round = 0; round <= n; ++round) {
         if (lane == 0) {
@@ -89,19 +92,18 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 if (!L.fixed[k]) L.idx[f++] = k;
             L.f = f;
         }
-        if (lane < n) L.delta[lane] = 0.0;
+        for (int k = lane; k < n; k += kLanes) L.delta[k] = 0.0;
         __syncthreads();
         const int f = L.f;
         if (f == 0) break;
         for (int j = 0; j < f; ++j) {
-            const int i = lane;
-            if (i >= j && i < f) {
+            for (int i = j + ((lane - j) % kLanes + kLanes) % kLanes; i < f; i += kLanes) {
                 double v = a[size_t(L.idx[j]) * n + L.idx[i]];
                 if (i == j) v += mu * d[L.idx[j]];
                 m[tri(j, i, f)] = v;
             }
         }
-        if (lane < f) L.rhs[lane] = -g[L.idx[lane]];
+        for (int i = lane; i < f; i += kLanes) L.rhs[i] = -g[L.idx[i]];
         __syncthreads();
         // row and column of every packed entry (the trailing columns' entries are spread over the
         // lanes below)
@@ -109,11 +111,12 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             uint8_t* rowof = reinterpret_cast<uint8_t*>(m + tri_size(n));
             uint8_t* colof = rowof + tri_size(n);
             for (int c = 0; c < f; ++c)
-                if (lane >= c && lane < f) {
-                    const int e = tri(c, lane, f);
-                    rowof[e] = uint8_t(lane);
-                    colof[e] = uint8_t(c);
-                }
+                for (int i = lane; i < f; i += kLanes)
+                    if (i >= c) {
+                        const int e = tri(c, i, f);
+                        rowof[e] = uint8_t(i);
+                        colof[e] = uint8_t(c);
+                    }
         }
         __syncthreads();
         // Cholesky, right-looking: after column j is final, every entry (i, k), j < k <= i,
@@ -125,8 +128,7 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         const uint8_t* colof = rowof + tri_size(n);
         const int e_end = tri(f - 1, f - 1, f) + 1;
         for (int j = 0; j < f; ++j) {
-            const int i = lane;
-            if (i == j) L.diag = m[tri(j, j, f)];
+            if (lane == (j & (kLanes - 1))) L.diag = m[tri(j, j, f)];
             __syncthreads();
             const double diag = L.diag;
             if (!(diag > 0.0) || !isfinite(diag)) {
@@ -134,12 +136,14 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
                 return;
             }
             const double l = __dsqrt_rn(diag);
-            if (i == j) {
-                m[tri(j, j, f)] = l;
-            } else if (i > j && i < f) {
-                const double v = __ddiv_rn(m[tri(j, i, f)], l);
-                m[tri(j, i, f)] = v;
-                L.col[i] = v;
+            for (int i = lane; i < f; i += kLanes) {
+                if (i == j) {
+                    m[tri(j, j, f)] = l;
+                } else if (i > j) {
+                    const double v = __ddiv_rn(m[tri(j, i, f)], l);
+                    m[tri(j, i, f)] = v;
+                    L.col[i] = v;
+                }
             }
             __syncthreads();
             if (j + 1 < f) {
@@ -148,40 +152,69 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
             }
             __syncthreads();
         }
-        __syncthreads();
-        // forward substitution: lane i subtracts l(i, k) y_k for k = 0, 1, ... as y_k is final;
-        // y_k = (lane k's sum) / l(k, k) is formed in every lane alike, so no LDS round trip
+        // forward substitution: row i subtracts l(i, k) y_k for k = 0, 1, ... as y_k is final;
+        // y_k = (row k's sum) / l(k, k) is formed in every lane alike, so no LDS round trip
         {
-            double acc = lane < f ? L.rhs[lane] : 0.0;
-            for (int k = 0; k < f; ++k) {
-                const double yk = __ddiv_rn(lane_value(acc, k), m[tri(k, k, f)]);
-                if (lane == k) acc = yk;
-                else if (lane > k && lane < f) acc -= m[tri(k, lane, f)] * yk;
+            double acc[kSlots];
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const int i = lane + s * kLanes;
+                acc[s] = i < f ? L.rhs[i] : 0.0;
             }
-            if (lane < f) L.rhs[lane] = acc;
+            for (int k = 0; k < f; ++k) {
+                const double yk = __ddiv_rn(lane_value(k < kLanes ? acc[0] : acc[1], k % kLanes), m[tri(k, k, f)]);
+#pragma unroll
+                for (int s = 0; s < kSlots; ++s) {
+                    const int i = lane + s * kLanes;
+                    if (i == k) acc[s] = yk;
+                    else if (i > k && i < f) acc[s] -= m[tri(k, i, f)] * yk;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const int i = lane + s * kLanes;
+                if (i < f) L.rhs[i] = acc[s];
+            }
         }
         // back substitution, x_i = (b_i - sum_{k > i} l(k, i) x_k) / l(i, i) for i = f-1 ... 0:
-        // lane k keeps b_k, then x_k, in a register and forms the product l(k, i) x_k; every lane
-        // subtracts the products from b_i in ascending k, as the host does, and divides, so x_i
-        // is the same in all lanes and lane i keeps it (no LDS round trip between rows)
+        // the lane of row k keeps b_k, then x_k, in a register and forms the product l(k, i) x_k;
+        // every lane subtracts the products from b_i in ascending k, as the host does, and
+        // divides, so x_i is the same in all lanes and the lane of row i keeps it (no LDS round
+        // trip between rows)
         {
             __syncthreads();
-            double xk = lane < f ? L.rhs[lane] : 0.0;
-            for (int i = f - 1; i >= 0; --i) {
-                const double p = (lane > i && lane < f) ? m[tri(i, lane, f)] * xk : 0.0;
-                double acc = lane_value(xk, i);
-                for (int k = i + 1; k < f; ++k) acc -= lane_value(p, k);
-                const double xi = __ddiv_rn(acc, m[tri(i, i, f)]);
-                if (lane == i) xk = xi;
+            double xk[kSlots];
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const int k = lane + s * kLanes;
+                xk[s] = k < f ? L.rhs[k] : 0.0;
             }
-            if (lane < f) L.rhs[lane] = xk;
+            for (int i = f - 1; i >= 0; --i) {
+                double p[kSlots];
+#pragma unroll
+                for (int s = 0; s < kSlots; ++s) {
+                    const int k = lane + s * kLanes;
+                    p[s] = (k > i && k < f) ? m[tri(i, k, f)] * xk[s] : 0.0;
+                }
+                double acc = lane_value(i < kLanes ? xk[0] : xk[1], i % kLanes);
+                for (int k = i + 1; k < f; ++k) acc -= lane_value(k < kLanes ? p[0] : p[1], k % kLanes);
+                const double xi = __ddiv_rn(acc, m[tri(i, i, f)]);
+#pragma unroll
+                for (int s = 0; s < kSlots; ++s)
+                    if (lane + s * kLanes == i) xk[s] = xi;
+            }
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const int k = lane + s * kLanes;
+                if (k < f) L.rhs[k] = xk[s];
+            }
         }
         __syncthreads();
-        if (lane < f) L.delta[L.idx[lane]] = L.rhs[lane];
+        for (int i = lane; i < f; i += kLanes) L.delta[L.idx[i]] = L.rhs[i];
         __syncthreads();
         bool newly_fixed = false;
-        if (lane < f) {
-            const int k = L.idx[lane];
+        for (int i = lane; i < f; i += kLanes) {
+            const int k = L.idx[i];
             if (blocks(S, k, x[k], L.delta[k])) {
                 L.fixed[k] = 1;
                 newly_fixed = true;
@@ -192,24 +225,24 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
         if (!changed) break;
     }
     double* trial = S.trial + size_t(run) * n;
-    if (lane < n) {
-        const int k = lane;
+    for (int k = lane; k < n; k += kLanes) {
         const double t = dmin(dmax(x[k] + L.delta[k], S.lower[k]), S.upper[k]);
         trial[k] = t;
         L.step[k] = t - x[k];
         L.ratio[k] = __ddiv_rn(fabs(L.step[k]), scale_of(x[k], S.typical[k]));
     }
     __syncthreads();
-    if (lane < n) {
-        const int q = lane;
+    for (int q = lane; q < n; q += kLanes) {
         double aq = 0.0;
         for (int p = 0; p < n; ++p) aq += a[size_t(q) * n + p] * L.step[p];
         L.aq[q] = aq;
     }
     __syncthreads();
-    // dmax(0, .) per lane first: a NaN ratio drops out as in the host's sequential std::max,
+    // dmax(0, .) per variable first: a NaN ratio drops out as in the host's sequential std::max,
     // so the butterfly only sees ordered values and every lane ends with the same maximum
-    const double nrm = wave_max(dmax(0.0, lane < n ? L.ratio[lane] : 0.0));
+    double r = 0.0;
+    for (int k = lane; k < n; k += kLanes) r = dmax(r, dmax(0.0, L.ratio[k]));
+    const double nrm = wave_max(r);
     if (lane == 0) {
         double lin = 0.0, quad = 0.0;
         for (int q = 0; q < n; ++q) {
@@ -229,28 +262,31 @@ __device__ void solve_step(const State& S, int run, const double* a, const doubl
 __device__ void prepare_step(const State& S, const Options& o, int run, const double* a,
                              const double* g) {
     const int n = S.P;
-    const int k = threadIdx.x;
+    const int lane = threadIdx.x;
     const double* x = S.x + size_t(run) * n;
-    double sk = 1.0, akk = 0.0, term = 0.0;
-    if (k < n) {
-        sk = scale_of(x[k], S.typical[k]);
-        akk = a[size_t(k) * n + k];
-        term = akk * sk * sk;
+    double smax_l = 0.0;
+    for (int k = lane; k < n; k += kLanes) {
+        const double sk = scale_of(x[k], S.typical[k]);
+        smax_l = dmax(smax_l, dmax(0.0, a[size_t(k) * n + k] * sk * sk));
     }
-    const double smax = wave_max(dmax(0.0, term));
+    const double smax = wave_max(smax_l);
     const double merit = S.merit[run];
-    double ratio = 0.0;
-    if (k < n) {
+    double gmax_l = 0.0;
+    for (int k = lane; k < n; k += kLanes) {
+        const double sk = scale_of(x[k], S.typical[k]);
+        const double akk = a[size_t(k) * n + k];
         const double floor = smax > 0.0 ? __ddiv_rn(o.diag_floor * smax, sk * sk)
                                         : __ddiv_rn(1.0, sk * sk);
         S.d[size_t(run) * n + k] = dmax(akk, floor);
+        double ratio = 0.0;
         if (!blocks(S, k, x[k], -g[k])) {
             const double den = __dsqrt_rn(akk * merit);
             if (den > 0.0) ratio = __ddiv_rn(fabs(g[k]), den);
         }
+        gmax_l = dmax(gmax_l, dmax(0.0, ratio));
     }
-    const double gmax = wave_max(dmax(0.0, ratio));
-    if (k == 0) S.gradient_norm[run] = gmax;
+    const double gmax = wave_max(gmax_l);
+    if (lane == 0) S.gradient_norm[run] = gmax;
 }
 
 __global__ __launch_bounds__(256) void init_kernel(State S, Options o) {
@@ -379,9 +415,9 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
         S.a[size_t(run) * n * n + e] = v;
         finite = finite && isfinite(v);
     }
-    if (lane < n) {
-        const double v = g_c[size_t(j) * n + lane];
-        S.g[size_t(run) * n + lane] = v;
+    for (int k = lane; k < n; k += kLanes) {
+        const double v = g_c[size_t(j) * n + k];
+        S.g[size_t(run) * n + k] = v;
         finite = finite && isfinite(v);
     }
     const bool all_finite = __all(finite);

@@ -10,7 +10,7 @@
 
 namespace mko_lmdev {
 
-constexpr int kMaxVariables = 64; // one wavefront per run, one variable per lane
+constexpr int kMaxVariables = 127; // one wavefront per run, up to two variables per lane
 
 // Per-run LM state, structure of arrays over the n runs of a round ([n] unless noted).
 struct State {

@@ -510,9 +510,13 @@ __device__ __forceinline__ RayState<double> inverse_step(V3<double> q, V3<double
 // intersection's sqrt(D) (B - c s = sqrt(D) on the sphere), so the reverse step needs two square
 // roots instead of four. mu = n_s / n_j, imu = n_j / n_s. Formulas: inverse_step above and
 // refracting_step_backward (adjoint_backward.hpp:119-241) specialised to the regular branch.
+// kIndex: also the adjoints of the indices before (n_s) and after the surface, added to nbb and
+// nab (glass variables), with 1 / n_after formed as mu / n_s.
+template <bool kIndex = false>
 __device__ __forceinline__ RayState<double> fused_step_backward(
     V3<double> q, V3<double> d1, double c, double cs, double t, double n_s, double mu, double imu,
-    RayAdj<double>& adj, double& tb, double& cb_out) {
+    RayAdj<double>& adj, double& tb, double& cb_out, double* nbb = nullptr,
+    double* nab = nullptr) {
     // reconstruction
     const V3<double> N{-c * q.x, -c * q.y, 1.0 - c * q.z};
     const double root_k = dot(d1, N);
@@ -579,6 +583,15 @@ __device__ __forceinline__ RayState<double> fused_step_backward(
     in.p.z += -qq;
     in.d.z += -qq * s0;
     cb_out += cb;
+    if constexpr (kIndex) {
+        // mu = n_s / n_after: d(OPL)/d mu through the refraction, then the chain to both indices;
+        // the segment in medium n_s has length s0 + s
+        const double sin2 = 1.0 - cos_i * cos_i;
+        const double mub = dot(d1b, d) - ab * cos_i - kb * 2.0 * mu * sin2;
+        const double rna = rev_div(mu, n_s);
+        *nbb += mub * rna + (s0 + s);
+        *nab += -mub * mu * rna;
+    }
     adj = in;
     return prev;
 }

@@ -313,19 +313,39 @@ def _uneven_glass(i):
     return optics.synthetic_glass(*UNEVEN_GLASSES[i])
 
 
-def uneven_topology(index, seed=5, variants=4, jitter=0.01):
+def _buchdahl_fit(glass, lo_nm=400.0, hi_nm=700.0, nodes=201):
+    """Buchdahl coefficients (n0, nu1, nu2, nu3) of a Sellmeier glass: least squares over
+    `nodes` wavelengths evenly in [lo_nm, hi_nm] (the fitted glass then stands for the synthetic
+    one in a batch whose media carry glass variables)."""
+    B, C = glass
+    lam = np.linspace(lo_nm, hi_nm, nodes)
+    n = np.array([optics.sellmeier_index(B, C, x / 1000.0) for x in lam])
+    w = np.array([batch_mod.buchdahl_omega(float(x)) for x in lam])
+    A = np.stack([np.ones_like(w), w, w * w, w * w * w], axis=1)
+    coeff, *_ = np.linalg.lstsq(A, n, rcond=None)
+    return [float(c) for c in coeff]
+
+
+def uneven_topology(index, seed=5, variants=4, jitter=0.01, surfaces=None, variables=None):
     """One random topology of the uneven workload (seeded by (seed, index)): 8 ... 32 refracting
     spherical surfaces behind the stop forming weak positive elements (a cemented first element
     when the count is odd), 3 ... 5 fields up to 1 deg, 3 ... 7 wavelengths 450 ... 650 nm,
-    10 ... 96 variables (curvatures, then inner thicknesses, then the working distance; at most
-    2 (S - 1)); image plane at the paraxial focus, pupil radius 0.08 f. `variants` copies with
-    the variables jittered by N(0, jitter) (Irwin-Hall), the first unperturbed. Returns
-    (batch, variables)."""
+    10 ... 96 variables: the curvatures, the inner thicknesses and the working distance, then
+    the Buchdahl coefficients (n0, nu1, nu2) of the glass media in surface order, as many as the
+    drawn count asks for (media with glass variables are Buchdahl fits of their glass; the
+    others keep the Sellmeier index). Image plane at the paraxial focus, pupil radius 0.08 f.
+    `variants` copies with the variables jittered by N(0, jitter) (Irwin-Hall; n0 by jitter / 10),
+    the first unperturbed. `surfaces` / `variables` replace the drawn refracting-surface and
+    variable counts. Returns (batch, variables)."""
     g = np.random.default_rng([seed, index])
     n_refr = int(g.integers(8, 33))
     F = int(g.integers(3, 6))
     L = int(g.integers(3, 8))
     P_draw = int(g.integers(10, 97))
+    if surfaces is not None:
+        n_refr = int(surfaces)
+    if variables is not None:
+        P_draw = int(variables)
     # media after each refracting surface: alternate glass / air, the first element cemented
     # (two glasses) when n_refr is odd, so that the last medium is air
     media = []
@@ -372,22 +392,39 @@ def uneven_topology(index, seed=5, variants=4, jitter=0.01):
     theta = np.radians(fields_deg)
     b.field_angle[:] = theta
     b.field_target[:] = f * np.tan(theta)
-    kinds = ([(abi.VAR_CURVATURE, s) for s in range(1, n_refr + 1)] +
-             [(abi.VAR_THICKNESS, s) for s in range(1, n_refr)] +
-             [(abi.VAR_WORKING_DISTANCE, n_refr)])
+    kinds = ([(abi.VAR_CURVATURE, s, 0) for s in range(1, n_refr + 1)] +
+             [(abi.VAR_THICKNESS, s, 0) for s in range(1, n_refr)] +
+             [(abi.VAR_WORKING_DISTANCE, n_refr, 0)])
+    glass_media = [s for s in range(1, n_refr + 1) if glasses[s] is not None]
+    for s in glass_media:
+        kinds += [(abi.VAR_GLASS, s, comp) for comp in range(3)]
     kinds = kinds[:min(P_draw, len(kinds))]
+    fitted = sorted({s for k, s, _ in kinds if k == abi.VAR_GLASS})
+    if fitted:
+        b.buchdahl = np.zeros((variants, b.surfaces, 4))
+        for s in fitted:
+            b.buchdahl[:, s, :] = _buchdahl_fit(glasses[s])
     inf = math.inf
+    bounds = {abi.VAR_CURVATURE: (-0.2, 0.2, 0.01), abi.VAR_THICKNESS: (0.05, 1000.0, 1.0),
+              abi.VAR_WORKING_DISTANCE: (0.05, 1000.0, 1.0)}
+    names = {abi.VAR_CURVATURE: "c[%d]", abi.VAR_THICKNESS: "t[%d]",
+             abi.VAR_WORKING_DISTANCE: "t[%d]"}
     variables = VariableSet(
-        type=np.array([k for k, _ in kinds], dtype=np.int32),
-        batch_surface=np.array([s for _, s in kinds], dtype=np.int32),
-        lower=np.array([-0.2 if k == abi.VAR_CURVATURE else 0.05 for k, _ in kinds]),
-        upper=np.array([0.2 if k == abi.VAR_CURVATURE else 1000.0 for k, _ in kinds]),
-        typical=np.array([0.01 if k == abi.VAR_CURVATURE else 1.0 for k, _ in kinds]),
-        names=[("c[%d]" if k == abi.VAR_CURVATURE else "t[%d]") % s for k, s in kinds])
+        type=np.array([k for k, _, _ in kinds], dtype=np.int32),
+        batch_surface=np.array([s for _, s, _ in kinds], dtype=np.int32),
+        lower=np.array([-inf if k == abi.VAR_GLASS else bounds[k][0] for k, _, _ in kinds]),
+        upper=np.array([inf if k == abi.VAR_GLASS else bounds[k][1] for k, _, _ in kinds]),
+        typical=np.array([batch_mod.GLASS_TYPICAL[c] if k == abi.VAR_GLASS else bounds[k][2]
+                          for k, _, c in kinds]),
+        names=[("%s[%d]" % (("n0", "nu1", "nu2")[c], s)) if k == abi.VAR_GLASS
+               else names[k] % s for k, s, c in kinds],
+        component=(np.array([c for _, _, c in kinds], dtype=np.int32) if fitted else None))
     x0 = variables.extract(b, 0)
+    if fitted:
+        variables.apply(b, np.tile(x0, (variants, 1)))  # the fitted media's index tables
     xs = rng.perturbed_vectors(variables, x0, variants, seed=seed * 1000003 + index,
-                               relative=(jitter, jitter, jitter), distribution="irwin_hall",
-                               first_is_base=True)
+                               relative=(jitter, jitter, jitter, 0.1 * jitter),
+                               distribution="irwin_hall", first_is_base=True)
     variables.apply(b, xs)
     return b, variables
 
@@ -428,8 +465,8 @@ def point_singlet(point=(0.0, 2.0, -300.0), na=0.05, variants=4, seed=7, image_d
 def long_relay(variants=2, seed=3, fields_deg=(0.0, 0.5), pupil_radius=1.0):
     """64 surfaces (the most a batch holds): the stop and 63 weak spheres, alternately into a
     glass of constant index 1.5 and into air (surfaces 1 ... 62), the last one in air; lens and
-    air gaps 1 mm, image plane 50 mm behind. 64 variables (the most the fused normal equations
-    and the device LM control take): curvatures of surfaces 1 ... 32 and thicknesses 0 ... 31.
+    air gaps 1 mm, image plane 50 mm behind. 64 variables: curvatures of surfaces 1 ... 32 and
+    thicknesses 0 ... 31.
     Per variant the curvatures are jittered by 5 % (seeded)."""
     S = abi.MAX_SURFACES
     types = [abi.SURFACE_STOP] + [abi.SURFACE_SPHERE] * (S - 1)

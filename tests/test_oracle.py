@@ -498,13 +498,16 @@ def test_df64_oracle_matches_reference_bitwise(op):
 
 def test_uneven_topologies_cover_the_ranges_and_trace_cleanly():
     """The uneven workload's random topologies: 8 ... 32 refracting surfaces, 3 ... 5 fields,
-    3 ... 7 wavelengths, 10 ... 2 (S - 1) variables; at the seeded start points no ray fails
-    (oracle, hard mode) and the merit is finite."""
+    3 ... 7 wavelengths, 10 ... 96 variables (glass variables past the curvatures, thicknesses
+    and working distance); at the seeded start points no ray fails (oracle, hard mode) and the
+    merit is finite. A 32-surface topology takes all 96 variables."""
     seen = set()
     for i in range(12):
         b, v = systems.uneven_topology(i, variants=2)
         assert 9 <= b.surfaces <= 33 and 3 <= b.fields <= 5 and 3 <= b.wavelengths <= 7
-        assert 10 <= v.size <= min(96, 2 * (b.surfaces - 1))
+        assert 10 <= v.size <= 96
+        if v.size > 2 * (b.surfaces - 1):
+            assert v.has_glass
         s = systems.uneven_settings()
         _, _, st = oracle().residuals(b, s, threads=8)
         assert (st == 0).all()
@@ -512,6 +515,10 @@ def test_uneven_topologies_cover_the_ranges_and_trace_cleanly():
         assert np.all(np.isfinite(m)) and np.all(m > 0)
         seen.add((b.surfaces, b.fields, b.wavelengths, v.size))
     assert len(seen) == 12  # distinct topologies
+    b, v = systems.uneven_topology(0, variants=2, surfaces=32, variables=96)
+    assert b.surfaces == 33 and v.size == 96 and v.has_glass
+    _, _, st = oracle().residuals(b, systems.uneven_settings(), threads=8)
+    assert (st == 0).all()
 
 
 LIMIT_WORKLOADS = {
