@@ -106,6 +106,20 @@ int mko_adjoint_normal_device(const mko_batch* batch, const mko_residual_setting
                               const mko_variables* vars, double* a, double* g, void* workspace,
                               size_t workspace_bytes, void* stream);
 
+/* Vector-Jacobian product of the ADJOINT Jacobian without forming J: out [V][P] = sum over the
+ * trace rows of grad[v][row] * J[v][p][row], J the FP64 adjoint Jacobian of
+ * mko_adjoint_jacobian_device (piston-projected ray rows; penalty rows in soft mode) of a batch
+ * with one variant per run. grad [V][grad_stride] (grad_stride >= trace rows) and out in device
+ * memory. Each ray's reverse sweep is seeded with its cotangents (the sweep of
+ * ray_gradient_rows, include/mikroptim/merit/ray_jacobian.hpp:93, is linear in its seeds); sums
+ * in fixed order. FP64 trace; settings with constraint rows are rejected. Workspace:
+ * mko_vjp_workspace() bytes. The reference's bindings form J for this product
+ * (python/src/bind_trace.cpp:623); this entry point serves the torch autograd backward. */
+size_t mko_vjp_workspace(int variants, int fields, int wavelengths, int variables);
+int mko_vjp_device(const mko_batch* batch, const mko_residual_settings* settings,
+                   const mko_variables* vars, const double* grad, int64_t grad_stride,
+                   double* out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Normal equations of each run: a [runs][P][P] = J^T J (a[q*P + p]), g [runs][P] = J^T r0.
  * Every element is one FMA chain over the rows in ascending order (bitwise equal to
  * normal_equations_host). Replaces gram_kernel + DeviceNormalEquations::compute

@@ -1128,12 +1128,11 @@ struct RowSink {
 };
 
 // backward_ray (adjoint_backward.hpp:496-580) over the record, reverse arithmetic in RB.
-template <class RB, class RR, class TJ>
+template <class RB, class RR, class Sink>
 __device__ __forceinline__ void backward_ray(const TraceParams& P, const SurfLds& L,
                                              const ParamMap& map, const Record<RR>& rec,
                                              bool reached_image, V3<double> M, double kappa,
-                                             RB seed_opl, RB seed_pen, const RowSink<TJ>& sink,
-                                             int l) {
+                                             RB seed_opl, RB seed_pen, const Sink& sink, int l) {
     const int S = P.S;
     const int count = rec.count;
     mkod::RayAdj<RB> adj{{RB(0), RB(0), RB(0)}, {RB(0), RB(0), RB(0)}};
@@ -1225,11 +1224,11 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
         if (reached) {
             sw += s.weight;
             swo += s.weight * r.opl;
-            backward_ray<RB, RR, TJ>(P, L, map, rec, true, M, kappa, RB(1), RB(0),
+            backward_ray<RB, RR>(P, L, map, rec, true, M, kappa, RB(1), RB(0),
                                      RowSink<TJ>{col0, R, row, 1.0}, t.l);
         }
         if (kSoft && soft.first != MKO_STATUS_OK) {
-            backward_ray<RB, RR, TJ>(P, L, map, rec, reached, M, kappa, RB(0), RB(1),
+            backward_ray<RB, RR>(P, L, map, rec, reached, M, kappa, RB(0), RB(1),
                                      RowSink<TJ>{col0, R, P.rays + row, t.pscale * s.rootw}, t.l);
         }
     }
@@ -1275,6 +1274,92 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
             col[k] = ok ? TJ(t.scale * s.rootw * (double(col[k]) - mean)) : TJ(0);
         }
     }
+}
+
+// ---- vector-Jacobian product J^T v without J ---------------------------------------------
+
+// Sink of the VJP sweep: lane `lane` adds each column's value into its own slot of acc [P][64]
+// (a column is written once per ray and sweep; the lane's rays follow one another).
+struct AccSink {
+    double* acc;
+    int lane;
+    __device__ __forceinline__ void write(int p, double value) const {
+        if (p >= 0) acc[p * 64 + lane] += value;
+    }
+};
+
+// J^T v of one (variant = run, tuple), one wave per tuple, the tuple's rays 64 at a time. The
+// reverse sweep is linear in its seeds (ray_gradient_rows, ray_jacobian.hpp:93), so each ray
+// runs it once with
+//   seed_opl = s (v_k sqrt(W_k) - W_k m),  m = sum_j v_j sqrt(W_j) / W_ok (rays in the piston),
+//   seed_pen = pscale sqrt(W_k) u_k      (soft mode, rays with a failure),
+// v and u being the cotangents of the ray's residual and penalty rows: sum_k v_k J(k, p) with
+// J(k, p) = s sqrt(W_k) (g_kp - gbar_p) equals sum_k g_kp s (v_k sqrt(W_k) - W_k m), so neither
+// J nor the piston-projected columns are formed. A first pass traces the tuple for W_ok and the
+// sum behind m. part[tuple][p]: the tuple's share, lanes summed in a fixed tree.
+template <bool kSoft>
+__global__ __launch_bounds__(64) void vjp_kernel(TraceParams P, ParamMap map, const double* grad,
+                                                 int64_t gstride, double* part) {
+    extern __shared__ double acc[]; // [P][64]
+    __shared__ SurfLds L;
+    const Tuple t = tuple_of(P, blockIdx.x);
+    stage_surfaces(P, t, L);
+    const int lane = threadIdx.x;
+    for (int p = 0; p < map.P; ++p) acc[p * 64 + lane] = 0.0;
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const double a = P.pupil_radius[t.v];
+    double sin_t, cos_t;
+    sincos(P.field_angle[vf], &sin_t, &cos_t);
+    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+    const double* gv = grad + size_t(t.v) * size_t(gstride);
+    double sw = 0.0, svw = 0.0;
+    for (int k = lane; k < t.n; k += 64) {
+        const Sample smp = sample_of(P, t, k);
+        Ray r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+        Soft soft{0.0, MKO_STATUS_OK};
+        NoRecord nr;
+        const uint8_t st = trace_ray<kSoft>(P, L, r, M, kappa, soft, nr);
+        if (mkod::in_piston(st)) {
+            sw += smp.weight;
+            svw += gv[t.offset + k] * smp.rootw;
+        }
+    }
+    const double W = wave_sum(sw), SV = wave_sum(svw);
+    const bool valid = W > 0.0;
+    const double m = valid ? SV / W : 0.0;
+    Record<double> rec;
+    for (int k = lane; k < t.n; k += 64) {
+        const Sample smp = sample_of(P, t, k);
+        Ray r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+        Soft soft{0.0, MKO_STATUS_OK};
+        rec.count = 0;
+        rec.image_branch = 0;
+        const uint8_t st = trace_ray<kSoft>(P, L, r, M, kappa, soft, rec);
+        const bool reached = mkod::in_piston(st);
+        const double beta =
+            valid && reached ? t.scale * (gv[t.offset + k] * smp.rootw - smp.weight * m) : 0.0;
+        double gamma = 0.0;
+        if (kSoft && soft.first != MKO_STATUS_OK)
+            gamma = t.pscale * smp.rootw * gv[P.rays + t.offset + k];
+        if (beta != 0.0 || gamma != 0.0)
+            backward_ray<double, double>(P, L, map, rec, reached, M, kappa, beta, gamma,
+                                         AccSink{acc, lane}, t.l);
+    }
+    for (int p = 0; p < map.P; ++p) {
+        const double total = wave_sum(acc[p * 64 + lane]);
+        if (lane == 0) part[size_t(blockIdx.x) * map.P + p] = total;
+    }
+}
+
+// out[v][p] = sum over the run's tuples of part, in (field, wavelength) order.
+__global__ void vjp_finalize_kernel(int V, int T, int Pn, const double* part, double* out) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= int64_t(V) * Pn) return;
+    const int64_t v = i / Pn, p = i - v * Pn;
+    double x = 0.0;
+    for (int t = 0; t < T; ++t) x += part[(v * T + t) * Pn + p];
+    out[i] = x;
 }
 
 // ---- ADJOINT fused with the normal equations (hard failure mode) ---------------------------
@@ -2661,6 +2746,43 @@ int mko_df64_ops_device(int op, int64_t n, const float* a, const float* b, float
         hipLaunchKernelGGL(df64_ops_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
                            static_cast<hipStream_t>(stream), op, n, a, b, out);
         check_hip(hipGetLastError(), "df64 ops launch");
+    })
+}
+
+size_t mko_vjp_workspace(int variants, int fields, int wavelengths, int variables) {
+    return ((size_t(variants) * size_t(fields) * size_t(wavelengths) * size_t(variables) *
+             sizeof(double)) + 255) & ~size_t(255);
+}
+
+int mko_vjp_device(const mko_batch* batch, const mko_residual_settings* settings,
+                   const mko_variables* vars, const double* grad, int64_t grad_stride,
+                   double* out, void* workspace, size_t workspace_bytes, void* stream) {
+    MKO_TRY({
+        no_rows(*settings);
+        if (settings->precision != MKO_PRECISION_FP64)
+            throw std::invalid_argument("the VJP traces in FP64 (precision MKO_PRECISION_FP64)");
+        const TraceParams P = make_params(*batch, *settings);
+        const ParamMap m = make_map(*vars, *batch, true);
+        if (grad_stride < P.trace_rows) throw std::invalid_argument("grad_stride smaller than trace rows");
+        if (P.V == 0) return 0;
+        if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large");
+        if (workspace_bytes < mko_vjp_workspace(P.V, P.F, P.L, m.P))
+            throw std::invalid_argument("workspace too small");
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        double* part = static_cast<double*>(workspace);
+        const dim3 grid(unsigned(P.V) * unsigned(P.T));
+        const size_t lds = size_t(m.P) * 64 * sizeof(double);
+        if (P.soft)
+            hipLaunchKernelGGL(vjp_kernel<true>, grid, dim3(64), lds, s, P, m, grad, grad_stride,
+                               part);
+        else
+            hipLaunchKernelGGL(vjp_kernel<false>, grid, dim3(64), lds, s, P, m, grad, grad_stride,
+                               part);
+        check_hip(hipGetLastError(), "vjp launch");
+        const int64_t n = int64_t(P.V) * m.P;
+        hipLaunchKernelGGL(vjp_finalize_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s,
+                           P.V, P.T, m.P, part, out);
+        check_hip(hipGetLastError(), "vjp finalize launch");
     })
 }
 

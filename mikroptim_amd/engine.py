@@ -25,7 +25,8 @@ EXPORTS = [
     "mko_last_error", "mko_residuals_device", "mko_merit_device", "mko_merit_device_noalloc",
     "mko_point_batch_device",
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
-    "mko_adjoint_normal_device", "mko_gram_device", "mko_df64_ops_device",
+    "mko_adjoint_normal_device", "mko_vjp_workspace", "mko_vjp_device", "mko_gram_device",
+    "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
@@ -71,6 +72,11 @@ def library():
         "mko_adjoint_normal_workspace": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
         "mko_adjoint_normal_device": ([bp, sp, vp, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_size_t, C.c_void_p], C.c_int),
+        "mko_vjp_workspace": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
+        "mko_vjp_device": ([bp, sp, vp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_size_t,
+                            C.c_void_p], C.c_int),
+        "mko_point_batch_device": ([bp, C.c_int, C.c_int, C.c_int, vp, C.c_void_p, C.c_void_p,
+                                    C.c_void_p] + [C.c_void_p] * 8 + [C.c_void_p], C.c_int),
         "mko_df64_ops_device": ([C.c_int, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p], C.c_int),
         "mko_engine_create": ([C.c_int, C.c_size_t], C.c_void_p),
