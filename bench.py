@@ -397,6 +397,29 @@ def main():
                                    (4 * 2.0 ** -24 * nJ * nr))),
         "runs": 8}
 
+    # config 4: J^T J and J^T r0 on FP64 MFMA and as split-bf16 x3 products (the fused path),
+    # each against the FP64 FMA chain (bitwise the host's normal_equations_host) at the same 8
+    # final points; and the two Gram kernels' device time on the stored FP64 Jacobians of 256 runs
+    _, _, a64m, g64m = eng.jacobian(base, settings, variables, xs, method="adjoint",
+                                    precision="fp64_mfma", want_jacobian=False)
+    normal_eq = {
+        "fp64_mfma": {
+            "A_rel_frobenius": float(np.max(np.linalg.norm(a64m - ad, axis=(1, 2)) /
+                                            np.linalg.norm(ad, axis=(1, 2)))),
+            "g_rel": float(np.max(np.linalg.norm(g64m - gd, axis=1) / np.linalg.norm(gd, axis=1)))},
+        "split_bf16x3": {"A_rel_frobenius": mixed_err["A_rel_frobenius"],
+                         "g_rel": mixed_err["g_rel"]},
+        "reference": "FP64 FMA chain (normal_equations_host order), runs 8"}
+    xg = res_adj.x[:256]
+    for name, prec in (("chain_fp64", "fp64"), ("mfma_fp64", "fp64_mfma")):
+        eng.kernel_timing(True)
+        eng.jacobian(base, settings, variables, xg, method="adjoint", precision=prec,
+                     want_jacobian=False)
+        ms, n = eng.kernel_timing(False)["gram"]
+        flop = len(xg) * rays * (P + 1) * (P + 2)
+        normal_eq[name + "_ms_256_runs"] = ms
+        normal_eq[name + "_tflops"] = flop / (ms * 1e-3) / 1e12 if ms > 0 else None
+
     uneven = None if args.no_uneven else uneven_groups(eng, threads=args.threads)
 
     # per-cycle progress of the two LMs on the same starts: cycles and device seconds each needs
@@ -434,6 +457,7 @@ def main():
         # cycles of the adjoint + mixed LM session that ran as replays of its captured hipGraph
         "lm_graph_cycles_adjoint_mixed": graph_adj,
         "mixed_normal_equations_error_vs_fp64": mixed_err,
+        "normal_equations_fp64_mfma_vs_bf16x3": normal_eq,
         "merit_trace_rays_per_s": trace_rate,
         "constraint_rows": rows,
         "uneven_topologies": uneven,

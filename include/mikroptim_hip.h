@@ -127,6 +127,14 @@ int mko_vjp_device(const mko_batch* batch, const mko_residual_settings* settings
 int mko_gram_device(const double* jacobian, const double* r0, int runs, int variables,
                     int64_t rows, double* a, double* g, void* stream);
 
+/* The same normal equations on FP64 MFMA (v_mfma_f64_16x16x4_f64, 16 x 16 tiles, FP64
+ * accumulation): equal to mko_gram_device up to rounding (another summation order, so not
+ * bitwise the host chain); up to 127 variables. Replaces gram_kernel
+ * (src/cuda/normal_equations.cu:48) where bitwise parity with normal_equations_host is not
+ * required (MKO_PRECISION_FP64_MFMA in the host layer). */
+int mko_gram_mfma_device(const double* jacobian, const double* r0, int runs, int variables,
+                         int64_t rows, double* a, double* g, void* stream);
+
 /* df64 (double-float) arithmetic check: out[i] = a[i] op b[i] over n pairs (hi, lo) of floats in
  * device memory ([n][2] each); op 0 add, 1 sub, 2 mul, 3 mul by the float b.hi, 4 div, 5 sqrt(a).
  * The operations of the MKO_PRECISION_DF64 trace, for comparison with a host implementation

@@ -60,7 +60,11 @@ enum {
 /* Arithmetic of the device path. FP64: every kernel in double. MIXED: forward trace (OPL,
  * residuals, merit) in double, reverse sweep of the adjoint in float, JtJ on bf16x3 MFMA with
  * float accumulation, Jtr and the LM step in double. */
-enum { MKO_PRECISION_FP64 = 0, MKO_PRECISION_MIXED = 1, MKO_PRECISION_DF64 = 2 };
+enum { MKO_PRECISION_FP64 = 0, MKO_PRECISION_MIXED = 1, MKO_PRECISION_DF64 = 2,
+       /* Jacobian precision of the host layer: FP64 Jacobian with its normal equations on FP64
+        * MFMA (mko_gram_mfma_device) instead of the host's FMA chain (no counterpart in the
+        * reference's Precision) */
+       MKO_PRECISION_FP64_MFMA = 3 };
 
 /* One batch of V variants of one topology. `type`, `pupil_mode`, `wavelength_weight` and
  * `field_weight` are batch-wide and always live in host memory; the per-variant arrays are host

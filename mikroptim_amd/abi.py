@@ -18,7 +18,7 @@ STENCIL_CENTRAL, STENCIL_ONE_SIDED, STENCIL_TRUNCATED = 0, 1, 2
 JACOBIAN_FD, JACOBIAN_ADJOINT = 0, 1
 (LM_RUNNING, LM_CONVERGED_MERIT, LM_CONVERGED_STEP, LM_CONVERGED_GRADIENT, LM_CONVERGED_STALL,
  LM_MAX_ITERATIONS, LM_DAMPING_LIMIT, LM_NUMERICAL_ERROR) = range(8)
-PRECISION_FP64, PRECISION_MIXED, PRECISION_DF64 = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED, PRECISION_DF64, PRECISION_FP64_MFMA = 0, 1, 2, 3
 # ConstraintType / ConstraintMode / EdgeHeight (include/mikroptim/model/constraints.hpp:28,76,69)
 (CON_FOCAL_LENGTH, CON_PARFOCAL_LENGTH, CON_WORKING_DISTANCE, CON_PARAXIAL_FOCUS,
  CON_PETZVAL) = range(5)

@@ -796,7 +796,8 @@ public:
         if (vars.count < 1) throw std::invalid_argument("no variables");
         if (method != MKO_JACOBIAN_FD && method != MKO_JACOBIAN_ADJOINT)
             throw std::invalid_argument("unknown Jacobian method");
-        if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED)
+        if (precision != MKO_PRECISION_FP64 && precision != MKO_PRECISION_MIXED &&
+            precision != MKO_PRECISION_FP64_MFMA)
             throw std::invalid_argument("unknown precision");
         validate_values(base, 0, base.variants);
         if (soft_split_applies(s, vars, method, precision) && !J && !r0 && (a || g)) {
@@ -934,7 +935,7 @@ public:
                 double* da = a_.as<double>(size_t(count) * P * P);
                 double* dg = g_.as<double>(size_t(count) * P);
                 timer.run(kGram, [&] {
-                    dev_check(mko_gram_device(dJ, dr0, count, P, R, da, dg, nullptr));
+                    dev_check(gram(precision, dJ, dr0, count, P, R, da, dg));
                 });
                 if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
                 if (g) copy_out(g + size_t(first) * P, dg, size_t(count) * P * 8);
@@ -1093,7 +1094,7 @@ public:
                             T);
             }
             timer.run(kGram, [&] {
-                dev_check(mko_gram_device(dJ, dr0, c, P, R, a_out, g_out, nullptr));
+                dev_check(gram(precision, dJ, dr0, c, P, R, a_out, g_out));
             });
         }
     }
@@ -1177,6 +1178,15 @@ public:
         };
         part(clean, hs, MKO_PRECISION_MIXED);
         part(other, s, MKO_PRECISION_FP64);
+    }
+
+    // Normal equations of stored Jacobians: the host's FMA chain (bitwise), or FP64 MFMA for
+    // MKO_PRECISION_FP64_MFMA.
+    static int gram(int precision, const double* J, const double* r0, int runs, int P, int64_t R,
+                    double* a, double* g) {
+        return precision == MKO_PRECISION_FP64_MFMA
+                   ? mko_gram_mfma_device(J, r0, runs, P, R, a, g, nullptr)
+                   : mko_gram_device(J, r0, runs, P, R, a, g, nullptr);
     }
 
     // The fused mixed-precision normal equations apply: hard failure mode, at most 96 variables;

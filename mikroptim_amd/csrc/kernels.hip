@@ -2027,6 +2027,82 @@ __global__ __launch_bounds__(256) void gram_kernel(const double* J, const double
     }
 }
 
+// ---- Gram on FP64 MFMA ----------------------------------------------------------------------
+
+typedef __attribute__((ext_vector_type(4))) double f64x4;
+
+// [J r0]^T [J r0] of one run per workgroup with v_mfma_f64_16x16x4_f64: the run's P + 1 columns
+// staged through LDS 32 rows at a time (doubles, row stride 33), the upper 16 x 16 tiles of the
+// (P + 1) x (P + 1) result spread over the 4 waves (kTiles per wave), FP64 accumulation in the
+// MFMA unit. Operand lane l: A(m = l % 16, k = l / 16), B(k = l / 16, n = l % 16); result lane l,
+// element e: row 4 e + l / 16, column l % 16 (measured on MI355X). Not the host's summation
+// order (unlike gram_kernel): the results differ from normal_equations_host by rounding.
+template <int kTiles>
+__global__ __launch_bounds__(256) void gram_mfma_kernel(const double* J, const double* r0, int P,
+                                                        int64_t rows, double* a, double* g) {
+    constexpr int kC = 32, kLd = kC + 1;
+    extern __shared__ double Xs[]; // [16 tg][kLd]
+    const int run = blockIdx.x;
+    const int Q = P + 1;
+    const int tg = (Q + 15) / 16, cols = 16 * tg;
+    const int ntiles = tg * (tg + 1) / 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    f64x4 acc[kTiles];
+    int tI[kTiles], tJ[kTiles];
+#pragma unroll
+    for (int i = 0; i < kTiles; ++i) {
+        acc[i] = f64x4{0.0, 0.0, 0.0, 0.0};
+        const int tt = wave + 4 * i;
+        tI[i] = -1;
+        tJ[i] = -1;
+        if (tt < ntiles) tile_of(tt, tg, tI[i], tJ[i]);
+    }
+    const double* Jr = J + size_t(run) * P * rows;
+    const double* rr = r0 + size_t(run) * rows;
+    const int m = lane & 15, kk = lane >> 4;
+    for (int64_t base = 0; base < rows; base += kC) {
+        for (int e = tid; e < cols * kC; e += 256) {
+            const int c = e / kC, rw = e - c * kC;
+            const int64_t row = base + rw;
+            double v = 0.0;
+            if (row < rows) v = c < P ? Jr[size_t(c) * rows + row] : c == P ? rr[row] : 0.0;
+            Xs[c * kLd + rw] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kTiles; ++i) {
+            if (tI[i] < 0) continue;
+            const double* xa = Xs + (tI[i] * 16 + m) * kLd + kk;
+            const double* xb = Xs + (tJ[i] * 16 + m) * kLd + kk;
+#pragma unroll
+            for (int st = 0; st < kC / 4; ++st)
+                acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[4 * st], xb[4 * st], acc[i], 0, 0,
+                                                              0);
+        }
+        __syncthreads();
+    }
+    double* A = a + size_t(run) * P * P;
+    double* gr = g + size_t(run) * P;
+#pragma unroll
+    for (int i = 0; i < kTiles; ++i) {
+        if (tI[i] < 0) continue;
+        const int col = tJ[i] * 16 + m;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = tI[i] * 16 + 4 * e + kk;
+            const double v = acc[i][e];
+            if (row < P && col < P) {
+                A[size_t(row) * P + col] = v;
+                A[size_t(col) * P + row] = v;
+            } else if (row < P && col == P) {
+                gr[row] = v;
+            } else if (row == P && col < P) {
+                gr[col] = v;
+            }
+        }
+    }
+}
+
 // ---- helpers ---------------------------------------------------------------------------------
 
 void check_hip(hipError_t e, const char* what) {
@@ -2746,6 +2822,25 @@ int mko_df64_ops_device(int op, int64_t n, const float* a, const float* b, float
         hipLaunchKernelGGL(df64_ops_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
                            static_cast<hipStream_t>(stream), op, n, a, b, out);
         check_hip(hipGetLastError(), "df64 ops launch");
+    })
+}
+
+int mko_gram_mfma_device(const double* jacobian, const double* r0, int runs, int variables,
+                         int64_t rows, double* a, double* g, void* stream) {
+    MKO_TRY({
+        if (variables < 1 || variables > 127) throw std::invalid_argument("1 ... 127 variables");
+        if (rows < 0) throw std::invalid_argument("negative row count");
+        if (runs <= 0) return 0;
+        const int Q = variables + 1, tg = (Q + 15) / 16;
+        const size_t lds = size_t(16 * tg) * 33 * sizeof(double);
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (tg <= 4)
+            hipLaunchKernelGGL(gram_mfma_kernel<3>, dim3(unsigned(runs)), dim3(256), lds, s,
+                               jacobian, r0, variables, rows, a, g);
+        else
+            hipLaunchKernelGGL(gram_mfma_kernel<9>, dim3(unsigned(runs)), dim3(256), lds, s,
+                               jacobian, r0, variables, rows, a, g);
+        check_hip(hipGetLastError(), "gram mfma launch");
     })
 }
 

@@ -26,6 +26,7 @@ EXPORTS = [
     "mko_point_batch_device",
     "mko_fd_difference_device", "mko_adjoint_jacobian_device", "mko_adjoint_normal_workspace",
     "mko_adjoint_normal_device", "mko_vjp_workspace", "mko_vjp_device", "mko_gram_device",
+    "mko_gram_mfma_device",
     "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
@@ -72,6 +73,8 @@ def library():
         "mko_adjoint_normal_workspace": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
         "mko_adjoint_normal_device": ([bp, sp, vp, C.c_void_p, C.c_void_p, C.c_void_p,
                                        C.c_size_t, C.c_void_p], C.c_int),
+        "mko_gram_mfma_device": ([C.c_void_p, _dp, C.c_int, C.c_int, C.c_int64, _dp, _dp,
+                                  C.c_void_p], C.c_int),
         "mko_vjp_workspace": ([C.c_int, C.c_int, C.c_int, C.c_int], C.c_size_t),
         "mko_vjp_device": ([bp, sp, vp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_size_t,
                             C.c_void_p], C.c_int),
@@ -120,7 +123,8 @@ def _check(rc):
         raise HipError(library().mko_last_error().decode())
 
 
-_PRECISION = {"fp64": abi.PRECISION_FP64, "mixed": abi.PRECISION_MIXED}
+_PRECISION = {"fp64": abi.PRECISION_FP64, "mixed": abi.PRECISION_MIXED,
+              "fp64_mfma": abi.PRECISION_FP64_MFMA}
 _METHOD = {"fd": abi.JACOBIAN_FD, "adjoint": abi.JACOBIAN_ADJOINT}
 
 
