@@ -158,6 +158,14 @@ void mko_engine_destroy(mko_engine* engine);
  * updates). Returns the totals since the previous call, resets them and sets the enable flag. */
 int mko_engine_kernel_timing(mko_engine* engine, int enable, double* ms, int64_t* launches);
 
+/* Launch shapes of the engine's kernels (mko_tuning): tuple-pack segment of the FP64 RESIDUALS /
+ * MERIT traces and waves per workgroup of the fused normal equations, 0 = automatic; every later
+ * host-layer call on the engine (and its LM sessions) uses them. Packed traces equal unpacked
+ * ones bit for bit. No counterpart in the reference (fixed pack and block shapes,
+ * include/mikroptim/cuda/tuple_packs.hpp:61). */
+int mko_engine_set_tuning(mko_engine* engine, const mko_tuning* tuning);
+int mko_engine_get_tuning(const mko_engine* engine, mko_tuning* tuning);
+
 /* Constraint rows (settings->extra, see mikroptim_types.h): every host-layer entry point below
  * appends them to each variant's residual vector, R = trace_rows + E + E_t rows, as the
  * reference's FdJacobianOptions::extra / trace_extra do: residuals, J and r0 carry R rows, the

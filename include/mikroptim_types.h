@@ -183,6 +183,17 @@ typedef struct mko_variables {
     const int32_t* component;     /* [P] glass variables' component 0 / 1 / 2, or NULL */
 } mko_variables;
 
+/* Launch shapes of an engine (mko_engine_set_tuning; mko_engine_autotune measures them): no
+ * counterpart in the reference, whose tuple-pack sizes and blocks per SM are fixed
+ * (include/mikroptim/cuda/tuple_packs.hpp:61, src/cuda/wavefront_block.hpp:504-526). Results do
+ * not depend on them: packed traces equal unpacked ones bit for bit; the fused normal
+ * equations' waves change only their rounding. */
+typedef struct {
+    int32_t pack_rays;   /* 0: automatic; 32, 64, 128: segment of a tuple pack (tuples of at
+                          * most that many rays share a 256-lane workgroup); 256: no packing */
+    int32_t fused_waves; /* 0: automatic; 1, 2, 4: waves per workgroup of the fused kernel */
+} mko_tuning;
+
 typedef struct mko_lm_options {
     int32_t max_iterations; /* 100 */
     double ftol, xtol, gtol; /* 1e-10 each */

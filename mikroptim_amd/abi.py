@@ -72,6 +72,10 @@ class Variables(C.Structure):
                 ("lower", _dp), ("upper", _dp), ("typical", _dp), ("component", _i32p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("pack_rays", C.c_int32), ("fused_waves", C.c_int32)]
+
+
 class LmOptions(C.Structure):
     _fields_ = [
         ("max_iterations", C.c_int32), ("ftol", C.c_double), ("xtol", C.c_double),

@@ -1200,6 +1200,8 @@ public:
     }
 
     int device() const { return device_; }
+    const mko_internal::Tuning& tuning() const { return tuning_; }
+    void set_tuning(const mko_internal::Tuning& t) { tuning_ = t; }
     KernelTimer timer;
 
 private:
@@ -1333,6 +1335,7 @@ private:
 
     int device_;
     size_t limit_;
+    mko_internal::Tuning tuning_;
     DeviceBatch batch_, tmpl_, work_, rowfd_;
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
     Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
@@ -1510,6 +1513,7 @@ void solve_step(Run& s, const Bounds& b, size_t n, double mu) {
 class Lm {
 public:
     ~Lm() { graph_release(); }
+    Engine& engine() { return eng_; }
     Lm(Engine& eng, const mko_batch& base, const mko_residual_settings& s, const mko_variables& v,
        int runs, const double* x0, const mko_lm_options& o, int precision)
         : eng_(eng), base_(base), settings_(s), vars_(v), runs_(runs), o_(o), precision_(precision) {
@@ -2450,15 +2454,40 @@ int mko_engine_kernel_timing(mko_engine* e, int enable, double* ms, int64_t* lau
     })
 }
 
+int mko_engine_set_tuning(mko_engine* e, const mko_tuning* t) {
+    MKO_HOST_TRY({
+        if (!e || !t) throw std::invalid_argument("engine and tuning needed");
+        if (t->pack_rays != 0 && t->pack_rays != 32 && t->pack_rays != 64 &&
+            t->pack_rays != 128 && t->pack_rays != 256)
+            throw std::invalid_argument("pack_rays must be 0, 32, 64, 128 or 256");
+        if (t->fused_waves != 0 && t->fused_waves != 1 && t->fused_waves != 2 &&
+            t->fused_waves != 4)
+            throw std::invalid_argument("fused_waves must be 0, 1, 2 or 4");
+        e->engine.set_tuning(mko_internal::Tuning{t->pack_rays, t->fused_waves});
+        return 0;
+    })
+}
+
+int mko_engine_get_tuning(const mko_engine* e, mko_tuning* t) {
+    MKO_HOST_TRY({
+        if (!e || !t) throw std::invalid_argument("engine and tuning needed");
+        t->pack_rays = e->engine.tuning().pack_rays;
+        t->fused_waves = e->engine.tuning().fused_waves;
+        return 0;
+    })
+}
+
 int mko_engine_residuals(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                          double* res, double* opd, uint8_t* st) {
-    MKO_HOST_TRY(e->engine.residuals(*b, *s, res, opd, st); return 0;)
+    MKO_HOST_TRY(mko_internal::TuningScope ts(e->engine.tuning());
+                 e->engine.residuals(*b, *s, res, opd, st); return 0;)
 }
 
 int mko_engine_prefilter(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                          const mko_prefilter_options* o, const double* extra,
                          mko_prefilter_result* r) {
-    MKO_HOST_TRY(e->engine.prefilter(*b, *s, *o, extra, *r); return 0;)
+    MKO_HOST_TRY(mko_internal::TuningScope ts(e->engine.tuning());
+                 e->engine.prefilter(*b, *s, *o, extra, *r); return 0;)
 }
 
 int mko_prefilter_decide(const mko_batch* b, const mko_residual_settings* s,
@@ -2476,18 +2505,21 @@ int mko_prefilter_decide(const mko_batch* b, const mko_residual_settings* s,
 
 int mko_engine_surface_extrema(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                                double* extrema) {
-    MKO_HOST_TRY(e->engine.surface_extrema(*b, *s, extrema); return 0;)
+    MKO_HOST_TRY(mko_internal::TuningScope ts(e->engine.tuning());
+                 e->engine.surface_extrema(*b, *s, extrema); return 0;)
 }
 
 int mko_engine_merit(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                      const double* extra, double* merit, int64_t* failed, int64_t* hard) {
-    MKO_HOST_TRY(e->engine.merit(*b, *s, extra, merit, failed, hard); return 0;)
+    MKO_HOST_TRY(mko_internal::TuningScope ts(e->engine.tuning());
+                 e->engine.merit(*b, *s, extra, merit, failed, hard); return 0;)
 }
 
 int mko_engine_jacobian(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
                         const mko_variables* v, int runs, const double* x0, int method,
                         int precision, double* J, double* r0, double* a, double* g) {
-    MKO_HOST_TRY(e->engine.jacobian(*base, *s, *v, runs, x0, method, precision, J, r0, a, g);
+    MKO_HOST_TRY(mko_internal::TuningScope ts(e->engine.tuning());
+                 e->engine.jacobian(*base, *s, *v, runs, x0, method, precision, J, r0, a, g);
                  return 0;)
 }
 
@@ -2495,6 +2527,7 @@ mko_lm* mko_lm_create(mko_engine* e, const mko_batch* base, const mko_residual_s
                       const mko_variables* v, int runs, const double* x0,
                       const mko_lm_options* o, int precision) {
     try {
+        mko_internal::TuningScope ts(e->engine.tuning());
         return new mko_lm{Lm(e->engine, *base, *s, *v, runs, x0, *o, precision)};
     } catch (const std::exception& ex) {
         mko_detail::last_error = ex.what();
@@ -2502,7 +2535,10 @@ mko_lm* mko_lm_create(mko_engine* e, const mko_batch* base, const mko_residual_s
     }
 }
 
-int mko_lm_cycle(mko_lm* lm) { MKO_HOST_TRY(return lm->lm.cycle() ? 1 : 0;) }
+int mko_lm_cycle(mko_lm* lm) {
+    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.engine().tuning());
+                 return lm->lm.cycle() ? 1 : 0;)
+}
 
 int mko_lm_results(mko_lm* lm, double* x, mko_lm_run* out, int* cycles) {
     MKO_HOST_TRY(lm->lm.results(x, out, cycles); return 0;)
@@ -2512,9 +2548,15 @@ void mko_lm_destroy(mko_lm* lm) { delete lm; }
 
 int mko_lm_graph_cycles(const mko_lm* lm) { return lm ? lm->lm.graph_cycles() : 0; }
 
-int mko_lm_cycle_begin(mko_lm* lm) { MKO_HOST_TRY(return lm->lm.cycle_begin();) }
+int mko_lm_cycle_begin(mko_lm* lm) {
+    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.engine().tuning());
+                 return lm->lm.cycle_begin();)
+}
 
-int mko_lm_cycle_end(mko_lm* lm) { MKO_HOST_TRY(return lm->lm.cycle_end() ? 1 : 0;) }
+int mko_lm_cycle_end(mko_lm* lm) {
+    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.engine().tuning());
+                 return lm->lm.cycle_end() ? 1 : 0;)
+}
 
 int mko_engine_normal_equations(mko_engine* e, const mko_batch* base,
                                 const mko_residual_settings* s, const mko_variables* v, int count,
@@ -2524,6 +2566,7 @@ int mko_engine_normal_equations(mko_engine* e, const mko_batch* base,
         if (count < 0) throw std::invalid_argument("negative run count");
         if (count == 0) return 0;
         check_batch(*base);
+        mko_internal::TuningScope ts(e->engine.tuning());
         GatheredBatch gb;
         const mko_batch* b = base;
         if (base->variants != 1) {
@@ -2552,6 +2595,7 @@ int mko_engine_point_merit(mko_engine* e, const mko_batch* base, const mko_resid
                 throw std::invalid_argument("base must have one variant, one per run, or runs given");
         }
         validate_values(*b, 0, b->variants);
+        mko_internal::TuningScope ts(e->engine.tuning());
         e->engine.point_merit(*b, *s, *v, count, x, merit, failed, hard);
         return 0;
     })
@@ -2561,6 +2605,7 @@ int mko_optimize_lm(mko_engine* e, const mko_batch* base, const mko_residual_set
                     const mko_variables* v, int runs, const double* x0, const mko_lm_options* o,
                     int precision, double* x_out, mko_lm_run* run_out, int* cycles_out) {
     MKO_HOST_TRY({
+        mko_internal::TuningScope ts(e->engine.tuning());
         Lm lm(e->engine, *base, *s, *v, runs, x0, *o, precision);
         while (lm.cycle()) {
         }

@@ -12,6 +12,24 @@ namespace mko_internal {
 // position and direction on the last surface and the OPL at the reference sphere (NaN for a ray
 // that failed), structure of arrays: data[(c * slots + slot) * rays + row], c = p.x, p.y, p.z,
 // d.x, d.y, d.z, opl. Variant v of a launch uses slot slot[v] (slot == NULL: slot_offset + v).
+// Launch shapes the engine's autotuner chooses: pack_rays (0: automatic; 32, 64, 128: the
+// segment of a tuple pack, used where the tuples fit it; 256: one tuple per workgroup) and
+// fused_waves (0: automatic; 1, 2, 4: waves per workgroup of the fused normal equations). The
+// kernels of the calling thread use them; the host layer sets them for each of its calls.
+struct Tuning {
+    int pack_rays = 0;
+    int fused_waves = 0;
+};
+void set_thread_tuning(const Tuning& t);
+Tuning thread_tuning();
+struct TuningScope {
+    Tuning saved;
+    explicit TuningScope(const Tuning& t) : saved(thread_tuning()) { set_thread_tuning(t); }
+    ~TuningScope() { set_thread_tuning(saved); }
+    TuningScope(const TuningScope&) = delete;
+    TuningScope& operator=(const TuningScope&) = delete;
+};
+
 struct RayStates {
     double* data;
     const int32_t* slot;

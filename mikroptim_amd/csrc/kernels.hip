@@ -590,6 +590,164 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     }
 }
 
+// ---- tuple packs: consecutive tuples of a variant share a workgroup -------------------------
+
+// Surface tables of the pack's tuples (tuple t0 + j in Ls[j]); as stage_surfaces per tuple.
+__device__ __forceinline__ void stage_surfaces_pack(const TraceParams& P, int v, int t0, int tpb,
+                                                    SurfLds* Ls) {
+    const int n = tpb * P.S;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int j = e / P.S, s = e - j * P.S;
+        const int tt = t0 + j;
+        if (tt >= P.T) continue;
+        const int l = tt - (tt / P.L) * P.L;
+        const size_t vs = size_t(v) * P.S + s;
+        const bool sphere = (P.sphere_mask >> s) & 1ull;
+        Ls[j].c[s] = sphere ? P.curvature[vs] : 0.0;
+        Ls[j].t[s] = P.thickness[vs];
+        Ls[j].n[s] = P.index[vs * P.L + l];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        const int j = e / P.S, s = e - j * P.S;
+        if (t0 + j >= P.T || s < 1) continue;
+        Ls[j].mu[s] = __ddiv_rn(Ls[j].n[s - 1], Ls[j].n[s]);
+        Ls[j].imu[s] = __ddiv_rn(Ls[j].n[s], Ls[j].n[s - 1]);
+    }
+    __syncthreads();
+}
+
+// Sum over a pack segment of `seg` lanes (a power of two, 32 ... 256): for seg >= 64 the wave
+// sums of the segment's waves added in wave order, for seg < 64 the xor butterfly over the
+// segment's lanes (offsets seg / 2 ... 1). The same tree as block_sum of one tuple per workgroup,
+// whose other lanes (and waves) hold zeros: the steps it has beyond the segment add exact zeros,
+// so the sums are equal bit for bit.
+template <class T>
+__device__ __forceinline__ T seg_sum(T v, int seg, T* lds) {
+    if (seg >= 64) {
+        v = wave_sum(v);
+        const int wave = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) lds[wave] = v;
+        __syncthreads();
+        const int w0 = (int(threadIdx.x) / seg) * (seg / 64);
+        T total = lds[w0];
+        for (int w = 1; w < seg / 64; ++w) total += lds[w0 + w];
+        __syncthreads();
+        return total;
+    }
+    for (int off = seg / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// RESIDUALS (kMode 0) / MERIT partials (kMode 1) of tuples of at most `seg` rays (seg 32, 64 or
+// 128), 256 / seg consecutive tuples of one variant per workgroup (make_tuple_packs,
+// include/mikroptim/cuda/tuple_packs.hpp:61, with a pack size fixed per launch): tuple t0 + j
+// on lanes [j seg, (j + 1) seg), one ray per lane, its surface table in Ls[j]. Every value is
+// the one trace_kernel computes for the tuple alone (FP64 trace, no extrema).
+template <int kMode, bool kSoft, bool kStates>
+__global__ __launch_bounds__(kBlock) void trace_pack_kernel(TraceParams P, int seg,
+                                                            double* residuals, double* opd,
+                                                            uint8_t* status, double* partial,
+                                                            int* pfailed, int* phard,
+                                                            mko_internal::RayStates states) {
+    extern __shared__ SurfLds Ls[];
+    __shared__ double red[kWaves];
+    __shared__ int ired[kWaves];
+    const int tpb = kBlock / seg;
+    const int bpv = (P.T + tpb - 1) / tpb;
+    const int v = int(blockIdx.x) / bpv;
+    const int t0 = (int(blockIdx.x) - v * bpv) * tpb;
+    if (P.live && v >= *P.live) return;
+    stage_surfaces_pack(P, v, t0, tpb, Ls);
+    const int j = int(threadIdx.x) / seg, k = int(threadIdx.x) - j * seg;
+    const int tt = t0 + j;
+    const bool in_pack = tt < P.T;
+    const SurfLds& L = Ls[j];
+    const Tuple t = tuple_of(P, v * P.T + (in_pack ? tt : 0));
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const double a = P.pupil_radius[t.v];
+    double sin_t, cos_t;
+    sincos(P.field_angle[vf], &sin_t, &cos_t);
+    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+    const bool have = in_pack && k < t.n;
+    double opl = 0.0, pen = 0.0, sw = 0.0, swo = 0.0;
+    uint8_t st = MKO_STATUS_UNSUPPORTED;
+    Sample smp{};
+    if (have) {
+        smp = sample_of(P, t, k);
+        mkod::RayT<double> r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+        Soft soft{0.0, MKO_STATUS_OK};
+        if constexpr (kStates) {
+            LastState last;
+            last.count = 0;
+            last.image_branch = 0;
+            st = trace_ray<kSoft, kTraceFastMath>(P, L, r, M, kappa, soft, last);
+            const int64_t row = t.offset + k;
+            const bool okr = st == MKO_STATUS_OK;
+            if (okr) {
+                states.data[state_index(states, t.v, 0, row)] = last.p.x;
+                states.data[state_index(states, t.v, 1, row)] = last.p.y;
+                states.data[state_index(states, t.v, 2, row)] = last.p.z;
+                states.data[state_index(states, t.v, 3, row)] = last.d.x;
+                states.data[state_index(states, t.v, 4, row)] = last.d.y;
+                states.data[state_index(states, t.v, 5, row)] = last.d.z;
+            }
+            states.data[state_index(states, t.v, 6, row)] = okr ? r.opl : __builtin_nan("");
+        } else {
+            NoRecord nr;
+            st = trace_ray<kSoft, kTraceFastMath>(P, L, r, M, kappa, soft, nr);
+        }
+        opl = r.opl;
+        pen = soft.penalty;
+        if (mkod::in_piston(st)) {
+            sw += smp.weight;
+            swo += smp.weight * opl;
+        }
+    }
+    const double w_ok = seg_sum(sw, seg, red);
+    const double sum = seg_sum(swo, seg, red);
+    const bool valid = w_ok > 0.0;
+    const double piston = valid ? sum / w_ok : 0.0;
+    double r2 = 0.0, p2 = 0.0;
+    int nfail = 0, nhard = 0;
+    if (have) {
+        const bool ok = valid && mkod::in_piston(st);
+        const double e = ok ? opl - piston : 0.0;
+        const double r = t.scale * smp.rootw * e;
+        double pr = 0.0;
+        if (kSoft) {
+            const double fp = st == MKO_STATUS_OK ? 0.0 : mkod::is_soft(st) ? pen : pen + 1.0;
+            pr = t.pscale * smp.rootw * fp;
+        }
+        const int64_t row = t.offset + k;
+        if (kMode == 0) {
+            const size_t base = size_t(t.v) * P.stride;
+            residuals[base + row] = r;
+            if (kSoft) residuals[base + P.rays + row] = pr;
+            if (opd) opd[size_t(t.v) * P.rays + row] = e;
+            if (status) status[size_t(t.v) * P.rays + row] = st;
+        } else {
+            r2 = r * r;
+            p2 = pr * pr;
+            nfail = st != MKO_STATUS_OK;
+            nhard = !mkod::in_piston(st);
+        }
+    }
+    if (kMode == 1) {
+        const double tr2 = seg_sum(r2, seg, red);
+        const double tp2 = kSoft ? seg_sum(p2, seg, red) : 0.0;
+        const int tf = seg_sum(nfail, seg, ired);
+        const int th = seg_sum(nhard, seg, ired);
+        if (k == 0 && in_pack) {
+            const size_t idx = size_t(v) * P.T + size_t(tt);
+            partial[idx] = kSoft ? tr2 + tp2 : tr2;
+            pfailed[idx] = tf;
+            phard[idx] = th;
+        }
+    }
+}
+
 // Extrema of every variant from the tuple partials: out[v][s] = {sqrt(max height^2), min cos I}
 // over the variant's tuples (extrema_finalize_kernel of the reference, wavefront_block.hpp:414).
 __global__ void extrema_finalize_kernel(int V, int T, int S, const double* partial, double* out) {
@@ -2189,6 +2347,26 @@ TraceParams make_params(const mko_batch& b, const mko_residual_settings& s) {
     return P;
 }
 
+// Launch shapes of the calling thread (set by the host layer for its calls; the device-layer
+// entry points run with the defaults).
+thread_local mko_internal::Tuning g_tuning{};
+
+// Segment of a tuple pack for this launch (0: one tuple per workgroup): tuples of at most 128
+// rays (FP64 traces without extrema) share a workgroup in segments of 32, 64 or 128 lanes, the
+// smallest that holds the largest tuple, or the tuning's pack_rays where the tuples fit it
+// (pack_rays 256 turns packing off).
+int pack_segment(const TraceParams& P, bool packable) {
+    if (!packable || P.precision != MKO_PRECISION_FP64) return 0;
+    int nmax = 0;
+    for (int f = 0; f < P.F; ++f)
+        nmax = std::max(nmax, P.rings * (P.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : P.spokes));
+    const int want = g_tuning.pack_rays;
+    if (want >= kBlock || nmax > 128 || P.T < 2) return 0;
+    int seg = nmax <= 32 ? 32 : nmax <= 64 ? 64 : 128;
+    if (want == 32 || want == 64 || want == 128) seg = std::max(seg, want);
+    return seg;
+}
+
 int rays_per_lane(const TraceParams& P) {
     int nmax = 0;
     for (int f = 0; f < P.F; ++f)
@@ -2242,6 +2420,32 @@ void launch_trace(int mode, const TraceParams& P, double* res, double* opd, uint
                   const mko_internal::RayStates* states = nullptr) {
     if (int64_t(P.V) * P.T > 0x7fffffffLL) throw std::invalid_argument("batch too large for one launch");
     const int rpl = rays_per_lane(P);
+    const int seg = pack_segment(P, ext == nullptr);
+    if (seg > 0) {
+        if (states && (mode != 1 || P.precision != MKO_PRECISION_FP64))
+            throw std::invalid_argument("ray states need the MERIT trace and FP64");
+        const int tpb = kBlock / seg;
+        const dim3 grid(unsigned(P.V) * unsigned((P.T + tpb - 1) / tpb));
+        const size_t lds = size_t(tpb) * sizeof(SurfLds);
+        const mko_internal::RayStates none{};
+        const mko_internal::RayStates& rs = states ? *states : none;
+#define MKO_PACK(M, SOFT, ST)                                                                      \
+    hipLaunchKernelGGL((trace_pack_kernel<M, SOFT, ST>), grid, dim3(kBlock), lds, stream, P, seg,   \
+                       res, opd, st, part, pf, ph, rs)
+        if (states) {
+            if (P.soft) MKO_PACK(1, true, true);
+            else MKO_PACK(1, false, true);
+        } else if (mode == 0) {
+            if (P.soft) MKO_PACK(0, true, false);
+            else MKO_PACK(0, false, false);
+        } else {
+            if (P.soft) MKO_PACK(1, true, false);
+            else MKO_PACK(1, false, false);
+        }
+#undef MKO_PACK
+        check_hip(hipGetLastError(), "trace pack kernel launch");
+        return;
+    }
     if (states) {
         if (mode != 1 || P.precision != MKO_PRECISION_FP64)
             throw std::invalid_argument("ray states need the MERIT trace and FP64");
@@ -2461,7 +2665,7 @@ int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* s
             w + up(tuples * m.P * m.P * sizeof(float)) +
             up(tuples * kFusedSlots * 2 * m.P * sizeof(double)));
         const hipStream_t s = static_cast<hipStream_t>(stream);
-        const int nw = fused_waves(P, waves);
+        const int nw = fused_waves(P, waves ? waves : g_tuning.fused_waves);
         if (states.data) {
             if (P.precision != MKO_PRECISION_FP64)
                 throw std::invalid_argument("stored ray states need the FP64 trace");
@@ -2554,6 +2758,9 @@ int prefilter_stats(const mko_batch* batch, const mko_residual_settings* setting
         check_hip(hipGetLastError(), "prefilter finalize launch");
     })
 }
+
+void set_thread_tuning(const Tuning& t) { g_tuning = t; }
+Tuning thread_tuning() { return g_tuning; }
 
 int merit_finalize(int V, int T, const double* partial, const int* partial_counts,
                    const double* extra, const double* traced, double* merit, int64_t* failed,

@@ -28,7 +28,8 @@ EXPORTS = [
     "mko_adjoint_normal_device", "mko_vjp_workspace", "mko_vjp_device", "mko_gram_device",
     "mko_gram_mfma_device",
     "mko_df64_ops_device",
-    "mko_engine_create", "mko_engine_destroy", "mko_engine_residuals", "mko_engine_merit",
+    "mko_engine_create", "mko_engine_destroy", "mko_engine_set_tuning", "mko_engine_get_tuning",
+    "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
     "mko_engine_point_merit", "mko_optimize_lm", "mko_lm_create", "mko_lm_cycle",
@@ -84,6 +85,8 @@ def library():
                                  C.c_void_p], C.c_int),
         "mko_engine_create": ([C.c_int, C.c_size_t], C.c_void_p),
         "mko_engine_destroy": ([C.c_void_p], None),
+        "mko_engine_set_tuning": ([C.c_void_p, C.POINTER(abi.Tuning)], C.c_int),
+        "mko_engine_get_tuning": ([C.c_void_p, C.POINTER(abi.Tuning)], C.c_int),
         "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
         "mko_engine_merit": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)], C.c_int),
@@ -252,6 +255,21 @@ class TraceEngine:
         self.close()
 
     KERNEL_FAMILIES = ("trace", "adjoint", "gram", "fd_difference", "point_batch", "lm_control")
+
+    def tuning(self, pack_rays=None, fused_waves=None):
+        """Launch shapes of this engine's kernels (mko_tuning): returns the current (pack_rays,
+        fused_waves) and sets those given (0 = automatic; pack_rays 32 / 64 / 128, 256 = no
+        tuple packs; fused_waves 1 / 2 / 4)."""
+        t = abi.Tuning()
+        _check(library().mko_engine_get_tuning(self._h, C.byref(t)))
+        old = (t.pack_rays, t.fused_waves)
+        if pack_rays is not None:
+            t.pack_rays = int(pack_rays)
+        if fused_waves is not None:
+            t.fused_waves = int(fused_waves)
+        if pack_rays is not None or fused_waves is not None:
+            _check(library().mko_engine_set_tuning(self._h, C.byref(t)))
+        return old
 
     def kernel_timing(self, enable=True):
         """Device ms and launch counts per kernel family since the last call (HIP events on the
