@@ -165,6 +165,23 @@ def uneven_groups(eng, topologies=64, per_topology=128, max_iterations=20, threa
     sync()
     t_alone = time.perf_counter() - t0
     same = all(np.array_equal(a.x, b.x) for a, b in zip(together, alone))
+    # autotuned launch shapes per topology: the first pass times the candidates, the second
+    # runs on the cached choices
+    previous = eng.tuning()
+    t0 = time.perf_counter()
+    eng.optimize_groups(groups, opt, precision="mixed", autotune=True)
+    sync()
+    t_tune_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tuned = eng.optimize_groups(groups, opt, precision="mixed", autotune=True)
+    sync()
+    t_tuned = time.perf_counter() - t0
+    chosen = {}
+    for g in groups:
+        key = "pack%d_waves%d" % eng.autotune(*g)
+        chosen[key] = chosen.get(key, 0) + 1
+    eng.tuning(*previous)
+    tuned_iters = int(sum(int(np.sum(r.iterations)) for r in tuned))
     iters = int(sum(int(np.sum(r.iterations)) for r in together))
     shapes = [(g[0].surfaces, g[2].size, g[0].fields, g[0].wavelengths)
               for g in groups]
@@ -178,6 +195,9 @@ def uneven_groups(eng, topologies=64, per_topology=128, max_iterations=20, threa
             "run_iterations": iters, "seconds_concurrent": t_together,
             "run_iter_per_s": iters / t_together, "seconds_one_after_another": t_alone,
             "concurrency_speedup": t_alone / t_together, "results_equal": same,
+            "autotune": {"chosen": chosen, "seconds_first_with_trials": t_tune_first,
+                         "seconds_cached": t_tuned, "run_iterations": tuned_iters,
+                         "run_iter_per_s": tuned_iters / t_tuned},
             "lm_cycles_max": max(r.cycles for r in together)}
 
 

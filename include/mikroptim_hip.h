@@ -166,6 +166,17 @@ int mko_engine_kernel_timing(mko_engine* engine, int enable, double* ms, int64_t
 int mko_engine_set_tuning(mko_engine* engine, const mko_tuning* tuning);
 int mko_engine_get_tuning(const mko_engine* engine, mko_tuning* tuning);
 
+/* Autotuner: times the candidate launch shapes of the MERIT trace (tuple packing off, or each
+ * pack segment that holds the batch's largest tuple) and of the fused normal equations (1, 2 or 4
+ * waves per workgroup where they apply) on the first min(runs, 64) points x [runs][P] of the base
+ * (one variant, or one per point), keeps the fastest, caches the choice per shape and sets it as
+ * the engine's tuning; `chosen` (may be NULL) receives it. LM sessions keep the tuning the engine
+ * had when they were created. No counterpart in the reference (fixed pack and block shapes,
+ * include/mikroptim/cuda/tuple_packs.hpp:61, src/cuda/wavefront_block.hpp:504-526). */
+int mko_engine_autotune(mko_engine* engine, const mko_batch* base,
+                        const mko_residual_settings* settings, const mko_variables* variables,
+                        int runs, const double* x, mko_tuning* chosen);
+
 /* Constraint rows (settings->extra, see mikroptim_types.h): every host-layer entry point below
  * appends them to each variant's residual vector, R = trace_rows + E + E_t rows, as the
  * reference's FdJacobianOptions::extra / trace_extra do: residuals, J and r0 carry R rows, the

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -1180,6 +1181,114 @@ public:
         part(other, s, MKO_PRECISION_FP64);
     }
 
+    // Autotuner (no counterpart in the reference, whose tuple-pack sizes and blocks per SM are
+    // fixed: include/mikroptim/cuda/tuple_packs.hpp:61, src/cuda/wavefront_block.hpp:504-526).
+    // For the batch's shape the launch shapes of the LM cycle's two dominant kernels are timed on
+    // up to 64 of the caller's points and the fastest kept: the tuple-pack segment of the MERIT
+    // trace (packing off, or each segment that holds the largest tuple) and the waves per
+    // workgroup of the fused normal equations (where they apply). Each candidate runs twice and
+    // the second run's kernel time decides. The choice is cached per shape (a later call on the
+    // same shape times nothing) and becomes the engine's tuning. Results do not depend on the
+    // choice except for the fused kernel's rounding.
+    mko_internal::Tuning autotune(const mko_batch& base, const mko_residual_settings& s,
+                                  const mko_variables& v, int runs, const double* x) {
+        check_batch(base);
+        if (runs < 1) throw std::invalid_argument("autotune needs at least one point");
+        if (v.count < 1) throw std::invalid_argument("no variables");
+        int nmax = 0;
+        for (int f = 0; f < base.fields; ++f)
+            nmax = std::max(nmax, s.rings * (base.pupil_mode[f] == MKO_PUPIL_AXIS ? 1 : s.spokes));
+        std::string key = std::to_string(base.surfaces) + "/" + std::to_string(base.fields) + "/" +
+                          std::to_string(base.wavelengths) + "/" + std::to_string(s.rings) + "/" +
+                          std::to_string(s.spokes) + "/" + std::to_string(v.count) + "/" +
+                          std::to_string(s.failure) + "/" + std::to_string(base.source) + "/" +
+                          std::to_string(s.extra ? s.extra->count : 0);
+        for (int f = 0; f < base.fields; ++f) key += char('0' + base.pupil_mode[f]);
+        auto hit = tuned_.find(key);
+        if (hit != tuned_.end()) {
+            tuning_ = hit->second;
+            return tuning_;
+        }
+        const int n = std::min(runs, 64);
+        // the first n points: a batch of one variant per point is cut to its first n variants
+        mko_batch b = base;
+        if (b.variants > 1) {
+            if (b.variants != runs)
+                throw std::invalid_argument("base must have one variant or one per point");
+            b.variants = n;
+        }
+        const size_t P = size_t(v.count);
+        std::vector<double> merit(n), a(size_t(n) * P * P), g(size_t(n) * P);
+        std::vector<int64_t> failed(n), hard(n);
+        // the caller's timing totals are kept aside while the trials use the timer
+        timer.collect();
+        double saved_ms[kFamilies];
+        int64_t saved_n[kFamilies];
+        std::copy(timer.ms, timer.ms + kFamilies, saved_ms);
+        std::copy(timer.launches, timer.launches + kFamilies, saved_n);
+        const bool was_enabled = timer.enabled;
+        auto trial = [&](const mko_internal::Tuning& t, KernelFamily fam, auto&& work) {
+            mko_internal::TuningScope ts(t);
+            double best = 0.0;
+            for (int rep = 0; rep < 2; ++rep) {
+                timer.reset();
+                timer.enabled = true;
+                work();
+                timer.collect();
+                best = timer.ms[fam];
+            }
+            return best;
+        };
+        mko_internal::Tuning chosen{};
+        try {
+            std::vector<int> packs{0};
+            if (nmax <= 128 && base.fields * base.wavelengths >= 2 &&
+                s.precision == MKO_PRECISION_FP64) {
+                packs = {256};
+                for (int seg : {32, 64, 128})
+                    if (seg >= nmax) packs.push_back(seg);
+            }
+            double best = -1.0;
+            for (int pk : packs) {
+                const double ms = trial(mko_internal::Tuning{pk, 0}, kTrace, [&] {
+                    point_merit(b, s, v, n, x, merit.data(), failed.data(), hard.data());
+                });
+                if (best < 0.0 || ms < best) {
+                    best = ms;
+                    chosen.pack_rays = pk;
+                }
+            }
+            if (s.failure == MKO_FAILURE_HARD && fused_ok(s, v, true)) {
+                std::vector<int> waves{2, 4};
+                if (nmax <= 64) waves.insert(waves.begin(), 1);
+                best = -1.0;
+                for (int w : waves) {
+                    const double ms = trial(mko_internal::Tuning{chosen.pack_rays, w}, kAdjoint, [&] {
+                        jacobian(b, s, v, n, x, MKO_JACOBIAN_ADJOINT, MKO_PRECISION_MIXED,
+                                 nullptr, nullptr, a.data(), g.data());
+                    });
+                    if (best < 0.0 || ms < best) {
+                        best = ms;
+                        chosen.fused_waves = w;
+                    }
+                }
+            }
+        } catch (...) {
+            timer.reset();
+            std::copy(saved_ms, saved_ms + kFamilies, timer.ms);
+            std::copy(saved_n, saved_n + kFamilies, timer.launches);
+            timer.enabled = was_enabled;
+            throw;
+        }
+        timer.reset();
+        std::copy(saved_ms, saved_ms + kFamilies, timer.ms);
+        std::copy(saved_n, saved_n + kFamilies, timer.launches);
+        timer.enabled = was_enabled;
+        tuned_[key] = chosen;
+        tuning_ = chosen;
+        return chosen;
+    }
+
     // Normal equations of stored Jacobians: the host's FMA chain (bitwise), or FP64 MFMA for
     // MKO_PRECISION_FP64_MFMA.
     static int gram(int precision, const double* J, const double* r0, int runs, int P, int64_t R,
@@ -1336,6 +1445,7 @@ private:
     int device_;
     size_t limit_;
     mko_internal::Tuning tuning_;
+    std::map<std::string, mko_internal::Tuning> tuned_; // autotuner's choices per shape
     DeviceBatch batch_, tmpl_, work_, rowfd_;
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
     Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
@@ -1513,10 +1623,12 @@ void solve_step(Run& s, const Bounds& b, size_t n, double mu) {
 class Lm {
 public:
     ~Lm() { graph_release(); }
-    Engine& engine() { return eng_; }
+    // launch shapes of this session: the engine's tuning when the session was created
+    const mko_internal::Tuning& tuning() const { return tuning_; }
     Lm(Engine& eng, const mko_batch& base, const mko_residual_settings& s, const mko_variables& v,
        int runs, const double* x0, const mko_lm_options& o, int precision)
-        : eng_(eng), base_(base), settings_(s), vars_(v), runs_(runs), o_(o), precision_(precision) {
+        : eng_(eng), base_(base), settings_(s), vars_(v), runs_(runs), o_(o), precision_(precision),
+          tuning_(eng.tuning()) {
         check_batch(base);
         if (runs < 1) throw std::invalid_argument("no runs");
         if (base.variants != 1 && base.variants != runs)
@@ -2363,6 +2475,7 @@ private:
     int runs_;
     mko_lm_options o_;
     int precision_;
+    mko_internal::Tuning tuning_;
     size_t P_ = 0;
     Bounds bounds_{};
     std::vector<double> x0_;
@@ -2477,6 +2590,20 @@ int mko_engine_get_tuning(const mko_engine* e, mko_tuning* t) {
     })
 }
 
+int mko_engine_autotune(mko_engine* e, const mko_batch* base, const mko_residual_settings* s,
+                        const mko_variables* v, int runs, const double* x, mko_tuning* chosen) {
+    MKO_HOST_TRY({
+        if (!e || !base || !s || !v || !x) throw std::invalid_argument("autotune arguments missing");
+        validate_values(*base, 0, base->variants);
+        const mko_internal::Tuning t = e->engine.autotune(*base, *s, *v, runs, x);
+        if (chosen) {
+            chosen->pack_rays = t.pack_rays;
+            chosen->fused_waves = t.fused_waves;
+        }
+        return 0;
+    })
+}
+
 int mko_engine_residuals(mko_engine* e, const mko_batch* b, const mko_residual_settings* s,
                          double* res, double* opd, uint8_t* st) {
     MKO_HOST_TRY(mko_internal::TuningScope ts(e->engine.tuning());
@@ -2536,7 +2663,7 @@ mko_lm* mko_lm_create(mko_engine* e, const mko_batch* base, const mko_residual_s
 }
 
 int mko_lm_cycle(mko_lm* lm) {
-    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.engine().tuning());
+    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.tuning());
                  return lm->lm.cycle() ? 1 : 0;)
 }
 
@@ -2549,12 +2676,12 @@ void mko_lm_destroy(mko_lm* lm) { delete lm; }
 int mko_lm_graph_cycles(const mko_lm* lm) { return lm ? lm->lm.graph_cycles() : 0; }
 
 int mko_lm_cycle_begin(mko_lm* lm) {
-    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.engine().tuning());
+    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.tuning());
                  return lm->lm.cycle_begin();)
 }
 
 int mko_lm_cycle_end(mko_lm* lm) {
-    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.engine().tuning());
+    MKO_HOST_TRY(mko_internal::TuningScope ts(lm->lm.tuning());
                  return lm->lm.cycle_end() ? 1 : 0;)
 }
 

@@ -29,6 +29,7 @@ EXPORTS = [
     "mko_gram_mfma_device",
     "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_set_tuning", "mko_engine_get_tuning",
+    "mko_engine_autotune",
     "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
@@ -87,6 +88,8 @@ def library():
         "mko_engine_destroy": ([C.c_void_p], None),
         "mko_engine_set_tuning": ([C.c_void_p, C.POINTER(abi.Tuning)], C.c_int),
         "mko_engine_get_tuning": ([C.c_void_p, C.POINTER(abi.Tuning)], C.c_int),
+        "mko_engine_autotune": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.POINTER(abi.Tuning)],
+                                C.c_int),
         "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
         "mko_engine_merit": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)], C.c_int),
@@ -271,6 +274,18 @@ class TraceEngine:
             _check(library().mko_engine_set_tuning(self._h, C.byref(t)))
         return old
 
+    def autotune(self, base, settings, variables, x):
+        """Times the candidate launch shapes on the first min(n, 64) points x [n][P] (base: one
+        variant, or one per point), sets the fastest as this engine's tuning (cached per shape)
+        and returns it as (pack_rays, fused_waves). LM sessions keep the tuning of the engine at
+        their creation."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        t = abi.Tuning()
+        b, s, v = base.c_struct(), settings.c_struct(), variables.c_struct()
+        _check(library().mko_engine_autotune(self._h, C.byref(b), C.byref(s), C.byref(v),
+                                             x.shape[0], abi.ptr(x), C.byref(t)))
+        return t.pack_rays, t.fused_waves
+
     def kernel_timing(self, enable=True):
         """Device ms and launch counts per kernel family since the last call (HIP events on the
         launch stream); then resets and enables/disables timing."""
@@ -387,15 +402,30 @@ class TraceEngine:
                                          abi.ptr(x), res, C.byref(cyc)))
         return LmResult(x, list(res), cyc.value)
 
-    def optimize_groups(self, groups, options=None, precision="fp64"):
+    def optimize_groups(self, groups, options=None, precision="fp64", autotune=False):
         """Uneven batches: runs of different topologies (surface, field, wavelength or variable
         counts), one batched LM per topology group; groups = [(base, settings, variables, x0)].
         Returns the LmResult of every group, in order. The groups advance cycle by cycle
         together; groups whose cycle is a captured hipGraph (device control, fused mixed adjoint)
         run concurrently on their own streams. Each group's result is that of optimize() on it,
-        bit for bit. (The reference's batches hold a single topology,
+        bit for bit. With autotune, each group's launch shapes are first chosen by autotune() on
+        its own points (the fused kernel's rounding may then differ from the untuned run).
+        (The reference's batches hold a single topology,
         include/mikroptim/batch/system_batch.hpp:26.)"""
-        sessions = [self.lm(b, s, v, x0, options, precision) for b, s, v, x0 in groups]
+        sessions = []
+        previous = self.tuning()
+        try:
+            for b, s, v, x0 in groups:
+                if autotune:
+                    self.autotune(b, s, v, x0)
+                sessions.append(self.lm(b, s, v, x0, options, precision))
+        except BaseException:
+            for sess in sessions:
+                sess.close()
+            raise
+        finally:
+            if autotune:
+                self.tuning(*previous)
         try:
             # every group's cycle is begun before any is waited for: the groups whose cycles are
             # captured graphs run concurrently, each on its own stream
