@@ -787,9 +787,13 @@ public:
 
     // J / r0 / A / g for `runs` points; base has 1 variant or one per run. Rows per run:
     // trace rows + constraint rows.
+    // failed (NULL: unknown): the failed-ray count of each run's point, which the soft split
+    // then takes instead of tracing the points; rows_settings (NULL: s): the settings of the
+    // constraint rows' central differences.
     void jacobian(const mko_batch& base, const mko_residual_settings& s, const mko_variables& vars,
                   int runs, const double* x0, int method, int precision, double* J, double* r0,
-                  double* a, double* g) {
+                  double* a, double* g, const int64_t* failed = nullptr,
+                  const mko_residual_settings* rows_settings = nullptr) {
         DeviceGuard guard(device_);
         check_batch(base);
         if (base.variants != 1 && base.variants != runs)
@@ -802,9 +806,10 @@ public:
             throw std::invalid_argument("unknown precision");
         validate_values(base, 0, base.variants);
         if (soft_split_applies(s, vars, method, precision) && !J && !r0 && (a || g)) {
-            soft_split(base, s, vars, runs, x0, a, g);
+            soft_split(base, s, vars, runs, x0, a, g, failed);
             return;
         }
+        const mko_residual_settings& rs = rows_settings ? *rows_settings : s;
         RowPlan rp = plan_rows(base, s);
         if (rp.paraxial) validate_paraxial(base, 0, base.variants);
         upload_rows(rp);
@@ -911,7 +916,7 @@ public:
                     double* Je = je_.as<double>(size_t(count) * P * E);
                     double* re = re_.as<double>(size_t(count) * E);
                     rows_fd(tmpl_.view, nullptr, base.variants != 1, count, dx, dplus, dminus, dsp,
-                            dsm, dsten, s, rp, P, Je, re, E, 0);
+                            dsm, dsten, rs, rp, P, Je, re, E, 0);
                     dev_check(mko_internal::extra_normal(count, P, E, Je, re, da, dg, nullptr));
                 }
                 if (a) copy_out(a + size_t(first) * P * P, da, size_t(count) * P * P * 8);
@@ -930,7 +935,7 @@ public:
                 });
                 if (rp.count())
                     rows_fd(tmpl_.view, nullptr, base.variants != 1, count, dx, dplus, dminus, dsp,
-                            dsm, dsten, s, rp, P, dJ, dr0, R, T);
+                            dsm, dsten, rs, rp, P, dJ, dr0, R, T);
             }
             if (a || g) {
                 double* da = a_.as<double>(size_t(count) * P * P);
@@ -1006,9 +1011,11 @@ public:
     void jacobian_dev(const mko_batch& tmpl, const int32_t* tidx, int count, const double* xd,
                       const mko_residual_settings& s, int P, const double* lower,
                       const double* upper, const double* typical, int method, int precision,
-                      double* ad, double* gd, const mko_internal::RayStates* states = nullptr) {
+                      double* ad, double* gd, const mko_internal::RayStates* states = nullptr,
+                      const mko_residual_settings* rows_settings = nullptr) {
         DeviceGuard guard(device_);
         if (count <= 0) return;
+        const mko_residual_settings& rs = rows_settings ? *rows_settings : s;
         RowPlan rp = plan_rows(tmpl, s);
         upload_rows(rp);
         const int64_t T = trace_rows_of(tmpl, s), R = T + rp.count(), rays = rays_of(tmpl, s);
@@ -1075,7 +1082,7 @@ public:
                     const int E = rp.count();
                     double* Je = je_.as<double>(size_t(c) * P * E);
                     double* re = re_.as<double>(size_t(c) * E);
-                    rows_fd(tmpl, ti, 0, c, x, dplus, dminus, dsp, dsm, dsten, s, rp, P, Je, re, E,
+                    rows_fd(tmpl, ti, 0, c, x, dplus, dminus, dsp, dsm, dsten, rs, rp, P, Je, re, E,
                             0);
                     dev_check(mko_internal::extra_normal(c, P, E, Je, re, a_out, g_out, nullptr));
                 }
@@ -1091,7 +1098,7 @@ public:
                                                              dr0, nullptr, R, nullptr));
                 });
                 if (rp.count())
-                    rows_fd(tmpl, ti, 0, c, x, dplus, dminus, dsp, dsm, dsten, s, rp, P, dJ, dr0, R,
+                    rows_fd(tmpl, ti, 0, c, x, dplus, dminus, dsp, dsm, dsten, rs, rp, P, dJ, dr0, R,
                             T);
             }
             timer.run(kGram, [&] {
@@ -1151,13 +1158,22 @@ public:
         hs.failure = MKO_FAILURE_HARD;
         return fused_ok(hs, v);
     }
+    // failed_in (NULL: traced here): the failed-ray count of each point. The trace rows of the
+    // clean runs are hard mode's; their constraint rows keep the caller's settings (the
+    // differences' perturbed points may fail rays).
     void soft_split(const mko_batch& base, const mko_residual_settings& s, const mko_variables& vars,
-                    int runs, const double* x0, double* a, double* g) {
+                    int runs, const double* x0, double* a, double* g,
+                    const int64_t* failed_in = nullptr) {
         const size_t P = size_t(vars.count);
         const size_t nr = size_t(runs);
-        std::vector<double> merit(nr);
-        std::vector<int64_t> failed(nr), hard(nr);
-        point_merit(base, s, vars, runs, x0, merit.data(), failed.data(), hard.data());
+        std::vector<int64_t> failed(nr);
+        if (failed_in) {
+            std::copy_n(failed_in, nr, failed.begin());
+        } else {
+            std::vector<double> merit(nr);
+            std::vector<int64_t> hard(nr);
+            point_merit(base, s, vars, runs, x0, merit.data(), failed.data(), hard.data());
+        }
         std::vector<int> clean, other;
         for (int i = 0; i < runs; ++i) (failed[size_t(i)] == 0 ? clean : other).push_back(i);
         mko_residual_settings hs = s;
@@ -1170,7 +1186,7 @@ public:
             GatheredBatch gb;
             const mko_batch& b = base.variants == 1 ? base : gb.gather(base, ids.data(), n);
             jacobian(b, ps, vars, int(n), xs.data(), MKO_JACOBIAN_ADJOINT, prec, nullptr, nullptr,
-                     a ? as.data() : nullptr, g ? gs.data() : nullptr);
+                     a ? as.data() : nullptr, g ? gs.data() : nullptr, nullptr, &s);
             for (size_t j = 0; j < n; ++j) {
                 const size_t i = size_t(ids[j]);
                 if (a) std::copy_n(as.data() + j * P * P, P * P, a + i * P * P);
@@ -1692,7 +1708,7 @@ public:
     // caller may begin other LMs' cycles, which then run concurrently on the device.
     int cycle_begin() {
         if (done_) return 0;
-        if (device_ && graph_eligible() && graph_ready()) {
+        if (device_ && graph_eligible() && graph_ready() && graph_fits()) {
             graph_launch();
             pending_ = 1;
             return 1;
@@ -1825,8 +1841,10 @@ private:
             g_.assign(list.size() * P_, 0.0);
             std::vector<int> global;
             const mko_batch& tmpl = template_for(list, global);
+            std::vector<int64_t> failed(list.size()); // the points' failed rays, for the soft split
+            for (size_t j = 0; j < list.size(); ++j) failed[j] = state_[size_t(list[j])].failed;
             eng_.jacobian(tmpl, settings_, vars_, int(list.size()), xs_.data(), o_.jacobian_method,
-                          precision_, nullptr, nullptr, a_.data(), g_.data());
+                          precision_, nullptr, nullptr, a_.data(), g_.data(), failed.data());
             for (size_t j = 0; j < list.size(); ++j) {
                 Run& s = state_[size_t(list[j])];
                 std::copy_n(a_.data() + j * P_ * P_, P_ * P_, s.a.begin());
@@ -2060,8 +2078,10 @@ private:
         S.lower = bnd;
         S.upper = bnd + P;
         S.typical = bnd + 2 * P;
-        lists_ = lists_mem_.as<int32_t>(4 * N + 8);
-        counts_ = lists_ + 4 * N; // [0] Jacobian, [1] cycle, [2] evaluate
+        lists_ = lists_mem_.as<int32_t>(6 * N + 8);
+        // [0] Jacobian, [1] cycle, [2] evaluate, [3] / [4] the Jacobian list's runs without /
+        // with failed rays (soft split), [5] runs of the next Jacobian list with failed rays
+        counts_ = lists_ + 6 * N;
     }
 
     // template index per run of the round (global run ids) when runs have their own template
@@ -2095,6 +2115,17 @@ private:
                        use_states_ ? &states_ : nullptr);
         eng_.timer.run(kLmControl, [&] { mko_lmdev::launch_start(ds_, fm, ff, fh, nullptr); });
         hip_check(hipGetLastError(), "LM start");
+        count_failing();
+    }
+
+    // soft split: how many runs of the next cycle's Jacobian list have failed rays (the FP64
+    // share of its normal equations), read at the end of each cycle to choose its form
+    void count_failing() {
+        if (!soft_split_) return;
+        eng_.timer.run(kLmControl, [&] {
+            mko_lmdev::launch_count_failing(ds_, o_.max_iterations, counts_ + 5, nullptr);
+        });
+        failing_next_ = dev_count(5);
     }
 
     bool dev_round_cycle() {
@@ -2113,52 +2144,57 @@ private:
         });
         const int cj = dev_count(0);
         if (cj > 0) {
-            // soft mode, mixed adjoint: the runs whose point has no failed ray first (hard
-            // mode's normal equations, fused kernel), then the others (FP64), as
-            // Engine::soft_split orders the work
-            int clean = 0;
-            if (soft_split_) {
-                const size_t ncj = size_t(cj), nn = size_t(n);
-                std::vector<int32_t> lj(ncj);
-                std::vector<int64_t> fl(nn);
-                hip_check(hipMemcpy(lj.data(), list_j, size_t(cj) * 4, hipMemcpyDeviceToHost),
-                          "LM list");
-                hip_check(hipMemcpy(fl.data(), ds_.failed, size_t(n) * 8, hipMemcpyDeviceToHost),
-                          "LM failed rays");
-                const auto mid = std::stable_partition(
-                    lj.begin(), lj.end(), [&](int32_t r) { return fl[size_t(r)] == 0; });
-                clean = int(mid - lj.begin());
-                hip_check(hipMemcpy(list_j, lj.data(), size_t(cj) * 4, hipMemcpyHostToDevice),
-                          "LM list");
-            }
-            eng_.timer.run(kLmControl, [&] {
-                mko_lmdev::launch_gather(ds_, list_j, cj, false, rix, xs, tidx, nullptr);
-            });
             double* ac = ac_.as<double>(size_t(cj) * P_ * P_);
             double* gc = gc_.as<double>(size_t(cj) * P_);
             mko_internal::RayStates st = states_;
             st.slot = list_j;
             if (soft_split_) {
+                // soft mode, mixed adjoint (Engine::soft_split): the runs whose point has no
+                // failed ray take hard mode's normal equations (fused kernel), the others FP64;
+                // the device splits the list, each part is gathered, solved and prepared alone
+                int32_t* list_cl = lists_ + 4 * n;
+                int32_t* list_ft = lists_ + 5 * n;
+                eng_.timer.run(kLmControl, [&] {
+                    mko_lmdev::launch_split(ds_, list_j, counts_ + 0, list_cl, counts_ + 3,
+                                            list_ft, counts_ + 4, nullptr);
+                });
+                int32_t split[2] = {0, 0};
+                hip_check(hipMemcpy(split, counts_ + 3, sizeof(split), hipMemcpyDeviceToHost),
+                          "LM split counts");
                 mko_residual_settings hs = settings_;
                 hs.failure = MKO_FAILURE_HARD;
-                if (clean > 0)
-                    eng_.jacobian_dev(dev_tmpl_.view, tidx, clean, xs, hs, int(P_), ds_.lower,
-                                      ds_.upper, ds_.typical, o_.jacobian_method,
-                                      MKO_PRECISION_MIXED, ac, gc, use_states_ ? &st : nullptr);
-                if (cj > clean)
-                    eng_.jacobian_dev(dev_tmpl_.view, tidx ? tidx + clean : nullptr, cj - clean,
-                                      xs + size_t(clean) * P_, settings_, int(P_), ds_.lower,
-                                      ds_.upper, ds_.typical, o_.jacobian_method,
-                                      MKO_PRECISION_FP64, ac + size_t(clean) * P_ * P_,
-                                      gc + size_t(clean) * P_, nullptr);
+                for (int part = 0; part < 2; ++part) {
+                    const int c = split[part];
+                    if (c == 0) continue;
+                    const int32_t* list = part == 0 ? list_cl : list_ft;
+                    eng_.timer.run(kLmControl, [&] {
+                        mko_lmdev::launch_gather(ds_, list, c, false, rix, xs, tidx, nullptr);
+                    });
+                    st.slot = list;
+                    if (part == 0)
+                        eng_.jacobian_dev(dev_tmpl_.view, tidx, c, xs, hs, int(P_), ds_.lower,
+                                          ds_.upper, ds_.typical, o_.jacobian_method,
+                                          MKO_PRECISION_MIXED, ac, gc,
+                                          use_states_ ? &st : nullptr, &settings_);
+                    else
+                        eng_.jacobian_dev(dev_tmpl_.view, tidx, c, xs, settings_, int(P_),
+                                          ds_.lower, ds_.upper, ds_.typical, o_.jacobian_method,
+                                          MKO_PRECISION_FP64, ac, gc, nullptr);
+                    eng_.timer.run(kLmControl, [&] {
+                        mko_lmdev::launch_prepare(ds_, dopt_, list, c, ac, gc, nullptr);
+                    });
+                }
             } else {
+                eng_.timer.run(kLmControl, [&] {
+                    mko_lmdev::launch_gather(ds_, list_j, cj, false, rix, xs, tidx, nullptr);
+                });
                 eng_.jacobian_dev(dev_tmpl_.view, tidx, cj, xs, settings_, int(P_), ds_.lower,
                                   ds_.upper, ds_.typical, o_.jacobian_method, precision_, ac, gc,
                                   use_states_ ? &st : nullptr);
+                eng_.timer.run(kLmControl, [&] {
+                    mko_lmdev::launch_prepare(ds_, dopt_, list_j, cj, ac, gc, nullptr);
+                });
             }
-            eng_.timer.run(kLmControl, [&] {
-                mko_lmdev::launch_prepare(ds_, dopt_, list_j, cj, ac, gc, nullptr);
-            });
         }
         eng_.timer.run(kLmControl, [&] {
             mko_lmdev::launch_select(ds_, mko_lmdev::kListCycle, o_.max_iterations, nullptr,
@@ -2167,6 +2203,7 @@ private:
         const int cc = dev_count(1);
         if (cc == 0) return false;
         ++cycles_;
+        eager_pending_ = true; // the cycle's last launches are still running on the null stream
         eng_.timer.run(kLmControl, [&] {
             mko_lmdev::launch_step(ds_, dopt_, list_c, cc, nullptr);
             mko_lmdev::launch_select(ds_, mko_lmdev::kListEvaluate, o_.max_iterations, list_c,
@@ -2189,6 +2226,7 @@ private:
             mko_lmdev::launch_update(ds_, dopt_, list_c, cc, pos, fm, ff, fh, nullptr);
         });
         hip_check(hipGetLastError(), "LM cycle");
+        count_failing();
         return true;
     }
 
@@ -2212,11 +2250,16 @@ private:
     // The graph owns its work buffers (nothing the engine reallocates between cycles) and is
     // rebuilt when a round starts. Same kernels, same inputs per run: the results are those of
     // the eager cycle bit for bit.
+    // Soft failure mode (soft split): the cycle splits its Jacobian list on the device, the runs
+    // without failed rays take the fused kernel, the others the FP64 stored-J path on buffers
+    // for up to gdirty_cap_ runs; a cycle whose list holds more such runs (known from the
+    // previous cycle's count) runs eagerly.
     bool graph_eligible() const {
         return graph_wanted_ && !graph_failed_ && use_states_ && settings_.extra == nullptr &&
                precision_ == MKO_PRECISION_MIXED && o_.jacobian_method == MKO_JACOBIAN_ADJOINT &&
-               Engine::fused_ok(settings_, vars_, true);
+               (Engine::fused_ok(settings_, vars_, true) || soft_split_);
     }
+    bool graph_fits() const { return !soft_split_ || failing_next_ <= gdirty_cap_; }
 
     void graph_reset() {
         if (gexec_) (void)hipGraphExecDestroy(gexec_);
@@ -2269,7 +2312,7 @@ private:
         const mko_batch& tmpl = dev_tmpl_.view;
         if (!gs_) hip_check(hipStreamCreateWithFlags(&gs_, hipStreamNonBlocking), "graph stream");
         if (!counts_host_)
-            hip_check(hipHostMalloc(reinterpret_cast<void**>(&counts_host_), 16, hipHostMallocDefault),
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&counts_host_), 32, hipHostMallocDefault),
                       "pinned counts");
         // buffers of n runs, allocated before the capture (within half of the memory budget;
         // else the eager cycle, which tranches)
@@ -2278,7 +2321,14 @@ private:
         const size_t wbytes = mko_adjoint_normal_workspace(n, tmpl.fields, tmpl.wavelengths, int(P));
         const size_t need =
             wbytes + N * (Strides(tmpl).doubles() * 8 + P * P * 8 + P * 16 + T * 16 + 64);
-        if (need > eng_.budget() / 2) return false;
+        const size_t half = eng_.budget() / 2;
+        if (need > half) return false;
+        // soft split: FP64 stored-J buffers for as many runs with failed rays as fit
+        const int64_t R = trace_rows_of(tmpl, settings_);
+        const size_t per_dirty =
+            Strides(tmpl).doubles() * 8 + (P + 1) * size_t(R) * 8 + P * P * 8 + P * 16 + 64;
+        gdirty_cap_ = soft_split_ ? int(std::min<size_t>(N, (half - need) / per_dirty)) : 0;
+        const size_t D = size_t(gdirty_cap_);
         gwork_.allocate(tmpl, n);
         void* ws = gws_.reserve(wbytes);
         double* ac = gac_.as<double>(N * P * P);
@@ -2291,6 +2341,14 @@ private:
         int64_t* fh = gfh_.as<int64_t>(N);
         double* part = gpart_.as<double>(N * T);
         int* pc = gpc_.as<int>(N * T * 2);
+        // soft split buffers (D runs with failed rays)
+        if (D > 0) gwork2_.allocate(tmpl, int(D));
+        double* xs2 = D ? gxs2_.as<double>(D * P) : nullptr;
+        int32_t* tidx2 = rix && D ? gtidx2_.as<int32_t>(D) : nullptr;
+        double* J2 = D ? gj2_.as<double>(D * P * size_t(R)) : nullptr;
+        double* r02 = D ? gr02_.as<double>(D * size_t(R)) : nullptr;
+        double* ac2 = D ? gac2_.as<double>(D * P * P) : nullptr;
+        double* gc2 = D ? ggc2_.as<double>(D * P) : nullptr;
         int32_t* vt = gvt_.as<int32_t>(P);
         int32_t* vs = gvs_.as<int32_t>(P);
         hip_check(hipMemcpy(vt, vars_.type, P * 4, hipMemcpyHostToDevice), "graph vars");
@@ -2311,33 +2369,58 @@ private:
         int32_t* list_e = lists_ + 2 * n;
         int32_t* pos = lists_ + 3 * n;
         const int32_t *cj = counts_ + 0, *cc = counts_ + 1, *ce = counts_ + 2;
+        // soft split: the fused part works on the runs without failed rays (list_cl, count [3])
+        // under hard mode's settings, the FP64 part on the others (list_ft, count [4])
+        int32_t* list_cl = lists_ + 4 * n;
+        int32_t* list_ft = lists_ + 5 * n;
+        const int32_t* cf = soft_split_ ? counts_ + 3 : cj;
+        const int32_t* cd = counts_ + 4;
+        int32_t* list_f = soft_split_ ? list_cl : list_j;
+        mko_residual_settings hs = settings_;
+        if (soft_split_) hs.failure = MKO_FAILURE_HARD;
         mko_internal::RayStates st_j = states_, st_e = states_;
-        st_j.slot = list_j;
+        st_j.slot = list_f;
         st_e.slot = list_e;
-        auto point_batch = [this, &tmpl, tidx, n, dv, xs](const int32_t* live) {
+        auto point_batch = [this, &tmpl, &dv](DeviceBatch& w, const int32_t* ti, int count,
+                                              const double* x, const int32_t* live) {
             dev_check(mko_internal::point_batch(
-                &tmpl, tidx, 0, n, 1, &dv, xs, nullptr, nullptr, gwork_.mut(gwork_.view.curvature),
-                gwork_.mut(gwork_.view.thickness), gwork_.mut(gwork_.view.index),
-                gwork_.mut(gwork_.view.pupil_radius), gwork_.mut(gwork_.view.field_angle),
-                gwork_.mut(gwork_.view.field_target), gwork_.mut(gwork_.view.reference_curvature),
-                gwork_.mut(gwork_.view.paraxial_index),
-                gwork_.mut(gwork_.view.buchdahl), gs_, live));
+                &tmpl, ti, 0, count, 1, &dv, x, nullptr, nullptr, w.mut(w.view.curvature),
+                w.mut(w.view.thickness), w.mut(w.view.index), w.mut(w.view.pupil_radius),
+                w.mut(w.view.field_angle), w.mut(w.view.field_target),
+                w.mut(w.view.reference_curvature), w.mut(w.view.paraxial_index),
+                w.mut(w.view.buchdahl), gs_, live));
         };
-        // the parts of the cycle in order; `which`: the list whose count > 0 makes the part a
+        const int Di = int(D);
+        // the parts of the cycle in order; `which`: the count whose value > 0 makes the part a
         // timed launch, as in the eager cycle (-1: always)
-        const std::vector<GraphPart> parts = {
+        std::vector<GraphPart> parts = {
             {kLmControl, -1, [&] {
                  mko_lmdev::launch_select(ds_, mko_lmdev::kListJacobian, o_.max_iterations,
                                           nullptr, nullptr, list_j, counts_ + 0, nullptr, gs_);
-                 mko_lmdev::launch_gather(ds_, list_j, n, false, rix, xs, tidx, gs_, cj);
+                 if (soft_split_)
+                     mko_lmdev::launch_split(ds_, list_j, cj, list_cl, counts_ + 3, list_ft,
+                                             counts_ + 4, gs_);
+                 mko_lmdev::launch_gather(ds_, list_f, n, false, rix, xs, tidx, gs_, cf);
              }},
-            {kPointBatch, 0, [&] { point_batch(cj); }},
-            {kAdjoint, 0, [&] {
-                 dev_check(mko_internal::adjoint_normal_states(&gwork_.view, &settings_, &vars_,
-                                                               ac, gc, ws, wbytes, st_j, gs_, cj));
+            {kPointBatch, soft_split_ ? 3 : 0, [&] { point_batch(gwork_, tidx, n, xs, cf); }},
+            {kAdjoint, soft_split_ ? 3 : 0, [&] {
+                 dev_check(mko_internal::adjoint_normal_states(&gwork_.view, &hs, &vars_, ac, gc,
+                                                               ws, wbytes, st_j, gs_, cf));
              }},
             {kLmControl, -1, [&] {
-                 mko_lmdev::launch_prepare(ds_, dopt_, list_j, n, ac, gc, gs_, cj);
+                 mko_lmdev::launch_prepare(ds_, dopt_, list_f, n, ac, gc, gs_, cf);
+                 mko_lmdev::launch_gather(ds_, list_ft, Di, false, rix, xs2, tidx2, gs_, cd);
+             }},
+            {kPointBatch, 4, [&] { point_batch(gwork2_, tidx2, Di, xs2, cd); }},
+            {kAdjoint, 4, [&] {
+                 dev_check(mko_internal::adjoint_jacobian(&gwork2_.view, &settings_, &vars_, J2,
+                                                          r02, nullptr, R, gs_, cd));
+             }},
+            {kGram, 4, [&] {
+                 dev_check(mko_internal::gram_chain(J2, r02, Di, int(P), R, ac2, gc2, gs_, cd));
+             }},
+            {kLmControl, -1, [&] {
+                 mko_lmdev::launch_prepare(ds_, dopt_, list_ft, Di, ac2, gc2, gs_, cd);
                  mko_lmdev::launch_select(ds_, mko_lmdev::kListCycle, o_.max_iterations, nullptr,
                                           nullptr, list_c, counts_ + 1, nullptr, gs_);
                  mko_lmdev::launch_step(ds_, dopt_, list_c, n, gs_, cc);
@@ -2345,15 +2428,19 @@ private:
                                           list_c, counts_ + 1, list_e, counts_ + 2, pos, gs_);
                  mko_lmdev::launch_gather(ds_, list_e, n, true, rix, xs, tidx, gs_, ce);
              }},
-            {kPointBatch, 2, [&] { point_batch(ce); }},
+            {kPointBatch, 2, [&] { point_batch(gwork_, tidx, n, xs, ce); }},
             {kTrace, 2, [&] {
                  dev_check(mko_internal::merit_states(&gwork_.view, &settings_, fm, ff, fh, part,
                                                       pc, st_e, gs_, ce));
              }},
             {kLmControl, -1, [&] {
                  mko_lmdev::launch_update(ds_, dopt_, list_c, n, pos, fm, ff, fh, gs_, cc);
+                 if (soft_split_)
+                     mko_lmdev::launch_count_failing(ds_, o_.max_iterations, counts_ + 5, gs_);
              }},
         };
+        if (Di == 0) // no FP64 part: its point batch, adjoint and Gram are left out
+            parts.erase(parts.begin() + 4, parts.begin() + 7);
         std::vector<const GraphPart*> all;
         for (const GraphPart& p : parts) all.push_back(&p);
         gexec_ = graph_capture_parts(all);
@@ -2380,8 +2467,12 @@ private:
         return false;
     }
 
-    // Launches the cycle (and the read of its three counts) on gs_ without waiting.
+    // Launches the cycle (and the read of its counts) on gs_ without waiting.
     void graph_launch() {
+        if (eager_pending_) { // an eager cycle's launches on the null stream come first
+            hip_check(hipStreamSynchronize(nullptr), "LM cycle");
+            eager_pending_ = false;
+        }
         if (eng_.timer.enabled) {
             for (const GraphTimed& t : gparts_) {
                 hip_check(hipEventRecord(t.a, gs_), "hipEventRecord");
@@ -2393,7 +2484,7 @@ private:
             hip_check(hipGraphLaunch(gexec_, gs_), "graph launch");
             timed_launch_ = false;
         }
-        hip_check(hipMemcpyAsync(counts_host_, counts_, 3 * sizeof(int32_t), hipMemcpyDeviceToHost,
+        hip_check(hipMemcpyAsync(counts_host_, counts_, 6 * sizeof(int32_t), hipMemcpyDeviceToHost,
                                  gs_),
                   "LM counts");
     }
@@ -2411,13 +2502,18 @@ private:
                 eng_.timer.add(t.fam, ms);
             }
         }
+        if (soft_split_) {
+            if (counts_host_[4] > gdirty_cap_)
+                throw std::runtime_error("LM cycle: more runs with failed rays than its buffers");
+            failing_next_ = counts_host_[5];
+        }
         if (counts_host_[1] == 0) return false;
         ++cycles_;
         return true;
     }
 
     bool graph_round_cycle() {
-        if (!graph_ready()) return dev_round_cycle();
+        if (!graph_ready() || !graph_fits()) return dev_round_cycle();
         graph_launch();
         return graph_finish();
     }
@@ -2512,8 +2608,12 @@ private:
     hipGraphExec_t gexec_ = nullptr;
     int32_t* counts_host_ = nullptr;
     std::vector<GraphTimed> gparts_;
-    DeviceBatch gwork_;
+    DeviceBatch gwork_, gwork2_;
     Buffer gws_, gac_, ggc_, gxs_, gtidx_, gfm_, gff_, gfh_, gpart_, gpc_, gvt_, gvs_, gvc_;
+    Buffer gxs2_, gtidx2_, gj2_, gr02_, gac2_, ggc2_; // soft split: the FP64 part
+    int gdirty_cap_ = 0;     // runs with failed rays the captured cycle's FP64 part holds
+    int failing_next_ = 0;   // runs with failed rays in the next cycle's Jacobian list
+    bool eager_pending_ = false;
 };
 
 } // namespace mko_host

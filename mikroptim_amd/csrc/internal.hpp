@@ -101,10 +101,15 @@ int rows_merit(int V, const double* rows, int64_t stride, int64_t offset, int E,
 int extra_normal(int runs, int P, int E, const double* Je, const double* re, double* a, double* g,
                  void* stream);
 
-// mko_adjoint_jacobian_device (FP64) with J / r0 rows at `stride` (0: trace rows).
+// mko_adjoint_jacobian_device (FP64) with J / r0 rows at `stride` (0: trace rows); `live`
+// (device, NULL: unused): only variants < *live are computed (launch sized for all).
 int adjoint_jacobian(const mko_batch* batch, const mko_residual_settings* settings,
                      const mko_variables* vars, double* jacobian, double* r0, uint8_t* status,
-                     int64_t stride, void* stream);
+                     int64_t stride, void* stream, const int32_t* live = nullptr);
+
+// mko_gram_device (the bitwise FMA chain) with the runs limited to *live (device, NULL: unused).
+int gram_chain(const double* jacobian, const double* r0, int runs, int variables, int64_t rows,
+               double* a, double* g, void* stream, const int32_t* live);
 
 // mko_fd_difference_device with J column (i, p) at jacobian + (i * P + p) * jstride + joffset,
 // r0 of run i at r0 + i * jstride + joffset; status / status_changes may be NULL.

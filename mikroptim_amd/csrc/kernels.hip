@@ -1342,6 +1342,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
                                                          double* r0, uint8_t* status) {
     __shared__ SurfLds L;
     __shared__ double red[kWaves];
+    if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
     const Tuple t = tuple_of(P, blockIdx.x);
     stage_surfaces(P, t, L);
     const size_t vf = size_t(t.v) * P.F + t.f;
@@ -2136,13 +2137,15 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
 // ascending order starting from 0 (gram_dot, normal_equations.hpp:19), products of the stored
 // J values: bitwise the host definition.
 __global__ __launch_bounds__(256) void gram_kernel(const double* J, const double* r0, int runs, int P,
-                                                   int64_t rows, double* a, double* g) {
+                                                   int64_t rows, double* a, double* g,
+                                                   const int32_t* live) {
     constexpr int kT = 16, kC = 64;
     __shared__ double As[kT][kC + 1];
     __shared__ double Bs[kT][kC + 1];
     const int T = (P + 1 + kT - 1) / kT;
     const int tiles = T * (T + 1) / 2;
     const int run = blockIdx.x / tiles;
+    if (live && run >= *live) return;
     int tile = blockIdx.x - run * tiles;
     int tq = 0;
     while (tile >= T - tq) {
@@ -2959,9 +2962,10 @@ namespace mko_internal {
 
 int adjoint_jacobian(const mko_batch* batch, const mko_residual_settings* settings,
                      const mko_variables* vars, double* jacobian, double* r0, uint8_t* status,
-                     int64_t stride, void* stream) {
+                     int64_t stride, void* stream, const int32_t* live) {
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
+        P.live = live;
         if (stride > 0) {
             if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
             P.stride = stride;
@@ -2980,6 +2984,18 @@ int adjoint_jacobian(const mko_batch* batch, const mko_residual_settings* settin
         if (P.soft) launch_adjoint_rpl<true, double, double, double>(rpl, P, m, Jd, r0, status, s);
         else launch_adjoint_rpl<false, double, double, double>(rpl, P, m, Jd, r0, status, s);
         check_hip(hipGetLastError(), "adjoint launch");
+    })
+}
+
+int gram_chain(const double* jacobian, const double* r0, int runs, int variables, int64_t rows,
+               double* a, double* g, void* stream, const int32_t* live) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        const int T = (variables + 1 + 15) / 16;
+        const dim3 grid(unsigned(runs) * unsigned(T * (T + 1) / 2));
+        hipLaunchKernelGGL(gram_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
+                           jacobian, r0, runs, variables, rows, a, g, live);
+        check_hip(hipGetLastError(), "gram launch");
     })
 }
 
@@ -3090,15 +3106,7 @@ int mko_vjp_device(const mko_batch* batch, const mko_residual_settings* settings
 
 int mko_gram_device(const double* jacobian, const double* r0, int runs, int variables,
                     int64_t rows, double* a, double* g, void* stream) {
-    MKO_TRY({
-        if (runs <= 0) return 0;
-        const int T = (variables + 1 + 15) / 16;
-        const dim3 grid(unsigned(runs) * unsigned(T * (T + 1) / 2));
-        const hipStream_t s = static_cast<hipStream_t>(stream);
-        hipLaunchKernelGGL(gram_kernel, grid, dim3(256), 0, s, jacobian, r0, runs, variables, rows,
-                           a, g);
-        check_hip(hipGetLastError(), "gram launch");
-    })
+    return mko_internal::gram_chain(jacobian, r0, runs, variables, rows, a, g, stream, nullptr);
 }
 
 } // extern "C"

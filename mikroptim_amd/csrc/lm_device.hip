@@ -384,6 +384,72 @@ __global__ __launch_bounds__(1024) void select_kernel(State S, int kind, int max
     if (tid == 0) *count = base_s;
 }
 
+// Stable split of the Jacobian list `in` (*in_count runs) in one workgroup of 1024 threads: the
+// runs whose point has no failed ray into `clean` (in order, count *clean_count), the others
+// into `other` (in order, count *other_count).
+__global__ __launch_bounds__(1024) void split_kernel(State S, const int32_t* in,
+                                                     const int32_t* in_count, int32_t* clean,
+                                                     int32_t* clean_count, int32_t* other,
+                                                     int32_t* other_count) {
+    __shared__ int32_t wave_clean[16], wave_other[16];
+    __shared__ int32_t base_c, base_o;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int total = *in_count;
+    if (tid == 0) base_c = base_o = 0;
+    __syncthreads();
+    for (int start = 0; start < total; start += 1024) {
+        const int e = start + tid;
+        const int run = e < total ? in[e] : -1;
+        const bool c = run >= 0 && S.failed[run] == 0;
+        const bool o = run >= 0 && !c;
+        const uint64_t mc = __ballot(c), mo = __ballot(o);
+        const uint64_t below = (uint64_t(1) << lane) - 1;
+        if (lane == 0) {
+            wave_clean[wave] = __popcll(mc);
+            wave_other[wave] = __popcll(mo);
+        }
+        __syncthreads();
+        int oc = base_c, oo = base_o;
+        for (int w = 0; w < wave; ++w) {
+            oc += wave_clean[w];
+            oo += wave_other[w];
+        }
+        if (c) clean[oc + __popcll(mc & below)] = run;
+        if (o) other[oo + __popcll(mo & below)] = run;
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 0; w < 16; ++w) {
+                base_c += wave_clean[w];
+                base_o += wave_other[w];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *clean_count = base_c;
+        *other_count = base_o;
+    }
+}
+
+// Number of runs the next Jacobian list holds (kListJacobian's test) whose point has failed rays.
+__global__ __launch_bounds__(1024) void count_failing_kernel(State S, int max_iterations,
+                                                             int32_t* count) {
+    __shared__ int32_t wave_total[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int n = 0;
+    for (int run = tid; run < S.n; run += 1024)
+        n += S.status[run] == MKO_LM_RUNNING && S.need_jacobian[run] &&
+             S.iterations[run] < max_iterations && S.failed[run] > 0;
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if (lane == 0) wave_total[wave] = n;
+    __syncthreads();
+    if (tid == 0) {
+        int sum = 0;
+        for (int w = 0; w < 16; ++w) sum += wave_total[w];
+        *count = sum;
+    }
+}
+
 __global__ __launch_bounds__(64) void gather_kernel(State S, const int32_t* list, int count,
                                                     const int32_t* dcount, int trial,
                                                     const int32_t* round_index, double* xs,
@@ -582,6 +648,17 @@ void launch_select(const State& s, int kind, int max_iterations, const int32_t* 
                    void* stream) {
     hipLaunchKernelGGL(select_kernel, dim3(1), dim3(1024), 0, st(stream), s, kind, max_iterations,
                        in, in_count, list, count, pos);
+}
+
+void launch_split(const State& s, const int32_t* in, const int32_t* in_count, int32_t* clean,
+                  int32_t* clean_count, int32_t* other, int32_t* other_count, void* stream) {
+    hipLaunchKernelGGL(split_kernel, dim3(1), dim3(1024), 0, st(stream), s, in, in_count, clean,
+                       clean_count, other, other_count);
+}
+
+void launch_count_failing(const State& s, int max_iterations, int32_t* count, void* stream) {
+    hipLaunchKernelGGL(count_failing_kernel, dim3(1), dim3(1024), 0, st(stream), s,
+                       max_iterations, count);
 }
 
 void launch_gather(const State& s, const int32_t* list, int count, bool trial,

@@ -49,6 +49,12 @@ void launch_start(const State& s, const double* merit, const int64_t* failed, co
 void launch_select(const State& s, int kind, int max_iterations, const int32_t* in,
                    const int32_t* in_count, int32_t* list, int32_t* count, int32_t* pos,
                    void* stream);
+// Stable split of the list `in` (count *in_count): runs with failed == 0 into `clean` (count
+// *clean_count), the others into `other` (count *other_count), each in the order of `in`.
+void launch_split(const State& s, const int32_t* in, const int32_t* in_count, int32_t* clean,
+                  int32_t* clean_count, int32_t* other, int32_t* other_count, void* stream);
+// *count = runs that kListJacobian would list now and whose point has failed rays.
+void launch_count_failing(const State& s, int max_iterations, int32_t* count, void* stream);
 // xs[j] = (trial ? state.trial : state.x)[list[j]], tidx[j] = round_index[list[j]].
 void launch_gather(const State& s, const int32_t* list, int count, bool trial,
                    const int32_t* round_index, double* xs, int32_t* tidx, void* stream,

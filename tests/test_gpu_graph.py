@@ -92,8 +92,10 @@ def test_graph_kernel_timing_counts_only_launches_with_work(engine):
     assert all(r.iterations <= 6 for r in res.runs)
 
 
-@pytest.mark.parametrize("case", ["fd", "soft", "rows"])
+@pytest.mark.parametrize("case", ["fd", "rows", "soft"])
 def test_lms_outside_the_graph_run_eagerly(engine, case):
+    """FD Jacobians and constraint rows keep the eager cycle; soft failure mode with the mixed
+    adjoint replays the captured cycle (its Jacobian list split on the device)."""
     from mikroptim_amd import batch as bt
     base, v, s, x0 = _objective(2)
     method, precision = abi.JACOBIAN_ADJOINT, "mixed"
@@ -107,20 +109,28 @@ def test_lms_outside_the_graph_run_eagerly(engine, case):
                      precision=precision)
     while sess.cycle():
         pass
-    assert sess.graph_cycles() == 0
+    if case == "soft":
+        assert sess.graph_cycles() > 0
+    else:
+        assert sess.graph_cycles() == 0
     sess.close()
 
 
 def test_uneven_groups_concurrent_equal_sequential(engine):
     """optimize_groups begins every group's cycle before waiting for any: the captured cycles of
     the groups run concurrently on their own streams; each group's result equals optimize() on
-    it bit for bit, graph groups and eager groups (soft failure mode) alike."""
+    it bit for bit, graph groups (hard and soft failure mode) and eager groups (constraint rows)
+    alike."""
+    from mikroptim_amd import batch as bt
     groups = []
-    for i in range(5):
+    for i in range(6):
         b, v = systems.uneven_topology(i, variants=3)
-        # the last group in soft failure mode: outside the graph, cycled eagerly
-        failure = abi.FAILURE_SOFT if i == 4 else abi.FAILURE_HARD
-        groups.append((b.take([0]), systems.uneven_settings(failure), v, v.extract(b)))
+        # group 4 in soft failure mode (captured cycle with the device split); group 5 with a
+        # constraint row: outside the graph, cycled eagerly
+        s = systems.uneven_settings(abi.FAILURE_SOFT if i == 4 else abi.FAILURE_HARD)
+        if i == 5:
+            s.constraints = [bt.focal_length(50.0, 1e-3)]
+        groups.append((b.take([0]), s, v, v.extract(b)))
     opt = abi.lm_options(max_iterations=6, threads=4, jacobian_method=abi.JACOBIAN_ADJOINT)
     together = engine.optimize_groups(groups, opt, precision="mixed")
     for (base, s, v, x0), r in zip(groups, together):
@@ -134,5 +144,5 @@ def test_uneven_groups_concurrent_equal_sequential(engine):
         begun = [(sess, sess.cycle_begin()) for sess in active]
         active = [sess for sess, rc in begun if rc != 0 and sess.cycle_end()]
     for (base, s, v, x0), sess in zip(groups, sessions):
-        assert (sess.graph_cycles() > 0) == (s.failure == abi.FAILURE_HARD)
+        assert (sess.graph_cycles() > 0) == (not s.constraints)
         sess.close()

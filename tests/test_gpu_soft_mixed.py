@@ -11,6 +11,7 @@ import pytest
 
 from mikroptim_amd import abi, systems
 from oracle_lib import reference
+from test_gpu_configs import U_BF, U_F
 
 pytestmark = pytest.mark.gpu
 
@@ -74,3 +75,55 @@ def test_soft_mixed_failure_free_objective_is_hard_mode(engine):
     rh = engine.optimize(base, hard, v, x0, abi.lm_options(**kw), precision="mixed")
     assert all(r.failed_rays == 0 for r in rs.runs)
     assert np.array_equal(rs.x, rh.x) and np.array_equal(rs.merit, rh.merit)
+
+
+def test_soft_split_with_constraint_rows(engine):
+    """Soft split with extrema and paraxial constraint rows: the failing runs' normal equations
+    are the FP64 soft path's bit for bit; the clean runs' (fused trace rows, constraint rows by
+    central differences under the soft settings) are the FP64 soft path's within the
+    mixed-precision bound of their trace rows."""
+    b, v = _mixed_batch()
+    s = systems.singlet_settings(abi.FAILURE_SOFT)
+    s.constraints = systems.singlet_constraints()
+    x = v.extract(b)
+    _, f, _ = engine.merit(b, s)
+    failed, clean = np.flatnonzero(f > 0), np.flatnonzero(f == 0)
+    assert len(failed) == 2
+    _, _, a, g = engine.jacobian(b, s, v, x, method="adjoint", precision="mixed",
+                                 want_jacobian=False)
+    J, r0, a64, g64 = engine.jacobian(b, s, v, x, method="adjoint", precision="fp64")
+    assert np.array_equal(a[failed], a64[failed]) and np.array_equal(g[failed], g64[failed])
+    nJ = np.linalg.norm(J.reshape(len(x), -1), axis=1)[clean]
+    nr = np.linalg.norm(r0, axis=1)[clean]
+    eps_a = 3 * U_BF ** 2 + 512 * U_F + 2 * U_F
+    assert np.all(np.linalg.norm(a[clean] - a64[clean], axis=(1, 2)) <= eps_a * nJ ** 2)
+    assert np.all(np.linalg.norm(g[clean] - g64[clean], axis=1) <= 4 * U_F * nJ * nr)
+
+
+def test_soft_graph_cycles_equal_eager_cycles(engine):
+    """The captured soft-mode cycle (device split of the Jacobian list, fused kernel for the runs
+    without failed rays, FP64 stored J for the others) takes the eager cycle's steps bit for bit,
+    through the penalty rounds, and replays while runs have failed rays."""
+    from mikroptim_amd.engine import TraceEngine
+
+    b, v = _mixed_batch()
+    s = systems.singlet_settings(abi.FAILURE_SOFT)
+    x0 = v.extract(b)
+    kw = dict(max_iterations=8, jacobian_method=abi.JACOBIAN_ADJOINT)
+    g = engine.lm(b, s, v, x0, abi.lm_options(**kw), precision="mixed")
+    other = TraceEngine(0)
+    try:
+        e = other.lm(b, s, v, x0, abi.lm_options(graph=False, **kw), precision="mixed")
+        while True:
+            more_g, more_e = g.cycle(), e.cycle()
+            assert more_g == more_e
+            rg, re_ = g.results(), e.results()
+            assert np.array_equal(rg.x, re_.x) and np.array_equal(rg.merit, re_.merit)
+            if not more_g:
+                break
+        assert g.graph_cycles() > 0 and e.graph_cycles() == 0
+        assert any(r.penalty_rounds > 0 for r in rg.runs)
+        e.close()
+    finally:
+        g.close()
+        other.close()
