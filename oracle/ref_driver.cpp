@@ -6,6 +6,7 @@
 // of the reference, so tests can compare the oracle's restatement with the original bit for bit.
 
 #include "mikroptim_hip.h"
+#include "mikroptim_hip_reference.hpp"
 #include "mikroptim_types.h"
 
 #include "mikroptim/batch/batch_merit.hpp"
@@ -240,86 +241,9 @@ mko::LmOptions lm_options_of(const mko_lm_options& o, const mko_residual_setting
 }
 
 // ---- the reference's LM plug-in over libmikroptim_hip.so -------------------------------------
-// HipLmBackend is the adapter a maintainer adds on the reference side: mko::LmBackend
-// (include/mikroptim/optim/levenberg_marquardt.hpp:171-182) with its two pure virtuals answered by
-// the library's mko_engine_normal_equations / mko_engine_point_merit. The reference's own
-// optimize_lm_rounds (levenberg_marquardt.hpp:280) drives it. The library's entry points come in
-// as function pointers, so this driver does not link against it.
-
-struct HipCalls {
-    decltype(&mko_engine_normal_equations) normal_equations;
-    decltype(&mko_engine_point_merit) point_merit;
-    decltype(&mko_last_error) last_error;
-};
-
-class HipLmBackend final : public mko::LmBackend {
-public:
-    HipLmBackend(const HipCalls& calls, mko_engine* engine, const mko::SystemBatchHost& base,
-                 const mko_variables& vars, const mko::LmOptions& o, int precision)
-        : calls_(calls), engine_(engine), vars_(vars), precision_(precision) {
-        if (o.jacobian.extra.count != 0 || o.jacobian.trace_extra.count != 0)
-            throw std::invalid_argument("HipLmBackend: constraint rows are not forwarded here");
-        const mko::BatchTopology& t = base.topology();
-        for (mko::SurfaceKind k : base.type_data()) type_.push_back(uint8_t(k));
-        for (mko::PupilMode m : t.pupil_mode) mode_.push_back(uint8_t(m));
-        b_.variants = base.variants();
-        b_.surfaces = base.surfaces();
-        b_.wavelengths = base.wavelengths();
-        b_.fields = base.fields();
-        b_.source = t.source == mko::SourceKind::Point ? MKO_SOURCE_POINT : MKO_SOURCE_COLLIMATED;
-        b_.reference_sphere = int32_t(t.reference_sphere);
-        b_.reference_wavelength_nm = base.reference_wavelength_nm;
-        for (int i = 0; i < 3; ++i) b_.source_point[i] = base.source_point[size_t(i)];
-        b_.source_na = base.source_na;
-        b_.type = type_.data();
-        b_.pupil_mode = mode_.data();
-        b_.wavelength_weight = base.wavelength_weight_data().data();
-        b_.field_weight = base.field_weight_data().data();
-        b_.curvature = base.curvature_data().data();
-        b_.thickness = base.thickness_data().data();
-        b_.index = base.index_data().data();
-        b_.pupil_radius = base.pupil_radius_data().data();
-        b_.field_angle = base.field_angle_data().data();
-        b_.field_target = base.field_target_data().data();
-        b_.reference_curvature =
-            b_.reference_sphere != MKO_REF_LAST_SURFACE ? base.reference_curvature_data().data()
-                                                        : nullptr;
-        b_.paraxial_index = base.paraxial_index_data().data();
-        const mko::ResidualSettings& q = o.jacobian.quadrature;
-        s_.rings = q.rings;
-        s_.spokes = q.spokes;
-        s_.failure = q.failure == mko::FailureMode::Hard ? MKO_FAILURE_HARD : MKO_FAILURE_SOFT;
-        s_.fail_weight = q.fail_weight;
-        s_.precision = MKO_PRECISION_FP64;
-        s_.extra = nullptr;
-        method_ = o.jacobian_method == mko::JacobianMethod::Adjoint ? MKO_JACOBIAN_ADJOINT
-                                                                    : MKO_JACOBIAN_FD;
-    }
-    void normal_equations(std::span<const int> runs, std::span<const double> x,
-                          std::span<double> a, std::span<double> g) override {
-        check(calls_.normal_equations(engine_, &b_, &s_, &vars_, int(runs.size()), runs.data(),
-                                      x.data(), method_, precision_, a.data(), g.data()));
-    }
-    void merit(std::span<const int> runs, std::span<const double> x, std::span<double> merit,
-               std::span<std::int64_t> failed, std::span<std::int64_t> hard_failed) override {
-        check(calls_.point_merit(engine_, &b_, &s_, &vars_, int(runs.size()), runs.data(),
-                                 x.data(), merit.data(), failed.data(), hard_failed.data()));
-    }
-
-private:
-    void check(int rc) const {
-        if (rc == MKO_ERR_INVALID) throw std::invalid_argument(calls_.last_error());
-        if (rc != 0) throw std::runtime_error(calls_.last_error());
-    }
-    HipCalls calls_;
-    mko_engine* engine_;
-    mko_variables vars_;
-    int precision_;
-    int method_ = MKO_JACOBIAN_ADJOINT;
-    std::vector<uint8_t> type_, mode_;
-    mko_batch b_{};
-    mko_residual_settings s_{};
-};
+// mko::hip::HipLmBackend (include/mikroptim_hip_reference.hpp) is the adapter a maintainer adds on
+// the reference side; the reference's own optimize_lm_rounds drives it. The library's entry
+// points come in as function pointers (HipApi), so this driver does not link against it.
 
 void copy_results(const mko::LmResult& r, int runs, size_t P, double* x_out, mko_lm_run* run_out,
                   int* cycles_out) {
@@ -588,22 +512,43 @@ int ref_optimize_lm_hip_backend(void* const* calls, void* engine, const mko_batc
                                 int precision, double* x_out, mko_lm_run* run_out,
                                 int* cycles_out) {
     REF_TRY({
-        HipCalls c{};
-        c.normal_equations = reinterpret_cast<decltype(c.normal_equations)>(calls[0]);
-        c.point_merit = reinterpret_cast<decltype(c.point_merit)>(calls[1]);
-        c.last_error = reinterpret_cast<decltype(c.last_error)>(calls[2]);
+        mko::hip::HipApi api{};
+        api.normal_equations = reinterpret_cast<decltype(api.normal_equations)>(calls[0]);
+        api.point_merit = reinterpret_cast<decltype(api.point_merit)>(calls[1]);
+        api.last_error = reinterpret_cast<decltype(api.last_error)>(calls[2]);
         const mko::SystemBatchHost h = batch_of(*base);
         const mko::VariableSet set(description_of(*base, *vars), h.topology());
         const mko::LmOptions o = lm_options_of(*options, *s);
-        mko_engine* eng = static_cast<mko_engine*>(engine);
-        const mko_variables v = *vars;
-        const mko::LmBackendFactory make = [&](const mko::LmOptions& ro,
-                                               const mko::SystemBatchHost& rb) {
-            return std::unique_ptr<mko::LmBackend>(new HipLmBackend(c, eng, rb, v, ro, precision));
-        };
-        const mko::LmResult r = mko::optimize_lm_rounds(
-            make, set, h, std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o);
+        // the caller's variable description (its bounds drive the FD patterns exactly as in the
+        // library's own LM); the LM control takes its bounds from `set`
+        const mko::LmResult r = mko::hip::optimize_lm_hip(
+            static_cast<mko_engine*>(engine), set, h,
+            std::span<const double>(x0, size_t(runs) * size_t(vars->count)), o, precision, api,
+            vars);
         copy_results(r, runs, size_t(vars->count), x_out, run_out, cycles_out);
+    })
+}
+
+// The variable description the reference-side adapter derives from the reference's VariableSet
+// (mko::hip::VariablesView over description_of(base, vars)): type / batch_surface / component
+// [P], lower / upper / typical [P]; count = its P.
+int ref_variables_view(const mko_batch* base, const mko_variables* vars, int32_t* count,
+                       int32_t* type, int32_t* surface, int32_t* component, double* lower,
+                       double* upper, double* typical) {
+    REF_TRY({
+        const mko::SystemBatchHost h = batch_of(*base);
+        const mko::VariableSet set(description_of(*base, *vars), h.topology());
+        const mko::hip::VariablesView view(set);
+        const mko_variables& v = view.c();
+        *count = v.count;
+        for (int k = 0; k < v.count && k < vars->count; ++k) {
+            type[k] = v.type[k];
+            surface[k] = v.batch_surface[k];
+            component[k] = v.component ? v.component[k] : 0;
+            lower[k] = v.lower[k];
+            upper[k] = v.upper[k];
+            typical[k] = v.typical[k];
+        }
     })
 }
 

@@ -248,6 +248,20 @@ class Lib:
                      C.byref(cycles)))
         return x, list(res), cycles.value
 
+    def variables_view(self, base, variables):
+        """The reference-side adapter's variable description (mko::hip::VariablesView of the
+        reference's VariableSet built from this description; reference build only)."""
+        P = variables.size
+        out = {k: np.zeros(P, dtype=np.int32) for k in ("type", "surface", "component")}
+        out.update({k: np.zeros(P) for k in ("lower", "upper", "typical")})
+        count = C.c_int32(0)
+        self.check(self.fn("variables_view")(
+            C.byref(base.c_struct()), C.byref(variables.c_struct()), C.byref(count),
+            *[abi.ptr(out[k], C.c_int32) for k in ("type", "surface", "component")],
+            *[abi.ptr(out[k]) for k in ("lower", "upper", "typical")]))
+        out["count"] = count.value
+        return out
+
     def gauss_legendre(self, n):
         u = np.zeros(n)
         w = np.zeros(n)
