@@ -120,7 +120,9 @@ enum {
     MKO_CON_MIN_CENTER_THICKNESS = 8,
     MKO_CON_MAX_CENTER_THICKNESS = 9,
     MKO_CON_MIN_EDGE_THICKNESS = 10,
-    MKO_CON_MAX_INCIDENCE_ANGLE = 11
+    MKO_CON_MAX_INCIDENCE_ANGLE = 11,
+    MKO_CON_SINE_CONDITION = 12,
+    MKO_CON_DISTORTION = 13
 };
 enum { MKO_MODE_EQUAL = 0, MKO_MODE_MIN = 1, MKO_MODE_MAX = 2 };
 enum { MKO_EDGE_SEMI_DIAMETER = 0, MKO_EDGE_RAYS = 1 };
@@ -129,8 +131,12 @@ enum { MKO_EDGE_SEMI_DIAMETER = 0, MKO_EDGE_RAYS = 1 };
  * mechanical_constraints.hpp:38). Surfaces are table (batch) surfaces.
  *   paraxial:   r = sqrt(w) * mode(q) with q = f_s (1 - f_s phi) | offset + sum_{j>=surface}
  *               t_j - L_s | t_{S-1} - w_s | a (A + t_{S-1} C / n'_L) | |kappa_P| - kappa_max
- *               (target = f_s, L_s, w_s, 0, kappa_max; mode Min: min(q - shift, 0), Max:
- *               max(q + shift, 0))
+ *               | OSC(zone) (sine condition, 0 where undefined) | |D_field| - D_max
+ *               (distortion, -D_max where undefined)
+ *               (target = f_s, L_s, w_s, 0, kappa_max, 0, D_max; mode Min: min(q - shift, 0),
+ *               Max: max(q + shift, 0)). The sine condition and distortion trace one real ray
+ *               per row at lambda_ref (paraxial_index): the zone's meridional ray of the axis
+ *               bundle, the field's chief ray (collimated source only).
  *   mechanical: r = sqrt(w) * max(0, g + shift), g = value - limit (max semi-diameter, max
  *               center thickness) or limit - value (others); target = limit (mm, or
  *               cos(theta_max) for the incidence angle), offset = clearance. */
@@ -145,6 +151,8 @@ typedef struct mko_constraint {
     double semi_diameter, next_semi_diameter; /* edge thickness at fixed heights [mm] */
     int32_t edge_height;  /* MKO_EDGE_* (min edge thickness) */
     double shift;         /* augmented-Lagrangian shift theta >= 0 (0: plain penalty) */
+    double zone;          /* sine condition: pupil zone rho in (0, 1] */
+    int32_t field;        /* distortion: field f >= 0 (rows of fields >= F are inactive) */
 } mko_constraint;
 
 /* The rows of a residual vector after the ray rows: first the items that need no trace

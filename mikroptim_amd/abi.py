@@ -24,6 +24,7 @@ PRECISION_FP64, PRECISION_MIXED, PRECISION_DF64, PRECISION_FP64_MFMA = 0, 1, 2, 
  CON_PETZVAL) = range(5)
 (CON_MAX_SEMI_DIAMETER, CON_MIN_CENTER_THICKNESS, CON_MAX_CENTER_THICKNESS,
  CON_MIN_EDGE_THICKNESS, CON_MAX_INCIDENCE_ANGLE) = range(7, 12)
+CON_SINE_CONDITION, CON_DISTORTION = 12, 13
 MODE_EQUAL, MODE_MIN, MODE_MAX = 0, 1, 2
 EDGE_SEMI_DIAMETER, EDGE_RAYS = 0, 1
 
@@ -53,7 +54,8 @@ class Constraint(C.Structure):
         ("type", C.c_int32), ("mode", C.c_int32), ("weight", C.c_double), ("target", C.c_double),
         ("surface", C.c_int32), ("next_surface", C.c_int32), ("offset", C.c_double),
         ("semi_diameter", C.c_double), ("next_semi_diameter", C.c_double),
-        ("edge_height", C.c_int32), ("shift", C.c_double),
+        ("edge_height", C.c_int32), ("shift", C.c_double), ("zone", C.c_double),
+        ("field", C.c_int32),
     ]
 
 

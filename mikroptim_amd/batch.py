@@ -136,6 +136,8 @@ class Constraint:
     next_semi_diameter: float = 0.0
     edge_height: int = abi.EDGE_SEMI_DIAMETER
     shift: float = 0.0
+    zone: float = 0.0   # sine condition: pupil zone rho in (0, 1]
+    field: int = 0      # distortion: field index
 
     @property
     def needs_extrema(self):
@@ -145,7 +147,8 @@ class Constraint:
     def c_struct(self):
         return abi.Constraint(self.type, self.mode, self.weight, self.target, self.surface,
                               self.next_surface, self.offset, self.semi_diameter,
-                              self.next_semi_diameter, self.edge_height, self.shift)
+                              self.next_semi_diameter, self.edge_height, self.shift, self.zone,
+                              self.field)
 
 
 def focal_length(target, weight=1.0):
@@ -167,6 +170,19 @@ def paraxial_focus(weight=1.0):
 
 def petzval(kappa_max, weight=1.0, shift=0.0):
     return Constraint(abi.CON_PETZVAL, weight, kappa_max, abi.MODE_MAX, shift=shift)
+
+
+def sine_condition(zone, weight=1.0):
+    """OSC (Conrady) of the meridional ray of the axis bundle at pupil zone rho = zone, target 0
+    (one row per zone; make_paraxial_constraints uses the zones 1/sqrt(2) and 1 by default)."""
+    return Constraint(abi.CON_SINE_CONDITION, weight, 0.0, zone=zone)
+
+
+def distortion(max_distortion, field, weight=1.0, shift=0.0):
+    """max(0, |D_f| - D_max) of field f, D_f = (y_f - h_f) / h_f with y_f the real chief ray's
+    height in the image plane (one row per field of the source)."""
+    return Constraint(abi.CON_DISTORTION, weight, max_distortion, abi.MODE_MAX, field=field,
+                      shift=shift)
 
 
 def max_semi_diameter(surface, limit, weight=1.0, clearance=0.0, shift=0.0):

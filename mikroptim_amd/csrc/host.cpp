@@ -173,7 +173,10 @@ void validate_values(const mko_batch& b, int first, int count) {
 
 // ---- constraint rows (ExtraResiduals / TraceExtraResiduals) -----------------------------------
 
-inline bool con_paraxial(int type) { return type >= MKO_CON_FOCAL_LENGTH && type <= MKO_CON_PETZVAL; }
+inline bool con_paraxial(int type) {
+    return (type >= MKO_CON_FOCAL_LENGTH && type <= MKO_CON_PETZVAL) ||
+           type == MKO_CON_SINE_CONDITION || type == MKO_CON_DISTORTION;
+}
 inline bool con_mechanical(int type) {
     return type >= MKO_CON_MAX_SEMI_DIAMETER && type <= MKO_CON_MAX_INCIDENCE_ANGLE;
 }
@@ -185,9 +188,12 @@ inline bool con_needs_extrema(const mko_constraint& c) {
     return c.type == MKO_CON_MAX_SEMI_DIAMETER || c.type == MKO_CON_MAX_INCIDENCE_ANGLE ||
            (c.type == MKO_CON_MIN_EDGE_THICKNESS && c.edge_height == MKO_EDGE_RAYS);
 }
+// rows that read the batch's paraxial_index: the paraxial trace (ParaxialConstraints::
+// needs_trace) and the real rays at lambda_ref of the sine condition and distortion
 inline bool con_needs_paraxial(const mko_constraint& c) {
     return c.type == MKO_CON_FOCAL_LENGTH || c.type == MKO_CON_PARAXIAL_FOCUS ||
-           c.type == MKO_CON_PETZVAL;
+           c.type == MKO_CON_PETZVAL || c.type == MKO_CON_SINE_CONDITION ||
+           c.type == MKO_CON_DISTORTION;
 }
 
 // check_paraxial_constraints (src/optim/paraxial_residuals.cpp:152-255) and
@@ -219,6 +225,19 @@ void check_constraint(const mko_constraint& c, int k, const mko_batch& b) {
         case MKO_CON_PETZVAL:
             if (c.mode != MKO_MODE_MAX) fail("defined as an upper bound only");
             if (c.target < 0.0) fail("kappa_max must be >= 0");
+            break;
+        case MKO_CON_SINE_CONDITION:
+            if (c.mode != MKO_MODE_EQUAL) fail("defined as an equation only");
+            if (c.target != 0.0) fail("target must be 0");
+            if (!(std::isfinite(c.zone) && c.zone > 0.0 && c.zone <= 1.0))
+                fail("pupil zone must lie in (0, 1]");
+            if (b.source != MKO_SOURCE_COLLIMATED) fail("needs a collimated source");
+            break;
+        case MKO_CON_DISTORTION:
+            if (c.mode != MKO_MODE_MAX) fail("defined as an upper bound only");
+            if (c.target < 0.0) fail("D_max must be >= 0");
+            if (c.field < 0) fail("field must be >= 0");
+            if (b.source != MKO_SOURCE_COLLIMATED) fail("needs a collimated source");
             break;
         default:
             break;

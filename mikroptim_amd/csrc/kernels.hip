@@ -766,18 +766,25 @@ __global__ void extrema_finalize_kernel(int V, int T, int S, const double* parti
 
 // Constraint rows of every variant (one thread per variant), rows[v * stride + offset + k] for
 // the `count` items in row order; extrema [V][S][2] for the items that need them.
-__global__ void constraint_rows_kernel(int V, int S, const double* curvature,
+__global__ void constraint_rows_kernel(int V, int S, int F, const double* curvature,
                                        const double* thickness, const double* paraxial_index,
-                                       const double* pupil_radius, uint64_t sphere_mask,
+                                       const double* pupil_radius, const double* field_angle,
+                                       const double* field_target, uint64_t sphere_mask,
                                        const mko_constraint* items, int count, int need_paraxial,
                                        const double* extrema, double* rows, int64_t stride,
                                        int64_t offset) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    const mkoc::VariantRows vr{curvature + size_t(v) * S, thickness + size_t(v) * S,
+    const mkoc::VariantRows vr{curvature + size_t(v) * S,
+                               thickness + size_t(v) * S,
                                paraxial_index ? paraxial_index + size_t(v) * S : nullptr,
-                               sphere_mask, pupil_radius[v], S};
-    mkoc::Paraxial p{1.0, 0.0, 0.0, 0.0};
+                               sphere_mask,
+                               pupil_radius[v],
+                               S,
+                               field_angle + size_t(v) * F,
+                               field_target + size_t(v) * F,
+                               F};
+    mkoc::Paraxial p{1.0, 0.0, 0.0, 0.0, false, false, 0.0, 0.0, 0.0};
     if (need_paraxial) p = mkoc::paraxial(vr.c, vr.t, vr.n, S);
     const double* ext = extrema ? extrema + size_t(v) * S * 2 : nullptr;
     double* out = rows + size_t(v) * stride + offset;
@@ -2790,8 +2797,9 @@ int constraint_rows(const mko_batch* batch, const mko_constraint* items, int cou
             throw std::invalid_argument("paraxial constraint rows need the batch's paraxial_index");
         hipLaunchKernelGGL(constraint_rows_kernel, dim3((batch->variants + 127) / 128), dim3(128),
                            0, static_cast<hipStream_t>(stream), batch->variants, batch->surfaces,
-                           batch->curvature, batch->thickness, batch->paraxial_index,
-                           batch->pupil_radius, mask, items, count, need_paraxial, extrema, rows,
+                           batch->fields, batch->curvature, batch->thickness,
+                           batch->paraxial_index, batch->pupil_radius, batch->field_angle,
+                           batch->field_target, mask, items, count, need_paraxial, extrema, rows,
                            stride, offset);
         check_hip(hipGetLastError(), "constraint rows launch");
     })

@@ -15,14 +15,16 @@ catalogue (Sellmeier, polynomial), constant and Buchdahl materials, collimated a
 the three reference spheres with the ideal image point, curvature / thickness / working-distance
 variables, glass variables on Buchdahl and constant media (a constant medium becomes Buchdahl
 with n0 = n, as convert_glass_variables does), paraxial (focal length, parfocal length, working
-distance, paraxial focus, Petzval) and mechanical constraints. Ghost plates, ghost-coefficient
-variables, glass variables on catalogue glasses (they need the Buchdahl fit), the ghost-penalty,
-glass-hull (with glass variables), sine-condition and distortion constraints, smoothed extrema
-and traced reference points are rejected with a FormatError naming the field; tolerances and a
-glass hull without glass variables are ignored with a warning.
+distance, paraxial focus, Petzval, sine condition per zone, distortion per field) and mechanical
+constraints. Ghost plates, ghost-coefficient variables, glass variables on catalogue glasses
+(they need the Buchdahl fit), the ghost-penalty and glass-hull (with glass variables)
+constraints, smoothed extrema and traced reference points are rejected with a FormatError
+naming the field; tolerances and a glass hull without glass variables are ignored with a
+warning.
 """
 
 import json
+import dataclasses
 import math
 import os
 from dataclasses import dataclass, field
@@ -255,10 +257,13 @@ VARIABLE_TYPES = {"curvature": abi.VAR_CURVATURE, "thickness": abi.VAR_THICKNESS
                   "working_distance": abi.VAR_WORKING_DISTANCE, "glass": abi.VAR_GLASS}
 # kDefaultTypical* (include/mikroptim/model/variables.hpp:67-69)
 DEFAULT_TYPICAL = {"curvature": 0.01, "thickness": 1.0, "working_distance": 1.0}
-UNSUPPORTED_CONSTRAINTS = ("ghost_penalty", "sine_condition", "distortion")
+UNSUPPORTED_CONSTRAINTS = ("ghost_penalty",)
 PARAXIAL_TYPES = {"focal_length": abi.CON_FOCAL_LENGTH, "parfocal_length": abi.CON_PARFOCAL_LENGTH,
                   "working_distance": abi.CON_WORKING_DISTANCE,
-                  "paraxial_focus": abi.CON_PARAXIAL_FOCUS, "petzval": abi.CON_PETZVAL}
+                  "paraxial_focus": abi.CON_PARAXIAL_FOCUS, "petzval": abi.CON_PETZVAL,
+                  "sine_condition": abi.CON_SINE_CONDITION, "distortion": abi.CON_DISTORTION}
+# kDefaultSineConditionZones (include/mikroptim/model/constraints.hpp:99)
+DEFAULT_SINE_CONDITION_ZONES = (0.70710678118654752440, 1.0)
 MECHANICAL_TYPES = {"max_semi_diameter": abi.CON_MAX_SEMI_DIAMETER,
                     "min_center_thickness": abi.CON_MIN_CENTER_THICKNESS,
                     "max_center_thickness": abi.CON_MAX_CENTER_THICKNESS,
@@ -522,7 +527,7 @@ def _medium_after(sysf, i):
 
 
 GAP_TYPES = ("min_center_thickness", "max_center_thickness", "min_edge_thickness")
-EQUALITY_ONLY = ("focal_length", "paraxial_focus")
+EQUALITY_ONLY = ("focal_length", "paraxial_focus", "sine_condition")
 MECHANICAL_FIELDS = ("medium", "semi_diameter", "thickness", "angle", "clearance", "height")
 
 
@@ -598,14 +603,14 @@ def _constraints(sec, sysf, origin):
                 raise FormatError(p + ".mode", "unknown mode", origin)
             if t in EQUALITY_ONLY and it["mode"] != "equal":
                 raise FormatError(p + ".mode", f"{t} is an equation only", origin)
-            if t == "petzval" and it["mode"] != "max":
-                raise FormatError(p + ".mode", "petzval is an upper bound only", origin)
+            if t in ("petzval", "distortion") and it["mode"] != "max":
+                raise FormatError(p + ".mode", f"{t} is an upper bound only", origin)
             if mech:
                 raise FormatError(p + ".mode", f"{t} has no mode", origin)
         w = _num(it["weight"], p + ".weight", origin) if "weight" in it else 1.0
         if not (math.isfinite(w) and w >= 0.0):
             raise FormatError(p + ".weight", "weight must be finite and >= 0", origin)
-        mode_name = it.get("mode", "max" if t == "petzval" else "equal")
+        mode_name = it.get("mode", "max" if t in ("petzval", "distortion") else "equal")
         shift = it.get("shift", [])
         if shift:
             if not (mech or mode_name != "equal"):
@@ -623,6 +628,10 @@ def _constraints(sec, sysf, origin):
                     raise FormatError(f"{p}.{key}", "only for parfocal_length", origin)
         if t != "petzval" and "max_curvature" in it:
             raise FormatError(p + ".max_curvature", "only for petzval", origin)
+        if t != "distortion" and "max_distortion" in it:
+            raise FormatError(p + ".max_distortion", "only for distortion", origin)
+        if t != "sine_condition" and it.get("zones"):
+            raise FormatError(p + ".zones", "only for sine_condition", origin)
         if mech:
             gap = t in GAP_TYPES
             forbid = {"semi_diameter": t != "max_semi_diameter", "thickness": not gap,
@@ -686,6 +695,25 @@ def _constraints(sec, sysf, origin):
             target = 0.0
             if sysf.source != "collimated":
                 raise FormatError(p + ".type", "paraxial_focus needs a collimated source", origin)
+        elif t == "distortion":
+            if "max_distortion" not in it:
+                raise FormatError(p + ".max_distortion", "required for distortion (largest "
+                                  "allowed |D| as a fraction)", origin)
+            target = _num(it["max_distortion"], p + ".max_distortion", origin)
+            if not (math.isfinite(target) and target >= 0.0):
+                raise FormatError(p + ".max_distortion", "must be finite and >= 0", origin)
+            if sysf.source != "collimated":
+                raise FormatError(p + ".type", "distortion needs a collimated source", origin)
+        elif t == "sine_condition":
+            target = 0.0
+            zones = [_num(z, f"{p}.zones[{j}]", origin) for j, z in enumerate(it.get("zones", []))]
+            for j, z in enumerate(zones):
+                if not (math.isfinite(z) and 0.0 < z <= 1.0):
+                    raise FormatError(f"{p}.zones[{j}]", "pupil zone must lie in (0, 1]", origin)
+                if z in zones[:j]:
+                    raise FormatError(f"{p}.zones[{j}]", f"pupil zone {z} twice", origin)
+            if sysf.source != "collimated":
+                raise FormatError(p + ".type", "sine_condition needs a collimated source", origin)
         else:
             if t not in sysf.target:
                 raise FormatError(p + ".type", f"target.{t} missing", origin)
@@ -707,6 +735,16 @@ def _constraints(sec, sysf, origin):
             c.offset = float(it.get("offset", 0.0))
         if len(shift) == 1:
             c.shift = shift[0]
+        if t == "sine_condition":
+            # one row per zone, in file order (make_paraxial_constraints)
+            for z in zones or DEFAULT_SINE_CONDITION_ZONES:
+                paraxial.append((k, dataclasses.replace(c, zone=float(z))))
+            continue
+        if t == "distortion":
+            # one row per field of the source; the axis rows stay 0
+            for f in range(len(sysf.field_angles_deg)):
+                paraxial.append((k, dataclasses.replace(c, field=f)))
+            continue
         paraxial.append((k, c))
     return paraxial + mechanical
 

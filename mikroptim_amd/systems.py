@@ -78,6 +78,33 @@ def singlet(variants=1024, seed=1, perturb=0.05):
     return b, variables
 
 
+def concentric(a, gaps, n, image_distance, fields_deg=(0.0,), variants=1):
+    """A system concentric to its stop (the sine-condition and distortion oracle of the
+    reference's tests, tests/test_sine_condition.cpp:58-81): the stop (radius a, in the medium
+    n[0]) and spheres at distances z_k = gaps[0] + ... + gaps[k-1] with their centres in the
+    stop's centre (c_k = -1/z_k), medium n[k] behind sphere k; the last sphere's thickness is
+    image_distance. One wavelength (587.56 nm, constant indices); field targets 0."""
+    S = len(n)
+    fields_deg = tuple(fields_deg)
+    b = SystemBatch([abi.SURFACE_STOP] + [abi.SURFACE_SPHERE] * (S - 1), [587.56], [1.0],
+                    [1.0] * len(fields_deg),
+                    [abi.PUPIL_AXIS if f == 0.0 else abi.PUPIL_HALF for f in fields_deg],
+                    variants, 587.56)
+    z = 0.0
+    c, t = [0.0], [gaps[0]]
+    for k in range(1, S):
+        z += gaps[k - 1]
+        c.append(-1.0 / z)
+        t.append(gaps[k] if k < len(gaps) else image_distance)
+    b.curvature[:] = c
+    b.thickness[:] = t
+    b.index[:, :, 0] = n
+    b.paraxial_index = np.tile(np.asarray(n, dtype=np.float64), (variants, 1))
+    b.pupil_radius[:] = a
+    b.field_angle[:] = np.radians(fields_deg)
+    return b
+
+
 def singlet_multistart(variants=1000, seed=11):
     """Multistart variants of the singlet for the prefilter: curvatures x(1 + 0.5 xi), lens
     thickness and working distance x(1 + 0.3 xi) (xi uniform, perturbed_vector seeding), pupil
@@ -110,6 +137,13 @@ def singlet_variables():
 
 def singlet_settings(failure=abi.FAILURE_HARD):
     return ResidualSettings(rings=6, spokes=12, failure=failure)
+
+
+def singlet_image_constraints(fields=3, max_distortion=0.001):
+    """Sine condition at the zones 1/sqrt(2) and 1 (make_paraxial_constraints' default zones)
+    and the distortion of every field (one row per field, the axis row inactive)."""
+    return ([batch_mod.sine_condition(0.70710678118654752440), batch_mod.sine_condition(1.0)]
+            + [batch_mod.distortion(max_distortion, f) for f in range(fields)])
 
 
 def singlet_constraints():

@@ -609,6 +609,9 @@ void variant_extrema(const Batch& B, int v, const Layout& lay, double* out) {
 // paraxial_trace / petzval_sum, include/mikroptim/paraxial/ynu_trace.hpp:53-93 -------------------
 struct Paraxial {
     double A = 1.0, B = 0.0, C = 0.0, D = 1.0, power = 0.0, petzval_curvature = 0.0;
+    // compute_paraxial (paraxial.cpp:145-190): the quantities of the sine condition
+    bool afocal = false, exit_pupil_at_infinity = false;
+    double focal_length = 0.0, back_focal_distance = 0.0, exit_pupil_position = 0.0;
 };
 
 Paraxial batch_paraxial(const Batch& B, int v) {
@@ -643,7 +646,18 @@ Paraxial batch_paraxial(const Batch& B, int v) {
     p.B = chief.first;
     p.D = n_image * chief.second;
     p.power = -p.C;
-    if (p.C == 0.0) p.power = 0.0;
+    p.afocal = p.C == 0.0;
+    if (p.afocal) {
+        p.power = 0.0;
+        p.focal_length = std::numeric_limits<double>::infinity();
+        p.back_focal_distance = std::numeric_limits<double>::quiet_NaN();
+    } else {
+        p.focal_length = 1.0 / p.power;
+        p.back_focal_distance = -marginal.first / marginal.second;
+    }
+    p.exit_pupil_at_infinity = p.D == 0.0;
+    p.exit_pupil_position = p.exit_pupil_at_infinity ? std::numeric_limits<double>::quiet_NaN()
+                                                     : -chief.first / chief.second;
     double ps = 0.0, nb = 1.0;
     for (int j = 0; j < S; ++j) {
         ps += c[j] * (n[j] - nb) / (nb * n[j]);
@@ -655,7 +669,10 @@ Paraxial batch_paraxial(const Batch& B, int v) {
 
 // ---- constraint rows: paraxial_residual, src/optim/paraxial_residuals.cpp:257-371;
 // mechanical_residual, src/optim/mechanical_constraints.cpp:58-281 -----------------------------
-inline bool con_paraxial(int type) { return type >= MKO_CON_FOCAL_LENGTH && type <= MKO_CON_PETZVAL; }
+inline bool con_paraxial(int type) {
+    return (type >= MKO_CON_FOCAL_LENGTH && type <= MKO_CON_PETZVAL) ||
+           type == MKO_CON_SINE_CONDITION || type == MKO_CON_DISTORTION;
+}
 inline bool con_mechanical(int type) {
     return type >= MKO_CON_MAX_SEMI_DIAMETER && type <= MKO_CON_MAX_INCIDENCE_ANGLE;
 }
@@ -669,7 +686,8 @@ bool con_needs_extrema(const mko_constraint& c) {
 }
 inline bool con_needs_paraxial(const mko_constraint& c) {
     return c.type == MKO_CON_FOCAL_LENGTH || c.type == MKO_CON_PARAXIAL_FOCUS ||
-           c.type == MKO_CON_PETZVAL;
+           c.type == MKO_CON_PETZVAL || c.type == MKO_CON_SINE_CONDITION ||
+           c.type == MKO_CON_DISTORTION;
 }
 
 // check_paraxial_constraints (paraxial_residuals.cpp:152-255), check_mechanical_constraints
@@ -701,6 +719,19 @@ void check_constraint(const mko_constraint& c, int k, const Batch& B) {
         case MKO_CON_PETZVAL:
             if (c.mode != MKO_MODE_MAX) fail("defined as an upper bound only");
             if (c.target < 0.0) fail("kappa_max must be >= 0");
+            break;
+        case MKO_CON_SINE_CONDITION:
+            if (c.mode != MKO_MODE_EQUAL) fail("defined as an equation only");
+            if (c.target != 0.0) fail("target must be 0");
+            if (!(std::isfinite(c.zone) && c.zone > 0.0 && c.zone <= 1.0))
+                fail("pupil zone must lie in (0, 1]");
+            if (B.b.source != MKO_SOURCE_COLLIMATED) fail("needs a collimated source");
+            break;
+        case MKO_CON_DISTORTION:
+            if (c.mode != MKO_MODE_MAX) fail("defined as an upper bound only");
+            if (c.target < 0.0) fail("D_max must be >= 0");
+            if (c.field < 0) fail("field must be >= 0");
+            if (B.b.source != MKO_SOURCE_COLLIMATED) fail("needs a collimated source");
             break;
         default:
             break;
@@ -737,6 +768,52 @@ inline double sag(double curvature, double height) {
     return curvature * h2 / (1.0 + std::sqrt((1.0 - a) * (1.0 + a)));
 }
 
+// A collimated ray at lambda_ref (the variant's table with one wavelength, the paraxial index):
+// collimated_ray(a, theta, rho, 0) traced through surfaces 1 .. S-1 in hard mode
+// (batch_sine_condition, paraxial_residuals.cpp:68-91; batch_chief_ray_height,
+// system_batch.cpp:999-1030). False where the ray fails.
+bool lambda_ref_ray(const Batch& B, int v, double theta, double rho, Ray& ray) {
+    const int S = B.S();
+    const Table tb{S, 1, B.b.type, B.b.curvature + size_t(v) * S, B.b.thickness + size_t(v) * S,
+                   B.b.paraxial_index + size_t(v) * S};
+    const double r = B.b.pupil_radius[v] * rho;
+    ray.p = Vec3{r * std::sin(0.0), r * std::cos(0.0), 0.0};
+    ray.d = Vec3{0.0, std::sin(theta), std::cos(theta)};
+    ray.opl = dot(ray.p, ray.d);
+    return trace_ray(tb, 0, ray, nullptr, nullptr, nullptr) == Ok;
+}
+
+// conrady_osc (include/mikroptim/paraxial/sine_condition.hpp:25-40) of the zone's ray; NaN where
+// undefined
+double sine_condition(const Batch& B, int v, const Paraxial& pd, double rho) {
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    Ray ray;
+    if (!lambda_ref_ray(B, v, 0.0, rho, ray)) return nan;
+    const int S = B.S();
+    const double height = rho * B.b.pupil_radius[v];
+    const Vec3& p = ray.p;
+    const Vec3& d = ray.d;
+    if (d.y == 0.0 || pd.afocal || !std::isfinite(pd.focal_length)) return nan;
+    const double f_image = B.b.paraxial_index[size_t(v) * S + size_t(S - 1)] * pd.focal_length;
+    const double f_real = -height / d.y;
+    double factor = 1.0;
+    if (!pd.exit_pupil_at_infinity) {
+        const double L = p.z - p.y * d.z / d.y;
+        factor = (pd.back_focal_distance - pd.exit_pupil_position) / (L - pd.exit_pupil_position);
+    }
+    return f_real / f_image * factor - 1.0;
+}
+
+// the real chief ray's height in the image plane, NaN where it fails or runs backward
+double chief_ray_height(const Batch& B, int v, int f) {
+    Ray ray;
+    const double theta = B.b.field_angle[size_t(v) * size_t(B.F()) + size_t(f)];
+    if (!lambda_ref_ray(B, v, theta, 0.0, ray) || !(ray.d.z > 0.0))
+        return std::numeric_limits<double>::quiet_NaN();
+    const double w = B.b.thickness[size_t(v) * B.S() + size_t(B.S() - 1)];
+    return ray.p.y + ray.d.y * ((w - ray.p.z) / ray.d.z);
+}
+
 double constraint_row(const mko_constraint& c, const Batch& B, int v, const Paraxial& p,
                       const double* ext) {
     const int S = B.S();
@@ -756,6 +833,22 @@ double constraint_row(const mko_constraint& c, const Batch& B, int v, const Para
             q = B.b.pupil_radius[v] *
                 (p.A + t[S - 1] * p.C / B.b.paraxial_index[size_t(v) * S + size_t(S - 1)]);
             break;
+        case MKO_CON_SINE_CONDITION: {
+            const double osc = sine_condition(B, v, p, c.zone);
+            q = std::isfinite(osc) ? osc : 0.0;
+            break;
+        }
+        case MKO_CON_DISTORTION: {
+            q = -c.target;
+            if (c.field < B.F()) {
+                const double h = B.b.field_target[size_t(v) * size_t(B.F()) + size_t(c.field)];
+                if (h != 0.0) {
+                    const double y = chief_ray_height(B, v, c.field);
+                    if (std::isfinite(y)) q = std::abs((y - h) / h) - c.target;
+                }
+            }
+            break;
+        }
         default: q = std::abs(p.petzval_curvature) - c.target; break;
         }
         if (c.shift != 0.0) {

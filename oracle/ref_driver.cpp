@@ -102,7 +102,8 @@ Extras extras_of(const mko_residual_settings& s) {
     mko::MechanicalConstraints traced;
     for (int k = 0; k < s.extra->count; ++k) {
         const mko_constraint& c = s.extra->items[k];
-        if (c.type <= MKO_CON_PETZVAL) {
+        if (c.type <= MKO_CON_PETZVAL || c.type == MKO_CON_SINE_CONDITION ||
+            c.type == MKO_CON_DISTORTION) {
             mko::ParaxialConstraint pc;
             pc.type = static_cast<mko::ConstraintType>(c.type);
             pc.mode = static_cast<mko::ConstraintMode>(c.mode);
@@ -111,6 +112,8 @@ Extras extras_of(const mko_residual_settings& s) {
             pc.reference_surface = c.surface;
             pc.offset = c.offset;
             pc.shift = c.shift;
+            pc.zone = c.zone;
+            pc.field = c.field;
             mko::ParaxialConstraints one;
             one.items.push_back(pc);
             parts.push_back(mko::make_paraxial_residuals(one));

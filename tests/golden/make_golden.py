@@ -53,6 +53,14 @@ def main():
         J, r0, _ = fn(b.take([0, 1]), soft, v, x0)
         out[f"singlet_{key}_rows_J"] = J[:, :, T:]
         out[f"singlet_{key}_rows_r0"] = r0[:, T:]
+    # sine-condition and distortion rows, and their FD Jacobian columns
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    s.constraints = systems.singlet_image_constraints()
+    out["singlet_image_rows"] = ref.constraint_rows(b, s)
+    J, _, _ = ref.fd_jacobian(b.take([0, 1]), s, v, v.extract(b)[:2])
+    out["singlet_image_fd_J"] = J[:, :, J.shape[2] - len(s.constraints):]
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    s.constraints = systems.singlet_constraints()
     # FP32 prefilter stats (prefilter_stats_cpu) of the multistart singlet batch
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from test_prefilter import stats  # noqa: E402

@@ -301,3 +301,147 @@ def test_lm_with_constraints_equals_reference(method):
         for a, c in zip(ro, rr):
             assert (a.merit, a.status, a.iterations, a.accepted) == \
                    (c.merit, c.status, c.iterations, c.accepted)
+
+
+# ---- sine condition and distortion (paraxial_residuals.cpp:68-91, :290-315) --------------------
+
+SQRT_HALF = 0.70710678118654752440
+# the reference's tolerances (tests/tolerances.hpp:456, :578)
+SINE_CONDITION_EXACT = 1e-14
+CHIEF_RAY_ANALYTIC = 1e-12
+
+
+def _rows_of(batch, constraints, failure=abi.FAILURE_HARD):
+    s = systems.ResidualSettings(rings=3, spokes=4, failure=failure, constraints=constraints)
+    return oracle().constraint_rows(batch, s)
+
+
+def test_sine_condition_concentric_systems_satisfy_it_exactly():
+    """Systems concentric to the stop are free of coma: OSC = 0 up to rounding in every zone and
+    at large aperture, also with glass in image space
+    (SineCondition.ConcentricSystemsSatisfyItExactly, tests/test_sine_condition.cpp:104)."""
+    cases = [(4.0, [10.0, 3.0], [1.0, 1.6, 1.0]), (7.0, [10.0, 4.0], [1.0, 1.9, 1.0]),
+             (5.0, [10.0], [1.0, 1.7]), (4.0, [8.0, 2.0, 3.0], [1.0, 1.75, 1.45, 1.0])]
+    zones = (0.25, 0.5, SQRT_HALF, 1.0)
+    for a, gaps, n in cases:
+        b = systems.concentric(a, gaps, n, 20.0)
+        rows = _rows_of(b, [bt.sine_condition(z) for z in zones])
+        assert np.all(np.abs(rows) <= SINE_CONDITION_EXACT), (a, gaps, n, rows)
+
+
+def test_sine_condition_shifted_stop_violates_it():
+    """SineCondition.ShiftedStopViolatesIt (tests/test_sine_condition.cpp:134): the stop 4 mm
+    closer to the surfaces breaks the concentricity, OSC != 0; and a heavier weight scales the
+    row by its square root."""
+    b = systems.concentric(4.0, [10.0, 3.0], [1.0, 1.6, 1.0], 20.0)
+    b.thickness[0, 0] = 6.0
+    r1 = _rows_of(b, [bt.sine_condition(1.0)])[0, 0]
+    r4 = _rows_of(b, [bt.sine_condition(1.0, weight=4.0)])[0, 0]
+    assert abs(r1) > 1e-3 and r4 == 2.0 * r1
+
+
+def test_sine_condition_small_aperture_is_third_order():
+    """At small aperture the OSC is third-order coma, -S_II / (2 H) with S_II ~ H^3: it grows
+    as the square of the zone, so the zone 1/sqrt(2) has half the rim's OSC
+    (SineCondition.SmallApertureMatchesSeidelComa, tests/test_sine_condition.cpp:162, with its
+    tolerance 2 x 1e-2, in the form that needs no Seidel sums)."""
+    b, _ = systems.singlet(1)
+    b.pupil_radius[:] = 0.1
+    rim, zone = _rows_of(b, [bt.sine_condition(1.0), bt.sine_condition(SQRT_HALF)])[0]
+    assert rim != 0.0 and abs(zone - 0.5 * rim) <= 2.0 * 1e-2 * abs(0.5 * rim)
+
+
+def test_chief_ray_height_through_plates_is_analytic():
+    """Plane plates shift the chief ray only: y = sum t_i tan(theta_i), n_i sin(theta_i) =
+    sin(theta) at lambda_ref (ReferencePoint.ChiefRayHeightThroughPlatesIsAnalytic,
+    tests/test_reference_point.cpp:120), read through the distortion row with D_max = 0."""
+    b, _ = systems.singlet(1)
+    b.curvature[:, 1:] = 0.0
+    fields = np.radians([5.0, 20.0])
+    b.field_angle[0, 1:] = fields
+    n = b.paraxial_index[0, 1]
+    t = b.thickness[0]
+    for k, theta in enumerate(fields, start=1):
+        y = t[0] * math.tan(theta) + t[1] * math.tan(math.asin(math.sin(theta) / n)) \
+            + t[2] * math.tan(theta)
+        h = 0.5 * y
+        b.field_target[0, k] = h
+        row = _rows_of(b, [bt.distortion(0.0, k)])[0, 0]
+        # row = |(y - h) / h| = 1 + (y - 2h) / h
+        assert abs(h * (1.0 + row) - y) <= CHIEF_RAY_ANALYTIC * (1.0 + abs(y))
+
+
+def test_distortion_of_concentric_system_is_analytic():
+    """A system concentric to the stop images the chief ray at L tan(theta), L the distance of the
+    image plane from the stop; with the image plane in the focus (L = f) it is free of distortion,
+    shifted by delta D = delta / f (ReferencePoint.DistortionOfConcentricSystemIsAnalytic,
+    tests/test_reference_point.cpp:157). Axis rows and rows of absent fields are 0."""
+    gaps, n, fields = [10.0, 3.0], [1.0, 1.6, 1.0], (0.0, 4.0, 12.0)
+    probe = systems.concentric(4.0, gaps, n, 20.0, fields)
+    f = 1.0 / oracle().paraxial(probe, 0)[4]
+    w0 = f - sum(gaps)
+    for delta in (0.0, 0.05, -0.2):
+        b = systems.concentric(4.0, gaps, n, w0 + delta, fields)
+        b.field_target[0] = f * np.tan(np.radians(fields))
+        rows = _rows_of(b, [bt.distortion(0.0, k, weight=4.0) for k in range(3)]
+                        + [bt.distortion(0.0, 7, weight=4.0)])[0]
+        assert rows[0] == 0.0 and rows[3] == 0.0
+        for k in (1, 2):
+            assert abs(rows[k] - 2.0 * abs(delta / f)) <= 2.0 * CHIEF_RAY_ANALYTIC
+        limit = 0.5 * abs(delta / f) + 1e-9
+        r = _rows_of(b, [bt.distortion(limit, 2, weight=4.0)])[0, 0]
+        assert abs(r - 2.0 * max(0.0, abs(delta / f) - limit)) <= 2.0 * CHIEF_RAY_ANALYTIC
+
+
+def test_sine_condition_failed_ray_gives_zero_row():
+    """A zone ray that misses a surface has no OSC: its row is 0 (the ray rows penalise the
+    failure; SineCondition.FailedRayGivesZeroResidual, tests/test_sine_condition.cpp:212)."""
+    b, _ = systems.singlet(1)
+    b.pupil_radius[:] = 60.0  # the rim ray misses the first lens surface (radius 50 mm)
+    rim, zone = _rows_of(b, [bt.sine_condition(1.0), bt.sine_condition(0.1)])[0]
+    assert rim == 0.0 and zone != 0.0
+
+
+def test_image_constraints_equal_reference():
+    """Sine-condition and distortion rows, their place in residuals and merit, and their FD /
+    adjoint Jacobian columns are the reference's own values bit for bit."""
+    b, v = systems.singlet(4, seed=7)
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    s.constraints = systems.singlet_image_constraints()
+    rows = oracle().constraint_rows(b, s)
+    assert np.count_nonzero(rows[:, :2]) == 8 and np.all(rows[:, 2] == 0.0)
+    assert np.array_equal(rows, FIXTURES["singlet_image_rows"])
+    x0 = v.extract(b)[:2]
+    J, r0, _ = oracle().fd_jacobian(b.take([0, 1]), s, v, x0)
+    T = J.shape[2] - len(s.constraints)
+    assert np.array_equal(J[:, :, T:], FIXTURES["singlet_image_fd_J"])
+    ref = reference()
+    if ref is not None:
+        assert np.array_equal(rows, ref.constraint_rows(b, s))
+        assert np.array_equal(oracle().merit(b, s)[0], ref.merit(b, s)[0])
+        for fn in ("fd_jacobian", "adjoint_jacobian"):
+            Jo, r0o, _ = getattr(oracle(), fn)(b.take([0, 1]), s, v, x0)
+            Jr, r0r, _ = getattr(ref, fn)(b.take([0, 1]), s, v, x0)
+            assert np.array_equal(Jo, Jr) and np.array_equal(r0o, r0r)
+
+
+def test_invalid_image_constraints_are_rejected():
+    b, _ = systems.singlet(1)
+    s = systems.singlet_settings()
+    bad = [bt.Constraint(abi.CON_SINE_CONDITION, 1.0, 0.0, abi.MODE_MIN, zone=1.0),  # equation
+           bt.Constraint(abi.CON_SINE_CONDITION, 1.0, 0.1, zone=1.0),                # target 0
+           bt.sine_condition(0.0), bt.sine_condition(1.5),                            # zone
+           bt.Constraint(abi.CON_DISTORTION, 1.0, 0.01, abi.MODE_EQUAL, field=1),     # max only
+           bt.distortion(-0.01, 1), bt.distortion(0.01, -1)]
+    for con in bad:
+        s.constraints = [con]
+        with pytest.raises(RuntimeError):
+            oracle().constraint_rows(b, s)
+    pb, _ = systems.singlet(1)
+    pb.source = abi.SOURCE_POINT
+    pb.source_point[:] = [0.0, 0.0, -100.0]
+    pb.source_na = 0.05
+    for con in (bt.sine_condition(1.0), bt.distortion(0.01, 0)):
+        s.constraints = [con]
+        with pytest.raises(RuntimeError):
+            oracle().constraint_rows(pb, s)

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from mikroptim_amd import abi, systems
+from mikroptim_amd import batch as bt
 from oracle_lib import oracle
 
 pytestmark = pytest.mark.gpu
@@ -207,6 +208,94 @@ def test_lm_with_rows_mixed_objective(engine):
                          precision="mixed")
     _same_lm(rd, rh)
     assert np.all(rd.merit < m0)
+
+
+# ---- sine condition and distortion (real rays at lambda_ref) ----------------------------------
+
+def _image_rows(n=6, seed=7):
+    b, v = systems.singlet(n, seed=seed)
+    s = systems.singlet_settings(abi.FAILURE_HARD)
+    s.constraints = systems.singlet_image_constraints()
+    return b, v, s
+
+
+def test_image_rows_match_oracle(engine):
+    """The zone rays of the sine condition are traced with the oracle's operations (rounded one
+    by one, correctly rounded divisions and roots): bit for bit. The chief rays of the distortion
+    rows start with sin / cos of the field angle, which the device's library may round
+    differently by an ulp: 1e-12 relative."""
+    b, _, s = _image_rows()
+    res, _, _ = engine.residuals(b, s)
+    T = res.shape[1] - len(s.constraints)
+    rows = res[:, T:]
+    ro = oracle().constraint_rows(b, s)
+    assert np.array_equal(rows[:, :2], ro[:, :2]) and np.count_nonzero(ro[:, :2]) > 0
+    assert np.all(rows[:, 2] == 0.0)  # axis field
+    assert np.abs(rows[:, 3:] - ro[:, 3:]).max() <= 1e-12 * np.abs(ro[:, 3:]).max()
+    assert np.count_nonzero(ro[:, 3:]) > 0
+    m, _, _ = engine.merit(b, s)
+    assert _rel(m, oracle().merit(b, s)[0]) <= 1e-9
+
+
+def test_image_rows_concentric_systems(engine):
+    """OSC = 0 up to rounding for systems concentric to the stop, every zone; their distortion
+    with the image plane shifted by delta from the focus is delta / f."""
+    zones = (0.25, 0.5, 0.70710678118654752440, 1.0)
+    for a, gaps, n in ((4.0, [10.0, 3.0], [1.0, 1.6, 1.0]), (5.0, [10.0], [1.0, 1.7]),
+                       (4.0, [8.0, 2.0, 3.0], [1.0, 1.75, 1.45, 1.0])):
+        b = systems.concentric(a, gaps, n, 20.0)
+        s = systems.ResidualSettings(rings=3, spokes=4, failure=abi.FAILURE_HARD,
+                                     constraints=[bt.sine_condition(z) for z in zones])
+        res, _, _ = engine.residuals(b, s)
+        assert np.all(np.abs(res[0, -4:]) <= 1e-14)
+    gaps, n, fields = [10.0, 3.0], [1.0, 1.6, 1.0], (0.0, 4.0, 12.0)
+    f = 1.0 / oracle().paraxial(systems.concentric(4.0, gaps, n, 20.0, fields), 0)[4]
+    for delta in (0.05, -0.2):
+        b = systems.concentric(4.0, gaps, n, f - sum(gaps) + delta, fields)
+        b.field_target[0] = f * np.tan(np.radians(fields))
+        s = systems.ResidualSettings(rings=3, spokes=4, failure=abi.FAILURE_HARD,
+                                     constraints=[bt.distortion(0.0, k) for k in range(3)])
+        res, _, _ = engine.residuals(b, s)
+        assert res[0, -3] == 0.0
+        assert np.all(np.abs(res[0, -2:] - abs(delta / f)) <= 1e-12)
+
+
+@pytest.mark.parametrize("method", ["fd", "adjoint"])
+def test_image_rows_in_jacobians(engine, method):
+    """The rows' derivatives are central differences of the rows at the FD pattern's points, as
+    in the reference (extra_rows_cpu): the sine-condition columns bit for bit, the distortion
+    columns within the chief ray's ulp over the FD step."""
+    b, v, s = _image_rows()
+    x0 = v.extract(b)[:3]
+    base = b.take([0])
+    J, r0, a, g = engine.jacobian(base, s, v, x0, method=method)
+    fn = oracle().fd_jacobian if method == "fd" else oracle().adjoint_jacobian
+    Jo, r0o, _ = fn(base, s, v, x0)
+    T = J.shape[2] - len(s.constraints)
+    assert np.array_equal(J[:, :, T:T + 2], Jo[:, :, T:T + 2])
+    dist = np.abs(J[:, :, T + 2:] - Jo[:, :, T + 2:])
+    assert dist.max() <= 1e-6 * max(1.0, np.abs(Jo[:, :, T + 2:]).max())
+    assert np.count_nonzero(Jo[:, :, T:T + 2]) > 0
+
+
+def test_lm_with_image_rows(engine):
+    """The LM with sine-condition and distortion rows: device control equals host control bit for
+    bit, the merit decreases, and it follows the oracle's LM."""
+    b, v, s = _image_rows(4, seed=3)
+    for c in s.constraints:
+        c.weight = 1e4
+    x0 = v.extract(b)[:3]
+    base = b.take([0])
+    kw = dict(max_iterations=8, threads=4, jacobian_method=abi.JACOBIAN_ADJOINT)
+    rd = engine.optimize(base, s, v, x0, abi.lm_options(**kw))
+    rh = engine.optimize(base, s, v, x0, abi.lm_options(host_control=True, **kw))
+    _same_lm(rd, rh)
+    pts = base.take(np.zeros(3, dtype=np.int64))
+    v.apply(pts, x0)
+    m0, _, _ = engine.merit(pts, s)
+    assert np.all(rd.merit < m0)
+    _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, abi.lm_options(**kw))
+    assert np.allclose(rd.merit, [r.merit for r in runs_o], rtol=1e-3)
 
 
 def test_device_entry_points_reject_rows(engine):

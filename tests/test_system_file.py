@@ -23,7 +23,8 @@ import os
 import numpy as np
 import pytest
 
-from mikroptim_amd import abi, rng
+from mikroptim_amd import abi, rng, systems
+from mikroptim_amd import batch as bt
 from mikroptim_amd import system_file as sf
 from oracle_lib import oracle
 
@@ -125,6 +126,32 @@ def test_paraxial_rows_from_example_file():
     assert abs(r[3] - r[0]) <= 1e-9 * abs(r[0])
     assert abs(r[4] - (5.0 + 5.0 + 6.0 + 90.0 - 115.0)) <= 1e-12
     assert r[5] == math.sqrt(10.0) * (90.0 - 95.0)
+
+
+def test_sine_condition_and_distortion_items(tmp_path):
+    """A sine_condition entry becomes one row per zone (default 1/sqrt(2) and 1), a distortion
+    entry one row per field of the source (make_paraxial_constraints); the rows evaluate like the
+    oracle's constructors."""
+    def edit(d):
+        d["source"]["field_angles_deg"] = [0.0, 2.0, 4.0]
+        d["source"]["field_weights"] = [1.0, 1.0, 1.0]
+        d["constraints"]["items"] = [{"type": "sine_condition", "weight": 4.0},
+                                     {"type": "distortion", "max_distortion": 0.001},
+                                     {"type": "focal_length", "weight": 1.0}]
+    sysf = sf.read_system(_edited(tmp_path, edit))
+    c = sysf.constraints
+    assert [x.type for x in c[:5]] == [abi.CON_SINE_CONDITION] * 2 + [abi.CON_DISTORTION] * 3
+    assert [x.zone for x in c[:2]] == [0.70710678118654752440, 1.0]
+    assert [x.field for x in c[2:5]] == [0, 1, 2]
+    assert all(x.mode == abi.MODE_MAX and x.target == 0.001 for x in c[2:5])
+    assert sysf.constraint_items[:5] == [0, 0, 1, 1, 1]
+    ld = sf.load(_edited(tmp_path, edit))
+    rows = oracle().constraint_rows(ld.batch, ld.settings)[0]
+    s = systems.ResidualSettings(rings=3, spokes=4, failure=abi.FAILURE_HARD)
+    s.constraints = [bt.sine_condition(0.70710678118654752440, 4.0), bt.sine_condition(1.0, 4.0)] \
+        + [bt.distortion(0.001, f) for f in range(3)] + [c[5]]
+    assert np.array_equal(rows, oracle().constraint_rows(ld.batch, s)[0])
+    assert rows[0] != 0.0 and rows[2] == 0.0
 
 
 def test_mechanical_items_expand_per_surface():
@@ -299,7 +326,21 @@ def _set(path, value):
     (_set(["constraints", "items", 1, "reference_surface"], 3),
      "constraints.items[1].reference_surface"),
     (_set(["constraints", "items", 2], {"type": "focal_length"}), "constraints.items[2].type"),
-    (_set(["constraints", "items", 2], {"type": "sine_condition"}), "constraints.items[2].type"),
+    (_set(["constraints", "items", 2], {"type": "ghost_penalty"}), "constraints.items[2].type"),
+    (_set(["constraints", "items", 2], {"type": "sine_condition", "zones": [1.5]}),
+     "constraints.items[2].zones[0]"),
+    (_set(["constraints", "items", 2], {"type": "sine_condition", "zones": [0.5, 0.5]}),
+     "constraints.items[2].zones[1]"),
+    (_set(["constraints", "items", 2], {"type": "sine_condition", "mode": "min"}),
+     "constraints.items[2].mode"),
+    (_set(["constraints", "items", 2], {"type": "distortion"}),
+     "constraints.items[2].max_distortion"),
+    (_set(["constraints", "items", 2], {"type": "distortion", "max_distortion": 0.01,
+                                        "mode": "equal"}), "constraints.items[2].mode"),
+    (_set(["constraints", "items", 2], {"type": "working_distance", "max_distortion": 0.01}),
+     "constraints.items[2].max_distortion"),
+    (_set(["constraints", "items", 2], {"type": "working_distance", "zones": [0.5]}),
+     "constraints.items[2].zones"),
     (_set(["constraints", "items", 2], {"type": "max_semi_diameter", "surfaces": [1, 1]}),
      "constraints.items[2].surfaces[1]"),
     (_set(["constraints", "items", 2], {"type": "min_center_thickness", "thickness": 1.0,
