@@ -376,17 +376,33 @@ def main():
     # workload: LM cycles/s with the rows that need no extrema, and with rows from the surface
     # extrema as well (their columns come from central differences of each run's FD batch,
     # traced with extrema, as in the reference); and the MERIT trace rate with extrema
+    # with extrema, the differences trace only the candidate tuples at the 2P difference points
+    # (checked; the same rows as tracing every tuple), and, for comparison, every tuple
     ext0 = eng.surface_extrema(base, settings)
     rows = {}
-    for name, ext in (("without_extrema", None), ("with_extrema", ext0)):
+    for name, ext, full in (("without_extrema", None, False), ("with_extrema", ext0, False),
+                            ("with_extrema_every_tuple", ext0, True)):
         st = copy.copy(settings)
         st.constraints = systems.objective_constraints(base, ext)
-        steps = args.steps if ext is None else max(1, min(args.steps, 3))
-        n_r, t_r, kt_r, res_r, jac_r = run_lm(eng, base, st, variables, x0, abi.JACOBIAN_ADJOINT,
-                                              "mixed", 1, steps, args.threads)
+        steps = args.steps if ext is None else max(1, min(args.steps, 3 if not full else 1))
+        eng.tuning(full_row_traces=full)
+        stats0 = eng.row_trace_stats()
+        try:
+            n_r, t_r, kt_r, res_r, jac_r = run_lm(eng, base, st, variables, x0,
+                                                  abi.JACOBIAN_ADJOINT, "mixed", 1, steps,
+                                                  args.threads)
+        finally:
+            eng.tuning(full_row_traces=False)
+        stats1 = eng.row_trace_stats()
         rows[name] = {"lm_iter_per_s": n_r / t_r, "steps": n_r, "rows": len(st.constraints),
                       "jacobians": jac_r,
                       "kernel_ms": {k: v[0] for k, v in kt_r.items()}}
+        if ext is not None and not full:
+            rows[name]["candidate_chunks"] = stats1[0] - stats0[0]
+            rows[name]["chunks_traced_whole_after_check"] = stats1[1] - stats0[1]
+    if rows["with_extrema_every_tuple"]["lm_iter_per_s"]:
+        rows["candidate_speedup"] = (rows["with_extrema"]["lm_iter_per_s"]
+                                     / rows["with_extrema_every_tuple"]["lm_iter_per_s"])
     st = copy.copy(settings)
     eng.surface_extrema(batch, st)
     eng.kernel_timing(True)

@@ -166,6 +166,13 @@ int mko_engine_kernel_timing(mko_engine* engine, int enable, double* ms, int64_t
 int mko_engine_set_tuning(mko_engine* engine, const mko_tuning* tuning);
 int mko_engine_get_tuning(const mko_engine* engine, mko_tuning* tuning);
 
+/* Constraint rows from trace extrema differentiated by central differences (the adjoint paths'
+ * rows, extra_rows_cpu, src/optim/adjoint_jacobian.cpp:245-337): number of chunks whose
+ * difference points traced only the candidate tuples, and of those traced whole again because a
+ * run missed the check's margin (mko_tuning.full_row_traces). No counterpart in the reference. */
+int mko_engine_row_trace_stats(const mko_engine* engine, int64_t* candidate_chunks,
+                               int64_t* full_chunks);
+
 /* Autotuner: times the candidate launch shapes of the MERIT trace (tuple packing off, or each
  * pack segment that holds the batch's largest tuple) and of the fused normal equations (1, 2 or 4
  * waves per workgroup where they apply) on the first min(runs, 64) points x [runs][P] of the base

@@ -200,6 +200,9 @@ typedef struct {
     int32_t pack_rays;   /* 0: automatic; 32, 64, 128: segment of a tuple pack (tuples of at
                           * most that many rays share a 256-lane workgroup); 256: no packing */
     int32_t fused_waves; /* 0: automatic; 1, 2, 4: waves per workgroup of the fused kernel */
+    int32_t full_row_traces; /* 0: the central differences of constraint rows from trace
+                              * extrema trace the candidate tuples at the difference points
+                              * (the same extrema, checked); 1: every tuple */
 } mko_tuning;
 
 typedef struct mko_lm_options {

@@ -75,7 +75,8 @@ class Variables(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("pack_rays", C.c_int32), ("fused_waves", C.c_int32)]
+    _fields_ = [("pack_rays", C.c_int32), ("fused_waves", C.c_int32),
+                ("full_row_traces", C.c_int32)]
 
 
 class LmOptions(C.Structure):

@@ -1239,9 +1239,11 @@ public:
                           std::to_string(s.failure) + "/" + std::to_string(base.source) + "/" +
                           std::to_string(s.extra ? s.extra->count : 0);
         for (int f = 0; f < base.fields; ++f) key += char('0' + base.pupil_mode[f]);
+        const int full_rows = tuning_.full_row_traces; // not a launch shape: kept
         auto hit = tuned_.find(key);
         if (hit != tuned_.end()) {
             tuning_ = hit->second;
+            tuning_.full_row_traces = full_rows;
             return tuning_;
         }
         const int n = std::min(runs, 64);
@@ -1275,6 +1277,7 @@ public:
             return best;
         };
         mko_internal::Tuning chosen{};
+        chosen.full_row_traces = full_rows;
         try {
             std::vector<int> packs{0};
             if (nmax <= 128 && base.fields * base.wavelengths >= 2 &&
@@ -1285,7 +1288,7 @@ public:
             }
             double best = -1.0;
             for (int pk : packs) {
-                const double ms = trial(mko_internal::Tuning{pk, 0}, kTrace, [&] {
+                const double ms = trial(mko_internal::Tuning{pk, 0, full_rows}, kTrace, [&] {
                     point_merit(b, s, v, n, x, merit.data(), failed.data(), hard.data());
                 });
                 if (best < 0.0 || ms < best) {
@@ -1298,7 +1301,7 @@ public:
                 if (nmax <= 64) waves.insert(waves.begin(), 1);
                 best = -1.0;
                 for (int w : waves) {
-                    const double ms = trial(mko_internal::Tuning{chosen.pack_rays, w}, kAdjoint, [&] {
+                    const double ms = trial(mko_internal::Tuning{chosen.pack_rays, w, full_rows}, kAdjoint, [&] {
                         jacobian(b, s, v, n, x, MKO_JACOBIAN_ADJOINT, MKO_PRECISION_MIXED,
                                  nullptr, nullptr, a.data(), g.data());
                     });
@@ -1428,18 +1431,7 @@ private:
                 rowfd_.mut(rowfd_.view.buchdahl), nullptr));
         });
         double* ex = nullptr;
-        if (rp.Et) {
-            const size_t Tt = size_t(tmpl.fields) * tmpl.wavelengths, S = size_t(tmpl.surfaces);
-            double* part = part_.as<double>(n * Tt);
-            int* pc = partc_.as<int>(n * Tt * 2);
-            double* ep = ext_part_.as<double>(n * Tt * S * 2);
-            ex = ext_.as<double>(n * S * 2);
-            timer.run(kTrace, [&] {
-                dev_check(mko_internal::trace_extrema(&rowfd_.view, &s, 1, nullptr, 0, nullptr,
-                                                      nullptr, part, pc, nullptr, ep, ex,
-                                                      rp.surfaces, nullptr));
-            });
-        }
+        if (rp.Et) ex = rows_extrema(tmpl, tidx, template_per_run, count, x, s, rp, P);
         double* rows = rows_.as<double>(n * E);
         dev_check(mko_internal::constraint_rows(&rowfd_.view, rp.dev, E, rp.paraxial, ex, rows, E,
                                                 0, nullptr));
@@ -1449,6 +1441,85 @@ private:
         });
     }
 
+    // Extrema [count * (2P + 1)][S][2] of the rows' FD batch (rowfd_). The rows need the largest
+    // height / smallest cos I over all rays at each difference point; the central differences
+    // move them by far less than they differ between tuples, so only the candidate tuples --
+    // those within a relative 1e-3 of the run's extrema at its point, or with failed rays -- are
+    // traced at the 2P difference points (the others' blocks exit at once). A check then asks
+    // that the other tuples would have to move 10 times as far as the candidates' extrema did to
+    // reach them; a chunk where a run misses that margin is traced whole. The extrema, and so
+    // the rows and their differences, are those of tracing every tuple: the same tuple traces,
+    // the same maximum.
+    double* rows_extrema(const mko_batch& tmpl, const int32_t* tidx, int template_per_run,
+                         int count, const double* x, const mko_residual_settings& s,
+                         const RowPlan& rp, int P) {
+        const int per_run = 2 * P + 1;
+        const size_t n = size_t(count) * per_run;
+        const size_t Tt = size_t(tmpl.fields) * tmpl.wavelengths, S = size_t(tmpl.surfaces);
+        double* part = part_.as<double>(n * Tt);
+        int* pc = partc_.as<int>(n * Tt * 2);
+        double* ep = ext_part_.as<double>(n * Tt * S * 2);
+        double* ex = ext_.as<double>(n * S * 2);
+        auto full = [&](const uint8_t* mask) {
+            timer.run(kTrace, [&] {
+                dev_check(mko_internal::trace_extrema(&rowfd_.view, &s, 1, nullptr, 0, nullptr,
+                                                      nullptr, part, pc, nullptr, ep, ex,
+                                                      rp.surfaces, nullptr, mask, per_run));
+            });
+        };
+        if (mko_internal::thread_tuning().full_row_traces || Tt < 2) {
+            full(nullptr);
+            return ex;
+        }
+        // the runs' points, every tuple
+        rowbase_.allocate(tmpl, count);
+        timer.run(kPointBatch, [&] {
+            dev_check(mko_internal::point_batch(
+                &tmpl, tidx, template_per_run, count, 1, &dvars_, x, nullptr, nullptr,
+                rowbase_.mut(rowbase_.view.curvature), rowbase_.mut(rowbase_.view.thickness),
+                rowbase_.mut(rowbase_.view.index), rowbase_.mut(rowbase_.view.pupil_radius),
+                rowbase_.mut(rowbase_.view.field_angle), rowbase_.mut(rowbase_.view.field_target),
+                rowbase_.mut(rowbase_.view.reference_curvature),
+                rowbase_.mut(rowbase_.view.paraxial_index), rowbase_.mut(rowbase_.view.buchdahl),
+                nullptr));
+        });
+        const size_t c = size_t(count);
+        double* part0 = cand_part_.as<double>(c * Tt);
+        int* pc0 = cand_pc_.as<int>(c * Tt * 2);
+        double* ep0 = cand_ep_.as<double>(c * Tt * S * 2);
+        double* ex0 = cand_ex_.as<double>(c * S * 2);
+        timer.run(kTrace, [&] {
+            dev_check(mko_internal::trace_extrema(&rowbase_.view, &s, 1, nullptr, 0, nullptr,
+                                                  nullptr, part0, pc0, nullptr, ep0, ex0,
+                                                  rp.surfaces, nullptr));
+        });
+        uint8_t* mask = cand_mask_.as<uint8_t>(c * Tt);
+        double* nc = cand_nc_.as<double>(c * S * 2);
+        int* ncount = cand_count_.as<int>(c);
+        int* flag = cand_flag_.as<int>(c);
+        dev_check(mko_internal::extrema_candidates(count, int(Tt), int(S), rp.surfaces, ep0, pc0,
+                                                   ex0, 1e-3, mask, nc, ncount, nullptr));
+        full(mask);
+        dev_check(mko_internal::extrema_check(count, per_run, int(S), rp.surfaces, ex, ex0, nc,
+                                              ncount, 10.0, flag, nullptr));
+        std::vector<int> flags(c);
+        hip_check(hipMemcpy(flags.data(), flag, c * sizeof(int), hipMemcpyDeviceToHost),
+                  "extrema check");
+        ++candidate_chunks_;
+        if (std::any_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) {
+            ++candidate_fallbacks_;
+            full(nullptr);
+        }
+        return ex;
+    }
+
+public:
+    // chunks whose rows' differences used candidate tuples, and those traced whole after the check
+    std::pair<int64_t, int64_t> candidate_stats() const {
+        return {candidate_chunks_, candidate_fallbacks_};
+    }
+
+private:
     int tranche_size(int items, size_t per_item) const {
         if (items <= 0) return 1;
         const size_t b = budget();
@@ -1481,7 +1552,9 @@ private:
     size_t limit_;
     mko_internal::Tuning tuning_;
     std::map<std::string, mko_internal::Tuning> tuned_; // autotuner's choices per shape
-    DeviceBatch batch_, tmpl_, work_, rowfd_;
+    DeviceBatch batch_, tmpl_, work_, rowfd_, rowbase_;
+    Buffer cand_part_, cand_pc_, cand_ep_, cand_ex_, cand_mask_, cand_nc_, cand_count_, cand_flag_;
+    int64_t candidate_chunks_ = 0, candidate_fallbacks_ = 0;
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
     Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
     std::vector<char> items_host_;
@@ -2695,7 +2768,21 @@ int mko_engine_set_tuning(mko_engine* e, const mko_tuning* t) {
         if (t->fused_waves != 0 && t->fused_waves != 1 && t->fused_waves != 2 &&
             t->fused_waves != 4)
             throw std::invalid_argument("fused_waves must be 0, 1, 2 or 4");
-        e->engine.set_tuning(mko_internal::Tuning{t->pack_rays, t->fused_waves});
+        if (t->full_row_traces != 0 && t->full_row_traces != 1)
+            throw std::invalid_argument("full_row_traces must be 0 or 1");
+        e->engine.set_tuning(
+            mko_internal::Tuning{t->pack_rays, t->fused_waves, t->full_row_traces});
+        return 0;
+    })
+}
+
+int mko_engine_row_trace_stats(const mko_engine* e, int64_t* candidate_chunks,
+                               int64_t* full_chunks) {
+    MKO_HOST_TRY({
+        if (!e) throw std::invalid_argument("engine needed");
+        const auto st = e->engine.candidate_stats();
+        if (candidate_chunks) *candidate_chunks = st.first;
+        if (full_chunks) *full_chunks = st.second;
         return 0;
     })
 }
@@ -2705,6 +2792,7 @@ int mko_engine_get_tuning(const mko_engine* e, mko_tuning* t) {
         if (!e || !t) throw std::invalid_argument("engine and tuning needed");
         t->pack_rays = e->engine.tuning().pack_rays;
         t->fused_waves = e->engine.tuning().fused_waves;
+        t->full_row_traces = e->engine.tuning().full_row_traces;
         return 0;
     })
 }

@@ -19,6 +19,7 @@ namespace mko_internal {
 struct Tuning {
     int pack_rays = 0;
     int fused_waves = 0;
+    int full_row_traces = 0; // 1: the constraint rows' differences trace every tuple
 };
 void set_thread_tuning(const Tuning& t);
 Tuning thread_tuning();
@@ -69,10 +70,25 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
 // extrema [V][S][2] = {largest ray height, smallest cos I} over the rays with status Ok
 // (surface_extrema_cpu, src/batch/batch_extrema.cpp:75). ext_partial == NULL: no extrema.
 // `surfaces`: bit s set = surface s is reduced (the others stay at the neutral 0 / 1).
+// tuple_mask (NULL: every tuple; mode 1 with extrema only): tuple t of variant v is traced
+// where tuple_mask[(v / mask_group) * T + t] is set, the others leave neutral partials.
 int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
                   double* residuals, int64_t stride, double* opd, uint8_t* status,
                   double* partial, int* partial_counts, const RayStates* states,
-                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream);
+                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream,
+                  const uint8_t* tuple_mask = nullptr, int mask_group = 1);
+
+// Candidate tuples of the constraint rows' central differences (the tuples whose extrema come
+// within delta of the run's on a reduced surface, or that hold a failed ray) and the extrema of
+// the other tuples: mask [runs][T], nc [runs][S][2], ncount [runs]. See extrema_check.
+int extrema_candidates(int runs, int T, int S, uint64_t surfaces, const double* ep0,
+                       const int* pfailed0, const double* ex0, double delta, uint8_t* mask,
+                       double* nc, int* ncount, void* stream);
+// flag[r] = 1 where the candidates' extrema over the run's per_run difference points moved by
+// D and the other tuples' extrema at the run's point lie within K D of them.
+int extrema_check(int runs, int per_run, int S, uint64_t surfaces, const double* ex,
+                  const double* ex0, const double* nc, const int* ncount, double K, int* flag,
+                  void* stream);
 
 // FP32 prefilter stats of a device batch [V] (prefilter_kernel + second stage); workspace of
 // 24 bytes per (variant, tuple).

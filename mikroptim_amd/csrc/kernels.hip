@@ -76,6 +76,10 @@ struct TraceParams {
     // device count of the variants to trace (NULL: all V): blocks of variants >= *live exit at
     // once, so that a launch sized for V serves a captured LM cycle whose count the device sets
     const int32_t* live;
+    // kExt traces: tuple t of variant v is traced only where tuple_mask[(v / mask_group) * T + t]
+    // is set (NULL: every tuple); the others leave neutral partials (extrema 0 / 1, merit 0)
+    const uint8_t* tuple_mask;
+    int mask_group;
     int64_t rays, trace_rows, stride;
     double gl_u[MKO_MAX_RINGS], gl_w[MKO_MAX_RINGS];
     double field_weight[kMaxFields];
@@ -409,6 +413,24 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     __shared__ double red[kWaves];
     __shared__ int ired[kWaves];
     if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
+    if constexpr (kExt) {
+        if (P.tuple_mask) {
+            const int v = int(blockIdx.x) / P.T, tt = int(blockIdx.x) - v * P.T;
+            if (!P.tuple_mask[size_t(v / P.mask_group) * P.T + tt]) {
+                if (int(threadIdx.x) < P.S) {
+                    double* out = ext_partial + (size_t(blockIdx.x) * P.S + threadIdx.x) * 2;
+                    out[0] = 0.0;
+                    out[1] = 1.0;
+                }
+                if (kMode == 1 && threadIdx.x == 0) {
+                    partial[blockIdx.x] = 0.0;
+                    pfailed[blockIdx.x] = 0;
+                    phard[blockIdx.x] = 0;
+                }
+                return;
+            }
+        }
+    }
     const Tuple t = tuple_of(P, blockIdx.x);
     stage_surfaces(P, t, L);
     const size_t vf = size_t(t.v) * P.F + t.f;
@@ -762,6 +784,76 @@ __global__ void extrema_finalize_kernel(int V, int T, int S, const double* parti
     }
     out[i * 2] = __dsqrt_rn(h);
     out[i * 2 + 1] = c;
+}
+
+// Candidate tuples of the rows' central differences, one thread per run: from the tuple partials
+// ep0 [runs][T][S][2] (largest height^2, smallest cos I) of the runs' points and their extrema
+// ex0 [runs][S][2], tuple t is a candidate where, on a surface of `surfaces`, its largest height
+// reaches (1 - delta) of the run's or its smallest cos I lies within delta (1 - cos) of the run's,
+// or where it holds a failed ray (pfailed0 [runs][T]). mask [runs][T]; nc [runs][S][2] = largest
+// height / smallest cos I over the other tuples (0 / 1 without any); ncount [runs] = their number.
+__global__ void extrema_candidates_kernel(int runs, int T, int S, uint64_t surfaces,
+                                          const double* ep0, const int* pfailed0,
+                                          const double* ex0, double delta, uint8_t* mask,
+                                          double* nc, int* ncount) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    const double* e0 = ex0 + size_t(r) * S * 2;
+    int others = 0;
+    for (int s = 0; s < S; ++s) {
+        nc[(size_t(r) * S + s) * 2] = 0.0;
+        nc[(size_t(r) * S + s) * 2 + 1] = 1.0;
+    }
+    for (int t = 0; t < T; ++t) {
+        const double* p = ep0 + (size_t(r) * T + t) * S * 2;
+        bool cand = pfailed0[size_t(r) * T + t] != 0;
+        for (int s = 0; s < S && !cand; ++s) {
+            if (!((surfaces >> s) & 1ull)) continue;
+            const double hmax = e0[2 * s], cmin = e0[2 * s + 1];
+            cand = __dsqrt_rn(p[2 * s]) >= (1.0 - delta) * hmax ||
+                   p[2 * s + 1] <= cmin + delta * (1.0 - cmin);
+        }
+        mask[size_t(r) * T + t] = cand ? 1 : 0;
+        if (!cand) {
+            ++others;
+            for (int s = 0; s < S; ++s) {
+                double* q = nc + (size_t(r) * S + s) * 2;
+                const double h = __dsqrt_rn(p[2 * s]);
+                q[0] = h > q[0] ? h : q[0];
+                q[1] = p[2 * s + 1] < q[1] ? p[2 * s + 1] : q[1];
+            }
+        }
+    }
+    ncount[r] = others;
+}
+
+// Check of the candidate tuples after the differences' traces, one thread per run: on every
+// surface of `surfaces` the candidates' extrema over the run's per_run points (ex [runs *
+// per_run][S][2]) moved by at most D from the run's (ex0); the other tuples would have to move
+// by more than K D to reach them (nc). flag[r] = 1 where that margin is missing.
+__global__ void extrema_check_kernel(int runs, int per_run, int S, uint64_t surfaces,
+                                     const double* ex, const double* ex0, const double* nc,
+                                     const int* ncount, double K, int* flag) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= runs) return;
+    int bad = 0;
+    if (ncount[r] > 0) {
+        for (int s = 0; s < S; ++s) {
+            if (!((surfaces >> s) & 1ull)) continue;
+            const double h0 = ex0[(size_t(r) * S + s) * 2], c0 = ex0[(size_t(r) * S + s) * 2 + 1];
+            double dh = 0.0, dc = 0.0, hmin = h0, cmax = c0;
+            for (int j = 0; j < per_run; ++j) {
+                const double* e = ex + ((size_t(r) * per_run + j) * S + s) * 2;
+                dh = fmax(dh, fabs(e[0] - h0));
+                dc = fmax(dc, fabs(e[1] - c0));
+                hmin = fmin(hmin, e[0]);
+                cmax = fmax(cmax, e[1]);
+            }
+            const double hn = nc[(size_t(r) * S + s) * 2], cn = nc[(size_t(r) * S + s) * 2 + 1];
+            if (!(hn + K * dh < hmin) || !(cn - K * dc > cmax)) bad = 1;
+        }
+    }
+    flag[r] = bad;
 }
 
 // Constraint rows of every variant (one thread per variant), rows[v * stride + offset + k] for
@@ -2715,10 +2807,17 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
 int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
                   double* residuals, int64_t stride, double* opd, uint8_t* status,
                   double* partial, int* partial_counts, const RayStates* states,
-                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream) {
+                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream,
+                  const uint8_t* tuple_mask, int mask_group) {
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
         P.ext_mask = surfaces;
+        if (tuple_mask) {
+            if (!ext_partial || mode != 1 || mask_group < 1)
+                throw std::invalid_argument("a tuple mask needs the MERIT trace with extrema");
+            P.tuple_mask = tuple_mask;
+            P.mask_group = mask_group;
+        }
         if (mode == 0) {
             if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
             P.stride = stride;
@@ -2735,6 +2834,30 @@ int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings,
                                0, s, P.V, P.T, P.S, ext_partial, extrema);
             check_hip(hipGetLastError(), "extrema finalize launch");
         }
+    })
+}
+
+int extrema_candidates(int runs, int T, int S, uint64_t surfaces, const double* ep0,
+                       const int* pfailed0, const double* ex0, double delta, uint8_t* mask,
+                       double* nc, int* ncount, void* stream) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        hipLaunchKernelGGL(extrema_candidates_kernel, dim3((runs + 127) / 128), dim3(128), 0,
+                           static_cast<hipStream_t>(stream), runs, T, S, surfaces, ep0, pfailed0,
+                           ex0, delta, mask, nc, ncount);
+        check_hip(hipGetLastError(), "extrema candidates launch");
+    })
+}
+
+int extrema_check(int runs, int per_run, int S, uint64_t surfaces, const double* ex,
+                  const double* ex0, const double* nc, const int* ncount, double K, int* flag,
+                  void* stream) {
+    MKO_TRY({
+        if (runs <= 0) return 0;
+        hipLaunchKernelGGL(extrema_check_kernel, dim3((runs + 127) / 128), dim3(128), 0,
+                           static_cast<hipStream_t>(stream), runs, per_run, S, surfaces, ex, ex0,
+                           nc, ncount, K, flag);
+        check_hip(hipGetLastError(), "extrema check launch");
     })
 }
 

@@ -29,7 +29,7 @@ EXPORTS = [
     "mko_gram_mfma_device",
     "mko_df64_ops_device",
     "mko_engine_create", "mko_engine_destroy", "mko_engine_set_tuning", "mko_engine_get_tuning",
-    "mko_engine_autotune",
+    "mko_engine_autotune", "mko_engine_row_trace_stats",
     "mko_engine_residuals", "mko_engine_merit",
     "mko_engine_surface_extrema", "mko_engine_prefilter", "mko_prefilter_decide",
     "mko_engine_jacobian", "mko_engine_kernel_timing", "mko_engine_normal_equations",
@@ -90,6 +90,8 @@ def library():
         "mko_engine_get_tuning": ([C.c_void_p, C.POINTER(abi.Tuning)], C.c_int),
         "mko_engine_autotune": ([C.c_void_p, bp, sp, vp, C.c_int, _dp, C.POINTER(abi.Tuning)],
                                 C.c_int),
+        "mko_engine_row_trace_stats": ([C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+                                       C.c_int),
         "mko_engine_residuals": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_uint8)], C.c_int),
         "mko_engine_merit": ([C.c_void_p, bp, sp, _dp, _dp, C.POINTER(C.c_int64),
                               C.POINTER(C.c_int64)], C.c_int),
@@ -259,10 +261,12 @@ class TraceEngine:
 
     KERNEL_FAMILIES = ("trace", "adjoint", "gram", "fd_difference", "point_batch", "lm_control")
 
-    def tuning(self, pack_rays=None, fused_waves=None):
+    def tuning(self, pack_rays=None, fused_waves=None, full_row_traces=None):
         """Launch shapes of this engine's kernels (mko_tuning): returns the current (pack_rays,
         fused_waves) and sets those given (0 = automatic; pack_rays 32 / 64 / 128, 256 = no
-        tuple packs; fused_waves 1 / 2 / 4)."""
+        tuple packs; fused_waves 1 / 2 / 4). full_row_traces (True: the constraint rows'
+        differences trace every tuple instead of the checked candidate tuples) is set when
+        given and not part of the returned pair."""
         t = abi.Tuning()
         _check(library().mko_engine_get_tuning(self._h, C.byref(t)))
         old = (t.pack_rays, t.fused_waves)
@@ -270,9 +274,18 @@ class TraceEngine:
             t.pack_rays = int(pack_rays)
         if fused_waves is not None:
             t.fused_waves = int(fused_waves)
-        if pack_rays is not None or fused_waves is not None:
+        if full_row_traces is not None:
+            t.full_row_traces = int(bool(full_row_traces))
+        if pack_rays is not None or fused_waves is not None or full_row_traces is not None:
             _check(library().mko_engine_set_tuning(self._h, C.byref(t)))
         return old
+
+    def row_trace_stats(self):
+        """(chunks whose constraint-row differences traced only candidate tuples, chunks traced
+        whole again after the check)."""
+        a, b = C.c_int64(0), C.c_int64(0)
+        _check(library().mko_engine_row_trace_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def autotune(self, base, settings, variables, x):
         """Times the candidate launch shapes on the first min(n, 64) points x [n][P] (base: one

@@ -210,6 +210,36 @@ def test_lm_with_rows_mixed_objective(engine):
     assert np.all(rd.merit < m0)
 
 
+def test_row_differences_on_candidate_tuples_equal_full_traces(engine):
+    """The adjoint paths difference the rows from extrema over the 2P + 1 points of each run; at
+    the difference points only the candidate tuples (extrema within 1e-3 of the run's, or failed
+    rays) are traced. Stored-J and mixed normal equations equal those of tracing every tuple bit
+    for bit, on the objective (35 tuples) and the singlet in soft mode with failed rays."""
+    sb, sv, ss = _singlet(6, failure=abi.FAILURE_SOFT)
+    sb.pupil_radius[1] = 110.0  # outer rings miss: failed rays in the soft trace
+    cases = [(*_objective(2), 2), (sb, sv, ss, 3)]
+    for b, v, s, n in cases:
+        base, x0 = b.take([0]), v.extract(b)[:n]
+        if b is sb:
+            base, x0 = b.take(np.arange(n)), v.extract(b)[:n]
+        before = engine.row_trace_stats()
+        J, r0, a, g = engine.jacobian(base, s, v, x0, method="adjoint", precision="fp64")
+        after = engine.row_trace_stats()
+        assert after[0] > before[0] and after[1] == before[1]  # candidates used, check passed
+        _, _, am, gm = engine.jacobian(base, s, v, x0, method="adjoint", precision="mixed",
+                                       want_jacobian=False)
+        engine.tuning(full_row_traces=True)
+        try:
+            Jf, r0f, af, gf = engine.jacobian(base, s, v, x0, method="adjoint", precision="fp64")
+            _, _, amf, gmf = engine.jacobian(base, s, v, x0, method="adjoint",
+                                             precision="mixed", want_jacobian=False)
+        finally:
+            engine.tuning(full_row_traces=False)
+        assert np.array_equal(J, Jf) and np.array_equal(r0, r0f)
+        assert np.array_equal(a, af) and np.array_equal(g, gf)
+        assert np.array_equal(am, amf) and np.array_equal(gm, gmf)
+
+
 # ---- sine condition and distortion (real rays at lambda_ref) ----------------------------------
 
 def _image_rows(n=6, seed=7):
