@@ -76,13 +76,14 @@ def cpu_model():
     return None
 
 
-def convergence(eng, base, settings, variables, x0, method, precision, cycles, threads):
+def convergence(eng, base, settings, variables, x0, method, precision, cycles, threads,
+                graph=True):
     """Mean merit over the runs after each LM cycle and the device time of the cycles so far
     (each cycle timed alone; the merit download between cycles is not counted)."""
     from mikroptim_amd import abi
 
     opt = abi.lm_options(max_iterations=cycles, threads=threads, jacobian_method=method,
-                         reject_new_failures=True)
+                         reject_new_failures=True, graph=graph)
     sess = eng.lm(base, settings, variables, x0, opt, precision)
     elapsed, trace = 0.0, []
     for _ in range(cycles):
@@ -461,7 +462,7 @@ def main():
     # per-cycle progress of the two LMs on the same starts: cycles and device seconds each needs
     # to reach a mean merit both reach (LM iter/s alone does not show what a cycle buys)
     traces = {name: convergence(eng, base, settings, variables, x0, method, prec,
-                                args.convergence_cycles, args.threads)
+                                args.convergence_cycles, args.threads, graph=not args.no_graph)
               for name, method, prec in (("adjoint_mixed", abi.JACOBIAN_ADJOINT, "mixed"),
                                          ("fd_fp64", abi.JACOBIAN_FD, "fp64"))}
     to_merit = time_to_merit(traces)
@@ -473,7 +474,8 @@ def main():
     st = copy.copy(settings)
     st.failure = abi.FAILURE_SOFT
     n_s, t_s, kt_s, res_s, jac_s = run_lm(eng, base, st, variables, x0, abi.JACOBIAN_ADJOINT,
-                                          "mixed", 1, max(1, min(args.steps, 5)), args.threads)
+                                          "mixed", 1, max(1, min(args.steps, 5)), args.threads,
+                                          graph=not args.no_graph)
     soft = {"lm_iter_per_s": n_s / t_s, "steps": n_s, "jacobians": jac_s,
             "failed_rays_final": int(sum(r.failed_rays for r in res_s.runs)),
             "kernel_ms": {k: v[0] for k, v in kt_s.items()}}

@@ -217,11 +217,13 @@ int mko_engine_surface_extrema(mko_engine* engine, const mko_batch* batch,
  * (include/mikroptim/cuda/fd_jacobian.hpp:58, adjoint_jacobian.hpp:70): base has one variant
  * (shared template) or one per run; x0 [runs][P]. Outputs (host, any may be NULL):
  * J [runs][P][R] double, r0 [runs][R], a [runs][P][P], g [runs][P].
- * method ADJOINT with precision MKO_PRECISION_MIXED, J and r0 NULL, <= 96 variables and no glass
- * variables takes a and g from mko_adjoint_normal_device: in hard failure mode for every
- * run, in soft mode for the runs whose point has no failed ray (their penalty rows and penalty
- * derivatives are zero, so their normal equations are hard mode's). Every other run or
- * combination computes J in FP64 and a, g with mko_gram_device. */
+ * method ADJOINT with precision MKO_PRECISION_MIXED, J and r0 NULL and <= 96 variables takes a
+ * and g from the fused kernel of mko_adjoint_normal_device (with glass variables on the ray
+ * states of a MERIT trace of the same points): in hard failure mode for every run, in soft mode
+ * for the runs whose point has no failed ray (their penalty rows and penalty derivatives are
+ * zero, so their normal equations are hard mode's). MKO_PRECISION_FP64_MFMA computes J in FP64
+ * and a, g with mko_gram_mfma_device. Every other run or combination computes J in FP64 and
+ * a, g with mko_gram_device. */
 int mko_engine_jacobian(mko_engine* engine, const mko_batch* base,
                         const mko_residual_settings* settings, const mko_variables* vars,
                         int runs, const double* x0, int method, int precision, double* jacobian,
