@@ -377,8 +377,8 @@ def main():
     # workload: LM cycles/s with the rows that need no extrema, and with rows from the surface
     # extrema as well (their columns come from central differences of each run's FD batch,
     # traced with extrema, as in the reference); and the MERIT trace rate with extrema
-    # with extrema, the differences trace only the candidate tuples at the 2P difference points
-    # (checked; the same rows as tracing every tuple), and, for comparison, every tuple
+    # with extrema, the differences trace only the candidate rays at the 2P difference points
+    # (checked; the same rows as tracing every ray), and, for comparison, every ray
     ext0 = eng.surface_extrema(base, settings)
     rows = {}
     for name, ext, full in (("without_extrema", None, False), ("with_extrema", ext0, False),

@@ -168,7 +168,7 @@ int mko_engine_get_tuning(const mko_engine* engine, mko_tuning* tuning);
 
 /* Constraint rows from trace extrema differentiated by central differences (the adjoint paths'
  * rows, extra_rows_cpu, src/optim/adjoint_jacobian.cpp:245-337): number of chunks whose
- * difference points traced only the candidate tuples, and of those traced whole again because a
+ * difference points traced only the candidate rays, and of those traced whole again because a
  * run missed the check's margin (mko_tuning.full_row_traces). No counterpart in the reference. */
 int mko_engine_row_trace_stats(const mko_engine* engine, int64_t* candidate_chunks,
                                int64_t* full_chunks);

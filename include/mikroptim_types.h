@@ -201,8 +201,8 @@ typedef struct {
                           * most that many rays share a 256-lane workgroup); 256: no packing */
     int32_t fused_waves; /* 0: automatic; 1, 2, 4: waves per workgroup of the fused kernel */
     int32_t full_row_traces; /* 0: the central differences of constraint rows from trace
-                              * extrema trace the candidate tuples at the difference points
-                              * (the same extrema, checked); 1: every tuple */
+                              * extrema trace the candidate rays at the difference points
+                              * (the same extrema, checked); 1: every ray */
 } mko_tuning;
 
 typedef struct mko_lm_options {

@@ -273,6 +273,7 @@ struct RowPlan {
     int E = 0, Et = 0;
     bool paraxial = false;
     uint64_t surfaces = 0; // surfaces whose extrema the rows read
+    uint64_t hsurf = 0, csurf = 0; // of those, the ones read for heights / for cos I
     const mko_constraint* dev = nullptr;
     int count() const { return E + Et; }
 };
@@ -292,6 +293,12 @@ RowPlan plan_rows(const mko_batch& b, const mko_residual_settings& s) {
         if (con_needs_extrema(c)) {
             r.surfaces |= 1ull << c.surface;
             if (c.type == MKO_CON_MIN_EDGE_THICKNESS) r.surfaces |= 1ull << c.next_surface;
+            if (c.type == MKO_CON_MAX_INCIDENCE_ANGLE) {
+                r.csurf |= 1ull << c.surface;
+            } else {
+                r.hsurf |= 1ull << c.surface;
+                if (c.type == MKO_CON_MIN_EDGE_THICKNESS) r.hsurf |= 1ull << c.next_surface;
+            }
         }
     }
     return r;
@@ -1374,9 +1381,14 @@ private:
     static size_t rows_fd_bytes(const RowPlan& rp, const mko_batch& b, int P) {
         if (rp.count() == 0) return 0;
         const size_t T = size_t(b.fields) * b.wavelengths, S = size_t(b.surfaces);
+        // the candidate rays' buffers of the run's point: its batch, partials and extrema, ray
+        // flags (at most 2048 rays per tuple), tuple counts and the candidate list
+        const size_t cand = rp.Et ? Strides(b).doubles() * 8 + T * (16 + S * 16 + 2048 + 12) +
+                                        size_t(kCandidateCap) * 4 + S * 16 + 64
+                                  : 0;
         return size_t(2 * P + 1) * (Strides(b).doubles() * 8 + size_t(rp.count()) * 8 +
                                     (rp.Et ? T * 16 + (T + 1) * S * 16 : 0)) +
-               size_t(P + 1) * rp.count() * 8;
+               size_t(P + 1) * rp.count() * 8 + cand;
     }
 
     // FD Jacobian of the central FD batch fdb (count runs x (2P + 1) variants): RESIDUALS of
@@ -1442,14 +1454,16 @@ private:
     }
 
     // Extrema [count * (2P + 1)][S][2] of the rows' FD batch (rowfd_). The rows need the largest
-    // height / smallest cos I over all rays at each difference point; the central differences
-    // move them by far less than they differ between tuples, so only the candidate tuples --
-    // those within a relative 1e-3 of the run's extrema at its point, or with failed rays -- are
-    // traced at the 2P difference points (the others' blocks exit at once). A check then asks
-    // that the other tuples would have to move 10 times as far as the candidates' extrema did to
-    // reach them; a chunk where a run misses that margin is traced whole. The extrema, and so
-    // the rows and their differences, are those of tracing every tuple: the same tuple traces,
-    // the same maximum.
+    // height / smallest cos I over all rays at each difference point; a central difference moves
+    // a ray height by ~1e-5 of itself, far less than the rays' heights differ. So each run's
+    // point is traced whole first, and only the candidate rays -- within a relative 1e-3 of the
+    // run's extrema at its point on the surfaces the rows read, or not ending with status Ok --
+    // are traced at the 2P difference points. A check then asks that the other rays would have
+    // to move 10 times as far as the candidates' extrema did to reach them; a chunk where a run
+    // misses that margin (or has more than kCandidateCap candidates) is traced whole. The
+    // extrema, and so the rows and their differences, are those of tracing every ray: the same
+    // ray traces, the same maximum.
+    static constexpr int kCandidateCap = 8192;
     double* rows_extrema(const mko_batch& tmpl, const int32_t* tidx, int template_per_run,
                          int count, const double* x, const mko_residual_settings& s,
                          const RowPlan& rp, int P) {
@@ -1460,18 +1474,18 @@ private:
         int* pc = partc_.as<int>(n * Tt * 2);
         double* ep = ext_part_.as<double>(n * Tt * S * 2);
         double* ex = ext_.as<double>(n * S * 2);
-        auto full = [&](const uint8_t* mask) {
+        auto full = [&] {
             timer.run(kTrace, [&] {
                 dev_check(mko_internal::trace_extrema(&rowfd_.view, &s, 1, nullptr, 0, nullptr,
                                                       nullptr, part, pc, nullptr, ep, ex,
-                                                      rp.surfaces, nullptr, mask, per_run));
+                                                      rp.surfaces, nullptr));
             });
         };
-        if (mko_internal::thread_tuning().full_row_traces || Tt < 2) {
-            full(nullptr);
+        if (mko_internal::thread_tuning().full_row_traces || s.precision != MKO_PRECISION_FP64) {
+            full();
             return ex;
         }
-        // the runs' points, every tuple
+        // the runs' points, every ray
         rowbase_.allocate(tmpl, count);
         timer.run(kPointBatch, [&] {
             dev_check(mko_internal::point_batch(
@@ -1484,6 +1498,7 @@ private:
                 nullptr));
         });
         const size_t c = size_t(count);
+        const int64_t rays = rays_of(tmpl, s);
         double* part0 = cand_part_.as<double>(c * Tt);
         int* pc0 = cand_pc_.as<int>(c * Tt * 2);
         double* ep0 = cand_ep_.as<double>(c * Tt * S * 2);
@@ -1493,28 +1508,34 @@ private:
                                                   nullptr, part0, pc0, nullptr, ep0, ex0,
                                                   rp.surfaces, nullptr));
         });
-        uint8_t* mask = cand_mask_.as<uint8_t>(c * Tt);
-        double* nc = cand_nc_.as<double>(c * S * 2);
-        int* ncount = cand_count_.as<int>(c);
+        uint8_t* flags = cand_flags_.as<uint8_t>(c * size_t(rays));
+        int* tcount = cand_tcount_.as<int>(c * Tt);
+        int* tstart = cand_tstart_.as<int>(c * (Tt + 1));
+        int* list = cand_list_.as<int>(c * kCandidateCap);
+        int* others = cand_count_.as<int>(c);
+        int* overflow = cand_over_.as<int>(c);
         int* flag = cand_flag_.as<int>(c);
-        dev_check(mko_internal::extrema_candidates(count, int(Tt), int(S), rp.surfaces, ep0, pc0,
-                                                   ex0, 1e-3, mask, nc, ncount, nullptr));
-        full(mask);
-        dev_check(mko_internal::extrema_check(count, per_run, int(S), rp.surfaces, ex, ex0, nc,
-                                              ncount, 10.0, flag, nullptr));
-        std::vector<int> flags(c);
-        hip_check(hipMemcpy(flags.data(), flag, c * sizeof(int), hipMemcpyDeviceToHost),
+        constexpr double kDelta = 1e-3, kMargin = 10.0;
+        timer.run(kTrace, [&] {
+            dev_check(mko_internal::candidate_extrema(
+                &rowbase_.view, &rowfd_.view, &s, per_run, rp.hsurf, rp.csurf, kDelta, ex0, flags,
+                tcount, others, tstart, list, kCandidateCap, overflow, ex, nullptr));
+        });
+        dev_check(mko_internal::extrema_check(count, per_run, int(S), rp.hsurf, rp.csurf, kDelta,
+                                              ex, ex0, others, overflow, kMargin, flag, nullptr));
+        std::vector<int> flags_h(c);
+        hip_check(hipMemcpy(flags_h.data(), flag, c * sizeof(int), hipMemcpyDeviceToHost),
                   "extrema check");
         ++candidate_chunks_;
-        if (std::any_of(flags.begin(), flags.end(), [](int f) { return f != 0; })) {
+        if (std::any_of(flags_h.begin(), flags_h.end(), [](int f) { return f != 0; })) {
             ++candidate_fallbacks_;
-            full(nullptr);
+            full();
         }
         return ex;
     }
 
 public:
-    // chunks whose rows' differences used candidate tuples, and those traced whole after the check
+    // chunks whose rows' differences used candidate rays, and those traced whole after the check
     std::pair<int64_t, int64_t> candidate_stats() const {
         return {candidate_chunks_, candidate_fallbacks_};
     }
@@ -1553,7 +1574,8 @@ private:
     mko_internal::Tuning tuning_;
     std::map<std::string, mko_internal::Tuning> tuned_; // autotuner's choices per shape
     DeviceBatch batch_, tmpl_, work_, rowfd_, rowbase_;
-    Buffer cand_part_, cand_pc_, cand_ep_, cand_ex_, cand_mask_, cand_nc_, cand_count_, cand_flag_;
+    Buffer cand_part_, cand_pc_, cand_ep_, cand_ex_, cand_flags_, cand_tcount_, cand_tstart_,
+        cand_list_, cand_count_, cand_over_, cand_flag_;
     int64_t candidate_chunks_ = 0, candidate_fallbacks_ = 0;
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
     Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;

@@ -70,25 +70,28 @@ int point_batch(const mko_batch* templ, const int32_t* template_index, int templ
 // extrema [V][S][2] = {largest ray height, smallest cos I} over the rays with status Ok
 // (surface_extrema_cpu, src/batch/batch_extrema.cpp:75). ext_partial == NULL: no extrema.
 // `surfaces`: bit s set = surface s is reduced (the others stay at the neutral 0 / 1).
-// tuple_mask (NULL: every tuple; mode 1 with extrema only): tuple t of variant v is traced
-// where tuple_mask[(v / mask_group) * T + t] is set, the others leave neutral partials.
 int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
                   double* residuals, int64_t stride, double* opd, uint8_t* status,
                   double* partial, int* partial_counts, const RayStates* states,
-                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream,
-                  const uint8_t* tuple_mask = nullptr, int mask_group = 1);
+                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream);
 
-// Candidate tuples of the constraint rows' central differences (the tuples whose extrema come
-// within delta of the run's on a reduced surface, or that hold a failed ray) and the extrema of
-// the other tuples: mask [runs][T], nc [runs][S][2], ncount [runs]. See extrema_check.
-int extrema_candidates(int runs, int T, int S, uint64_t surfaces, const double* ep0,
-                       const int* pfailed0, const double* ex0, double delta, uint8_t* mask,
-                       double* nc, int* ncount, void* stream);
-// flag[r] = 1 where the candidates' extrema over the run's per_run difference points moved by
-// D and the other tuples' extrema at the run's point lie within K D of them.
-int extrema_check(int runs, int per_run, int S, uint64_t surfaces, const double* ex,
-                  const double* ex0, const double* nc, const int* ncount, double K, int* flag,
-                  void* stream);
+// Extrema of the constraint rows' differences over candidate rays. base: the runs' points (V
+// runs), fd: their difference batch (V * per_run variants); ex0 [V][S][2] the extrema at the
+// points. Marks the candidate rays (flags [V * rays]; failed rays, heights within delta of the
+// largest on hmask, cos I within delta (1 - cos) of the smallest on cmask), lists them in order
+// (tstart [V][T + 1], list [V][cap], overflow [V]; tcount [V][T], others [V] workspace) and
+// writes ext [V * per_run][S][2] over the candidates (extrema_check tells whether that is the
+// extrema of every ray). FP64 traces only.
+int candidate_extrema(const mko_batch* base, const mko_batch* fd,
+                      const mko_residual_settings* settings, int per_run, uint64_t hmask,
+                      uint64_t cmask, double delta, const double* ex0, uint8_t* flags,
+                      int* tcount, int* others, int* tstart, int* list, int cap, int* overflow,
+                      double* ext, void* stream);
+// flag[r] = 1 where the list overflowed, or the candidates' extrema over the run's per_run points
+// moved by D and the other rays (bounded through delta) lie within K D of them.
+int extrema_check(int runs, int per_run, int S, uint64_t hmask, uint64_t cmask, double delta,
+                  const double* ex, const double* ex0, const int* others, const int* overflow,
+                  double K, int* flag, void* stream);
 
 // FP32 prefilter stats of a device batch [V] (prefilter_kernel + second stage); workspace of
 // 24 bytes per (variant, tuple).

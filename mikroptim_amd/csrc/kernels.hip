@@ -76,10 +76,6 @@ struct TraceParams {
     // device count of the variants to trace (NULL: all V): blocks of variants >= *live exit at
     // once, so that a launch sized for V serves a captured LM cycle whose count the device sets
     const int32_t* live;
-    // kExt traces: tuple t of variant v is traced only where tuple_mask[(v / mask_group) * T + t]
-    // is set (NULL: every tuple); the others leave neutral partials (extrema 0 / 1, merit 0)
-    const uint8_t* tuple_mask;
-    int mask_group;
     int64_t rays, trace_rows, stride;
     double gl_u[MKO_MAX_RINGS], gl_w[MKO_MAX_RINGS];
     double field_weight[kMaxFields];
@@ -413,24 +409,6 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     __shared__ double red[kWaves];
     __shared__ int ired[kWaves];
     if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
-    if constexpr (kExt) {
-        if (P.tuple_mask) {
-            const int v = int(blockIdx.x) / P.T, tt = int(blockIdx.x) - v * P.T;
-            if (!P.tuple_mask[size_t(v / P.mask_group) * P.T + tt]) {
-                if (int(threadIdx.x) < P.S) {
-                    double* out = ext_partial + (size_t(blockIdx.x) * P.S + threadIdx.x) * 2;
-                    out[0] = 0.0;
-                    out[1] = 1.0;
-                }
-                if (kMode == 1 && threadIdx.x == 0) {
-                    partial[blockIdx.x] = 0.0;
-                    pfailed[blockIdx.x] = 0;
-                    phard[blockIdx.x] = 0;
-                }
-                return;
-            }
-        }
-    }
     const Tuple t = tuple_of(P, blockIdx.x);
     stage_surfaces(P, t, L);
     const size_t vf = size_t(t.v) * P.F + t.f;
@@ -786,60 +764,22 @@ __global__ void extrema_finalize_kernel(int V, int T, int S, const double* parti
     out[i * 2 + 1] = c;
 }
 
-// Candidate tuples of the rows' central differences, one thread per run: from the tuple partials
-// ep0 [runs][T][S][2] (largest height^2, smallest cos I) of the runs' points and their extrema
-// ex0 [runs][S][2], tuple t is a candidate where, on a surface of `surfaces`, its largest height
-// reaches (1 - delta) of the run's or its smallest cos I lies within delta (1 - cos) of the run's,
-// or where it holds a failed ray (pfailed0 [runs][T]). mask [runs][T]; nc [runs][S][2] = largest
-// height / smallest cos I over the other tuples (0 / 1 without any); ncount [runs] = their number.
-__global__ void extrema_candidates_kernel(int runs, int T, int S, uint64_t surfaces,
-                                          const double* ep0, const int* pfailed0,
-                                          const double* ex0, double delta, uint8_t* mask,
-                                          double* nc, int* ncount) {
+// Check of the candidate rays after the differences' traces, one thread per run: on every height
+// surface the candidates' largest height over the run's per_run points (ex [runs * per_run][S][2])
+// moved by at most D from the run's (ex0); the other rays lie below (1 - delta) of the run's
+// largest height at its point and would have to move by more than K D to reach the candidates'
+// smallest; likewise for the smallest cos I on the incidence surfaces (the others above
+// cmin + delta (1 - cmin)). flag[r] = 1 where that margin is missing or the list overflowed.
+__global__ void extrema_check_kernel(int runs, int per_run, int S, uint64_t hmask, uint64_t cmask,
+                                     double delta, const double* ex, const double* ex0,
+                                     const int* others, const int* overflow, double K,
+                                     int* flag) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= runs) return;
-    const double* e0 = ex0 + size_t(r) * S * 2;
-    int others = 0;
-    for (int s = 0; s < S; ++s) {
-        nc[(size_t(r) * S + s) * 2] = 0.0;
-        nc[(size_t(r) * S + s) * 2 + 1] = 1.0;
-    }
-    for (int t = 0; t < T; ++t) {
-        const double* p = ep0 + (size_t(r) * T + t) * S * 2;
-        bool cand = pfailed0[size_t(r) * T + t] != 0;
-        for (int s = 0; s < S && !cand; ++s) {
-            if (!((surfaces >> s) & 1ull)) continue;
-            const double hmax = e0[2 * s], cmin = e0[2 * s + 1];
-            cand = __dsqrt_rn(p[2 * s]) >= (1.0 - delta) * hmax ||
-                   p[2 * s + 1] <= cmin + delta * (1.0 - cmin);
-        }
-        mask[size_t(r) * T + t] = cand ? 1 : 0;
-        if (!cand) {
-            ++others;
-            for (int s = 0; s < S; ++s) {
-                double* q = nc + (size_t(r) * S + s) * 2;
-                const double h = __dsqrt_rn(p[2 * s]);
-                q[0] = h > q[0] ? h : q[0];
-                q[1] = p[2 * s + 1] < q[1] ? p[2 * s + 1] : q[1];
-            }
-        }
-    }
-    ncount[r] = others;
-}
-
-// Check of the candidate tuples after the differences' traces, one thread per run: on every
-// surface of `surfaces` the candidates' extrema over the run's per_run points (ex [runs *
-// per_run][S][2]) moved by at most D from the run's (ex0); the other tuples would have to move
-// by more than K D to reach them (nc). flag[r] = 1 where that margin is missing.
-__global__ void extrema_check_kernel(int runs, int per_run, int S, uint64_t surfaces,
-                                     const double* ex, const double* ex0, const double* nc,
-                                     const int* ncount, double K, int* flag) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= runs) return;
-    int bad = 0;
-    if (ncount[r] > 0) {
+    int bad = overflow[r];
+    if (!bad && others[r] > 0) {
         for (int s = 0; s < S; ++s) {
-            if (!((surfaces >> s) & 1ull)) continue;
+            if (!(((hmask | cmask) >> s) & 1ull)) continue;
             const double h0 = ex0[(size_t(r) * S + s) * 2], c0 = ex0[(size_t(r) * S + s) * 2 + 1];
             double dh = 0.0, dc = 0.0, hmin = h0, cmax = c0;
             for (int j = 0; j < per_run; ++j) {
@@ -849,11 +789,208 @@ __global__ void extrema_check_kernel(int runs, int per_run, int S, uint64_t surf
                 hmin = fmin(hmin, e[0]);
                 cmax = fmax(cmax, e[1]);
             }
-            const double hn = nc[(size_t(r) * S + s) * 2], cn = nc[(size_t(r) * S + s) * 2 + 1];
-            if (!(hn + K * dh < hmin) || !(cn - K * dc > cmax)) bad = 1;
+            if (((hmask >> s) & 1ull) && !((1.0 - delta) * h0 + K * dh < hmin)) bad = 1;
+            if (((cmask >> s) & 1ull) && !(c0 + delta * (1.0 - c0) - K * dc > cmax)) bad = 1;
         }
     }
     flag[r] = bad;
+}
+
+// ---- candidate rays of the constraint rows' central differences ----------------------------
+
+// Per-ray candidates at the runs' points: one 256-lane workgroup per (run, tuple) of the points'
+// batch (variant = run), each ray traced as trace_ray_ext traces it. A ray is a candidate where it
+// does not end with status Ok (at a difference point it may), or where on a height surface
+// (hmask) its height reaches (1 - delta) of the run's largest, or on an incidence surface (cmask)
+// its cos I lies within delta (1 - cos) of the run's smallest (ex0 [runs][S][2]). flags [runs *
+// rays] (1 = candidate), tcount [runs][T] candidates per tuple, others [runs] (atomic count of the
+// non-candidates).
+template <bool kSoft>
+__global__ __launch_bounds__(kBlock) void ray_candidates_kernel(TraceParams P, uint64_t hmask,
+                                                               uint64_t cmask, double delta,
+                                                               const double* ex0, uint8_t* flags,
+                                                               int* tcount, int* others) {
+    __shared__ SurfLds L;
+    __shared__ double thr[2][MKO_MAX_SURFACES];
+    __shared__ int red[kWaves];
+    const Tuple t = tuple_of(P, blockIdx.x);
+    if (int(threadIdx.x) < P.S) {
+        const double hmax = ex0[(size_t(t.v) * P.S + threadIdx.x) * 2];
+        const double cmin = ex0[(size_t(t.v) * P.S + threadIdx.x) * 2 + 1];
+        thr[0][threadIdx.x] = (1.0 - delta) * hmax;
+        thr[1][threadIdx.x] = cmin + delta * (1.0 - cmin);
+    }
+    stage_surfaces(P, t, L); // synchronises
+    const size_t vf = size_t(t.v) * P.F + t.f;
+    const double a = P.pupil_radius[t.v];
+    double sin_t, cos_t;
+    sincos(P.field_angle[vf], &sin_t, &cos_t);
+    const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+    const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+    const uint64_t mask = hmask | cmask;
+    int cands = 0, rest = 0;
+    for (int k = threadIdx.x; k < t.n; k += kBlock) {
+        const Sample smp = sample_of(P, t, k);
+        mkod::RayT<double> r = ray_as<double>(source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]));
+        Soft soft{0.0, MKO_STATUS_OK};
+        bool cand = false;
+        auto mark = [&](int sf, double x, double y, double z, double c, double dx, double dy,
+                        double dz) {
+            if (!((mask >> sf) & 1ull)) return;
+            double h2 = 0.0, ci = 1.0;
+            surface_extremum(x, y, z, c, dx, dy, dz, h2, ci);
+            if (((hmask >> sf) & 1ull) && __dsqrt_rn(h2) >= thr[0][sf]) cand = true;
+            if (((cmask >> sf) & 1ull) && ci <= thr[1][sf]) cand = true;
+        };
+        mark(0, r.p.x, r.p.y, r.p.z, 0.0, r.d.x, r.d.y, r.d.z);
+        uint8_t st = MKO_STATUS_OK;
+        for (int sf = 1; sf < P.S; ++sf) {
+            const double dx = r.d.x, dy = r.d.y, dz = r.d.z;
+            uint8_t branch = 0;
+            const uint8_t sst = mkod::surface_step<kSoft, kTraceFastMath>(
+                r, L.c[sf], L.t[sf - 1], L.n[sf - 1], L.mu[sf], soft, branch);
+            if (sst != MKO_STATUS_OK) {
+                st = sst;
+                break;
+            }
+            mark(sf, r.p.x, r.p.y, r.p.z, L.c[sf], dx, dy, dz);
+        }
+        if (st == MKO_STATUS_OK) {
+            if (!(r.d.z > 0.0)) {
+                st = MKO_STATUS_BACKWARD;
+            } else {
+                uint8_t rb = 0;
+                const uint8_t rst = mkod::reference_step<kSoft, kTraceFastMath>(
+                    r, M, kappa, P.ref_kind, L.n[P.S - 1], soft, rb);
+                st = !mkod::in_piston(rst) ? rst : kSoft ? soft.first : uint8_t(MKO_STATUS_OK);
+            }
+        }
+        if (st != MKO_STATUS_OK) cand = true;
+        flags[size_t(t.v) * P.rays + t.offset + k] = cand ? 1 : 0;
+        cands += cand;
+        rest += !cand;
+    }
+    const int tc = block_sum(cands, red);
+    const int tr = block_sum(rest, red);
+    if (threadIdx.x == 0) {
+        tcount[blockIdx.x] = tc;
+        if (tr) atomicAdd(others + t.v, tr);
+    }
+}
+
+// Ordered candidate list of each run (one 1024-thread workgroup per run): the rows with a flag in
+// ascending order (so grouped by tuple), list [runs][cap]; tstart [runs][T + 1] the tuples'
+// first entries; overflow[run] = 1 where more than cap rays are candidates.
+__global__ __launch_bounds__(1024) void candidate_list_kernel(int T, int64_t rays, int cap,
+                                                              const uint8_t* flags,
+                                                              const int* tcount, int* tstart,
+                                                              int* list, int* overflow) {
+    __shared__ int32_t wave_total[16];
+    __shared__ int32_t base_s;
+    const int r = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        int acc = 0;
+        for (int t = 0; t < T; ++t) {
+            tstart[size_t(r) * (T + 1) + t] = acc;
+            acc += tcount[size_t(r) * T + t];
+        }
+        tstart[size_t(r) * (T + 1) + T] = acc;
+        overflow[r] = acc > cap ? 1 : 0;
+        base_s = 0;
+    }
+    __syncthreads();
+    if (tstart[size_t(r) * (T + 1) + T] > cap) return;
+    const uint8_t* f = flags + size_t(r) * rays;
+    for (int64_t start = 0; start < rays; start += 1024) {
+        const int64_t e = start + tid;
+        const bool pick = e < rays && f[e] != 0;
+        const uint64_t m = __ballot(pick);
+        const int before = __popcll(m & ((uint64_t(1) << lane) - 1));
+        if (lane == 0) wave_total[wave] = __popcll(m);
+        __syncthreads();
+        int offset = base_s;
+        for (int w = 0; w < wave; ++w) offset += wave_total[w];
+        if (pick) list[size_t(r) * cap + offset + before] = int(e);
+        __syncthreads();
+        if (tid == 0) {
+            int sum = 0;
+            for (int w = 0; w < 16; ++w) sum += wave_total[w];
+            base_s += sum;
+        }
+        __syncthreads();
+    }
+}
+
+// The extrema [V][S][2] of the differences' batch over the candidate rays of each variant's run
+// (run = v / per_run): one wavefront per variant, the tuples holding candidates in order, their
+// rays traced and reduced exactly as trace_kernel's extrema trace (trace_ray_ext; a wave with a
+// ray that does not end Ok traces its valid rays again), then sqrt of the largest height^2 and
+// the smallest cos I (extrema_finalize_kernel). Equal to the extrema over every ray wherever the
+// extremal ray of each surface is a candidate.
+template <bool kSoft>
+__global__ __launch_bounds__(64) void candidate_extrema_kernel(TraceParams P, int per_run,
+                                                               int cap, const int* tstart,
+                                                               const int* list, double* ext) {
+    __shared__ SurfLds L;
+    const int v = blockIdx.x;
+    const int run = v / per_run;
+    const int lane = threadIdx.x;
+    const int* ts = tstart + size_t(run) * (P.T + 1);
+    const int* lst = list + size_t(run) * cap;
+    double eh = 0.0, ec = 1.0;
+    // an overflowed list was not written: neutral extrema (the check sends the chunk to the
+    // whole trace)
+    const int ntuples = ts[P.T] > cap ? 0 : P.T;
+    for (int tt = 0; tt < ntuples; ++tt) {
+        const int lo = ts[tt], hi = ts[tt + 1];
+        if (lo == hi) continue; // uniform across the wave
+        const Tuple t = tuple_of(P, v * P.T + tt);
+        stage_surfaces(P, t, L);
+        const size_t vf = size_t(t.v) * P.F + t.f;
+        const double a = P.pupil_radius[t.v];
+        double sin_t, cos_t;
+        sincos(P.field_angle[vf], &sin_t, &cos_t);
+        const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
+        const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
+        for (int base = lo; base < hi; base += 64) {
+            const int e = base + lane;
+            const bool have = e < hi;
+            Sample smp{};
+            Ray r0{};
+            if (have) {
+                const int k = int(int64_t(lst[e]) - t.offset);
+                smp = sample_of(P, t, k);
+                r0 = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
+            }
+            mkod::RayT<double> r = ray_as<double>(r0);
+            Soft soft{0.0, MKO_STATUS_OK};
+            double pend_h = 0.0, pend_c = 1.0;
+            NoRecord rec;
+            const uint8_t sti = trace_ray_ext<kSoft, kTraceFastMath>(
+                P, L, r, M, kappa, soft, have, lane, pend_h, pend_c, false, rec);
+            const uint64_t have_mask = __ballot(have);
+            const uint64_t ok_mask = __ballot(have && sti == MKO_STATUS_OK);
+            if (ok_mask == have_mask) {
+                if (lane < P.S) {
+                    eh = pend_h > eh ? pend_h : eh;
+                    ec = pend_c < ec ? pend_c : ec;
+                }
+            } else {
+                mkod::RayT<double> rr = ray_as<double>(r0);
+                Soft s2{0.0, MKO_STATUS_OK};
+                NoRecord nr;
+                trace_ray_ext<kSoft, kTraceFastMath>(P, L, rr, M, kappa, s2,
+                                                     have && sti == MKO_STATUS_OK, lane, eh, ec,
+                                                     true, nr);
+            }
+        }
+        __syncthreads(); // L is restaged for the next tuple
+    }
+    if (lane < P.S) {
+        ext[(size_t(v) * P.S + lane) * 2] = __dsqrt_rn(eh);
+        ext[(size_t(v) * P.S + lane) * 2 + 1] = ec;
+    }
 }
 
 // Constraint rows of every variant (one thread per variant), rows[v * stride + offset + k] for
@@ -2807,17 +2944,10 @@ int merit_states(const mko_batch* batch, const mko_residual_settings* settings, 
 int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings, int mode,
                   double* residuals, int64_t stride, double* opd, uint8_t* status,
                   double* partial, int* partial_counts, const RayStates* states,
-                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream,
-                  const uint8_t* tuple_mask, int mask_group) {
+                  double* ext_partial, double* extrema, uint64_t surfaces, void* stream) {
     MKO_TRY({
         TraceParams P = make_params(*batch, *settings);
         P.ext_mask = surfaces;
-        if (tuple_mask) {
-            if (!ext_partial || mode != 1 || mask_group < 1)
-                throw std::invalid_argument("a tuple mask needs the MERIT trace with extrema");
-            P.tuple_mask = tuple_mask;
-            P.mask_group = mask_group;
-        }
         if (mode == 0) {
             if (stride < P.trace_rows) throw std::invalid_argument("stride smaller than trace rows");
             P.stride = stride;
@@ -2837,26 +2967,51 @@ int trace_extrema(const mko_batch* batch, const mko_residual_settings* settings,
     })
 }
 
-int extrema_candidates(int runs, int T, int S, uint64_t surfaces, const double* ep0,
-                       const int* pfailed0, const double* ex0, double delta, uint8_t* mask,
-                       double* nc, int* ncount, void* stream) {
+int candidate_extrema(const mko_batch* base, const mko_batch* fd,
+                      const mko_residual_settings* settings, int per_run, uint64_t hmask,
+                      uint64_t cmask, double delta, const double* ex0, uint8_t* flags,
+                      int* tcount, int* others, int* tstart, int* list, int cap, int* overflow,
+                      double* ext, void* stream) {
     MKO_TRY({
-        if (runs <= 0) return 0;
-        hipLaunchKernelGGL(extrema_candidates_kernel, dim3((runs + 127) / 128), dim3(128), 0,
-                           static_cast<hipStream_t>(stream), runs, T, S, surfaces, ep0, pfailed0,
-                           ex0, delta, mask, nc, ncount);
-        check_hip(hipGetLastError(), "extrema candidates launch");
+        TraceParams Pb = make_params(*base, *settings);
+        TraceParams Pf = make_params(*fd, *settings);
+        if (settings->precision != MKO_PRECISION_FP64)
+            throw std::invalid_argument("candidate rays trace in FP64");
+        if (Pf.V != Pb.V * per_run || Pf.T != Pb.T || Pf.rays != Pb.rays)
+            throw std::invalid_argument("candidate rays: batch shapes differ");
+        if (Pb.V == 0) return 0;
+        Pb.ext_mask = Pf.ext_mask = hmask | cmask;
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        check_hip(hipMemsetAsync(others, 0, size_t(Pb.V) * sizeof(int), s), "memset");
+        const dim3 gb(unsigned(Pb.V) * unsigned(Pb.T));
+        if (Pb.soft)
+            hipLaunchKernelGGL(ray_candidates_kernel<true>, gb, dim3(kBlock), 0, s, Pb, hmask,
+                               cmask, delta, ex0, flags, tcount, others);
+        else
+            hipLaunchKernelGGL(ray_candidates_kernel<false>, gb, dim3(kBlock), 0, s, Pb, hmask,
+                               cmask, delta, ex0, flags, tcount, others);
+        check_hip(hipGetLastError(), "ray candidates launch");
+        hipLaunchKernelGGL(candidate_list_kernel, dim3(unsigned(Pb.V)), dim3(1024), 0, s, Pb.T,
+                           Pb.rays, cap, flags, tcount, tstart, list, overflow);
+        check_hip(hipGetLastError(), "candidate list launch");
+        if (Pf.soft)
+            hipLaunchKernelGGL(candidate_extrema_kernel<true>, dim3(unsigned(Pf.V)), dim3(64), 0,
+                               s, Pf, per_run, cap, tstart, list, ext);
+        else
+            hipLaunchKernelGGL(candidate_extrema_kernel<false>, dim3(unsigned(Pf.V)), dim3(64), 0,
+                               s, Pf, per_run, cap, tstart, list, ext);
+        check_hip(hipGetLastError(), "candidate extrema launch");
     })
 }
 
-int extrema_check(int runs, int per_run, int S, uint64_t surfaces, const double* ex,
-                  const double* ex0, const double* nc, const int* ncount, double K, int* flag,
-                  void* stream) {
+int extrema_check(int runs, int per_run, int S, uint64_t hmask, uint64_t cmask, double delta,
+                  const double* ex, const double* ex0, const int* others, const int* overflow,
+                  double K, int* flag, void* stream) {
     MKO_TRY({
         if (runs <= 0) return 0;
         hipLaunchKernelGGL(extrema_check_kernel, dim3((runs + 127) / 128), dim3(128), 0,
-                           static_cast<hipStream_t>(stream), runs, per_run, S, surfaces, ex, ex0,
-                           nc, ncount, K, flag);
+                           static_cast<hipStream_t>(stream), runs, per_run, S, hmask, cmask,
+                           delta, ex, ex0, others, overflow, K, flag);
         check_hip(hipGetLastError(), "extrema check launch");
     })
 }

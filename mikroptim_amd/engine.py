@@ -265,8 +265,8 @@ class TraceEngine:
         """Launch shapes of this engine's kernels (mko_tuning): returns the current (pack_rays,
         fused_waves) and sets those given (0 = automatic; pack_rays 32 / 64 / 128, 256 = no
         tuple packs; fused_waves 1 / 2 / 4). full_row_traces (True: the constraint rows'
-        differences trace every tuple instead of the checked candidate tuples) is set when
-        given and not part of the returned pair."""
+        differences trace every ray instead of the checked candidate rays) is set when given
+        and not part of the returned pair."""
         t = abi.Tuning()
         _check(library().mko_engine_get_tuning(self._h, C.byref(t)))
         old = (t.pack_rays, t.fused_waves)
@@ -281,7 +281,7 @@ class TraceEngine:
         return old
 
     def row_trace_stats(self):
-        """(chunks whose constraint-row differences traced only candidate tuples, chunks traced
+        """(chunks whose constraint-row differences traced only candidate rays, chunks traced
         whole again after the check)."""
         a, b = C.c_int64(0), C.c_int64(0)
         _check(library().mko_engine_row_trace_stats(self._h, C.byref(a), C.byref(b)))

@@ -210,11 +210,12 @@ def test_lm_with_rows_mixed_objective(engine):
     assert np.all(rd.merit < m0)
 
 
-def test_row_differences_on_candidate_tuples_equal_full_traces(engine):
+def test_row_differences_on_candidate_rays_equal_full_traces(engine):
     """The adjoint paths difference the rows from extrema over the 2P + 1 points of each run; at
-    the difference points only the candidate tuples (extrema within 1e-3 of the run's, or failed
-    rays) are traced. Stored-J and mixed normal equations equal those of tracing every tuple bit
-    for bit, on the objective (35 tuples) and the singlet in soft mode with failed rays."""
+    the difference points only the candidate rays (heights / cos I within 1e-3 of the run's
+    extrema, or not ending with status Ok) are traced. Stored-J and mixed normal equations equal
+    those of tracing every ray bit for bit, on the objective and the singlet in soft mode with
+    failed rays."""
     sb, sv, ss = _singlet(6, failure=abi.FAILURE_SOFT)
     sb.pupil_radius[1] = 110.0  # outer rings miss: failed rays in the soft trace
     cases = [(*_objective(2), 2), (sb, sv, ss, 3)]
