@@ -2368,16 +2368,18 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
 
 // ---- Gram: A = [J r0]^T [J r0] per run -----------------------------------------------------
 
-// 16 x 16 output tile per workgroup over the upper triangle of the (P+1) x (P+1) matrix;
-// rows staged through LDS in chunks of 64. Each element is one FMA chain over the rows in
-// ascending order starting from 0 (gram_dot, normal_equations.hpp:19), products of the stored
-// J values: bitwise the host definition.
+// 64 x 64 output tile per workgroup over the upper tiles of the (P+1) x (P+1) matrix, 256 threads
+// each holding a 4 x 4 block of elements: 16 independent FMA chains per thread (the chains' latency
+// is what bounds a Gram of long columns). Rows are staged through LDS 32 at a time, row-major
+// [row][column], so a thread reads its 4 consecutive columns as two 16-byte words. Each element is
+// one FMA chain over the rows in ascending order starting from 0 (gram_dot,
+// normal_equations.hpp:19), products of the stored J values: bitwise the host definition.
 __global__ __launch_bounds__(256) void gram_kernel(const double* J, const double* r0, int runs, int P,
                                                    int64_t rows, double* a, double* g,
                                                    const int32_t* live) {
-    constexpr int kT = 16, kC = 64;
-    __shared__ double As[kT][kC + 1];
-    __shared__ double Bs[kT][kC + 1];
+    constexpr int kT = 64, kC = 32, kLd = kT + 2; // row stride in doubles (16-byte aligned rows)
+    __shared__ __align__(16) double As[kC][kLd];
+    __shared__ __align__(16) double Bs[kC][kLd];
     const int T = (P + 1 + kT - 1) / kT;
     const int tiles = T * (T + 1) / 2;
     const int run = blockIdx.x / tiles;
@@ -2388,38 +2390,60 @@ __global__ __launch_bounds__(256) void gram_kernel(const double* J, const double
         tile -= T - tq;
         ++tq;
     }
-    const int tp = tq + tile; // tp >= tq
+    const int tp = tq + tile; // column tile >= row tile
+    const bool diag = tp == tq;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int p = tp * kT + tx; // row of A tile (column of J)
-    const int q = tq * kT + ty;
     const double* Jr = J + size_t(run) * P * rows;
     const double* rr = r0 + size_t(run) * rows;
-    double acc = 0.0;
-    for (int64_t r0c = 0; r0c < rows; r0c += kC) {
+    auto column = [&](int col, int64_t row) {
+        return col < P ? Jr[size_t(col) * rows + row] : col == P ? rr[row] : 0.0;
+    };
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    const double(*Bsrc)[kLd] = diag ? As : Bs;
+    for (int64_t base = 0; base < rows; base += kC) {
+        const int n = int(rows - base < kC ? rows - base : kC);
         for (int e = threadIdx.x; e < kT * kC; e += 256) {
-            const int cc = e / kC, rw = e - cc * kC;
-            const int64_t row = r0c + rw;
-            const int pc = tp * kT + cc, qc = tq * kT + cc;
-            double va = 0.0, vb = 0.0;
-            if (row < rows) {
-                va = pc < P ? Jr[size_t(pc) * rows + row] : pc == P ? rr[row] : 0.0;
-                vb = qc < P ? Jr[size_t(qc) * rows + row] : qc == P ? rr[row] : 0.0;
-            }
-            As[cc][rw] = va;
-            Bs[cc][rw] = vb;
+            const int c = e / kC, rw = e - c * kC;
+            const int64_t row = base + rw;
+            As[rw][c] = rw < n ? column(tq * kT + c, row) : 0.0;
+            if (!diag) Bs[rw][c] = rw < n ? column(tp * kT + c, row) : 0.0;
         }
         __syncthreads();
-        const int n = int(rows - r0c < kC ? rows - r0c : kC);
-        for (int rw = 0; rw < n; ++rw) acc = __fma_rn(As[tx][rw], Bs[ty][rw], acc);
+        for (int rw = 0; rw < n; ++rw) {
+            const double2 a01 = *reinterpret_cast<const double2*>(&As[rw][ty * 4]);
+            const double2 a23 = *reinterpret_cast<const double2*>(&As[rw][ty * 4 + 2]);
+            const double2 b01 = *reinterpret_cast<const double2*>(&Bsrc[rw][tx * 4]);
+            const double2 b23 = *reinterpret_cast<const double2*>(&Bsrc[rw][tx * 4 + 2]);
+            const double av[4] = {a01.x, a01.y, a23.x, a23.y};
+            const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __fma_rn(av[i], bv[j], acc[i][j]);
+        }
         __syncthreads();
     }
-    if (p <= P && q <= P && q <= p) {
-        // element (p, q) with q <= p; mirror into A, column P is g
-        if (p < P && q < P) {
-            a[size_t(run) * P * P + size_t(q) * P + p] = acc;
-            a[size_t(run) * P * P + size_t(p) * P + q] = acc;
-        } else if (p == P && q < P) {
-            g[size_t(run) * P + q] = acc;
+    double* A = a + size_t(run) * P * P;
+    double* gr = g + size_t(run) * P;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int p = tq * kT + ty * 4 + i;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int q = tp * kT + tx * 4 + j;
+            const double v = acc[i][j];
+            if (p < P && q < P) {
+                A[size_t(p) * P + q] = v;
+                A[size_t(q) * P + p] = v;
+            } else if (p < P && q == P) {
+                gr[p] = v;
+            } else if (p == P && q < P) {
+                gr[q] = v;
+            }
         }
     }
 }
@@ -3277,7 +3301,7 @@ int gram_chain(const double* jacobian, const double* r0, int runs, int variables
                double* a, double* g, void* stream, const int32_t* live) {
     MKO_TRY({
         if (runs <= 0) return 0;
-        const int T = (variables + 1 + 15) / 16;
+        const int T = (variables + 1 + 63) / 64;
         const dim3 grid(unsigned(runs) * unsigned(T * (T + 1) / 2));
         hipLaunchKernelGGL(gram_kernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
                            jacobian, r0, runs, variables, rows, a, g, live);
