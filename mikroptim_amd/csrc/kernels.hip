@@ -1760,6 +1760,8 @@ __global__ void vjp_finalize_kernel(int V, int T, int Pn, const double* part, do
 // ---- ADJOINT fused with the normal equations (hard failure mode) ---------------------------
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 constexpr int kMaxVariablesFused = 96;
 constexpr int kFusedSlots = 4; // S / Tr slots per tuple in the workspace, one per wave (<= 4)
@@ -1804,13 +1806,6 @@ struct GradLayout {
     }
 };
 
-__device__ __forceinline__ uint16_t bf16_bits(float x) {
-    // round to nearest even (finite inputs)
-    uint32_t u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-__device__ __forceinline__ float bf16_value(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 __device__ __forceinline__ double first_lane(double x) {
     const int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
@@ -1892,56 +1887,50 @@ __device__ __forceinline__ void backward_fused(const TraceParams& P, const SurfL
     }
 }
 
-// 64-bit lane moves of the transposed reduction below.
-template <int kCtrl>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), kCtrl, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), kCtrl, 0xf, 0xf, false);
+// Lane k's value of x (k wave-uniform).
+__device__ __forceinline__ double lane_value(double x, int k) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), k);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), k);
     return __hiloint2double(hi, lo);
 }
-// x, y -> the sums of the pairs (lane, lane ^ 32) (kHalf 32) or (lane, lane ^ 16) (kHalf 16):
-// lanes below the half keep x's parameter, lanes above y's (v_permlane32_swap /
-// v_permlane16_swap exchange exactly the halves that each lane does not keep).
-template <int kHalf>
-__device__ __forceinline__ double swap_sum(double x, double y) {
-    const unsigned xl = unsigned(__double2loint(x)), xh = unsigned(__double2hiint(x));
-    const unsigned yl = unsigned(__double2loint(y)), yh = unsigned(__double2hiint(y));
-    if constexpr (kHalf == 32) {
-        const auto lo = __builtin_amdgcn_permlane32_swap(xl, yl, false, false);
-        const auto hi = __builtin_amdgcn_permlane32_swap(xh, yh, false, false);
-        return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
-    } else {
-        const auto lo = __builtin_amdgcn_permlane16_swap(xl, yl, false, false);
-        const auto hi = __builtin_amdgcn_permlane16_swap(xh, yh, false, false);
-        return __hiloint2double(int(hi[0]), int(lo[0])) + __hiloint2double(int(hi[1]), int(lo[1]));
+
+// Row sums of one wave's 64 rays in the gradient block, transposed: lane p returns
+// A[h] = sum_k W_k G[p'][k] and B[h] = sum_k V_k G[p'][k] for row p' = p + 64 h (h < kRows), G the
+// stored floats of ray k of the wave, W_k and V_k lane k's values, summed in ray order (one FMA
+// chain per row). Each lane reads its row's 64 floats as 16-byte words; rows >= P give 0.
+template <int kNW, int kRows>
+__device__ __forceinline__ void wave_row_sums(const unsigned char* G, int Pn, int wave, int lane,
+                                              double wk, double vk, double (&A)[2],
+                                              double (&B)[2]) {
+    using Lay = GradLayout<kNW>;
+    A[0] = A[1] = B[0] = B[1] = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < 8; ++j) {
+        float gv[kRows][8];
+#pragma unroll
+        for (int h = 0; h < kRows; ++h) {
+            const int p = lane + 64 * h;
+            float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f), y = x;
+            if (p < Pn) {
+                const unsigned char* at =
+                    G + size_t(p) * Lay::kRowBytes + 32 * ((8 * wave + j) ^ (p & Lay::kMask));
+                x = *reinterpret_cast<const float4*>(at);
+                y = *reinterpret_cast<const float4*>(at + 16);
+            }
+            gv[h][0] = x.x; gv[h][1] = x.y; gv[h][2] = x.z; gv[h][3] = x.w;
+            gv[h][4] = y.x; gv[h][5] = y.y; gv[h][6] = y.z; gv[h][7] = y.w;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double w = lane_value(wk, 8 * j + i), v = lane_value(vk, 8 * j + i);
+#pragma unroll
+            for (int h = 0; h < kRows; ++h) {
+                const double g = double(gv[h][i]);
+                A[h] = __fma_rn(w, g, A[h]);
+                B[h] = __fma_rn(v, g, B[h]);
+            }
+        }
     }
-}
-// Transposed wave sum of 16 values per lane: lane L returns the sum over the 64 lanes of value
-// number L >> 2 (so lanes 4q ... 4q + 3 hold value q). Each step pairs lanes that differ in one
-// more bit and keeps half of the values: bit 5 and bit 4 by the permlane swaps, bit 3 by the row
-// mirror (lane i <-> 15 - i), bit 2 by the half-row mirror (i <-> 7 - i); then bits 1 and 0 sum
-// plainly. A fixed tree: the result does not depend on anything but the 64 x 16 inputs.
-__device__ __forceinline__ double transpose_sum16(const double (&v)[16], int lane) {
-    double a[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) a[j] = swap_sum<32>(v[j], v[j + 8]);
-    double b[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = swap_sum<16>(a[j], a[j + 4]);
-    const bool b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
-    double c[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        const double keep = b3 ? b[j + 2] : b[j];
-        const double send = b3 ? b[j] : b[j + 2];
-        c[j] = keep + dpp_f64<0x140>(send); // row_mirror
-    }
-    const double keep = b2 ? c[1] : c[0];
-    const double send = b2 ? c[0] : c[1];
-    double r = keep + dpp_f64<0x141>(send); // row_half_mirror
-    r += dpp_f64<0x4e>(r);                  // quad_perm 2,3,0,1 (lane ^ 2)
-    r += dpp_f64<0xb1>(r);                  // quad_perm 1,0,3,2 (lane ^ 1)
-    return r;
 }
 
 template <int kNW, class T>
@@ -2096,9 +2085,14 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
                 for (int c = 0; c < 7; ++c) st[c] = states.data[state_index(states, t.v, c, row)];
             }
         }
-        // this thread's entries of the block and this wave's pilots start at zero
-        for (int p = 0; p < Pn; ++p) *reinterpret_cast<float*>(G + Lay::float_off(p, tid)) = 0.0f;
+        // the block (all threads, 16-byte words) and this wave's pilots start at zero
+        {
+            float4* z = reinterpret_cast<float4*>(G);
+            const int words = Pn * (Lay::kRowBytes / 16);
+            for (int e = tid; e < words; e += kThreads) z[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         for (int p = lane; p < Pn; p += 64) own_pilot[p] = 0.0;
+        __syncthreads();
         double wk = 0.0, opl = 0.0;
         bool ok = false;
         Sample smp{};
@@ -2132,22 +2126,39 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
                                    SwzSink<kNW>{G, own_pilot, tid, leader}, t.l);
         }
         __syncthreads();
+        // this wave's weight sum W_w; the first chunk with a reached ray fixes o = sum W OPL / W
+        const double ww = wave_sum(wk);
+        double W = 0.0;
         if (!shift_ready) {
-            // c = sum_w W_w pilot_w / W and o = sum W OPL / W over this chunk (every thread
-            // forms the same values from the waves' sums)
-            const double ww = wave_sum(wk), wo = wave_sum(wk * opl);
+            const double wo = wave_sum(wk * opl);
             if (lane == 0) {
                 wsum_l[wave] = ww;
                 osum_l[wave] = wo;
             }
             __syncthreads();
-            double W = 0.0, WO = 0.0;
+            double WO = 0.0;
 #pragma unroll
             for (int w = 0; w < kNW; ++w) {
                 W += wsum_l[w];
                 WO += osum_l[w];
             }
             if (!(W > 0.0)) continue; // no reached ray yet: nothing of this chunk counts
+            o_shift = WO / W;
+        }
+        // A_p = sum_k W_k G_pk and B_p = sum_k W_k (OPL_k - o) G_pk over the wave's rays, G the
+        // stored floats relative to the wave's pilot: the parts of S and Tr that do not depend on
+        // the shift c (S = A + (pilot - c) W_w, Tr = B + (pilot - c) V_w)
+        const double ow = ok ? opl - o_shift : 0.0;
+        const double vk = wk * ow;
+        double A[2], B[2];
+        if constexpr (kPMax > 64) {
+            if (Pn > 64) wave_row_sums<kNW, 2>(G, Pn, wave, lane, wk, vk, A, B);
+            else wave_row_sums<kNW, 1>(G, Pn, wave, lane, wk, vk, A, B);
+        } else {
+            wave_row_sums<kNW, 1>(G, Pn, wave, lane, wk, vk, A, B);
+        }
+        const double vw = wave_sum(vk);
+        if (!shift_ready) {
             // c0: the weight-averaged pilot of the waves; then c = c0 + S0 / W with S0 the sum of
             // W (g - c0) over the chunk, i.e. the chunk's weighted mean gradient (the waves' S0
             // pass through their slots of the workspace, summed in wave order)
@@ -2157,25 +2168,11 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
                 for (int w = 0; w < kNW; ++w) c += wsum_l[w] * pilot[w * Pn + p];
                 shift[p] = c / W;
             }
-            o_shift = WO / W;
-            shift_ready = true;
             __syncthreads();
 #pragma unroll
-            for (int g = 0; g < kGroups; ++g) {
-                if (g * 16 < Pn) {
-                    double v[16];
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int p = g * 16 + j;
-                        v[j] = p < Pn ? wk * (double(*reinterpret_cast<const float*>(
-                                                  G + Lay::float_off(p, tid))) +
-                                              (own_pilot[p] - shift[p]))
-                                      : 0.0;
-                    }
-                    const double r = transpose_sum16(v, lane);
-                    const int pq = g * 16 + (lane >> 2);
-                    if ((lane & 3) == 0 && pq < Pn) slot_st[pq] = r;
-                }
+            for (int h = 0; h < 2; ++h) {
+                const int p = lane + 64 * h;
+                if (p < Pn) slot_st[p] = A[h] + (own_pilot[p] - shift[p]) * ww;
             }
             __threadfence_block();
             __syncthreads();
@@ -2186,39 +2183,31 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
                 for (int w = 1; w < kNW; ++w) s0 += st0[w * 2 * Pn];
                 shift[p] += s0 / W;
             }
+            shift_ready = true;
             __syncthreads();
         }
-        // this wave's pilots become its deltas pilot - c
-        for (int p = lane; p < Pn; p += 64) own_pilot[p] -= shift[p];
-        __syncthreads();
-        // S and Tr of this wave's rays: W (g - c) = W (G + delta) per parameter, then the same
-        // times (OPL - o); one sum at a time keeps 16 values live
-        const double ow = ok ? opl - o_shift : 0.0;
+        // S and Tr of this wave's rays into its slot; the wave's deltas pilot - c, as floats
+        // over its pilots (read by the bf16 conversion below)
+        float* delta = reinterpret_cast<float*>(own_pilot);
+        double d[2] = {0.0, 0.0};
 #pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-            if (g * 16 < Pn) {
-#pragma unroll
-                for (int pass = 0; pass < 2; ++pass) {
-                    const double f = pass == 0 ? wk : wk * ow;
-                    double v[16];
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int p = g * 16 + j;
-                        v[j] = p < Pn ? f * (double(*reinterpret_cast<const float*>(
-                                                 G + Lay::float_off(p, tid))) +
-                                             own_pilot[p])
-                                      : 0.0;
-                    }
-                    const double r = transpose_sum16(v, lane);
-                    // lanes 4q hold parameter 16 g + q: this wave's slot of the tuple's S / Tr
-                    const int pq = g * 16 + (lane >> 2);
-                    if ((lane & 3) == 0 && pq < Pn) {
-                        double* dst = slot_st + pass * Pn + pq;
-                        *dst = accumulated ? *dst + r : r;
-                    }
-                }
+        for (int h = 0; h < 2; ++h) {
+            const int p = lane + 64 * h;
+            if (p < Pn) {
+                d[h] = own_pilot[p] - shift[p];
+                double* dst = slot_st + p;
+                const double s = A[h] + d[h] * ww, tr = B[h] + d[h] * vw;
+                dst[0] = accumulated ? dst[0] + s : s;
+                dst[Pn] = accumulated ? dst[Pn] + tr : tr;
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int p = lane + 64 * h;
+            if (p < Pn) delta[p] = float(d[h]);
+        }
+        __syncthreads();
         accumulated = true;
         // u = sqrt(W) (g - c) of the thread's ray, split into bf16 hi + lo in place (the ray's
         // 32-byte group belongs to 8 lanes of this wave: its floats are read before they are
@@ -2226,15 +2215,41 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
         {
             const float swk = __builtin_sqrtf(float(wk));
             const int grp = tid >> 3, i = tid & 7;
-            for (int p = 0; p < Pn; ++p) {
-                const float u =
-                    swk * (*reinterpret_cast<const float*>(G + Lay::float_off(p, tid)) +
-                           float(own_pilot[p]));
-                const uint16_t hi = bf16_bits(u);
-                const uint16_t lo = bf16_bits(u - bf16_value(hi));
-                const int s = Lay::slot(grp, p);
-                *reinterpret_cast<uint16_t*>(G + Lay::hi_off(p, s) + 2 * i) = hi;
-                *reinterpret_cast<uint16_t*>(G + Lay::lo_off(p, s) + 2 * i) = lo;
+            // rows in blocks of 16: for p = p0 + j (p0 a multiple of 16) the swizzle p & kMask is
+            // m = j & kMask, so float_off(p, tid) = p RB + (4 tid ^ 32 m), hi_off(p, grp ^ m) + 2 i
+            // = p RB + (hbase ^ (32 m + 16 ((m >> 3) & 1))) and lo_off(...) + 2 i = that ^ 16;
+            // two rows at a time share the packed bf16 conversions
+            const int fbase = 4 * tid;
+            const int hbase = 32 * grp + 16 * ((grp >> 3) & 1) + 2 * i;
+            for (int p0 = 0; p0 < Pn; p0 += 16) {
+                unsigned char* row = G + size_t(p0) * Lay::kRowBytes;
+#pragma unroll 2
+                for (int j = 0; j < 16; j += 2) {
+                    if (p0 + j < Pn) {
+                        const bool two = p0 + j + 1 < Pn;
+                        const int m0 = j & Lay::kMask, m1 = (j + 1) & Lay::kMask;
+                        unsigned char* r0 = row + j * Lay::kRowBytes;
+                        unsigned char* r1 = r0 + Lay::kRowBytes;
+                        f32x2 u;
+                        u.x = swk * (*reinterpret_cast<const float*>(r0 + (fbase ^ (32 * m0))) +
+                                     delta[p0 + j]);
+                        u.y = two ? swk * (*reinterpret_cast<const float*>(
+                                               r1 + (fbase ^ (32 * m1))) +
+                                           delta[p0 + j + 1])
+                                  : 0.0f;
+                        const bf16x2 hi = __builtin_convertvector(u, bf16x2);
+                        const f32x2 d{u.x - float(hi[0]), u.y - float(hi[1])};
+                        const bf16x2 lo = __builtin_convertvector(d, bf16x2);
+                        const int h0 = hbase ^ (32 * m0 + 16 * ((m0 >> 3) & 1));
+                        const int h1 = hbase ^ (32 * m1 + 16 * ((m1 >> 3) & 1));
+                        *reinterpret_cast<__bf16*>(r0 + h0) = hi[0];
+                        *reinterpret_cast<__bf16*>(r0 + (h0 ^ 16)) = lo[0];
+                        if (two) {
+                            *reinterpret_cast<__bf16*>(r1 + h1) = hi[1];
+                            *reinterpret_cast<__bf16*>(r1 + (h1 ^ 16)) = lo[1];
+                        }
+                    }
+                }
             }
         }
         __syncthreads();
