@@ -13,14 +13,13 @@ src/optim/mechanical_constraints.cpp:87-146) and the optimize output (docs/cli.m
 Supported is what this path computes: stop / plane / sphere surfaces without decenter or tilt,
 catalogue (Sellmeier, polynomial), constant and Buchdahl materials, collimated and point sources,
 the three reference spheres with the ideal image point, curvature / thickness / working-distance
-variables, glass variables on Buchdahl and constant media (a constant medium becomes Buchdahl
-with n0 = n, as convert_glass_variables does), paraxial (focal length, parfocal length, working
-distance, paraxial focus, Petzval, sine condition per zone, distortion per field) and mechanical
-constraints. Ghost plates, ghost-coefficient variables, glass variables on catalogue glasses
-(they need the Buchdahl fit), the ghost-penalty and glass-hull (with glass variables)
-constraints, smoothed extrema and traced reference points are rejected with a FormatError
-naming the field; tolerances and a glass hull without glass variables are ignored with a
-warning.
+variables, glass variables on every glass medium (as convert_glass_variables does: a catalogue
+glass becomes the Buchdahl fit of its index, a constant medium Buchdahl with n0 = n), paraxial
+(focal length, parfocal length, working distance, paraxial focus, Petzval, sine condition per
+zone, distortion per field) and mechanical constraints. Ghost plates, ghost-coefficient
+variables, the ghost-penalty and glass-hull (with glass variables) constraints, smoothed extrema
+and traced reference points are rejected with a FormatError naming the field; tolerances and a
+glass hull without glass variables are ignored with a warning.
 """
 
 import json
@@ -33,6 +32,7 @@ import numpy as np
 
 from . import abi
 from . import batch as bt
+from . import buchdahl
 from . import optics
 
 FORMAT_SYSTEM = "mikroptim.system/1"
@@ -215,6 +215,9 @@ class SystemFile:
     glasses: dict = field(default_factory=dict)
     warnings: list = field(default_factory=list)
     path: str = ""
+    # GlassConversion per glass variable's medium: surface, from ("catalog", "constant",
+    # "buchdahl"), name, max_fit_error, n0, nu (src/optim/glass_variables.cpp:26-62)
+    glass_conversions: list = field(default_factory=list)
 
 
 def _num(v, path, origin):
@@ -484,16 +487,25 @@ def _variables(sec, sysf, origin):
                     raise FormatError(tp, f"variable twice: {t} of surface {i} is already in "
                                       f"variables.items[{prev.item}]", origin)
             if t == "glass":
-                # convert_glass_variables (src/optim/glass_variables.cpp:26-62): a constant
-                # medium becomes Buchdahl (n0 = n, nu = 0); a catalogue glass needs the Buchdahl
-                # fit (fit_buchdahl), not part of this path
+                # convert_glass_variables (src/optim/glass_variables.cpp:26-62): a catalogue glass
+                # becomes the Buchdahl fit of its index (3 dispersion terms, 400-700 nm, the
+                # fit's defaults), a constant medium Buchdahl with n0 = n and nu = 0; n0, nu1 and
+                # nu2 are the variables, nu3 stays fixed
                 m = sysf.surfaces[i].material
+                conv = {"surface": i, "from": "buchdahl", "name": None, "max_fit_error": 0.0}
                 if isinstance(m, tuple) and m[0] == "constant":
                     sysf.surfaces[i].material = ("buchdahl", m[1], 0.0, 0.0, 0.0)
-                elif not (isinstance(m, tuple) and m[0] == "buchdahl"):
-                    raise FormatError(tp, f"glass variable on the catalogue glass '{m}' needs "
-                                      "its Buchdahl fit (not part of this path; give the medium "
-                                      "as a buchdahl material)", origin)
+                    conv["from"] = "constant"
+                elif isinstance(m, str):
+                    try:
+                        f = buchdahl.fit_glass(sysf.glasses[m], 3)
+                    except buchdahl.FitError as e:
+                        raise FormatError(tp, str(e), origin) from None
+                    sysf.surfaces[i].material = f.material()
+                    conv.update({"from": "catalog", "name": m, "max_fit_error": f.max_abs_error})
+                conv["n0"], conv["nu"] = sysf.surfaces[i].material[1], \
+                    sysf.surfaces[i].material[2:]
+                sysf.glass_conversions.append(conv)
                 for c in range(3):
                     name = ("n0", "nu1", "nu2")[c] + f"[{i}]"
                     out.append(ResolvedVariable(vt, i, -math.inf, math.inf,

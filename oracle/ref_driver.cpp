@@ -13,6 +13,8 @@
 #include "mikroptim/math/df64.hpp"
 #include "mikroptim/batch/batch_residuals.hpp"
 #include "mikroptim/batch/system_batch.hpp"
+#include "mikroptim/glass/buchdahl_fit.hpp"
+#include "mikroptim/glass/dispersion.hpp"
 #include "mikroptim/model/system.hpp"
 #include "mikroptim/optim/adjoint_jacobian.hpp"
 #include "mikroptim/optim/fd_jacobian.hpp"
@@ -606,6 +608,43 @@ int ref_prefilter_decide(const mko_batch* b, const mko_residual_settings* s, con
         std::copy(r.reject.begin(), r.reject.end(), reject);
         *reference_merit = r.reference_merit;
         *reference_variant = r.reference_variant;
+    })
+}
+
+// The reference's fit_buchdahl (src/glass/buchdahl_fit.cpp) of a glass's index: model 0
+// Sellmeier (A, B[terms], C[terms]), 1 polynomial (coefficients in b, exponents in c). The index
+// function evaluates the reference's inline formulas (dispersion.hpp: sellmeier_index_n,
+// polynomial_index) with the range check of Glass::index; Glass::index itself lives in
+// src/glass/catalog.cpp with the JSON reader, which needs a library this image lacks. nodes: 0
+// Chebyshev, 1 equidistant. out: n0, nu1, nu2, nu3, max_abs_error, rms_error.
+int ref_fit_buchdahl(int model, double A, const double* b, const double* c, int terms,
+                     double range_lo, double range_hi, int dispersion_terms, double lambda_min,
+                     double lambda_max, int samples, int nodes, int check_samples, double* out) {
+    REF_TRY({
+        mko::BuchdahlFitOptions o;
+        o.lambda_min_nm = lambda_min;
+        o.lambda_max_nm = lambda_max;
+        o.samples = samples;
+        o.nodes = nodes == 0 ? mko::BuchdahlFitNodes::Chebyshev : mko::BuchdahlFitNodes::Equidistant;
+        o.check_samples = check_samples;
+        if (!(lambda_min >= range_lo && lambda_min <= range_hi && lambda_max >= range_lo &&
+              lambda_max <= range_hi))
+            throw std::invalid_argument("fit range outside the glass range");
+        const std::vector<double> B(b, b + terms), C(c, c + terms);
+        const auto index = [&](double lambda_nm) {
+            if (!(lambda_nm >= range_lo && lambda_nm <= range_hi))
+                throw std::invalid_argument("wavelength outside the glass range");
+            const double l = mko::nm_to_um(lambda_nm);
+            return model == 1 ? mko::polynomial_index(B.data(), C.data(), terms, l)
+                              : mko::sellmeier_index_n(A, B.data(), C.data(), terms, l);
+        };
+        const mko::BuchdahlFit f = mko::fit_buchdahl(index, dispersion_terms, o);
+        out[0] = f.n0;
+        out[1] = f.nu[0];
+        out[2] = f.nu[1];
+        out[3] = f.nu[2];
+        out[4] = f.max_abs_error;
+        out[5] = f.rms_error;
     })
 }
 

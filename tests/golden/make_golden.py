@@ -73,6 +73,17 @@ def main():
         opt = abi.lm_options(max_iterations=8, threads=2, jacobian_method=method)
         x, _, _ = ref.optimize_lm(lb.take([0]), s, lv, lv.extract(lb)[:2], opt)
         out[f"singlet_rows_lm_{key}_x"] = x
+    # Buchdahl fits (fit_buchdahl) of the catalogue glasses: n0, nu1, nu2, nu3, max, rms per
+    # glass, for 2 / 3 dispersion terms and Chebyshev / equidistant nodes
+    from mikroptim_amd import system_file as sf
+    for cat in ("basis", "synthetisch"):
+        glasses = sf.load_glass_catalog(os.path.join(ROOT, "tests", "golden", "data", "glass",
+                                                     cat + ".json"))
+        for terms in (2, 3):
+            for nodes in ("chebyshev", "equidistant"):
+                rows = [ref.fit_buchdahl(g, terms, nodes=nodes) for g in glasses.values()
+                        if g.range_nm[0] <= 400.0 and g.range_nm[1] >= 700.0]
+                out[f"buchdahl_fit_{cat}_{terms}_{nodes}"] = np.array(rows)
     np.savez_compressed(os.path.join(ROOT, "tests", "golden", "reference_outputs.npz"), **out)
     print("wrote", sorted(out))
 

@@ -262,6 +262,24 @@ class Lib:
         out["count"] = count.value
         return out
 
+    def fit_buchdahl(self, glass, dispersion_terms=3, lambda_min=400.0, lambda_max=700.0,
+                     samples=301, nodes="chebyshev", check_samples=3001):
+        """The reference's fit_buchdahl of a catalogue glass (reference build only): n0, nu1,
+        nu2, nu3, max_abs_error, rms_error."""
+        poly = glass.model == "polynomial"
+        b = np.array(glass.coefficients if poly else glass.B, dtype=np.float64)
+        c = np.array(glass.exponents if poly else glass.C, dtype=np.float64)
+        out = np.zeros(6)
+        f = self.fn("fit_buchdahl")
+        f.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_int,
+                      C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, _dp]
+        f.restype = C.c_int
+        self.check(f(1 if poly else 0, glass.A, abi.ptr(b), abi.ptr(c), len(b),
+                     glass.range_nm[0], glass.range_nm[1], dispersion_terms, lambda_min,
+                     lambda_max, samples, 0 if nodes == "chebyshev" else 1, check_samples,
+                     abi.ptr(out)))
+        return out
+
     def gauss_legendre(self, n):
         u = np.zeros(n)
         w = np.zeros(n)

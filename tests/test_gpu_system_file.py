@@ -7,6 +7,7 @@ are held to the reference's CPU-GPU differences: merit within 1e-8 relative, sam
 that come from surface extrema follow CudaMechanical.TraceResidualsFromEngineMatchCpu
 (tests/test_cuda_mechanical.cpp:68): tolerance sqrt(w) * 2 * max(1e-12, 5e-12) per row."""
 
+import json
 import math
 import os
 
@@ -23,6 +24,7 @@ SYSTEMS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "da
 BIEGUNG = os.path.join(SYSTEMS, "einzellinse_bk7_biegung.json")
 MECHANIK = os.path.join(SYSTEMS, "zweilinser_mechanik.json")
 SOFT = os.path.join(SYSTEMS, "weiches_versagen_start.json")
+GLASS = os.path.join(os.path.dirname(SYSTEMS), "glass")
 
 
 def test_biegung_merit_and_rows(engine):
@@ -106,3 +108,33 @@ def test_optimize_report(engine):
     rep = sf.optimize(MECHANIK, engine, max_iterations=5, runs=2, seed=3)
     assert len(rep["mechanical_constraints"]) == 13 and rep["best_run"] in (0, 1)
     assert rep["settings"]["perturbation"] == 0.05 and rep["settings"]["seed"] == 3
+
+
+def test_catalogue_glass_variable(engine, tmp_path):
+    """A glass variable on the singlet's N-BK7: the medium becomes the Buchdahl fit of N-BK7's
+    index (convert_glass_variables); the converted file's trace and Jacobians (glass columns
+    included) against the oracle, and the LM with the glass free ends below the curvature-only
+    optimum."""
+    from test_gpu_limits import _check_jacobians, _check_trace
+    d = json.load(open(BIEGUNG, encoding="utf-8"))
+    d["variables"]["items"].append({"type": "glass", "surface": 1})
+    (tmp_path / "data" / "systems").mkdir(parents=True)
+    (tmp_path / "data" / "glass").mkdir()
+    (tmp_path / "data" / "glass" / "basis.json").write_text(
+        open(os.path.join(GLASS, "basis.json"), encoding="utf-8").read())
+    path = tmp_path / "data" / "systems" / "glass_variable.json"
+    path.write_text(json.dumps(d))
+    ld = sf.load(str(path))
+    b, v, s = ld.batch, ld.variables, ld.settings
+    assert ld.system.glass_conversions[0]["name"] == "N-BK7"
+    assert v.names[-3:] == ["n0[1]", "nu1[1]", "nu2[1]"]
+    hard = sf.load(str(path), failure=abi.FAILURE_HARD).settings
+    _check_trace(engine, b, hard)
+    _check_jacobians(engine, b, v, hard, runs=1)
+    x0 = v.extract(b, 0)[None, :]
+    r = engine.optimize(b, s, v, x0, abi.lm_options(jacobian_method=abi.JACOBIAN_ADJOINT))
+    # the oracle's LM from the same start: 1.29e-6 after its 100 iterations (the dispersion
+    # coefficients drift along a flat valley: no glass hull bounds them)
+    assert r.status[0] in ("converged_merit", "converged_step", "converged_gradient",
+                           "converged_stall", "max_iterations")
+    assert r.merit[0] < 1.557334133980554e-06

@@ -23,7 +23,7 @@ import os
 import numpy as np
 import pytest
 
-from mikroptim_amd import abi, rng, systems
+from mikroptim_amd import abi, buchdahl, rng, systems
 from mikroptim_amd import batch as bt
 from mikroptim_amd import system_file as sf
 from oracle_lib import oracle
@@ -315,8 +315,6 @@ def _set(path, value):
      "variables.items[1].surface"),
     (_set(["variables", "items", 1], {"type": "curvature", "surface": 1}),
      "variables.items[1].surface"),
-    (_set(["variables", "items", 1], {"type": "glass", "surface": 1}),     # catalogue glass
-     "variables.items[1].surface"),
     (_set(["variables", "items", 1], {"type": "ghost_coefficient", "plate": 0}),
      "variables.items[1].type"),
     (_set(["constraints", "items", 0, "mode"], "max"), "constraints.items[0].mode"),
@@ -449,8 +447,8 @@ def test_soft_failure_start_as_documented():
 def test_glass_variables_on_constant_and_buchdahl_media(tmp_path):
     """Glass variables (resolve_variables expands one entry into n0, nu1, nu2; typical sizes 1,
     0.01, 0.001; no bounds): a constant medium becomes Buchdahl with n0 = n (the index is
-    unchanged, so is the merit), the columns follow the oracle's FD; a catalogue glass needs the
-    Buchdahl fit and is refused."""
+    unchanged, so is the merit), the columns follow the oracle's FD; a catalogue glass becomes the
+    Buchdahl fit of its index (refused when its valid range does not hold 400-700 nm)."""
     def edit(d):
         d["variables"]["items"].append({"type": "glass", "surface": 1})
     path = _edited(tmp_path, edit, PETZVAL)
@@ -476,10 +474,32 @@ def test_glass_variables_on_constant_and_buchdahl_media(tmp_path):
     k = len(names) - 3
     assert np.linalg.norm(Ja[0, k] - Jf[0, k]) <= 1e-4 * np.linalg.norm(Jf[0, k])
 
+    # a catalogue glass becomes the Buchdahl fit of its index (3 terms, 400-700 nm): the
+    # medium's index at the file's wavelength is the fit's, within max_fit_error of CAF2's
     def catalogue(d):
         d["variables"]["items"].append({"type": "glass", "surface": 3})   # CAF2
-    with pytest.raises(sf.FormatError) as e:
-        sf.read_system(_edited(tmp_path, catalogue, PETZVAL))
+    conv = sf.read_system(_edited(tmp_path, catalogue, PETZVAL))
+    fit = buchdahl.fit_glass(conv.glasses["CAF2"], 3)
+    assert conv.surfaces[3].material == fit.material()
+    assert conv.glass_conversions[-1] == {"surface": 3, "from": "catalog", "name": "CAF2",
+                                          "max_fit_error": fit.max_abs_error, "n0": fit.n0,
+                                          "nu": fit.nu}
+    ld = sf.load(_edited(tmp_path, catalogue, PETZVAL))
+    assert list(ld.batch.buchdahl[0, 3]) == [fit.n0, *fit.nu]
+    for l, lam in enumerate(conv.wavelengths_nm):
+        assert ld.batch.index[0, 3, l] == fit.index(lam)
+        assert abs(fit.index(lam) - conv.glasses["CAF2"].index(lam)) <= fit.max_abs_error
+    assert ld.variables.names[-3:] == ["n0[3]", "nu1[3]", "nu2[3]"]
+
+    # a catalogue glass whose valid range does not hold the fit range is refused
+    def uncovered(d):
+        d["glass_catalogs"].append("data/glass/synthetisch.json")
+        d["surfaces"][3]["material"] = "SYN-UV"                            # 250 ... 650 nm
+        d["variables"]["items"].append({"type": "glass", "surface": 3})
+    (tmp_path / "data" / "glass" / "synthetisch.json").write_text(
+        open(os.path.join(DATA, "glass", "synthetisch.json"), encoding="utf-8").read())
+    with pytest.raises(sf.FormatError, match="SYN-UV.*400, 700.*250, 650") as e:
+        sf.read_system(_edited(tmp_path, uncovered, PETZVAL))
     assert e.value.json_path == "variables.items[3].surface"
 
     def bounded(d):
