@@ -116,6 +116,7 @@ enum {
     MKO_CON_WORKING_DISTANCE = 2,
     MKO_CON_PARAXIAL_FOCUS = 3,
     MKO_CON_PETZVAL = 4,
+    MKO_CON_GLASS_HULL = 6,
     MKO_CON_MAX_SEMI_DIAMETER = 7,
     MKO_CON_MIN_CENTER_THICKNESS = 8,
     MKO_CON_MAX_CENTER_THICKNESS = 9,
@@ -139,7 +140,14 @@ enum { MKO_EDGE_SEMI_DIAMETER = 0, MKO_EDGE_RAYS = 1 };
  *               bundle, the field's chief ray (collimated source only).
  *   mechanical: r = sqrt(w) * max(0, g + shift), g = value - limit (max semi-diameter, max
  *               center thickness) or limit - value (others); target = limit (mm, or
- *               cos(theta_max) for the incidence angle), offset = clearance. */
+ *               cos(theta_max) for the incidence angle), offset = clearance.
+ *   glass hull: r = sqrt(w) * e * grad[field] (glass_hull_residual, src/optim/glass_variables.cpp:
+ *               147-155) for the medium after `surface` (its Buchdahl n0, nu1, nu2 normalized
+ *               by the hull's center and scale to u): e = max(0, phi), phi = signed distance
+ *               of u to the hull + margin (3-d hull) or distance (lower dimension); grad the
+ *               unit direction u - closest point (outside) or the nearest face's normal
+ *               (inside); target = margin >= 0, field = component 0 / 1 / 2 (three rows per
+ *               glass medium); the hull is mko_extra_rows.glass_hull. */
 typedef struct mko_constraint {
     int32_t type;         /* MKO_CON_* */
     int32_t mode;         /* MKO_MODE_* (paraxial types) */
@@ -162,6 +170,15 @@ typedef struct mko_constraint {
 typedef struct mko_extra_rows {
     int32_t count;
     const mko_constraint* items; /* [count] */
+    /* The convex hull of the catalogue glasses in normalized (n0, nu1, nu2) coordinates
+     * (GlassHull, include/mikroptim/glass/glass_hull.hpp), for MKO_CON_GLASS_HULL rows, or NULL:
+     * [0] dimension (0 point, 1 segment, 2 polygon, 3 polyhedron), [1] records n, [2..4] center
+     * m, [5..7] scale s (u = (x - m) / s), then n records of 13 doubles: triangle vertices a, b,
+     * c and (dimension 3) its plane, outward unit normal and offset (inside: normal . u <=
+     * offset). Dimension 2: the polygon's fan triangles; 1: one record, the segment a - b;
+     * 0: one record, the point a. glass_hull_size = 8 + 13 n. */
+    const double* glass_hull;
+    int32_t glass_hull_size;
 } mko_extra_rows;
 
 typedef struct mko_residual_settings {

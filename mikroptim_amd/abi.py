@@ -25,6 +25,7 @@ PRECISION_FP64, PRECISION_MIXED, PRECISION_DF64, PRECISION_FP64_MFMA = 0, 1, 2, 
 (CON_MAX_SEMI_DIAMETER, CON_MIN_CENTER_THICKNESS, CON_MAX_CENTER_THICKNESS,
  CON_MIN_EDGE_THICKNESS, CON_MAX_INCIDENCE_ANGLE) = range(7, 12)
 CON_SINE_CONDITION, CON_DISTORTION = 12, 13
+CON_GLASS_HULL = 6
 MODE_EQUAL, MODE_MIN, MODE_MAX = 0, 1, 2
 EDGE_SEMI_DIAMETER, EDGE_RAYS = 0, 1
 
@@ -60,7 +61,8 @@ class Constraint(C.Structure):
 
 
 class ExtraRows(C.Structure):
-    _fields_ = [("count", C.c_int32), ("items", C.POINTER(Constraint))]
+    _fields_ = [("count", C.c_int32), ("items", C.POINTER(Constraint)),
+                ("glass_hull", C.POINTER(C.c_double)), ("glass_hull_size", C.c_int32)]
 
 
 class ResidualSettings(C.Structure):

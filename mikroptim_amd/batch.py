@@ -137,7 +137,7 @@ class Constraint:
     edge_height: int = abi.EDGE_SEMI_DIAMETER
     shift: float = 0.0
     zone: float = 0.0   # sine condition: pupil zone rho in (0, 1]
-    field: int = 0      # distortion: field index
+    field: int = 0      # distortion: field index; glass hull: component 0 / 1 / 2
 
     @property
     def needs_extrema(self):
@@ -207,6 +207,14 @@ def min_edge_thickness(surface, next_surface, limit, weight=1.0, clearance=0.0,
     return c
 
 
+def glass_hull(surface, weight=1e4, margin=0.0):
+    """The three rows of the glass-hull penalty of the medium after `surface` (components n0,
+    nu1, nu2 of sqrt(w) e grad; glass_hull_residual, src/optim/glass_variables.cpp:147-155);
+    the hull itself is ResidualSettings.glass_hull. Default weight kDefaultGlassHullWeight."""
+    return [Constraint(abi.CON_GLASS_HULL, weight, margin, surface=surface, field=k)
+            for k in range(3)]
+
+
 def max_incidence_angle(surface, angle_deg, weight=1.0, shift=0.0):
     """limit = cos(theta_max) (make_mechanical_constraints)."""
     limit = math.cos(angle_deg * 3.14159265358979323846 / 180.0)
@@ -225,6 +233,8 @@ class ResidualSettings:
     fail_weight: float = 1e6
     precision: int = abi.PRECISION_FP64  # number type of the RESIDUALS / MERIT traces
     constraints: list = field(default_factory=list)
+    # the hull of the glass-hull rows (mko_extra_rows.glass_hull; glass_hull.GlassHull.packed())
+    glass_hull: object = None
 
     def row_constraints(self):
         """Constraints in row order: those without extrema, then those with."""
@@ -241,9 +251,14 @@ class ResidualSettings:
         if self.constraints:
             items = (abi.Constraint * len(self.constraints))(
                 *[c.c_struct() for c in self.constraints])
-            rows = abi.ExtraRows(len(self.constraints), items)
+            hull = None
+            if self.glass_hull is not None:
+                hull = np.ascontiguousarray(self.glass_hull, dtype=np.float64)
+            rows = abi.ExtraRows(len(self.constraints), items,
+                                 abi.ptr(hull) if hull is not None else None,
+                                 0 if hull is None else hull.size)
             s.extra = C.pointer(rows)
-            s._keep = (items, rows)
+            s._keep = (items, rows, hull)
         return s
 
 

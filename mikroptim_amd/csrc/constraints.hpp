@@ -9,6 +9,9 @@
 //                    include/mikroptim/trace/trace.hpp:110, surface_ops.hpp:122-203)
 //   mechanical rows  sag / mechanical_value / mechanical_residual,
 //                    src/optim/mechanical_constraints.cpp:58-281
+//   glass-hull rows  glass_hull_residual, src/optim/glass_variables.cpp:135-155;
+//                    project_onto_hull, src/glass/convex_hull.cpp:397-452 (the closest point of a
+//                    triangle by Voronoi regions: Ericson, Real-Time Collision Detection, 5.1.5)
 // Every product, sum, quotient and root is rounded on its own (contraction off, correctly rounded
 // division and square root), in the reference's operation order, so a row is bitwise the value of
 // the reference's host code for the same variant and extrema.
@@ -147,7 +150,149 @@ struct VariantRows {
     const double* field_angle;  // [F]
     const double* field_target; // [F]
     int F;
+    const double* buchdahl;     // [S][4] (n0, nu1, nu2, nu3), or NULL
+    const double* glass_hull;   // mko_extra_rows.glass_hull (device copy), or NULL
 };
+
+// ---- glass hull: the closest point of the hull to u and the penalty row ------------------------
+
+__device__ inline double hull_dot(const double* a, const double* b) {
+#pragma clang fp contract(off)
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+__device__ inline double hull_length(const double* p, const double* q) {
+#pragma clang fp contract(off)
+    const double d[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+    return __dsqrt_rn(hull_dot(d, d));
+}
+// closest point of the segment a-b to p (a for a degenerate segment)
+__device__ inline void hull_on_segment(const double* p, const double* a, const double* b,
+                                       double* q) {
+#pragma clang fp contract(off)
+    const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double len2 = hull_dot(ab, ab);
+    if (!(len2 > 0.0)) {
+        q[0] = a[0]; q[1] = a[1]; q[2] = a[2];
+        return;
+    }
+    const double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    double t = __ddiv_rn(hull_dot(ap, ab), len2);
+    t = t < 0.0 ? 0.0 : (1.0 < t ? 1.0 : t);
+    for (int j = 0; j < 3; ++j) q[j] = a[j] + ab[j] * t;
+}
+// closest point of the triangle abc to p: the Voronoi regions of the corners, the edges and the
+// face; a triangle of zero area falls back to the nearest of its three edges
+__device__ inline void hull_on_triangle(const double* p, const double* a, const double* b,
+                                        const double* c, double* q) {
+#pragma clang fp contract(off)
+    const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const double ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    const double d1 = hull_dot(ab, ap), d2 = hull_dot(ac, ap);
+    auto set = [q](const double* x) { q[0] = x[0]; q[1] = x[1]; q[2] = x[2]; };
+    if (d1 <= 0.0 && d2 <= 0.0) return set(a);
+    const double bp[3] = {p[0] - b[0], p[1] - b[1], p[2] - b[2]};
+    const double d3 = hull_dot(ab, bp), d4 = hull_dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return set(b);
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double t = __ddiv_rn(d1, d1 - d3);
+        for (int j = 0; j < 3; ++j) q[j] = a[j] + ab[j] * t;
+        return;
+    }
+    const double cp[3] = {p[0] - c[0], p[1] - c[1], p[2] - c[2]};
+    const double d5 = hull_dot(ab, cp), d6 = hull_dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return set(c);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double t = __ddiv_rn(d2, d2 - d6);
+        for (int j = 0; j < 3; ++j) q[j] = a[j] + ac[j] * t;
+        return;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double t = __ddiv_rn(d4 - d3, (d4 - d3) + (d5 - d6));
+        for (int j = 0; j < 3; ++j) q[j] = b[j] + (c[j] - b[j]) * t;
+        return;
+    }
+    const double sum = va + vb + vc;
+    if (!(sum > 0.0)) {
+        double best[3], r[3];
+        hull_on_segment(p, a, b, best);
+        double bd = hull_length(p, best);
+        hull_on_segment(p, b, c, r);
+        double d = hull_length(p, r);
+        if (d < bd) {
+            bd = d;
+            best[0] = r[0]; best[1] = r[1]; best[2] = r[2];
+        }
+        hull_on_segment(p, a, c, r);
+        d = hull_length(p, r);
+        if (d < bd) {
+            best[0] = r[0]; best[1] = r[1]; best[2] = r[2];
+        }
+        return set(best);
+    }
+    const double v = __ddiv_rn(vb, sum), w = __ddiv_rn(vc, sum);
+    for (int j = 0; j < 3; ++j) q[j] = a[j] + (ab[j] * v + ac[j] * w);
+}
+
+// Component `comp` of sqrt(w) e grad for the Buchdahl coordinates x = (n0, nu1, nu2): u = (x - m)
+// / s; a 3-d hull with u behind every face plane gives phi = max_f (n_f . u - o_f) + margin and
+// grad = that face's normal; otherwise the nearest of the records' closest points gives the
+// distance d (phi = d, + margin for the 3-d hull) and grad = (u - closest) / d.
+__device__ inline double glass_hull_row(const double* hull, const double* x, double weight,
+                                        double margin, int comp) {
+#pragma clang fp contract(off)
+    const int dim = int(hull[0]), n = int(hull[1]);
+    const double* rec = hull + 8;
+    double u[3];
+    for (int j = 0; j < 3; ++j) u[j] = __ddiv_rn(x[j] - hull[2 + j], hull[5 + j]);
+    double grad[3] = {0.0, 0.0, 0.0};
+    double phi;
+    bool inside_3d = false;
+    if (dim == 3) {
+        double worst = -__builtin_inf();
+        int wf = 0;
+        for (int f = 0; f < n; ++f) {
+            const double sd = hull_dot(rec + 13 * f + 9, u) - rec[13 * f + 12];
+            if (sd > worst) {
+                worst = sd;
+                wf = f;
+            }
+        }
+        if (worst <= 0.0) {
+            inside_3d = true;
+            phi = worst + margin;
+            for (int j = 0; j < 3; ++j) grad[j] = rec[13 * wf + 9 + j];
+        }
+    }
+    if (!inside_3d) {
+        double best = __builtin_inf(), closest[3] = {0.0, 0.0, 0.0}, q[3];
+        const int m = dim >= 2 ? n : 1;
+        for (int f = 0; f < m; ++f) {
+            const double* r = rec + 13 * f;
+            if (dim >= 2) hull_on_triangle(u, r, r + 3, r + 6, q);
+            else if (dim == 1) hull_on_segment(u, r, r + 3, q);
+            else {
+                q[0] = r[0]; q[1] = r[1]; q[2] = r[2];
+            }
+            const double d = hull_length(u, q);
+            if (d < best) {
+                best = d;
+                closest[0] = q[0]; closest[1] = q[1]; closest[2] = q[2];
+            }
+        }
+        if (best > 0.0) {
+            const double inv = __ddiv_rn(1.0, best);
+            for (int j = 0; j < 3; ++j) grad[j] = (u[j] - closest[j]) * inv;
+        }
+        phi = dim == 3 ? best + margin : best;
+    }
+    const double e = 0.0 < phi ? phi : 0.0;
+    if (!(e > 0.0)) return 0.0;
+    return (__dsqrt_rn(weight) * e) * grad[comp];
+}
 
 // OSC of the zone's meridional ray (conrady_osc): NaN where undefined.
 __device__ inline double sine_condition(const VariantRows& v, const Paraxial& pd, double rho) {
@@ -186,6 +331,9 @@ __device__ inline double row(const mko_constraint& k, const VariantRows& v, cons
                              const double* ext) {
 #pragma clang fp contract(off)
     const int S = v.S;
+    if (k.type == MKO_CON_GLASS_HULL)
+        return glass_hull_row(v.glass_hull, v.buchdahl + 4 * k.surface, k.weight, k.target,
+                              k.field);
     if (is_paraxial(k.type)) {
         double q = 0.0;
         switch (k.type) {

@@ -205,6 +205,14 @@ void check_constraint(const mko_constraint& c, int k, const mko_batch& b) {
     const int S = b.surfaces;
     if (!(std::isfinite(c.weight) && c.weight >= 0.0)) fail("weight must be finite and >= 0");
     if (!(std::isfinite(c.shift) && c.shift >= 0.0)) fail("shift must be finite and >= 0");
+    if (c.type == MKO_CON_GLASS_HULL) {
+        if (c.surface < 0 || c.surface >= S) fail("surface out of range");
+        if (c.field < 0 || c.field > 2) fail("component must be 0, 1 or 2");
+        if (!(std::isfinite(c.target) && c.target >= 0.0)) fail("margin must be finite and >= 0");
+        if (c.shift != 0.0) fail("no shift for the glass hull");
+        if (!b.buchdahl) fail("batch has no Buchdahl coefficients");
+        return;
+    }
     if (con_paraxial(c.type)) {
         if (c.mode < MKO_MODE_EQUAL || c.mode > MKO_MODE_MAX) fail("unknown mode");
         if (!std::isfinite(c.target)) fail("target must be finite");
@@ -275,8 +283,29 @@ struct RowPlan {
     uint64_t surfaces = 0; // surfaces whose extrema the rows read
     uint64_t hsurf = 0, csurf = 0; // of those, the ones read for heights / for cos I
     const mko_constraint* dev = nullptr;
+    // the glass hull of MKO_CON_GLASS_HULL rows: host array and its device copy
+    const double* hull_host = nullptr;
+    int hull_size = 0;
+    const double* hull = nullptr;
     int count() const { return E + Et; }
 };
+
+// The layout of mko_extra_rows.glass_hull (header, records of 13 doubles, finite, scale > 0).
+void check_glass_hull(const double* h, int size) {
+    const auto fail = [](const std::string& what) {
+        throw std::invalid_argument("glass hull: " + what);
+    };
+    if (!h || size < 8 + 13) fail("missing or shorter than one record");
+    for (int i = 0; i < size; ++i)
+        if (!std::isfinite(h[i])) fail("non-finite entry " + std::to_string(i));
+    const double dim = h[0], n = h[1];
+    if (!(dim == 0.0 || dim == 1.0 || dim == 2.0 || dim == 3.0)) fail("dimension not 0 ... 3");
+    if (!(n >= 1.0 && n == std::floor(n) && 8.0 + 13.0 * n == double(size)))
+        fail("size is not 8 + 13 records");
+    if (dim < 2.0 && n != 1.0) fail("a point or segment hull has one record");
+    for (int j = 0; j < 3; ++j)
+        if (!(h[5 + j] > 0.0)) fail("scale must be > 0");
+}
 
 RowPlan plan_rows(const mko_batch& b, const mko_residual_settings& s) {
     RowPlan r;
@@ -288,6 +317,13 @@ RowPlan plan_rows(const mko_batch& b, const mko_residual_settings& s) {
             if (con_needs_extrema(s.extra->items[k]) == (pass == 1))
                 r.items.push_back(s.extra->items[k]);
     for (const mko_constraint& c : r.items) {
+        if (c.type == MKO_CON_GLASS_HULL && !r.hull_host) {
+            check_glass_hull(s.extra->glass_hull, s.extra->glass_hull_size);
+            r.hull_host = s.extra->glass_hull;
+            r.hull_size = s.extra->glass_hull_size;
+            if (c.target > 0.0 && r.hull_host[0] != 3.0)
+                throw std::invalid_argument("glass hull: a margin needs a 3-d hull");
+        }
         (con_needs_extrema(c) ? r.Et : r.E) += 1;
         r.paraxial = r.paraxial || con_needs_paraxial(c);
         if (con_needs_extrema(c)) {
@@ -679,7 +715,8 @@ public:
             });
             if (rp.count())
                 dev_check(mko_internal::constraint_rows(&batch_.view, rp.dev, rp.count(),
-                                                        rp.paraxial, ex, dres, R, T, nullptr));
+                                                        rp.paraxial, ex, dres, R, T, nullptr,
+                                                        rp.hull));
             if (res) copy_out(res + size_t(first) * R, dres, size_t(count) * R * 8);
             if (opd) copy_out(opd + size_t(first) * rays, dopd, size_t(count) * rays * 8);
             if (status) copy_out(status + size_t(first) * rays, dst, size_t(count) * rays);
@@ -798,7 +835,7 @@ public:
             if (rp.count()) {
                 double* rows = rows_.as<double>(size_t(count) * rp.count());
                 dev_check(mko_internal::constraint_rows(&dev, rp.dev, rp.count(), rp.paraxial, ex,
-                                                        rows, rp.count(), 0, nullptr));
+                                                        rows, rp.count(), 0, nullptr, rp.hull));
                 double* e = (rp.E || dextra) ? esum_.as<double>(size_t(count)) : nullptr;
                 double* t = rp.Et ? tsum_.as<double>(size_t(count)) : nullptr;
                 dev_check(mko_internal::rows_merit(count, rows, rp.count(), 0, rp.E, rp.Et, dextra,
@@ -1370,6 +1407,16 @@ private:
             items_host_.assign(src, src + bytes);
         }
         rp.dev = static_cast<const mko_constraint*>(d);
+        if (rp.hull_host) {
+            const size_t hb = size_t(rp.hull_size) * sizeof(double);
+            const char* hs = reinterpret_cast<const char*>(rp.hull_host);
+            void* hd = hull_.reserve(hb);
+            if (hull_host_.size() != hb || std::memcmp(hull_host_.data(), hs, hb) != 0) {
+                hip_check(hipMemcpy(hd, hs, hb, hipMemcpyHostToDevice), "glass hull");
+                hull_host_.assign(hs, hs + hb);
+            }
+            rp.hull = static_cast<const double*>(hd);
+        }
     }
     // device bytes per variant of a MERIT / RESIDUALS call for its constraint rows
     static size_t rows_bytes(const RowPlan& rp, const mko_batch& b) {
@@ -1409,7 +1456,7 @@ private:
                                                   nullptr));
             if (rp.count())
                 dev_check(mko_internal::constraint_rows(&fdb, rp.dev, rp.count(), rp.paraxial, ex,
-                                                        fres, R, T, nullptr));
+                                                        fres, R, T, nullptr, rp.hull));
         });
         int64_t* dch = ch_.as<int64_t>(size_t(count) * P);
         timer.run(kFdDifference, [&] {
@@ -1446,7 +1493,7 @@ private:
         if (rp.Et) ex = rows_extrema(tmpl, tidx, template_per_run, count, x, s, rp, P);
         double* rows = rows_.as<double>(n * E);
         dev_check(mko_internal::constraint_rows(&rowfd_.view, rp.dev, E, rp.paraxial, ex, rows, E,
-                                                0, nullptr));
+                                                0, nullptr, rp.hull));
         timer.run(kFdDifference, [&] {
             dev_check(mko_internal::fd_difference(rows, E, nullptr, 0, count, P, E, sp, sm, sten, J,
                                                   r0, nullptr, jstride, joffset, nullptr));
@@ -1578,8 +1625,8 @@ private:
         cand_list_, cand_count_, cand_over_, cand_flag_;
     int64_t candidate_chunks_ = 0, candidate_fallbacks_ = 0;
     Buffer out_res_, out_opd_, out_st_, part_, partc_, out_merit_, out_failed_, out_hard_, extra_;
-    Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_;
-    std::vector<char> items_host_;
+    Buffer ext_part_, ext_, rows_, esum_, tsum_, je_, re_, items_, hull_;
+    std::vector<char> items_host_, hull_host_;
     Buffer states_, smerit_, sfailed_, shard_, spart_, spc_; // ray states of the glass mixed path
     Buffer x_, plus_, minus_, jac_, r0_, fdres_, fdst_, sp_, sm_, sten_, ch_, a_, g_, vtype_, vsurf_,
         vcomp_;
@@ -1785,7 +1832,12 @@ public:
         if (rp.paraxial) validate_paraxial(base, 0, base.variants);
         if (s.extra && s.extra->count > 0) {
             items_own_.assign(s.extra->items, s.extra->items + s.extra->count);
-            extra_own_ = mko_extra_rows{int32_t(items_own_.size()), items_own_.data()};
+            if (s.extra->glass_hull && s.extra->glass_hull_size > 0)
+                hull_own_.assign(s.extra->glass_hull,
+                                 s.extra->glass_hull + s.extra->glass_hull_size);
+            extra_own_ = mko_extra_rows{int32_t(items_own_.size()), items_own_.data(),
+                                        hull_own_.empty() ? nullptr : hull_own_.data(),
+                                        int32_t(hull_own_.size())};
             settings_.extra = &extra_own_;
         } else {
             settings_.extra = nullptr;
@@ -2680,6 +2732,7 @@ private:
     mko_batch base_;
     mko_residual_settings settings_;
     std::vector<mko_constraint> items_own_;
+    std::vector<double> hull_own_;
     mko_extra_rows extra_own_{};
     mko_variables vars_;
     int runs_;

@@ -108,7 +108,7 @@ int merit_finalize(int V, int T, const double* partial, const int* partial_count
 // (device memory, row order) with the extrema [V][S][2] of the traced items.
 int constraint_rows(const mko_batch* batch, const mko_constraint* items, int count,
                     int need_paraxial, const double* extrema, double* rows, int64_t stride,
-                    int64_t offset, void* stream);
+                    int64_t offset, void* stream, const double* glass_hull = nullptr);
 
 // e_sum[v] = (extra ? extra[v] + : ) sum of the first E rows squared, t_sum[v] = sum of the
 // next Et rows squared (either output may be NULL).

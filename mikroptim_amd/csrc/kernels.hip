@@ -1001,7 +1001,8 @@ __global__ void constraint_rows_kernel(int V, int S, int F, const double* curvat
                                        const double* field_target, uint64_t sphere_mask,
                                        const mko_constraint* items, int count, int need_paraxial,
                                        const double* extrema, double* rows, int64_t stride,
-                                       int64_t offset) {
+                                       int64_t offset, const double* buchdahl,
+                                       const double* glass_hull) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const mkoc::VariantRows vr{curvature + size_t(v) * S,
@@ -1012,7 +1013,9 @@ __global__ void constraint_rows_kernel(int V, int S, int F, const double* curvat
                                S,
                                field_angle + size_t(v) * F,
                                field_target + size_t(v) * F,
-                               F};
+                               F,
+                               buchdahl ? buchdahl + size_t(v) * S * 4 : nullptr,
+                               glass_hull};
     mkoc::Paraxial p{1.0, 0.0, 0.0, 0.0, false, false, 0.0, 0.0, 0.0};
     if (need_paraxial) p = mkoc::paraxial(vr.c, vr.t, vr.n, S);
     const double* ext = extrema ? extrema + size_t(v) * S * 2 : nullptr;
@@ -3104,9 +3107,11 @@ int merit_finalize(int V, int T, const double* partial, const int* partial_count
 
 int constraint_rows(const mko_batch* batch, const mko_constraint* items, int count,
                     int need_paraxial, const double* extrema, double* rows, int64_t stride,
-                    int64_t offset, void* stream) {
+                    int64_t offset, void* stream, const double* glass_hull) {
     MKO_TRY({
         if (batch->variants <= 0 || count <= 0) return 0;
+        if (glass_hull && !batch->buchdahl)
+            throw std::invalid_argument("glass-hull rows need the batch's Buchdahl coefficients");
         uint64_t mask = 0;
         for (int i = 0; i < batch->surfaces; ++i)
             if (batch->type[i] == MKO_SURFACE_SPHERE) mask |= 1ull << i;
@@ -3117,7 +3122,7 @@ int constraint_rows(const mko_batch* batch, const mko_constraint* items, int cou
                            batch->fields, batch->curvature, batch->thickness,
                            batch->paraxial_index, batch->pupil_radius, batch->field_angle,
                            batch->field_target, mask, items, count, need_paraxial, extrema, rows,
-                           stride, offset);
+                           stride, offset, batch->buchdahl, glass_hull);
         check_hip(hipGetLastError(), "constraint rows launch");
     })
 }

@@ -16,10 +16,11 @@ the three reference spheres with the ideal image point, curvature / thickness / 
 variables, glass variables on every glass medium (as convert_glass_variables does: a catalogue
 glass becomes the Buchdahl fit of its index, a constant medium Buchdahl with n0 = n), paraxial
 (focal length, parfocal length, working distance, paraxial focus, Petzval, sine condition per
-zone, distortion per field) and mechanical constraints. Ghost plates, ghost-coefficient
-variables, the ghost-penalty and glass-hull (with glass variables) constraints, smoothed extrema
-and traced reference points are rejected with a FormatError naming the field; tolerances and a
-glass hull without glass variables are ignored with a warning.
+zone, distortion per field), glass-hull (three rows per glass medium over the convex hull of
+the catalogues' glasses) and mechanical constraints. Ghost plates, ghost-coefficient variables,
+the ghost-penalty constraint, smoothed extrema and traced reference points are rejected with a
+FormatError naming the field; tolerances and a glass hull without glass variables are ignored
+with a warning.
 """
 
 import json
@@ -33,6 +34,7 @@ import numpy as np
 from . import abi
 from . import batch as bt
 from . import buchdahl
+from . import glass_hull
 from . import optics
 
 FORMAT_SYSTEM = "mikroptim.system/1"
@@ -218,6 +220,8 @@ class SystemFile:
     # GlassConversion per glass variable's medium: surface, from ("catalog", "constant",
     # "buchdahl"), name, max_fit_error, n0, nu (src/optim/glass_variables.cpp:26-62)
     glass_conversions: list = field(default_factory=list)
+    # the glass hull of a glass_hull item with glass variables (glass_hull.GlassHull), or None
+    glass_hull: object = None
 
 
 def _num(v, path, origin):
@@ -261,6 +265,7 @@ VARIABLE_TYPES = {"curvature": abi.VAR_CURVATURE, "thickness": abi.VAR_THICKNESS
 # kDefaultTypical* (include/mikroptim/model/variables.hpp:67-69)
 DEFAULT_TYPICAL = {"curvature": 0.01, "thickness": 1.0, "working_distance": 1.0}
 UNSUPPORTED_CONSTRAINTS = ("ghost_penalty",)
+DEFAULT_GLASS_HULL_WEIGHT = 1e4  # kDefaultGlassHullWeight
 PARAXIAL_TYPES = {"focal_length": abi.CON_FOCAL_LENGTH, "parfocal_length": abi.CON_PARFOCAL_LENGTH,
                   "working_distance": abi.CON_WORKING_DISTANCE,
                   "paraxial_focus": abi.CON_PARAXIAL_FOCUS, "petzval": abi.CON_PETZVAL,
@@ -593,20 +598,48 @@ def _constraints(sec, sysf, origin):
                           "path", origin)
     S = len(sysf.surfaces)
     items = sec.get("items", [])
-    paraxial, mechanical = [], []
+    paraxial, hull_rows, mechanical = [], [], []
     for k, it in enumerate(items):
         p = f"constraints.items[{k}]"
         t = it.get("type")
         if t in UNSUPPORTED_CONSTRAINTS:
             raise FormatError(p + ".type", f"'{t}' constraints are not part of this path", origin)
         if t == "glass_hull":
-            # a penalty on glass variables only (optimize_command.cpp:1530-1539): without them
-            # the entry adds no row; its rows with glass variables are not part of this path
-            if any(x.type == abi.VAR_GLASS for x in sysf.variables):
-                raise FormatError(p + ".type", "glass_hull rows are not part of this path",
-                                  origin)
-            sysf.warnings.append(f"{p}: glass_hull without glass variables adds no residual")
+            # no target, no mode, at most one entry; weight (default 1e4) and margin (default
+            # 0) finite and >= 0 (check_constraints, src/model/constraints.cpp:626-657)
+            if "mode" in it:
+                raise FormatError(p + ".mode", "glass_hull has no mode (omit the field)", origin)
+            for j in range(k):
+                if items[j].get("type") == t:
+                    raise FormatError(p + ".type", f"constraint twice: glass_hull is already in "
+                                      f"constraints.items[{j}]", origin)
+            w = _num(it["weight"], p + ".weight", origin) if "weight" in it \
+                else DEFAULT_GLASS_HULL_WEIGHT
+            if not (math.isfinite(w) and w >= 0.0):
+                raise FormatError(p + ".weight", "weight must be finite and >= 0", origin)
+            margin = _num(it["margin"], p + ".margin", origin) if "margin" in it else 0.0
+            if not (math.isfinite(margin) and margin >= 0.0):
+                raise FormatError(p + ".margin", "margin must be finite and >= 0", origin)
+            glass = [x for x in sysf.variables if x.type == abi.VAR_GLASS and x.component == 0]
+            if not glass:
+                # a penalty on glass variables only (optimize_command.cpp:1528-1539)
+                sysf.warnings.append(f"{p}: glass_hull without glass variables adds no residual")
+                continue
+            # make_glass_hull over every glass of the catalogues, then one element per glass
+            # medium (make_glass_hull_penalty, src/optim/glass_variables.cpp:65-101)
+            try:
+                hull = glass_hull.make_glass_hull(sysf.glasses)
+            except (buchdahl.FitError, ValueError) as e:
+                raise FormatError(p + ".type", str(e), origin) from None
+            if margin > 0.0 and hull.dimension < 3:
+                raise FormatError(p + ".margin", "a margin needs a three-dimensional hull; the "
+                                  f"catalogues' glasses give dimension {hull.dimension}", origin)
+            sysf.glass_hull = hull
+            for x in glass:
+                hull_rows.extend((k, c) for c in bt.glass_hull(x.surface, w, margin))
             continue
+        if "margin" in it:
+            raise FormatError(p + ".margin", "only for glass_hull", origin)
         if t not in PARAXIAL_TYPES and t not in MECHANICAL_TYPES:
             raise FormatError(p + ".type", f"unknown constraint type '{t}'", origin)
         mech = t in MECHANICAL_TYPES
@@ -758,7 +791,7 @@ def _constraints(sec, sysf, origin):
                 paraxial.append((k, dataclasses.replace(c, field=f)))
             continue
         paraxial.append((k, c))
-    return paraxial + mechanical
+    return paraxial + hull_rows + mechanical
 
 
 # ---- the batch (make_system_batch) -------------------------------------------------------------------
@@ -849,6 +882,8 @@ def load(path, variants=1, rings=10, spokes=8, failure=abi.FAILURE_SOFT, fail_we
     sysf = read_system(path)
     s = bt.ResidualSettings(rings=rings, spokes=spokes, failure=failure, fail_weight=fail_weight)
     s.constraints = list(sysf.constraints)
+    if sysf.glass_hull is not None:
+        s.glass_hull = sysf.glass_hull.packed()
     return Loaded(sysf, make_batch(sysf, variants), variable_set(sysf), s)
 
 
@@ -920,9 +955,16 @@ def optimize(path, engine, runs=1, max_iterations=100, jacobian="fd", precision=
     rows0 = engine.residuals(start, s)[0][0, -E:] if E else []
     rows1 = engine.residuals(end, s)[0][0, -E:] if E else []
     row_of = {id(c): i for i, c in enumerate(s.row_constraints())}
-    paraxial, mechanical = [], []
+    paraxial, mechanical, hull_elements = [], [], {}
     for c, item in zip(s.constraints, sysf.constraint_items):
         r = row_of[id(c)]
+        if c.type == abi.CON_GLASS_HULL:
+            # the excess max(0, phi) of each glass medium = |rows| / sqrt(w) (|grad| = 1)
+            el = hull_elements.setdefault(c.surface, {"surface": c.surface, "rows": [],
+                                                      "weight": c.weight, "margin": c.target,
+                                                      "item": item})
+            el["rows"].append(r)
+            continue
         entry = {"item": item, "type": CONSTRAINT_NAMES[c.type], "weight": c.weight}
         if c.type in PARAXIAL_TYPES.values():
             entry.update({"mode": MODE_NAMES[c.mode], "target": c.target,
@@ -938,6 +980,21 @@ def optimize(path, engine, runs=1, max_iterations=100, jacobian="fd", precision=
                           "limit": c.target, "residual_start": _finite(rows0[r]),
                           "residual_result": _finite(rows1[r]), "satisfied": rows1[r] == 0.0})
             mechanical.append(entry)
+    hull_report = None
+    if hull_elements:
+        h = sysf.glass_hull
+        first = next(iter(hull_elements.values()))
+        sw = math.sqrt(first["weight"])
+
+        def excess(rows, idx):
+            return _finite(math.sqrt(sum(rows[i] ** 2 for i in idx)) / sw) if sw > 0 else None
+        hull_report = {"glasses": len(h.names), "dimension": h.dimension,
+                       "vertices": h.vertex_names(), "weight": first["weight"],
+                       "margin": first["margin"],
+                       "elements": [{"surface": e["surface"],
+                                     "excess_start": excess(rows0, e["rows"]),
+                                     "excess_result": excess(rows1, e["rows"])}
+                                    for e in hull_elements.values()]}
     status = [abi.LM_STATUS_NAMES[r.status] for r in res.runs]
     soft = failure != abi.FAILURE_HARD
     run_objs = [{"run": i, "status": status[i], "iterations": r.iterations,
@@ -963,7 +1020,7 @@ def optimize(path, engine, runs=1, max_iterations=100, jacobian="fd", precision=
                        "upper": _finite(x.upper), "typical": x.typical,
                        "start": float(starts[0][k]), "result": float(res.x[best][k])}
                       for k, x in enumerate(sysf.variables)],
-        "constraints": paraxial, "glass_hull": None, "mechanical_constraints": mechanical,
+        "constraints": paraxial, "glass_hull": hull_report, "mechanical_constraints": mechanical,
         "ghost_diagnosis": None, "iterations": None, "runs": run_objs, "best_run": best,
         "global_search": None,
         "result": {"status": status[best], "converged": status[best].startswith("converged"),

@@ -22,6 +22,7 @@
 #include "gauss_legendre_table.h"
 
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -418,6 +419,7 @@ struct Layout {
     // constraint rows in row order: the items without extrema (E), then those with (E_t)
     std::vector<mko_constraint> items;
     int E = 0, Et = 0;
+    const double* hull = nullptr; // mko_extra_rows.glass_hull of glass-hull rows
     bool soft() const { return settings.failure == MKO_FAILURE_SOFT; }
     int64_t penalty_rows() const { return soft() ? rays : 0; }
     int64_t trace_rows() const { return rays + penalty_rows(); }
@@ -454,7 +456,18 @@ inline Layout make_layout(const Batch& B, const mko_residual_settings& s) {
             for (int k = 0; k < s.extra->count; ++k)
                 if (con_needs_extrema(s.extra->items[k]) == (pass == 1))
                     lay.items.push_back(s.extra->items[k]);
-        for (const mko_constraint& c : lay.items) (con_needs_extrema(c) ? lay.Et : lay.E) += 1;
+        for (const mko_constraint& c : lay.items) {
+            (con_needs_extrema(c) ? lay.Et : lay.E) += 1;
+            if (c.type == MKO_CON_GLASS_HULL) {
+                const double* h = s.extra->glass_hull;
+                const int n = s.extra->glass_hull_size;
+                if (!h || n < 21 || !(h[0] >= 0.0 && h[0] <= 3.0) || 8 + 13 * int(h[1]) != n)
+                    throw std::invalid_argument("glass-hull rows need a valid glass hull");
+                if (c.target > 0.0 && h[0] != 3.0)
+                    throw std::invalid_argument("glass hull: a margin needs a 3-d hull");
+                lay.hull = h;
+            }
+        }
     }
     return lay;
 }
@@ -699,6 +712,15 @@ void check_constraint(const mko_constraint& c, int k, const Batch& B) {
     const int S = B.S();
     if (!(std::isfinite(c.weight) && c.weight >= 0.0)) fail("weight must be finite and >= 0");
     if (!(std::isfinite(c.shift) && c.shift >= 0.0)) fail("shift must be finite and >= 0");
+    if (c.type == MKO_CON_GLASS_HULL) {
+        // check_glass_hull_penalty, src/optim/glass_variables.cpp:103-133
+        if (c.surface < 0 || c.surface >= S) fail("surface out of range");
+        if (c.field < 0 || c.field > 2) fail("component must be 0, 1 or 2");
+        if (!(std::isfinite(c.target) && c.target >= 0.0)) fail("margin must be finite and >= 0");
+        if (c.shift != 0.0) fail("no shift for the glass hull");
+        if (!B.b.buchdahl) fail("batch has no Buchdahl coefficients");
+        return;
+    }
     if (con_paraxial(c.type)) {
         if (c.mode < MKO_MODE_EQUAL || c.mode > MKO_MODE_MAX) fail("unknown mode");
         if (!std::isfinite(c.target)) fail("target must be finite");
@@ -814,10 +836,121 @@ double chief_ray_height(const Batch& B, int v, int f) {
     return ray.p.y + ray.d.y * ((w - ray.p.z) / ray.d.z);
 }
 
+// ---- glass hull (project_onto_hull, src/glass/convex_hull.cpp:152-215, 397-452; the closest
+// point of a triangle by Voronoi regions, Ericson, Real-Time Collision Detection, 5.1.5) --------
+
+using P3 = std::array<double, 3>;
+inline double dot3(const P3& a, const P3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline P3 sub3(const P3& a, const P3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline P3 axpy3(const P3& a, const P3& d, double t) {
+    return {a[0] + d[0] * t, a[1] + d[1] * t, a[2] + d[2] * t};
+}
+inline double len3(const P3& a) { return std::sqrt(dot3(a, a)); }
+inline P3 at3(const double* r) { return {r[0], r[1], r[2]}; }
+
+P3 closest_on_segment(const P3& p, const P3& a, const P3& b) {
+    const P3 ab = sub3(b, a);
+    const double len2 = dot3(ab, ab);
+    if (!(len2 > 0.0)) return a;
+    const double t = std::clamp(dot3(sub3(p, a), ab) / len2, 0.0, 1.0);
+    return axpy3(a, ab, t);
+}
+
+P3 closest_on_triangle(const P3& p, const P3& a, const P3& b, const P3& c) {
+    const P3 ab = sub3(b, a), ac = sub3(c, a), ap = sub3(p, a);
+    const double d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return a;
+    const P3 bp = sub3(p, b);
+    const double d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return axpy3(a, ab, d1 / (d1 - d3));
+    const P3 cp = sub3(p, c);
+    const double d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return axpy3(a, ac, d2 / (d2 - d6));
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+        return axpy3(b, sub3(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6)));
+    const double sum = va + vb + vc;
+    if (!(sum > 0.0)) {
+        P3 best = closest_on_segment(p, a, b);
+        double bd = len3(sub3(p, best));
+        for (const P3& q : {closest_on_segment(p, b, c), closest_on_segment(p, a, c)}) {
+            const double d = len3(sub3(p, q));
+            if (d < bd) {
+                bd = d;
+                best = q;
+            }
+        }
+        return best;
+    }
+    const double v = vb / sum, w = vc / sum;
+    return {a[0] + (ab[0] * v + ac[0] * w), a[1] + (ab[1] * v + ac[1] * w),
+            a[2] + (ab[2] * v + ac[2] * w)};
+}
+
+// glass_hull_residual (src/optim/glass_variables.cpp:135-155) of the medium's (n0, nu1, nu2):
+// component `comp` of sqrt(w) max(0, phi) grad
+double glass_hull_row(const double* hull, const P3& x, double weight, double margin, int comp) {
+    const int dim = int(hull[0]), n = int(hull[1]);
+    const double* rec = hull + 8;
+    const P3 u{(x[0] - hull[2]) / hull[5], (x[1] - hull[3]) / hull[6], (x[2] - hull[4]) / hull[7]};
+    P3 grad{0.0, 0.0, 0.0};
+    double phi = 0.0;
+    bool inside = false;
+    if (dim == 3) {
+        double worst = -std::numeric_limits<double>::infinity();
+        int wf = 0;
+        for (int f = 0; f < n; ++f) {
+            const double sd = dot3(at3(rec + 13 * f + 9), u) - rec[13 * f + 12];
+            if (sd > worst) {
+                worst = sd;
+                wf = f;
+            }
+        }
+        if (worst <= 0.0) {
+            inside = true;
+            phi = worst + margin;
+            grad = at3(rec + 13 * wf + 9);
+        }
+    }
+    if (!inside) {
+        double best = std::numeric_limits<double>::infinity();
+        P3 closest{0.0, 0.0, 0.0};
+        const int m = dim >= 2 ? n : 1;
+        for (int f = 0; f < m; ++f) {
+            const double* r = rec + 13 * f;
+            const P3 q = dim >= 2   ? closest_on_triangle(u, at3(r), at3(r + 3), at3(r + 6))
+                         : dim == 1 ? closest_on_segment(u, at3(r), at3(r + 3))
+                                    : at3(r);
+            const double d = len3(sub3(u, q));
+            if (d < best) {
+                best = d;
+                closest = q;
+            }
+        }
+        if (best > 0.0) {
+            const double inv = 1.0 / best;
+            grad = {(u[0] - closest[0]) * inv, (u[1] - closest[1]) * inv,
+                    (u[2] - closest[2]) * inv};
+        }
+        phi = dim == 3 ? best + margin : best;
+    }
+    const double e = std::max(0.0, phi);
+    if (!(e > 0.0)) return 0.0;
+    return (std::sqrt(weight) * e) * grad[size_t(comp)];
+}
+
 double constraint_row(const mko_constraint& c, const Batch& B, int v, const Paraxial& p,
-                      const double* ext) {
+                      const double* ext, const double* hull = nullptr) {
     const int S = B.S();
     const double* t = B.b.thickness + size_t(v) * S;
+    if (c.type == MKO_CON_GLASS_HULL) {
+        const double* bd = B.b.buchdahl + (size_t(v) * S + size_t(c.surface)) * 4;
+        return glass_hull_row(hull, P3{bd[0], bd[1], bd[2]}, c.weight, c.target, c.field);
+    }
     if (con_paraxial(c.type)) {
         double q = 0.0;
         switch (c.type) {
@@ -899,7 +1032,8 @@ void variant_rows(const Batch& B, int v, const Layout& lay, const double* ext, d
     const int n = untraced_only ? lay.E : lay.E + lay.Et;
     for (int k = 0; k < n; ++k) need_p = need_p || con_needs_paraxial(lay.items[size_t(k)]);
     if (need_p) p = batch_paraxial(B, v);
-    for (int k = 0; k < n; ++k) out[k] = constraint_row(lay.items[size_t(k)], B, v, p, ext);
+    for (int k = 0; k < n; ++k)
+        out[k] = constraint_row(lay.items[size_t(k)], B, v, p, ext, lay.hull);
 }
 
 // rows of variant v into out (extrema from its own trace when E_t > 0)

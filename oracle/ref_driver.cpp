@@ -14,6 +14,8 @@
 #include "mikroptim/batch/batch_residuals.hpp"
 #include "mikroptim/batch/system_batch.hpp"
 #include "mikroptim/glass/buchdahl_fit.hpp"
+#include "mikroptim/glass/glass_hull.hpp"
+#include "mikroptim/optim/glass_variables.hpp"
 #include "mikroptim/glass/dispersion.hpp"
 #include "mikroptim/model/system.hpp"
 #include "mikroptim/optim/adjoint_jacobian.hpp"
@@ -645,6 +647,37 @@ int ref_fit_buchdahl(int model, double A, const double* b, const double* c, int 
         out[3] = f.nu[2];
         out[4] = f.max_abs_error;
         out[5] = f.rms_error;
+    })
+}
+
+// The reference's glass hull (GlassHull, src/glass/glass_hull.cpp; convex_hull,
+// src/glass/convex_hull.cpp) of n glasses' coordinates (n0, nu1, nu2) and its rows
+// glass_hull_residual (src/optim/glass_variables.cpp:147-155) at m points x: out [m][3]; info:
+// dimension, center (3), scale (3), then the hull's vertex count and triangle count.
+int ref_glass_hull_rows(const double* coords, int n, double weight, double margin,
+                        const double* x, int m, double* out, double* info) {
+    REF_TRY({
+        std::vector<mko::GlassHullPoint> pts(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i) {
+            pts[size_t(i)].name = "G" + std::to_string(i);
+            pts[size_t(i)].coordinates = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+        }
+        mko::GlassHullPenalty pen;
+        pen.hull = mko::GlassHull(std::move(pts));
+        pen.weight = weight;
+        pen.margin = margin;
+        for (int k = 0; k < m; ++k) {
+            const mko::Point3 r =
+                mko::glass_hull_residual(pen, {x[3 * k], x[3 * k + 1], x[3 * k + 2]});
+            for (int j = 0; j < 3; ++j) out[3 * k + j] = r[size_t(j)];
+        }
+        info[0] = pen.hull.dimension();
+        for (int j = 0; j < 3; ++j) {
+            info[1 + j] = pen.hull.center()[size_t(j)];
+            info[4 + j] = pen.hull.scale()[size_t(j)];
+        }
+        info[7] = double(pen.hull.hull().vertices.size());
+        info[8] = double(pen.hull.hull().triangles.size());
     })
 }
 

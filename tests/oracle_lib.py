@@ -280,6 +280,21 @@ class Lib:
                      abi.ptr(out)))
         return out
 
+    def glass_hull_rows(self, coords, weight, margin, x):
+        """The reference's glass hull of the (n0, nu1, nu2) rows of `coords` and its residuals
+        glass_hull_residual at the points x [m][3] (reference build only): (rows [m][3], info:
+        dimension, center, scale, vertex and triangle counts)."""
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        out = np.zeros((len(x), 3))
+        info = np.zeros(9)
+        f = self.fn("glass_hull_rows")
+        f.argtypes = [_dp, C.c_int, C.c_double, C.c_double, _dp, C.c_int, _dp, _dp]
+        f.restype = C.c_int
+        self.check(f(abi.ptr(coords), len(coords), weight, margin, abi.ptr(x), len(x),
+                     abi.ptr(out), abi.ptr(info)))
+        return out, info
+
     def gauss_legendre(self, n):
         u = np.zeros(n)
         w = np.zeros(n)
