@@ -95,3 +95,18 @@ def test_lm_device_equals_host_and_reaches_the_hull(engine, tmp_path, method):
     before, after = np.linalg.norm(hull_rows(x), axis=1), np.linalg.norm(hull_rows(rd.x), axis=1)
     assert np.all(after <= np.maximum(before, 1e-3))
     assert before.max() > 1.0 and after.max() < 0.1 * before.max()
+
+
+def test_optimize_report_glass_hull(engine, tmp_path):
+    """`optimize` on the file reports the hull (glasses, dimension, vertices, weight, margin) and
+    the glass medium's excess before and after, as the optimize/1 output's glass_hull."""
+    ld = _load(tmp_path)
+    rep = sf.optimize(ld.system.path, engine, max_iterations=10, jacobian="adjoint")
+    g = rep["glass_hull"]
+    h = ld.system.glass_hull
+    assert g["glasses"] == 39 and g["dimension"] == 3 and g["vertices"] == h.vertex_names()
+    assert g["weight"] == 1e4 and g["margin"] == 0.02
+    (e,) = g["elements"]
+    assert e["surface"] == 1 and e["excess_start"] > 0.0
+    assert e["excess_result"] < e["excess_start"]
+    assert not any(c["type"] == "glass_hull" for c in rep["constraints"])
