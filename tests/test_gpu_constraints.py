@@ -183,12 +183,14 @@ def test_lm_with_rows_follows_oracle(engine):
     b, v, s = _singlet(4, seed=3)
     x0 = v.extract(b)[:3]
     base = b.take([0])
-    for method in (abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT):
+    # merits after 10 cycles: FD 2.1e-8, adjoint 8.0e-10 relative measured (the Jacobians'
+    # device-host differences, the rows from extrema to 1e-10, carried along the LM)
+    for method, rtol in ((abi.JACOBIAN_FD, 1e-6), (abi.JACOBIAN_ADJOINT, 1e-7)):
         opt = abi.lm_options(max_iterations=10, threads=4, jacobian_method=method)
         res = engine.optimize(base, s, v, x0, opt)
         _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, opt)
         mo = np.array([r.merit for r in runs_o])
-        assert np.allclose(res.merit, mo, rtol=1e-3)
+        assert np.all(np.abs(res.merit / mo - 1.0) <= rtol)
         pts = base.take(np.zeros(3, dtype=np.int64))
         v.apply(pts, x0)
         m0, _, _ = engine.merit(pts, s)
@@ -326,7 +328,8 @@ def test_lm_with_image_rows(engine):
     m0, _, _ = engine.merit(pts, s)
     assert np.all(rd.merit < m0)
     _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, abi.lm_options(**kw))
-    assert np.allclose(rd.merit, [r.merit for r in runs_o], rtol=1e-3)
+    # 2.7e-9 relative measured (the distortion rows agree to 1e-12, the sine rows bit for bit)
+    assert np.all(np.abs(rd.merit / np.array([r.merit for r in runs_o]) - 1.0) <= 1e-7)
 
 
 def test_device_entry_points_reject_rows(engine):

@@ -66,9 +66,10 @@ def test_glass_lm_follows_oracle(engine):
     pts = base.take(np.zeros(3, dtype=np.int64))
     v.apply(pts, x0)
     m0, _, _ = engine.merit(pts, s)
-    for method in (abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT):
+    # merits after 10 cycles: FD 1.3e-7, adjoint 1.8e-12 relative measured
+    for method, rtol in ((abi.JACOBIAN_FD, 1e-5), (abi.JACOBIAN_ADJOINT, 1e-9)):
         opt = abi.lm_options(max_iterations=10, threads=4, jacobian_method=method)
         res = engine.optimize(base, s, v, x0, opt)
         _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, opt)
-        assert np.allclose(res.merit, [r.merit for r in runs_o], rtol=1e-3)
+        assert np.all(np.abs(res.merit / np.array([r.merit for r in runs_o]) - 1.0) <= rtol)
         assert np.all(res.merit < m0)

@@ -199,12 +199,15 @@ def test_lm_fd_and_adjoint_follow_the_oracle(engine, singlet):
     s = systems.singlet_settings(abi.FAILURE_HARD)
     x0 = v.extract(b)[:4]
     base = b.take([0])
-    for method in (abi.JACOBIAN_FD, abi.JACOBIAN_ADJOINT):
+    # after 15 cycles the merits agree to the Jacobians' device-host differences carried along
+    # the bending valley: FD columns ~1e-5 relative (the residuals' difference over the central
+    # step) -> merits 2.2e-7 measured; adjoint columns ~1e-12 -> 2.0e-11 measured
+    for method, rtol in ((abi.JACOBIAN_FD, 1e-5), (abi.JACOBIAN_ADJOINT, 1e-8)):
         opt = abi.lm_options(max_iterations=15, threads=4, jacobian_method=method)
         res = engine.optimize(base, s, v, x0, opt)
         xo, runs_o, cyc_o = oracle().optimize_lm(base, s, v, x0, opt)
         mo = np.array([r.merit for r in runs_o])
-        assert np.allclose(res.merit, mo, rtol=1e-3)
+        assert np.all(np.abs(res.merit / mo - 1.0) <= rtol)
 
 
 def test_lm_adjoint_mixed_converges_like_fp64(engine, singlet, objective):
@@ -420,7 +423,7 @@ def test_df64_trace_matches_fp64_oracle(engine, singlet, objective, which):
 
 def test_uneven_groups_follow_the_oracle(engine):
     """Uneven batches (runs of different topologies): one LM per topology group, each against
-    the oracle's LM on the same group (adjoint Jacobian, FP64), merits to 1e-3 after 5 cycles;
+    the oracle's LM on the same group (adjoint Jacobian, FP64), merits to 1e-7 after 5 cycles;
     the mixed-precision path decreases every run's merit."""
     groups = []
     for i in range(4):
@@ -432,7 +435,8 @@ def test_uneven_groups_follow_the_oracle(engine):
     for (base, s, v, x0), res, rm in zip(groups, results, mixed):
         _, runs_o, _ = oracle().optimize_lm(base, s, v, x0, opt)
         mo = np.array([r.merit for r in runs_o])
-        assert np.allclose(res.merit, mo, rtol=1e-3)
+        # adjoint LM, 5 cycles: 2.7e-9 relative at most measured over these groups
+        assert np.all(np.abs(res.merit / mo - 1.0) <= 1e-7)
         pts = base.take(np.zeros(len(x0), dtype=np.int64))
         v.apply(pts, x0)
         m0, _, _ = engine.merit(pts, s)
