@@ -161,3 +161,24 @@ def test_errors():
         bd.fit(p, 3, bd.FitOptions(lambda_min_nm=140.0))
     with pytest.raises(bd.FitError, match="not finite"):
         bd.fit(lambda lam: math.inf if lam > 600.0 else 1.5, 3)
+
+
+def test_other_glass_models_equal_reference():
+    """Polynomial glasses (integer powers by repeated squaring, a fractional power by pow) and a
+    5-term Sellmeier with a constant, fitted bit for bit as the reference fits them."""
+    ref = reference()
+    if ref is None:
+        pytest.skip("reference build absent")
+    cat = sf.parse_glass_catalog({"format": "mikroptim.glass/1", "glasses": [
+        {"name": "S5", "model": "sellmeier", "A": 0.05, "B": [0.9, 0.2, 0.05, 0.01, 0.8],
+         "C": [0.005, 0.02, 0.04, 0.06, 90.0], "range_nm": [300, 2500]},
+        {"name": "P", "model": "polynomial", "terms": [[2.27, 0], [-0.0094, 2], [0.0110, -2],
+                                                       [0.000149, -4], [-1.1e-5, -6],
+                                                       [8.0e-7, -8]], "range_nm": [365, 1100]},
+        {"name": "Q", "model": "polynomial", "terms": [[2.1, 0], [0.01, -2.5]],
+         "range_nm": [400, 800]}]})
+    for g in cat.values():
+        for terms in (2, 3):
+            for nodes in ("chebyshev", "equidistant"):
+                mine = _row(bd.fit_glass(g, terms, bd.FitOptions(nodes=nodes)))
+                assert np.array_equal(mine, ref.fit_buchdahl(g, terms, nodes=nodes)), g.name
