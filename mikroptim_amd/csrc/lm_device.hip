@@ -58,18 +58,38 @@ __device__ __forceinline__ int tri(int j, int i, int f) { return j * f - (j * (j
 // entries of the packed lower triangle of a P x P matrix
 __host__ __device__ __forceinline__ int tri_size(int P) { return P * (P + 1) / 2; }
 
-struct SolveLds {
-    double rhs[kMaxP];
-    double delta[kMaxP];
-    double step[kMaxP];
-    double aq[kMaxP];
-    double ratio[kMaxP];
-    double col[kMaxP]; // column j of the Cholesky factor while rows below it are updated
-    int idx[kMaxP];
-    uint8_t fixed[kMaxP];
+struct SolveScalars {
     double diag;
     int f;
 };
+
+// The solve's per-variable arrays, P entries each, in the dynamic LDS after the packed triangle
+// and its row / column bytes (sized by P, not by the largest P: more runs' workgroups fit a CU).
+struct SolveLds {
+    double* rhs;
+    double* delta;
+    double* step;
+    double* aq;
+    double* ratio;
+    double* col; // column j of the Cholesky factor while rows below it are updated
+    int* idx;
+    uint8_t* fixed;
+    double& diag;
+    int& f;
+};
+
+__host__ __device__ __forceinline__ size_t solve_arrays_offset(int P) {
+    return (size_t(tri_size(P)) * (sizeof(double) + 2) + 7) / 8 * 8;
+}
+
+__device__ __forceinline__ SolveLds solve_lds(double* m, int P, SolveScalars& sc) {
+    double* v = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(m) +
+                                          solve_arrays_offset(P));
+    int* idx = reinterpret_cast<int*>(v + 6 * size_t(P));
+    return SolveLds{v,          v + P,     v + 2 * P, v + 3 * P,
+                    v + 4 * P,  v + 5 * P, idx,       reinterpret_cast<uint8_t*>(idx + P),
+                    sc.diag,    sc.f};
+}
 
 // solve_step: (A_FF + mu D_FF) delta_F = -g_F on the free variables with the active set; writes
 // trial, step_norm, predicted and solved of the run (trial, step_norm and predicted only when
@@ -467,10 +487,11 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
                                                          int count, const int32_t* dcount,
                                                          const double* a_c,
                                                          const double* g_c) {
-    __shared__ SolveLds L;
+    __shared__ SolveScalars sc;
     __shared__ int finite_s;
     extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles, then the
-                                      // row and column (uint8) of each entry
+                                      // row and column (uint8) of each entry, then the arrays
+    SolveLds L = solve_lds(m_tri, S.P, sc);
     const int j = blockIdx.x;
     if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
@@ -515,9 +536,10 @@ __global__ __launch_bounds__(kLanes) void prepare_kernel(State S, Options o, con
 
 __global__ __launch_bounds__(kLanes) void step_kernel(State S, Options o, const int32_t* list,
                                                       int count, const int32_t* dcount) {
-    __shared__ SolveLds L;
+    __shared__ SolveScalars sc;
     extern __shared__ double m_tri[]; // packed lower triangle, P (P + 1) / 2 doubles, then the
-                                      // row and column (uint8) of each entry
+                                      // row and column (uint8) of each entry, then the arrays
+    SolveLds L = solve_lds(m_tri, S.P, sc);
     const int j = blockIdx.x;
     if (j >= count || (dcount && j >= *dcount)) return;
     const int run = list[j];
@@ -626,7 +648,9 @@ __global__ __launch_bounds__(64) void fd_pattern_kernel(int count, int P, const 
 
 inline hipStream_t st(void* s) { return static_cast<hipStream_t>(s); }
 inline size_t tri_bytes(int P) {
-    return size_t(tri_size(P)) * (sizeof(double) + 2) + 7; // + the entries' row and column bytes
+    // the triangle with its entries' row and column bytes, then the solve's arrays (6 doubles,
+    // an int and a byte per variable)
+    return solve_arrays_offset(P) + size_t(P) * (6 * sizeof(double) + sizeof(int) + 1);
 }
 inline unsigned blocks_of(int n, int per) { return unsigned((n + per - 1) / per); }
 
