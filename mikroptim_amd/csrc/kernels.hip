@@ -2314,8 +2314,12 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
 // upper triangle (mirrored on output); the tuples' S, Tr and factors are staged through LDS in
 // chunks of 8 tuples.
 constexpr int kFinalizeChunk = 8;
+// kE entries per thread: 5 up to 49 variables, 10 up to 71, 19 up to 96 (the kernel is
+// instantiated per bound, so the accumulators of the smaller counts do not hold registers and
+// more workgroups fit a CU).
 constexpr int kFinalizeEntries = (kMaxVariablesFused * (kMaxVariablesFused + 1) / 2 + 255) / 256;
 
+template <int kE>
 __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
     int T, int Pn, int slots, const float* partC, const double* partST, const TupleNormal* partT,
     double* a, double* g, const int32_t* live) {
@@ -2325,10 +2329,10 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
     if (live && v >= *live) return;
     const int tid = threadIdx.x;
     const int nu = Pn * (Pn + 1) / 2;
-    int eq[kFinalizeEntries], ep[kFinalizeEntries];
-    double acc[kFinalizeEntries];
+    int eq[kE], ep[kE];
+    double acc[kE];
 #pragma unroll
-    for (int i = 0; i < kFinalizeEntries; ++i) {
+    for (int i = 0; i < kE; ++i) {
         acc[i] = 0.0;
         // upper-triangle index u -> (q, p), q <= p, row-major over q
         const int u = tid + i * 256;
@@ -2362,7 +2366,7 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
         for (int tt = 0; tt < nt; ++tt) {
             const float* C = partC + (size_t(v) * T + t0 + tt) * size_t(Pn) * Pn;
 #pragma unroll
-            for (int i = 0; i < kFinalizeEntries; ++i) {
+            for (int i = 0; i < kE; ++i) {
                 if (tid + i * 256 < nu) {
                     const int q = eq[i], p = ep[i];
                     const double c = double(C[size_t(q) * Pn + p]);
@@ -2375,7 +2379,7 @@ __global__ __launch_bounds__(256) void adjoint_normal_finalize_kernel(
     }
     double* A = a + size_t(v) * Pn * Pn;
 #pragma unroll
-    for (int i = 0; i < kFinalizeEntries; ++i) {
+    for (int i = 0; i < kE; ++i) {
         if (tid + i * 256 < nu) {
             A[size_t(eq[i]) * Pn + ep[i]] = acc[i];
             A[size_t(ep[i]) * Pn + eq[i]] = acc[i];
@@ -2957,8 +2961,12 @@ int adjoint_normal_states(const mko_batch* batch, const mko_residual_settings* s
             launch_fused_waves<false, false>(nw, P, m, partC, partST, partT, states, s);
         }
         check_hip(hipGetLastError(), "adjoint normal launch");
-        hipLaunchKernelGGL(adjoint_normal_finalize_kernel, dim3(unsigned(P.V)), dim3(256), 0, s, P.T,
-                           m.P, nw, partC, partST, partT, a, g, live);
+        const int nu = m.P * (m.P + 1) / 2;
+        const auto finalize = nu <= 5 * 256    ? adjoint_normal_finalize_kernel<5>
+                              : nu <= 10 * 256 ? adjoint_normal_finalize_kernel<10>
+                                               : adjoint_normal_finalize_kernel<kFinalizeEntries>;
+        hipLaunchKernelGGL(finalize, dim3(unsigned(P.V)), dim3(256), 0, s, P.T, m.P, nw, partC,
+                           partST, partT, a, g, live);
         check_hip(hipGetLastError(), "adjoint normal finalize launch");
     })
 }
