@@ -476,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
                     states.data[state_index(states, t.v, 6, row)] =
                         okr ? mkod::to_double(r.opl) : __builtin_nan("");
                 }
-                opl[i] = mkod::to_double(r.opl);
+                opl[i] = mkod::in_piston(sti) ? mkod::to_double(r.opl) : 0.0;
                 pen[i] = soft.penalty;
                 if (mkod::in_piston(sti)) {
                     sw += s.weight;
@@ -511,7 +511,9 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
                 NoRecord nr;
                 st[i] = trace_ray<kSoft, kFast>(P, L, r, M, kappa, soft, nr);
             }
-            opl[i] = mkod::to_double(r.opl);
+            // a ray's OPL counts only with an in-piston status; selecting it here leaves the ray
+            // dead on the surface loop's failure exits (no copies of it kept for them)
+            opl[i] = mkod::in_piston(st[i]) ? mkod::to_double(r.opl) : 0.0;
             pen[i] = soft.penalty;
             if (mkod::in_piston(st[i])) {
                 sw += s.weight;

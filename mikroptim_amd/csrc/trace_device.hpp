@@ -6,12 +6,16 @@
 //   surface propagation  include/mikroptim/trace/trace.hpp:175-229 (stop/plane/sphere only)
 //   reference spheres    include/mikroptim/trace/reference_sphere.hpp:56-143
 //   reverse steps        include/mikroptim/trace/adjoint_backward.hpp:115-241, 373-464
-// The device code is compiled with FMA contraction (hipcc default); results therefore differ
-// from the FP64 CPU oracle by rounding only, which the parity tests bound per ray.
+// The FP64 surface step spells its fmas out (no contraction left to the compiler, so every kernel
+// that inlines it computes the same bits); the rest is compiled with FMA contraction (hipcc
+// default). Results differ from the FP64 CPU oracle by rounding only, which the parity tests
+// bound per ray.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "df64.hpp"
 #include "mikroptim_types.h"
@@ -167,13 +171,138 @@ struct RayT {
 };
 using Ray = RayT<double>;
 
+// The FP64 surface step with its arithmetic spelled out: each product that fuses into a sum is
+// one fma and every other operation is rounded on its own (no contraction is left to the
+// compiler), so every kernel that inlines the step computes the same bits, in hard and in soft
+// failure mode.
+struct StepQuadratic {
+    double s0, px, py, H, B, D;
+};
+// transfer_to_tangent_plane and the coefficients of intersect_sphere's quadratic
+template <bool kFast>
+__device__ __forceinline__ StepQuadratic step_quadratic(const RayT<double>& r, double c,
+                                                        double t) {
+#pragma clang fp contract(off)
+    StepQuadratic q;
+    q.s0 = fdiv<kFast>(-(r.p.z - t), r.d.z);
+    q.px = __builtin_fma(q.s0, r.d.x, r.p.x);
+    q.py = __builtin_fma(q.s0, r.d.y, r.p.y);
+    q.H = c * __builtin_fma(q.py, q.py, q.px * q.px);
+    q.B = __builtin_fma(-c, __builtin_fma(r.d.y, q.py, r.d.x * q.px), r.d.z);
+    q.D = __builtin_fma(q.B, q.B, -(c * q.H));
+    return q;
+}
+__device__ __forceinline__ double dot_fma(V3<double> a, V3<double> b) {
+#pragma clang fp contract(off)
+    return __builtin_fma(a.z, b.z, __builtin_fma(a.y, b.y, a.x * b.x));
+}
+// the intersection p + s d (p.z = 0 on the tangent plane) and the surface normal there
+__device__ __forceinline__ V3<double> step_point(const StepQuadratic& q, V3<double> d, double s) {
+    return {__builtin_fma(s, d.x, q.px), __builtin_fma(s, d.y, q.py), __builtin_fma(s, d.z, 0.0)};
+}
+__device__ __forceinline__ V3<double> step_normal(V3<double> p, double c) {
+#pragma clang fp contract(off)
+    return {-c * p.x, -c * p.y, __builtin_fma(-c, p.z, 1.0)};
+}
+// refract_sphere: k = 1 - mu^2 (1 - cos^2 I) and, for k >= 0, mu d + (sqrt(k) - mu cos I) N
+__device__ __forceinline__ double refract_k(double mu, double cosI) {
+#pragma clang fp contract(off)
+    return __builtin_fma(-(mu * mu), __builtin_fma(-cosI, cosI, 1.0), 1.0);
+}
+__device__ __forceinline__ V3<double> refract_regular(V3<double> d, V3<double> N, double mu,
+                                                      double cosI, double sqrt_k) {
+#pragma clang fp contract(off)
+    const double g = __builtin_fma(-mu, cosI, sqrt_k);
+    return {__builtin_fma(N.x, g, d.x * mu), __builtin_fma(N.y, g, d.y * mu),
+            __builtin_fma(N.z, g, d.z * mu)};
+}
+
+// Hard failure mode: the step's operations run for every ray and one test of the four failure
+// conditions (backward, missed, no positive root, TIR; the precedence of the early returns of
+// the soft form) ends it, so a surface costs one exit of the surface loop instead of four. The
+// ray is updated in every case: the callers drop a ray whose step failed.
+template <bool kFast>
+__device__ __forceinline__ uint8_t surface_step_hard(RayT<double>& r, double c, double t,
+                                                     double nb, double mu) {
+#pragma clang fp contract(off)
+    const V3<double> d = r.d;
+    const StepQuadratic q = step_quadratic<kFast>(r, c, t);
+    const double denom = q.B + fsqrt<kFast>(q.D);
+    const double s = fdiv<kFast>(q.H, denom);
+    const V3<double> p = step_point(q, d, s);
+    const V3<double> N = step_normal(p, c);
+    const double cosI = dot_fma(d, N);
+    const double k = refract_k(mu, cosI);
+    r.opl = __builtin_fma(nb, q.s0 + s, r.opl);
+    r.p = p;
+    r.d = refract_regular(d, N, mu, cosI, fsqrt<kFast>(k));
+    uint8_t st = k < 0.0 ? uint8_t(MKO_STATUS_TIR) : uint8_t(MKO_STATUS_OK);
+    st = (q.D < 0.0 || !(denom > 0.0)) ? uint8_t(MKO_STATUS_MISSED) : st;
+    return !(d.z > 0.0) ? uint8_t(MKO_STATUS_BACKWARD) : st;
+}
+
+// Soft failure mode: a miss on a curved surface continues from the quadratic's vertex (s = B / c,
+// unit normal) and TIR continues along the surface tangent, each adding its penalty.
+template <bool kFast>
+__device__ __forceinline__ uint8_t surface_step_soft(RayT<double>& r, double c, double t,
+                                                     double nb, double mu, Soft& soft,
+                                                     uint8_t& branch) {
+#pragma clang fp contract(off)
+    const V3<double> d = r.d;
+    if (!(d.z > 0.0)) return MKO_STATUS_BACKWARD;
+    const StepQuadratic q = step_quadratic<kFast>(r, c, t);
+    double s;
+    bool missed = false;
+    if (q.D < 0.0) {
+        if (!(c > 0.0) && !(c < 0.0)) return MKO_STATUS_MISSED;
+        soft.add(-q.D, MKO_STATUS_SOFT_MISSED);
+        s = q.B / c;
+        missed = true;
+    } else {
+        const double denom = q.B + fsqrt<kFast>(q.D);
+        if (!(denom > 0.0)) return MKO_STATUS_MISSED;
+        s = fdiv<kFast>(q.H, denom);
+    }
+    const V3<double> p = step_point(q, d, s);
+    r.p = p;
+    r.opl = __builtin_fma(nb, q.s0 + s, r.opl);
+    V3<double> N = step_normal(p, c);
+    if (missed) {
+        const double w = 1.0 / __builtin_sqrt(dot_fma(N, N));
+        N = V3<double>{w * N.x, w * N.y, w * N.z};
+    }
+    const double cosI = dot_fma(d, N);
+    const double k = refract_k(mu, cosI);
+    if (k < 0.0) {
+        soft.add(-k, MKO_STATUS_SOFT_TIR);
+        const V3<double> tau{__builtin_fma(-cosI, N.x, d.x), __builtin_fma(-cosI, N.y, d.y),
+                             __builtin_fma(-cosI, N.z, d.z)};
+        const double w = 1.0 / __builtin_sqrt(dot_fma(tau, tau));
+        r.d = V3<double>{w * tau.x, w * tau.y, w * tau.z};
+        branch = uint8_t((missed ? kBranchSoftMissed : 0) | kBranchSoftTIR);
+        return MKO_STATUS_OK;
+    }
+    r.d = refract_regular(d, N, mu, cosI, fsqrt<kFast>(k));
+    branch = missed ? kBranchSoftMissed : kBranchRegular;
+    return MKO_STATUS_OK;
+}
+
 // One surface step (propagate_surface_local with transfer, sphere intersection and refraction).
 // c = effective curvature (0 for stop and plane), t = vertex distance from the previous surface,
 // nb = medium before, mu = nb / n_after. Returns a TraceStatus (Ok also for soft continuations,
-// whose kind goes to `soft` and whose branch bits go to `branch`).
+// whose kind goes to `soft` and whose branch bits go to `branch`). FP64 takes the spelled-out
+// forms above; FP32 (the prefilter) and df64 the general form below.
 template <bool kSoft, bool kFast = false, class R>
 __device__ __forceinline__ uint8_t surface_step(RayT<R>& r, R c, R t, R nb, R mu, Soft& soft,
                                                 uint8_t& branch) {
+    if constexpr (std::is_same_v<R, double>) {
+        if constexpr (kSoft) {
+            return surface_step_soft<kFast>(r, c, t, nb, mu, soft, branch);
+        } else {
+            branch = kBranchRegular;
+            return surface_step_hard<kFast>(r, c, t, nb, mu);
+        }
+    }
     const R zero = num<R>(0.0), one = num<R>(1.0);
     if (!(r.d.z > zero)) return MKO_STATUS_BACKWARD;
     // transfer_to_tangent_plane
