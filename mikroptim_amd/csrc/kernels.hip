@@ -136,14 +136,14 @@ struct Sample {
     double rho, sphi, cphi, weight, rootw;
 };
 
-// PupilQuadrature::sample (quadrature.hpp:71-92): ring-major index k = i * M + j.
-__device__ __forceinline__ Sample sample_of(const TraceParams& P, const Tuple& t, int k) {
-    const int i = k / t.M;
-    const int j = k - i * t.M;
-    Sample s;
+// PupilQuadrature::sample (quadrature.hpp:71-92): ring-major index k = i * M + j; ring i gives
+// the radius and the weights, spoke j the angle.
+__device__ __forceinline__ void ring_of(const TraceParams& P, const Tuple& t, int i, Sample& s) {
     s.rho = __dsqrt_rn(P.gl_u[i]);
     s.weight = __ddiv_rn(P.gl_w[i], double(t.M));
     s.rootw = __dsqrt_rn(s.weight);
+}
+__device__ __forceinline__ void spoke_of(const TraceParams& P, const Tuple& t, int j, Sample& s) {
     const uint8_t mode = P.pupil_mode[t.f];
     double phi = 0.0;
     if (mode == MKO_PUPIL_HALF)
@@ -156,7 +156,50 @@ __device__ __forceinline__ Sample sample_of(const TraceParams& P, const Tuple& t
     } else {
         sincos(phi, &s.sphi, &s.cphi);
     }
+}
+__device__ __forceinline__ Sample sample_of(const TraceParams& P, const Tuple& t, int k) {
+    const int i = k / t.M;
+    const int j = k - i * t.M;
+    Sample s;
+    ring_of(P, t, i, s);
+    spoke_of(P, t, j, s);
     return s;
+}
+
+// The tuple's ring and spoke values in LDS, formed once per workgroup by the functions above (the
+// same bits as sample_of): a ray's sample is then two table reads instead of a square root, a
+// division and an FP64 sincos per ray. Tuples with more spokes than the table holds compute
+// their samples per ray.
+constexpr int kTableSpokes = 128;
+struct SampleTable {
+    double rho[MKO_MAX_RINGS], weight[MKO_MAX_RINGS], rootw[MKO_MAX_RINGS];
+    double sphi[kTableSpokes], cphi[kTableSpokes];
+};
+// The caller's next __syncthreads publishes the table.
+__device__ __forceinline__ void stage_samples(const TraceParams& P, const Tuple& t,
+                                              SampleTable& T) {
+    if (t.M > kTableSpokes) return;
+    const int x = threadIdx.x;
+    if (x < P.rings) {
+        Sample s;
+        ring_of(P, t, x, s);
+        T.rho[x] = s.rho;
+        T.weight[x] = s.weight;
+        T.rootw[x] = s.rootw;
+    }
+    for (int j = x; j < t.M; j += int(blockDim.x)) {
+        Sample s;
+        spoke_of(P, t, j, s);
+        T.sphi[j] = s.sphi;
+        T.cphi[j] = s.cphi;
+    }
+}
+__device__ __forceinline__ Sample sample_at(const TraceParams& P, const Tuple& t,
+                                            const SampleTable& T, int k) {
+    if (t.M > kTableSpokes) return sample_of(P, t, k);
+    const int i = k / t.M;
+    const int j = k - i * t.M;
+    return Sample{T.rho[i], T.sphi[j], T.cphi[j], T.weight[i], T.rootw[i]};
 }
 
 // Surface table of (variant, wavelength) in LDS.
@@ -406,10 +449,12 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
                                                        mko_internal::RayStates states,
                                                        double* ext_partial) {
     __shared__ SurfLds L;
+    __shared__ SampleTable ST;
     __shared__ double red[kWaves];
     __shared__ int ired[kWaves];
     if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
     const Tuple t = tuple_of(P, blockIdx.x);
+    stage_samples(P, t, ST);
     stage_surfaces(P, t, L);
     const size_t vf = size_t(t.v) * P.F + t.f;
     const double a = P.pupil_radius[t.v];
@@ -435,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
             Sample s{};
             Ray r0{};
             if (have) {
-                s = sample_of(P, t, k);
+                s = sample_at(P, t, ST, k);
                 r0 = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
             }
             mkod::RayT<R> r = ray_as<R>(r0);
@@ -486,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
             continue;
         }
         if (k < t.n) {
-            const Sample s = sample_of(P, t, k);
+            const Sample s = sample_at(P, t, ST, k);
             mkod::RayT<R> r = ray_as<R>(source_ray(P, t, s, a, sin_t, cos_t, L.n[0]));
             Soft soft{0.0, MKO_STATUS_OK};
             constexpr bool kFast = kTraceFastMath && std::is_same_v<R, double>;
@@ -531,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     for (int i = 0; i < kRpl; ++i) {
         const int k = threadIdx.x + i * kBlock;
         if (k < t.n) {
-            const Sample s = sample_of(P, t, k);
+            const Sample s = sample_at(P, t, ST, k);
             const bool ok = valid && mkod::in_piston(st[i]);
             const double e = ok ? opl[i] - piston : 0.0;
             const double r = t.scale * s.rootw * e;

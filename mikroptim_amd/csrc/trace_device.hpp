@@ -236,9 +236,12 @@ __device__ __forceinline__ uint8_t surface_step_hard(RayT<double>& r, double c, 
     r.opl = __builtin_fma(nb, q.s0 + s, r.opl);
     r.p = p;
     r.d = refract_regular(d, N, mu, cosI, fsqrt<kFast>(k));
-    uint8_t st = k < 0.0 ? uint8_t(MKO_STATUS_TIR) : uint8_t(MKO_STATUS_OK);
-    st = (q.D < 0.0 || !(denom > 0.0)) ? uint8_t(MKO_STATUS_MISSED) : st;
-    return !(d.z > 0.0) ? uint8_t(MKO_STATUS_BACKWARD) : st;
+    const bool backward = !(d.z > 0.0), missed = q.D < 0.0 || !(denom > 0.0), tir = k < 0.0;
+    if (backward || missed || tir) [[unlikely]]
+        return backward ? uint8_t(MKO_STATUS_BACKWARD)
+               : missed ? uint8_t(MKO_STATUS_MISSED)
+                        : uint8_t(MKO_STATUS_TIR);
+    return MKO_STATUS_OK;
 }
 
 // Soft failure mode: a miss on a curved surface continues from the quadratic's vertex (s = B / c,
