@@ -2124,6 +2124,12 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
     double sw = 0.0, swo = 0.0, o_shift = 0.0;
     bool shift_ready = false;
     const int nchunks = (t.n + kThreads - 1) / kThreads;
+    // when the tuple's spoke count divides the chunk, a lane's spoke (k mod M) is the same in every
+    // chunk: its angle is formed once here and a ray's sample needs only its ring (the same
+    // functions as sample_of, hence the same bits)
+    const bool fixed_spoke = kThreads % t.M == 0;
+    Sample spoke{};
+    if (fixed_spoke) spoke_of(P, t, tid % t.M, spoke);
     for (int ch = 0; ch < nchunks; ++ch) {
         const int k = ch * kThreads + tid;
         double st[7];
@@ -2149,7 +2155,12 @@ void adjoint_normal_kernel(TraceParams P, ParamMap map,
         Ray start;
         LastState last;
         if (k < t.n) {
-            smp = sample_of(P, t, k);
+            if (fixed_spoke) {
+                smp = spoke;
+                ring_of(P, t, k / t.M, smp);
+            } else {
+                smp = sample_of(P, t, k);
+            }
             Ray r = source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]);
             start = r;
             last.count = 0;
