@@ -2,8 +2,9 @@
 reference-sphere kinds the other GPU tests leave out: point sources (Axis / Half / Full pupils),
 the image reference spheres (exit pupil, infinite), 64 surfaces with 64 variables (the fused
 normal equations' and the device LM control's maximum), tuples of 1024 and 2048 rays (4 and 8
-rays per lane), and 32 fields x 32 wavelengths. The oracle equals the reference's own CPU code
-bit for bit on the same workloads (tests/test_oracle.py). Tolerances as in test_gpu_parity.py."""
+rays per lane), tuples of 160 and 256 spokes, and 32 fields x 32 wavelengths. The oracle equals
+the reference's own CPU code bit for bit on the same workloads (tests/test_oracle.py). Tolerances
+as in test_gpu_parity.py."""
 
 import numpy as np
 import pytest
@@ -139,6 +140,20 @@ def test_big_tuples(engine, spokes):
     s = systems.ResidualSettings(rings=16, spokes=spokes, failure=abi.FAILURE_SOFT)
     _check_trace(engine, b, s)
     _check_jacobians(engine, b, v, s, runs=1, fd=False)
+
+
+# ---- more spokes than the trace kernel's sample table holds ---------------------------------------
+
+@pytest.mark.parametrize("rings,spokes", [(2, 160), (1, 256)])
+def test_many_spokes(engine, rings, spokes):
+    """Tuples of more than 128 spokes: the trace kernel forms each ray's pupil sample on its own
+    instead of reading the workgroup's LDS table of rings and spokes. 256 spokes divide the fused
+    kernel's 256-ray chunk (a lane keeps its spoke from chunk to chunk), 160 do not."""
+    b, v = systems.singlet(variants=2)
+    s = systems.ResidualSettings(rings=rings, spokes=spokes, failure=abi.FAILURE_HARD)
+    _check_trace(engine, b, s)
+    _check_jacobians(engine, b, v, s, runs=2, fd=False)
+    _check_mixed(engine, b, v, s, runs=2)
 
 
 # ---- 32 fields x 32 wavelengths --------------------------------------------------------------------
