@@ -858,6 +858,7 @@ __global__ __launch_bounds__(kBlock) void ray_candidates_kernel(TraceParams P, u
                                                                const double* ex0, uint8_t* flags,
                                                                int* tcount, int* others) {
     __shared__ SurfLds L;
+    __shared__ SampleTable ST;
     __shared__ double thr[2][MKO_MAX_SURFACES];
     __shared__ int red[kWaves];
     const Tuple t = tuple_of(P, blockIdx.x);
@@ -867,6 +868,7 @@ __global__ __launch_bounds__(kBlock) void ray_candidates_kernel(TraceParams P, u
         thr[0][threadIdx.x] = (1.0 - delta) * hmax;
         thr[1][threadIdx.x] = cmin + delta * (1.0 - cmin);
     }
+    stage_samples(P, t, ST);
     stage_surfaces(P, t, L); // synchronises
     const size_t vf = size_t(t.v) * P.F + t.f;
     const double a = P.pupil_radius[t.v];
@@ -877,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void ray_candidates_kernel(TraceParams P, u
     const uint64_t mask = hmask | cmask;
     int cands = 0, rest = 0;
     for (int k = threadIdx.x; k < t.n; k += kBlock) {
-        const Sample smp = sample_of(P, t, k);
+        const Sample smp = sample_at(P, t, ST, k);
         mkod::RayT<double> r = ray_as<double>(source_ray(P, t, smp, a, sin_t, cos_t, L.n[0]));
         Soft soft{0.0, MKO_STATUS_OK};
         bool cand = false;
@@ -1627,9 +1629,11 @@ template <bool kSoft, int kRpl, class RB, class RR, class TJ>
 __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap map, TJ* J,
                                                          double* r0, uint8_t* status) {
     __shared__ SurfLds L;
+    __shared__ SampleTable ST;
     __shared__ double red[kWaves];
     if (P.live && int(blockIdx.x) / P.T >= *P.live) return;
     const Tuple t = tuple_of(P, blockIdx.x);
+    stage_samples(P, t, ST);
     stage_surfaces(P, t, L);
     const size_t vf = size_t(t.v) * P.F + t.f;
     const double a = P.pupil_radius[t.v];
@@ -1657,7 +1661,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
             col0[int64_t(p) * R + row] = TJ(0);
             if (kSoft) col0[int64_t(p) * R + P.rays + row] = TJ(0);
         }
-        const Sample s = sample_of(P, t, k);
+        const Sample s = sample_at(P, t, ST, k);
         Ray r = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
         Soft soft{0.0, MKO_STATUS_OK};
         rec.count = 0;
@@ -1685,7 +1689,7 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
     for (int i = 0; i < kRpl; ++i) {
         const int k = threadIdx.x + i * kBlock;
         if (k >= t.n) continue;
-        const Sample s = sample_of(P, t, k);
+        const Sample s = sample_at(P, t, ST, k);
         const bool ok = valid && mkod::in_piston(st[i]);
         const double e = ok ? opl[i] - piston : 0.0;
         const int64_t row = t.offset + k;
@@ -1707,14 +1711,14 @@ __global__ __launch_bounds__(kBlock) void adjoint_kernel(TraceParams P, ParamMap
 #pragma unroll
         for (int i = 0; i < kRpl; ++i) {
             const int k = threadIdx.x + i * kBlock;
-            if (k < t.n && mkod::in_piston(st[i])) part += sample_of(P, t, k).weight * double(col[k]);
+            if (k < t.n && mkod::in_piston(st[i])) part += sample_at(P, t, ST, k).weight * double(col[k]);
         }
         const double mean = valid ? block_sum(part, red) / w_ok : 0.0;
 #pragma unroll
         for (int i = 0; i < kRpl; ++i) {
             const int k = threadIdx.x + i * kBlock;
             if (k >= t.n) continue;
-            const Sample s = sample_of(P, t, k);
+            const Sample s = sample_at(P, t, ST, k);
             const bool ok = valid && mkod::in_piston(st[i]);
             col[k] = ok ? TJ(t.scale * s.rootw * (double(col[k]) - mean)) : TJ(0);
         }
