@@ -137,6 +137,30 @@ __device__ __forceinline__ double rev_sqrt(double x) {
 #endif
 }
 
+// rev_sqrt that also returns 1 / (2 sqrt(x)): the Goldschmidt iteration's refined half reciprocal
+// (~1e-15 relative), for callers that divide by twice the root.
+__device__ __forceinline__ double rev_sqrt_half_rcp(double x, double& half_rcp) {
+#if MKO_REVERSE_FAST_MATH
+    if (!(x > 0.0)) {
+        const double g = __builtin_sqrt(x);
+        half_rcp = 0.5 / g;
+        return g;
+    }
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    const double d = __builtin_fma(-g, g, x);
+    half_rcp = h;
+    return __builtin_fma(d, h, g);
+#else
+    const double g = __builtin_sqrt(x);
+    half_rcp = 0.5 / g;
+    return g;
+#endif
+}
+
 template <class R>
 struct RayAdj {
     V3<R> p, d;
@@ -653,7 +677,8 @@ __device__ __forceinline__ RayState<double> fused_step_backward(
     const V3<double> N{-c * q.x, -c * q.y, 1.0 - c * q.z};
     const double root_k = dot(d1, N);
     const double kp = 1.0 - imu * imu * (1.0 - root_k * root_k);
-    const double cos_i = rev_sqrt(kp); // = sqrt(D) of the forward intersection
+    double inv; // 1 / (2 cos_i)
+    const double cos_i = rev_sqrt_half_rcp(kp, inv); // = sqrt(D) of the forward intersection
     const V3<double> d = imu * d1 + (cos_i - imu * root_k) * N;
     const double rdz = rev_rcp(d.z);
     const double s = q.z * rdz;
@@ -692,7 +717,6 @@ __device__ __forceinline__ RayState<double> fused_step_backward(
     const double sb = dot(qb, d) + n_s;
     in.d += s * qb;
     double s0b = n_s;
-    const double inv = rev_rcp(2.0 * cos_i);
     const double hb = sb * inv;
     const double bb = -sb * s * 2.0 * inv;
     cb += sb * s * s * inv;
