@@ -86,8 +86,10 @@ struct TraceParams {
 // Adjoint column map: variable index of curvature[s] / thickness[s] (-1 if not a variable).
 struct ParamMap {
     int P;
-    int8_t curv_col[MKO_MAX_SURFACES];
-    int8_t thick_col[MKO_MAX_SURFACES];
+    // 32-bit entries: the reverse sweeps index them by the (uniform) surface, which scalar loads
+    // from the kernel arguments serve only at dword granularity
+    int32_t curv_col[MKO_MAX_SURFACES];
+    int32_t thick_col[MKO_MAX_SURFACES];
     // glass variables (AdjointParameterMap::glass / glass_slope, ray_jacobian.hpp:28-38): the
     // variables of (n0, nu1, nu2) of the medium after surface s, and the Buchdahl coordinate
     // w_l of each wavelength (dn/d(n0, nu1, nu2) = 1, w_l, w_l^2)
@@ -2834,6 +2836,7 @@ OmegaTable omega_table(const mko_batch& b) {
 
 // Column map of the variables (AdjointParameters, src/optim/adjoint_jacobian.cpp:79-127); glass
 // variables need the batch's wavelengths (host) for their slopes, and `allow_glass`.
+static_assert(sizeof(TraceParams) + sizeof(ParamMap) < 3584, "kernel arguments within 4 KB");
 ParamMap make_map(const mko_variables& vars, const mko_batch& batch, bool allow_glass) {
     const int S = batch.surfaces;
     ParamMap m{};
@@ -2849,20 +2852,21 @@ ParamMap make_map(const mko_variables& vars, const mko_batch& batch, bool allow_
         if (s < 0 || s >= S) throw std::invalid_argument("variable surface out of range");
         if (vars.type[k] < MKO_VAR_CURVATURE || vars.type[k] > MKO_VAR_GLASS)
             throw std::invalid_argument("unsupported variable type");
-        int8_t* slot = nullptr;
         if (vars.type[k] == MKO_VAR_GLASS) {
             if (!allow_glass)
                 throw std::invalid_argument("glass variables need the FP64 adjoint (not the fused "
                                             "normal equations)");
             if (!vars.component || vars.component[k] < 0 || vars.component[k] > 2)
                 throw std::invalid_argument("glass variables need components 0 ... 2");
-            slot = &m.glass_col[s][vars.component[k]];
+            int8_t& slot = m.glass_col[s][vars.component[k]];
+            if (slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
+            slot = int8_t(k);
             m.glass = 1;
         } else {
-            slot = vars.type[k] == MKO_VAR_CURVATURE ? &m.curv_col[s] : &m.thick_col[s];
+            int32_t& slot = vars.type[k] == MKO_VAR_CURVATURE ? m.curv_col[s] : m.thick_col[s];
+            if (slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
+            slot = k;
         }
-        if (*slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
-        *slot = int8_t(k);
     }
     if (m.glass) {
         const OmegaTable t = omega_table(batch);
