@@ -86,15 +86,15 @@ struct TraceParams {
 // Adjoint column map: variable index of curvature[s] / thickness[s] (-1 if not a variable).
 struct ParamMap {
     int P;
-    // 32-bit entries: the reverse sweeps index them by the (uniform) surface, which scalar loads
-    // from the kernel arguments serve only at dword granularity
+    // 32-bit entries (here and in glass_col): the reverse sweeps index them by the (uniform)
+    // surface, which scalar loads from the kernel arguments serve only at dword granularity
     int32_t curv_col[MKO_MAX_SURFACES];
     int32_t thick_col[MKO_MAX_SURFACES];
     // glass variables (AdjointParameterMap::glass / glass_slope, ray_jacobian.hpp:28-38): the
     // variables of (n0, nu1, nu2) of the medium after surface s, and the Buchdahl coordinate
     // w_l of each wavelength (dn/d(n0, nu1, nu2) = 1, w_l, w_l^2)
     int glass;
-    int8_t glass_col[MKO_MAX_SURFACES][3];
+    int32_t glass_col[MKO_MAX_SURFACES][3];
     double omega[kMaxWavelengths];
 };
 
@@ -2858,9 +2858,9 @@ ParamMap make_map(const mko_variables& vars, const mko_batch& batch, bool allow_
                                             "normal equations)");
             if (!vars.component || vars.component[k] < 0 || vars.component[k] > 2)
                 throw std::invalid_argument("glass variables need components 0 ... 2");
-            int8_t& slot = m.glass_col[s][vars.component[k]];
+            int32_t& slot = m.glass_col[s][vars.component[k]];
             if (slot >= 0) throw std::invalid_argument("parameter belongs to two variables");
-            slot = int8_t(k);
+            slot = k;
             m.glass = 1;
         } else {
             int32_t& slot = vars.type[k] == MKO_VAR_CURVATURE ? m.curv_col[s] : m.thick_col[s];
