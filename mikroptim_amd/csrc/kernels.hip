@@ -465,6 +465,17 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     const V3<double> M{0.0, P.field_target[vf], L.t[P.S - 1]};
     const double kappa = P.ref_kind != MKO_REF_LAST_SURFACE ? P.reference_curvature[vf] : 0.0;
 
+    // when the spoke count divides the workgroup, lane x keeps spoke x mod M in every ray slot i
+    // and its ring advances by 256 / M per slot: the samples without a division per ray
+    const bool lane_spoke = kBlock % t.M == 0 && t.M <= kTableSpokes;
+    const int ring0 = int(threadIdx.x) / t.M, spoke0 = int(threadIdx.x) - ring0 * t.M;
+    const int ring_step = kBlock / t.M;
+    auto sample_slot = [&](int i, int k) {
+        if (!lane_spoke) return sample_at(P, t, ST, k);
+        const int ring = ring0 + i * ring_step;
+        return Sample{ST.rho[ring], ST.sphi[spoke0], ST.cphi[spoke0], ST.weight[ring],
+                      ST.rootw[ring]};
+    };
     double opl[kRpl], pen[kRpl];
     uint8_t st[kRpl];
     double sw = 0.0, swo = 0.0;
@@ -482,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
             Sample s{};
             Ray r0{};
             if (have) {
-                s = sample_at(P, t, ST, k);
+                s = sample_slot(i, k);
                 r0 = source_ray(P, t, s, a, sin_t, cos_t, L.n[0]);
             }
             mkod::RayT<R> r = ray_as<R>(r0);
@@ -533,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
             continue;
         }
         if (k < t.n) {
-            const Sample s = sample_at(P, t, ST, k);
+            const Sample s = sample_slot(i, k);
             mkod::RayT<R> r = ray_as<R>(source_ray(P, t, s, a, sin_t, cos_t, L.n[0]));
             Soft soft{0.0, MKO_STATUS_OK};
             constexpr bool kFast = kTraceFastMath && std::is_same_v<R, double>;
@@ -578,7 +589,7 @@ __global__ __launch_bounds__(kBlock) void trace_kernel(TraceParams P, double* re
     for (int i = 0; i < kRpl; ++i) {
         const int k = threadIdx.x + i * kBlock;
         if (k < t.n) {
-            const Sample s = sample_at(P, t, ST, k);
+            const Sample s = sample_slot(i, k);
             const bool ok = valid && mkod::in_piston(st[i]);
             const double e = ok ? opl[i] - piston : 0.0;
             const double r = t.scale * s.rootw * e;
